@@ -1,0 +1,430 @@
+// Context, workspace arena, errors, launch timing and the device-wide primitives
+// (scans, min/max, fills) the operators are built from.
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include "aln_internal.h"
+
+namespace aln {
+
+static thread_local char g_err[512] = "";
+
+int set_error(int code, const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+int check_launch(const char *name) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess)
+    return set_error(ALN_E_HIP, "launch of %s failed: %s", name, hipGetErrorString(e));
+  return ALN_OK;
+}
+
+int ws_alloc(aln_ctx *c, size_t bytes, void **out) {
+  bytes = (bytes + 255) & ~(size_t)255;
+  if (bytes == 0) bytes = 256;
+  while (c->cur < c->chunks.size()) {
+    auto &ch = c->chunks[c->cur];
+    if (c->off + bytes <= ch.size) {
+      *out = ch.ptr + c->off;
+      c->off += bytes;
+      return ALN_OK;
+    }
+    ++c->cur;
+    c->off = 0;
+  }
+  size_t want = bytes;
+  if (!c->chunks.empty() && c->chunks.back().size * 2 > want) want = c->chunks.back().size * 2;
+  if (want < ((size_t)64 << 20)) want = (size_t)64 << 20;
+  void *p = nullptr;
+  if (hipMalloc(&p, want) != hipSuccess) {
+    (void)hipGetLastError();
+    want = bytes;
+    if (hipMalloc(&p, want) != hipSuccess) {
+      (void)hipGetLastError();
+      return set_error(ALN_E_NOMEM, "workspace allocation of %zu bytes failed", bytes);
+    }
+  }
+  c->chunks.push_back({(char *)p, want});
+  c->cur = c->chunks.size() - 1;
+  c->off = bytes;
+  *out = p;
+  return ALN_OK;
+}
+
+void ws_reset(aln_ctx *c) {
+  c->cur = 0;
+  c->off = 0;
+}
+
+int pinned(aln_ctx *c, size_t bytes, void **out) {
+  if (c->pinned_size < bytes) {
+    if (c->pinned) (void)hipHostFree(c->pinned);
+    c->pinned = nullptr;
+    size_t want = bytes < 65536 ? 65536 : bytes;
+    if (hipHostMalloc(&c->pinned, want, hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      c->pinned_size = 0;
+      return set_error(ALN_E_NOMEM, "pinned allocation of %zu bytes failed", want);
+    }
+    c->pinned_size = want;
+  }
+  *out = c->pinned;
+  return ALN_OK;
+}
+
+int copy_to_host(aln_ctx *c, void *h, const void *d, size_t bytes) {
+  if (bytes == 0) return ALN_OK;
+  void *p;
+  ALN_TRY(pinned(c, bytes, &p));
+  ALN_HIP(hipMemcpyAsync(p, d, bytes, hipMemcpyDeviceToHost, c->stream));
+  ALN_HIP(hipStreamSynchronize(c->stream));
+  memcpy(h, p, bytes);
+  return ALN_OK;
+}
+
+int copy_to_device(aln_ctx *c, void *d, const void *h, size_t bytes) {
+  if (bytes == 0) return ALN_OK;
+  ALN_HIP(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, c->stream));
+  ALN_HIP(hipStreamSynchronize(c->stream));
+  return ALN_OK;
+}
+
+void prof_begin(aln_ctx *c, const char *name, int *slot) {
+  *slot = -1;
+  if (!c->profiling) return;
+  aln_ctx::Pending p;
+  p.name = name;
+  for (int i = 0; i < 2; ++i) {
+    hipEvent_t e;
+    if (!c->event_pool.empty()) {
+      e = c->event_pool.back();
+      c->event_pool.pop_back();
+    } else if (hipEventCreate(&e) != hipSuccess) {
+      (void)hipGetLastError();
+      return;
+    }
+    (i == 0 ? p.a : p.b) = e;
+  }
+  if (hipEventRecord(p.a, c->stream) != hipSuccess) {
+    (void)hipGetLastError();
+    return;
+  }
+  c->pending.push_back(p);
+  *slot = (int)c->pending.size() - 1;
+}
+
+void prof_end(aln_ctx *c, int slot) {
+  if (slot < 0 || slot >= (int)c->pending.size()) return;
+  if (hipEventRecord(c->pending[slot].b, c->stream) != hipSuccess) (void)hipGetLastError();
+}
+
+int prof_collect(aln_ctx *c) {
+  if (c->pending.empty()) return ALN_OK;
+  ALN_HIP(hipStreamSynchronize(c->stream));
+  for (auto &p : c->pending) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+      auto &acc = c->prof[p.name];
+      acc.first += ms;
+      acc.second += 1;
+    } else {
+      (void)hipGetLastError();
+    }
+    c->event_pool.push_back(p.a);
+    c->event_pool.push_back(p.b);
+  }
+  c->pending.clear();
+  return ALN_OK;
+}
+
+// ---- device-wide primitives ----------------------------------------------------------------
+
+constexpr int SCAN_BLOCK = 256;
+constexpr int SCAN_ITEMS = 16;
+constexpr int SCAN_TILE = SCAN_BLOCK * SCAN_ITEMS;
+
+template <typename T>
+__device__ __forceinline__ uint64_t block_excl_scan(uint64_t v, uint64_t *total) {
+  // block-wide exclusive scan of one value per thread (SCAN_BLOCK threads, wave 64)
+  __shared__ uint64_t wsum[SCAN_BLOCK / 64];
+  unsigned lane = __lane_id(), wid = threadIdx.x / 64;
+  uint64_t x = v;
+  for (int o = 1; o < 64; o <<= 1) {
+    uint64_t y = __shfl_up(x, o, 64);
+    if (lane >= (unsigned)o) x += y;
+  }
+  if (lane == 63) wsum[wid] = x;
+  __syncthreads();
+  uint64_t base = 0, tot = 0;
+  for (int w = 0; w < SCAN_BLOCK / 64; ++w) {
+    uint64_t s = wsum[w];
+    if (w < (int)wid) base += s;
+    tot += s;
+  }
+  __syncthreads();
+  *total = tot;
+  return base + x - v;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(SCAN_BLOCK) k_tile_sums(const T *__restrict__ in, uint64_t n,
+                                                          uint64_t *__restrict__ sums) {
+  uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
+  uint64_t s = 0;
+  for (int j = 0; j < SCAN_ITEMS; ++j) {
+    uint64_t i = base + (uint64_t)j * SCAN_BLOCK + threadIdx.x;
+    if (i < n) s += (uint64_t)in[i];
+  }
+  uint64_t tot;
+  (void)block_excl_scan<T>(s, &tot);
+  if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+// Scans `sums` (m entries) in place sequentially by one block, chunk by chunk.
+__global__ void __launch_bounds__(SCAN_BLOCK) k_scan_sums(uint64_t *sums, uint64_t m,
+                                                          uint64_t *total) {
+  uint64_t carry = 0;
+  for (uint64_t b = 0; b < m; b += SCAN_BLOCK) {
+    uint64_t i = b + threadIdx.x;
+    uint64_t v = i < m ? sums[i] : 0;
+    uint64_t tot;
+    uint64_t ex = block_excl_scan<uint64_t>(v, &tot);
+    if (i < m) sums[i] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) *total = carry;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(SCAN_BLOCK) k_tile_scan(const T *__restrict__ in, uint64_t n,
+                                                          const uint64_t *__restrict__ offs,
+                                                          uint64_t *__restrict__ out) {
+  // blocked arrangement within the tile: thread t owns items t*ITEMS .. t*ITEMS+ITEMS-1
+  uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
+  uint64_t v[SCAN_ITEMS];
+  uint64_t s = 0;
+  for (int j = 0; j < SCAN_ITEMS; ++j) {
+    uint64_t i = base + j;
+    v[j] = i < n ? (uint64_t)in[i] : 0;
+    s += v[j];
+  }
+  uint64_t tot;
+  uint64_t run = offs[blockIdx.x] + block_excl_scan<T>(s, &tot);
+  for (int j = 0; j < SCAN_ITEMS; ++j) {
+    uint64_t i = base + j;
+    if (i < n) out[i] = run;
+    run += v[j];
+  }
+}
+
+template <typename T>
+static int exclusive_scan_impl(aln_ctx *c, const T *d_in, uint64_t n, uint64_t *d_out,
+                               uint64_t *h_total) {
+  uint64_t nt = (n + SCAN_TILE - 1) / SCAN_TILE;
+  if (nt == 0) nt = 1;
+  uint64_t *sums, *tot;
+  ALN_TRY(ws(c, nt, &sums));
+  ALN_TRY(ws(c, 1, &tot));
+  {
+    ProfScope p(c, "scan");
+    hipLaunchKernelGGL(k_tile_sums<T>, dim3((unsigned)nt), dim3(SCAN_BLOCK), 0, c->stream, d_in,
+                       n, sums);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SCAN_BLOCK), 0, c->stream, sums, nt, tot);
+    hipLaunchKernelGGL(k_tile_scan<T>, dim3((unsigned)nt), dim3(SCAN_BLOCK), 0, c->stream,
+                       d_in, n, sums, d_out);
+  }
+  ALN_TRY(check_launch("exclusive_scan"));
+  if (h_total) ALN_TRY(copy_to_host(c, h_total, tot, 8));
+  return ALN_OK;
+}
+
+int exclusive_scan_u32(aln_ctx *c, const uint32_t *d_in, uint64_t n, uint64_t *d_out,
+                       uint64_t *h_total) {
+  return exclusive_scan_impl<uint32_t>(c, d_in, n, d_out, h_total);
+}
+int exclusive_scan_u64(aln_ctx *c, const uint64_t *d_in, uint64_t n, uint64_t *d_out,
+                       uint64_t *h_total) {
+  return exclusive_scan_impl<uint64_t>(c, d_in, n, d_out, h_total);
+}
+
+__global__ void __launch_bounds__(256) k_minmax(const int64_t *__restrict__ d, uint64_t n,
+                                                unsigned long long *mn, unsigned long long *mx) {
+  int64_t lo = INT64_MAX, hi = INT64_MIN;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * 256) {
+    int64_t v = d[i];
+    lo = v < lo ? v : lo;
+    hi = v > hi ? v : hi;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    int64_t a = __shfl_xor(lo, o, 64), b = __shfl_xor(hi, o, 64);
+    lo = a < lo ? a : lo;
+    hi = b > hi ? b : hi;
+  }
+  if (__lane_id() == 0) {
+    // order-preserving map of int64 to unsigned for atomicMin/Max
+    atomicMin(mn, (unsigned long long)lo ^ 0x8000000000000000ull);
+    atomicMax(mx, (unsigned long long)hi ^ 0x8000000000000000ull);
+  }
+}
+
+int minmax_i64(aln_ctx *c, const int64_t *d, uint64_t n, int64_t *h_min, int64_t *h_max) {
+  unsigned long long *mm;
+  ALN_TRY(ws(c, 2, &mm));
+  unsigned long long init[2] = {~0ull, 0ull};
+  ALN_HIP(hipMemcpyAsync(mm, init, 16, hipMemcpyHostToDevice, c->stream));
+  if (n) {
+    ProfScope p(c, "minmax");
+    hipLaunchKernelGGL(k_minmax, dim3(grid_for(n, 256, 4096)), dim3(256), 0, c->stream, d, n, mm,
+                       mm + 1);
+  }
+  ALN_TRY(check_launch("minmax"));
+  unsigned long long h[2];
+  ALN_TRY(copy_to_host(c, h, mm, 16));
+  *h_min = (int64_t)(h[0] ^ 0x8000000000000000ull);
+  *h_max = (int64_t)(h[1] ^ 0x8000000000000000ull);
+  return ALN_OK;
+}
+
+template <typename T>
+__global__ void k_fill(T *d, uint64_t n, T v) {
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * 256)
+    d[i] = v;
+}
+__global__ void k_iota(uint32_t *d, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * 256)
+    d[i] = (uint32_t)i;
+}
+
+int fill_u64(aln_ctx *c, uint64_t *d, uint64_t n, uint64_t v) {
+  if (!n) return ALN_OK;
+  hipLaunchKernelGGL(k_fill<uint64_t>, dim3(grid_for(n, 256, 8192)), dim3(256), 0, c->stream, d,
+                     n, v);
+  return check_launch("fill_u64");
+}
+int fill_u32(aln_ctx *c, uint32_t *d, uint64_t n, uint32_t v) {
+  if (!n) return ALN_OK;
+  hipLaunchKernelGGL(k_fill<uint32_t>, dim3(grid_for(n, 256, 8192)), dim3(256), 0, c->stream, d,
+                     n, v);
+  return check_launch("fill_u32");
+}
+int fill_i64(aln_ctx *c, int64_t *d, uint64_t n, int64_t v) {
+  if (!n) return ALN_OK;
+  hipLaunchKernelGGL(k_fill<int64_t>, dim3(grid_for(n, 256, 8192)), dim3(256), 0, c->stream, d,
+                     n, v);
+  return check_launch("fill_i64");
+}
+int iota_u32(aln_ctx *c, uint32_t *d, uint64_t n) {
+  if (!n) return ALN_OK;
+  hipLaunchKernelGGL(k_iota, dim3(grid_for(n, 256, 8192)), dim3(256), 0, c->stream, d, n);
+  return check_launch("iota");
+}
+
+}  // namespace aln
+
+using namespace aln;
+
+extern "C" {
+
+const char *aln_last_error(void) { return g_err; }
+const char *aln_version(void) { return "alenka_amd 0.1 (gfx950)"; }
+
+int aln_create(int device, void *stream, aln_ctx **out) {
+  ALN_REQUIRE(out, "aln_create: out is NULL");
+  int n = 0;
+  ALN_HIP(hipGetDeviceCount(&n));
+  ALN_REQUIRE(device >= 0 && device < n, "aln_create: device %d of %d", device, n);
+  ALN_HIP(hipSetDevice(device));
+  aln_ctx *c = new aln_ctx();
+  c->device = device;
+  c->stream = (hipStream_t)stream;
+  *out = c;
+  return ALN_OK;
+}
+
+void aln_destroy(aln_ctx *c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto &ch : c->chunks) (void)hipFree(ch.ptr);
+  if (c->pinned) (void)hipHostFree(c->pinned);
+  for (auto &p : c->pending) {
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
+  }
+  for (auto e : c->event_pool) (void)hipEventDestroy(e);
+  delete c;
+}
+
+int aln_set_stream(aln_ctx *c, void *stream) {
+  ALN_REQUIRE(c, "aln_set_stream: ctx is NULL");
+  c->stream = (hipStream_t)stream;
+  return ALN_OK;
+}
+
+int aln_sync(aln_ctx *c) {
+  ALN_REQUIRE(c, "aln_sync: ctx is NULL");
+  ALN_HIP(hipStreamSynchronize(c->stream));
+  return ALN_OK;
+}
+
+int aln_profile_enable(aln_ctx *c, int on) {
+  ALN_REQUIRE(c, "aln_profile_enable: ctx is NULL");
+  c->profiling = on != 0;
+  return ALN_OK;
+}
+
+int aln_profile_reset(aln_ctx *c) {
+  ALN_REQUIRE(c, "aln_profile_reset: ctx is NULL");
+  ALN_TRY(prof_collect(c));
+  c->prof.clear();
+  return ALN_OK;
+}
+
+int aln_profile_get(aln_ctx *c, const char *kernel, double *h_ms, uint64_t *h_launches) {
+  ALN_REQUIRE(c && kernel && h_ms && h_launches, "aln_profile_get: NULL argument");
+  ALN_TRY(prof_collect(c));
+  auto it = c->prof.find(kernel);
+  *h_ms = it == c->prof.end() ? 0.0 : it->second.first;
+  *h_launches = it == c->prof.end() ? 0 : it->second.second;
+  return ALN_OK;
+}
+
+// MurmurHash64A (MurmurHash2_64.cu:63-104), host side.
+uint64_t aln_murmur64a(const void *key, int len, uint32_t seed) {
+  const uint64_t m = 0xc6a4a7935bd1e995ULL;
+  uint64_t h = (uint64_t)seed ^ ((uint64_t)(int64_t)len * m);
+  const unsigned char *p = (const unsigned char *)key;
+  int nb = len / 8;
+  for (int b = 0; b < nb; ++b) {
+    uint64_t k;
+    memcpy(&k, p + 8 * b, 8);
+    k *= m;
+    k ^= k >> 47;
+    k *= m;
+    h ^= k;
+    h *= m;
+  }
+  const unsigned char *t = p + 8 * nb;
+  uint64_t tail = 0;
+  int rem = len & 7;
+  for (int i = rem - 1; i >= 0; --i) tail = (tail << 8) | t[i];
+  if (rem) {
+    h ^= tail;
+    h *= m;
+  }
+  h ^= h >> 47;
+  h *= m;
+  h ^= h >> 47;
+  return h;
+}
+
+}  // extern "C"

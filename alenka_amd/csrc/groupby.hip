@@ -1,0 +1,449 @@
+// GROUP BY with SUM / COUNT / MIN / MAX, and row hashing.
+//
+// The reference groups by sorting (order_inplace, operators.cu:367-406), finding run heads
+// (CudaSet::GroupBy, cm.cu:910-956) and reducing each run (moderngpu segreduce,
+// select.cu:481-684); across segments it merges partial groups by MurmurHash64A of the key
+// tuple on the host (add / count_avg, merge.cu:92-496).  Integer sums are exact, so the
+// result does not depend on the order rows are added in, and this kernel aggregates in an
+// open-addressing hash table instead:
+//   * keys are packed into one 63-bit code (key_i - min_i in bits_i bits, key 0 most
+//     significant), inserted by 64-bit CAS with linear probing;
+//   * runs of equal codes in adjacent lanes (clustered input) are pre-reduced with a
+//     segmented wave scan so one lane per run issues the atomics;
+//   * the occupied slots are compacted and put in the reference's output order: ascending
+//     key tuple (one segment) or ascending MurmurHash64A(keys, 8*nkeys, seed) (the
+//     multi-segment merge order).
+// Key tuples wider than 63 bits fall back to the reference's own scheme: stable sort,
+// run heads, per-run reduction.
+#include <string.h>
+
+#include <vector>
+
+#include "aln_internal.h"
+
+namespace aln {
+
+int sort_keys_impl(aln_ctx *c, const int64_t *const *keys, const int32_t *desc, int nkeys,
+                   uint64_t n, uint32_t *d_perm);
+
+constexpr uint64_t GB_EMPTY = ~0ull;
+
+struct GbArgs {
+  const int64_t *keys[ALN_MAX_KEYS];
+  int64_t kmin[ALN_MAX_KEYS];
+  uint32_t shift[ALN_MAX_KEYS];
+  uint64_t fmask[ALN_MAX_KEYS];
+  const int64_t *vals[ALN_MAX_AGGS];
+  int32_t aggs[ALN_MAX_AGGS];
+  int64_t *tab[ALN_MAX_AGGS];
+  int nk, nv;
+};
+
+__device__ __forceinline__ int64_t agg_combine(int a, int64_t x, int64_t y) {
+  if (a == ALN_AGG_MIN) return x < y ? x : y;
+  if (a == ALN_AGG_MAX) return x > y ? x : y;
+  return (int64_t)((uint64_t)x + (uint64_t)y);  // SUM, COUNT
+}
+
+__device__ __forceinline__ uint64_t murmur64a_tuple(const int64_t *v, int nk, uint32_t seed) {
+  const uint64_t m = 0xc6a4a7935bd1e995ULL;
+  uint64_t h = (uint64_t)seed ^ ((uint64_t)(8 * nk) * m);
+  for (int i = 0; i < nk; ++i) {
+    uint64_t k = (uint64_t)v[i];
+    k *= m;
+    k ^= k >> 47;
+    k *= m;
+    h ^= k;
+    h *= m;
+  }
+  h ^= h >> 47;
+  h *= m;
+  h ^= h >> 47;
+  return h;
+}
+
+__global__ void __launch_bounds__(256) k_gb_init(uint64_t *tkey, GbArgs a, uint64_t cap) {
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < cap;
+       i += (uint64_t)gridDim.x * 256) {
+    tkey[i] = GB_EMPTY;
+    for (int v = 0; v < a.nv; ++v) {
+      int ag = a.aggs[v];
+      a.tab[v][i] = ag == ALN_AGG_MIN ? INT64_MAX : ag == ALN_AGG_MAX ? INT64_MIN : 0;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) k_gb_insert(GbArgs a, uint64_t n, uint64_t *tkey,
+                                                   uint64_t cap, uint32_t *overflow) {
+  const unsigned lane = __lane_id();
+  const uint64_t mask = cap - 1;
+  for (uint64_t i0 = (uint64_t)blockIdx.x * 256; i0 < n; i0 += (uint64_t)gridDim.x * 256) {
+    uint64_t i = i0 + threadIdx.x;
+    bool valid = i < n;
+    uint64_t code = 0;
+    if (valid)
+      for (int k = 0; k < a.nk; ++k)
+        code |= (((uint64_t)a.keys[k][i] - (uint64_t)a.kmin[k]) & a.fmask[k]) << a.shift[k];
+    int64_t val[ALN_MAX_AGGS];
+    for (int v = 0; v < a.nv; ++v)
+      val[v] = !valid ? 0 : a.aggs[v] == ALN_AGG_COUNT ? 1 : a.vals[v][i];
+    // runs of equal codes across adjacent lanes
+    uint64_t prev = __shfl_up(code, 1, 64);
+    bool pvalid = __shfl_up((int)valid, 1, 64) != 0;
+    bool head = lane == 0 || !pvalid || prev != code;
+    bool f = head;
+    for (int o = 1; o < 64; o <<= 1) {
+      bool fu = __shfl_up((int)f, o, 64) != 0;
+      for (int v = 0; v < a.nv; ++v) {
+        int64_t vu = __shfl_up(val[v], o, 64);
+        if (lane >= (unsigned)o && !f) val[v] = agg_combine(a.aggs[v], vu, val[v]);
+      }
+      if (lane >= (unsigned)o) f = f || fu;
+    }
+    uint64_t nextc = __shfl_down(code, 1, 64);
+    bool nvalid = __shfl_down((int)valid, 1, 64) != 0;
+    bool tail = valid && (lane == 63 || !nvalid || nextc != code);
+    if (tail) {
+      uint64_t h = murmur64a_u64(code, 0x9747b28cu) & mask;
+      uint64_t probes = 0;
+      while (true) {
+        uint64_t cur = tkey[h];
+        if (cur == GB_EMPTY) {
+          unsigned long long old = atomicCAS((unsigned long long *)&tkey[h],
+                                             (unsigned long long)GB_EMPTY,
+                                             (unsigned long long)code);
+          cur = old == GB_EMPTY ? code : old;
+        }
+        if (cur == code) break;
+        h = (h + 1) & mask;
+        if (++probes > cap) {
+          atomicExch(overflow, 1u);
+          h = GB_EMPTY;
+          break;
+        }
+      }
+      if (h != GB_EMPTY)
+        for (int v = 0; v < a.nv; ++v) {
+          int ag = a.aggs[v];
+          int64_t *t = &a.tab[v][h];
+          if (ag == ALN_AGG_MIN)
+            atomicMin((long long *)t, (long long)val[v]);
+          else if (ag == ALN_AGG_MAX)
+            atomicMax((long long *)t, (long long)val[v]);
+          else
+            atomicAdd((unsigned long long *)t, (unsigned long long)val[v]);
+        }
+    }
+  }
+}
+
+__global__ void k_gb_flags(const uint64_t *tkey, uint64_t cap, uint32_t *flag) {
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < cap;
+       i += (uint64_t)gridDim.x * 256)
+    flag[i] = tkey[i] != GB_EMPTY;
+}
+
+__global__ void k_gb_compact(const uint64_t *tkey, uint64_t cap, const uint32_t *flag,
+                             const uint64_t *pos, uint64_t *gcode, uint32_t *gslot) {
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < cap;
+       i += (uint64_t)gridDim.x * 256)
+    if (flag[i]) {
+      gcode[pos[i]] = tkey[i];
+      gslot[pos[i]] = (uint32_t)i;
+    }
+}
+
+// Decode group keys from codes, compute the merge-order hash of each group.
+__global__ void k_gb_hash(GbArgs a, const uint64_t *gcode, uint64_t g, uint32_t seed,
+                          uint64_t *hash) {
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < g;
+       i += (uint64_t)gridDim.x * 256) {
+    int64_t kv[ALN_MAX_KEYS];
+    for (int k = 0; k < a.nk; ++k)
+      kv[k] = (int64_t)((uint64_t)a.kmin[k] + ((gcode[i] >> a.shift[k]) & a.fmask[k]));
+    hash[i] = murmur64a_tuple(kv, a.nk, seed);
+  }
+}
+
+struct GbOut {
+  int64_t *keys[ALN_MAX_KEYS];
+  int64_t *vals[ALN_MAX_AGGS];
+};
+
+__global__ void k_gb_emit(GbArgs a, GbOut o, const uint64_t *gcode, const uint32_t *gslot,
+                          uint64_t g) {
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < g;
+       i += (uint64_t)gridDim.x * 256) {
+    uint32_t s = gslot[i];
+    for (int k = 0; k < a.nk; ++k)
+      o.keys[k][i] = (int64_t)((uint64_t)a.kmin[k] + ((gcode[i] >> a.shift[k]) & a.fmask[k]));
+    for (int v = 0; v < a.nv; ++v) o.vals[v][i] = a.tab[v][s];
+  }
+}
+
+// ---- sort-based fallback ------------------------------------------------------------------
+__global__ void k_gb_heads(GbArgs a, const uint32_t *perm, uint64_t n, uint32_t *head) {
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * 256) {
+    bool h = i == 0;
+    if (!h)
+      for (int k = 0; k < a.nk; ++k)
+        if (a.keys[k][perm[i]] != a.keys[k][perm[i - 1]]) h = true;
+    head[i] = h;
+  }
+}
+
+__global__ void k_gb_sorted_agg(GbArgs a, GbOut o, const uint32_t *perm, const uint32_t *head,
+                                const uint64_t *gid_ex, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * 256) {
+    uint64_t g = gid_ex[i] + head[i] - 1;  // group index of sorted row i
+    uint32_t r = perm[i];
+    if (head[i])
+      for (int k = 0; k < a.nk; ++k) o.keys[k][g] = a.keys[k][r];
+    for (int v = 0; v < a.nv; ++v) {
+      int ag = a.aggs[v];
+      int64_t x = ag == ALN_AGG_COUNT ? 1 : a.vals[v][r];
+      if (ag == ALN_AGG_MIN)
+        atomicMin((long long *)&o.vals[v][g], (long long)x);
+      else if (ag == ALN_AGG_MAX)
+        atomicMax((long long *)&o.vals[v][g], (long long)x);
+      else
+        atomicAdd((unsigned long long *)&o.vals[v][g], (unsigned long long)x);
+    }
+  }
+}
+
+// Reorders the sort-path groups into hash order (in place through temporaries).
+__global__ void k_gb_hash_keys(GbOut o, int nk, uint64_t g, uint32_t seed, uint64_t *hash,
+                               uint32_t *idx) {
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < g;
+       i += (uint64_t)gridDim.x * 256) {
+    int64_t kv[ALN_MAX_KEYS];
+    for (int k = 0; k < nk; ++k) kv[k] = o.keys[k][i];
+    hash[i] = murmur64a_tuple(kv, nk, seed);
+    idx[i] = (uint32_t)i;
+  }
+}
+
+__global__ void k_gb_gather(const uint64_t *gc, const uint32_t *gs, const uint32_t *gi, uint64_t g,
+                            uint64_t *oc, uint32_t *os) {
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < g;
+       i += (uint64_t)gridDim.x * 256) {
+    oc[i] = gc[gi[i]];
+    os[i] = gs[gi[i]];
+  }
+}
+
+__global__ void k_permute_i64(const int64_t *in, const uint32_t *perm, uint64_t n, int64_t *out) {
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * 256)
+    out[i] = in[perm[i]];
+}
+
+__global__ void k_hash_rows(GbArgs a, uint64_t n, uint32_t seed, uint64_t *out) {
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * 256) {
+    int64_t kv[ALN_MAX_KEYS];
+    for (int k = 0; k < a.nk; ++k) kv[k] = a.keys[k][i];
+    out[i] = murmur64a_tuple(kv, a.nk, seed);
+  }
+}
+
+static int groupby_sorted(aln_ctx *c, GbArgs &a, uint64_t n, int order, uint32_t seed, GbOut &o,
+                          uint64_t cap, uint64_t *h_ngroups) {
+  uint32_t *perm, *head;
+  uint64_t *gid;
+  ALN_TRY(ws(c, n, &perm));
+  ALN_TRY(ws(c, n, &head));
+  ALN_TRY(ws(c, n, &gid));
+  const int64_t *keys[ALN_MAX_KEYS];
+  for (int k = 0; k < a.nk; ++k) keys[k] = a.keys[k];
+  ALN_TRY(sort_keys_impl(c, keys, nullptr, a.nk, n, perm));
+  hipLaunchKernelGGL(k_gb_heads, dim3(grid_for(n, 256, 65536)), dim3(256), 0, c->stream, a, perm, n,
+                     head);
+  ALN_TRY(check_launch("gb_heads"));
+  uint64_t g = 0;
+  ALN_TRY(exclusive_scan_u32(c, head, n, gid, &g));
+  *h_ngroups = g;
+  if (g > cap)
+    return set_error(ALN_E_CAPACITY, "aln_groupby: %llu groups, capacity %llu",
+                     (unsigned long long)g, (unsigned long long)cap);
+  for (int v = 0; v < a.nv; ++v) {
+    int ag = a.aggs[v];
+    ALN_TRY(fill_i64(c, o.vals[v], g, ag == ALN_AGG_MIN ? INT64_MAX : ag == ALN_AGG_MAX ? INT64_MIN : 0));
+  }
+  {
+    ProfScope p(c, "groupby_sorted_agg");
+    hipLaunchKernelGGL(k_gb_sorted_agg, dim3(grid_for(n, 256, 65536)), dim3(256), 0, c->stream, a,
+                       o, perm, head, gid, n);
+  }
+  ALN_TRY(check_launch("gb_sorted_agg"));
+  if (order == ALN_ORDER_HASH && g > 1) {
+    uint64_t *hash;
+    uint32_t *idx;
+    int64_t *tmp;
+    ALN_TRY(ws(c, g, &hash));
+    ALN_TRY(ws(c, g, &idx));
+    ALN_TRY(ws(c, g, &tmp));
+    hipLaunchKernelGGL(k_gb_hash_keys, dim3(grid_for(g, 256, 65536)), dim3(256), 0, c->stream, o,
+                       a.nk, g, seed, hash, idx);
+    ALN_TRY(check_launch("gb_hash_keys"));
+    ALN_TRY(radix_sort_pairs(c, hash, idx, g, 64));
+    auto permute = [&](int64_t *col) -> int {
+      hipLaunchKernelGGL(k_permute_i64, dim3(grid_for(g, 256, 65536)), dim3(256), 0, c->stream, col,
+                         idx, g, tmp);
+      ALN_TRY(check_launch("permute"));
+      ALN_HIP(hipMemcpyAsync(col, tmp, 8 * g, hipMemcpyDeviceToDevice, c->stream));
+      return ALN_OK;
+    };
+    for (int k = 0; k < a.nk; ++k) ALN_TRY(permute(o.keys[k]));
+    for (int v = 0; v < a.nv; ++v) ALN_TRY(permute(o.vals[v]));
+  }
+  ALN_HIP(hipStreamSynchronize(c->stream));
+  return ALN_OK;
+}
+
+}  // namespace aln
+
+using namespace aln;
+
+extern "C" {
+
+int aln_groupby(aln_ctx *c, const int64_t *const *d_keys, int nkeys, const int64_t *const *d_vals,
+                const int32_t *aggs, int nvals, uint64_t n, int order, uint32_t seed,
+                int64_t *const *d_out_keys, int64_t *const *d_out_vals, uint64_t cap,
+                uint64_t *h_ngroups) {
+  ALN_REQUIRE(c && d_keys && h_ngroups && nkeys >= 1 && nkeys <= ALN_MAX_KEYS,
+              "aln_groupby: bad key arguments");
+  ALN_REQUIRE(nvals >= 0 && nvals <= ALN_MAX_AGGS && (nvals == 0 || (aggs && d_vals && d_out_vals)),
+              "aln_groupby: bad aggregate arguments");
+  ALN_REQUIRE(order >= ALN_ORDER_KEYS && order <= ALN_ORDER_NONE, "aln_groupby: order %d", order);
+  ALN_REQUIRE(n < (1ull << 32), "aln_groupby: more than 2^32 rows");
+  ws_reset(c);
+  *h_ngroups = 0;
+  if (n == 0) return ALN_OK;
+  GbArgs a;
+  memset(&a, 0, sizeof(a));
+  a.nk = nkeys;
+  a.nv = nvals;
+  uint32_t bits[ALN_MAX_KEYS], total = 0;
+  for (int k = 0; k < nkeys; ++k) {
+    ALN_REQUIRE(d_keys[k], "aln_groupby: key %d is NULL", k);
+    int64_t mn, mx;
+    ALN_TRY(minmax_i64(c, d_keys[k], n, &mn, &mx));
+    bits[k] = bits_for_range_u((uint64_t)mx - (uint64_t)mn);
+    a.keys[k] = d_keys[k];
+    a.kmin[k] = mn;
+    a.fmask[k] = bits[k] == 64 ? ~0ull : ((1ull << bits[k]) - 1);
+    total += bits[k];
+  }
+  for (int v = 0; v < nvals; ++v) {
+    ALN_REQUIRE(aggs[v] == ALN_AGG_COUNT || aggs[v] == ALN_AGG_SUM || aggs[v] == ALN_AGG_MIN ||
+                    aggs[v] == ALN_AGG_MAX,
+                "aln_groupby: aggregate %d", aggs[v]);
+    ALN_REQUIRE(aggs[v] == ALN_AGG_COUNT || d_vals[v], "aln_groupby: value %d is NULL", v);
+    a.vals[v] = d_vals[v];
+    a.aggs[v] = aggs[v];
+  }
+  GbOut o;
+  memset(&o, 0, sizeof(o));
+  for (int k = 0; k < nkeys; ++k) o.keys[k] = d_out_keys[k];
+  for (int v = 0; v < nvals; ++v) o.vals[v] = d_out_vals[v];
+
+  if (total > 63) {
+    for (int k = 0; k < nkeys; ++k) a.shift[k] = 0;
+    return groupby_sorted(c, a, n, order, seed, o, cap, h_ngroups);
+  }
+  uint32_t sh = total;
+  for (int k = 0; k < nkeys; ++k) {
+    sh -= bits[k];
+    a.shift[k] = sh;
+  }
+  uint64_t tcap = next_pow2(2 * n < 1024 ? 1024 : 2 * n);
+  uint64_t *tkey;
+  uint32_t *overflow;
+  ALN_TRY(ws(c, tcap, &tkey));
+  for (int v = 0; v < nvals; ++v) ALN_TRY(ws(c, tcap, &a.tab[v]));
+  ALN_TRY(ws(c, 1, &overflow));
+  ALN_TRY(fill_u32(c, overflow, 1, 0));
+  hipLaunchKernelGGL(k_gb_init, dim3(grid_for(tcap, 256, 65536)), dim3(256), 0, c->stream, tkey, a,
+                     tcap);
+  ALN_TRY(check_launch("gb_init"));
+  {
+    ProfScope p(c, "groupby_insert");
+    hipLaunchKernelGGL(k_gb_insert, dim3(grid_for(n, 256, 1u << 20)), dim3(256), 0, c->stream, a, n,
+                       tkey, tcap, overflow);
+  }
+  ALN_TRY(check_launch("gb_insert"));
+  uint32_t *flag;
+  uint64_t *pos;
+  ALN_TRY(ws(c, tcap, &flag));
+  ALN_TRY(ws(c, tcap, &pos));
+  hipLaunchKernelGGL(k_gb_flags, dim3(grid_for(tcap, 256, 65536)), dim3(256), 0, c->stream, tkey, tcap,
+                     flag);
+  ALN_TRY(check_launch("gb_flags"));
+  uint64_t g = 0;
+  ALN_TRY(exclusive_scan_u32(c, flag, tcap, pos, &g));
+  uint32_t hov = 0;
+  ALN_TRY(copy_to_host(c, &hov, overflow, sizeof(uint32_t)));
+  if (hov) return set_error(ALN_E_CAPACITY, "aln_groupby: hash table overflow");
+  *h_ngroups = g;
+  if (g > cap)
+    return set_error(ALN_E_CAPACITY, "aln_groupby: %llu groups, capacity %llu",
+                     (unsigned long long)g, (unsigned long long)cap);
+  uint64_t *gcode;
+  uint32_t *gslot;
+  ALN_TRY(ws(c, g, &gcode));
+  ALN_TRY(ws(c, g, &gslot));
+  hipLaunchKernelGGL(k_gb_compact, dim3(grid_for(tcap, 256, 65536)), dim3(256), 0, c->stream, tkey,
+                     tcap, flag, pos, gcode, gslot);
+  ALN_TRY(check_launch("gb_compact"));
+  if (order == ALN_ORDER_KEYS) {
+    ALN_TRY(radix_sort_pairs(c, gcode, gslot, g, total));
+  } else if (order == ALN_ORDER_HASH) {
+    uint64_t *hash;
+    ALN_TRY(ws(c, g, &hash));
+    hipLaunchKernelGGL(k_gb_hash, dim3(grid_for(g, 256, 65536)), dim3(256), 0, c->stream, a, gcode,
+                       g, seed, hash);
+    ALN_TRY(check_launch("gb_hash"));
+    // sort (hash, group index) then carry the codes along through the index
+    uint32_t *gi;
+    uint64_t *gcode2;
+    uint32_t *gslot2;
+    ALN_TRY(ws(c, g, &gi));
+    ALN_TRY(ws(c, g, &gcode2));
+    ALN_TRY(ws(c, g, &gslot2));
+    ALN_TRY(iota_u32(c, gi, g));
+    ALN_TRY(radix_sort_pairs(c, hash, gi, g, 64));
+    hipLaunchKernelGGL(k_gb_gather, dim3(grid_for(g, 256, 65536)), dim3(256), 0, c->stream, gcode,
+                       gslot, gi, g, gcode2, gslot2);
+    ALN_TRY(check_launch("gb_gather"));
+    gcode = gcode2;
+    gslot = gslot2;
+  }
+  {
+    ProfScope p(c, "groupby_emit");
+    hipLaunchKernelGGL(k_gb_emit, dim3(grid_for(g, 256, 65536)), dim3(256), 0, c->stream, a, o,
+                       gcode, gslot, g);
+  }
+  ALN_TRY(check_launch("gb_emit"));
+  ALN_HIP(hipStreamSynchronize(c->stream));
+  return ALN_OK;
+}
+
+int aln_hash_rows(aln_ctx *c, const int64_t *const *d_keys, int nkeys, uint64_t n, uint32_t seed,
+                  uint64_t *d_out) {
+  ALN_REQUIRE(c && d_keys && nkeys >= 1 && nkeys <= ALN_MAX_KEYS && (n == 0 || d_out),
+              "aln_hash_rows: bad arguments");
+  if (n == 0) return ALN_OK;
+  GbArgs a;
+  memset(&a, 0, sizeof(a));
+  a.nk = nkeys;
+  for (int k = 0; k < nkeys; ++k) a.keys[k] = d_keys[k];
+  hipLaunchKernelGGL(k_hash_rows, dim3(grid_for(n, 256, 65536)), dim3(256), 0, c->stream, a, n, seed,
+                     d_out);
+  return check_launch("hash_rows");
+}
+
+}  // extern "C"

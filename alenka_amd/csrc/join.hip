@@ -1,0 +1,260 @@
+// Inner equi-join on int64 keys.
+//
+// The reference joins by sorting: the right (build) side is stably sorted by key
+// (operators.cu:1117-1157), each left (probe) segment is sorted carrying its row ids
+// (operators.cu:1184-1199), and moderngpu inner_join merges them into (left, right) pairs
+// (operators.cu:1239-1284).  Here the build side goes into an open-addressing table keyed
+// by MurmurHash64A (linear probing), and each probe row is looked up directly:
+//   * unique build keys (the primary-key case): a slot holds the build row;
+//   * duplicate build keys: the build side is radix-sorted by key (stable, so equal keys
+//     keep ascending row order) and a slot holds a run of equal keys.
+// Pairs come out in probe-row order and, for one probe row, in ascending build-row order;
+// the count pass records each probe row's slot so the write pass does not probe again.
+#include <string.h>
+
+#include "aln_internal.h"
+
+namespace aln {
+
+constexpr uint32_t J_EMPTY = 0xFFFFFFFFu;
+constexpr uint32_t J_SEED = 0x5bd1e995u;
+constexpr int J_BLOCK = 256;
+constexpr int J_ITEMS = 8;
+constexpr int J_TILE = J_BLOCK * J_ITEMS;
+
+// Build: slot value = index into `keys` (a build row, or a run of equal keys).
+__global__ void __launch_bounds__(256) k_join_insert(const int64_t *__restrict__ keys, uint64_t n,
+                                                     uint32_t *slot, int64_t *tk, uint64_t cap,
+                                                     uint32_t *dup) {
+  const uint64_t mask = cap - 1;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * 256) {
+    int64_t key = keys[i];
+    uint32_t id = (uint32_t)i;
+    uint64_t h = murmur64a_u64((uint64_t)key, J_SEED) & mask;
+    for (uint64_t probes = 0; probes <= cap; ++probes) {
+      uint32_t cur = slot[h];
+      if (cur == J_EMPTY) {
+        cur = atomicCAS(&slot[h], J_EMPTY, id);
+        if (cur == J_EMPTY) {
+          tk[h] = key;
+          break;
+        }
+      }
+      // occupied: compare with the key of the row (or run) that owns it
+      int64_t other = keys[cur];
+      if (other == key) {
+        atomicExch(dup, 1u);
+        break;
+      }
+      h = (h + 1) & mask;
+    }
+  }
+}
+
+__device__ __forceinline__ uint32_t join_find(const uint32_t *__restrict__ slot,
+                                              const int64_t *__restrict__ tk, uint64_t mask,
+                                              int64_t key) {
+  uint64_t h = murmur64a_u64((uint64_t)key, J_SEED) & mask;
+  for (uint64_t probes = 0; probes <= mask; ++probes) {
+    uint32_t s = slot[h];
+    if (s == J_EMPTY) return J_EMPTY;
+    if (tk[h] == key) return s;
+    h = (h + 1) & mask;
+  }
+  return J_EMPTY;
+}
+
+// PASS 0: per probe row, the matching slot value and per-tile pair counts.
+__global__ void __launch_bounds__(J_BLOCK) k_join_probe_count(
+    const int64_t *__restrict__ pk, uint64_t np, const uint32_t *__restrict__ slot,
+    const int64_t *__restrict__ tk, uint64_t mask, const uint32_t *__restrict__ rlen,
+    uint32_t *__restrict__ match, uint32_t *__restrict__ tcount) {
+  uint64_t base = (uint64_t)blockIdx.x * J_TILE;
+  uint32_t s = 0;
+  for (int j = 0; j < J_ITEMS; ++j) {
+    uint64_t p = base + (uint64_t)j * J_BLOCK + threadIdx.x;
+    if (p < np) {
+      uint32_t m = join_find(slot, tk, mask, pk[p]);
+      match[p] = m;
+      if (m != J_EMPTY) s += rlen ? rlen[m] : 1;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  __shared__ uint32_t wsum[J_BLOCK / 64];
+  if (__lane_id() == 0) wsum[threadIdx.x / 64] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int w = 0; w < J_BLOCK / 64; ++w) t += wsum[w];
+    tcount[blockIdx.x] = t;
+  }
+}
+
+__global__ void __launch_bounds__(J_BLOCK) k_join_probe_write(
+    uint64_t np, const uint32_t *__restrict__ match, const uint32_t *__restrict__ rstart,
+    const uint32_t *__restrict__ rlen, const uint32_t *__restrict__ srow,
+    const uint64_t *__restrict__ toff, uint32_t *__restrict__ out_p, uint32_t *__restrict__ out_b) {
+  __shared__ uint64_t wsum[J_BLOCK / 64];
+  // blocked: thread t owns probe rows base + t*ITEMS .. +ITEMS-1 (keeps probe-row order)
+  uint64_t base = (uint64_t)blockIdx.x * J_TILE + (uint64_t)threadIdx.x * J_ITEMS;
+  uint32_t cnt[J_ITEMS], m[J_ITEMS];
+  uint64_t s = 0;
+  for (int j = 0; j < J_ITEMS; ++j) {
+    uint64_t p = base + j;
+    m[j] = p < np ? match[p] : J_EMPTY;
+    cnt[j] = m[j] == J_EMPTY ? 0 : (rlen ? rlen[m[j]] : 1);
+    s += cnt[j];
+  }
+  unsigned lane = __lane_id(), wid = threadIdx.x / 64;
+  uint64_t x = s;
+  for (int o = 1; o < 64; o <<= 1) {
+    uint64_t y = __shfl_up(x, o, 64);
+    if (lane >= (unsigned)o) x += y;
+  }
+  if (lane == 63) wsum[wid] = x;
+  __syncthreads();
+  uint64_t before = 0;
+  for (int w = 0; w < (int)wid; ++w) before += wsum[w];
+  uint64_t pos = toff[blockIdx.x] + before + x - s;
+  for (int j = 0; j < J_ITEMS; ++j) {
+    if (!cnt[j]) continue;
+    uint32_t p = (uint32_t)(base + j);
+    if (!rlen) {
+      out_p[pos] = p;
+      out_b[pos] = m[j];
+      ++pos;
+    } else {
+      uint32_t st = rstart[m[j]];
+      for (uint32_t t = 0; t < cnt[j]; ++t, ++pos) {
+        out_p[pos] = p;
+        out_b[pos] = srow[st + t];
+      }
+    }
+  }
+}
+
+// Runs mode helpers
+__global__ void k_join_codes(const int64_t *k, uint64_t n, int64_t kmin, uint64_t *code,
+                             uint32_t *row) {
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * 256) {
+    code[i] = (uint64_t)k[i] - (uint64_t)kmin;
+    row[i] = (uint32_t)i;
+  }
+}
+__global__ void k_join_heads(const uint64_t *code, uint64_t n, uint32_t *head) {
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * 256)
+    head[i] = i == 0 || code[i] != code[i - 1];
+}
+__global__ void k_join_runs(const uint64_t *code, const uint32_t *head, const uint64_t *pos,
+                            uint64_t n, int64_t kmin, uint32_t *rstart, uint32_t *rlen,
+                            int64_t *rkey) {
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * 256) {
+    if (!head[i]) continue;
+    uint64_t r = pos[i];
+    uint64_t e = i + 1;
+    while (e < n && !head[e]) ++e;
+    rstart[r] = (uint32_t)i;
+    rlen[r] = (uint32_t)(e - i);
+    rkey[r] = (int64_t)((uint64_t)kmin + code[i]);
+  }
+}
+
+}  // namespace aln
+
+using namespace aln;
+
+extern "C" int aln_hash_join(aln_ctx *c, const int64_t *d_build_keys, uint64_t nbuild,
+                             const int64_t *d_probe_keys, uint64_t nprobe, uint32_t *d_probe_idx,
+                             uint32_t *d_build_idx, uint64_t cap, uint64_t *h_npairs) {
+  ALN_REQUIRE(c && h_npairs, "aln_hash_join: bad arguments");
+  ALN_REQUIRE(nbuild < 0xFFFFFFFFull && nprobe < 0xFFFFFFFFull, "aln_hash_join: more than 2^32-1 rows");
+  ALN_REQUIRE((nbuild == 0 || d_build_keys) && (nprobe == 0 || d_probe_keys),
+              "aln_hash_join: NULL keys");
+  ws_reset(c);
+  *h_npairs = 0;
+  if (nbuild == 0 || nprobe == 0) return ALN_OK;
+  uint64_t tcap = next_pow2(2 * nbuild < 1024 ? 1024 : 2 * nbuild);
+  uint32_t *slot, *dup;
+  int64_t *tk;
+  ALN_TRY(ws(c, tcap, &slot));
+  ALN_TRY(ws(c, tcap, &tk));
+  ALN_TRY(ws(c, 1, &dup));
+  ALN_TRY(fill_u32(c, slot, tcap, J_EMPTY));
+  ALN_TRY(fill_u32(c, dup, 1, 0));
+  {
+    ProfScope p(c, "join_build");
+    hipLaunchKernelGGL(k_join_insert, dim3(grid_for(nbuild, 256, 65536)), dim3(256), 0, c->stream,
+                       d_build_keys, nbuild, slot, tk, tcap, dup);
+  }
+  ALN_TRY(check_launch("join_build"));
+  uint32_t hdup = 0;
+  ALN_TRY(copy_to_host(c, &hdup, dup, 4));
+  uint32_t *rstart = nullptr, *rlen = nullptr, *srow = nullptr;
+  if (hdup) {
+    // runs mode: stable sort of the build side by key, one slot per run of equal keys
+    int64_t kmin, kmax;
+    ALN_TRY(minmax_i64(c, d_build_keys, nbuild, &kmin, &kmax));
+    uint64_t *code;
+    uint32_t *head;
+    uint64_t *pos;
+    ALN_TRY(ws(c, nbuild, &code));
+    ALN_TRY(ws(c, nbuild, &srow));
+    ALN_TRY(ws(c, nbuild, &head));
+    ALN_TRY(ws(c, nbuild, &pos));
+    hipLaunchKernelGGL(k_join_codes, dim3(grid_for(nbuild, 256, 65536)), dim3(256), 0, c->stream,
+                       d_build_keys, nbuild, kmin, code, srow);
+    ALN_TRY(check_launch("join_codes"));
+    ALN_TRY(radix_sort_pairs(c, code, srow, nbuild, bits_for_range_u((uint64_t)kmax - (uint64_t)kmin)));
+    hipLaunchKernelGGL(k_join_heads, dim3(grid_for(nbuild, 256, 65536)), dim3(256), 0, c->stream, code,
+                       nbuild, head);
+    ALN_TRY(check_launch("join_heads"));
+    uint64_t nruns = 0;
+    ALN_TRY(exclusive_scan_u32(c, head, nbuild, pos, &nruns));
+    int64_t *rkey;
+    ALN_TRY(ws(c, nruns, &rstart));
+    ALN_TRY(ws(c, nruns, &rlen));
+    ALN_TRY(ws(c, nruns, &rkey));
+    hipLaunchKernelGGL(k_join_runs, dim3(grid_for(nbuild, 256, 65536)), dim3(256), 0, c->stream, code,
+                       head, pos, nbuild, kmin, rstart, rlen, rkey);
+    ALN_TRY(check_launch("join_runs"));
+    ALN_TRY(fill_u32(c, slot, tcap, J_EMPTY));
+    {
+      ProfScope p(c, "join_build");
+      hipLaunchKernelGGL(k_join_insert, dim3(grid_for(nruns, 256, 65536)), dim3(256), 0, c->stream,
+                         (const int64_t *)rkey, nruns, slot, tk, tcap, dup);
+    }
+    ALN_TRY(check_launch("join_build_runs"));
+  }
+  uint64_t ntiles = (nprobe + J_TILE - 1) / J_TILE;
+  uint32_t *match, *tcount;
+  uint64_t *toff;
+  ALN_TRY(ws(c, nprobe, &match));
+  ALN_TRY(ws(c, ntiles, &tcount));
+  ALN_TRY(ws(c, ntiles, &toff));
+  {
+    ProfScope p(c, "join_probe");
+    hipLaunchKernelGGL(k_join_probe_count, dim3((unsigned)ntiles), dim3(J_BLOCK), 0, c->stream,
+                       d_probe_keys, nprobe, slot, tk, tcap - 1, rlen, match, tcount);
+  }
+  ALN_TRY(check_launch("join_probe_count"));
+  uint64_t total = 0;
+  ALN_TRY(exclusive_scan_u32(c, tcount, ntiles, toff, &total));
+  *h_npairs = total;
+  if (total > cap)
+    return set_error(ALN_E_CAPACITY, "aln_hash_join: %llu pairs, capacity %llu",
+                     (unsigned long long)total, (unsigned long long)cap);
+  if (total == 0) return ALN_OK;
+  ALN_REQUIRE(d_probe_idx && d_build_idx, "aln_hash_join: no output buffers");
+  {
+    ProfScope p(c, "join_write");
+    hipLaunchKernelGGL(k_join_probe_write, dim3((unsigned)ntiles), dim3(J_BLOCK), 0, c->stream, nprobe,
+                       match, rstart, rlen, srow, toff, d_probe_idx, d_build_idx);
+  }
+  ALN_TRY(check_launch("join_probe_write"));
+  ALN_HIP(hipStreamSynchronize(c->stream));
+  return ALN_OK;
+}

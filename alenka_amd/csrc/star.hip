@@ -1,0 +1,612 @@
+// Fused star pipeline: filter -> join -> join -> group-by -> top-k, the TPC-H q3 shape
+// the reference runs as FILTER x3, a two-way JOIN, a GROUP BY SELECT and ORDER ... LIMIT
+// (readme.md:48-80; emit_filter operators.cu:2306-2408, emit_multijoin 940-1582,
+// emit_select 1801-2084, emit_order 1641-1798, Store(limit) cm.cu:1493-1498).
+//
+// The group key (m_key, m_g1, m_g2) contains the mid table's unique join key, so every
+// group is one passing mid row, and the fact side aggregates straight into per-mid-row
+// accumulators found through the join's hash table.  Kernels:
+//   D  dim scan     : decode + range predicate; passing keys set bits in a bitmap over the
+//                     key range (a perfect hash of the dense dimension key).
+//   M  mid scan     : decode + ranges + bitmap probe of m_fk; passing rows are compacted
+//                     (key, g1, g2) with one atomic per tile.
+//   B  build        : open-addressing table, 8-byte slots (group index << 32 | key code),
+//                     MurmurHash64A slot hash, linear probing, capacity ~1.5x entries.
+//   F  fact scan    : decode + ranges + probe + SUM((a0+a1*x)*(b0+b1*y)), COUNT(*).  Fact
+//                     rows clustered on the join key hit the same group in adjacent lanes;
+//                     a segmented wave scan leaves one atomic pair per run.
+//   T  top-k        : bitonic sort of 1024-candidate tiles, keep `limit` per tile, repeat.
+// Zone maps: a segment whose min/max fails a range is skipped ('N'); one that lies inside
+// every range skips predicate evaluation ('A').
+#include <string.h>
+
+#include <vector>
+
+#include "aln_internal.h"
+
+namespace aln {
+
+constexpr int ST_BLOCK = 256;
+constexpr int ST_ITEMS = 16;
+constexpr int ST_TILE = ST_BLOCK * ST_ITEMS;
+constexpr uint64_t ST_EMPTY = ~0ull;
+constexpr uint32_t ST_SEED = 0x1b873593u;
+
+struct ScanDesc {
+  const uint32_t *words[ALN_MAX_COLS];
+  const aln_seg *segs[ALN_MAX_COLS];
+  int32_t rcol[ALN_MAX_RANGES];
+  int64_t rlo[ALN_MAX_RANGES], rhi[ALN_MAX_RANGES];
+  int nr;
+  uint32_t seg_rows, tps;  // tiles per segment
+  const char *modes;       // device, or null
+};
+
+__device__ __forceinline__ int64_t sd_decode(const ScanDesc &d, int c, uint32_t seg,
+                                             uint32_t local) {
+  return decode_row(d.words[c], d.segs[c][seg], local);
+}
+
+// Zone-map mode of a segment: 'N' skip, 'A' all rows pass the ranges, 'R' evaluate.
+__device__ __forceinline__ char sd_mode(const ScanDesc &d, uint32_t seg) {
+  if (d.modes) return d.modes[seg];
+  char m = 'A';
+  for (int k = 0; k < d.nr; ++k) {
+    const aln_seg &s = d.segs[d.rcol[k]][seg];
+    if (s.vmax < d.rlo[k] || s.vmin > d.rhi[k]) return 'N';
+    if (!(s.vmin >= d.rlo[k] && s.vmax <= d.rhi[k])) m = 'R';
+  }
+  return m;
+}
+
+__device__ __forceinline__ bool sd_pass(const ScanDesc &d, char mode, uint32_t seg,
+                                        uint32_t local) {
+  if (mode == 'A') return true;
+  bool p = true;
+  for (int k = 0; k < d.nr; ++k) {
+    int64_t v = sd_decode(d, d.rcol[k], seg, local);
+    p = p && v >= d.rlo[k] && v <= d.rhi[k];
+  }
+  return p;
+}
+
+// ---- D: dimension scan into a key bitmap --------------------------------------------------
+__global__ void __launch_bounds__(ST_BLOCK) k_star_dim(ScanDesc d, int kcol, int64_t kmin,
+                                                       uint64_t range, uint32_t *bitmap,
+                                                       unsigned long long *npass,
+                                                       uint32_t *dup) {
+  const uint32_t seg = blockIdx.x / d.tps, tile = blockIdx.x % d.tps;
+  const uint32_t nrows = d.segs[0][seg].nrows, r0 = tile * ST_TILE;
+  const char mode = sd_mode(d, seg);
+  if (r0 >= nrows || mode == 'N') return;
+  uint32_t cnt = 0;
+  for (int j = 0; j < ST_ITEMS; ++j) {
+    uint32_t local = r0 + j * ST_BLOCK + threadIdx.x;
+    if (local < nrows && sd_pass(d, mode, seg, local)) {
+      uint64_t b = (uint64_t)sd_decode(d, kcol, seg, local) - (uint64_t)kmin;
+      if (b < range) {
+        uint32_t bit = 1u << (b & 31);
+        uint32_t old = atomicOr(&bitmap[b >> 5], bit);
+        if (old & bit) atomicExch(dup, 1u);
+        ++cnt;
+      }
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o, 64);
+  if (__lane_id() == 0 && cnt) atomicAdd(npass, (unsigned long long)cnt);
+}
+
+// ---- M: mid scan, probe the dimension bitmap, compact passing rows ------------------------
+__global__ void __launch_bounds__(ST_BLOCK) k_star_mid(ScanDesc d, int kcol, int fkcol, int g1col,
+                                                       int g2col, const uint32_t *__restrict__ bitmap,
+                                                       int64_t dkmin, uint64_t drange,
+                                                       unsigned long long *counter, uint64_t cap,
+                                                       int64_t *__restrict__ okey,
+                                                       int64_t *__restrict__ og1,
+                                                       int64_t *__restrict__ og2) {
+  __shared__ uint32_t wc[ST_BLOCK / 64];
+  __shared__ unsigned long long base_s;
+  const uint32_t seg = blockIdx.x / d.tps, tile = blockIdx.x % d.tps;
+  const uint32_t nrows = d.segs[0][seg].nrows, r0 = tile * ST_TILE;
+  const char mode = sd_mode(d, seg);
+  if (r0 >= nrows || mode == 'N') return;
+  const unsigned lane = __lane_id(), wid = threadIdx.x / 64;
+  bool pass[ST_ITEMS];
+  uint32_t mine = 0;
+  for (int j = 0; j < ST_ITEMS; ++j) {
+    uint32_t local = r0 + j * ST_BLOCK + threadIdx.x;
+    bool p = local < nrows && sd_pass(d, mode, seg, local);
+    if (p) {
+      uint64_t b = (uint64_t)sd_decode(d, fkcol, seg, local) - (uint64_t)dkmin;
+      p = b < drange && ((bitmap[b >> 5] >> (b & 31)) & 1);
+    }
+    pass[j] = p;
+    mine += p;
+  }
+  uint32_t wsum = mine;
+  for (int o = 32; o > 0; o >>= 1) wsum += __shfl_xor(wsum, o, 64);
+  if (lane == 0) wc[wid] = wsum;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int w = 0; w < ST_BLOCK / 64; ++w) t += wc[w];
+    base_s = t ? atomicAdd(counter, (unsigned long long)t) : 0;
+  }
+  __syncthreads();
+  uint64_t pos = base_s;
+  for (int w = 0; w < (int)wid; ++w) pos += wc[w];
+  // order inside the tile: (item, lane) within the wave; fine, output order is free
+  uint32_t x = mine;
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= (unsigned)o) x += y;
+  }
+  pos += x - mine;
+  for (int j = 0; j < ST_ITEMS; ++j) {
+    if (!pass[j]) continue;
+    uint32_t local = r0 + j * ST_BLOCK + threadIdx.x;
+    if (pos < cap) {
+      okey[pos] = sd_decode(d, kcol, seg, local);
+      og1[pos] = g1col >= 0 ? sd_decode(d, g1col, seg, local) : 0;
+      og2[pos] = g2col >= 0 ? sd_decode(d, g2col, seg, local) : 0;
+    }
+    ++pos;
+  }
+}
+
+__device__ __forceinline__ uint64_t st_slot(uint32_t code, uint64_t cap) {
+  return __umul64hi(murmur64a_u64(code, ST_SEED), cap);
+}
+
+// ---- B: build the join table over the compacted mid rows ----------------------------------
+__global__ void __launch_bounds__(256) k_star_build(const int64_t *__restrict__ okey, uint64_t n,
+                                                    int64_t kmin, uint64_t *table, uint64_t cap,
+                                                    uint32_t *dup) {
+  for (uint64_t j = (uint64_t)blockIdx.x * 256 + threadIdx.x; j < n;
+       j += (uint64_t)gridDim.x * 256) {
+    uint32_t code = (uint32_t)((uint64_t)okey[j] - (uint64_t)kmin);
+    uint64_t entry = ((uint64_t)j << 32) | code;
+    uint64_t h = st_slot(code, cap);
+    for (uint64_t probes = 0; probes < cap; ++probes) {
+      unsigned long long old = atomicCAS((unsigned long long *)&table[h], (unsigned long long)ST_EMPTY,
+                                         (unsigned long long)entry);
+      if (old == ST_EMPTY) break;
+      if ((uint32_t)old == code) {
+        atomicExch(dup, 1u);
+        break;
+      }
+      h = h + 1 == cap ? 0 : h + 1;
+    }
+  }
+}
+
+__device__ __forceinline__ uint32_t st_find(const uint64_t *__restrict__ table, uint64_t cap,
+                                            uint32_t code) {
+  uint64_t h = st_slot(code, cap);
+  for (uint64_t probes = 0; probes < cap; ++probes) {
+    uint64_t e = table[h];
+    if (e == ST_EMPTY) return 0xFFFFFFFFu;
+    if ((uint32_t)e == code) return (uint32_t)(e >> 32);
+    h = h + 1 == cap ? 0 : h + 1;
+  }
+  return 0xFFFFFFFFu;
+}
+
+// ---- F: fact scan, probe, aggregate -------------------------------------------------------
+__global__ void __launch_bounds__(ST_BLOCK) k_star_fact(ScanDesc d, int fkcol, int xcol, int ycol,
+                                                        int64_t a0, int64_t a1, int64_t b0,
+                                                        int64_t b1, const uint64_t *__restrict__ table,
+                                                        uint64_t cap, int64_t mkmin, uint64_t mrange,
+                                                        unsigned long long *__restrict__ sum,
+                                                        unsigned int *__restrict__ cnt,
+                                                        unsigned long long *matched) {
+  const uint32_t seg = blockIdx.x / d.tps, tile = blockIdx.x % d.tps;
+  const uint32_t nrows = d.segs[0][seg].nrows, r0 = tile * ST_TILE;
+  const char mode = sd_mode(d, seg);
+  if (r0 >= nrows || mode == 'N') return;
+  const unsigned lane = __lane_id();
+  uint32_t nmatch = 0;
+  for (int j = 0; j < ST_ITEMS; ++j) {
+    uint32_t local = r0 + j * ST_BLOCK + threadIdx.x;
+    uint32_t g = 0xFFFFFFFFu;
+    int64_t v = 0;
+    if (local < nrows && sd_pass(d, mode, seg, local)) {
+      uint64_t code = (uint64_t)sd_decode(d, fkcol, seg, local) - (uint64_t)mkmin;
+      if (code < mrange) g = st_find(table, cap, (uint32_t)code);
+      if (g != 0xFFFFFFFFu) {
+        int64_t x = sd_decode(d, xcol, seg, local), y = sd_decode(d, ycol, seg, local);
+        v = wmul((int64_t)((uint64_t)a0 + (uint64_t)wmul(a1, x)),
+                 (int64_t)((uint64_t)b0 + (uint64_t)wmul(b1, y)));
+      }
+    }
+    // segmented reduction over runs of equal group index in adjacent lanes
+    uint32_t prev = __shfl_up(g, 1, 64);
+    bool f = lane == 0 || prev != g;
+    uint32_t c = g != 0xFFFFFFFFu;
+    for (int o = 1; o < 64; o <<= 1) {
+      int64_t vu = __shfl_up(v, o, 64);
+      uint32_t cu = __shfl_up(c, o, 64);
+      bool fu = __shfl_up((int)f, o, 64) != 0;
+      if (lane >= (unsigned)o && !f) {
+        v = (int64_t)((uint64_t)v + (uint64_t)vu);
+        c += cu;
+      }
+      if (lane >= (unsigned)o) f = f || fu;
+    }
+    uint32_t next = __shfl_down(g, 1, 64);
+    bool tail = g != 0xFFFFFFFFu && (lane == 63 || next != g);
+    if (tail) {
+      atomicAdd(&sum[g], (unsigned long long)v);
+      atomicAdd(&cnt[g], c);
+      nmatch += c;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) nmatch += __shfl_xor(nmatch, o, 64);
+  if (lane == 0 && nmatch) atomicAdd(matched, (unsigned long long)nmatch);
+}
+
+// ---- T: top-k over groups (sum DESC, g1 ASC, key ASC, g2 ASC) -----------------------------
+struct Cand {
+  int64_t sum, g1, key, g2;
+  uint32_t j, valid;
+};
+
+__device__ __forceinline__ bool cand_better(const Cand &a, const Cand &b) {
+  if (a.valid != b.valid) return a.valid > b.valid;
+  if (a.sum != b.sum) return a.sum > b.sum;
+  if (a.g1 != b.g1) return a.g1 < b.g1;
+  if (a.key != b.key) return a.key < b.key;
+  if (a.g2 != b.g2) return a.g2 < b.g2;
+  return a.j < b.j;
+}
+
+constexpr int TK_TILE2 = 1024;
+
+__global__ void __launch_bounds__(256) k_star_topk_tile(const Cand *__restrict__ in, uint64_t n,
+                                                        const unsigned long long *__restrict__ sum,
+                                                        const unsigned int *__restrict__ cnt,
+                                                        const int64_t *__restrict__ okey,
+                                                        const int64_t *__restrict__ og1,
+                                                        const int64_t *__restrict__ og2,
+                                                        uint32_t k, Cand *__restrict__ out) {
+  __shared__ Cand s[TK_TILE2];
+  uint64_t base = (uint64_t)blockIdx.x * TK_TILE2;
+  for (int t = threadIdx.x; t < TK_TILE2; t += 256) {
+    uint64_t i = base + t;
+    Cand c;
+    if (in) {
+      if (i < n) c = in[i];
+      else c.valid = 0;
+    } else if (i < n && cnt[i] > 0) {
+      c.sum = (int64_t)sum[i];
+      c.g1 = og1[i];
+      c.key = okey[i];
+      c.g2 = og2[i];
+      c.j = (uint32_t)i;
+      c.valid = 1;
+    } else {
+      c.valid = 0;
+    }
+    if (!c.valid) {
+      c.sum = c.g1 = c.key = c.g2 = 0;
+      c.j = 0xFFFFFFFFu;
+    }
+    s[t] = c;
+  }
+  __syncthreads();
+  for (int size = 2; size <= TK_TILE2; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int t = threadIdx.x; t < TK_TILE2 / 2; t += 256) {
+        int lo = 2 * t - (t & (stride - 1)), hi = lo + stride;
+        bool up = (lo & size) == 0;
+        Cand a = s[lo], b = s[hi];
+        bool sw = up ? cand_better(b, a) : cand_better(a, b);
+        if (sw) {
+          s[lo] = b;
+          s[hi] = a;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (uint32_t t = threadIdx.x; t < k; t += 256) out[(uint64_t)blockIdx.x * k + t] = s[t];
+}
+
+__global__ void k_star_group_flags(const unsigned int *cnt, uint64_t n, uint32_t *flag) {
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * 256)
+    flag[i] = cnt[i] > 0;
+}
+
+__global__ void k_star_group_codes(const uint32_t *flag, const uint64_t *pos, uint64_t n,
+                                   const int64_t *okey, int64_t kmin, uint64_t *code,
+                                   uint32_t *idx) {
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * 256)
+    if (flag[i]) {
+      code[pos[i]] = (uint64_t)okey[i] - (uint64_t)kmin;
+      idx[pos[i]] = (uint32_t)i;
+    }
+}
+
+__global__ void k_star_group_emit(const uint32_t *idx, uint64_t g, const int64_t *okey,
+                                  const int64_t *og1, const int64_t *og2,
+                                  const unsigned long long *sum, const unsigned int *cnt,
+                                  aln_star_row *out) {
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < g;
+       i += (uint64_t)gridDim.x * 256) {
+    uint32_t j = idx[i];
+    aln_star_row r;
+    r.key = okey[j];
+    r.g1 = og1[j];
+    r.g2 = og2[j];
+    r.sum = (int64_t)sum[j];
+    r.count = cnt[j];
+    out[i] = r;
+  }
+}
+
+// Reads the segment descriptors of one column to the host and returns min/max over them.
+static int col_range(aln_ctx *c, const aln_col &col, int64_t *mn, int64_t *mx) {
+  std::vector<aln_seg> h(col.nsegs);
+  ALN_TRY(copy_to_host(c, h.data(), col.segs, sizeof(aln_seg) * col.nsegs));
+  *mn = INT64_MAX;
+  *mx = INT64_MIN;
+  for (auto &s : h) {
+    if (s.nrows == 0) continue;
+    *mn = s.vmin < *mn ? s.vmin : *mn;
+    *mx = s.vmax > *mx ? s.vmax : *mx;
+  }
+  return ALN_OK;
+}
+
+static int make_scan(aln_ctx *c, const aln_table_scan &t, int need_col, ScanDesc *d,
+                     uint64_t *nblocks) {
+  ALN_REQUIRE(t.cols && t.ncols >= 1 && t.ncols <= ALN_MAX_COLS && t.seg_rows > 0,
+              "aln_star_agg: bad table scan");
+  ALN_REQUIRE(t.nranges >= 0 && t.nranges <= ALN_MAX_RANGES, "aln_star_agg: %d ranges", t.nranges);
+  ALN_REQUIRE(need_col < t.ncols, "aln_star_agg: column slot %d of %d", need_col, t.ncols);
+  memset(d, 0, sizeof(*d));
+  for (int i = 0; i < t.ncols; ++i) {
+    ALN_REQUIRE(t.cols[i].nsegs == t.cols[0].nsegs, "aln_star_agg: columns differ in segments");
+    d->words[i] = t.cols[i].words;
+    d->segs[i] = t.cols[i].segs;
+  }
+  for (int k = 0; k < t.nranges; ++k) {
+    ALN_REQUIRE(t.ranges[k].col >= 0 && t.ranges[k].col < t.ncols, "aln_star_agg: range column");
+    d->rcol[k] = t.ranges[k].col;
+    d->rlo[k] = t.ranges[k].lo;
+    d->rhi[k] = t.ranges[k].hi;
+  }
+  d->nr = t.nranges;
+  d->seg_rows = t.seg_rows;
+  d->tps = (t.seg_rows + ST_TILE - 1) / ST_TILE;
+  if (t.seg_modes) {
+    char *dm;
+    ALN_TRY(ws(c, t.cols[0].nsegs, &dm));
+    ALN_HIP(hipMemcpyAsync(dm, t.seg_modes, t.cols[0].nsegs, hipMemcpyHostToDevice, c->stream));
+    d->modes = dm;
+  }
+  *nblocks = (uint64_t)t.cols[0].nsegs * d->tps;
+  ALN_REQUIRE(*nblocks < (1ull << 31), "aln_star_agg: grid too large");
+  return ALN_OK;
+}
+
+struct PhaseTimer {
+  aln_ctx *c;
+  hipEvent_t e[5];
+  bool on;
+  explicit PhaseTimer(aln_ctx *c_) : c(c_), on(c_->profiling) {
+    for (auto &x : e) x = nullptr;
+    if (on)
+      for (auto &x : e)
+        if (hipEventCreate(&x) != hipSuccess) on = false;
+  }
+  void mark(int i) {
+    if (on) (void)hipEventRecord(e[i], c->stream);
+  }
+  double ms(int a, int b) {
+    float t = 0;
+    if (!on || hipEventElapsedTime(&t, e[a], e[b]) != hipSuccess) return 0.0;
+    return t;
+  }
+  ~PhaseTimer() {
+    for (auto &x : e)
+      if (x) (void)hipEventDestroy(x);
+  }
+};
+
+}  // namespace aln
+
+using namespace aln;
+
+extern "C" int aln_star_agg(aln_ctx *c, const aln_star_args *a, aln_star_row *h_top,
+                            uint32_t *h_ntop, aln_star_row *d_groups, uint64_t groups_cap,
+                            aln_star_stats *h_stats) {
+  ALN_REQUIRE(c && a && h_ntop, "aln_star_agg: bad arguments");
+  ALN_REQUIRE(a->limit <= TK_TILE2 / 2, "aln_star_agg: limit %u above %d", a->limit, TK_TILE2 / 2);
+  ALN_REQUIRE(a->limit == 0 || h_top, "aln_star_agg: no top-k buffer");
+  ws_reset(c);
+  *h_ntop = 0;
+  aln_star_stats st;
+  memset(&st, 0, sizeof(st));
+  PhaseTimer tm(c);
+
+  ScanDesc dd, md, fd;
+  uint64_t dblocks, mblocks, fblocks;
+  ALN_TRY(make_scan(c, a->dim, a->d_key, &dd, &dblocks));
+  ALN_TRY(make_scan(c, a->mid, a->m_key > a->m_fk ? a->m_key : a->m_fk, &md, &mblocks));
+  ALN_REQUIRE(a->m_g1 < a->mid.ncols && a->m_g2 < a->mid.ncols, "aln_star_agg: group columns");
+  ALN_TRY(make_scan(c, a->fact, a->f_fk, &fd, &fblocks));
+  ALN_REQUIRE(a->f_x >= 0 && a->f_x < a->fact.ncols && a->f_y >= 0 && a->f_y < a->fact.ncols,
+              "aln_star_agg: aggregate columns");
+
+  // key ranges from the zone maps
+  int64_t dkmin, dkmax, mkmin, mkmax;
+  ALN_TRY(col_range(c, a->dim.cols[a->d_key], &dkmin, &dkmax));
+  ALN_TRY(col_range(c, a->mid.cols[a->m_key], &mkmin, &mkmax));
+  if (a->dim.cols[0].nsegs == 0 || a->mid.cols[0].nsegs == 0 || dkmax < dkmin || mkmax < mkmin) {
+    if (h_stats) *h_stats = st;
+    return ALN_OK;
+  }
+  uint64_t drange = (uint64_t)dkmax - (uint64_t)dkmin + 1;
+  uint64_t mrange = (uint64_t)mkmax - (uint64_t)mkmin + 1;
+  if (drange == 0 || drange > (1ull << 34))
+    return set_error(ALN_E_UNSUPPORTED, "aln_star_agg: dimension key range %llu too wide for a bitmap",
+                     (unsigned long long)drange);
+  if (mrange == 0 || mrange >= 0xFFFFFFFFull)
+    return set_error(ALN_E_UNSUPPORTED, "aln_star_agg: mid key range %llu wider than 32 bits",
+                     (unsigned long long)mrange);
+  uint64_t nmid = 0;
+  for (uint32_t s = 0; s < a->mid.cols[0].nsegs; ++s) nmid += a->mid.seg_rows;  // upper bound
+
+  uint64_t bwords = (drange + 31) / 32;
+  uint32_t *bitmap, *dup;
+  unsigned long long *ctr;
+  ALN_TRY(ws(c, bwords, &bitmap));
+  ALN_TRY(ws(c, 2, &dup));
+  ALN_TRY(ws(c, 4, &ctr));
+  ALN_TRY(fill_u32(c, bitmap, bwords, 0));
+  ALN_TRY(fill_u32(c, dup, 2, 0));
+  ALN_TRY(fill_u64(c, (uint64_t *)ctr, 4, 0));
+
+  tm.mark(0);
+  {
+    ProfScope p(c, "star_dim");
+    hipLaunchKernelGGL(k_star_dim, dim3((unsigned)dblocks), dim3(ST_BLOCK), 0, c->stream, dd, a->d_key,
+                       dkmin, drange, bitmap, ctr + 0, dup + 0);
+  }
+  ALN_TRY(check_launch("star_dim"));
+  tm.mark(1);
+
+  // mid scan: compact passing rows (capacity = all mid rows, at most)
+  int64_t *okey, *og1, *og2;
+  ALN_TRY(ws(c, nmid, &okey));
+  ALN_TRY(ws(c, nmid, &og1));
+  ALN_TRY(ws(c, nmid, &og2));
+  {
+    ProfScope p(c, "star_mid");
+    hipLaunchKernelGGL(k_star_mid, dim3((unsigned)mblocks), dim3(ST_BLOCK), 0, c->stream, md, a->m_key,
+                       a->m_fk, a->m_g1, a->m_g2, bitmap, dkmin, drange, ctr + 1, nmid, okey, og1, og2);
+  }
+  ALN_TRY(check_launch("star_mid"));
+  unsigned long long hc[2];
+  uint32_t hdup[2];
+  ALN_TRY(copy_to_host(c, hc, ctr, 16));
+  ALN_TRY(copy_to_host(c, hdup, dup, 8));
+  if (hdup[0]) return set_error(ALN_E_UNSUPPORTED, "aln_star_agg: duplicate dimension keys");
+  st.dim_pass = hc[0];
+  uint64_t m = hc[1];
+  st.mid_pass = m;
+
+  uint64_t tcap = m + m / 2 + 1024;
+  uint64_t *table;
+  unsigned long long *sum;
+  unsigned int *cnt;
+  ALN_TRY(ws(c, tcap, &table));
+  ALN_TRY(ws(c, m ? m : 1, &sum));
+  ALN_TRY(ws(c, m ? m : 1, &cnt));
+  ALN_TRY(fill_u64(c, table, tcap, ST_EMPTY));
+  ALN_TRY(fill_u64(c, (uint64_t *)sum, m, 0));
+  ALN_TRY(fill_u32(c, cnt, m, 0));
+  if (m) {
+    ProfScope p(c, "star_build");
+    hipLaunchKernelGGL(k_star_build, dim3(grid_for(m, 256, 65536)), dim3(256), 0, c->stream, okey, m,
+                       mkmin, table, tcap, dup + 1);
+  }
+  ALN_TRY(check_launch("star_build"));
+  tm.mark(2);
+  if (m) {
+    ProfScope p(c, "star_fact");
+    hipLaunchKernelGGL(k_star_fact, dim3((unsigned)fblocks), dim3(ST_BLOCK), 0, c->stream, fd, a->f_fk,
+                       a->f_x, a->f_y, a->a0, a->a1, a->b0, a->b1, table, tcap, mkmin, mrange, sum, cnt,
+                       ctr + 2);
+  }
+  ALN_TRY(check_launch("star_fact"));
+  tm.mark(3);
+
+  // top-k
+  uint32_t k = a->limit;
+  if (k && m) {
+    const Cand *in = nullptr;
+    uint64_t n = m;
+    while (true) {
+      uint64_t nt = (n + TK_TILE2 - 1) / TK_TILE2;
+      Cand *out;
+      ALN_TRY(ws(c, nt * k, &out));
+      {
+        ProfScope p(c, "star_topk");
+        hipLaunchKernelGGL(k_star_topk_tile, dim3((unsigned)nt), dim3(256), 0, c->stream, in, n, sum, cnt,
+                           okey, og1, og2, k, out);
+      }
+      ALN_TRY(check_launch("star_topk"));
+      in = out;
+      n = nt * k;
+      if (nt == 1) break;
+    }
+    std::vector<Cand> hcand(k);
+    ALN_TRY(copy_to_host(c, hcand.data(), in, sizeof(Cand) * k));
+    uint32_t nt = 0;
+    for (uint32_t i = 0; i < k; ++i) {
+      if (!hcand[i].valid) break;
+      h_top[nt].key = hcand[i].key;
+      h_top[nt].g1 = hcand[i].g1;
+      h_top[nt].g2 = hcand[i].g2;
+      h_top[nt].sum = hcand[i].sum;
+      h_top[nt].count = 0;
+      ++nt;
+    }
+    // counts of the winners
+    for (uint32_t i = 0; i < nt; ++i) {
+      unsigned int cc;
+      ALN_TRY(copy_to_host(c, &cc, cnt + hcand[i].j, 4));
+      h_top[i].count = cc;
+    }
+    *h_ntop = nt;
+  }
+  tm.mark(4);
+  ALN_TRY(copy_to_host(c, hc, ctr + 2, 8));
+  uint32_t bdup = 0;
+  ALN_TRY(copy_to_host(c, &bdup, dup + 1, 4));
+  if (bdup) return set_error(ALN_E_UNSUPPORTED, "aln_star_agg: duplicate mid keys");
+  st.fact_pass = hc[0];
+
+  // every group, in ascending key order
+  if ((a->flags & 1) && m) {
+    uint32_t *flag, *idx;
+    uint64_t *pos, *code;
+    ALN_TRY(ws(c, m, &flag));
+    ALN_TRY(ws(c, m, &pos));
+    hipLaunchKernelGGL(k_star_group_flags, dim3(grid_for(m, 256, 65536)), dim3(256), 0, c->stream, cnt, m,
+                       flag);
+    ALN_TRY(check_launch("star_group_flags"));
+    uint64_t g = 0;
+    ALN_TRY(exclusive_scan_u32(c, flag, m, pos, &g));
+    st.groups = g;
+    if (g > groups_cap) {
+      if (h_stats) *h_stats = st;
+      return set_error(ALN_E_CAPACITY, "aln_star_agg: %llu groups, capacity %llu",
+                       (unsigned long long)g, (unsigned long long)groups_cap);
+    }
+    if (g) {
+      ALN_REQUIRE(d_groups, "aln_star_agg: no group buffer");
+      ALN_TRY(ws(c, g, &code));
+      ALN_TRY(ws(c, g, &idx));
+      hipLaunchKernelGGL(k_star_group_codes, dim3(grid_for(m, 256, 65536)), dim3(256), 0, c->stream, flag,
+                         pos, m, okey, mkmin, code, idx);
+      ALN_TRY(check_launch("star_group_codes"));
+      ALN_TRY(radix_sort_pairs(c, code, idx, g, bits_for_range_u(mrange - 1)));
+      hipLaunchKernelGGL(k_star_group_emit, dim3(grid_for(g, 256, 65536)), dim3(256), 0, c->stream, idx, g,
+                         okey, og1, og2, sum, cnt, d_groups);
+      ALN_TRY(check_launch("star_group_emit"));
+      ALN_HIP(hipStreamSynchronize(c->stream));
+    }
+  }
+  ALN_HIP(hipStreamSynchronize(c->stream));
+  st.ms_dim = tm.ms(0, 1);
+  st.ms_mid = tm.ms(1, 2);
+  st.ms_fact = tm.ms(2, 3);
+  st.ms_topk = tm.ms(3, 4);
+  if (h_stats) *h_stats = st;
+  return ALN_OK;
+}

@@ -1,0 +1,334 @@
+"""Python handles over the C ABI: a context bound to torch's current HIP stream, the
+segmented bit-packed column layout, and one function per operator entry point.
+
+torch provides device memory and the stream; every computation runs in
+libalenka_hip.so.  Shapes are checked here before any launch.
+"""
+import ctypes as C
+
+import torch
+
+from . import _lib as L
+
+
+def _p(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else C.c_void_p(0)
+
+
+class Context:
+    """An aln_ctx on `device`, enqueueing on torch's current stream of that device."""
+
+    def __init__(self, device=0):
+        if not torch.cuda.is_available():
+            raise RuntimeError("alenka_amd needs a HIP device; none is visible")
+        self.lib = L.load()
+        self.device = torch.device("cuda", device)
+        torch.cuda.set_device(self.device)
+        self.stream = torch.cuda.current_stream(self.device)
+        h = C.c_void_p()
+        L.check(self.lib.aln_create(device, C.c_void_p(self.stream.cuda_stream), C.byref(h)),
+                "aln_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.aln_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sync(self):
+        L.check(self.lib.aln_sync(self.h), "aln_sync")
+
+    # ---- profiling -------------------------------------------------------------------
+    def profile(self, on=True):
+        L.check(self.lib.aln_profile_enable(self.h, int(on)))
+
+    def profile_reset(self):
+        L.check(self.lib.aln_profile_reset(self.h))
+
+    def profile_get(self, kernel):
+        ms = C.c_double()
+        n = C.c_uint64()
+        L.check(self.lib.aln_profile_get(self.h, kernel.encode(), C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def empty(self, n, dtype=torch.int64):
+        return torch.empty(max(1, int(n)), dtype=dtype, device=self.device)
+
+
+class Column:
+    """One column in the segmented FOR bit-packed layout, resident in HBM."""
+
+    def __init__(self, words, dsegs, hsegs, nrows, seg_rows, decimals, keep=None):
+        self.words = words        # torch int32 tensor (the 32-bit words)
+        self.dsegs = dsegs        # torch uint8 tensor holding nsegs aln_seg records
+        self.hsegs = hsegs        # ctypes array of Seg (host copy of the zone maps)
+        self.nrows = int(nrows)
+        self.seg_rows = int(seg_rows)
+        self.decimals = int(decimals)
+        self.keep = keep          # tensors a plain column aliases
+
+    @property
+    def nsegs(self):
+        return len(self.hsegs)
+
+    @classmethod
+    def encode(cls, ctx, vals, seg_rows, decimals=0, force_bits=0):
+        """FOR + dense bit-packing of an int64 device tensor, seg_rows rows per segment."""
+        assert vals.dtype == torch.int64 and vals.is_cuda and vals.is_contiguous()
+        n = vals.numel()
+        nsegs = (n + seg_rows - 1) // seg_rows
+        hsegs = (L.Seg * max(1, nsegs))()
+        words = C.c_uint64()
+        L.check(ctx.lib.aln_encode_plan(ctx.h, _p(vals), n, seg_rows, force_bits, hsegs,
+                                        C.byref(words)), "aln_encode_plan")
+        dwords = ctx.empty(words.value, torch.int32)
+        dsegs = ctx.empty(max(1, nsegs) * C.sizeof(L.Seg), torch.uint8)
+        L.check(ctx.lib.aln_encode_pack(ctx.h, _p(vals), n, seg_rows, hsegs, _p(dwords), _p(dsegs)),
+                "aln_encode_pack")
+        hs = (L.Seg * nsegs)()
+        C.memmove(hs, hsegs, C.sizeof(L.Seg) * nsegs)
+        return cls(dwords, dsegs, hs, n, seg_rows, decimals)
+
+    @classmethod
+    def plain(cls, ctx, vals, seg_rows, decimals=0):
+        """An int64 device tensor used in place as a 64-bit column (no copy)."""
+        assert vals.dtype == torch.int64 and vals.is_cuda and vals.is_contiguous()
+        n = vals.numel()
+        nsegs = (n + seg_rows - 1) // seg_rows
+        dsegs = ctx.empty(max(1, nsegs) * C.sizeof(L.Seg), torch.uint8)
+        col = L.Col()
+        L.check(ctx.lib.aln_plain_col(ctx.h, _p(vals), n, seg_rows, decimals, _p(dsegs),
+                                      C.byref(col)), "aln_plain_col")
+        hs = (L.Seg * nsegs)()
+        for s in range(nsegs):
+            r0 = s * seg_rows
+            hs[s].word_off, hs[s].base, hs[s].bits = 2 * r0, 0, 64
+            hs[s].nrows = min(seg_rows, n - r0)
+            hs[s].vmin, hs[s].vmax = -(1 << 63), (1 << 63) - 1
+        return cls(vals.view(torch.int32), dsegs, hs, n, seg_rows, decimals, keep=vals)
+
+    def desc(self):
+        c = L.Col()
+        c.words = self.words.data_ptr()
+        c.segs = self.dsegs.data_ptr()
+        c.nsegs = self.nsegs
+        c.decimals = self.decimals
+        return c
+
+    def packed_bytes(self, segs=None):
+        """Stored bytes of the given segments (all by default): ceil(nrows*bits/8) each."""
+        idx = range(self.nsegs) if segs is None else segs
+        return sum((self.hsegs[s].nrows * self.hsegs[s].bits + 7) // 8 for s in idx)
+
+    def decode(self, ctx, begin=0, end=None):
+        end = self.nrows if end is None else end
+        out = ctx.empty(end - begin)
+        d = self.desc()
+        L.check(ctx.lib.aln_decode(ctx.h, C.byref(d), self.seg_rows, begin, end, _p(out)),
+                "aln_decode")
+        return out[: end - begin]
+
+
+def _cols_array(cols):
+    arr = (L.Col * len(cols))()
+    for i, c in enumerate(cols):
+        arr[i] = c.desc()
+    return arr
+
+
+def scan_filter(ctx, cols, prog, seg_modes=None, cap=None):
+    """Ascending global row ids of the rows that pass `prog` (compiled predicate)."""
+    seg_rows = cols[0].seg_rows
+    arr = _cols_array(cols)
+    ins = L.insn_array(prog)
+    n = cols[0].nrows
+    cap = n if cap is None else cap
+    out = ctx.empty(cap, torch.int32)
+    cnt = C.c_uint64()
+    modes = None if seg_modes is None else "".join(seg_modes).encode()
+    L.check(ctx.lib.aln_scan_filter(ctx.h, arr, len(cols), seg_rows, ins, len(prog), modes,
+                                    _p(out), cap, C.byref(cnt)), "aln_scan_filter")
+    return out[: cnt.value]
+
+
+def gather(ctx, col, rows=None, n=None):
+    n = rows.numel() if rows is not None else (col.nrows if n is None else n)
+    out = ctx.empty(n)
+    d = col.desc()
+    L.check(ctx.lib.aln_gather(ctx.h, C.byref(d), col.seg_rows, _p(rows), n, _p(out)), "aln_gather")
+    return out[:n]
+
+
+def eval_rows(ctx, cols, prog, rows=None, n=None):
+    n = rows.numel() if rows is not None else (cols[0].nrows if n is None else n)
+    out = ctx.empty(n)
+    arr = _cols_array(cols)
+    ins = L.insn_array(prog)
+    L.check(ctx.lib.aln_eval(ctx.h, arr, len(cols), cols[0].seg_rows, ins, len(prog), _p(rows), n,
+                             _p(out)), "aln_eval")
+    return out[:n]
+
+
+def apply_perm(ctx, vals, perm):
+    out = ctx.empty(vals.numel())
+    L.check(ctx.lib.aln_apply_perm(ctx.h, _p(vals), _p(perm), vals.numel(), _p(out)), "aln_apply_perm")
+    return out[: vals.numel()]
+
+
+def hash_join(ctx, build_keys, probe_keys, cap=None):
+    """(probe_idx, build_idx) int32 tensors in probe-row order."""
+    nb, npb = build_keys.numel(), probe_keys.numel()
+    cap = max(npb, 1) if cap is None else cap
+    while True:
+        pi = ctx.empty(cap, torch.int32)
+        bi = ctx.empty(cap, torch.int32)
+        cnt = C.c_uint64()
+        rc = ctx.lib.aln_hash_join(ctx.h, _p(build_keys), nb, _p(probe_keys), npb, _p(pi), _p(bi), cap,
+                                   C.byref(cnt))
+        if rc == L.ALN_E_CAPACITY and cnt.value > cap:
+            cap = cnt.value
+            continue
+        L.check(rc, "aln_hash_join")
+        return pi[: cnt.value], bi[: cnt.value]
+
+
+def groupby(ctx, keys, vals, aggs, order=L.ORDER_KEYS, seed=100, cap=None):
+    """Returns (out_keys list, out_vals list) trimmed to the group count."""
+    n = keys[0].numel()
+    cap = max(n, 1) if cap is None else cap
+    ok = [ctx.empty(cap) for _ in keys]
+    ov = [ctx.empty(cap) for _ in vals]
+    kp = (C.c_void_p * len(keys))(*[k.data_ptr() for k in keys])
+    vp = (C.c_void_p * max(1, len(vals)))(*[(v.data_ptr() if v is not None else 0) for v in vals])
+    okp = (C.c_void_p * len(ok))(*[k.data_ptr() for k in ok])
+    ovp = (C.c_void_p * max(1, len(ov)))(*[v.data_ptr() for v in ov])
+    ag = (C.c_int32 * max(1, len(aggs)))(*aggs)
+    g = C.c_uint64()
+    L.check(ctx.lib.aln_groupby(ctx.h, kp, len(keys), vp, ag, len(vals), n, order, seed, okp, ovp, cap,
+                                C.byref(g)), "aln_groupby")
+    return [k[: g.value] for k in ok], [v[: g.value] for v in ov]
+
+
+def sort_keys(ctx, keys, desc=None):
+    n = keys[0].numel()
+    desc = desc or [0] * len(keys)
+    perm = ctx.empty(n, torch.int32)
+    kp = (C.c_void_p * len(keys))(*[k.data_ptr() for k in keys])
+    dp = (C.c_int32 * len(keys))(*desc)
+    L.check(ctx.lib.aln_sort_keys(ctx.h, kp, dp, len(keys), n, _p(perm)), "aln_sort_keys")
+    return perm[:n]
+
+
+def topk(ctx, keys, desc, k):
+    n = keys[0].numel()
+    out = ctx.empty(max(k, 1), torch.int32)
+    kp = (C.c_void_p * len(keys))(*[x.data_ptr() for x in keys])
+    dp = (C.c_int32 * len(keys))(*desc)
+    nout = C.c_uint32()
+    L.check(ctx.lib.aln_topk(ctx.h, kp, dp, len(keys), n, k, _p(out), C.byref(nout)), "aln_topk")
+    return out[: nout.value]
+
+
+def radix_sort_pairs(ctx, keys_u64, vals_u32, bits):
+    L.check(ctx.lib.aln_radix_sort_pairs(ctx.h, _p(keys_u64), _p(vals_u32), keys_u64.numel(), bits),
+            "aln_radix_sort_pairs")
+
+
+def hash_rows(ctx, keys, seed):
+    n = keys[0].numel()
+    out = ctx.empty(n)
+    kp = (C.c_void_p * len(keys))(*[k.data_ptr() for k in keys])
+    L.check(ctx.lib.aln_hash_rows(ctx.h, kp, len(keys), n, seed, _p(out)), "aln_hash_rows")
+    ctx.sync()
+    return out[:n]
+
+
+# ---- synthetic workloads -----------------------------------------------------------------
+def gen_flat(ctx, seed, n):
+    t = [ctx.empty(n) for _ in range(4)]
+    L.check(ctx.lib.aln_gen_flat(ctx.h, seed, n, *[_p(x) for x in t]), "aln_gen_flat")
+    return [x[:n] for x in t]
+
+
+def gen_dim(ctx, seed, n):
+    t = [ctx.empty(n) for _ in range(2)]
+    L.check(ctx.lib.aln_gen_dim(ctx.h, seed, n, *[_p(x) for x in t]), "aln_gen_dim")
+    return [x[:n] for x in t]
+
+
+def gen_orders(ctx, seed, n, ndim):
+    t = [ctx.empty(n) for _ in range(4)]
+    L.check(ctx.lib.aln_gen_orders(ctx.h, seed, n, ndim, *[_p(x) for x in t]), "aln_gen_orders")
+    return [x[:n] for x in t]
+
+
+def gen_fact_rows(ctx, seed, norders):
+    n = C.c_uint64()
+    L.check(ctx.lib.aln_gen_fact_rows(ctx.h, seed, norders, C.byref(n)), "aln_gen_fact_rows")
+    return n.value
+
+
+def gen_fact(ctx, seed, norders, cols=(True, True, True, True)):
+    n = gen_fact_rows(ctx, seed, norders)
+    t = [ctx.empty(n) if want else None for want in cols]
+    L.check(ctx.lib.aln_gen_fact(ctx.h, seed, norders, *[_p(x) for x in t]), "aln_gen_fact")
+    return [x[:n] if x is not None else None for x in t]
+
+
+def dict_first_seen(ctx, vals, nvals):
+    code_of = (C.c_int64 * nvals)()
+    out = ctx.empty(vals.numel())
+    L.check(ctx.lib.aln_dict_first_seen(ctx.h, _p(vals), vals.numel(), nvals, code_of, _p(out)),
+            "aln_dict_first_seen")
+    return list(code_of), out[: vals.numel()]
+
+
+def star_agg(ctx, dim, mid, fact, spec, limit=10, want_groups=False, groups_cap=None):
+    """Fused filter-join-join-aggregate-top-k.  dim/mid/fact: dicts with 'cols' (list of
+    Column), 'ranges' (list of (col, lo, hi)) and optional 'modes'.  spec: dict of the
+    column slots d_key, m_key, m_fk, m_g1, m_g2, f_fk, f_x, f_y and a0, a1, b0, b1."""
+    keep = []
+
+    def scan(t):
+        cols = _cols_array(t["cols"])
+        rg = (L.Range * max(1, len(t["ranges"])))()
+        for i, (c, lo, hi) in enumerate(t["ranges"]):
+            rg[i].col, rg[i].lo, rg[i].hi = c, lo, hi
+        ts = L.TableScan()
+        ts.cols = cols
+        ts.ncols = len(t["cols"])
+        ts.seg_rows = t["cols"][0].seg_rows
+        modes = t.get("modes")
+        ts.seg_modes = None if modes is None else "".join(modes).encode()
+        ts.ranges = rg
+        ts.nranges = len(t["ranges"])
+        keep.extend([cols, rg, ts.seg_modes])
+        return ts
+
+    a = L.StarArgs()
+    a.dim, a.mid, a.fact = scan(dim), scan(mid), scan(fact)
+    for k in ("d_key", "m_key", "m_fk", "m_g1", "m_g2", "f_fk", "f_x", "f_y", "a0", "a1", "b0", "b1"):
+        setattr(a, k, spec[k])
+    a.limit = limit
+    a.flags = 1 if want_groups else 0
+    top = (L.StarRow * max(1, limit))()
+    ntop = C.c_uint32()
+    st = L.StarStats()
+    groups = None
+    cap = 0
+    if want_groups:
+        cap = groups_cap or mid["cols"][0].nrows
+        groups = ctx.empty(max(1, cap) * C.sizeof(L.StarRow), torch.uint8)
+    L.check(ctx.lib.aln_star_agg(ctx.h, C.byref(a), top, C.byref(ntop), _p(groups), cap, C.byref(st)),
+            "aln_star_agg")
+    rows = [(top[i].key, top[i].g1, top[i].g2, top[i].sum, top[i].count) for i in range(ntop.value)]
+    g = None
+    if want_groups:
+        g = groups[: st.groups * C.sizeof(L.StarRow)].view(torch.int64).view(-1, 5)
+    return rows, g, st

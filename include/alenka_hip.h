@@ -1,0 +1,275 @@
+/*
+ * alenka_hip.h - C ABI of the MI355X relational operator hot path
+ * (filter -> decode -> join -> group-by -> order/top-k).
+ *
+ * Conventions
+ *   - every entry point returns an int status: ALN_OK (0) or a negative ALN_E_* code;
+ *     aln_last_error() gives the message of the calling thread's last failure.  Nothing
+ *     calls exit(); the host layer turns a status into the reference's process_error().
+ *   - "d_" arguments are device pointers (HBM), "h_" arguments host pointers.
+ *   - all device work is enqueued on the context's stream; entry points that return a
+ *     count through an h_ pointer synchronise that stream before returning.
+ *   - an aln_ctx is not re-entrant: use one per host thread / stream.
+ *
+ * Reference interfaces replaced (file:line in antonmks/Alenka):
+ *   filter()          filter.h:18-19, filter.cu:133-643      -> aln_compile_rpn + aln_scan_filter
+ *   zone_map_check()  zone_map.h:17, zone_map.cu:720-1135    -> aln_zone_map_check
+ *   filter_op()       cm.cu:3609-3666                        -> aln_scan_filter
+ *   pfor_decompress() compress.cu:180-267                    -> aln_decode / fused decode in scans
+ *   pfor_compress()   compress.cu:411-527                    -> aln_encode_plan + aln_encode_pack
+ *   mygather/copyColumns cm.cu:3305-3413                     -> aln_gather, aln_eval
+ *   emit_multijoin + mgpu::inner_join operators.cu:940-1582  -> aln_hash_join
+ *   CudaSet::GroupBy + select() segreduce cm.cu:910-956, select.cu:481-580 -> aln_groupby
+ *   add()/count_avg() merge.cu:92-496 (hash-ordered merge)   -> aln_groupby(order=ALN_ORDER_HASH)
+ *   update_permutation/apply_permutation sorts.cu:17-125     -> aln_sort_keys, aln_apply_perm
+ *   emit_order + Store(limit) operators.cu:1641-1798, cm.cu:1493-1498 -> aln_topk
+ *   MurmurHash64A     MurmurHash2_64.cu:63-104               -> aln_murmur64a, aln_hash_rows
+ */
+#ifndef ALENKA_HIP_H
+#define ALENKA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------------------ */
+#define ALN_OK 0
+#define ALN_E_ARG (-1)         /* invalid argument / shape                                */
+#define ALN_E_HIP (-2)         /* a HIP runtime call failed                               */
+#define ALN_E_NOMEM (-3)       /* device allocation failed                                */
+#define ALN_E_UNSUPPORTED (-4) /* construct the reference also rejects or leaves undefined */
+#define ALN_E_CAPACITY (-5)    /* output capacity too small: *h_count holds the need      */
+
+const char *aln_last_error(void);
+const char *aln_version(void);
+
+/* ---- context ----------------------------------------------------------------------- */
+typedef struct aln_ctx aln_ctx;
+/* stream: a hipStream_t (NULL = the device's null stream). */
+int aln_create(int device, void *stream, aln_ctx **out);
+void aln_destroy(aln_ctx *ctx);
+int aln_set_stream(aln_ctx *ctx, void *stream);
+int aln_sync(aln_ctx *ctx);
+/* Per-kernel timing with HIP events recorded on the context stream around every launch
+ * (off by default).  aln_profile_get returns the summed milliseconds and launch count of
+ * one kernel name since the last aln_profile_reset. */
+int aln_profile_enable(aln_ctx *ctx, int on);
+int aln_profile_reset(aln_ctx *ctx);
+int aln_profile_get(aln_ctx *ctx, const char *kernel, double *h_ms, uint64_t *h_launches);
+
+/* ---- RPN as the reference's parser queues carry it (operators.cu:68-233) ------------- */
+enum {
+  ALN_RPN_NAME = 1,   /* arg = column slot                                              */
+  ALN_RPN_NUMBER = 2, /* val = integer digits, arg = precision (emit_decimal)          */
+  ALN_RPN_STRING = 3, /* val = dictionary code of the literal in the column it is
+                         compared with, -1 if the literal is not in the dictionary      */
+  ALN_RPN_CMP = 4,    /* arg = comparison code, fl.l:121-127: 1 <, 2 >, 3 !=, 4 ==, 5 <=, 6 >= */
+  ALN_RPN_AND = 5,
+  ALN_RPN_OR = 6,
+  ALN_RPN_ADD = 7,
+  ALN_RPN_MINUS = 8,
+  ALN_RPN_MUL = 9,
+  ALN_RPN_DIV = 10
+};
+typedef struct {
+  int32_t type;
+  int32_t arg;
+  int64_t val;
+} aln_rpn;
+
+/* ---- resolved device program (decimal scaling already folded in) -------------------- */
+enum {
+  ALN_I_COL = 1,   /* push col[a] * k                         */
+  ALN_I_CONST = 2, /* push k                                  */
+  ALN_I_ADD = 3,   /* r = pop, l = pop, push l + r           */
+  ALN_I_SUB = 4,
+  ALN_I_MUL = 5,
+  ALN_I_DIV = 6, /* l / r, truncating; r == 0 yields 0        */
+  ALN_I_CMP = 7, /* a = comparison code, push (l op r)        */
+  ALN_I_AND = 8,
+  ALN_I_OR = 9
+};
+typedef struct {
+  int32_t op;
+  int32_t a;
+  int64_t k;
+} aln_insn;
+#define ALN_MAX_INSN 64
+#define ALN_MAX_STACK 16
+#define ALN_MAX_COLS 16
+
+/* Compile a reference RPN predicate (mode 0, filter.cu:133-643 semantics) or arithmetic
+ * expression (mode 1, select.cu:703-790 semantics) into a device program.  col_dec holds
+ * decimal_zeroes of every column slot.  *out_prec receives the result precision (mode 1). */
+int aln_compile_rpn(const aln_rpn *rpn, int nrpn, const int32_t *col_dec, int ncols, int mode,
+                    aln_insn *out, int *nout, int32_t *out_prec);
+
+/* Zone-map classification of one segment (zone_map.cu:720-1135): 'A' all rows pass,
+ * 'N' none, 'R' evaluate.  seg_min/seg_max: per column slot, decoded values. */
+int aln_zone_map_check(const aln_rpn *rpn, int nrpn, const int64_t *seg_min,
+                       const int64_t *seg_max, const int32_t *col_dec, int ncols,
+                       int not_compressed, char *out);
+
+/* ---- compressed column layout (the CudaSet columns of cm.h:681-789, resident in HBM) -- */
+/* One segment of one column: frame-of-reference codes (value - base), `bits` wide, packed
+ * LSB-first and densely into 32-bit words starting at words[word_off].  bits == 0 means
+ * every value equals base; a plain int64 array is bits == 64, base == 0. */
+typedef struct {
+  uint64_t word_off;
+  int64_t base;
+  int64_t vmin; /* zone map: smallest decoded value of the segment */
+  int64_t vmax; /* zone map: largest decoded value of the segment  */
+  uint32_t bits;
+  uint32_t nrows;
+} aln_seg;
+
+typedef struct {
+  const uint32_t *words; /* device */
+  const aln_seg *segs;   /* device, nsegs entries */
+  uint32_t nsegs;
+  int32_t decimals; /* decimal_zeroes (cm.cu:2941-2946) */
+} aln_col;
+
+/* Encoding: plan computes per-segment min/max/bits/word offsets (h_segs, nsegs entries,
+ * nsegs = ceil(nrows/seg_rows)) and the total word count; pack writes the words and a
+ * device copy of the descriptors.  force_bits > 0 forces that code width. */
+int aln_encode_plan(aln_ctx *ctx, const int64_t *d_vals, uint64_t nrows, uint32_t seg_rows,
+                    uint32_t force_bits, aln_seg *h_segs, uint64_t *h_words);
+int aln_encode_pack(aln_ctx *ctx, const int64_t *d_vals, uint64_t nrows, uint32_t seg_rows,
+                    const aln_seg *h_segs, uint32_t *d_words, aln_seg *d_segs);
+/* Describe a plain device int64 array as a column (fills d_segs, nsegs entries). */
+int aln_plain_col(aln_ctx *ctx, const int64_t *d_vals, uint64_t nrows, uint32_t seg_rows,
+                  int32_t decimals, aln_seg *d_segs, aln_col *out);
+/* Decode rows [row_begin, row_end) to int64. */
+int aln_decode(aln_ctx *ctx, const aln_col *col, uint32_t seg_rows, uint64_t row_begin,
+               uint64_t row_end, int64_t *d_out);
+
+/* ---- scan: fused decode + predicate + stream compaction (filter_op, cm.cu:3609-3666) --
+ * cols: host array of ncols column descriptors sharing one segmentation (seg_rows rows per
+ * segment).  seg_modes: host array of nsegs zone-map results ('A','N','R'; NULL = all 'R').
+ * Writes the ascending global row ids of passing rows. */
+int aln_scan_filter(aln_ctx *ctx, const aln_col *cols, int ncols, uint32_t seg_rows,
+                    const aln_insn *prog, int nprog, const char *seg_modes, uint32_t *d_row_ids,
+                    uint64_t cap, uint64_t *h_count);
+
+/* Gather decoded values of rows d_rows[0..n) (d_rows NULL = rows 0..n-1). */
+int aln_gather(aln_ctx *ctx, const aln_col *col, uint32_t seg_rows, const uint32_t *d_rows,
+               uint64_t n, int64_t *d_out);
+/* Evaluate an expression program over rows d_rows[0..n) (NULL = 0..n-1). */
+int aln_eval(aln_ctx *ctx, const aln_col *cols, int ncols, uint32_t seg_rows,
+             const aln_insn *prog, int nprog, const uint32_t *d_rows, uint64_t n,
+             int64_t *d_out);
+/* d_out[i] = d_in[d_perm[i]] (apply_permutation, sorts.cu:86-125). */
+int aln_apply_perm(aln_ctx *ctx, const int64_t *d_in, const uint32_t *d_perm, uint64_t n,
+                   int64_t *d_out);
+
+/* ---- hash join (replaces the sort-merge join of operators.cu:1117-1284) -------------
+ * Inner equi-join on int64 keys.  Pairs are written in probe-row order and, within one
+ * probe row, in ascending build-row order. */
+int aln_hash_join(aln_ctx *ctx, const int64_t *d_build_keys, uint64_t nbuild,
+                  const int64_t *d_probe_keys, uint64_t nprobe, uint32_t *d_probe_idx,
+                  uint32_t *d_build_idx, uint64_t cap, uint64_t *h_npairs);
+
+/* ---- group-by SUM/COUNT/MIN/MAX ------------------------------------------------------ */
+enum { ALN_AGG_COUNT = 0, ALN_AGG_SUM = 2, ALN_AGG_MIN = 4, ALN_AGG_MAX = 5 };
+enum {
+  ALN_ORDER_KEYS = 0, /* ascending key tuple: one-segment reference output (cm.cu:910-956) */
+  ALN_ORDER_HASH = 1, /* ascending MurmurHash64A(keys, 8*nkeys, seed): multi-segment merge
+                         order (merge.cu:138-203)                                         */
+  ALN_ORDER_NONE = 2
+};
+#define ALN_MAX_KEYS 4
+#define ALN_MAX_AGGS 8
+/* d_keys/d_vals/d_out_*: host arrays of device pointers.  COUNT ignores its value column. */
+int aln_groupby(aln_ctx *ctx, const int64_t *const *d_keys, int nkeys,
+                const int64_t *const *d_vals, const int32_t *aggs, int nvals, uint64_t n,
+                int order, uint32_t seed, int64_t *const *d_out_keys,
+                int64_t *const *d_out_vals, uint64_t cap, uint64_t *h_ngroups);
+
+/* ---- ORDER BY: stable multi-key sort (emit_order, operators.cu:1641-1798) ------------
+ * Writes the stable permutation sorting rows by keys[0] (most significant) .. keys[n-1],
+ * desc[i] != 0 for DESC.  Ties keep input order. */
+int aln_sort_keys(aln_ctx *ctx, const int64_t *const *d_keys, const int32_t *desc, int nkeys,
+                  uint64_t n, uint32_t *d_perm);
+/* First k rows of that stable order (ORDER BY ... LIMIT k). */
+int aln_topk(aln_ctx *ctx, const int64_t *const *d_keys, const int32_t *desc, int nkeys,
+             uint64_t n, uint32_t k, uint32_t *d_out_idx, uint32_t *h_nout);
+/* Stable radix sort of (u64 key, u32 value) pairs by the low `bits` bits of key. */
+int aln_radix_sort_pairs(aln_ctx *ctx, uint64_t *d_keys, uint32_t *d_vals, uint64_t n,
+                         uint32_t bits);
+
+/* ---- hashing -------------------------------------------------------------------------- */
+uint64_t aln_murmur64a(const void *key, int len, uint32_t seed);
+/* d_out[i] = MurmurHash64A of the row's nkeys int64 values laid out contiguously. */
+int aln_hash_rows(aln_ctx *ctx, const int64_t *const *d_keys, int nkeys, uint64_t n,
+                  uint32_t seed, uint64_t *d_out);
+
+/* ---- fused star pipeline: filter -> join -> join -> group-by -> top-k ----------------
+ * The TPC-H q3 shape of the reference's readme (readme.md:48-80):
+ *   dim  D: rows passing d_ranges; key d_key
+ *   mid  M: rows passing m_ranges whose m_fk is a passing D key; unique key m_key;
+ *           group columns m_g1, m_g2 (functionally dependent on m_key)
+ *   fact F: rows passing f_ranges whose f_fk is a passing M key; aggregate per M row
+ *           SUM((a0 + a1*f_x) * (b0 + b1*f_y)) and COUNT(*)
+ * then ORDER BY sum DESC, g1 ASC, m_key ASC LIMIT k.  A range is lo <= value <= hi on the
+ * decoded value of column slot `col` of its table; ranges of one table are ANDed. */
+typedef struct {
+  int32_t col;
+  int32_t pad;
+  int64_t lo;
+  int64_t hi;
+} aln_range;
+#define ALN_MAX_RANGES 8
+typedef struct {
+  const aln_col *cols; /* host array */
+  int32_t ncols;
+  uint32_t seg_rows;
+  const char *seg_modes; /* host, may be NULL */
+  const aln_range *ranges;
+  int32_t nranges;
+} aln_table_scan;
+typedef struct {
+  aln_table_scan dim, mid, fact;
+  int32_t d_key;
+  int32_t m_key, m_fk, m_g1, m_g2;
+  int32_t f_fk, f_x, f_y;
+  int64_t a0, a1, b0, b1;
+  uint32_t limit;
+  uint32_t flags; /* bit 0: also return every group (d_groups_*) */
+} aln_star_args;
+typedef struct {
+  int64_t key, g1, g2, sum, count;
+} aln_star_row;
+/* Row counts of each stage; the ms_* phase times are filled when profiling is enabled. */
+typedef struct {
+  uint64_t dim_pass, mid_pass, fact_pass, groups;
+  double ms_dim, ms_mid, ms_fact, ms_topk;
+} aln_star_stats;
+/* Groups (if flags & 1) are written to d_groups (cap rows) in ascending key order. */
+int aln_star_agg(aln_ctx *ctx, const aln_star_args *args, aln_star_row *h_top, uint32_t *h_ntop,
+                 aln_star_row *d_groups, uint64_t groups_cap, aln_star_stats *h_stats);
+
+/* ---- synthetic workloads (no reference counterpart: the bench's seeded inputs) --------
+ * Counter-based SplitMix64 streams; see datagen.hip for the exact specification.  All
+ * output pointers are device int64 arrays and may be NULL to skip a column. */
+int aln_gen_flat(aln_ctx *ctx, uint64_t seed, uint64_t n, int64_t *d_key, int64_t *d_date,
+                 int64_t *d_price, int64_t *d_disc);
+int aln_gen_dim(aln_ctx *ctx, uint64_t seed, uint64_t n, int64_t *d_key, int64_t *d_seg_idx);
+int aln_gen_orders(aln_ctx *ctx, uint64_t seed, uint64_t n, uint64_t ndim, int64_t *d_key,
+                   int64_t *d_cust, int64_t *d_date, int64_t *d_prio);
+int aln_gen_fact_rows(aln_ctx *ctx, uint64_t seed, uint64_t norders, uint64_t *h_nfact);
+int aln_gen_fact(aln_ctx *ctx, uint64_t seed, uint64_t norders, int64_t *d_okey,
+                 int64_t *d_price, int64_t *d_disc, int64_t *d_ship);
+/* Dictionary codes in first-seen order (compress_char, cm.cu:1808-1821) for values that
+ * index a string table [0, nvals): h_code_of_val[v] = code or -1; d_codes (optional) gets
+ * the code of every row. */
+int aln_dict_first_seen(aln_ctx *ctx, const int64_t *d_vals, uint64_t n, int64_t nvals,
+                        int64_t *h_code_of_val, int64_t *d_codes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ALENKA_HIP_H */

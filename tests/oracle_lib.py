@@ -1,0 +1,212 @@
+"""numpy wrappers of the CPU oracle (oracle/liboracle.so): the checker of every parity test."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from alenka_amd import _lib as L
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE = os.path.join(ROOT, "oracle", "liboracle.so")
+REF_MURMUR = os.path.join(ROOT, "oracle", "_ref", "libmurmur_ref.so")
+
+_o = None
+
+
+def load():
+    global _o
+    if _o is None:
+        src = os.path.join(ROOT, "oracle", "oracle.c")
+        if not os.path.exists(ORACLE) or os.path.getmtime(ORACLE) < os.path.getmtime(src):
+            subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
+        _o = C.CDLL(ORACLE)
+        _o.orc_murmur64a.restype = C.c_uint64
+        _o.orc_murmur64a.argtypes = [C.c_char_p, C.c_int, C.c_uint32]
+        _o.orc_day_to_ymd.restype = C.c_int32
+        _o.orc_gen_fact_rows.restype = C.c_uint64
+        _o.orc_bits_for_range.restype = C.c_uint32
+        _o.orc_unpack.restype = C.c_int64
+        _o.orc_pfor_encode.restype = C.c_uint64
+        _o.orc_pfor_delta_encode.restype = C.c_uint64
+        _o.orc_pfor_decode.restype = C.c_int64
+        _o.orc_compact.restype = C.c_uint64
+        _o.orc_groupby_sorted.restype = C.c_uint64
+        _o.orc_groupby_segmented.restype = C.c_uint64
+        _o.orc_join_sortmerge.restype = C.c_uint64
+    return _o
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _ptrs(arrs):
+    return (C.c_void_p * max(1, len(arrs)))(*[a.ctypes.data for a in arrs])
+
+
+def murmur64a(b: bytes, seed: int) -> int:
+    return load().orc_murmur64a(b, len(b), seed)
+
+
+def day_to_ymd(d):
+    return load().orc_day_to_ymd(C.c_int32(d))
+
+
+def gen_flat(seed, n):
+    o = load()
+    cols = [np.empty(n, np.int64) for _ in range(4)]
+    o.orc_gen_flat(C.c_uint64(seed), C.c_uint64(n), *[_ptr(c) for c in cols])
+    return cols
+
+
+def gen_dim(seed, n):
+    cols = [np.empty(n, np.int64) for _ in range(2)]
+    load().orc_gen_dim(C.c_uint64(seed), C.c_uint64(n), *[_ptr(c) for c in cols])
+    return cols
+
+
+def gen_orders(seed, n, ndim):
+    cols = [np.empty(n, np.int64) for _ in range(4)]
+    load().orc_gen_orders(C.c_uint64(seed), C.c_uint64(n), C.c_uint64(ndim), *[_ptr(c) for c in cols])
+    return cols
+
+
+def gen_fact(seed, norders):
+    o = load()
+    n = o.orc_gen_fact_rows(C.c_uint64(seed), C.c_uint64(norders))
+    cols = [np.empty(n, np.int64) for _ in range(4)]
+    o.orc_gen_fact(C.c_uint64(seed), C.c_uint64(norders), *[_ptr(c) for c in cols])
+    return cols
+
+
+def dict_first_seen(vals, nvals):
+    code_of = np.empty(nvals, np.int64)
+    codes = np.empty(len(vals), np.int64)
+    v = np.ascontiguousarray(vals, np.int64)
+    load().orc_dict_first_seen(_ptr(v), C.c_uint64(len(v)), C.c_int64(nvals), _ptr(code_of), _ptr(codes))
+    return code_of, codes
+
+
+def pack(vals, base, bits):
+    v = np.ascontiguousarray(vals, np.int64)
+    words = np.zeros(max(1, (len(v) * bits + 31) // 32), np.uint32)
+    load().orc_pack(_ptr(v), C.c_uint64(len(v)), C.c_int64(base), C.c_uint32(bits), _ptr(words))
+    return words
+
+
+def unpack(words, i, base, bits):
+    return load().orc_unpack(_ptr(words), C.c_uint64(i), C.c_int64(base), C.c_uint32(bits))
+
+
+def pfor_encode(vals, delta=False):
+    o = load()
+    v = np.ascontiguousarray(vals, np.int64)
+    fn = o.orc_pfor_delta_encode if delta else o.orc_pfor_encode
+    need = fn(_ptr(v), C.c_uint32(len(v)), None, C.c_uint64(0))
+    buf = np.zeros(need, np.uint8)
+    fn(_ptr(v), C.c_uint32(len(v)), _ptr(buf), C.c_uint64(need))
+    return buf.tobytes()
+
+
+def pfor_decode(buf: bytes, cap):
+    out = np.empty(max(1, cap), np.int64)
+    b = np.frombuffer(buf, np.uint8).copy()
+    n = load().orc_pfor_decode(_ptr(b), C.c_uint64(len(b)), _ptr(out), C.c_uint64(cap))
+    if n < 0:
+        raise ValueError("bad segment buffer")
+    return out[:n]
+
+
+def filter_mask(ops, cols, col_dec):
+    cols = [np.ascontiguousarray(c, np.int64) for c in cols]
+    n = len(cols[0]) if cols else 0
+    mask = np.zeros(max(1, n), np.uint8)
+    dec = (C.c_int32 * max(1, len(col_dec)))(*col_dec)
+    rc = load().orc_filter(L.rpn_array(ops), len(ops), _ptrs(cols), dec, C.c_uint64(n), _ptr(mask))
+    if rc != 0:
+        raise L.AlnError(rc, "orc_filter")
+    return mask[:n].astype(bool)
+
+
+def eval_expr(ops, cols, col_dec):
+    cols = [np.ascontiguousarray(c, np.int64) for c in cols]
+    n = len(cols[0])
+    out = np.empty(max(1, n), np.int64)
+    prec = C.c_int32()
+    dec = (C.c_int32 * max(1, len(col_dec)))(*col_dec)
+    rc = load().orc_eval(L.rpn_array(ops), len(ops), _ptrs(cols), dec, C.c_uint64(n), _ptr(out),
+                         C.byref(prec))
+    if rc != 0:
+        raise L.AlnError(rc, "orc_eval")
+    return out[:n], prec.value
+
+
+def zone_map_check(ops, mins, maxs, col_dec, not_compressed=False):
+    n = max(1, len(col_dec))
+    mn = (C.c_int64 * n)(*mins)
+    mx = (C.c_int64 * n)(*maxs)
+    dec = (C.c_int32 * n)(*col_dec)
+    out = C.create_string_buffer(1)
+    rc = load().orc_zone_map_check(L.rpn_array(ops), len(ops), mn, mx, dec, int(not_compressed), out)
+    if rc != 0:
+        raise L.AlnError(rc, "orc_zone_map_check")
+    return out.raw.decode()
+
+
+def compact(mask, row0=0):
+    m = np.ascontiguousarray(mask, np.uint8)
+    ids = np.empty(max(1, len(m)), np.uint32)
+    k = load().orc_compact(_ptr(m), C.c_uint64(len(m)), C.c_uint32(row0), _ptr(ids))
+    return ids[:k]
+
+
+def groupby(keys, vals, aggs, seg_bounds=None, seed=100):
+    keys = [np.ascontiguousarray(k, np.int64) for k in keys]
+    vals = [np.ascontiguousarray(v, np.int64) for v in vals]
+    n = len(keys[0])
+    ok = [np.empty(max(1, n), np.int64) for _ in keys]
+    ov = [np.empty(max(1, n), np.int64) for _ in vals]
+    ag = (C.c_int32 * max(1, len(aggs)))(*aggs)
+    o = load()
+    if seg_bounds is None or len(seg_bounds) <= 2:
+        g = o.orc_groupby_sorted(_ptrs(keys), len(keys), _ptrs(vals), ag, len(vals), C.c_uint64(n),
+                                 _ptrs(ok), _ptrs(ov))
+    else:
+        sb = np.ascontiguousarray(seg_bounds, np.uint64)
+        g = o.orc_groupby_segmented(_ptrs(keys), len(keys), _ptrs(vals), ag, len(vals), C.c_uint64(n),
+                                    _ptr(sb), len(sb) - 1, C.c_uint32(seed), _ptrs(ok), _ptrs(ov))
+    return [k[:g] for k in ok], [v[:g] for v in ov]
+
+
+def join_sortmerge(lkeys, rkeys, lseg_bounds=None):
+    lk = np.ascontiguousarray(lkeys, np.int64)
+    rk = np.ascontiguousarray(rkeys, np.int64)
+    o = load()
+    if lseg_bounds is None:
+        sb, ns = np.array([0, len(lk)], np.uint64), 1
+    else:
+        sb, ns = np.ascontiguousarray(lseg_bounds, np.uint64), len(lseg_bounds) - 1
+    need = o.orc_join_sortmerge(_ptr(lk), C.c_uint64(len(lk)), _ptr(rk), C.c_uint64(len(rk)), _ptr(sb),
+                                ns, None, None, C.c_uint64(0))
+    ol = np.empty(max(1, need), np.uint32)
+    orr = np.empty(max(1, need), np.uint32)
+    o.orc_join_sortmerge(_ptr(lk), C.c_uint64(len(lk)), _ptr(rk), C.c_uint64(len(rk)), _ptr(sb), ns,
+                         _ptr(ol), _ptr(orr), C.c_uint64(need))
+    return ol[:need], orr[:need]
+
+
+def order(keys, desc):
+    keys = [np.ascontiguousarray(k, np.int64) for k in keys]
+    n = len(keys[0])
+    perm = np.empty(max(1, n), np.uint32)
+    d = (C.c_int32 * len(keys))(*desc)
+    load().orc_order(_ptrs(keys), d, len(keys), C.c_uint64(n), _ptr(perm))
+    return perm[:n]
+
+
+def kahan_sum(v):
+    a = np.ascontiguousarray(v, np.float64)
+    out = C.c_double()
+    load().orc_kahan_sum_f64(_ptr(a), C.c_uint64(len(a)), C.byref(out))
+    return out.value

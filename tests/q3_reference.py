@@ -1,0 +1,60 @@
+"""The TPC-H q3 pipeline of the reference's readme (readme.md:48-80) evaluated with the
+CPU oracle's operators, step by step as the reference script runs them:
+
+  OFI := FILTER orders BY o_orderdate < 19950315;
+  CF  := FILTER customer BY c_mktsegment == "BUILDING";
+  LF  := FILTER lineitem BY shipdate > 19950315;
+  OLC := SELECT ... FROM LF JOIN OFI ON orderkey = o_orderkey JOIN CF ON o_custkey = c_custkey;
+  F   := SELECT o_orderkey, o_orderdate, o_shippriority, SUM(price*(1-discount)), COUNT(o_orderkey)
+         FROM OLC GROUP BY o_orderkey, o_orderdate, o_shippriority;
+  RES := ORDER F BY sum_revenue DESC, orderdate1 ASC;  ... LIMIT 10
+"""
+import numpy as np
+
+import oracle_lib as O
+from alenka_amd import _lib as L
+
+SEGMENTS = ["AUTOMOBILE", "BUILDING", "FURNITURE", "HOUSEHOLD", "MACHINERY"]
+DATE = 19950315
+
+
+def tables(seed, ndim, norders):
+    c_key, c_seg_idx = O.gen_dim(seed, ndim)
+    code_of, c_seg = O.dict_first_seen(c_seg_idx, len(SEGMENTS))
+    o_key, o_cust, o_date, o_prio = O.gen_orders(seed, norders, ndim)
+    l_okey, l_price, l_disc, l_ship = O.gen_fact(seed, norders)
+    return dict(c_key=c_key, c_seg=c_seg, code_of=code_of, o_key=o_key, o_cust=o_cust, o_date=o_date,
+                o_prio=o_prio, l_okey=l_okey, l_price=l_price, l_disc=l_disc, l_ship=l_ship)
+
+
+def run(t, limit=10):
+    building = int(t["code_of"][SEGMENTS.index("BUILDING")])
+    cf = O.filter_mask([(L.RPN_NAME, 0, 0), (L.RPN_STRING, 0, building), (L.RPN_CMP, 4, 0)],
+                       [t["c_seg"]], [0])
+    ofi = O.filter_mask([(L.RPN_NAME, 0, 0), (L.RPN_NUMBER, 0, DATE), (L.RPN_CMP, 1, 0)],
+                        [t["o_date"]], [0])
+    lf = O.filter_mask([(L.RPN_NAME, 0, 0), (L.RPN_NUMBER, 0, DATE), (L.RPN_CMP, 2, 0)],
+                       [t["l_ship"]], [0])
+    c_ids = O.compact(cf)
+    o_ids = O.compact(ofi)
+    l_ids = O.compact(lf)
+    # OLC1 := OFI JOIN CF ON o_custkey = c_custkey (left = orders, right = customers)
+    pl, pr = O.join_sortmerge(t["o_cust"][o_ids], t["c_key"][c_ids])
+    olc1 = o_ids[pl]
+    # OLC := LF JOIN OLC1 ON orderkey = o_orderkey
+    ql, qr = O.join_sortmerge(t["l_okey"][l_ids], t["o_key"][olc1])
+    lrow = l_ids[ql]
+    orow = olc1[qr]
+    rev, prec = O.eval_expr([(L.RPN_NAME, 0, 0), (L.RPN_NUMBER, 0, 1), (L.RPN_NAME, 1, 0),
+                             (L.RPN_MINUS, 0, 0), (L.RPN_MUL, 0, 0)],
+                            [t["l_price"][lrow], t["l_disc"][lrow]], [2, 2])
+    assert prec == 4
+    gk, gv = O.groupby([t["o_key"][orow], t["o_date"][orow], t["o_prio"][orow]],
+                       [rev, rev], [L.AGG_SUM, L.AGG_COUNT])
+    perm = O.order([gv[0], gk[1]], [1, 0])
+    top = [(int(gk[0][i]), int(gk[1][i]), int(gk[2][i]), int(gv[0][i]), int(gv[1][i]))
+           for i in perm[:limit]]
+    groups = np.stack([gk[0], gk[1], gk[2], gv[0], gv[1]], axis=1) if len(gk[0]) else np.zeros((0, 5), np.int64)
+    stats = dict(dim_pass=int(cf.sum()), mid_pass=int(len(olc1)), fact_pass=int(len(lrow)),
+                 groups=int(len(gk[0])))
+    return top, groups, stats

@@ -1,0 +1,177 @@
+"""CPU tests of the C ABI: the library loads, exports every entry point the header
+declares, and its host-side entry points (RPN compiler, zone maps, hashing) agree with the
+oracle.  Compiled programs are run by a small Python interpreter of the instruction set."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from alenka_amd import _lib as L
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    text = open(os.path.join(ROOT, "include", "alenka_hip.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(aln_\w+)\s*\(", text, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = L.load()
+    names = header_functions()
+    assert len(names) >= 30
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(names) == set(L.SIGNATURES), set(names) ^ set(L.SIGNATURES)
+    assert b"gfx950" in lib.aln_version()
+
+
+def test_no_oracle_in_product():
+    """The product library neither links nor names the oracle."""
+    data = open(L.LIB_PATH, "rb").read()
+    assert b"liboracle" not in data and b"orc_" not in data
+
+
+def run_prog(prog, cols):
+    """Reference interpreter of aln_insn programs (int64 wrap-around semantics)."""
+    M = (1 << 64) - 1
+
+    def wrap(x):
+        x &= M
+        return x - (1 << 64) if x >> 63 else x
+
+    n = len(cols[0])
+    out = np.empty(n, np.int64)
+    for i in range(n):
+        st = []
+        for op, a, k in prog:
+            if op == L.I_COL:
+                st.append(wrap(int(cols[a][i]) * k))
+            elif op == L.I_CONST:
+                st.append(k)
+            else:
+                r, l = st.pop(), st.pop()
+                if op == L.I_ADD:
+                    v = wrap(l + r)
+                elif op == L.I_SUB:
+                    v = wrap(l - r)
+                elif op == L.I_MUL:
+                    v = wrap(l * r)
+                elif op == L.I_DIV:
+                    if r == 0:
+                        v = 0
+                    elif l == -(1 << 63) and r == -1:
+                        v = l
+                    else:  # C truncating division
+                        v = (abs(l) // abs(r)) * (1 if (l < 0) == (r < 0) else -1)
+                elif op == L.I_CMP:
+                    v = int({1: l < r, 2: l > r, 3: l != r, 4: l == r, 5: l <= r, 6: l >= r}[a])
+                elif op == L.I_AND:
+                    v = int(bool(l) and bool(r))
+                else:
+                    v = int(bool(l) or bool(r))
+                st.append(v)
+        out[i] = st[-1]
+    return out
+
+
+NAME, NUM, STR, CMP, AND, OR = L.RPN_NAME, L.RPN_NUMBER, L.RPN_STRING, L.RPN_CMP, L.RPN_AND, L.RPN_OR
+ADD, MINUS, MUL, DIV = L.RPN_ADD, L.RPN_MINUS, L.RPN_MUL, L.RPN_DIV
+
+
+def random_rpn(r, ncols, depth=0):
+    """A random well-formed predicate RPN over columns and literals."""
+    def operand(d):
+        c = r.random()
+        if c < 0.45 or d > 1:
+            return [(NAME, int(r.integers(0, ncols)), 0)]
+        if c < 0.8:
+            return [(NUM, int(r.integers(0, 3)), int(r.integers(-200, 200)))]
+        return operand(d + 1) + operand(d + 1) + [(int(r.choice([ADD, MINUS, MUL, DIV])), 0, 0)]
+
+    def pred(d):
+        if d < 2 and r.random() < 0.4:
+            return pred(d + 1) + pred(d + 1) + [(int(r.choice([AND, OR])), 0, 0)]
+        return operand(0) + operand(0) + [(CMP, int(r.integers(1, 7)), 0)]
+
+    return pred(depth)
+
+
+def test_compiled_predicates_match_oracle():
+    r = np.random.default_rng(11)
+    cols = [r.integers(-300, 300, 64) for _ in range(3)]
+    dec = [0, 2, 1]
+    checked = 0
+    for _ in range(400):
+        ops = random_rpn(r, 3)
+        try:
+            want = O.filter_mask(ops, cols, dec)
+        except L.AlnError as e:
+            with pytest.raises(L.AlnError):
+                L.compile_rpn(ops, dec)
+            continue
+        prog, _ = L.compile_rpn(ops, dec)
+        got = run_prog(prog, cols).astype(bool)
+        assert np.array_equal(got, want), ops
+        checked += 1
+    assert checked > 200
+
+
+def test_compiled_expressions_match_oracle():
+    r = np.random.default_rng(12)
+    cols = [r.integers(-1000, 1000, 64) for _ in range(3)]
+    dec = [2, 0, 2]
+    for _ in range(300):
+        a = [(NAME, int(r.integers(0, 3)), 0)]
+        b = [(NUM, int(r.integers(0, 3)), int(r.integers(-50, 50)))]
+        ops = (a + b if r.random() < 0.5 else b + a) + [(int(r.choice([ADD, MINUS, MUL, DIV])), 0, 0)]
+        if r.random() < 0.5:
+            ops = ops + [(NAME, int(r.integers(0, 3)), 0), (int(r.choice([ADD, MINUS, MUL])), 0, 0)]
+        want, wp = O.eval_expr(ops, cols, dec)
+        prog, p = L.compile_rpn(ops, dec, mode=1)
+        assert p == wp
+        assert np.array_equal(run_prog(prog, cols), want), ops
+
+
+def test_zone_map_matches_oracle_random():
+    r = np.random.default_rng(13)
+    for _ in range(500):
+        ops = random_rpn(r, 2)
+        mins = [int(r.integers(-300, 300)) for _ in range(2)]
+        maxs = [m + int(r.integers(0, 200)) for m in mins]
+        dec = [int(r.integers(0, 3)) for _ in range(2)]
+        try:
+            want = O.zone_map_check(ops, mins, maxs, dec)
+        except L.AlnError:
+            with pytest.raises(L.AlnError):
+                L.zone_map_check(ops, mins, maxs, dec)
+            continue
+        assert L.zone_map_check(ops, mins, maxs, dec) == want, ops
+
+
+def test_fast_interval_shape_detection_is_a_conjunction():
+    """The scan's interval fast path applies only to AND-chains of column-vs-constant terms."""
+    prog, _ = L.compile_rpn([(NAME, 0, 0), (NUM, 0, 5), (CMP, 1, 0), (NAME, 1, 0), (NUM, 0, 2),
+                             (CMP, 6, 0), (AND, 0, 0)], [0, 0])
+    assert prog == [(L.I_COL, 0, 1), (L.I_CONST, 0, 5), (L.I_CMP, 1, 0), (L.I_COL, 1, 1),
+                    (L.I_CONST, 0, 2), (L.I_CMP, 6, 0), (L.I_AND, 0, 0)]
+
+
+def test_errors_are_status_codes():
+    with pytest.raises(L.AlnError) as e:
+        L.compile_rpn([(NAME, 0, 0), (CMP, 1, 0)], [0])
+    assert e.value.code == L.ALN_E_ARG
+    with pytest.raises(L.AlnError) as e:
+        L.compile_rpn([(NAME, 0, 0), (STR, 0, 1), (CMP, 1, 0)], [0])  # '<' on a string
+    assert e.value.code == L.ALN_E_UNSUPPORTED
+    with pytest.raises(L.AlnError):
+        L.compile_rpn([(NAME, 5, 0), (NUM, 0, 1), (CMP, 1, 0)], [0])  # unknown column slot
+
+
+def test_host_murmur_matches_oracle():
+    r = np.random.default_rng(14)
+    for n in range(0, 40):
+        b = r.integers(0, 256, n, dtype=np.uint8).tobytes()
+        assert L.murmur64a(b, 100) == O.murmur64a(b, 100)

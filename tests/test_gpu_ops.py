@@ -1,0 +1,315 @@
+"""GPU parity tests: every operator through the C ABI against the CPU oracle, bit-exact."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import q3_reference as Q3
+from alenka_amd import _lib as L
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+
+def dev(ctx, a):
+    return torch.as_tensor(np.ascontiguousarray(a, np.int64)).to(ctx.device)
+
+
+def host(t):
+    return t.cpu().numpy()
+
+
+# ---- generators --------------------------------------------------------------------------
+def test_generators_match_oracle(ctx):
+    from alenka_amd import device as D
+
+    for got, want in zip(D.gen_flat(ctx, 7, 5000), O.gen_flat(7, 5000)):
+        assert np.array_equal(host(got), want)
+    for got, want in zip(D.gen_dim(ctx, 7, 3000), O.gen_dim(7, 3000)):
+        assert np.array_equal(host(got), want)
+    for got, want in zip(D.gen_orders(ctx, 7, 4001, 3000), O.gen_orders(7, 4001, 3000)):
+        assert np.array_equal(host(got), want)
+    for got, want in zip(D.gen_fact(ctx, 7, 4001), O.gen_fact(7, 4001)):
+        assert np.array_equal(host(got), want)
+    code_of, codes = D.dict_first_seen(ctx, D.gen_dim(ctx, 7, 3000)[1], 5)
+    w_code_of, w_codes = O.dict_first_seen(O.gen_dim(7, 3000)[1], 5)
+    assert code_of == list(w_code_of)
+    assert np.array_equal(host(codes), w_codes)
+
+
+# ---- codec -------------------------------------------------------------------------------
+CODEC_CASES = [
+    ("small", lambda r, n: r.integers(0, 11, n)),
+    ("dates", lambda r, n: r.integers(19920101, 19981231, n)),
+    ("negative", lambda r, n: r.integers(-(1 << 40), 1 << 20, n)),
+    ("constant", lambda r, n: np.full(n, -5)),
+    ("full64", lambda r, n: r.integers(-(1 << 63), (1 << 63) - 1, n, dtype=np.int64)),
+    ("bits33", lambda r, n: r.integers(0, 1 << 33, n)),
+]
+
+
+@pytest.mark.parametrize("name,gen", CODEC_CASES)
+@pytest.mark.parametrize("n,seg_rows", [(1, 1000), (10007, 1000), (70000, 65536)])
+def test_encode_decode(ctx, name, gen, n, seg_rows):
+    from alenka_amd import device as D
+
+    r = np.random.default_rng(n + seg_rows)
+    v = np.asarray(gen(r, n), np.int64)
+    col = D.Column.encode(ctx, dev(ctx, v), seg_rows)
+    assert np.array_equal(host(col.decode(ctx)), v)
+    words = host(col.words).view(np.uint32)
+    for s in range(col.nsegs):
+        sg = col.hsegs[s]
+        part = v[s * seg_rows: s * seg_rows + sg.nrows]
+        assert sg.vmin == part.min() and sg.vmax == part.max()
+        assert sg.bits == O.load().orc_bits_for_range(int(part.min()), int(part.max()))
+        if sg.bits:
+            nw = (sg.nrows * sg.bits + 31) // 32
+            want = O.pack(part, sg.base, sg.bits)[:nw]
+            assert np.array_equal(words[sg.word_off: sg.word_off + nw], want)
+    rows = np.sort(r.choice(n, size=min(n, 333), replace=False)).astype(np.int32)
+    got = D.gather(ctx, col, torch.as_tensor(rows).to(ctx.device))
+    assert np.array_equal(host(got), v[rows])
+
+
+# ---- scan / filter -----------------------------------------------------------------------
+NAME, NUM, CMP, AND, OR = L.RPN_NAME, L.RPN_NUMBER, L.RPN_CMP, L.RPN_AND, L.RPN_OR
+PREDICATES = [
+    ("date_lt", [(NAME, 1, 0), (NUM, 0, 19950315), (CMP, 1, 0)]),
+    ("range_and", [(NAME, 1, 0), (NUM, 0, 19940101), (CMP, 6, 0), (NAME, 1, 0), (NUM, 0, 19950101),
+                   (CMP, 1, 0), (AND, 0, 0), (NAME, 3, 0), (NUM, 0, 5), (CMP, 5, 0), (AND, 0, 0)]),
+    ("or", [(NAME, 0, 0), (NUM, 0, 1000), (CMP, 1, 0), (NAME, 3, 0), (NUM, 0, 10), (CMP, 4, 0), (OR, 0, 0)]),
+    ("neq", [(NAME, 3, 0), (NUM, 0, 0), (CMP, 3, 0)]),
+    ("decimal_const", [(NAME, 2, 0), (L.RPN_NUMBER, 2, 50000), (CMP, 2, 0)]),  # price > 500.00
+    ("reverse", [(NUM, 0, 19950315), (NAME, 1, 0), (CMP, 2, 0)]),  # 19950315 > date
+    ("arith", [(NAME, 2, 0), (NUM, 0, 1), (NAME, 3, 0), (L.RPN_MINUS, 0, 0), (L.RPN_MUL, 0, 0),
+               (NUM, 0, 500000000), (CMP, 2, 0)]),
+    ("col_col", [(NAME, 3, 0), (NAME, 0, 0), (CMP, 1, 0)]),  # disc*100 < key (scales as cm.cu:2299)
+    ("const_const", [(NUM, 0, 3), (NUM, 0, 4), (CMP, 1, 0)]),
+]
+
+
+@pytest.mark.parametrize("name,ops", PREDICATES)
+@pytest.mark.parametrize("zone_maps", [False, True])
+def test_scan_filter(ctx, name, ops, zone_maps):
+    from alenka_amd import device as D
+
+    n, seg_rows = 50000, 8192
+    cols = O.gen_flat(11, n)
+    # sort a copy of the date column by segment so zone maps classify segments A / N / R
+    cols[1] = np.sort(cols[1])
+    dec = [0, 0, 2, 2]
+    dcols = [D.Column.encode(ctx, dev(ctx, c), seg_rows, decimals=d) for c, d in zip(cols, dec)]
+    prog, _ = L.compile_rpn(ops, dec)
+    modes = None
+    if zone_maps:
+        modes = []
+        for s in range(dcols[0].nsegs):
+            mins = [c.hsegs[s].vmin for c in dcols]
+            maxs = [c.hsegs[s].vmax for c in dcols]
+            m = L.zone_map_check(ops, mins, maxs, dec)
+            assert m == O.zone_map_check(ops, mins, maxs, dec)
+            modes.append(m)
+    got = host(D.scan_filter(ctx, dcols, prog, modes)).astype(np.uint32)
+    if modes is None:
+        want = O.compact(O.filter_mask(ops, cols, dec))
+    else:
+        # the reference's filter_op: 'A' keeps the whole segment, 'N' none, 'R' evaluates
+        parts = []
+        for s, m in enumerate(modes):
+            lo, hi = s * seg_rows, min(n, (s + 1) * seg_rows)
+            if m == "A":
+                parts.append(np.arange(lo, hi, dtype=np.uint32))
+            elif m == "R":
+                seg = [c[lo:hi] for c in cols]
+                parts.append(O.compact(O.filter_mask(ops, seg, dec), lo))
+        want = np.concatenate(parts) if parts else np.zeros(0, np.uint32)
+    assert np.array_equal(got, want)
+
+
+def test_eval_revenue(ctx):
+    from alenka_amd import device as D
+
+    n = 40000
+    cols = O.gen_flat(3, n)
+    dec = [0, 0, 2, 2]
+    dcols = [D.Column.encode(ctx, dev(ctx, c), 4096, decimals=d) for c, d in zip(cols, dec)]
+    ops = [(NAME, 2, 0), (NUM, 0, 1), (NAME, 3, 0), (L.RPN_MINUS, 0, 0), (L.RPN_MUL, 0, 0)]
+    prog, prec = L.compile_rpn(ops, dec, mode=1)
+    want, wprec = O.eval_expr(ops, cols, dec)
+    assert prec == wprec == 4
+    rows = np.arange(0, n, 3, dtype=np.int32)
+    got = host(D.eval_rows(ctx, dcols, prog, torch.as_tensor(rows).to(ctx.device)))
+    assert np.array_equal(got, want[rows])
+    assert np.array_equal(want, cols[2] * (100 - cols[3]))  # closed form of the trace
+
+
+# ---- group-by ----------------------------------------------------------------------------
+def test_groupby_flat_query(ctx):
+    """Flat 1M-row filter + GROUP BY key: SUM(price*(1-discount)), COUNT(key), one segment."""
+    from alenka_amd import device as D
+
+    n = 200000
+    key, date, price, disc = O.gen_flat(42, n)
+    sel = date < 19950315
+    rev = price[sel] * (100 - disc[sel])
+    k = key[sel]
+    ok, ov = D.groupby(ctx, [dev(ctx, k)], [dev(ctx, rev), dev(ctx, rev)], [L.AGG_SUM, L.AGG_COUNT])
+    wk, wv = O.groupby([k], [rev, rev], [L.AGG_SUM, L.AGG_COUNT])
+    assert np.array_equal(host(ok[0]), wk[0])
+    assert np.array_equal(host(ov[0]), wv[0])
+    assert np.array_equal(host(ov[1]), wv[1])
+    # FP64 check of the exact decimal sums: within 1e-12 of a Kahan-summed float oracle
+    g = {int(a): i for i, a in enumerate(wk[0][:50])}
+    for kk, i in g.items():
+        m = k == kk
+        f = O.kahan_sum((price[sel][m] / 100.0) * (1.0 - disc[sel][m] / 100.0))
+        assert abs(wv[0][i] / 1e4 - f) <= 1e-12 * abs(f)
+
+
+@pytest.mark.parametrize("nkeys", [1, 2, 3])
+def test_groupby_hash_order(ctx, nkeys):
+    """Multi-segment output order: ascending MurmurHash64A of the key tuple (merge.cu)."""
+    from alenka_amd import device as D
+
+    r = np.random.default_rng(nkeys)
+    n = 30000
+    keys = [r.integers(0, 300, n), r.integers(-50, 50, n), r.integers(19920101, 19920131, n)][:nkeys]
+    v = r.integers(-1000, 1000, n)
+    bounds = [0, 10000, 20000, n]
+    aggs = [L.AGG_SUM, L.AGG_COUNT, L.AGG_MIN, L.AGG_MAX]
+    ok, ov = D.groupby(ctx, [dev(ctx, x) for x in keys], [dev(ctx, v)] * 4, aggs, order=L.ORDER_HASH,
+                       seed=100)
+    wk, wv = O.groupby(keys, [v] * 4, aggs, seg_bounds=bounds, seed=100)
+    for a, b in zip(ok, wk):
+        assert np.array_equal(host(a), b)
+    for a, b in zip(ov, wv):
+        assert np.array_equal(host(a), b)
+
+
+def test_groupby_wide_keys(ctx):
+    """Key tuples wider than 63 bits take the sort-based path."""
+    from alenka_amd import device as D
+
+    r = np.random.default_rng(5)
+    n = 20000
+    k1 = r.integers(-(1 << 62), 1 << 62, 200)[r.integers(0, 200, n)]
+    k2 = r.integers(0, 1 << 20, 50)[r.integers(0, 50, n)]
+    v = r.integers(0, 1 << 30, n)
+    for order in (L.ORDER_KEYS, L.ORDER_HASH):
+        ok, ov = D.groupby(ctx, [dev(ctx, k1), dev(ctx, k2)], [dev(ctx, v), dev(ctx, v)],
+                           [L.AGG_SUM, L.AGG_COUNT], order=order)
+        wk, wv = O.groupby([k1, k2], [v, v], [L.AGG_SUM, L.AGG_COUNT],
+                           seg_bounds=[0, 1, n] if order == L.ORDER_HASH else None)
+        for a, b in zip(ok + ov, wk + wv):
+            assert np.array_equal(host(a), b)
+
+
+# ---- join --------------------------------------------------------------------------------
+@pytest.mark.parametrize("dups", [False, True])
+def test_hash_join(ctx, dups):
+    from alenka_amd import device as D
+
+    r = np.random.default_rng(9 + dups)
+    nb, npb = 20000, 50000
+    if dups:
+        build = r.integers(0, 5000, nb)
+    else:
+        build = r.permutation(1 << 20)[:nb].astype(np.int64) - 300000
+    probe = np.concatenate([build[r.integers(0, nb, npb // 2)], r.integers(-400000, 1 << 20, npb - npb // 2)])
+    probe = r.permutation(probe)
+    pi, bi = D.hash_join(ctx, dev(ctx, build), dev(ctx, probe))
+    pi, bi = host(pi).astype(np.uint32), host(bi).astype(np.uint32)
+    # expected: probe-row order, then ascending build row
+    exp = []
+    where = {}
+    for j, b in enumerate(build):
+        where.setdefault(int(b), []).append(j)
+    for i, p in enumerate(probe):
+        for j in where.get(int(p), []):
+            exp.append((i, j))
+    exp = np.array(exp, np.uint32).reshape(-1, 2)
+    assert np.array_equal(np.stack([pi, bi], 1), exp)
+    # same multiset as the reference's sort-merge join
+    ol, orr = O.join_sortmerge(probe, build)
+    assert sorted(zip(ol.tolist(), orr.tolist())) == sorted(zip(pi.tolist(), bi.tolist()))
+
+
+# ---- sort / top-k ------------------------------------------------------------------------
+@pytest.mark.parametrize("wide", [False, True])
+def test_sort_and_topk(ctx, wide):
+    from alenka_amd import device as D
+
+    r = np.random.default_rng(21 + wide)
+    n = 60000
+    if wide:
+        keys = [r.integers(-(1 << 62), 1 << 62, n), r.integers(0, 1 << 40, n)]
+    else:
+        keys = [r.integers(0, 1000, n), r.integers(19920101, 19981231, n)]
+    desc = [1, 0]
+    perm = host(D.sort_keys(ctx, [dev(ctx, k) for k in keys], desc)).astype(np.uint32)
+    want = O.order(keys, desc)
+    assert np.array_equal(perm, want)
+    for k in (1, 10, 100):
+        top = host(D.topk(ctx, [dev(ctx, x) for x in keys], desc, k)).astype(np.uint32)
+        assert np.array_equal(top, want[:k])
+
+
+def test_radix_sort_pairs(ctx):
+    from alenka_amd import device as D
+
+    r = np.random.default_rng(3)
+    n = 100003
+    k = r.integers(0, 1 << 63, n, dtype=np.int64).astype(np.uint64) | np.uint64(1 << 63)
+    kt = torch.as_tensor(k.view(np.int64)).to(ctx.device)
+    vt = torch.arange(n, dtype=torch.int32, device=ctx.device)
+    D.radix_sort_pairs(ctx, kt, vt, 64)
+    order = np.argsort(k, kind="stable")
+    assert np.array_equal(host(kt).view(np.uint64), k[order])
+    assert np.array_equal(host(vt), order.astype(np.int32))
+
+
+def test_hash_rows(ctx):
+    from alenka_amd import device as D
+
+    r = np.random.default_rng(4)
+    a, b = r.integers(-(1 << 62), 1 << 62, 1000), r.integers(0, 100, 1000)
+    h = host(D.hash_rows(ctx, [dev(ctx, a), dev(ctx, b)], 100)).view(np.uint64)
+    for i in range(0, 1000, 37):
+        tup = np.array([a[i], b[i]], np.int64).tobytes()
+        assert int(h[i]) == O.murmur64a(tup, 100)
+
+
+# ---- fused q3 pipeline -------------------------------------------------------------------
+def encode_q3(ctx, t, seg_rows):
+    from alenka_amd import device as D
+
+    enc = lambda a, d=0: D.Column.encode(ctx, dev(ctx, a), seg_rows, decimals=d)
+    dim = [enc(t["c_key"]), enc(t["c_seg"])]
+    mid = [enc(t["o_key"]), enc(t["o_cust"]), enc(t["o_date"]), enc(t["o_prio"])]
+    fact = [enc(t["l_okey"]), enc(t["l_price"], 2), enc(t["l_disc"], 2), enc(t["l_ship"])]
+    return dim, mid, fact
+
+
+def q3_args(t, dim, mid, fact):
+    building = int(t["code_of"][Q3.SEGMENTS.index("BUILDING")])
+    D_ = dict(cols=dim, ranges=[(1, building, building)])
+    M_ = dict(cols=mid, ranges=[(2, -(1 << 63), Q3.DATE - 1)])
+    F_ = dict(cols=fact, ranges=[(3, Q3.DATE + 1, (1 << 63) - 1)])
+    spec = dict(d_key=0, m_key=0, m_fk=1, m_g1=2, m_g2=3, f_fk=0, f_x=1, f_y=2, a0=0, a1=1, b0=100, b1=-1)
+    return D_, M_, F_, spec
+
+
+@pytest.mark.parametrize("scale,seg_rows", [(0.02, 4096), (0.1, 65536)])
+def test_star_q3(ctx, scale, seg_rows):
+    from alenka_amd import device as D
+
+    ndim, norders = int(150000 * scale), int(1500000 * scale)
+    t = Q3.tables(1234, ndim, norders)
+    top, groups, stats = Q3.run(t)
+    dim, mid, fact = encode_q3(ctx, t, seg_rows)
+    D_, M_, F_, spec = q3_args(t, dim, mid, fact)
+    rows, g, st = D.star_agg(ctx, D_, M_, F_, spec, limit=10, want_groups=True)
+    assert rows == top
+    assert st.dim_pass == stats["dim_pass"] and st.mid_pass == stats["mid_pass"]
+    assert st.fact_pass == stats["fact_pass"] and st.groups == stats["groups"]
+    assert np.array_equal(host(g), groups)
