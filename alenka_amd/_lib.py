@@ -72,8 +72,9 @@ class StarRow(C.Structure):
 
 
 class StarStats(C.Structure):
-    _fields_ = [("dim_pass", C.c_uint64), ("mid_pass", C.c_uint64), ("fact_pass", C.c_uint64),
-                ("groups", C.c_uint64), ("ms_dim", C.c_double), ("ms_mid", C.c_double),
+    _fields_ = [("dim_pass", C.c_uint64), ("mid_probe", C.c_uint64), ("mid_pass", C.c_uint64),
+                ("fact_probe", C.c_uint64),
+                ("fact_pass", C.c_uint64), ("groups", C.c_uint64), ("ms_dim", C.c_double), ("ms_mid", C.c_double),
                 ("ms_fact", C.c_double), ("ms_topk", C.c_double)]
 
 

@@ -68,34 +68,35 @@ __global__ void __launch_bounds__(256) k_pack(const int64_t *__restrict__ v, uin
 
 // ---- row-wise kernels ---------------------------------------------------------------------
 
+// Rows outside the column (a segment past nsegs, or past a short last segment) read as 0.
 __global__ void __launch_bounds__(256) k_decode(const uint32_t *__restrict__ words,
-                                                const aln_seg *__restrict__ segs,
+                                                const aln_seg *__restrict__ segs, uint32_t nsegs,
                                                 uint32_t seg_rows, uint64_t r0, uint64_t n,
                                                 int64_t *__restrict__ out) {
   for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * 256) {
     uint64_t r = r0 + i;
-    uint32_t seg = (uint32_t)(r / seg_rows);
-    out[i] = decode_row(words, segs[seg], r - (uint64_t)seg * seg_rows);
+    uint64_t seg = r / seg_rows, local = r - seg * seg_rows;
+    out[i] = (seg < nsegs && local < segs[seg].nrows) ? decode_row(words, segs[seg], local) : 0;
   }
 }
 
 __global__ void __launch_bounds__(256) k_gather(const uint32_t *__restrict__ words,
-                                                const aln_seg *__restrict__ segs,
+                                                const aln_seg *__restrict__ segs, uint32_t nsegs,
                                                 uint32_t seg_rows,
                                                 const uint32_t *__restrict__ rows, uint64_t n,
                                                 int64_t *__restrict__ out) {
   for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * 256) {
     uint32_t r = rows ? rows[i] : (uint32_t)i;
-    uint32_t seg = r / seg_rows;
-    out[i] = decode_row(words, segs[seg], r - seg * seg_rows);
+    uint32_t seg = r / seg_rows, local = r - seg * seg_rows;
+    out[i] = (seg < nsegs && local < segs[seg].nrows) ? decode_row(words, segs[seg], local) : 0;
   }
 }
 
 __global__ void __launch_bounds__(VM_BLOCK) k_eval(const ColTable *__restrict__ ct,
                                                    const aln_insn *__restrict__ prog, int nprog,
-                                                   uint32_t seg_rows,
+                                                   uint32_t nsegs, uint32_t seg_rows,
                                                    const uint32_t *__restrict__ rows, uint64_t n,
                                                    int64_t *__restrict__ out) {
   __shared__ int64_t stk[ALN_MAX_STACK * VM_BLOCK];
@@ -104,7 +105,10 @@ __global__ void __launch_bounds__(VM_BLOCK) k_eval(const ColTable *__restrict__ 
     if (i < n) {
       uint32_t r = rows ? rows[i] : (uint32_t)i;
       uint32_t seg = r / seg_rows, local = r - seg * seg_rows;
-      auto load = [&](int c) -> int64_t { return decode_row(ct->words[c], ct->segs[c][seg], local); };
+      bool inside = seg < nsegs && local < ct->segs[0][seg < nsegs ? seg : 0].nrows;
+      auto load = [&](int c) -> int64_t {
+        return inside ? decode_row(ct->words[c], ct->segs[c][seg], local) : 0;
+      };
       out[i] = vm_run(prog, nprog, load, stk + threadIdx.x);
     }
   }
@@ -411,7 +415,7 @@ int aln_decode(aln_ctx *c, const aln_col *col, uint32_t seg_rows, uint64_t row_b
   if (!n) return ALN_OK;
   ProfScope p(c, "decode");
   hipLaunchKernelGGL(k_decode, dim3(grid_for(n, 256, 65536)), dim3(256), 0, c->stream, col->words,
-                     col->segs, seg_rows, row_begin, n, d_out);
+                     col->segs, col->nsegs, seg_rows, row_begin, n, d_out);
   return check_launch("decode");
 }
 
@@ -420,8 +424,9 @@ int aln_gather(aln_ctx *c, const aln_col *col, uint32_t seg_rows, const uint32_t
   ALN_REQUIRE(c && col && seg_rows > 0, "aln_gather: bad arguments");
   if (!n) return ALN_OK;
   ProfScope p(c, "gather");
+  if (col->nsegs == 0) return fill_i64(c, d_out, n, 0);
   hipLaunchKernelGGL(k_gather, dim3(grid_for(n, 256, 65536)), dim3(256), 0, c->stream, col->words,
-                     col->segs, seg_rows, d_rows, n, d_out);
+                     col->segs, col->nsegs, seg_rows, d_rows, n, d_out);
   return check_launch("gather");
 }
 
@@ -437,8 +442,9 @@ int aln_eval(aln_ctx *c, const aln_col *cols, int ncols, uint32_t seg_rows,
   ALN_TRY(upload_cols(c, cols, ncols, &ct));
   ALN_TRY(upload_prog(c, prog, nprog, &dp));
   ProfScope p(c, "eval");
+  if (cols[0].nsegs == 0) return fill_i64(c, d_out, n, 0);
   hipLaunchKernelGGL(k_eval, dim3(grid_for(n, VM_BLOCK, 65536)), dim3(VM_BLOCK), 0, c->stream, ct,
-                     dp, nprog, seg_rows, d_rows, n, d_out);
+                     dp, nprog, cols[0].nsegs, seg_rows, d_rows, n, d_out);
   return check_launch("eval");
 }
 

@@ -100,7 +100,8 @@ __global__ void __launch_bounds__(ST_BLOCK) k_star_dim(ScanDesc d, int kcol, int
 __global__ void __launch_bounds__(ST_BLOCK) k_star_mid(ScanDesc d, int kcol, int fkcol, int g1col,
                                                        int g2col, const uint32_t *__restrict__ bitmap,
                                                        int64_t dkmin, uint64_t drange,
-                                                       unsigned long long *counter, uint64_t cap,
+                                                       unsigned long long *counter,
+                                                       unsigned long long *probed, uint64_t cap,
                                                        int64_t *__restrict__ okey,
                                                        int64_t *__restrict__ og1,
                                                        int64_t *__restrict__ og2) {
@@ -112,11 +113,12 @@ __global__ void __launch_bounds__(ST_BLOCK) k_star_mid(ScanDesc d, int kcol, int
   if (r0 >= nrows || mode == 'N') return;
   const unsigned lane = __lane_id(), wid = threadIdx.x / 64;
   bool pass[ST_ITEMS];
-  uint32_t mine = 0;
+  uint32_t mine = 0, nprobe = 0;
   for (int j = 0; j < ST_ITEMS; ++j) {
     uint32_t local = r0 + j * ST_BLOCK + threadIdx.x;
     bool p = local < nrows && sd_pass(d, mode, seg, local);
     if (p) {
+      ++nprobe;
       uint64_t b = (uint64_t)sd_decode(d, fkcol, seg, local) - (uint64_t)dkmin;
       p = b < drange && ((bitmap[b >> 5] >> (b & 31)) & 1);
     }
@@ -125,6 +127,8 @@ __global__ void __launch_bounds__(ST_BLOCK) k_star_mid(ScanDesc d, int kcol, int
   }
   uint32_t wsum = mine;
   for (int o = 32; o > 0; o >>= 1) wsum += __shfl_xor(wsum, o, 64);
+  for (int o = 32; o > 0; o >>= 1) nprobe += __shfl_xor(nprobe, o, 64);
+  if (lane == 0 && nprobe) atomicAdd(probed, (unsigned long long)nprobe);
   if (lane == 0) wc[wid] = wsum;
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -199,18 +203,20 @@ __global__ void __launch_bounds__(ST_BLOCK) k_star_fact(ScanDesc d, int fkcol, i
                                                         uint64_t cap, int64_t mkmin, uint64_t mrange,
                                                         unsigned long long *__restrict__ sum,
                                                         unsigned int *__restrict__ cnt,
-                                                        unsigned long long *matched) {
+                                                        unsigned long long *matched,
+                                                        unsigned long long *probed) {
   const uint32_t seg = blockIdx.x / d.tps, tile = blockIdx.x % d.tps;
   const uint32_t nrows = d.segs[0][seg].nrows, r0 = tile * ST_TILE;
   const char mode = sd_mode(d, seg);
   if (r0 >= nrows || mode == 'N') return;
   const unsigned lane = __lane_id();
-  uint32_t nmatch = 0;
+  uint32_t nmatch = 0, nprobe = 0;
   for (int j = 0; j < ST_ITEMS; ++j) {
     uint32_t local = r0 + j * ST_BLOCK + threadIdx.x;
     uint32_t g = 0xFFFFFFFFu;
     int64_t v = 0;
     if (local < nrows && sd_pass(d, mode, seg, local)) {
+      ++nprobe;
       uint64_t code = (uint64_t)sd_decode(d, fkcol, seg, local) - (uint64_t)mkmin;
       if (code < mrange) g = st_find(table, cap, (uint32_t)code);
       if (g != 0xFFFFFFFFu) {
@@ -242,7 +248,9 @@ __global__ void __launch_bounds__(ST_BLOCK) k_star_fact(ScanDesc d, int fkcol, i
     }
   }
   for (int o = 32; o > 0; o >>= 1) nmatch += __shfl_xor(nmatch, o, 64);
+  for (int o = 32; o > 0; o >>= 1) nprobe += __shfl_xor(nprobe, o, 64);
   if (lane == 0 && nmatch) atomicAdd(matched, (unsigned long long)nmatch);
+  if (lane == 0 && nprobe) atomicAdd(probed, (unsigned long long)nprobe);
 }
 
 // ---- T: top-k over groups (sum DESC, g1 ASC, key ASC, g2 ASC) -----------------------------
@@ -465,10 +473,10 @@ extern "C" int aln_star_agg(aln_ctx *c, const aln_star_args *a, aln_star_row *h_
   unsigned long long *ctr;
   ALN_TRY(ws(c, bwords, &bitmap));
   ALN_TRY(ws(c, 2, &dup));
-  ALN_TRY(ws(c, 4, &ctr));
+  ALN_TRY(ws(c, 8, &ctr));
   ALN_TRY(fill_u32(c, bitmap, bwords, 0));
   ALN_TRY(fill_u32(c, dup, 2, 0));
-  ALN_TRY(fill_u64(c, (uint64_t *)ctr, 4, 0));
+  ALN_TRY(fill_u64(c, (uint64_t *)ctr, 8, 0));
 
   tm.mark(0);
   {
@@ -487,7 +495,8 @@ extern "C" int aln_star_agg(aln_ctx *c, const aln_star_args *a, aln_star_row *h_
   {
     ProfScope p(c, "star_mid");
     hipLaunchKernelGGL(k_star_mid, dim3((unsigned)mblocks), dim3(ST_BLOCK), 0, c->stream, md, a->m_key,
-                       a->m_fk, a->m_g1, a->m_g2, bitmap, dkmin, drange, ctr + 1, nmid, okey, og1, og2);
+                       a->m_fk, a->m_g1, a->m_g2, bitmap, dkmin, drange, ctr + 1, ctr + 4, nmid, okey, og1,
+                       og2);
   }
   ALN_TRY(check_launch("star_mid"));
   unsigned long long hc[2];
@@ -520,7 +529,7 @@ extern "C" int aln_star_agg(aln_ctx *c, const aln_star_args *a, aln_star_row *h_
     ProfScope p(c, "star_fact");
     hipLaunchKernelGGL(k_star_fact, dim3((unsigned)fblocks), dim3(ST_BLOCK), 0, c->stream, fd, a->f_fk,
                        a->f_x, a->f_y, a->a0, a->a1, a->b0, a->b1, table, tcap, mkmin, mrange, sum, cnt,
-                       ctr + 2);
+                       ctr + 2, ctr + 3);
   }
   ALN_TRY(check_launch("star_fact"));
   tm.mark(3);
@@ -565,11 +574,14 @@ extern "C" int aln_star_agg(aln_ctx *c, const aln_star_args *a, aln_star_row *h_
     *h_ntop = nt;
   }
   tm.mark(4);
-  ALN_TRY(copy_to_host(c, hc, ctr + 2, 8));
+  ALN_TRY(copy_to_host(c, hc, ctr + 2, 16));
   uint32_t bdup = 0;
   ALN_TRY(copy_to_host(c, &bdup, dup + 1, 4));
   if (bdup) return set_error(ALN_E_UNSUPPORTED, "aln_star_agg: duplicate mid keys");
   st.fact_pass = hc[0];
+  st.fact_probe = hc[1];
+  ALN_TRY(copy_to_host(c, hc, ctr + 4, 8));
+  st.mid_probe = hc[0];
 
   // every group, in ascending key order
   if ((a->flags & 1) && m) {

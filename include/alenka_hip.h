@@ -245,7 +245,12 @@ typedef struct {
 } aln_star_row;
 /* Row counts of each stage; the ms_* phase times are filled when profiling is enabled. */
 typedef struct {
-  uint64_t dim_pass, mid_pass, fact_pass, groups;
+  uint64_t dim_pass;   /* dimension rows passing (bitmap inserts)           */
+  uint64_t mid_probe;  /* mid rows passing the mid predicate (probes)       */
+  uint64_t mid_pass;   /* mid rows passing and joined (hash-table entries)  */
+  uint64_t fact_probe; /* fact rows passing the fact predicate (probes)     */
+  uint64_t fact_pass;  /* fact rows joined and aggregated                   */
+  uint64_t groups;     /* groups returned (flags bit 0)                     */
   double ms_dim, ms_mid, ms_fact, ms_topk;
 } aln_star_stats;
 /* Groups (if flags & 1) are written to d_groups (cap rows) in ascending key order. */

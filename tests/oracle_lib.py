@@ -26,6 +26,7 @@ def load():
         _o.orc_day_to_ymd.restype = C.c_int32
         _o.orc_gen_fact_rows.restype = C.c_uint64
         _o.orc_bits_for_range.restype = C.c_uint32
+        _o.orc_bits_for_range.argtypes = [C.c_int64, C.c_int64]
         _o.orc_unpack.restype = C.c_int64
         _o.orc_pfor_encode.restype = C.c_uint64
         _o.orc_pfor_delta_encode.restype = C.c_uint64
