@@ -93,6 +93,7 @@ int exclusive_scan_u32(aln_ctx *c, const uint32_t *d_in, uint64_t n, uint64_t *d
 int exclusive_scan_u64(aln_ctx *c, const uint64_t *d_in, uint64_t n, uint64_t *d_out,
                        uint64_t *h_total);
 int minmax_i64(aln_ctx *c, const int64_t *d, uint64_t n, int64_t *h_min, int64_t *h_max);
+int sum_u32(aln_ctx *c, const uint32_t *d_in, uint64_t n, uint64_t *d_out);  // no sync
 int radix_sort_pairs(aln_ctx *c, uint64_t *d_keys, uint32_t *d_vals, uint64_t n,
                      uint32_t bits);
 int fill_u64(aln_ctx *c, uint64_t *d, uint64_t n, uint64_t v);

@@ -4,6 +4,8 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <vector>
+
 #include "aln_internal.h"
 
 namespace aln {
@@ -253,8 +255,41 @@ int exclusive_scan_u64(aln_ctx *c, const uint64_t *d_in, uint64_t n, uint64_t *d
   return exclusive_scan_impl<uint64_t>(c, d_in, n, d_out, h_total);
 }
 
-__global__ void __launch_bounds__(256) k_minmax(const int64_t *__restrict__ d, uint64_t n,
-                                                unsigned long long *mn, unsigned long long *mx) {
+// Sum of a u32 array into *d_out (device), without atomics: per-block partial sums, then
+// one block adds the partials.
+__global__ void __launch_bounds__(256) k_sum_partials(const uint32_t *__restrict__ in, uint64_t n,
+                                                      uint64_t *__restrict__ part) {
+  uint64_t s = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256)
+    s += in[i];
+  uint64_t tot;
+  (void)block_excl_scan<uint64_t>(s, &tot);
+  if (threadIdx.x == 0) part[blockIdx.x] = tot;
+}
+__global__ void __launch_bounds__(256) k_sum_final(const uint64_t *__restrict__ part, uint32_t np,
+                                                   uint64_t *__restrict__ out) {
+  uint64_t s = 0;
+  for (uint32_t i = threadIdx.x; i < np; i += 256) s += part[i];
+  uint64_t tot;
+  (void)block_excl_scan<uint64_t>(s, &tot);
+  if (threadIdx.x == 0) *out = tot;
+}
+
+int sum_u32(aln_ctx *c, const uint32_t *d_in, uint64_t n, uint64_t *d_out) {
+  const unsigned nb = grid_for(n, 256 * 16, 1024);
+  uint64_t *part;
+  ALN_TRY(ws(c, nb, &part));
+  hipLaunchKernelGGL(k_sum_partials, dim3(nb), dim3(256), 0, c->stream, d_in, n, part);
+  hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(256), 0, c->stream, part, nb, d_out);
+  return check_launch("sum_u32");
+}
+
+// Min/max of an int64 array: per-block partials, then one block folds them (no atomics on
+// a shared address).
+__global__ void __launch_bounds__(256) k_minmax_part(const int64_t *__restrict__ d, uint64_t n,
+                                                     int64_t *__restrict__ pmin,
+                                                     int64_t *__restrict__ pmax) {
+  __shared__ int64_t smn[4], smx[4];
   int64_t lo = INT64_MAX, hi = INT64_MIN;
   for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * 256) {
@@ -268,27 +303,40 @@ __global__ void __launch_bounds__(256) k_minmax(const int64_t *__restrict__ d, u
     hi = b > hi ? b : hi;
   }
   if (__lane_id() == 0) {
-    // order-preserving map of int64 to unsigned for atomicMin/Max
-    atomicMin(mn, (unsigned long long)lo ^ 0x8000000000000000ull);
-    atomicMax(mx, (unsigned long long)hi ^ 0x8000000000000000ull);
+    smn[threadIdx.x / 64] = lo;
+    smx[threadIdx.x / 64] = hi;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < 4; ++w) {
+      lo = smn[w] < lo ? smn[w] : lo;
+      hi = smx[w] > hi ? smx[w] : hi;
+    }
+    lo = smn[0] < lo ? smn[0] : lo;
+    hi = smx[0] > hi ? smx[0] : hi;
+    pmin[blockIdx.x] = lo;
+    pmax[blockIdx.x] = hi;
   }
 }
 
 int minmax_i64(aln_ctx *c, const int64_t *d, uint64_t n, int64_t *h_min, int64_t *h_max) {
-  unsigned long long *mm;
-  ALN_TRY(ws(c, 2, &mm));
-  unsigned long long init[2] = {~0ull, 0ull};
-  ALN_HIP(hipMemcpyAsync(mm, init, 16, hipMemcpyHostToDevice, c->stream));
+  const unsigned nb = grid_for(n, 256 * 16, 1024);
+  int64_t *part;
+  ALN_TRY(ws(c, 2 * (size_t)nb, &part));
   if (n) {
     ProfScope p(c, "minmax");
-    hipLaunchKernelGGL(k_minmax, dim3(grid_for(n, 256, 4096)), dim3(256), 0, c->stream, d, n, mm,
-                       mm + 1);
+    hipLaunchKernelGGL(k_minmax_part, dim3(nb), dim3(256), 0, c->stream, d, n, part, part + nb);
   }
   ALN_TRY(check_launch("minmax"));
-  unsigned long long h[2];
-  ALN_TRY(copy_to_host(c, h, mm, 16));
-  *h_min = (int64_t)(h[0] ^ 0x8000000000000000ull);
-  *h_max = (int64_t)(h[1] ^ 0x8000000000000000ull);
+  std::vector<int64_t> h(2 * (size_t)nb);
+  *h_min = INT64_MAX;
+  *h_max = INT64_MIN;
+  if (!n) return ALN_OK;
+  ALN_TRY(copy_to_host(c, h.data(), part, 16ull * nb));
+  for (unsigned i = 0; i < nb; ++i) {
+    *h_min = h[i] < *h_min ? h[i] : *h_min;
+    *h_max = h[nb + i] > *h_max ? h[nb + i] : *h_max;
+  }
   return ALN_OK;
 }
 

@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "aln_internal.h"
+#include "stage.h"
 
 namespace aln {
 
@@ -70,20 +71,69 @@ __device__ __forceinline__ bool sd_pass(const ScanDesc &d, char mode, uint32_t s
   return p;
 }
 
+// Block prologue of every scan kernel: stage the range columns (when the zone map says the
+// tile needs evaluating) and `key` into LDS.
+__device__ __forceinline__ void star_stage(TileStage &ts, int *want, const ScanDesc &d, char mode,
+                                           int key, uint32_t seg, uint32_t r0, uint32_t nt) {
+  __shared__ int nwant;
+  if (threadIdx.x == 0) {
+    int n = 0;
+    if (mode == 'R')
+      for (int k = 0; k < d.nr; ++k) want[n++] = d.rcol[k];
+    if (key >= 0) want[n++] = key;
+    nwant = n;
+  }
+  __syncthreads();
+  stage_tile(ts, d.words, d.segs, want, nwant, seg, r0, nt);
+  __syncthreads();
+}
+
+__device__ __forceinline__ bool tile_pass(const ScanDesc &d, const TileStage &ts, char mode,
+                                          uint32_t seg, uint32_t r0, uint32_t i) {
+  if (mode == 'A') return true;
+  bool p = true;
+  for (int k = 0; k < d.nr; ++k) {
+    int64_t v = tile_get(ts, d.words, d.segs, d.rcol[k], seg, r0, i);
+    p = p && v >= d.rlo[k] && v <= d.rhi[k];
+  }
+  return p;
+}
+
 // ---- D: dimension scan into a key bitmap --------------------------------------------------
+// Per-tile counts go to arrays indexed by block (summed afterwards): a counter shared by
+// every block would serialise its atomics across the XCDs.
+__device__ __forceinline__ void block_count(uint32_t v, uint32_t *out) {
+  __shared__ uint32_t ws_[ST_BLOCK / 64];
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  __syncthreads();
+  if (__lane_id() == 0) ws_[threadIdx.x / 64] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int w = 0; w < ST_BLOCK / 64; ++w) t += ws_[w];
+    out[blockIdx.x] = t;
+  }
+}
+
 __global__ void __launch_bounds__(ST_BLOCK) k_star_dim(ScanDesc d, int kcol, int64_t kmin,
                                                        uint64_t range, uint32_t *bitmap,
-                                                       unsigned long long *npass,
-                                                       uint32_t *dup) {
+                                                       uint32_t *npass, uint32_t *dup) {
+  __shared__ TileStage ts;
+  __shared__ int want[ALN_MAX_RANGES + 1];
   const uint32_t seg = blockIdx.x / d.tps, tile = blockIdx.x % d.tps;
   const uint32_t nrows = d.segs[0][seg].nrows, r0 = tile * ST_TILE;
   const char mode = sd_mode(d, seg);
-  if (r0 >= nrows || mode == 'N') return;
+  if (r0 >= nrows || mode == 'N') {
+    if (threadIdx.x == 0) npass[blockIdx.x] = 0;
+    return;
+  }
+  const uint32_t nt = nrows - r0 < (uint32_t)ST_TILE ? nrows - r0 : (uint32_t)ST_TILE;
+  star_stage(ts, want, d, mode, kcol, seg, r0, nt);
   uint32_t cnt = 0;
   for (int j = 0; j < ST_ITEMS; ++j) {
-    uint32_t local = r0 + j * ST_BLOCK + threadIdx.x;
-    if (local < nrows && sd_pass(d, mode, seg, local)) {
-      uint64_t b = (uint64_t)sd_decode(d, kcol, seg, local) - (uint64_t)kmin;
+    uint32_t i = j * ST_BLOCK + threadIdx.x;
+    if (i < nt && tile_pass(d, ts, mode, seg, r0, i)) {
+      uint64_t b = (uint64_t)tile_get(ts, d.words, d.segs, kcol, seg, r0, i) - (uint64_t)kmin;
       if (b < range) {
         uint32_t bit = 1u << (b & 31);
         uint32_t old = atomicOr(&bitmap[b >> 5], bit);
@@ -92,52 +142,66 @@ __global__ void __launch_bounds__(ST_BLOCK) k_star_dim(ScanDesc d, int kcol, int
       }
     }
   }
-  for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o, 64);
-  if (__lane_id() == 0 && cnt) atomicAdd(npass, (unsigned long long)cnt);
+  block_count(cnt, npass);
 }
 
 // ---- M: mid scan, probe the dimension bitmap, compact passing rows ------------------------
 __global__ void __launch_bounds__(ST_BLOCK) k_star_mid(ScanDesc d, int kcol, int fkcol, int g1col,
                                                        int g2col, const uint32_t *__restrict__ bitmap,
                                                        int64_t dkmin, uint64_t drange,
-                                                       unsigned long long *counter,
-                                                       unsigned long long *probed, uint64_t cap,
+                                                       uint32_t *__restrict__ counts,
+                                                       uint32_t *__restrict__ probed,
                                                        int64_t *__restrict__ okey,
                                                        int64_t *__restrict__ og1,
-                                                       int64_t *__restrict__ og2) {
+                                                       int64_t *__restrict__ og2,
+                                                       uint32_t *__restrict__ mbits, int64_t mkmin,
+                                                       uint32_t *dup) {
+  // passing rows of tile b are written to entries [b * ST_TILE, b * ST_TILE + counts[b])
   __shared__ uint32_t wc[ST_BLOCK / 64];
-  __shared__ unsigned long long base_s;
+  __shared__ TileStage ts;
+  __shared__ int want[ALN_MAX_RANGES + 1];
   const uint32_t seg = blockIdx.x / d.tps, tile = blockIdx.x % d.tps;
   const uint32_t nrows = d.segs[0][seg].nrows, r0 = tile * ST_TILE;
   const char mode = sd_mode(d, seg);
-  if (r0 >= nrows || mode == 'N') return;
+  if (r0 >= nrows || mode == 'N') {
+    if (threadIdx.x == 0) {
+      counts[blockIdx.x] = 0;
+      probed[blockIdx.x] = 0;
+    }
+    return;
+  }
+  const uint32_t nt = nrows - r0 < (uint32_t)ST_TILE ? nrows - r0 : (uint32_t)ST_TILE;
+  star_stage(ts, want, d, mode, fkcol, seg, r0, nt);
   const unsigned lane = __lane_id(), wid = threadIdx.x / 64;
-  bool pass[ST_ITEMS];
+  uint32_t passmask = 0;
   uint32_t mine = 0, nprobe = 0;
   for (int j = 0; j < ST_ITEMS; ++j) {
-    uint32_t local = r0 + j * ST_BLOCK + threadIdx.x;
-    bool p = local < nrows && sd_pass(d, mode, seg, local);
+    uint32_t i = j * ST_BLOCK + threadIdx.x;
+    bool p = i < nt && tile_pass(d, ts, mode, seg, r0, i);
     if (p) {
       ++nprobe;
-      uint64_t b = (uint64_t)sd_decode(d, fkcol, seg, local) - (uint64_t)dkmin;
+      uint64_t b = (uint64_t)tile_get(ts, d.words, d.segs, fkcol, seg, r0, i) - (uint64_t)dkmin;
       p = b < drange && ((bitmap[b >> 5] >> (b & 31)) & 1);
     }
-    pass[j] = p;
+    if (p && mbits) {
+      uint64_t code = (uint64_t)tile_get(ts, d.words, d.segs, kcol, seg, r0, i) - (uint64_t)mkmin;
+      uint32_t bit = 1u << (code & 31);
+      if (atomicOr(&mbits[code >> 5], bit) & bit) atomicExch(dup, 1u);
+    }
+    passmask |= (uint32_t)p << j;
     mine += p;
   }
+  block_count(nprobe, probed);
   uint32_t wsum = mine;
   for (int o = 32; o > 0; o >>= 1) wsum += __shfl_xor(wsum, o, 64);
-  for (int o = 32; o > 0; o >>= 1) nprobe += __shfl_xor(nprobe, o, 64);
-  if (lane == 0 && nprobe) atomicAdd(probed, (unsigned long long)nprobe);
   if (lane == 0) wc[wid] = wsum;
   __syncthreads();
   if (threadIdx.x == 0) {
     uint32_t t = 0;
     for (int w = 0; w < ST_BLOCK / 64; ++w) t += wc[w];
-    base_s = t ? atomicAdd(counter, (unsigned long long)t) : 0;
+    counts[blockIdx.x] = t;
   }
-  __syncthreads();
-  uint64_t pos = base_s;
+  uint64_t pos = (uint64_t)blockIdx.x * ST_TILE;
   for (int w = 0; w < (int)wid; ++w) pos += wc[w];
   // order inside the tile: (item, lane) within the wave; fine, output order is free
   uint32_t x = mine;
@@ -147,13 +211,11 @@ __global__ void __launch_bounds__(ST_BLOCK) k_star_mid(ScanDesc d, int kcol, int
   }
   pos += x - mine;
   for (int j = 0; j < ST_ITEMS; ++j) {
-    if (!pass[j]) continue;
-    uint32_t local = r0 + j * ST_BLOCK + threadIdx.x;
-    if (pos < cap) {
-      okey[pos] = sd_decode(d, kcol, seg, local);
-      og1[pos] = g1col >= 0 ? sd_decode(d, g1col, seg, local) : 0;
-      og2[pos] = g2col >= 0 ? sd_decode(d, g2col, seg, local) : 0;
-    }
+    if (!((passmask >> j) & 1)) continue;
+    uint32_t i = j * ST_BLOCK + threadIdx.x;
+    okey[pos] = tile_get(ts, d.words, d.segs, kcol, seg, r0, i);
+    og1[pos] = g1col >= 0 ? tile_get(ts, d.words, d.segs, g1col, seg, r0, i) : 0;
+    og2[pos] = g2col >= 0 ? tile_get(ts, d.words, d.segs, g2col, seg, r0, i) : 0;
     ++pos;
   }
 }
@@ -163,12 +225,24 @@ __device__ __forceinline__ uint64_t st_slot(uint32_t code, uint64_t cap) {
 }
 
 // ---- B: build the join table over the compacted mid rows ----------------------------------
-__global__ void __launch_bounds__(256) k_star_build(const int64_t *__restrict__ okey, uint64_t n,
+// One block per mid tile region: entry i of tile b becomes group offs[b] + i; its key and
+// group columns move to dense arrays and the key goes into the table.
+__global__ void __launch_bounds__(256) k_star_build(const int64_t *__restrict__ rkey,
+                                                    const int64_t *__restrict__ rg1,
+                                                    const int64_t *__restrict__ rg2,
+                                                    const uint32_t *__restrict__ counts,
+                                                    const uint64_t *__restrict__ offs,
                                                     int64_t kmin, uint64_t *table, uint64_t cap,
-                                                    uint32_t *dup) {
-  for (uint64_t j = (uint64_t)blockIdx.x * 256 + threadIdx.x; j < n;
-       j += (uint64_t)gridDim.x * 256) {
-    uint32_t code = (uint32_t)((uint64_t)okey[j] - (uint64_t)kmin);
+                                                    int64_t *__restrict__ okey,
+                                                    int64_t *__restrict__ og1,
+                                                    int64_t *__restrict__ og2, uint32_t *dup) {
+  const uint32_t n = counts[blockIdx.x];
+  for (uint32_t i = threadIdx.x; i < n; i += 256) {
+    const uint64_t src = (uint64_t)blockIdx.x * ST_TILE + i, j = offs[blockIdx.x] + i;
+    okey[j] = rkey[src];
+    og1[j] = rg1[src];
+    og2[j] = rg2[src];
+    uint32_t code = (uint32_t)((uint64_t)rkey[src] - (uint64_t)kmin);
     uint64_t entry = ((uint64_t)j << 32) | code;
     uint64_t h = st_slot(code, cap);
     for (uint64_t probes = 0; probes < cap; ++probes) {
@@ -203,21 +277,32 @@ __global__ void __launch_bounds__(ST_BLOCK) k_star_fact(ScanDesc d, int fkcol, i
                                                         uint64_t cap, int64_t mkmin, uint64_t mrange,
                                                         unsigned long long *__restrict__ sum,
                                                         unsigned int *__restrict__ cnt,
-                                                        unsigned long long *matched,
-                                                        unsigned long long *probed) {
+                                                        uint32_t *__restrict__ matched,
+                                                        uint32_t *__restrict__ probed) {
+  __shared__ TileStage ts;
+  __shared__ int want[ALN_MAX_RANGES + 1];
   const uint32_t seg = blockIdx.x / d.tps, tile = blockIdx.x % d.tps;
   const uint32_t nrows = d.segs[0][seg].nrows, r0 = tile * ST_TILE;
   const char mode = sd_mode(d, seg);
-  if (r0 >= nrows || mode == 'N') return;
+  if (r0 >= nrows || mode == 'N') {
+    if (threadIdx.x == 0) {
+      matched[blockIdx.x] = 0;
+      probed[blockIdx.x] = 0;
+    }
+    return;
+  }
+  const uint32_t nt = nrows - r0 < (uint32_t)ST_TILE ? nrows - r0 : (uint32_t)ST_TILE;
+  star_stage(ts, want, d, mode, fkcol, seg, r0, nt);
   const unsigned lane = __lane_id();
   uint32_t nmatch = 0, nprobe = 0;
   for (int j = 0; j < ST_ITEMS; ++j) {
-    uint32_t local = r0 + j * ST_BLOCK + threadIdx.x;
+    uint32_t i = j * ST_BLOCK + threadIdx.x;
+    uint32_t local = r0 + i;
     uint32_t g = 0xFFFFFFFFu;
     int64_t v = 0;
-    if (local < nrows && sd_pass(d, mode, seg, local)) {
+    if (i < nt && tile_pass(d, ts, mode, seg, r0, i)) {
       ++nprobe;
-      uint64_t code = (uint64_t)sd_decode(d, fkcol, seg, local) - (uint64_t)mkmin;
+      uint64_t code = (uint64_t)tile_get(ts, d.words, d.segs, fkcol, seg, r0, i) - (uint64_t)mkmin;
       if (code < mrange) g = st_find(table, cap, (uint32_t)code);
       if (g != 0xFFFFFFFFu) {
         int64_t x = sd_decode(d, xcol, seg, local), y = sd_decode(d, ycol, seg, local);
@@ -247,16 +332,129 @@ __global__ void __launch_bounds__(ST_BLOCK) k_star_fact(ScanDesc d, int fkcol, i
       nmatch += c;
     }
   }
-  for (int o = 32; o > 0; o >>= 1) nmatch += __shfl_xor(nmatch, o, 64);
-  for (int o = 32; o > 0; o >>= 1) nprobe += __shfl_xor(nprobe, o, 64);
-  if (lane == 0 && nmatch) atomicAdd(matched, (unsigned long long)nmatch);
-  if (lane == 0 && nprobe) atomicAdd(probed, (unsigned long long)nprobe);
+  block_count(nmatch, matched);
+  block_count(nprobe, probed);
+}
+
+// ---- bitmap + rank join (dense key ranges) -----------------------------------------------
+// The passing mid keys set bits of a bitmap over [mkmin, mkmax]; a directory of set-bit
+// counts per 512-bit block (one 64-byte line) makes rank(key) - the key's position among
+// the passing keys - cost one directory read plus one line of the bitmap.  The group index
+// is that rank, so groups come out in key order, and probes from a tile of fact rows that
+// are clustered on the key touch a few contiguous lines instead of random slots.
+constexpr int BM_BLOCK_WORDS = 16;  // 512 bits
+
+__global__ void __launch_bounds__(256) k_bm_block_counts(const uint32_t *__restrict__ bits,
+                                                        uint64_t nwords, uint64_t nblk,
+                                                        uint32_t *__restrict__ cnt) {
+  for (uint64_t b = (uint64_t)blockIdx.x * 256 + threadIdx.x; b < nblk;
+       b += (uint64_t)gridDim.x * 256) {
+    uint32_t c = 0;
+    uint64_t w0 = b * BM_BLOCK_WORDS;
+    for (int k = 0; k < BM_BLOCK_WORDS; ++k)
+      if (w0 + k < nwords) c += __popc(bits[w0 + k]);
+    cnt[b] = c;
+  }
+}
+
+__device__ __forceinline__ uint64_t bm_rank(const uint32_t *__restrict__ bits,
+                                            const uint64_t *__restrict__ rank, uint64_t code) {
+  uint64_t w = code >> 5, blk = code >> 9;
+  uint64_t r = rank[blk];
+  for (uint64_t k = blk * BM_BLOCK_WORDS; k < w; ++k) r += __popc(bits[k]);
+  return r + __popc(bits[w] & ((1u << (code & 31)) - 1u));
+}
+
+// One block per mid tile region (see k_star_mid); entry i moves to position rank(key).
+__global__ void __launch_bounds__(256) k_bm_scatter(const int64_t *__restrict__ rkey,
+                                                   const int64_t *__restrict__ rg1,
+                                                   const int64_t *__restrict__ rg2,
+                                                   const uint32_t *__restrict__ counts,
+                                                   const uint32_t *__restrict__ bits,
+                                                   const uint64_t *__restrict__ rank, int64_t kmin,
+                                                   int64_t *__restrict__ skey,
+                                                   int64_t *__restrict__ sg1,
+                                                   int64_t *__restrict__ sg2) {
+  const uint32_t n = counts[blockIdx.x];
+  for (uint32_t i = threadIdx.x; i < n; i += 256) {
+    const uint64_t src = (uint64_t)blockIdx.x * ST_TILE + i;
+    const uint64_t j = bm_rank(bits, rank, (uint64_t)rkey[src] - (uint64_t)kmin);
+    skey[j] = rkey[src];
+    sg1[j] = rg1[src];
+    sg2[j] = rg2[src];
+  }
+}
+
+__global__ void __launch_bounds__(ST_BLOCK) k_star_fact_bm(ScanDesc d, int fkcol, int xcol, int ycol,
+                                                           int64_t a0, int64_t a1, int64_t b0,
+                                                           int64_t b1, const uint32_t *__restrict__ bits,
+                                                           const uint64_t *__restrict__ rank,
+                                                           int64_t mkmin, uint64_t mrange,
+                                                           unsigned long long *__restrict__ sum,
+                                                           unsigned int *__restrict__ cnt,
+                                                           uint32_t *__restrict__ matched,
+                                                           uint32_t *__restrict__ probed) {
+  __shared__ TileStage ts;
+  __shared__ int want[ALN_MAX_RANGES + 1];
+  const uint32_t seg = blockIdx.x / d.tps, tile = blockIdx.x % d.tps;
+  const uint32_t nrows = d.segs[0][seg].nrows, r0 = tile * ST_TILE;
+  const char mode = sd_mode(d, seg);
+  if (r0 >= nrows || mode == 'N') {
+    if (threadIdx.x == 0) {
+      matched[blockIdx.x] = 0;
+      probed[blockIdx.x] = 0;
+    }
+    return;
+  }
+  const uint32_t nt = nrows - r0 < (uint32_t)ST_TILE ? nrows - r0 : (uint32_t)ST_TILE;
+  star_stage(ts, want, d, mode, fkcol, seg, r0, nt);
+  const unsigned lane = __lane_id();
+  uint32_t nmatch = 0, nprobe = 0;
+  for (int j = 0; j < ST_ITEMS; ++j) {
+    uint32_t i = j * ST_BLOCK + threadIdx.x;
+    uint32_t local = r0 + i;
+    uint32_t g = 0xFFFFFFFFu;
+    int64_t v = 0;
+    if (i < nt && tile_pass(d, ts, mode, seg, r0, i)) {
+      ++nprobe;
+      uint64_t code = (uint64_t)tile_get(ts, d.words, d.segs, fkcol, seg, r0, i) - (uint64_t)mkmin;
+      if (code < mrange && ((bits[code >> 5] >> (code & 31)) & 1)) {
+        g = (uint32_t)bm_rank(bits, rank, code);
+        int64_t x = sd_decode(d, xcol, seg, local), y = sd_decode(d, ycol, seg, local);
+        v = wmul((int64_t)((uint64_t)a0 + (uint64_t)wmul(a1, x)),
+                 (int64_t)((uint64_t)b0 + (uint64_t)wmul(b1, y)));
+      }
+    }
+    uint32_t prev = __shfl_up(g, 1, 64);
+    bool f = lane == 0 || prev != g;
+    uint32_t c = g != 0xFFFFFFFFu;
+    for (int o = 1; o < 64; o <<= 1) {
+      int64_t vu = __shfl_up(v, o, 64);
+      uint32_t cu = __shfl_up(c, o, 64);
+      bool fu = __shfl_up((int)f, o, 64) != 0;
+      if (lane >= (unsigned)o && !f) {
+        v = (int64_t)((uint64_t)v + (uint64_t)vu);
+        c += cu;
+      }
+      if (lane >= (unsigned)o) f = f || fu;
+    }
+    uint32_t next = __shfl_down(g, 1, 64);
+    bool tail = g != 0xFFFFFFFFu && (lane == 63 || next != g);
+    if (tail) {
+      atomicAdd(&sum[g], (unsigned long long)v);
+      atomicAdd(&cnt[g], c);
+      nmatch += c;
+    }
+  }
+  block_count(nmatch, matched);
+  block_count(nprobe, probed);
 }
 
 // ---- T: top-k over groups (sum DESC, g1 ASC, key ASC, g2 ASC) -----------------------------
 struct Cand {
   int64_t sum, g1, key, g2;
   uint32_t j, valid;
+  uint32_t count, pad;
 };
 
 __device__ __forceinline__ bool cand_better(const Cand &a, const Cand &b) {
@@ -292,12 +490,14 @@ __global__ void __launch_bounds__(256) k_star_topk_tile(const Cand *__restrict__
       c.g2 = og2[i];
       c.j = (uint32_t)i;
       c.valid = 1;
+      c.count = cnt[i];
     } else {
       c.valid = 0;
     }
     if (!c.valid) {
       c.sum = c.g1 = c.key = c.g2 = 0;
       c.j = 0xFFFFFFFFu;
+      c.count = 0;
     }
     s[t] = c;
   }
@@ -465,73 +665,126 @@ extern "C" int aln_star_agg(aln_ctx *c, const aln_star_args *a, aln_star_row *h_
   if (mrange == 0 || mrange >= 0xFFFFFFFFull)
     return set_error(ALN_E_UNSUPPORTED, "aln_star_agg: mid key range %llu wider than 32 bits",
                      (unsigned long long)mrange);
-  uint64_t nmid = 0;
-  for (uint32_t s = 0; s < a->mid.cols[0].nsegs; ++s) nmid += a->mid.seg_rows;  // upper bound
-
+  // join structure for mid -> fact: bitmap + rank over dense key ranges (up to a 256 MB
+  // bitmap), MurmurHash open addressing otherwise or when flags bit 1 asks for it
+  const bool use_bm = !(a->flags & 2) && mrange <= (1ull << 31);
+  uint64_t mwords = use_bm ? (mrange + 31) / 32 : 0;
+  uint64_t mblk = use_bm ? (mwords + BM_BLOCK_WORDS - 1) / BM_BLOCK_WORDS : 0;
+  uint32_t *mbits = nullptr;
+  if (use_bm) {
+    ALN_TRY(ws(c, mwords, &mbits));
+    ALN_TRY(fill_u32(c, mbits, mwords, 0));
+  }
   uint64_t bwords = (drange + 31) / 32;
   uint32_t *bitmap, *dup;
-  unsigned long long *ctr;
+  uint64_t *ctr;  // device sums: 0 dim pass, 1 mid pass, 2 mid probe, 3 fact pass, 4 fact probe
+  uint32_t *dcnt, *mcnt, *mprobe, *fmatch, *fprobe;
   ALN_TRY(ws(c, bwords, &bitmap));
   ALN_TRY(ws(c, 2, &dup));
   ALN_TRY(ws(c, 8, &ctr));
+  ALN_TRY(ws(c, dblocks, &dcnt));
+  ALN_TRY(ws(c, mblocks, &mcnt));
+  ALN_TRY(ws(c, mblocks, &mprobe));
+  ALN_TRY(ws(c, fblocks, &fmatch));
+  ALN_TRY(ws(c, fblocks, &fprobe));
   ALN_TRY(fill_u32(c, bitmap, bwords, 0));
   ALN_TRY(fill_u32(c, dup, 2, 0));
-  ALN_TRY(fill_u64(c, (uint64_t *)ctr, 8, 0));
+  ALN_TRY(fill_u64(c, ctr, 8, 0));
 
   tm.mark(0);
   {
     ProfScope p(c, "star_dim");
     hipLaunchKernelGGL(k_star_dim, dim3((unsigned)dblocks), dim3(ST_BLOCK), 0, c->stream, dd, a->d_key,
-                       dkmin, drange, bitmap, ctr + 0, dup + 0);
+                       dkmin, drange, bitmap, dcnt, dup + 0);
   }
   ALN_TRY(check_launch("star_dim"));
   tm.mark(1);
 
-  // mid scan: compact passing rows (capacity = all mid rows, at most)
-  int64_t *okey, *og1, *og2;
-  ALN_TRY(ws(c, nmid, &okey));
-  ALN_TRY(ws(c, nmid, &og1));
-  ALN_TRY(ws(c, nmid, &og2));
+  // mid scan: passing rows of tile b land in entries [b * ST_TILE, ...) of the region arrays
+  int64_t *rkey, *rg1, *rg2;
+  ALN_TRY(ws(c, mblocks * ST_TILE, &rkey));
+  ALN_TRY(ws(c, mblocks * ST_TILE, &rg1));
+  ALN_TRY(ws(c, mblocks * ST_TILE, &rg2));
   {
     ProfScope p(c, "star_mid");
     hipLaunchKernelGGL(k_star_mid, dim3((unsigned)mblocks), dim3(ST_BLOCK), 0, c->stream, md, a->m_key,
-                       a->m_fk, a->m_g1, a->m_g2, bitmap, dkmin, drange, ctr + 1, ctr + 4, nmid, okey, og1,
-                       og2);
+                       a->m_fk, a->m_g1, a->m_g2, bitmap, dkmin, drange, mcnt, mprobe, rkey, rg1, rg2,
+                       mbits, mkmin, dup + 1);
   }
   ALN_TRY(check_launch("star_mid"));
-  unsigned long long hc[2];
+  ALN_TRY(sum_u32(c, dcnt, dblocks, ctr + 0));
+  ALN_TRY(sum_u32(c, mcnt, mblocks, ctr + 1));
+  ALN_TRY(sum_u32(c, mprobe, mblocks, ctr + 2));
+  uint64_t hc[8];
   uint32_t hdup[2];
-  ALN_TRY(copy_to_host(c, hc, ctr, 16));
+  ALN_TRY(copy_to_host(c, hc, ctr, 24));
   ALN_TRY(copy_to_host(c, hdup, dup, 8));
   if (hdup[0]) return set_error(ALN_E_UNSUPPORTED, "aln_star_agg: duplicate dimension keys");
+  if (hdup[1]) return set_error(ALN_E_UNSUPPORTED, "aln_star_agg: duplicate mid keys");
   st.dim_pass = hc[0];
-  uint64_t m = hc[1];
+  const uint64_t m = hc[1];
   st.mid_pass = m;
+  st.mid_probe = hc[2];
 
-  uint64_t tcap = m + m / 2 + 1024;
-  uint64_t *table;
   unsigned long long *sum;
   unsigned int *cnt;
-  ALN_TRY(ws(c, tcap, &table));
+  int64_t *okey, *og1, *og2;
   ALN_TRY(ws(c, m ? m : 1, &sum));
   ALN_TRY(ws(c, m ? m : 1, &cnt));
-  ALN_TRY(fill_u64(c, table, tcap, ST_EMPTY));
+  ALN_TRY(ws(c, m ? m : 1, &okey));
+  ALN_TRY(ws(c, m ? m : 1, &og1));
+  ALN_TRY(ws(c, m ? m : 1, &og2));
   ALN_TRY(fill_u64(c, (uint64_t *)sum, m, 0));
   ALN_TRY(fill_u32(c, cnt, m, 0));
-  if (m) {
-    ProfScope p(c, "star_build");
-    hipLaunchKernelGGL(k_star_build, dim3(grid_for(m, 256, 65536)), dim3(256), 0, c->stream, okey, m,
-                       mkmin, table, tcap, dup + 1);
+  uint64_t *table = nullptr, *rank = nullptr;
+  uint64_t tcap = 0;
+  if (use_bm) {
+    // rank directory, then the group attributes in rank (= key) order
+    uint32_t *bcnt;
+    ALN_TRY(ws(c, mblk, &bcnt));
+    ALN_TRY(ws(c, mblk, &rank));
+    {
+      ProfScope p(c, "star_build");
+      hipLaunchKernelGGL(k_bm_block_counts, dim3(grid_for(mblk, 256, 65536)), dim3(256), 0, c->stream,
+                         mbits, mwords, mblk, bcnt);
+    }
+    ALN_TRY(check_launch("bm_block_counts"));
+    ALN_TRY(exclusive_scan_u32(c, bcnt, mblk, rank, nullptr));
+    if (m) {
+      ProfScope p(c, "star_build");
+      hipLaunchKernelGGL(k_bm_scatter, dim3((unsigned)mblocks), dim3(256), 0, c->stream, rkey, rg1, rg2,
+                         mcnt, mbits, rank, mkmin, okey, og1, og2);
+    }
+    ALN_TRY(check_launch("bm_scatter"));
+  } else {
+    uint64_t *moff;
+    ALN_TRY(ws(c, mblocks, &moff));
+    ALN_TRY(exclusive_scan_u32(c, mcnt, mblocks, moff, nullptr));
+    tcap = m + m / 2 + 1024;
+    ALN_TRY(ws(c, tcap, &table));
+    ALN_TRY(fill_u64(c, table, tcap, ST_EMPTY));
+    if (m) {
+      ProfScope p(c, "star_build");
+      hipLaunchKernelGGL(k_star_build, dim3((unsigned)mblocks), dim3(256), 0, c->stream, rkey, rg1, rg2,
+                         mcnt, moff, mkmin, table, tcap, okey, og1, og2, dup + 1);
+    }
+    ALN_TRY(check_launch("star_build"));
   }
-  ALN_TRY(check_launch("star_build"));
   tm.mark(2);
   if (m) {
     ProfScope p(c, "star_fact");
-    hipLaunchKernelGGL(k_star_fact, dim3((unsigned)fblocks), dim3(ST_BLOCK), 0, c->stream, fd, a->f_fk,
-                       a->f_x, a->f_y, a->a0, a->a1, a->b0, a->b1, table, tcap, mkmin, mrange, sum, cnt,
-                       ctr + 2, ctr + 3);
+    if (use_bm)
+      hipLaunchKernelGGL(k_star_fact_bm, dim3((unsigned)fblocks), dim3(ST_BLOCK), 0, c->stream, fd,
+                         a->f_fk, a->f_x, a->f_y, a->a0, a->a1, a->b0, a->b1, mbits, rank, mkmin, mrange,
+                         sum, cnt, fmatch, fprobe);
+    else
+      hipLaunchKernelGGL(k_star_fact, dim3((unsigned)fblocks), dim3(ST_BLOCK), 0, c->stream, fd, a->f_fk,
+                         a->f_x, a->f_y, a->a0, a->a1, a->b0, a->b1, table, tcap, mkmin, mrange, sum, cnt,
+                         fmatch, fprobe);
+    ALN_TRY(check_launch("star_fact"));
+    ALN_TRY(sum_u32(c, fmatch, fblocks, ctr + 3));
+    ALN_TRY(sum_u32(c, fprobe, fblocks, ctr + 4));
   }
-  ALN_TRY(check_launch("star_fact"));
   tm.mark(3);
 
   // top-k
@@ -562,26 +815,18 @@ extern "C" int aln_star_agg(aln_ctx *c, const aln_star_args *a, aln_star_row *h_
       h_top[nt].g1 = hcand[i].g1;
       h_top[nt].g2 = hcand[i].g2;
       h_top[nt].sum = hcand[i].sum;
-      h_top[nt].count = 0;
+      h_top[nt].count = hcand[i].count;
       ++nt;
-    }
-    // counts of the winners
-    for (uint32_t i = 0; i < nt; ++i) {
-      unsigned int cc;
-      ALN_TRY(copy_to_host(c, &cc, cnt + hcand[i].j, 4));
-      h_top[i].count = cc;
     }
     *h_ntop = nt;
   }
   tm.mark(4);
-  ALN_TRY(copy_to_host(c, hc, ctr + 2, 16));
+  ALN_TRY(copy_to_host(c, hc, ctr, 40));
   uint32_t bdup = 0;
   ALN_TRY(copy_to_host(c, &bdup, dup + 1, 4));
   if (bdup) return set_error(ALN_E_UNSUPPORTED, "aln_star_agg: duplicate mid keys");
-  st.fact_pass = hc[0];
-  st.fact_probe = hc[1];
-  ALN_TRY(copy_to_host(c, hc, ctr + 4, 8));
-  st.mid_probe = hc[0];
+  st.fact_pass = hc[3];
+  st.fact_probe = hc[4];
 
   // every group, in ascending key order
   if ((a->flags & 1) && m) {
@@ -607,7 +852,7 @@ extern "C" int aln_star_agg(aln_ctx *c, const aln_star_args *a, aln_star_row *h_
       hipLaunchKernelGGL(k_star_group_codes, dim3(grid_for(m, 256, 65536)), dim3(256), 0, c->stream, flag,
                          pos, m, okey, mkmin, code, idx);
       ALN_TRY(check_launch("star_group_codes"));
-      ALN_TRY(radix_sort_pairs(c, code, idx, g, bits_for_range_u(mrange - 1)));
+      if (!use_bm) ALN_TRY(radix_sort_pairs(c, code, idx, g, bits_for_range_u(mrange - 1)));
       hipLaunchKernelGGL(k_star_group_emit, dim3(grid_for(g, 256, 65536)), dim3(256), 0, c->stream, idx, g,
                          okey, og1, og2, sum, cnt, d_groups);
       ALN_TRY(check_launch("star_group_emit"));

@@ -289,7 +289,8 @@ def dict_first_seen(ctx, vals, nvals):
     return list(code_of), out[: vals.numel()]
 
 
-def star_agg(ctx, dim, mid, fact, spec, limit=10, want_groups=False, groups_cap=None):
+def star_agg(ctx, dim, mid, fact, spec, limit=10, want_groups=False, groups_cap=None,
+             force_hash=False):
     """Fused filter-join-join-aggregate-top-k.  dim/mid/fact: dicts with 'cols' (list of
     Column), 'ranges' (list of (col, lo, hi)) and optional 'modes'.  spec: dict of the
     column slots d_key, m_key, m_fk, m_g1, m_g2, f_fk, f_x, f_y and a0, a1, b0, b1."""
@@ -316,7 +317,7 @@ def star_agg(ctx, dim, mid, fact, spec, limit=10, want_groups=False, groups_cap=
     for k in ("d_key", "m_key", "m_fk", "m_g1", "m_g2", "f_fk", "f_x", "f_y", "a0", "a1", "b0", "b1"):
         setattr(a, k, spec[k])
     a.limit = limit
-    a.flags = 1 if want_groups else 0
+    a.flags = (1 if want_groups else 0) | (2 if force_hash else 0)
     top = (L.StarRow * max(1, limit))()
     ntop = C.c_uint32()
     st = L.StarStats()

@@ -238,7 +238,9 @@ typedef struct {
   int32_t f_fk, f_x, f_y;
   int64_t a0, a1, b0, b1;
   uint32_t limit;
-  uint32_t flags; /* bit 0: also return every group (d_groups_*) */
+  uint32_t flags; /* bit 0: also return every group (d_groups);
+                     bit 1: join mid -> fact through the MurmurHash64A open-addressing table
+                     even when the mid key range is dense enough for the bitmap + rank index */
 } aln_star_args;
 typedef struct {
   int64_t key, g1, g2, sum, count;
@@ -253,7 +255,12 @@ typedef struct {
   uint64_t groups;     /* groups returned (flags bit 0)                     */
   double ms_dim, ms_mid, ms_fact, ms_topk;
 } aln_star_stats;
-/* Groups (if flags & 1) are written to d_groups (cap rows) in ascending key order. */
+/* Join structures: the dimension keys of passing rows set bits of a bitmap over the key
+ * range; the passing mid keys go into a bitmap with a per-512-bit rank directory when the
+ * mid key range is at most 2^31 (order-preserving: a group's index is the rank of its key),
+ * else into a MurmurHash64A open-addressing table.  Duplicate keys on either build side
+ * return ALN_E_UNSUPPORTED (the host layer then runs the unfused operators).
+ * Groups (if flags & 1) are written to d_groups (cap rows) in ascending key order. */
 int aln_star_agg(aln_ctx *ctx, const aln_star_args *args, aln_star_row *h_top, uint32_t *h_ntop,
                  aln_star_row *d_groups, uint64_t groups_cap, aln_star_stats *h_stats);
 
