@@ -1,0 +1,1269 @@
+// The reference's operator layer (operators.h / operators.cu) over HBM-resident tables:
+// the RPN queue emitters and the statement calls emit_filter / emit_select / emit_join /
+// emit_order / emit_store / emit_display, each running on the C-ABI kernels of
+// alenka_hip.h.  Variables live in a varNames catalog (cm.h:791) of CudaSet-shaped sets:
+// named columns, each either FOR + bit-packed in segments with zone maps (loaded tables)
+// or a plain int64 device array (operator results, `not_compressed`).
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <map>
+#include <memory>
+#include <set>
+#include <string>
+#include <vector>
+
+#include "../../include/alenka_hip.h"
+#include "../../include/alenka_ops.h"
+
+namespace {
+
+// ---- device memory -----------------------------------------------------------------------
+struct DBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  ~DBuf() {
+    if (p) (void)hipFree(p);
+  }
+};
+using DBufP = std::shared_ptr<DBuf>;
+
+DBufP dalloc(size_t bytes) {
+  auto b = std::make_shared<DBuf>();
+  if (bytes == 0) bytes = 8;
+  if (hipMalloc(&b->p, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  b->bytes = bytes;
+  return b;
+}
+
+struct Col {
+  int type = 0;       // 0 int/decimal, 2 char (dictionary code)
+  int decimals = 0;   // decimal_zeroes
+  DBufP words;        // packed words, or the int64 values of a plain column
+  DBufP dsegs;        // device aln_seg[nsegs]
+  std::vector<aln_seg> hsegs;
+  std::shared_ptr<std::vector<std::string>> dict;  // char: code -> string
+  uint32_t width = 0;
+  aln_col desc() const {
+    aln_col c;
+    c.words = (const uint32_t *)words->p;
+    c.segs = (const aln_seg *)dsegs->p;
+    c.nsegs = (uint32_t)hsegs.size();
+    c.decimals = decimals;
+    return c;
+  }
+};
+
+struct Tok {
+  std::string type;  // NAME NUMBER STRING CMP AND OR ADD MINUS MUL DIV JOIN COUNT SUM AVG MIN MAX SEL ORDER
+  std::string value;
+  int64_t num = 0;
+  uint32_t prec = 0;
+};
+
+struct Set {
+  std::vector<std::string> names;
+  std::map<std::string, Col> cols;
+  uint64_t nrows = 0;
+  uint32_t seg_rows = 1;
+  bool not_compressed = false;
+  bool filtered = false;
+  std::string source;
+  std::vector<Tok> pred;
+  // cached selection of a filtered set
+  DBufP sel;  // uint32 row ids of the source
+  uint64_t nsel = 0;
+  bool sel_ready = false;
+  uint32_t nsegs() const { return (uint32_t)((nrows + seg_rows - 1) / seg_rows); }
+};
+
+// ---- session state (the reference's globals, cm.h:66-117, operators.cu:20-50) -------------
+aln_ctx *g_ctx = nullptr;
+int g_device = 0;
+std::map<std::string, std::shared_ptr<Set>> varNames;
+std::vector<Tok> g_tok;
+std::vector<std::string> g_join;  // op_join
+int g_limit = 0;
+bool g_has_limit = false;
+bool g_join_bad = false;  // an unsupported join kind was queued for this statement
+uint32_t g_hash_seed = 100;
+int g_status = ALN_OK;
+char g_err[512] = "";
+
+int fail(int code, const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  g_status = code;
+  return code;
+}
+int fail_aln(int rc, const char *what) { return fail(rc, "%s: %s", what, aln_last_error()); }
+#define TRY(x)                    \
+  do {                            \
+    int _r = (x);                 \
+    if (_r != ALN_OK) return _r;  \
+  } while (0)
+#define ALN(x, what)                                   \
+  do {                                                 \
+    int _r = (x);                                      \
+    if (_r != ALN_OK) return fail_aln(_r, what);       \
+  } while (0)
+
+int ensure_ctx() {
+  if (g_ctx) return ALN_OK;
+  int rc = aln_create(g_device, nullptr, &g_ctx);
+  if (rc != ALN_OK) return fail_aln(rc, "alenka_init");
+  return ALN_OK;
+}
+
+std::shared_ptr<Set> find_var(const std::string &n) {
+  auto it = varNames.find(n);
+  return it == varNames.end() ? nullptr : it->second;
+}
+
+// ---- column construction ---------------------------------------------------------------
+int make_plain(const DBufP &vals, uint64_t n, uint32_t seg_rows, int decimals, Col *out) {
+  uint32_t nsegs = (uint32_t)((n + seg_rows - 1) / seg_rows);
+  DBufP ds = dalloc(sizeof(aln_seg) * (nsegs ? nsegs : 1));
+  if (!ds) return fail(ALN_E_NOMEM, "device allocation failed");
+  aln_col c;
+  ALN(aln_plain_col(g_ctx, (const int64_t *)vals->p, n, seg_rows, decimals, (aln_seg *)ds->p, &c),
+      "aln_plain_col");
+  out->words = vals;
+  out->dsegs = ds;
+  out->decimals = decimals;
+  out->hsegs.resize(nsegs);
+  for (uint32_t s = 0; s < nsegs; ++s) {
+    uint64_t r0 = (uint64_t)s * seg_rows;
+    aln_seg &g = out->hsegs[s];
+    g.word_off = 2 * r0;
+    g.base = 0;
+    g.bits = 64;
+    g.nrows = (uint32_t)(r0 + seg_rows <= n ? seg_rows : n - r0);
+    g.vmin = INT64_MIN;
+    g.vmax = INT64_MAX;
+  }
+  return ALN_OK;
+}
+
+int make_packed(const int64_t *d_vals, uint64_t n, uint32_t seg_rows, int decimals, Col *out) {
+  uint32_t nsegs = (uint32_t)((n + seg_rows - 1) / seg_rows);
+  out->hsegs.resize(nsegs);
+  uint64_t words = 0;
+  ALN(aln_encode_plan(g_ctx, d_vals, n, seg_rows, 0, out->hsegs.data(), &words), "aln_encode_plan");
+  out->words = dalloc(words * 4);
+  out->dsegs = dalloc(sizeof(aln_seg) * (nsegs ? nsegs : 1));
+  if (!out->words || !out->dsegs) return fail(ALN_E_NOMEM, "device allocation failed");
+  ALN(aln_encode_pack(g_ctx, d_vals, n, seg_rows, out->hsegs.data(), (uint32_t *)out->words->p,
+                      (aln_seg *)out->dsegs->p),
+      "aln_encode_pack");
+  out->decimals = decimals;
+  return ALN_OK;
+}
+
+// ---- views: a set's columns, resolved through a lazy filter ------------------------------
+struct View {
+  std::map<std::string, Col> cols;  // columns usable with seg_rows
+  uint64_t nrows = 0;
+  uint32_t seg_rows = 1;
+  uint32_t cycles = 1;  // the reference's segment loop count (group-by merge order)
+  DBufP rowids;         // int64: source row id of each row (for join ordering)
+};
+
+int resolve_string_codes(const Set &src, const std::vector<Tok> &t, std::vector<aln_rpn> *rpn,
+                         const std::map<std::string, int> &slot) {
+  for (size_t i = 0; i < t.size(); ++i) {
+    aln_rpn r;
+    memset(&r, 0, sizeof(r));
+    const Tok &k = t[i];
+    if (k.type == "NAME") {
+      r.type = ALN_RPN_NAME;
+      r.arg = slot.at(k.value);
+    } else if (k.type == "NUMBER") {
+      r.type = ALN_RPN_NUMBER;
+      r.val = k.num;
+      r.arg = (int32_t)k.prec;
+    } else if (k.type == "STRING") {
+      r.type = ALN_RPN_STRING;
+      r.val = -1;
+      // the literal is compared with the column named next to it (NAME STRING CMP or
+      // STRING NAME CMP); it becomes that column's dictionary code
+      std::string colname;
+      if (i > 0 && t[i - 1].type == "NAME") colname = t[i - 1].value;
+      if (i + 1 < t.size() && t[i + 1].type == "NAME") colname = t[i + 1].value;
+      auto it = src.cols.find(colname);
+      if (it != src.cols.end() && it->second.dict) {
+        auto &d = *it->second.dict;
+        auto f = std::find(d.begin(), d.end(), k.value);
+        if (f != d.end()) r.val = (int64_t)(f - d.begin());
+      }
+    } else if (k.type == "CMP") {
+      r.type = ALN_RPN_CMP;
+      r.arg = (int32_t)k.num;
+    } else if (k.type == "AND") {
+      r.type = ALN_RPN_AND;
+    } else if (k.type == "OR") {
+      r.type = ALN_RPN_OR;
+    } else if (k.type == "ADD") {
+      r.type = ALN_RPN_ADD;
+    } else if (k.type == "MINUS") {
+      r.type = ALN_RPN_MINUS;
+    } else if (k.type == "MUL") {
+      r.type = ALN_RPN_MUL;
+    } else if (k.type == "DIV") {
+      r.type = ALN_RPN_DIV;
+    } else {
+      return fail(ALN_E_UNSUPPORTED, "operation %s is not valid here", k.type.c_str());
+    }
+    rpn->push_back(r);
+  }
+  return ALN_OK;
+}
+
+// filter_op (cm.cu:3609-3666): zone maps per segment, then the fused scan.
+int run_filter(Set &f) {
+  if (f.sel_ready) return ALN_OK;
+  auto src = find_var(f.source);
+  if (!src) return fail(ALN_E_ARG, "Couldn't find %s", f.source.c_str());
+  std::map<std::string, int> slot;
+  std::vector<std::string> names;
+  for (auto &k : f.pred)
+    if (k.type == "NAME" && !slot.count(k.value)) {
+      if (!src->cols.count(k.value)) return fail(ALN_E_ARG, "Couldn't find column %s", k.value.c_str());
+      slot[k.value] = (int)names.size();
+      names.push_back(k.value);
+    }
+  if (names.empty()) {  // constant predicate: evaluate against any column
+    if (src->names.empty()) return fail(ALN_E_ARG, "filter on an empty set");
+    names.push_back(src->names[0]);
+  }
+  if (names.size() > ALN_MAX_COLS) return fail(ALN_E_UNSUPPORTED, "predicate reads too many columns");
+  std::vector<aln_rpn> rpn;
+  TRY(resolve_string_codes(*src, f.pred, &rpn, slot));
+  std::vector<aln_col> cols;
+  std::vector<int32_t> dec;
+  for (auto &n : names) {
+    cols.push_back(src->cols[n].desc());
+    dec.push_back(src->cols[n].decimals);
+  }
+  aln_insn prog[ALN_MAX_INSN];
+  int nprog = 0;
+  int32_t prec = 0;
+  ALN(aln_compile_rpn(rpn.data(), (int)rpn.size(), dec.data(), (int)dec.size(), 0, prog, &nprog, &prec),
+      "filter");
+  uint32_t nsegs = src->nsegs();
+  std::vector<char> modes(nsegs ? nsegs : 1, 'R');
+  std::vector<int64_t> mn(names.size()), mx(names.size());
+  for (uint32_t s = 0; s < nsegs; ++s) {
+    for (size_t c = 0; c < names.size(); ++c) {
+      mn[c] = src->cols[names[c]].hsegs[s].vmin;
+      mx[c] = src->cols[names[c]].hsegs[s].vmax;
+    }
+    ALN(aln_zone_map_check(rpn.data(), (int)rpn.size(), mn.data(), mx.data(), dec.data(),
+                           (int)dec.size(), src->not_compressed, &modes[s]),
+        "zone_map_check");
+  }
+  f.sel = dalloc(4 * (src->nrows ? src->nrows : 1));
+  if (!f.sel) return fail(ALN_E_NOMEM, "device allocation failed");
+  uint64_t cnt = 0;
+  ALN(aln_scan_filter(g_ctx, cols.data(), (int)cols.size(), src->seg_rows, prog, nprog, modes.data(),
+                      (uint32_t *)f.sel->p, src->nrows, &cnt),
+      "filter");
+  f.nsel = cnt;
+  f.sel_ready = true;
+  return ALN_OK;
+}
+
+// Columns `need` of variable `name`, decoded through its filter when it has one.
+int make_view(const std::string &name, const std::set<std::string> &need, View *v) {
+  auto a = find_var(name);
+  if (!a) return fail(ALN_E_ARG, "Couldn't find %s", name.c_str());
+  if (!a->filtered) {
+    for (auto &n : need) {
+      if (!a->cols.count(n)) return fail(ALN_E_ARG, "Couldn't find column %s", n.c_str());
+      v->cols[n] = a->cols[n];
+    }
+    v->nrows = a->nrows;
+    v->seg_rows = a->seg_rows;
+    v->cycles = a->not_compressed ? 1 : a->nsegs();
+    v->rowids = nullptr;
+    return ALN_OK;
+  }
+  TRY(run_filter(*a));
+  auto src = find_var(a->source);
+  const uint64_t k = a->nsel;
+  v->nrows = k;
+  v->seg_rows = (uint32_t)(k ? (k < 0xFFFFFFFFull ? k : 0xFFFFFFFFull) : 1);
+  v->cycles = src->not_compressed ? 1 : src->nsegs();
+  for (auto &n : need) {
+    if (!src->cols.count(n)) return fail(ALN_E_ARG, "Couldn't find column %s", n.c_str());
+    const Col &sc = src->cols[n];
+    DBufP vals = dalloc(8 * (k ? k : 1));
+    if (!vals) return fail(ALN_E_NOMEM, "device allocation failed");
+    aln_col d = sc.desc();
+    ALN(aln_gather(g_ctx, &d, src->seg_rows, (const uint32_t *)a->sel->p, k, (int64_t *)vals->p),
+        "gather");
+    Col c;
+    TRY(make_plain(vals, k, v->seg_rows, sc.decimals, &c));
+    c.type = sc.type;
+    c.dict = sc.dict;
+    c.width = sc.width;
+    v->cols[n] = c;
+  }
+  // source row ids as int64 (join output order needs the left segment)
+  v->rowids = dalloc(8 * (k ? k : 1));
+  if (!v->rowids) return fail(ALN_E_NOMEM, "device allocation failed");
+  if (k) {
+    std::vector<uint32_t> h(k);
+    if (hipMemcpy(h.data(), a->sel->p, 4 * k, hipMemcpyDeviceToHost) != hipSuccess)
+      return fail(ALN_E_HIP, "copy of row ids failed");
+    std::vector<int64_t> h64(h.begin(), h.end());
+    if (hipMemcpy(v->rowids->p, h64.data(), 8 * k, hipMemcpyHostToDevice) != hipSuccess)
+      return fail(ALN_E_HIP, "copy of row ids failed");
+  }
+  return ALN_OK;
+}
+
+int fetch_col(const Col &c, uint64_t n, uint32_t seg_rows, std::vector<int64_t> *out) {
+  out->resize(n);
+  if (!n) return ALN_OK;
+  DBufP tmp = dalloc(8 * n);
+  if (!tmp) return fail(ALN_E_NOMEM, "device allocation failed");
+  aln_col d = c.desc();
+  ALN(aln_decode(g_ctx, &d, seg_rows, 0, n, (int64_t *)tmp->p), "decode");
+  ALN(aln_sync(g_ctx), "sync");
+  if (hipMemcpy(out->data(), tmp->p, 8 * n, hipMemcpyDeviceToHost) != hipSuccess)
+    return fail(ALN_E_HIP, "device to host copy failed");
+  return ALN_OK;
+}
+
+// A plain int64 copy of a column (decoded).
+int plain_copy(const Col &c, uint64_t n, uint32_t seg_rows, DBufP *out) {
+  *out = dalloc(8 * (n ? n : 1));
+  if (!*out) return fail(ALN_E_NOMEM, "device allocation failed");
+  if (!n) return ALN_OK;
+  aln_col d = c.desc();
+  ALN(aln_decode(g_ctx, &d, seg_rows, 0, n, (int64_t *)(*out)->p), "decode");
+  return ALN_OK;
+}
+
+// ---- select-list parsing --------------------------------------------------------------------
+struct SelItem {
+  std::vector<Tok> expr;
+  std::string agg;  // "" (NULL), COUNT, SUM, AVG, MIN, MAX
+  std::string alias;
+};
+
+void parse_select(const std::vector<Tok> &t, std::vector<SelItem> *items, std::vector<Tok> *rest) {
+  SelItem cur;
+  for (auto &k : t) {
+    if (k.type == "SEL") {
+      cur.alias = k.value;
+      items->push_back(cur);
+      cur = SelItem();
+    } else if (k.type == "COUNT" || k.type == "SUM" || k.type == "AVG" || k.type == "MIN" ||
+               k.type == "MAX") {
+      cur.agg = k.type;
+    } else {
+      cur.expr.push_back(k);
+    }
+  }
+  *rest = cur.expr;  // trailing tokens: GROUP BY names, join conditions
+}
+
+int eval_expr(const View &v, const std::vector<Tok> &expr, DBufP *out, int *prec) {
+  std::map<std::string, int> slot;
+  std::vector<std::string> names;
+  for (auto &k : expr)
+    if (k.type == "NAME" && !slot.count(k.value)) {
+      if (!v.cols.count(k.value)) return fail(ALN_E_ARG, "Couldn't find column %s", k.value.c_str());
+      slot[k.value] = (int)names.size();
+      names.push_back(k.value);
+    }
+  Set dummy;
+  for (auto &n : names) dummy.cols[n] = v.cols.at(n);
+  std::vector<aln_rpn> rpn;
+  TRY(resolve_string_codes(dummy, expr, &rpn, slot));
+  std::vector<aln_col> cols;
+  std::vector<int32_t> dec;
+  for (auto &n : names) {
+    cols.push_back(v.cols.at(n).desc());
+    dec.push_back(v.cols.at(n).decimals);
+  }
+  *out = dalloc(8 * (v.nrows ? v.nrows : 1));
+  if (!*out) return fail(ALN_E_NOMEM, "device allocation failed");
+  if (names.empty()) {  // a constant
+    if (rpn.size() != 1 || rpn[0].type != ALN_RPN_NUMBER)
+      return fail(ALN_E_UNSUPPORTED, "expression without columns");
+    std::vector<int64_t> h(v.nrows, rpn[0].val);
+    *prec = rpn[0].arg;
+    if (v.nrows && hipMemcpy((*out)->p, h.data(), 8 * v.nrows, hipMemcpyHostToDevice) != hipSuccess)
+      return fail(ALN_E_HIP, "host to device copy failed");
+    return ALN_OK;
+  }
+  aln_insn prog[ALN_MAX_INSN];
+  int nprog = 0;
+  int32_t p = 0;
+  ALN(aln_compile_rpn(rpn.data(), (int)rpn.size(), dec.data(), (int)dec.size(), 1, prog, &nprog, &p),
+      "select expression");
+  ALN(aln_eval(g_ctx, cols.data(), (int)cols.size(), v.seg_rows, prog, nprog, nullptr, v.nrows,
+               (int64_t *)(*out)->p),
+      "select expression");
+  *prec = p;
+  return ALN_OK;
+}
+
+std::set<std::string> names_in(const std::vector<Tok> &t) {
+  std::set<std::string> s;
+  for (auto &k : t)
+    if (k.type == "NAME") s.insert(k.value);
+  return s;
+}
+
+void finish_statement() {
+  g_join_bad = false;
+  g_tok.clear();
+  g_join.clear();
+  g_has_limit = false;
+  g_limit = 0;
+}
+
+}  // namespace
+
+// =============================================================================================
+extern "C" {
+
+int alenka_init(int device) {
+  g_device = device;
+  if (hipSetDevice(device) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(ALN_E_HIP, "alenka_init: device %d", device);
+  }
+  g_status = ALN_OK;
+  return ensure_ctx();
+}
+
+int alenka_close(void) {
+  varNames.clear();
+  finish_statement();
+  if (g_ctx) aln_destroy(g_ctx);
+  g_ctx = nullptr;
+  return ALN_OK;
+}
+
+int alenka_status(void) { return g_status; }
+const char *alenka_last_error(void) { return g_err; }
+void alenka_set_hash_seed(uint32_t seed) { g_hash_seed = seed; }
+
+int alenka_table_create(const char *name, uint64_t nrows, uint32_t seg_rows) {
+  g_status = ALN_OK;
+  TRY(ensure_ctx());
+  if (!name || !seg_rows) return fail(ALN_E_ARG, "alenka_table_create: bad arguments");
+  auto s = std::make_shared<Set>();
+  s->nrows = nrows;
+  s->seg_rows = seg_rows;
+  varNames[name] = s;
+  return ALN_OK;
+}
+
+static int add_int(const char *table, const char *col, const int64_t *vals, bool device,
+                   int32_t decimals, int compress) {
+  g_status = ALN_OK;
+  TRY(ensure_ctx());
+  auto s = find_var(table ? table : "");
+  if (!s || !col) return fail(ALN_E_ARG, "alenka_table_add_int: unknown table");
+  DBufP d = dalloc(8 * (s->nrows ? s->nrows : 1));
+  if (!d) return fail(ALN_E_NOMEM, "device allocation failed");
+  if (s->nrows && hipMemcpy(d->p, vals, 8 * s->nrows, device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice) != hipSuccess)
+    return fail(ALN_E_HIP, "copy of column %s failed", col);
+  Col c;
+  if (compress)
+    TRY(make_packed((const int64_t *)d->p, s->nrows, s->seg_rows, decimals, &c));
+  else
+    TRY(make_plain(d, s->nrows, s->seg_rows, decimals, &c));
+  if (!s->cols.count(col)) s->names.push_back(col);
+  s->cols[col] = c;
+  return ALN_OK;
+}
+
+int alenka_table_add_int(const char *table, const char *col, const int64_t *h_vals, int32_t decimals,
+                         int compress) {
+  return add_int(table, col, h_vals, false, decimals, compress);
+}
+int alenka_table_add_int_device(const char *table, const char *col, const int64_t *d_vals,
+                                int32_t decimals, int compress) {
+  return add_int(table, col, d_vals, true, decimals, compress);
+}
+
+int alenka_table_add_char(const char *table, const char *col, const char *h_chars, uint32_t width) {
+  g_status = ALN_OK;
+  TRY(ensure_ctx());
+  auto s = find_var(table ? table : "");
+  if (!s || !col || !width) return fail(ALN_E_ARG, "alenka_table_add_char: bad arguments");
+  // dictionary in first-seen order (compress_char, cm.cu:1808-1821)
+  auto dict = std::make_shared<std::vector<std::string>>();
+  std::map<std::string, int64_t> code_of;
+  std::vector<int64_t> codes(s->nrows);
+  for (uint64_t i = 0; i < s->nrows; ++i) {
+    const char *p = h_chars + i * width;
+    std::string v(p, strnlen(p, width));
+    auto it = code_of.find(v);
+    if (it == code_of.end()) {
+      it = code_of.emplace(v, (int64_t)dict->size()).first;
+      dict->push_back(v);
+    }
+    codes[i] = it->second;
+  }
+  TRY(add_int(table, col, codes.data(), false, 0, 1));
+  s->cols[col].type = 2;
+  s->cols[col].dict = dict;
+  s->cols[col].width = width;
+  return ALN_OK;
+}
+
+int alenka_rows(const char *var, uint64_t *h_rows) {
+  auto s = find_var(var ? var : "");
+  if (!s || !h_rows) return fail(ALN_E_ARG, "alenka_rows: unknown variable");
+  if (s->filtered) {
+    TRY(run_filter(*s));
+    *h_rows = s->nsel;
+  } else {
+    *h_rows = s->nrows;
+  }
+  return ALN_OK;
+}
+
+int alenka_column_count(const char *var, int *h_n) {
+  auto s = find_var(var ? var : "");
+  if (!s || !h_n) return fail(ALN_E_ARG, "alenka_column_count: unknown variable");
+  auto base = s->filtered ? find_var(s->source) : s;
+  *h_n = (int)base->names.size();
+  return ALN_OK;
+}
+
+int alenka_column_info(const char *var, int i, char *buf, int len, int *h_type, int *h_decimals) {
+  auto s = find_var(var ? var : "");
+  if (!s) return fail(ALN_E_ARG, "alenka_column_info: unknown variable");
+  auto base = s->filtered ? find_var(s->source) : s;
+  if (i < 0 || i >= (int)base->names.size()) return fail(ALN_E_ARG, "alenka_column_info: index");
+  const std::string &n = base->names[i];
+  if (buf && len > 0) snprintf(buf, (size_t)len, "%s", n.c_str());
+  if (h_type) *h_type = base->cols[n].type;
+  if (h_decimals) *h_decimals = base->cols[n].decimals;
+  return ALN_OK;
+}
+
+int alenka_fetch_int(const char *var, const char *col, int64_t *h_out, uint64_t cap) {
+  g_status = ALN_OK;
+  View v;
+  TRY(make_view(var ? var : "", {col ? col : ""}, &v));
+  if (v.nrows > cap) return fail(ALN_E_CAPACITY, "alenka_fetch_int: %llu rows", (unsigned long long)v.nrows);
+  std::vector<int64_t> h;
+  TRY(fetch_col(v.cols[col], v.nrows, v.seg_rows, &h));
+  if (!h.empty()) memcpy(h_out, h.data(), 8 * h.size());
+  return ALN_OK;
+}
+
+int alenka_fetch_char(const char *var, const char *col, char *h_out, uint64_t cap_rows, uint32_t width) {
+  g_status = ALN_OK;
+  View v;
+  TRY(make_view(var ? var : "", {col ? col : ""}, &v));
+  const Col &c = v.cols[col];
+  if (!c.dict) return fail(ALN_E_ARG, "alenka_fetch_char: %s is not a char column", col);
+  if (v.nrows > cap_rows) return fail(ALN_E_CAPACITY, "alenka_fetch_char: %llu rows", (unsigned long long)v.nrows);
+  std::vector<int64_t> h;
+  TRY(fetch_col(c, v.nrows, v.seg_rows, &h));
+  memset(h_out, 0, (size_t)v.nrows * width);
+  for (uint64_t i = 0; i < v.nrows; ++i) {
+    const std::string &sv = (*c.dict)[(size_t)h[i]];
+    memcpy(h_out + i * width, sv.data(), std::min<size_t>(sv.size(), width));
+  }
+  return ALN_OK;
+}
+
+int alenka_drop(const char *var) {
+  varNames.erase(var ? var : "");
+  return ALN_OK;
+}
+
+// ---- RPN queue emitters ---------------------------------------------------------------------
+static void push(const char *type, const std::string &value = "", int64_t num = 0, uint32_t prec = 0) {
+  Tok t;
+  t.type = type;
+  t.value = value;
+  t.num = num;
+  t.prec = prec;
+  g_tok.push_back(t);
+}
+
+void emit_name(const char *name) { push("NAME", name); }
+void emit_number(int64_t val) { push("NUMBER", "", val, 0); }
+// emit_decimal (operators.cu:120-134): digits without the point, precision = digits after it
+void emit_decimal(const char *str) {
+  std::string s(str);
+  uint32_t precision = 0;
+  auto pos = s.find('.');
+  if (pos != std::string::npos) {
+    precision = (uint32_t)(s.length() - pos - 1);
+    s.erase(pos, 1);
+  }
+  push("NUMBER", "", (int64_t)std::stoll(s), precision);
+}
+// emit_string (operators.cu:82-94): surrounding double quotes are removed
+void emit_string(const char *str) {
+  std::string s(str);
+  if (!s.empty() && s[0] == '"') s = s.substr(1, s.size() >= 2 ? s.size() - 2 : 0);
+  push("STRING", s);
+}
+void emit_cmp(int val) { push("CMP", "", val); }
+void emit_and(void) { push("AND"); }
+void emit_or(void) { push("OR"); }
+void emit_add(void) { push("ADD"); }
+void emit_minus(void) { push("MINUS"); }
+void emit_mul(void) { push("MUL"); }
+void emit_div(void) { push("DIV"); }
+void emit_eq(void) { push("JOIN"); }
+void emit_count(void) { push("COUNT"); }
+void emit_sum(void) { push("SUM"); }
+void emit_average(void) { push("AVG"); }
+void emit_min(void) { push("MIN"); }
+void emit_max(void) { push("MAX"); }
+void emit_sel_name(const char *name) { push("SEL", name); }
+void emit_var_asc(const char *name) { push("ORDER", name, 0); }
+void emit_var_desc(const char *name) { push("ORDER", name, 1); }
+void emit_join_tab(const char *name, char tp) {
+  if (tp != 'I') g_join_bad = true;
+  g_join.push_back(name);
+}
+void emit_limit(int val) {
+  g_limit = val;
+  g_has_limit = true;
+}
+void clean_queues(void) { finish_statement(); }
+// process_error (operators.cu:2752-2765): recorded; nothing exits
+void process_error(int severity, const char *err) {
+  fail(severity > 1 ? ALN_E_ARG : ALN_OK, "%s", err);
+}
+
+// ---- emit_filter (operators.cu:2306-2408) ---------------------------------------------------
+void emit_filter(char *s, char *f) {
+  g_status = ALN_OK;
+  auto a = find_var(f ? f : "");
+  if (!a) {
+    fail(ALN_E_ARG, "Filter : couldn't find variable %s", f ? f : "");
+    finish_statement();
+    return;
+  }
+  auto b = std::make_shared<Set>();
+  b->filtered = true;
+  b->pred = g_tok;
+  if (a->filtered) {  // AND-chain with the source filter (operators.cu:2359-2392)
+    b->source = a->source;
+    b->pred.insert(b->pred.end(), a->pred.begin(), a->pred.end());
+    Tok t;
+    t.type = "AND";
+    b->pred.push_back(t);
+  } else {
+    b->source = f;
+  }
+  varNames[s] = b;
+  finish_statement();
+}
+
+// ---- emit_select (operators.cu:1801-2084, select.cu:359-952, merge.cu) -----------------------
+static int do_select(const char *s, const char *f, int grp_cnt) {
+  std::vector<SelItem> items;
+  std::vector<Tok> rest;
+  parse_select(g_tok, &items, &rest);
+  if (items.empty()) return fail(ALN_E_ARG, "Select : no output columns");
+  std::vector<std::string> groups;
+  for (auto &k : rest)
+    if (k.type == "NAME") groups.push_back(k.value);
+  if ((int)groups.size() < grp_cnt) return fail(ALN_E_ARG, "Select : GROUP BY list");
+  groups.erase(groups.begin(), groups.end() - grp_cnt);
+  std::set<std::string> need(groups.begin(), groups.end());
+  for (auto &it : items)
+    for (auto &n : names_in(it.expr)) need.insert(n);
+  View v;
+  TRY(make_view(f, need, &v));
+  auto out = std::make_shared<Set>();
+  out->not_compressed = true;
+  const bool any_agg = std::any_of(items.begin(), items.end(), [](const SelItem &i) { return !i.agg.empty(); });
+
+  if (grp_cnt == 0 && !any_agg) {  // projection
+    for (auto &it : items) {
+      DBufP vals;
+      int prec = 0;
+      TRY(eval_expr(v, it.expr, &vals, &prec));
+      Col c;
+      TRY(make_plain(vals, v.nrows, (uint32_t)(v.nrows ? v.nrows : 1), prec, &c));
+      if (it.expr.size() == 1 && it.expr[0].type == "NAME") {
+        const Col &srcc = v.cols[it.expr[0].value];
+        c.type = srcc.type;
+        c.dict = srcc.dict;
+        c.width = srcc.width;
+        c.decimals = srcc.decimals;
+      }
+      out->names.push_back(it.alias);
+      out->cols[it.alias] = c;
+    }
+    out->nrows = v.nrows;
+    out->seg_rows = (uint32_t)(v.nrows ? v.nrows : 1);
+    varNames[s] = out;
+    return ALN_OK;
+  }
+
+  // keys: the GROUP BY columns (a constant key when there is none: one output row)
+  std::vector<DBufP> keybufs;
+  std::vector<const int64_t *> keys;
+  std::vector<int> keydec;
+  if (grp_cnt == 0) {
+    DBufP z = dalloc(8 * (v.nrows ? v.nrows : 1));
+    if (!z) return fail(ALN_E_NOMEM, "device allocation failed");
+    if (v.nrows && hipMemset(z->p, 0, 8 * v.nrows) != hipSuccess) return fail(ALN_E_HIP, "memset failed");
+    keybufs.push_back(z);
+    keys.push_back((const int64_t *)z->p);
+    keydec.push_back(0);
+  } else {
+    if (grp_cnt > ALN_MAX_KEYS) return fail(ALN_E_UNSUPPORTED, "GROUP BY on more than %d columns", ALN_MAX_KEYS);
+    for (auto &g : groups) {
+      DBufP d;
+      TRY(plain_copy(v.cols[g], v.nrows, v.seg_rows, &d));
+      keybufs.push_back(d);
+      keys.push_back((const int64_t *)d->p);
+      keydec.push_back(v.cols[g].decimals);
+    }
+  }
+  // aggregate inputs
+  struct Out {
+    int kind;      // 0 key, 1 aggregate, 2 AVG (sum / count)
+    int key = -1;  // key index
+    int agg = -1, agg2 = -1;
+    int prec = 0;
+  };
+  std::vector<Out> outs;
+  std::vector<DBufP> valbufs;
+  std::vector<const int64_t *> vals;
+  std::vector<int32_t> aggs;
+  for (auto &it : items) {
+    Out o;
+    if (it.agg.empty()) {
+      if (it.expr.size() != 1 || it.expr[0].type != "NAME")
+        return fail(ALN_E_UNSUPPORTED, "Select : non-aggregate expression %s in a GROUP BY", it.alias.c_str());
+      auto g = std::find(groups.begin(), groups.end(), it.expr[0].value);
+      if (g == groups.end())
+        return fail(ALN_E_UNSUPPORTED, "Select : %s is neither grouped nor aggregated", it.expr[0].value.c_str());
+      o.kind = 0;
+      o.key = (int)(g - groups.begin());
+      o.prec = keydec[o.key];
+    } else {
+      DBufP vb;
+      int prec = 0;
+      TRY(eval_expr(v, it.expr, &vb, &prec));
+      valbufs.push_back(vb);
+      o.prec = it.agg == "COUNT" ? 0 : prec;
+      int code = it.agg == "COUNT" ? ALN_AGG_COUNT : it.agg == "MIN" ? ALN_AGG_MIN : it.agg == "MAX" ? ALN_AGG_MAX : ALN_AGG_SUM;
+      o.kind = it.agg == "AVG" ? 2 : 1;
+      o.agg = (int)aggs.size();
+      vals.push_back((const int64_t *)vb->p);
+      aggs.push_back(code);
+      if (o.kind == 2) {
+        o.agg2 = (int)aggs.size();
+        vals.push_back((const int64_t *)vb->p);
+        aggs.push_back(ALN_AGG_COUNT);
+      }
+    }
+    outs.push_back(o);
+  }
+  if (aggs.size() > ALN_MAX_AGGS) return fail(ALN_E_UNSUPPORTED, "more than %d aggregates", ALN_MAX_AGGS);
+  const uint64_t cap = v.nrows ? v.nrows : 1;
+  std::vector<DBufP> okb, ovb;
+  std::vector<int64_t *> ok, ov;
+  for (size_t i = 0; i < keys.size(); ++i) {
+    okb.push_back(dalloc(8 * cap));
+    ok.push_back((int64_t *)okb.back()->p);
+  }
+  for (size_t i = 0; i < aggs.size(); ++i) {
+    ovb.push_back(dalloc(8 * cap));
+    ov.push_back((int64_t *)ovb.back()->p);
+  }
+  // one source segment: key order (order_inplace + GroupBy); several: the hash-ordered
+  // merge of add()/count_avg() (merge.cu:92-496)
+  const int order = (grp_cnt > 0 && v.cycles > 1) ? ALN_ORDER_HASH : ALN_ORDER_KEYS;
+  uint64_t ng = 0;
+  if (v.nrows) {
+    ALN(aln_groupby(g_ctx, keys.data(), (int)keys.size(), vals.data(), aggs.data(), (int)aggs.size(),
+                    v.nrows, order, g_hash_seed, ok.data(), ov.data(), cap, &ng),
+        "group by");
+  }
+  if (grp_cnt == 0 && ng == 0) {
+    // aggregates over no rows: the reference returns an empty set (emit_select, no c_set)
+    varNames[s] = out;
+    return ALN_OK;
+  }
+  for (size_t i = 0; i < items.size(); ++i) {
+    const Out &o = outs[i];
+    Col c;
+    if (o.kind == 0) {
+      TRY(make_plain(okb[o.key], ng, (uint32_t)(ng ? ng : 1), o.prec, &c));
+      const Col &srcc = v.cols[groups[o.key]];
+      c.type = srcc.type;
+      c.dict = srcc.dict;
+      c.width = srcc.width;
+    } else if (o.kind == 1) {
+      TRY(make_plain(ovb[o.agg], ng, (uint32_t)(ng ? ng : 1), o.prec, &c));
+    } else {
+      // count_avg (merge.cu:446-460): scale 2 or less gains two digits, (sum*100)/count
+      std::vector<int64_t> sum(ng), cnt(ng);
+      if (hipMemcpy(sum.data(), ovb[o.agg]->p, 8 * ng, hipMemcpyDeviceToHost) != hipSuccess ||
+          hipMemcpy(cnt.data(), ovb[o.agg2]->p, 8 * ng, hipMemcpyDeviceToHost) != hipSuccess)
+        return fail(ALN_E_HIP, "device to host copy failed");
+      int prec = o.prec;
+      for (uint64_t g = 0; g < ng; ++g) sum[g] = prec <= 2 ? (sum[g] * 100) / cnt[g] : sum[g] / cnt[g];
+      if (prec <= 2) prec += 2;
+      DBufP d = dalloc(8 * (ng ? ng : 1));
+      if (ng && hipMemcpy(d->p, sum.data(), 8 * ng, hipMemcpyHostToDevice) != hipSuccess)
+        return fail(ALN_E_HIP, "host to device copy failed");
+      TRY(make_plain(d, ng, (uint32_t)(ng ? ng : 1), prec, &c));
+    }
+    out->names.push_back(items[i].alias);
+    out->cols[items[i].alias] = c;
+  }
+  out->nrows = ng;
+  out->seg_rows = (uint32_t)(ng ? ng : 1);
+  varNames[s] = out;
+  return ALN_OK;
+}
+
+void emit_select(const char *s, const char *f, int grp_cnt) {
+  g_status = ALN_OK;
+  if (!find_var(f ? f : ""))
+    fail(ALN_E_ARG, "Couldn't find(1) %s", f ? f : "");
+  else
+    do_select(s, f, grp_cnt);
+  finish_statement();
+}
+
+// ---- emit_join (operators.cu:797-928, emit_multijoin 940-1582) -------------------------------
+struct Cond {
+  std::vector<std::pair<std::string, std::string>> eqs;  // a = b, in order
+};
+
+// Splits the condition tokens (NAME NAME JOIN, combined with AND) into one condition per
+// join, in parse order.
+static int parse_conditions(const std::vector<Tok> &t, std::vector<Cond> *out) {
+  std::vector<Cond> stack;
+  std::vector<std::string> names;
+  for (auto &k : t) {
+    if (k.type == "NAME") {
+      names.push_back(k.value);
+    } else if (k.type == "JOIN") {
+      if (names.size() < 2) return fail(ALN_E_ARG, "Join : bad condition");
+      Cond c;
+      std::string b = names.back();
+      names.pop_back();
+      std::string a = names.back();
+      names.pop_back();
+      c.eqs.push_back({a, b});
+      stack.push_back(c);
+    } else if (k.type == "AND") {
+      if (stack.size() < 2) return fail(ALN_E_ARG, "Join : bad condition");
+      Cond c2 = stack.back();
+      stack.pop_back();
+      stack.back().eqs.insert(stack.back().eqs.end(), c2.eqs.begin(), c2.eqs.end());
+    } else {
+      return fail(ALN_E_UNSUPPORTED, "Join : operation %s in a join condition", k.type.c_str());
+    }
+  }
+  *out = stack;
+  return ALN_OK;
+}
+
+static bool has_col(const std::string &var, const std::string &col) {
+  auto s = find_var(var);
+  if (!s) return false;
+  auto base = s->filtered ? find_var(s->source) : s;
+  return base && base->cols.count(col);
+}
+
+static std::vector<std::string> all_cols(const std::string &var) {
+  auto s = find_var(var);
+  auto base = s->filtered ? find_var(s->source) : s;
+  return base->names;
+}
+
+// left JOIN right ON cond -> result `name` with the columns of both sides in `keep`.
+static int join_pair(const std::string &name, const std::string &left, const std::string &right,
+                     const Cond &cond, const std::set<std::string> &keep) {
+  // orient each equality: the left column belongs to the left variable
+  std::vector<std::pair<std::string, std::string>> eqs;
+  for (auto &e : cond.eqs) {
+    if (has_col(left, e.first) && has_col(right, e.second))
+      eqs.push_back(e);
+    else if (has_col(left, e.second) && has_col(right, e.first))
+      eqs.push_back({e.second, e.first});
+    else
+      return fail(ALN_E_ARG, "Join : couldn't find %s / %s", e.first.c_str(), e.second.c_str());
+  }
+  std::set<std::string> lneed, rneed;
+  for (auto &c : all_cols(left))
+    if (keep.count(c)) lneed.insert(c);
+  for (auto &c : all_cols(right))
+    if (keep.count(c) && !lneed.count(c)) rneed.insert(c);
+  for (auto &e : eqs) {
+    lneed.insert(e.first);
+    rneed.insert(e.second);
+  }
+  View lv, rv;
+  TRY(make_view(left, lneed, &lv));
+  TRY(make_view(right, rneed, &rv));
+  for (auto &e : eqs) {
+    if (lv.cols[e.first].type == 2 || rv.cols[e.second].type == 2)
+      return fail(ALN_E_UNSUPPORTED, "Join : string join keys (operators.cu:1030-1033)");
+  }
+  DBufP lk, rk;
+  TRY(plain_copy(lv.cols[eqs[0].first], lv.nrows, lv.seg_rows, &lk));
+  TRY(plain_copy(rv.cols[eqs[0].second], rv.nrows, rv.seg_rows, &rk));
+  uint64_t cap = lv.nrows ? lv.nrows : 1, np = 0;
+  DBufP pi, bi;
+  for (;;) {
+    pi = dalloc(4 * cap);
+    bi = dalloc(4 * cap);
+    if (!pi || !bi) return fail(ALN_E_NOMEM, "device allocation failed");
+    int rc = aln_hash_join(g_ctx, (const int64_t *)rk->p, rv.nrows, (const int64_t *)lk->p, lv.nrows,
+                           (uint32_t *)pi->p, (uint32_t *)bi->p, cap, &np);
+    if (rc == ALN_E_CAPACITY && np > cap) {
+      cap = np;
+      continue;
+    }
+    ALN(rc, "join");
+    break;
+  }
+  // gather every kept column through the pairs
+  auto gather_all = [&](const View &v, const std::set<std::string> &cols, const DBufP &idx, uint64_t n,
+                        std::map<std::string, Col> *dst) -> int {
+    for (auto &cn : cols) {
+      const Col &sc = v.cols.at(cn);
+      DBufP vals = dalloc(8 * (n ? n : 1));
+      if (!vals) return fail(ALN_E_NOMEM, "device allocation failed");
+      aln_col d = sc.desc();
+      if (n) ALN(aln_gather(g_ctx, &d, v.seg_rows, (const uint32_t *)idx->p, n, (int64_t *)vals->p), "join gather");
+      Col c;
+      TRY(make_plain(vals, n, (uint32_t)(n ? n : 1), sc.decimals, &c));
+      c.type = sc.type;
+      c.dict = sc.dict;
+      c.width = sc.width;
+      (*dst)[cn] = c;
+    }
+    return ALN_OK;
+  };
+  std::map<std::string, Col> lc, rc;
+  std::set<std::string> lall = lneed, rall = rneed;
+  TRY(gather_all(lv, lall, pi, np, &lc));
+  TRY(gather_all(rv, rall, bi, np, &rc));
+  // pair order of the reference: per left segment, ascending left key, then left row, then
+  // right row (operators.cu:1184-1240).  Pairs arrive in left-row order; sort them stably
+  // by (left segment, left key).
+  uint64_t n = np;
+  std::vector<uint32_t> perm;
+  if (n > 1) {
+    // left source row of each pair, and its segment
+    std::vector<uint32_t> hp(n);
+    if (hipMemcpy(hp.data(), pi->p, 4 * n, hipMemcpyDeviceToHost) != hipSuccess)
+      return fail(ALN_E_HIP, "device to host copy failed");
+    std::vector<int64_t> lrow(lv.nrows);
+    if (lv.rowids) {
+      if (lv.nrows && hipMemcpy(lrow.data(), lv.rowids->p, 8 * lv.nrows, hipMemcpyDeviceToHost) != hipSuccess)
+        return fail(ALN_E_HIP, "device to host copy failed");
+    } else {
+      for (uint64_t i = 0; i < lv.nrows; ++i) lrow[i] = (int64_t)i;
+    }
+    auto lsrc = find_var(left);
+    auto lbase = lsrc->filtered ? find_var(lsrc->source) : lsrc;
+    const uint64_t lseg_rows = lbase->not_compressed ? (lbase->nrows ? lbase->nrows : 1) : lbase->seg_rows;
+    std::vector<int64_t> seg(n);
+    for (uint64_t i = 0; i < n; ++i) seg[i] = lrow[hp[i]] / (int64_t)lseg_rows;
+    DBufP dseg = dalloc(8 * n);
+    if (hipMemcpy(dseg->p, seg.data(), 8 * n, hipMemcpyHostToDevice) != hipSuccess)
+      return fail(ALN_E_HIP, "host to device copy failed");
+    DBufP key;
+    TRY(plain_copy(lc[eqs[0].first], n, (uint32_t)n, &key));
+    const int64_t *ks[2] = {(const int64_t *)dseg->p, (const int64_t *)key->p};
+    int32_t desc[2] = {0, 0};
+    DBufP dperm = dalloc(4 * n);
+    ALN(aln_sort_keys(g_ctx, ks, desc, 2, n, (uint32_t *)dperm->p), "join order");
+    auto permute = [&](std::map<std::string, Col> &m) -> int {
+      for (auto &kv : m) {
+        DBufP src;
+        TRY(plain_copy(kv.second, n, (uint32_t)n, &src));
+        DBufP dst = dalloc(8 * n);
+        ALN(aln_apply_perm(g_ctx, (const int64_t *)src->p, (const uint32_t *)dperm->p, n, (int64_t *)dst->p),
+            "join order");
+        Col c;
+        TRY(make_plain(dst, n, (uint32_t)n, kv.second.decimals, &c));
+        c.type = kv.second.type;
+        c.dict = kv.second.dict;
+        c.width = kv.second.width;
+        kv.second = c;
+      }
+      return ALN_OK;
+    };
+    TRY(permute(lc));
+    TRY(permute(rc));
+  }
+  // extra equalities (AND a = b, operators.cu:1288-1348) keep the pairs where they hold
+  auto out = std::make_shared<Set>();
+  out->not_compressed = true;
+  out->nrows = n;
+  out->seg_rows = (uint32_t)(n ? n : 1);
+  for (auto &kv : lc) {
+    out->names.push_back(kv.first);
+    out->cols[kv.first] = kv.second;
+  }
+  for (auto &kv : rc)
+    if (!out->cols.count(kv.first)) {
+      out->names.push_back(kv.first);
+      out->cols[kv.first] = kv.second;
+    }
+  if (eqs.size() > 1 && n) {
+    std::vector<aln_col> cols;
+    std::vector<aln_rpn> rpn;
+    std::vector<int32_t> dec;
+    for (size_t e = 1; e < eqs.size(); ++e) {
+      int a = (int)cols.size();
+      cols.push_back(out->cols[eqs[e].first].desc());
+      cols.push_back(out->cols[eqs[e].second].desc());
+      dec.push_back(0);
+      dec.push_back(0);
+      rpn.push_back(aln_rpn{ALN_RPN_NAME, a, 0});
+      rpn.push_back(aln_rpn{ALN_RPN_NAME, a + 1, 0});
+      rpn.push_back(aln_rpn{ALN_RPN_CMP, 4, 0});
+      if (e > 1) rpn.push_back(aln_rpn{ALN_RPN_AND, 0, 0});
+    }
+    aln_insn prog[ALN_MAX_INSN];
+    int nprog = 0;
+    ALN(aln_compile_rpn(rpn.data(), (int)rpn.size(), dec.data(), (int)dec.size(), 0, prog, &nprog, nullptr),
+        "join condition");
+    DBufP sel = dalloc(4 * n);
+    uint64_t k = 0;
+    ALN(aln_scan_filter(g_ctx, cols.data(), (int)cols.size(), out->seg_rows, prog, nprog, nullptr,
+                        (uint32_t *)sel->p, n, &k),
+        "join condition");
+    for (auto &kv : out->cols) {
+      DBufP vals = dalloc(8 * (k ? k : 1));
+      aln_col d = kv.second.desc();
+      if (k) ALN(aln_gather(g_ctx, &d, out->seg_rows, (const uint32_t *)sel->p, k, (int64_t *)vals->p), "gather");
+      Col c;
+      TRY(make_plain(vals, k, (uint32_t)(k ? k : 1), kv.second.decimals, &c));
+      c.type = kv.second.type;
+      c.dict = kv.second.dict;
+      c.width = kv.second.width;
+      kv.second = c;
+    }
+    out->nrows = k;
+    out->seg_rows = (uint32_t)(k ? k : 1);
+  }
+  varNames[name] = out;
+  return ALN_OK;
+}
+
+static int do_join(const char *s, const char *j1) {
+  std::vector<SelItem> items;
+  std::vector<Tok> rest;
+  parse_select(g_tok, &items, &rest);
+  std::vector<Cond> conds;
+  TRY(parse_conditions(rest, &conds));
+  const size_t nt = g_join.size();
+  if (nt == 0 || conds.size() != nt) return fail(ALN_E_ARG, "Join : %zu tables, %zu conditions", nt, conds.size());
+  for (auto &it : items)
+    if (!it.agg.empty() || it.expr.size() != 1 || it.expr[0].type != "NAME")
+      return fail(ALN_E_UNSUPPORTED, "Join : select list entries must be columns");
+  std::set<std::string> keep;
+  for (auto &it : items) keep.insert(it.expr[0].value);
+  for (auto &c : conds)
+    for (auto &e : c.eqs) {
+      keep.insert(e.first);
+      keep.insert(e.second);
+    }
+  // the chain of emit_join (operators.cu:832-860): op_join holds the tables innermost
+  // first; the i-th join (1-based) uses the condition parsed (n+1-i)-th
+  std::string prev;
+  for (size_t i = 1; i <= nt; ++i) {
+    std::string name = i == nt ? std::string(s) + "\x01" : std::string(s) + std::to_string(i);
+    std::string left, right;
+    if (i == 1) {
+      right = g_join[0];
+      left = nt > 1 ? g_join[1] : std::string(j1);
+    } else {
+      left = i < nt ? g_join[i] : std::string(j1);
+      right = prev;
+    }
+    TRY(join_pair(name, left, right, conds[nt - i], keep));
+    if (!prev.empty()) varNames.erase(prev);
+    prev = name;
+  }
+  // the result keeps the selected columns under their aliases
+  auto r = find_var(prev);
+  auto out = std::make_shared<Set>();
+  out->not_compressed = true;
+  out->nrows = r->nrows;
+  out->seg_rows = r->seg_rows;
+  for (auto &it : items) {
+    const std::string &c = it.expr[0].value;
+    if (!r->cols.count(c)) return fail(ALN_E_ARG, "Join : couldn't find column %s", c.c_str());
+    out->names.push_back(it.alias);
+    out->cols[it.alias] = r->cols[c];
+  }
+  varNames.erase(prev);
+  varNames[s] = out;
+  return ALN_OK;
+}
+
+void emit_join(const char *s, const char *j1, int grp, int start_seg, int end_seg) {
+  (void)grp;
+  (void)start_seg;
+  (void)end_seg;
+  g_status = ALN_OK;
+  if (g_join_bad) {
+    fail(ALN_E_UNSUPPORTED, "Join : only inner joins are supported");
+  } else {
+    if (!find_var(j1 ? j1 : ""))
+      fail(ALN_E_ARG, "Join : couldn't find variable %s", j1 ? j1 : "");
+    else
+      do_join(s, j1);
+  }
+  finish_statement();
+}
+
+// ---- emit_order (operators.cu:1641-1798, sorts.cu:17-125) ----------------------------------
+static int do_order(const char *s, const char *f) {
+  std::vector<std::string> keys;
+  std::vector<int32_t> desc;
+  for (auto &k : g_tok) {
+    if (k.type == "NAME") {
+      keys.push_back(k.value);
+      desc.push_back(0);
+    } else if (k.type == "ORDER") {
+      keys.push_back(k.value);
+      desc.push_back((int32_t)k.num);
+    }
+  }
+  if (keys.empty()) return fail(ALN_E_ARG, "Order : no keys");
+  if (keys.size() > ALN_MAX_KEYS) return fail(ALN_E_UNSUPPORTED, "ORDER BY on more than %d keys", ALN_MAX_KEYS);
+  auto names = all_cols(f);
+  std::set<std::string> need(names.begin(), names.end());
+  for (auto &k : keys)
+    if (!need.count(k)) return fail(ALN_E_ARG, "Couldn't find name %s", k.c_str());
+  View v;
+  TRY(make_view(f, need, &v));
+  const uint64_t n = v.nrows;
+  std::vector<DBufP> kb;
+  std::vector<const int64_t *> kp;
+  for (auto &k : keys) {
+    DBufP d;
+    TRY(plain_copy(v.cols[k], n, v.seg_rows, &d));
+    kb.push_back(d);
+    kp.push_back((const int64_t *)d->p);
+  }
+  DBufP perm = dalloc(4 * (n ? n : 1));
+  if (n) ALN(aln_sort_keys(g_ctx, kp.data(), desc.data(), (int)keys.size(), n, (uint32_t *)perm->p), "order");
+  auto out = std::make_shared<Set>();
+  out->not_compressed = true;
+  out->nrows = n;
+  out->seg_rows = (uint32_t)(n ? n : 1);
+  for (auto &cn : names) {
+    DBufP src;
+    TRY(plain_copy(v.cols[cn], n, v.seg_rows, &src));
+    DBufP dst = dalloc(8 * (n ? n : 1));
+    if (n) ALN(aln_apply_perm(g_ctx, (const int64_t *)src->p, (const uint32_t *)perm->p, n, (int64_t *)dst->p), "order");
+    Col c;
+    TRY(make_plain(dst, n, out->seg_rows, v.cols[cn].decimals, &c));
+    c.type = v.cols[cn].type;
+    c.dict = v.cols[cn].dict;
+    c.width = v.cols[cn].width;
+    out->names.push_back(cn);
+    out->cols[cn] = c;
+  }
+  varNames[s] = out;
+  return ALN_OK;
+}
+
+void emit_order(const char *s, const char *f, int e, int ll) {
+  (void)e;
+  (void)ll;
+  g_status = ALN_OK;
+  if (!find_var(f ? f : ""))
+    fail(ALN_E_ARG, "Order : couldn't find variable %s", f ? f : "");
+  else
+    do_order(s, f);
+  finish_statement();
+}
+
+// ---- emit_store / emit_display: CudaSet::Store text output (cm.cu:1479-1762) ----------------
+static int store_text(const char *var, FILE *fp, const char *sep, int limit) {
+  auto names = all_cols(var);
+  View v;
+  TRY(make_view(var, std::set<std::string>(names.begin(), names.end()), &v));
+  uint64_t m = (limit != 0 && (uint64_t)limit < v.nrows) ? (uint64_t)limit : v.nrows;
+  std::vector<std::vector<int64_t>> data(names.size());
+  for (size_t j = 0; j < names.size(); ++j) {
+    TRY(fetch_col(v.cols[names[j]], v.nrows, v.seg_rows, &data[j]));
+  }
+  for (uint64_t i = 0; i < m; ++i) {
+    for (size_t j = 0; j < names.size(); ++j) {
+      const Col &c = v.cols[names[j]];
+      int64_t x = data[j][i];
+      if (c.dict) {
+        const std::string &sv = (*c.dict)[(size_t)x];
+        fprintf(fp, "%.*s", (int)c.width, sv.c_str());
+      } else if (c.decimals) {
+        std::string str = std::to_string(x);
+        while (str.length() <= (size_t)c.decimals) str = '0' + str;
+        str.insert(str.length() - c.decimals, ".");
+        fprintf(fp, "%s", str.c_str());
+      } else {
+        fprintf(fp, "%lld", (long long)x);
+      }
+      fputs(sep, fp);
+    }
+    if (i != m - 1) fputs("\n", fp);
+  }
+  return ALN_OK;
+}
+
+void emit_store(const char *s, const char *f, const char *sep) {
+  g_status = ALN_OK;
+  int limit = g_has_limit ? g_limit : 0;
+  if (!find_var(s ? s : "")) {
+    fail(ALN_E_ARG, "Store : couldn't find variable %s", s ? s : "");
+  } else {
+    FILE *fp = fopen(f, "w");
+    if (!fp) {
+      fail(ALN_E_ARG, "Could not open file %s", f);
+    } else {
+      store_text(s, fp, sep, limit);
+      fclose(fp);
+    }
+  }
+  finish_statement();
+}
+
+void emit_display(const char *s, const char *sep) {
+  g_status = ALN_OK;
+  int limit = g_has_limit ? g_limit : 0;
+  if (!find_var(s ? s : ""))
+    fail(ALN_E_ARG, "Display : couldn't find variable %s", s ? s : "");
+  else
+    store_text(s, stdout, sep, limit);
+  fflush(stdout);
+  finish_statement();
+}
+
+}  // extern "C"

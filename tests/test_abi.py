@@ -175,3 +175,19 @@ def test_host_murmur_matches_oracle():
     for n in range(0, 40):
         b = r.integers(0, 256, n, dtype=np.uint8).tobytes()
         assert L.murmur64a(b, 100) == O.murmur64a(b, 100)
+
+
+def test_operator_layer_exports_every_declared_symbol():
+    """include/alenka_ops.h: the reference's emit_* entry points, with C linkage."""
+    from alenka_amd import ops
+
+    text = open(os.path.join(ROOT, "include", "alenka_ops.h")).read()
+    names = sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*((?:alenka|emit|clean|process)_\w+)\s*\(",
+                                  text, re.M)))
+    lib = L.load()
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(names) == set(ops._SIG), set(names) ^ set(ops._SIG)
+    for n in ("emit_filter", "emit_select", "emit_join", "emit_order", "emit_store", "emit_name",
+              "emit_number", "emit_cmp", "emit_sum", "emit_count", "emit_sel_name"):
+        assert n in names

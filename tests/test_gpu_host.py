@@ -1,0 +1,148 @@
+"""The reference's scripts replayed through the operator layer (emit_* over the C ABI),
+issuing the calls in the order the reference's parser (bison.y:127-287) issues them, and
+checked against the CPU oracle."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import q3_reference as Q3
+from alenka_amd import _lib as L
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ops(ctx):
+    from alenka_amd.ops import Ops
+
+    o = Ops(0)
+    yield o
+    o.close()
+
+
+def flat_query(ops, seg_rows, n=300000, seed=42):
+    key, date, price, disc = O.gen_flat(seed, n)
+    ops.table("T", n, seg_rows, [("key", key, 0), ("date", date, 0), ("price", price, 2),
+                                 ("discount", disc, 2)])
+    # F := FILTER T BY date < 19950315;
+    ops.emit_name("date"); ops.emit_number(19950315); ops.emit_cmp(1); ops.emit_filter("F", "T")
+    ops.check(None, "filter")
+    # R := SELECT key AS k, SUM(price*(1-discount)) AS rev, COUNT(key) AS cnt FROM F GROUP BY key;
+    ops.emit_name("key"); ops.emit_sel_name("k")
+    ops.emit_name("price"); ops.emit_number(1); ops.emit_name("discount"); ops.emit_minus()
+    ops.emit_mul(); ops.emit_sum(); ops.emit_sel_name("rev")
+    ops.emit_name("key"); ops.emit_count(); ops.emit_sel_name("cnt")
+    ops.emit_name("key")
+    ops.emit_select("R", "F", 1)
+    ops.check(None, "select")
+    return key, date, price, disc
+
+
+@pytest.mark.parametrize("seg_rows", [1000000, 65536])
+def test_flat_filter_groupby(ops, seg_rows):
+    n = 300000
+    key, date, price, disc = flat_query(ops, seg_rows, n)
+    sel = date < 19950315
+    assert ops.rows("F") == int(sel.sum())
+    rev = price[sel] * (100 - disc[sel])
+    nsegs = (n + seg_rows - 1) // seg_rows
+    bounds = [min(n, s * seg_rows) for s in range(nsegs + 1)]
+    # group-by input segments are the source table's (emit_select loops over them)
+    seg_of = np.searchsorted(bounds, np.nonzero(sel)[0], side="right") - 1
+    fb = [int(np.searchsorted(seg_of, s)) for s in range(nsegs)] + [len(seg_of)]
+    wk, wv = O.groupby([key[sel]], [rev, rev], [L.AGG_SUM, L.AGG_COUNT],
+                       seg_bounds=fb if nsegs > 1 else None, seed=100)
+    assert np.array_equal(ops.fetch("R", "k"), wk[0])
+    assert np.array_equal(ops.fetch("R", "rev"), wv[0])
+    assert np.array_equal(ops.fetch("R", "cnt"), wv[1])
+    cols = ops.columns("R")
+    assert [c[0] for c in cols] == ["k", "rev", "cnt"]
+    assert [c[2] for c in cols] == [0, 4, 0]  # decimal scale of SUM(price*(1-discount))
+
+
+def test_filter_chain_and_zone_maps(ops):
+    n = 100000
+    key, date, price, disc = O.gen_flat(5, n)
+    order = np.argsort(date, kind="stable")  # clustered dates: zone maps skip segments
+    key, date, price, disc = key[order], date[order], price[order], disc[order]
+    ops.table("Z", n, 8192, [("key", key, 0), ("date", date, 0), ("price", price, 2),
+                             ("discount", disc, 2)])
+    ops.emit_name("date"); ops.emit_number(19940101); ops.emit_cmp(6); ops.emit_filter("Z1", "Z")
+    ops.emit_name("discount"); ops.emit_decimal("0.05"); ops.emit_cmp(5); ops.emit_filter("Z2", "Z1")
+    ops.check()
+    want = (date >= 19940101) & (disc <= 5)
+    assert ops.rows("Z2") == int(want.sum())
+    assert np.array_equal(ops.fetch("Z2", "key"), key[want])
+
+
+def q3_script(ops, t, seg_rows, out_path):
+    nd, no, nl = len(t["c_key"]), len(t["o_key"]), len(t["l_okey"])
+    ops.table("customer", nd, seg_rows, [("c_custkey", t["c_key"], 0), ("c_mktsegment", t["c_seg_str"], 10)])
+    ops.table("orders", no, seg_rows, [("o_orderkey", t["o_key"], 0), ("o_custkey", t["o_cust"], 0),
+                                       ("o_orderdate", t["o_date"], 0), ("o_shippriority", t["o_prio"], 0)])
+    ops.table("lineitem", nl, seg_rows, [("orderkey", t["l_okey"], 0), ("price", t["l_price"], 2),
+                                         ("discount", t["l_disc"], 2), ("shipdate", t["l_ship"], 0)])
+    # OFI := FILTER orders BY o_orderdate < 19950315;
+    ops.emit_name("o_orderdate"); ops.emit_number(19950315); ops.emit_cmp(1); ops.emit_filter("OFI", "orders")
+    # CF := FILTER customer BY c_mktsegment == "BUILDING";
+    ops.emit_name("c_mktsegment"); ops.emit_string('"BUILDING"'); ops.emit_cmp(4); ops.emit_filter("CF", "customer")
+    # LF := FILTER lineitem BY shipdate > 19950315;
+    ops.emit_name("shipdate"); ops.emit_number(19950315); ops.emit_cmp(2); ops.emit_filter("LF", "lineitem")
+    ops.check(None, "filters")
+    # OLC := SELECT ... FROM LF JOIN OFI ON orderkey = o_orderkey JOIN CF ON o_custkey = c_custkey;
+    for c in ("o_orderkey", "o_orderdate", "o_shippriority", "price", "discount"):
+        ops.emit_name(c); ops.emit_sel_name(c)
+    ops.emit_name("orderkey"); ops.emit_name("o_orderkey"); ops.emit_eq()
+    ops.emit_name("o_custkey"); ops.emit_name("c_custkey"); ops.emit_eq()
+    ops.emit_join_tab("CF", b"I"); ops.emit_join_tab("OFI", b"I")
+    ops.emit_join("OLC", "LF", 1, 0, -1)
+    ops.check(None, "join")
+    # F := SELECT o_orderkey AS o_orderkey1, o_orderdate AS orderdate1, o_shippriority AS
+    #      shippriority1, SUM(price*(1-discount)) AS sum_revenue, COUNT(o_orderkey) AS cnt
+    #      FROM OLC GROUP BY o_orderkey, o_orderdate, o_shippriority;
+    ops.emit_name("o_orderkey"); ops.emit_sel_name("o_orderkey1")
+    ops.emit_name("o_orderdate"); ops.emit_sel_name("orderdate1")
+    ops.emit_name("o_shippriority"); ops.emit_sel_name("shippriority1")
+    ops.emit_name("price"); ops.emit_number(1); ops.emit_name("discount"); ops.emit_minus(); ops.emit_mul()
+    ops.emit_sum(); ops.emit_sel_name("sum_revenue")
+    ops.emit_name("o_orderkey"); ops.emit_count(); ops.emit_sel_name("cnt")
+    ops.emit_name("o_orderkey"); ops.emit_name("o_orderdate"); ops.emit_name("o_shippriority")
+    ops.emit_select("F", "OLC", 3)
+    ops.check(None, "group by")
+    # RES := ORDER F BY sum_revenue DESC, orderdate1 ASC;
+    ops.emit_var_desc("sum_revenue"); ops.emit_var_asc("orderdate1"); ops.emit_order("RES", "F", 2, 0)
+    ops.check(None, "order")
+    # STORE RES INTO 'mytest.txt' USING ('|') LIMIT 10;
+    ops.emit_limit(10); ops.emit_store("RES", out_path, "|")
+    ops.check(None, "store")
+
+
+def test_q3_script(ops, tmp_path):
+    t = Q3.tables(77, 6000, 60000)
+    code_to_str = {int(c): Q3.SEGMENTS[v] for v, c in enumerate(t["code_of"]) if c >= 0}
+    t["c_seg_str"] = [code_to_str[int(c)] for c in t["c_seg"]]
+    top, groups, stats = Q3.run(t)
+    out = str(tmp_path / "q3.txt")
+    q3_script(ops, t, 16384, out)
+    assert ops.rows("OLC") == stats["fact_pass"]
+    got = np.stack([ops.fetch("F", c) for c in ("o_orderkey1", "orderdate1", "shippriority1",
+                                                 "sum_revenue", "cnt")], axis=1)
+    assert np.array_equal(got, groups)
+    res = [tuple(int(ops.fetch("RES", c)[i]) for c in ("o_orderkey1", "orderdate1", "shippriority1",
+                                                       "sum_revenue", "cnt")) for i in range(10)]
+    assert res == top
+    text = open(out).read().split("\n")
+    assert len(text) == 10
+    k, d, p, s, c = top[0]
+    frac = str(s)
+    while len(frac) <= 4:
+        frac = "0" + frac
+    assert text[0] == f"{k}|{d}|{p}|{frac[:-4]}.{frac[-4:]}|{c}|"
+
+
+def test_errors_do_not_exit(ops):
+    ops.emit_name("nothing"); ops.emit_number(1); ops.emit_cmp(1); ops.emit_filter("X", "no_such_table")
+    assert ops.lib.alenka_status() == L.ALN_E_ARG
+    assert b"couldn't find" in ops.lib.alenka_last_error()
