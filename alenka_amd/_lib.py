@@ -78,6 +78,32 @@ class StarStats(C.Structure):
                 ("ms_fact", C.c_double), ("ms_topk", C.c_double)]
 
 
+class BatchQuery(C.Structure):
+    _fields_ = [("pred_col", C.c_int32), ("key_col", C.c_int32), ("x_col", C.c_int32), ("y_col", C.c_int32),
+                ("lo", C.c_int64), ("hi", C.c_int64), ("key_div", C.c_int64),
+                ("a0", C.c_int64), ("a1", C.c_int64), ("b0", C.c_int64), ("b1", C.c_int64)]
+
+
+class BatchTuning(C.Structure):
+    _fields_ = [("hot_groups", C.c_int32), ("blocks_per_cu", C.c_int32), ("use_graph", C.c_int32)]
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("streams", C.c_int32), ("max_batch", C.c_int32), ("plain", C.c_int32), ("reserved", C.c_int32),
+                ("budget_bytes", C.c_uint64), ("tune", BatchTuning)]
+
+
+class ServerResult(C.Structure):
+    _fields_ = [("status", C.c_int32), ("kind", C.c_int32), ("batch_size", C.c_int32), ("reserved", C.c_int32),
+                ("batch", C.c_uint64), ("ngroups", C.c_uint64), ("queue_ms", C.c_double), ("run_ms", C.c_double),
+                ("latency_ms", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [(k, C.c_uint64) for k in ("submitted", "completed", "batches", "batched_queries",
+                                          "max_concurrent", "pending", "mem_in_use")]
+
+
 P = C.c_void_p
 U64, I64, U32, I32 = C.c_uint64, C.c_int64, C.c_uint32, C.c_int32
 PU64, PI64, PU32, PI32 = C.POINTER(C.c_uint64), C.POINTER(C.c_int64), C.POINTER(C.c_uint32), C.POINTER(C.c_int32)
@@ -116,11 +142,31 @@ SIGNATURES = {
     "aln_hash_rows": (C.c_int, [P, C.POINTER(P), C.c_int, U64, U32, P]),
     "aln_star_agg": (C.c_int, [P, C.POINTER(StarArgs), C.POINTER(StarRow), PU32, P, U64,
                                C.POINTER(StarStats)]),
+    "aln_batch_create": (C.c_int, [P, C.POINTER(Col), C.c_int, U32, C.POINTER(BatchQuery), C.c_int,
+                                   C.POINTER(BatchTuning), C.POINTER(P)]),
+    "aln_batch_set_tuning": (C.c_int, [P, C.POINTER(BatchTuning)]),
+    "aln_batch_run": (C.c_int, [P]),
+    "aln_batch_results": (C.c_int, [P, C.c_int, P, P, P, U64, PU64]),
+    "aln_batch_results_host": (C.c_int, [P, C.c_int, PI64, PI64, PI64, U64, PU64]),
+    "aln_batch_autotune": (C.c_int, [P, C.c_int, C.POINTER(BatchTuning), C.POINTER(C.c_double)]),
+    "aln_batch_destroy": (None, [P]),
+    "aln_server_create": (C.c_int, [C.c_int, C.POINTER(ServerConfig), C.POINTER(P)]),
+    "aln_server_add_table": (C.c_int, [P, C.POINTER(Col), C.c_int, U32, PI32]),
+    "aln_server_submit": (C.c_int, [P, I32, C.POINTER(BatchQuery), PU64]),
+    "aln_server_submit_star": (C.c_int, [P, C.POINTER(StarArgs), PU64]),
+    "aln_server_wait": (C.c_int, [P, U64, C.POINTER(ServerResult)]),
+    "aln_server_fetch": (C.c_int, [P, U64, PI64, PI64, PI64, U64]),
+    "aln_server_fetch_star": (C.c_int, [P, U64, C.POINTER(StarRow), U32, PU32]),
+    "aln_server_release": (C.c_int, [P, U64]),
+    "aln_server_stats": (C.c_int, [P, C.POINTER(ServerStats)]),
+    "aln_server_destroy": (None, [P]),
     "aln_gen_flat": (C.c_int, [P, U64, U64, P, P, P, P]),
     "aln_gen_dim": (C.c_int, [P, U64, U64, P, P]),
     "aln_gen_orders": (C.c_int, [P, U64, U64, U64, P, P, P, P]),
     "aln_gen_fact_rows": (C.c_int, [P, U64, U64, PU64]),
     "aln_gen_fact": (C.c_int, [P, U64, U64, P, P, P, P]),
+    "aln_gen_zipf": (C.c_int, [P, U64, U64, U64, C.c_double, P]),
+    "aln_gen_sort_input": (C.c_int, [P, U64, U64, P, P]),
     "aln_dict_first_seen": (C.c_int, [P, P, U64, I64, PI64, P]),
 }
 

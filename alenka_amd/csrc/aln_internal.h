@@ -96,6 +96,17 @@ int minmax_i64(aln_ctx *c, const int64_t *d, uint64_t n, int64_t *h_min, int64_t
 int sum_u32(aln_ctx *c, const uint32_t *d_in, uint64_t n, uint64_t *d_out);  // no sync
 int radix_sort_pairs(aln_ctx *c, uint64_t *d_keys, uint32_t *d_vals, uint64_t n,
                      uint32_t bits);
+// Batched queries (batch.hip): a batch whose device tables are carved from a caller-owned
+// region instead of hipMalloc (pool NULL: hipMalloc), and the bytes such a region needs.
+int batch_create(aln_ctx *c, const aln_col *cols, int ncols, uint32_t seg_rows,
+                 const aln_batch_query *qs, int nq, const aln_batch_tuning *tune, void *pool,
+                 size_t pool_bytes, aln_batch **out);
+size_t batch_pool_bytes(const uint64_t *ngroups, int nq);
+// Groups of query q into three host vectors (keys, sums, counts), resized to fit.
+int batch_fetch(aln_batch *b, int q, std::vector<int64_t> *keys_sums_counts);
+// Dense group count of keys in [mn, mx] grouped by floor(key / div) (0 if mn > mx).
+uint64_t dense_groups(int64_t mn, int64_t mx, int64_t div);
+int64_t floor_div(int64_t a, int64_t b);
 int fill_u64(aln_ctx *c, uint64_t *d, uint64_t n, uint64_t v);
 int fill_u32(aln_ctx *c, uint32_t *d, uint64_t n, uint32_t v);
 int fill_i64(aln_ctx *c, int64_t *d, uint64_t n, int64_t v);

@@ -1,0 +1,541 @@
+// Batched filter + GROUP BY: many queries over one table in a single persistent launch.
+//
+// Each query q keeps rows with lo_q <= pred_q <= hi_q and groups them by key_q / div_q
+// with SUM((a0 + a1*x) * (b0 + b1*y)) and COUNT(*) - the shape of the reference's
+// FILTER + SELECT ... GROUP BY (emit_filter operators.cu:2306, emit_select 1801-2084),
+// evaluated for all queries in one pass over the table (a shared scan):
+//   * a persistent grid walks the table's 4096-row tiles; the packed words of the shared
+//     predicate and key columns are staged in LDS once per tile for every query;
+//   * groups are dense over [min(key)/div, max(key)/div] (the zone maps give the range), so
+//     a group's slot is its index and output is in key order, as one-segment GROUP BY;
+//   * the first `hot` groups of every query (where skewed keys concentrate) are aggregated
+//     in LDS by each block and flushed once at the end; the others use global atomics.
+// A prepared batch owns its tables; aln_batch_run replays the launch sequence (table
+// fills + scan) from a captured HIP graph when asked to; aln_batch_autotune times the
+// grid / hot-set shapes on the real data and returns the fastest.
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "aln_internal.h"
+#include "stage.h"
+
+namespace aln {
+
+constexpr int BQ_MAX = ALN_BATCH_MAX;
+constexpr int BQ_HOT_MAX = 32;
+
+struct BQuery {
+  int32_t pred_col, key_col, x_col, y_col;
+  int64_t lo, hi;
+  int64_t div;       // >= 1
+  int64_t gbase;     // floor(min(key) / div)
+  uint64_t ngroups;  // dense group range
+  int64_t a0, a1, b0, b1;
+  unsigned long long *sum;  // [ngroups]
+  unsigned int *cnt;        // [ngroups]
+};
+
+struct BArgs {
+  const uint32_t *words[ALN_MAX_COLS];
+  const aln_seg *segs[ALN_MAX_COLS];
+  const BQuery *q;  // device [nq]
+  int nq;
+  int hot;
+  int stage_cols[2];
+  int nstage;
+  // columns every query shares (-1: per query)
+  int pred_col, key_col, x_col, y_col;
+  uint32_t seg_rows, tps;
+  uint64_t ntiles;
+};
+
+__device__ __forceinline__ int64_t floordiv(int64_t a, int64_t b) {
+  int64_t q = a / b;
+  return (a % b != 0 && ((a < 0) != (b < 0))) ? q - 1 : q;
+}
+
+__global__ void __launch_bounds__(STG_BLOCK) k_batch(BArgs a) {
+  __shared__ TileStage ts;
+  __shared__ int want[2];
+  __shared__ unsigned long long hsum[BQ_MAX][BQ_HOT_MAX];
+  __shared__ unsigned int hcnt[BQ_MAX][BQ_HOT_MAX];
+  for (int i = threadIdx.x; i < BQ_MAX * BQ_HOT_MAX; i += STG_BLOCK) {
+    hsum[i / BQ_HOT_MAX][i % BQ_HOT_MAX] = 0;
+    hcnt[i / BQ_HOT_MAX][i % BQ_HOT_MAX] = 0;
+  }
+  if (threadIdx.x < 2) want[threadIdx.x] = a.stage_cols[threadIdx.x];
+  __syncthreads();
+  for (uint64_t tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
+    const uint32_t seg = (uint32_t)(tile / a.tps), t = (uint32_t)(tile % a.tps);
+    const uint32_t nrows = a.segs[0][seg].nrows, r0 = t * STG_TILE;
+    if (r0 >= nrows) continue;  // uniform across the block
+    const uint32_t nt = nrows - r0 < (uint32_t)STG_TILE ? nrows - r0 : (uint32_t)STG_TILE;
+    __syncthreads();  // the previous tile's LDS reads are done
+    stage_tile(ts, a.words, a.segs, want, a.nstage, seg, r0, nt);
+    __syncthreads();
+    for (int j = 0; j < STG_ITEMS; ++j) {
+      const uint32_t i = j * STG_BLOCK + threadIdx.x;
+      if (i >= nt) continue;
+      const int64_t p0 = a.pred_col >= 0 ? tile_get(ts, a.words, a.segs, a.pred_col, seg, r0, i) : 0;
+      const int64_t k0 = a.key_col >= 0 ? tile_get(ts, a.words, a.segs, a.key_col, seg, r0, i) : 0;
+      int64_t x0 = 0, y0 = 0;
+      bool have_xy = false;
+      for (int qi = 0; qi < a.nq; ++qi) {
+        const BQuery &q = a.q[qi];
+        const int64_t p = a.pred_col >= 0 ? p0 : tile_get(ts, a.words, a.segs, q.pred_col, seg, r0, i);
+        if (p < q.lo || p > q.hi) continue;
+        const int64_t key = a.key_col >= 0 ? k0 : tile_get(ts, a.words, a.segs, q.key_col, seg, r0, i);
+        int64_t x, y;
+        if (a.x_col >= 0) {
+          if (!have_xy) {
+            x0 = tile_get(ts, a.words, a.segs, a.x_col, seg, r0, i);
+            y0 = tile_get(ts, a.words, a.segs, a.y_col, seg, r0, i);
+            have_xy = true;
+          }
+          x = x0;
+          y = y0;
+        } else {
+          x = tile_get(ts, a.words, a.segs, q.x_col, seg, r0, i);
+          y = tile_get(ts, a.words, a.segs, q.y_col, seg, r0, i);
+        }
+        const uint64_t g = (uint64_t)(floordiv(key, q.div) - q.gbase);
+        const int64_t v = wmul((int64_t)((uint64_t)q.a0 + (uint64_t)wmul(q.a1, x)),
+                               (int64_t)((uint64_t)q.b0 + (uint64_t)wmul(q.b1, y)));
+        if (g < (uint64_t)a.hot) {
+          atomicAdd(&hsum[qi][g], (unsigned long long)v);
+          atomicAdd(&hcnt[qi][g], 1u);
+        } else if (g < q.ngroups) {
+          atomicAdd(&q.sum[g], (unsigned long long)v);
+          atomicAdd(&q.cnt[g], 1u);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < a.nq * a.hot; i += STG_BLOCK) {
+    const int qi = i / a.hot, g = i % a.hot;
+    const unsigned int c = hcnt[qi][g];
+    if (c && (uint64_t)g < a.q[qi].ngroups) {
+      atomicAdd(&a.q[qi].sum[g], hsum[qi][g]);
+      atomicAdd(&a.q[qi].cnt[g], c);
+    }
+  }
+}
+
+__global__ void k_batch_clear(const BQuery *q, int nq) {
+  for (int qi = 0; qi < nq; ++qi) {
+    const uint64_t n = q[qi].ngroups;
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
+      q[qi].sum[i] = 0;
+      q[qi].cnt[i] = 0;
+    }
+  }
+}
+
+__global__ void k_batch_flags(const unsigned int *cnt, uint64_t n, uint32_t *flag) {
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256)
+    flag[i] = cnt[i] != 0;
+}
+__global__ void k_batch_emit(const uint32_t *flag, const uint64_t *pos, uint64_t n, int64_t gbase,
+                             const unsigned long long *sum, const unsigned int *cnt, int64_t *okey,
+                             int64_t *osum, int64_t *ocnt) {
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256)
+    if (flag[i]) {
+      const uint64_t o = pos[i];
+      okey[o] = gbase + (int64_t)i;
+      osum[o] = (int64_t)sum[i];
+      ocnt[o] = cnt[i];
+    }
+}
+
+}  // namespace aln
+
+struct aln_batch {
+  aln_ctx *ctx = nullptr;
+  aln::BArgs args;
+  std::vector<aln::BQuery> hq;
+  aln::BQuery *dq = nullptr;
+  std::vector<void *> allocs;
+  char *pool = nullptr;  // caller-owned region the tables are carved from (else hipMalloc)
+  size_t pool_size = 0, pool_used = 0;
+  uint32_t blocks = 0;
+  uint64_t max_groups = 0;
+  bool use_graph = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+using namespace aln;
+
+namespace {
+
+int batch_free(aln_batch *b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  for (void *p : b->allocs) (void)hipFree(p);
+  b->allocs.clear();
+  return ALN_OK;
+}
+
+int batch_alloc(aln_batch *b, size_t bytes, void **out) {
+  if (b->pool) {
+    const size_t a = (bytes + 255) & ~(size_t)255;
+    if (b->pool_used + a > b->pool_size)
+      return set_error(ALN_E_NOMEM, "aln_batch: region of %zu bytes is too small", b->pool_size);
+    *out = b->pool + b->pool_used;
+    b->pool_used += a;
+    return ALN_OK;
+  }
+  if (hipMalloc(out, bytes ? bytes : 8) != hipSuccess) {
+    (void)hipGetLastError();
+    return set_error(ALN_E_NOMEM, "aln_batch: allocation of %zu bytes failed", bytes);
+  }
+  b->allocs.push_back(*out);
+  return ALN_OK;
+}
+
+int batch_enqueue(aln_batch *b, hipStream_t st) {
+  hipLaunchKernelGGL(k_batch_clear, dim3(grid_for(b->max_groups, 256, 8192)), dim3(256), 0, st, b->dq,
+                     b->args.nq);
+  if (b->args.ntiles) hipLaunchKernelGGL(k_batch, dim3(b->blocks), dim3(STG_BLOCK), 0, st, b->args);
+  return check_launch("batch_scan");
+}
+
+int set_tuning(aln_batch *b, const aln_batch_tuning *tune) {
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->ctx->device) != hipSuccess || cus <= 0)
+    cus = 256;
+  const int bpc = tune && tune->blocks_per_cu > 0 ? tune->blocks_per_cu : 2;
+  b->blocks = (uint32_t)(bpc * cus);
+  if (b->blocks > b->args.ntiles) b->blocks = (uint32_t)(b->args.ntiles ? b->args.ntiles : 1);
+  b->args.hot = tune && tune->hot_groups >= 0 ? std::min(tune->hot_groups, BQ_HOT_MAX) : BQ_HOT_MAX;
+  b->use_graph = tune && tune->use_graph;
+  if (b->exec) {
+    (void)hipGraphExecDestroy(b->exec);
+    b->exec = nullptr;
+  }
+  if (b->graph) {
+    (void)hipGraphDestroy(b->graph);
+    b->graph = nullptr;
+  }
+  return ALN_OK;
+}
+
+}  // namespace
+
+namespace aln {
+
+int64_t floor_div(int64_t a, int64_t b) {
+  const int64_t q = a / b;
+  return (a % b != 0 && ((a < 0) != (b < 0))) ? q - 1 : q;
+}
+
+uint64_t dense_groups(int64_t mn, int64_t mx, int64_t div) {
+  if (mx < mn) return 0;
+  return (uint64_t)(floor_div(mx, div) - floor_div(mn, div)) + 1;
+}
+
+size_t batch_pool_bytes(const uint64_t *ngroups, int nq) {
+  auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  size_t t = al(sizeof(BQuery) * (size_t)nq);
+  for (int i = 0; i < nq; ++i) {
+    const size_t g = ngroups[i] ? ngroups[i] : 1;
+    t += al(8 * g) + al(4 * g);
+  }
+  return t;
+}
+
+int batch_create(aln_ctx *c, const aln_col *cols, int ncols, uint32_t seg_rows, const aln_batch_query *qs,
+                 int nq, const aln_batch_tuning *tune, void *pool, size_t pool_bytes, aln_batch **out) {
+  ALN_REQUIRE(c && cols && qs && out && seg_rows > 0, "aln_batch_create: bad arguments");
+  ALN_REQUIRE(ncols >= 1 && ncols <= ALN_MAX_COLS, "aln_batch_create: %d columns", ncols);
+  ALN_REQUIRE(nq >= 1 && nq <= BQ_MAX, "aln_batch_create: %d queries (1..%d)", nq, BQ_MAX);
+  const uint32_t nsegs = cols[0].nsegs;
+  for (int i = 0; i < ncols; ++i)
+    ALN_REQUIRE(cols[i].nsegs == nsegs, "aln_batch_create: columns differ in segments");
+  aln_batch *b = new aln_batch();
+  b->ctx = c;
+  b->pool = (char *)pool;
+  b->pool_size = pool ? pool_bytes : 0;
+  auto bail = [&](int rc) {
+    batch_free(b);
+    delete b;
+    return rc;
+  };
+  // zone maps give each key column's range
+  std::vector<std::vector<aln_seg>> hs(ncols);
+  auto range_of = [&](int col, int64_t *mn, int64_t *mx) -> int {
+    if (hs[col].empty() && nsegs) {
+      hs[col].resize(nsegs);
+      ALN_TRY(copy_to_host(c, hs[col].data(), cols[col].segs, sizeof(aln_seg) * nsegs));
+    }
+    *mn = INT64_MAX;
+    *mx = INT64_MIN;
+    bool open_ended = false;
+    uint64_t rows = 0;
+    for (auto &s : hs[col])
+      if (s.nrows) {
+        *mn = s.vmin < *mn ? s.vmin : *mn;
+        *mx = s.vmax > *mx ? s.vmax : *mx;
+        open_ended = open_ended || (s.bits == 64 && s.vmin == INT64_MIN && s.vmax == INT64_MAX);
+        rows += s.nrows;
+      }
+    // a plain 64-bit column carries no zone map: its words are the int64 values in row order
+    if (open_ended) {
+      for (uint32_t s = 0; s < nsegs; ++s)
+        if (hs[col][s].bits != 64 || hs[col][s].word_off != 2ull * s * seg_rows)
+          return set_error(ALN_E_ARG, "aln_batch_create: column %d has no zone map", col);
+      ALN_TRY(minmax_i64(c, (const int64_t *)cols[col].words, rows, mn, mx));
+    }
+    return ALN_OK;
+  };
+  // every column must split into the same segments (row counts)
+  std::vector<int64_t> cmin(ncols), cmax(ncols);
+  for (int col = ncols - 1; col >= 0; --col) {
+    int rc = range_of(col, &cmin[col], &cmax[col]);
+    if (rc) return bail(rc);
+    for (uint32_t s = 0; s < nsegs; ++s)
+      if (hs[col][s].nrows != hs[0][s].nrows || hs[0][s].nrows > seg_rows)
+        return bail(set_error(ALN_E_ARG, "aln_batch_create: column %d segment %u has %u rows", col, s,
+                              hs[col][s].nrows));
+  }
+  b->hq.resize(nq);
+  std::vector<int> stage;
+  memset(&b->args, 0, sizeof(b->args));
+  b->args.pred_col = qs[0].pred_col;
+  b->args.key_col = qs[0].key_col;
+  b->args.x_col = qs[0].x_col;
+  b->args.y_col = qs[0].y_col;
+  for (int i = 0; i < nq; ++i) {
+    const aln_batch_query &q = qs[i];
+    if (!(q.pred_col >= 0 && q.pred_col < ncols && q.key_col >= 0 && q.key_col < ncols && q.x_col >= 0 &&
+          q.x_col < ncols && q.y_col >= 0 && q.y_col < ncols))
+      return bail(set_error(ALN_E_ARG, "aln_batch_create: query %d column slot", i));
+    if (q.key_div < 1) return bail(set_error(ALN_E_ARG, "aln_batch_create: query %d key_div", i));
+    const int64_t mn = cmin[q.key_col], mx = cmax[q.key_col];
+    BQuery &bq = b->hq[i];
+    bq.pred_col = q.pred_col;
+    bq.key_col = q.key_col;
+    bq.x_col = q.x_col;
+    bq.y_col = q.y_col;
+    bq.lo = q.lo;
+    bq.hi = q.hi;
+    bq.div = q.key_div;
+    bq.a0 = q.a0;
+    bq.a1 = q.a1;
+    bq.b0 = q.b0;
+    bq.b1 = q.b1;
+    bq.gbase = mx < mn ? 0 : floor_div(mn, q.key_div);
+    bq.ngroups = dense_groups(mn, mx, q.key_div);
+    if (bq.ngroups > (1ull << 31))
+      return bail(set_error(ALN_E_UNSUPPORTED, "aln_batch_create: query %d has %llu dense groups", i,
+                            (unsigned long long)bq.ngroups));
+    b->max_groups = std::max(b->max_groups, bq.ngroups);
+    if (q.pred_col != b->args.pred_col) b->args.pred_col = -1;
+    if (q.key_col != b->args.key_col) b->args.key_col = -1;
+    if (q.x_col != b->args.x_col || q.y_col != b->args.y_col) b->args.x_col = b->args.y_col = -1;
+    for (int col : {q.pred_col, q.key_col})
+      if (std::find(stage.begin(), stage.end(), col) == stage.end()) stage.push_back(col);
+    void *p;
+    int r1 = batch_alloc(b, 8 * (bq.ngroups ? bq.ngroups : 1), &p);
+    if (r1) return bail(r1);
+    bq.sum = (unsigned long long *)p;
+    r1 = batch_alloc(b, 4 * (bq.ngroups ? bq.ngroups : 1), &p);
+    if (r1) return bail(r1);
+    bq.cnt = (unsigned int *)p;
+  }
+  void *p;
+  int r2 = batch_alloc(b, sizeof(BQuery) * nq, &p);
+  if (r2) return bail(r2);
+  b->dq = (BQuery *)p;
+  {
+    const int rc = copy_to_device(c, b->dq, b->hq.data(), sizeof(BQuery) * nq);
+    if (rc) return bail(rc);
+  }
+  for (int i = 0; i < ncols; ++i) {
+    b->args.words[i] = cols[i].words;
+    b->args.segs[i] = cols[i].segs;
+  }
+  b->args.q = b->dq;
+  b->args.nq = nq;
+  b->args.nstage = (int)std::min<size_t>(stage.size(), 2);
+  for (int i = 0; i < b->args.nstage; ++i) b->args.stage_cols[i] = stage[i];
+  b->args.seg_rows = seg_rows;
+  b->args.tps = (seg_rows + STG_TILE - 1) / STG_TILE;
+  b->args.ntiles = (uint64_t)nsegs * b->args.tps;
+  set_tuning(b, tune);
+  *out = b;
+  return ALN_OK;
+}
+
+}  // namespace aln
+
+extern "C" {
+
+int aln_batch_create(aln_ctx *c, const aln_col *cols, int ncols, uint32_t seg_rows,
+                     const aln_batch_query *qs, int nq, const aln_batch_tuning *tune, aln_batch **out) {
+  return batch_create(c, cols, ncols, seg_rows, qs, nq, tune, nullptr, 0, out);
+}
+
+int aln_batch_set_tuning(aln_batch *b, const aln_batch_tuning *tune) {
+  ALN_REQUIRE(b, "aln_batch_set_tuning: NULL batch");
+  return set_tuning(b, tune);
+}
+
+// Enqueues one run (clear tables + scan) on the context stream; no synchronisation.
+int aln_batch_run(aln_batch *b) {
+  ALN_REQUIRE(b, "aln_batch_run: NULL batch");
+  aln_ctx *c = b->ctx;
+  if (!b->use_graph) {
+    ProfScope p(c, "batch_scan");
+    return batch_enqueue(b, c->stream);
+  }
+  if (!b->exec) {
+    // Capture the launch sequence once (fill kernels only, no memset nodes) on a private
+    // stream: the context stream may be the legacy default stream, which cannot capture.
+    hipStream_t cs;
+    ALN_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    int rc = ALN_OK;
+    hipGraph_t g = nullptr;
+    if (e == hipSuccess) {
+      rc = batch_enqueue(b, cs);
+      e = hipStreamEndCapture(cs, &g);
+    }
+    (void)hipStreamDestroy(cs);
+    if (rc != ALN_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return set_error(ALN_E_HIP, "aln_batch_run: capture failed: %s", hipGetErrorString(e));
+    b->graph = g;
+    ALN_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  }
+  ProfScope p(c, "batch_scan");
+  ALN_HIP(hipGraphLaunch(b->exec, c->stream));
+  return ALN_OK;
+}
+
+}  // extern "C"
+
+// Groups of query q with at least one row, in key order (synchronises).  With host
+// pointers the groups are emitted into workspace and copied out.
+static int batch_results(aln_batch *b, int q, int64_t *keys, int64_t *sums, int64_t *counts, uint64_t cap,
+                         uint64_t *h_ngroups, bool host, std::vector<int64_t> *vec = nullptr) {
+  ALN_REQUIRE(b && h_ngroups && q >= 0 && q < b->args.nq, "aln_batch_results: bad arguments");
+  aln_ctx *c = b->ctx;
+  ws_reset(c);
+  const BQuery &bq = b->hq[q];
+  const uint64_t n = bq.ngroups;
+  *h_ngroups = 0;
+  if (!n) return ALN_OK;
+  uint32_t *flag;
+  uint64_t *pos;
+  ALN_TRY(ws(c, n, &flag));
+  ALN_TRY(ws(c, n, &pos));
+  hipLaunchKernelGGL(k_batch_flags, dim3(grid_for(n, 256, 65536)), dim3(256), 0, c->stream, bq.cnt, n, flag);
+  ALN_TRY(check_launch("batch_flags"));
+  uint64_t g = 0;
+  ALN_TRY(exclusive_scan_u32(c, flag, n, pos, &g));
+  *h_ngroups = g;
+  if (vec) {  // host vectors sized to the result
+    for (int i = 0; i < 3; ++i) vec[i].resize(g);
+    keys = vec[0].data();
+    sums = vec[1].data();
+    counts = vec[2].data();
+    cap = g;
+  }
+  if (g > cap)
+    return set_error(ALN_E_CAPACITY, "aln_batch_results: %llu groups, capacity %llu", (unsigned long long)g,
+                     (unsigned long long)cap);
+  if (!g) return ALN_OK;
+  ALN_REQUIRE(keys && sums && counts, "aln_batch_results: no output buffers");
+  int64_t *dk = keys, *ds = sums, *dc = counts;
+  if (host) {
+    ALN_TRY(ws(c, g, &dk));
+    ALN_TRY(ws(c, g, &ds));
+    ALN_TRY(ws(c, g, &dc));
+  }
+  hipLaunchKernelGGL(k_batch_emit, dim3(grid_for(n, 256, 65536)), dim3(256), 0, c->stream, flag, pos, n, bq.gbase,
+                     bq.sum, bq.cnt, dk, ds, dc);
+  ALN_TRY(check_launch("batch_emit"));
+  if (host) {
+    ALN_TRY(copy_to_host(c, keys, dk, 8 * g));
+    ALN_TRY(copy_to_host(c, sums, ds, 8 * g));
+    ALN_TRY(copy_to_host(c, counts, dc, 8 * g));
+  }
+  ALN_HIP(hipStreamSynchronize(c->stream));
+  return ALN_OK;
+}
+
+namespace aln {
+int batch_fetch(aln_batch *b, int q, std::vector<int64_t> *keys_sums_counts) {
+  uint64_t g = 0;
+  return batch_results(b, q, nullptr, nullptr, nullptr, 0, &g, true, keys_sums_counts);
+}
+}  // namespace aln
+
+extern "C" {
+
+int aln_batch_results(aln_batch *b, int q, int64_t *d_keys, int64_t *d_sums, int64_t *d_counts,
+                      uint64_t cap, uint64_t *h_ngroups) {
+  return batch_results(b, q, d_keys, d_sums, d_counts, cap, h_ngroups, false);
+}
+
+int aln_batch_results_host(aln_batch *b, int q, int64_t *h_keys, int64_t *h_sums, int64_t *h_counts,
+                           uint64_t cap, uint64_t *h_ngroups) {
+  return batch_results(b, q, h_keys, h_sums, h_counts, cap, h_ngroups, true);
+}
+
+void aln_batch_destroy(aln_batch *b) {
+  if (!b) return;
+  if (b->ctx) (void)hipStreamSynchronize(b->ctx->stream);
+  batch_free(b);
+  delete b;
+}
+
+// Times each candidate shape (grid size x hot-set size) on the batch and keeps the fastest.
+int aln_batch_autotune(aln_batch *b, int reps, aln_batch_tuning *best, double *h_ms) {
+  ALN_REQUIRE(b && best, "aln_batch_autotune: bad arguments");
+  aln_ctx *c = b->ctx;
+  if (reps < 1) reps = 1;
+  const int bpcs[] = {1, 2, 4};
+  const int hots[] = {0, 8, 32};
+  hipEvent_t e0, e1;
+  ALN_HIP(hipEventCreate(&e0));
+  ALN_HIP(hipEventCreate(&e1));
+  double best_ms = 1e30;
+  aln_batch_tuning bt;
+  memset(&bt, 0, sizeof(bt));
+  for (int bpc : bpcs)
+    for (int hot : hots) {
+      aln_batch_tuning t;
+      memset(&t, 0, sizeof(t));
+      t.blocks_per_cu = bpc;
+      t.hot_groups = hot;
+      t.use_graph = 0;
+      set_tuning(b, &t);
+      ALN_TRY(batch_enqueue(b, c->stream));  // warm
+      ALN_HIP(hipEventRecord(e0, c->stream));
+      for (int r = 0; r < reps; ++r) ALN_TRY(batch_enqueue(b, c->stream));
+      ALN_HIP(hipEventRecord(e1, c->stream));
+      ALN_HIP(hipEventSynchronize(e1));
+      float ms = 0;
+      ALN_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (ms / reps < best_ms) {
+        best_ms = ms / reps;
+        bt = t;
+      }
+    }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  bt.use_graph = best->use_graph;
+  *best = bt;
+  set_tuning(b, best);
+  if (h_ms) *h_ms = best_ms;
+  return ALN_OK;
+}
+
+}  // extern "C"

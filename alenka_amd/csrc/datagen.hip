@@ -49,7 +49,8 @@ enum {
   ST_FLAT_KEY = 1, ST_FLAT_DATE, ST_FLAT_PRICE, ST_FLAT_DISC,
   ST_DIM_SEG = 10,
   ST_ORD_SLOT = 20, ST_ORD_CUST, ST_ORD_DATE,
-  ST_FACT_CNT = 30, ST_FACT_PRICE, ST_FACT_DISC, ST_FACT_SHIP
+  ST_FACT_CNT = 30, ST_FACT_PRICE, ST_FACT_DISC, ST_FACT_SHIP,
+  ST_ZIPF = 40, ST_SORT_KEY = 41
 };
 constexpr uint64_t NDAYS = 2557, PRICE_LO = 90000, PRICE_SPAN = 10410001;
 
@@ -67,6 +68,59 @@ __device__ __forceinline__ int64_t order_key(uint64_t kslot, uint64_t i) {
 #define GRID_LOOP(i, n)                                                          \
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (n); \
        i += (uint64_t)gridDim.x * blockDim.x)
+
+// Zipf(s) over ranks 1..N by rejection-inversion (W. Hormann, G. Derflinger, "Rejection-
+// inversion to generate variates from monotone discrete distributions", ACM TOMACS 6(3),
+// 1996): invert the integral H of the hat h(x) = x^-s, round to the nearest rank and accept
+// unless the rank falls in the thin sliver between the hat and the histogram.  Trial t of
+// row i draws its uniform from counter 8*i + t; after 8 rejections the last rank is kept.
+struct ZipfParams {
+  double s, hx1, hn, sconst;
+  uint64_t n;
+};
+__host__ __device__ __forceinline__ double zf_expm1_over(double t) {  // (e^t - 1) / t
+  return fabs(t) > 1e-8 ? expm1(t) / t : 1.0 + t * 0.5 * (1.0 + t / 3.0 * (1.0 + 0.25 * t));
+}
+__host__ __device__ __forceinline__ double zf_log1p_over(double t) {  // log(1 + t) / t
+  return fabs(t) > 1e-8 ? log1p(t) / t : 1.0 - t * (0.5 - t * (1.0 / 3.0 - 0.25 * t));
+}
+__host__ __device__ __forceinline__ double zf_H(double s, double x) {  // integral of x^-s
+  const double lx = log(x);
+  return zf_expm1_over((1.0 - s) * lx) * lx;
+}
+__host__ __device__ __forceinline__ double zf_Hinv(double s, double y) {
+  double t = y * (1.0 - s);
+  if (t < -1.0) t = -1.0;  // rounding at the lower end
+  return exp(zf_log1p_over(t) * y);
+}
+__host__ __device__ __forceinline__ double zf_h(double s, double x) { return exp(-s * log(x)); }
+
+__global__ void k_gen_zipf(uint64_t seed, uint64_t n, ZipfParams z, int64_t *out) {
+  const uint64_t sk = stream_key(seed, ST_ZIPF);
+  GRID_LOOP(i, n) {
+    double k = 1.0;
+    for (int t = 0; t < 8; ++t) {
+      const uint64_t r = splitmix64(sk + (8 * i + t) * GOLDEN);
+      const double u01 = (double)(r >> 11) * (1.0 / 9007199254740992.0);
+      const double u = z.hn + u01 * (z.hx1 - z.hn);
+      const double x = zf_Hinv(z.s, u);
+      k = floor(x + 0.5);
+      if (k < 1.0) k = 1.0;
+      if (k > (double)z.n) k = (double)z.n;
+      if (k - x <= z.sconst || u >= zf_H(z.s, k + 0.5) - zf_h(z.s, k)) break;
+    }
+    out[i] = (int64_t)k - 1;
+  }
+}
+
+// Uniform 64-bit keys and their row numbers as payload (the ORDER BY input of the batch run).
+__global__ void k_gen_sort_input(uint64_t seed, uint64_t n, int64_t *keys, uint32_t *payload) {
+  const uint64_t sk = stream_key(seed, ST_SORT_KEY);
+  GRID_LOOP(i, n) {
+    keys[i] = (int64_t)splitmix64(sk + i * GOLDEN);
+    if (payload) payload[i] = (uint32_t)i;
+  }
+}
 
 __global__ void k_gen_flat(uint64_t seed, uint64_t n, int64_t *key, int64_t *date, int64_t *price,
                            int64_t *disc) {
@@ -125,15 +179,25 @@ __global__ void k_gen_fact(uint64_t seed, uint64_t norders, const uint32_t *cnt,
   }
 }
 
+// First row of each value: per-block minima (LDS), one atomic per block and value.
 __global__ void k_first_seen(const int64_t *v, uint64_t n, int64_t nvals,
                              unsigned long long *first, uint32_t *bad) {
+  __shared__ unsigned long long bmin[64];
+  const bool small = nvals <= 64;
+  if (small && threadIdx.x < 64) bmin[threadIdx.x] = ~0ull;
+  __syncthreads();
   GRID_LOOP(i, n) {
     int64_t x = v[i];
     if (x < 0 || x >= nvals)
       atomicExch(bad, 1u);
+    else if (small)
+      atomicMin(&bmin[x], (unsigned long long)i);
     else
       atomicMin(&first[x], (unsigned long long)i);
   }
+  __syncthreads();
+  if (small && threadIdx.x < nvals && bmin[threadIdx.x] != ~0ull)
+    atomicMin(&first[threadIdx.x], bmin[threadIdx.x]);
 }
 __global__ void k_map_codes(const int64_t *v, uint64_t n, const int64_t *code_of, int64_t *out) {
   GRID_LOOP(i, n) out[i] = code_of[v[i]];
@@ -228,7 +292,7 @@ int aln_dict_first_seen(aln_ctx *c, const int64_t *d_vals, uint64_t n, int64_t n
   ALN_TRY(fill_u64(c, (uint64_t *)first, (uint64_t)nvals, ~0ull));
   ALN_TRY(fill_u32(c, bad, 1, 0));
   if (n) {
-    hipLaunchKernelGGL(k_first_seen, dim3(grid_for(n, 256, 65536)), dim3(256), 0, c->stream, d_vals, n,
+    hipLaunchKernelGGL(k_first_seen, dim3(grid_for(n, 256 * 64, 4096)), dim3(256), 0, c->stream, d_vals, n,
                        nvals, first, bad);
     ALN_TRY(check_launch("first_seen"));
   }
@@ -251,6 +315,33 @@ int aln_dict_first_seen(aln_ctx *c, const int64_t *d_vals, uint64_t n, int64_t n
     ALN_TRY(check_launch("map_codes"));
     ALN_HIP(hipStreamSynchronize(c->stream));
   }
+  return ALN_OK;
+}
+
+int aln_gen_zipf(aln_ctx *c, uint64_t seed, uint64_t n, uint64_t nkeys, double s, int64_t *d_out) {
+  ALN_REQUIRE(c && (n == 0 || d_out), "aln_gen_zipf: bad arguments");
+  ALN_REQUIRE(nkeys >= 1 && nkeys < (1ull << 53) && s > 0.0 && s < 64.0, "aln_gen_zipf: nkeys %llu, s %g",
+              (unsigned long long)nkeys, s);
+  if (!n) return ALN_OK;
+  ZipfParams z;
+  z.s = s;
+  z.n = nkeys;
+  z.hx1 = zf_H(s, 1.5) - 1.0;
+  z.hn = zf_H(s, (double)nkeys + 0.5);
+  z.sconst = 2.0 - zf_Hinv(s, zf_H(s, 2.5) - zf_h(s, 2.0));
+  hipLaunchKernelGGL(k_gen_zipf, dim3(grid_for(n, 256, 65536)), dim3(256), 0, c->stream, seed, n, z, d_out);
+  ALN_TRY(check_launch("gen_zipf"));
+  ALN_HIP(hipStreamSynchronize(c->stream));
+  return ALN_OK;
+}
+
+int aln_gen_sort_input(aln_ctx *c, uint64_t seed, uint64_t n, int64_t *d_keys, uint32_t *d_payload) {
+  ALN_REQUIRE(c && (n == 0 || d_keys), "aln_gen_sort_input: bad arguments");
+  if (!n) return ALN_OK;
+  hipLaunchKernelGGL(k_gen_sort_input, dim3(grid_for(n, 256, 65536)), dim3(256), 0, c->stream, seed, n,
+                     d_keys, d_payload);
+  ALN_TRY(check_launch("gen_sort_input"));
+  ALN_HIP(hipStreamSynchronize(c->stream));
   return ALN_OK;
 }
 

@@ -1,5 +1,5 @@
 // Tile staging for scans over the bit-packed layout.  A block handles a tile of 4096 rows
-// of one segment; the packed words of up to three columns the tile reads for every row
+// of one segment; the packed words of up to two columns the tile reads for every row
 // (predicate and key columns) are copied into LDS with coalesced dword loads issued up
 // front, then decoded from LDS.  Columns wider than 31 bits, and columns read only for the
 // few rows that survive a filter or join, are decoded straight from global memory.
@@ -12,7 +12,7 @@ namespace aln {
 constexpr int STG_BLOCK = 256;
 constexpr int STG_ITEMS = 16;
 constexpr int STG_TILE = STG_BLOCK * STG_ITEMS;
-constexpr int STG_SLOTS = 3;
+constexpr int STG_SLOTS = 2;
 constexpr int STG_MAX_BITS = 31;
 constexpr int STG_SLOT_WORDS = (STG_TILE * STG_MAX_BITS + 31) / 32 + 2;
 

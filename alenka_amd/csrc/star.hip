@@ -310,7 +310,9 @@ __global__ void __launch_bounds__(ST_BLOCK) k_star_fact(ScanDesc d, int fkcol, i
                  (int64_t)((uint64_t)b0 + (uint64_t)wmul(b1, y)));
       }
     }
-    // segmented reduction over runs of equal group index in adjacent lanes
+    // segmented reduction over runs of equal group index in adjacent lanes (skipped when no
+    // lane of the wave joined, the common case)
+    if (__ballot(g != 0xFFFFFFFFu) == 0) continue;
     uint32_t prev = __shfl_up(g, 1, 64);
     bool f = lane == 0 || prev != g;
     uint32_t c = g != 0xFFFFFFFFu;
@@ -425,6 +427,7 @@ __global__ void __launch_bounds__(ST_BLOCK) k_star_fact_bm(ScanDesc d, int fkcol
                  (int64_t)((uint64_t)b0 + (uint64_t)wmul(b1, y)));
       }
     }
+    if (__ballot(g != 0xFFFFFFFFu) == 0) continue;
     uint32_t prev = __shfl_up(g, 1, 64);
     bool f = lane == 0 || prev != g;
     uint32_t c = g != 0xFFFFFFFFu;

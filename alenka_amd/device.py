@@ -6,6 +6,8 @@ libalenka_hip.so.  Shapes are checked here before any launch.
 """
 import ctypes as C
 
+import numpy as np
+
 import torch
 
 from . import _lib as L
@@ -112,6 +114,14 @@ class Column:
             hs[s].nrows = min(seg_rows, n - r0)
             hs[s].vmin, hs[s].vmax = -(1 << 63), (1 << 63) - 1
         return cls(vals.view(torch.int32), dsegs, hs, n, seg_rows, decimals, keep=vals)
+
+    def prefix(self, nsegs):
+        """The first nsegs segments as a column of their own (shares the device data)."""
+        nsegs = max(1, min(int(nsegs), self.nsegs))
+        hs = (L.Seg * nsegs)()
+        C.memmove(hs, self.hsegs, C.sizeof(L.Seg) * nsegs)
+        rows = sum(hs[s].nrows for s in range(nsegs))
+        return Column(self.words, self.dsegs, hs, rows, self.seg_rows, self.decimals, keep=self)
 
     def desc(self):
         c = L.Col()
@@ -281,6 +291,19 @@ def gen_fact(ctx, seed, norders, cols=(True, True, True, True)):
     return [x[:n] if x is not None else None for x in t]
 
 
+def gen_zipf(ctx, seed, n, nkeys, s):
+    out = ctx.empty(n)
+    L.check(ctx.lib.aln_gen_zipf(ctx.h, seed, n, nkeys, float(s), _p(out)), "aln_gen_zipf")
+    return out[:n]
+
+
+def gen_sort_input(ctx, seed, n, payload=True):
+    keys = ctx.empty(n)
+    pay = ctx.empty(n, torch.int32) if payload else None
+    L.check(ctx.lib.aln_gen_sort_input(ctx.h, seed, n, _p(keys), _p(pay)), "aln_gen_sort_input")
+    return keys[:n], (pay[:n] if payload else None)
+
+
 def dict_first_seen(ctx, vals, nvals):
     code_of = (C.c_int64 * nvals)()
     out = ctx.empty(vals.numel())
@@ -294,6 +317,26 @@ def star_agg(ctx, dim, mid, fact, spec, limit=10, want_groups=False, groups_cap=
     """Fused filter-join-join-aggregate-top-k.  dim/mid/fact: dicts with 'cols' (list of
     Column), 'ranges' (list of (col, lo, hi)) and optional 'modes'.  spec: dict of the
     column slots d_key, m_key, m_fk, m_g1, m_g2, f_fk, f_x, f_y and a0, a1, b0, b1."""
+    a, keep = star_args(dim, mid, fact, spec, limit, (1 if want_groups else 0) | (2 if force_hash else 0))
+    top = (L.StarRow * max(1, limit))()
+    ntop = C.c_uint32()
+    st = L.StarStats()
+    groups = None
+    cap = 0
+    if want_groups:
+        cap = groups_cap or mid["cols"][0].nrows
+        groups = ctx.empty(max(1, cap) * C.sizeof(L.StarRow), torch.uint8)
+    L.check(ctx.lib.aln_star_agg(ctx.h, C.byref(a), top, C.byref(ntop), _p(groups), cap, C.byref(st)),
+            "aln_star_agg")
+    rows = [(top[i].key, top[i].g1, top[i].g2, top[i].sum, top[i].count) for i in range(ntop.value)]
+    g = None
+    if want_groups:
+        g = groups[: st.groups * C.sizeof(L.StarRow)].view(torch.int64).view(-1, 5)
+    return rows, g, st
+
+
+def star_args(dim, mid, fact, spec, limit, flags):
+    """aln_star_args for three table scans; returns (args, host arrays to keep alive)."""
     keep = []
 
     def scan(t):
@@ -317,19 +360,139 @@ def star_agg(ctx, dim, mid, fact, spec, limit=10, want_groups=False, groups_cap=
     for k in ("d_key", "m_key", "m_fk", "m_g1", "m_g2", "f_fk", "f_x", "f_y", "a0", "a1", "b0", "b1"):
         setattr(a, k, spec[k])
     a.limit = limit
-    a.flags = (1 if want_groups else 0) | (2 if force_hash else 0)
-    top = (L.StarRow * max(1, limit))()
-    ntop = C.c_uint32()
-    st = L.StarStats()
-    groups = None
-    cap = 0
-    if want_groups:
-        cap = groups_cap or mid["cols"][0].nrows
-        groups = ctx.empty(max(1, cap) * C.sizeof(L.StarRow), torch.uint8)
-    L.check(ctx.lib.aln_star_agg(ctx.h, C.byref(a), top, C.byref(ntop), _p(groups), cap, C.byref(st)),
-            "aln_star_agg")
-    rows = [(top[i].key, top[i].g1, top[i].g2, top[i].sum, top[i].count) for i in range(ntop.value)]
-    g = None
-    if want_groups:
-        g = groups[: st.groups * C.sizeof(L.StarRow)].view(torch.int64).view(-1, 5)
-    return rows, g, st
+    a.flags = flags
+    return a, keep
+
+
+class Batch:
+    """A prepared batch of filter + GROUP BY queries over one table (aln_batch_*).
+    queries: list of dicts with pred_col, key_col, x_col, y_col, lo, hi, key_div, a0, a1,
+    b0, b1 (column slots index `cols`)."""
+
+    def __init__(self, ctx, cols, queries, hot_groups=-1, blocks_per_cu=0, use_graph=False):
+        self.ctx = ctx
+        self.cols = list(cols)
+        self._arr = _cols_array(self.cols)
+        qs = (L.BatchQuery * len(queries))()
+        for i, q in enumerate(queries):
+            for k in ("pred_col", "key_col", "x_col", "y_col", "lo", "hi", "key_div", "a0", "a1", "b0", "b1"):
+                setattr(qs[i], k, int(q[k]))
+        self.nq = len(queries)
+        self.tuning = L.BatchTuning(hot_groups, blocks_per_cu, 1 if use_graph else 0)
+        h = C.c_void_p()
+        L.check(ctx.lib.aln_batch_create(ctx.h, self._arr, len(self.cols), self.cols[0].seg_rows, qs,
+                                         self.nq, C.byref(self.tuning), C.byref(h)), "aln_batch_create")
+        self.h = h
+
+    def set_tuning(self, hot_groups=-1, blocks_per_cu=0, use_graph=False):
+        self.tuning = L.BatchTuning(hot_groups, blocks_per_cu, 1 if use_graph else 0)
+        L.check(self.ctx.lib.aln_batch_set_tuning(self.h, C.byref(self.tuning)), "aln_batch_set_tuning")
+
+    def autotune(self, reps=3, use_graph=False):
+        best = L.BatchTuning(0, 0, 1 if use_graph else 0)
+        ms = C.c_double()
+        L.check(self.ctx.lib.aln_batch_autotune(self.h, reps, C.byref(best), C.byref(ms)), "aln_batch_autotune")
+        self.tuning = best
+        return (best.hot_groups, best.blocks_per_cu), ms.value
+
+    def run(self):
+        L.check(self.ctx.lib.aln_batch_run(self.h), "aln_batch_run")
+
+    def results(self, q, cap):
+        keys, sums, cnts = self.ctx.empty(cap), self.ctx.empty(cap), self.ctx.empty(cap)
+        n = C.c_uint64()
+        L.check(self.ctx.lib.aln_batch_results(self.h, q, _p(keys), _p(sums), _p(cnts), cap, C.byref(n)),
+                "aln_batch_results")
+        k = n.value
+        return keys[:k], sums[:k], cnts[:k]
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.aln_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """The served form (aln_server_*): a request loop over device-resident tables with
+    continuous batching.  Columns passed to add_table must stay alive while it runs."""
+
+    def __init__(self, device=0, streams=2, max_batch=64, plain=False, budget_bytes=200 << 30,
+                 hot_groups=-1, blocks_per_cu=0, use_graph=False):
+        self.lib = L.load()
+        cfg = L.ServerConfig(streams, max_batch, 1 if plain else 0, 0, budget_bytes,
+                             L.BatchTuning(hot_groups, blocks_per_cu, 1 if use_graph else 0))
+        h = C.c_void_p()
+        L.check(self.lib.aln_server_create(device, C.byref(cfg), C.byref(h)), "aln_server_create")
+        self.h = h
+        self.tables = []
+
+    def add_table(self, cols):
+        arr = _cols_array(cols)
+        tid = C.c_int32()
+        L.check(self.lib.aln_server_add_table(self.h, arr, len(cols), cols[0].seg_rows, C.byref(tid)),
+                "aln_server_add_table")
+        self.tables.append(list(cols))
+        return tid.value
+
+    def submit(self, table, q):
+        bq = L.BatchQuery()
+        for k in ("pred_col", "key_col", "x_col", "y_col", "lo", "hi", "key_div", "a0", "a1", "b0", "b1"):
+            setattr(bq, k, int(q[k]))
+        rid = C.c_uint64()
+        L.check(self.lib.aln_server_submit(self.h, table, C.byref(bq), C.byref(rid)), "aln_server_submit")
+        return rid.value
+
+    def submit_star(self, dim, mid, fact, spec, limit=10):
+        a, keep = star_args(dim, mid, fact, spec, limit, 0)
+        rid = C.c_uint64()
+        L.check(self.lib.aln_server_submit_star(self.h, C.byref(a), C.byref(rid)), "aln_server_submit_star")
+        return rid.value
+
+    def wait(self, rid, check=True):
+        r = L.ServerResult()
+        rc = self.lib.aln_server_wait(self.h, rid, C.byref(r))
+        if check:
+            L.check(rc, "aln_server_wait")
+        return r
+
+    def fetch(self, rid):
+        r = self.wait(rid)
+        n = max(1, r.ngroups)
+        k, s, c = (np.empty(n, np.int64) for _ in range(3))
+        ptr = lambda a: a.ctypes.data_as(C.POINTER(C.c_int64))
+        L.check(self.lib.aln_server_fetch(self.h, rid, ptr(k), ptr(s), ptr(c), n), "aln_server_fetch")
+        g = r.ngroups
+        return k[:g], s[:g], c[:g]
+
+    def fetch_star(self, rid):
+        r = self.wait(rid)
+        cap = max(1, r.ngroups)
+        top = (L.StarRow * cap)()
+        n = C.c_uint32()
+        L.check(self.lib.aln_server_fetch_star(self.h, rid, top, cap, C.byref(n)), "aln_server_fetch_star")
+        return [(top[i].key, top[i].g1, top[i].g2, top[i].sum, top[i].count) for i in range(n.value)]
+
+    def release(self, rid):
+        L.check(self.lib.aln_server_release(self.h, rid), "aln_server_release")
+
+    def stats(self):
+        st = L.ServerStats()
+        L.check(self.lib.aln_server_stats(self.h, C.byref(st)), "aln_server_stats")
+        return {k: getattr(st, k) for k, _ in L.ServerStats._fields_}
+
+    def close(self):
+        if self.h:
+            self.lib.aln_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

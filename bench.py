@@ -114,6 +114,259 @@ def cpu_baseline(sample_scale, seed):
                         f"(oracle/liboracle.so, single thread) on {platform.processor() or platform.machine()}"))
 
 
+def batch_queries(seed, nq, nkeys, price_lo, price_span, ndays):
+    """nq filter + GROUP BY queries over the fact table [okey, price, disc, ship, zipf]:
+    selectivity log-uniform over [1%, 90%] (a price range, or every 4th query a ship-date
+    range), group cardinality log-uniform over [10, nkeys] (key = zipf // div), each
+    computing SUM(price * (1 - discount)) and COUNT(*)."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    qs = []
+    for q in range(nq):
+        sel = float(np.exp(rng.uniform(np.log(0.01), np.log(0.9))))
+        card = float(np.exp(rng.uniform(np.log(10), np.log(nkeys))))
+        div = max(1, int(round(nkeys / card)))
+        if q % 4 == 3:
+            w = max(1, int(sel * ndays))
+            d0 = int(rng.integers(0, ndays - w + 1))
+            lo, hi, pred = ymd(d0), ymd(d0 + w - 1), 3
+        else:
+            w = max(1, int(sel * price_span))
+            lo = price_lo + int(rng.integers(0, price_span - w + 1))
+            hi, pred = lo + w - 1, 1
+        qs.append(dict(pred_col=pred, key_col=4, x_col=1, y_col=2, lo=lo, hi=hi, key_div=div,
+                       a0=0, a1=1, b0=100, b1=-1, sel=sel, card=card))
+    return qs
+
+
+def ymd(day):
+    """yyyymmdd of `day` days after 1992-01-01."""
+    import datetime
+
+    d = datetime.date(1992, 1, 1) + datetime.timedelta(days=int(day))
+    return d.year * 10000 + d.month * 100 + d.day
+
+
+def plain_query(ctx, D, L, cols, q):
+    """One query through the per-statement operators: scan_filter -> eval -> group-by."""
+    p = q["pred_col"]
+    prog = [(L.I_COL, p, 1), (L.I_CONST, 0, q["lo"]), (L.I_CMP, 6, 0),
+            (L.I_COL, p, 1), (L.I_CONST, 0, q["hi"]), (L.I_CMP, 5, 0), (L.I_AND, 0, 0)]
+    rows = D.scan_filter(ctx, cols, prog)
+    key = D.eval_rows(ctx, cols, [(L.I_COL, q["key_col"], 1), (L.I_CONST, 0, q["key_div"]), (L.I_DIV, 0, 0)], rows)
+    rev = D.eval_rows(ctx, cols, [(L.I_COL, q["x_col"], 1), (L.I_CONST, 0, 100), (L.I_COL, q["y_col"], 1),
+                                  (L.I_SUB, 0, 0), (L.I_MUL, 0, 0)], rows)
+    return D.groupby(ctx, [key], [rev, rev], [L.AGG_SUM, L.AGG_COUNT])
+
+
+def batch_leg(ctx, D, fact, nfact, seed, nq, steps, sort_rows):
+    """The batched form: nq queries in one persistent launch (captured in a HIP graph, shape
+    chosen by the autotuner) against the same queries one at a time through the operators,
+    plus the ORDER BY of a sort_rows-row (int64 key, int32 payload) result."""
+    import numpy as np
+    import torch
+
+    from alenka_amd import _lib as L
+
+    nkeys = 10_000_000
+    t0 = time.time()
+    z = D.gen_zipf(ctx, seed, nfact, nkeys, 1.1)
+    zc = D.Column.encode(ctx, z, fact[0].seg_rows)
+    del z
+    torch.cuda.empty_cache()
+    cols = list(fact) + [zc]
+    qs = batch_queries(seed + 3, nq, nkeys, 90000, 10410001, 2557 + 121)
+    log(f"batch: zipf column built in {time.time() - t0:.1f}s; {nq} queries")
+    b = D.Batch(ctx, cols, qs, use_graph=True)
+    shape, tune_ms = b.autotune(reps=2, use_graph=True)
+    log(f"batch: autotuned hot_groups={shape[0]} blocks_per_cu={shape[1]} ({tune_ms:.2f} ms)")
+    b.run()
+    torch.cuda.synchronize()
+    lat = []
+    for _ in range(steps):
+        t1 = time.perf_counter()
+        b.run()
+        torch.cuda.synchronize()
+        lat.append(time.perf_counter() - t1)
+    # every query of the batch against its plain run: same groups, sums and counts
+    plain_lat, mismatches = [], 0
+    for i, q in enumerate(qs):
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        pk, pv = plain_query(ctx, D, L, cols, q)
+        torch.cuda.synchronize()
+        plain_lat.append(time.perf_counter() - t1)
+        bk, bs, bc = b.results(i, max(1, pk[0].numel()))
+        same = (bk.numel() == pk[0].numel() and torch.equal(bk, pk[0]) and torch.equal(bs, pv[0])
+                and torch.equal(bc, pv[1]))
+        mismatches += 0 if same else 1
+        del pk, pv, bk, bs, bc
+    b.close()
+    del cols
+    torch.cuda.empty_cache()
+    lat_ms = np.array(lat) * 1e3
+    plain_ms = np.array(plain_lat) * 1e3
+    batched_qps = nq / (lat_ms.mean() / 1e3)
+    plain_qps = nq / (plain_ms.sum() / 1e3)
+    # ORDER BY of a sort_rows-row result: int64 uniform key, int32 payload (row number)
+    keys, _ = D.gen_sort_input(ctx, seed, sort_rows, payload=False)
+    D.sort_keys(ctx, [keys])
+    torch.cuda.synchronize()
+    st = []
+    for _ in range(3):
+        t1 = time.perf_counter()
+        perm = D.sort_keys(ctx, [keys])
+        torch.cuda.synchronize()
+        st.append(time.perf_counter() - t1)
+    ok = bool((keys[perm.long()].diff() >= 0).all().item())
+    del keys, perm
+    torch.cuda.empty_cache()
+    sort_s = float(np.median(st))
+    return zc, {
+        "queries": nq,
+        "selectivity_range": [min(q["sel"] for q in qs), max(q["sel"] for q in qs)],
+        "group_cardinality_range": [min(q["card"] for q in qs), max(q["card"] for q in qs)],
+        "zipf_s": 1.1, "zipf_keys": nkeys,
+        "tuning": {"hot_groups": shape[0], "blocks_per_cu": shape[1], "graph": True},
+        "batched_qps": batched_qps,
+        "batched_ms_per_batch": {"mean": float(lat_ms.mean()), "p50": float(np.percentile(lat_ms, 50)),
+                                 "p99": float(np.percentile(lat_ms, 99))},
+        "plain_qps": plain_qps,
+        "plain_ms_per_query": {"mean": float(plain_ms.mean()), "p50": float(np.percentile(plain_ms, 50)),
+                               "p99": float(np.percentile(plain_ms, 99))},
+        "speedup_qps": batched_qps / plain_qps,
+        "results_match_plain": mismatches == 0,
+        "order_by": {"rows": sort_rows, "ms": sort_s * 1e3, "rows_per_s": sort_rows / sort_s,
+                     "algorithmic_gbs": 24 * sort_rows / sort_s / 1e9, "sorted": ok},
+    }
+
+
+def serve_tables(ctx, D, dim, mid, fact, zc, seed, scales):
+    """Device-resident tables of the served mix at each scale: flat tables of 1M x scale
+    rows, and prefixes (whole segments) of the 600M-row star tables: orders 1.5M x scale,
+    fact 6M x scale (with its Zipf column); the dimension table is used whole."""
+    import torch
+
+    seg = fact[0].seg_rows
+    flat, facts, stars = {}, {}, {}
+    for s in scales:
+        n = 1_000_000 * s
+        cols = []
+        for c, dec in zip(D.gen_flat(ctx, seed + s, n), (0, 0, 2, 2)):
+            cols.append(D.Column.encode(ctx, c, seg, decimals=dec))
+            del c
+        flat[s] = cols
+        nf = -(-6_000_000 * s // seg)
+        nm = -(-1_500_000 * s // seg)
+        facts[s] = [c.prefix(nf) for c in list(fact) + [zc]]
+        stars[s] = (dim, [c.prefix(nm) for c in mid], [c.prefix(nf) for c in fact])
+    torch.cuda.empty_cache()
+    return flat, facts, stars
+
+
+def serve_trace(seed, rate, duration, scales, code_building, nkeys):
+    """Poisson arrivals of requests drawn uniformly from the four templates (flat filter +
+    GROUP BY; the star pipeline at base and at compressed large scale; filter + GROUP BY
+    on the fact table with Zipf keys), each at a random scale."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    t, out = 0.0, []
+    while True:
+        t += float(rng.exponential(1.0 / rate))
+        if t >= duration:
+            return out
+        tmpl = int(rng.integers(0, 4))
+        s = int(scales[rng.integers(0, len(scales))])
+        if tmpl == 0:
+            day = int(rng.integers(365, 2557 - 365))
+            q = dict(pred_col=1, key_col=0, x_col=2, y_col=3, lo=0, hi=ymd(day) - 1, key_div=1,
+                     a0=0, a1=1, b0=100, b1=-1)
+        elif tmpl == 3:
+            q = batch_queries(int(rng.integers(1 << 30)), 1, nkeys, 90000, 10410001, 2557 + 121)[0]
+        else:
+            q = None
+        out.append((t, tmpl, s, q))
+
+
+def serve_run(D, srv, tids, stars, code_building, trace):
+    """Submits the trace at its arrival times from one client thread while a second
+    thread waits the requests in order; returns per-request latencies (ms) and the span."""
+    import threading
+
+    import numpy as np
+
+    spec = dict(d_key=0, m_key=0, m_fk=1, m_g1=2, m_g2=3, f_fk=0, f_x=1, f_y=2, a0=0, a1=1, b0=100, b1=-1)
+    ids = [None] * len(trace)
+    ready = [threading.Event() for _ in trace]
+    lat = np.zeros(len(trace))
+    status = np.zeros(len(trace), np.int64)
+    t_last = [0.0]
+
+    def waiter():
+        for i in range(len(trace)):
+            ready[i].wait()
+            r = srv.wait(ids[i], check=False)
+            lat[i], status[i] = r.latency_ms, r.status
+            srv.release(ids[i])
+            t_last[0] = time.perf_counter()
+
+    w = threading.Thread(target=waiter)
+    w.start()
+    t0 = time.perf_counter()
+    for i, (t, tmpl, s, q) in enumerate(trace):
+        d = t0 + t - time.perf_counter()
+        if d > 0:
+            time.sleep(d)
+        if tmpl in (1, 2):
+            dm, md, fc = stars[s]
+            ids[i] = srv.submit_star(dict(cols=dm, ranges=[(1, code_building, code_building)]),
+                                     dict(cols=md, ranges=[(2, -(1 << 63), DATE - 1)]),
+                                     dict(cols=fc, ranges=[(3, DATE + 1, (1 << 63) - 1)]), spec, limit=10)
+        else:
+            ids[i] = srv.submit(tids[(tmpl, s)], q)
+        ready[i].set()
+    w.join()
+    return lat, status, t_last[0] - t0
+
+
+def serve_leg(ctx, D, dim, mid, fact, zc, code_of, seed, rate, duration, budget_gb):
+    """The served form against serial plain execution on the same Poisson trace."""
+    import numpy as np
+    import torch
+
+    scales = [1, 10, 100]
+    flat, facts, stars = serve_tables(ctx, D, dim, mid, fact, zc, seed, scales)
+    b = int(code_of[SEGMENTS.index("BUILDING")])
+    trace = serve_trace(seed + 11, rate, duration, scales, b, 10_000_000)
+    out = {"rate_qps": rate, "duration_s": duration, "requests": len(trace), "scales": scales,
+           "budget_gb": budget_gb,
+           "mix": {k: sum(1 for x in trace if x[1] == i) for i, k in enumerate(("flat", "star", "star_large", "fact"))}}
+    for mode, kw in (("served", dict(streams=3, max_batch=64, plain=False)),
+                     ("plain", dict(streams=1, max_batch=1, plain=True))):
+        srv = D.Server(budget_bytes=int(budget_gb) << 30, **kw)
+        try:
+            tids = {}
+            for s in scales:
+                tids[(0, s)] = srv.add_table(flat[s])
+                tids[(3, s)] = srv.add_table(facts[s])
+            torch.cuda.synchronize()
+            lat, status, span = serve_run(D, srv, tids, stars, b, trace)
+            st = srv.stats()
+        finally:
+            srv.close()
+        ok = status == 0
+        out[mode] = {"qps": len(trace) / span, "p50_ms": float(np.percentile(lat, 50)),
+                     "p99_ms": float(np.percentile(lat, 99)), "mean_ms": float(lat.mean()),
+                     "failed": int((~ok).sum()), "batches": st["batches"],
+                     "batched_queries": st["batched_queries"], "max_concurrent": st["max_concurrent"]}
+        log(f"serve {mode}: {out[mode]}")
+    del flat, facts, stars
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -124,6 +377,14 @@ def main():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-scale", type=float, default=0.5)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-queries leg")
+    ap.add_argument("--batch-queries", type=int, default=64)
+    ap.add_argument("--batch-steps", type=int, default=5)
+    ap.add_argument("--sort-rows", type=int, default=100_000_000)
+    ap.add_argument("--no-serve", action="store_true", help="skip the served-mix leg")
+    ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--serve-seconds", type=float, default=8.0)
+    ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (collectives and "
@@ -179,6 +440,16 @@ def main():
         log("cpu baseline ...")
         cpu = cpu_baseline(args.cpu_sample_scale, args.seed)
 
+    batched = served = None
+    if not args.no_batch:
+        zc, batched = batch_leg(ctx, D, fact, nfact, args.seed, args.batch_queries, args.batch_steps,
+                                args.sort_rows)
+        log(f"batch: {batched['batched_qps']:.1f} q/s batched vs {batched['plain_qps']:.1f} q/s plain")
+        if not args.no_serve:
+            served = serve_leg(ctx, D, dim, mid, fact, zc, code_of, args.seed, args.serve_rate,
+                               args.serve_seconds, args.serve_budget_gb)
+        del zc
+
     value = nfact / (ms_per_step / 1000.0)
     out = {
         "metric": METRIC,
@@ -221,6 +492,8 @@ def main():
             "top1": list(rows[0]) if rows else None,
         },
         "cpu_baseline": cpu,
+        "batched": batched,
+        "served": served,
     }
     print(json.dumps(out), flush=True)
     ctx.close()

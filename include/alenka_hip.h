@@ -264,6 +264,98 @@ typedef struct {
 int aln_star_agg(aln_ctx *ctx, const aln_star_args *args, aln_star_row *h_top, uint32_t *h_ntop,
                  aln_star_row *d_groups, uint64_t groups_cap, aln_star_stats *h_stats);
 
+/* ---- batched filter + GROUP BY (shared scan) -------------------------------------------
+ * Up to ALN_BATCH_MAX queries over one table, each of the shape
+ *   F := FILTER T BY pred >= lo AND pred <= hi;            (emit_filter, operators.cu:2306)
+ *   R := SELECT key/key_div, SUM((a0 + a1*x) * (b0 + b1*y)), COUNT(*) FROM F GROUP BY ..;
+ *                                                           (emit_select, operators.cu:1801)
+ * answered by one scan of the table that reads every shared column once for all queries.
+ * Values are the columns' stored integers (decimals at their column scale); key/key_div is
+ * floor division; groups come back in key order, as one-segment GROUP BY returns them.
+ * There is no single reference counterpart: the reference runs each query as its own
+ * statements (a scan per query); aln_batch_* is the served form of those statements.
+ * A batch holds device tables sized from the key columns' zone maps (dense over
+ * [min/key_div, max/key_div], at most 2^31 groups per query) until aln_batch_destroy. */
+#define ALN_BATCH_MAX 64
+typedef struct {
+  int32_t pred_col, key_col, x_col, y_col; /* slots in the cols array */
+  int64_t lo, hi;                          /* inclusive bounds on pred */
+  int64_t key_div;                         /* >= 1 */
+  int64_t a0, a1, b0, b1;
+} aln_batch_query;
+typedef struct {
+  int32_t hot_groups;    /* groups [0, hot) of each query aggregated in LDS (0..32; <0: 32) */
+  int32_t blocks_per_cu; /* persistent grid size per compute unit (<=0: 2)                */
+  int32_t use_graph;     /* replay the launch sequence from a captured HIP graph            */
+} aln_batch_tuning;
+typedef struct aln_batch aln_batch;
+int aln_batch_create(aln_ctx *ctx, const aln_col *cols, int ncols, uint32_t seg_rows,
+                     const aln_batch_query *qs, int nq, const aln_batch_tuning *tune,
+                     aln_batch **out);
+int aln_batch_set_tuning(aln_batch *b, const aln_batch_tuning *tune);
+/* Enqueues one evaluation of every query on the context stream (no synchronisation). */
+int aln_batch_run(aln_batch *b);
+/* Groups of query q with at least one row (synchronises); ALN_E_CAPACITY if > cap. */
+int aln_batch_results(aln_batch *b, int q, int64_t *d_keys, int64_t *d_sums, int64_t *d_counts,
+                      uint64_t cap, uint64_t *h_ngroups);
+/* The same into host arrays. */
+int aln_batch_results_host(aln_batch *b, int q, int64_t *h_keys, int64_t *h_sums, int64_t *h_counts,
+                           uint64_t cap, uint64_t *h_ngroups);
+/* Times the candidate grid / hot-set shapes on the batch, applies and returns the fastest
+ * (best->use_graph is kept as passed in). */
+int aln_batch_autotune(aln_batch *b, int reps, aln_batch_tuning *best, double *h_ms);
+void aln_batch_destroy(aln_batch *b);
+
+/* ---- served form: request loop with continuous batching -------------------------------
+ * No single reference counterpart: the reference answers one script at a time (bison.cu
+ * main loop, main.cu:36-44 for its --QPS-test timing); this is the serving loop around the
+ * same operators.  `streams` worker threads each own a context on their own HIP stream;
+ * a worker that comes free takes the oldest waiting request and (unless `plain`) every
+ * other waiting query on the same table that fits `max_batch` and the memory budget, and
+ * answers them with one shared scan (aln_batch_*).  Requests whose device memory would
+ * exceed what the budget has free wait until running work releases it; a request larger
+ * than the whole budget fails with ALN_E_NOMEM.  In `plain` mode each query runs alone
+ * through aln_scan_filter -> aln_eval -> aln_groupby.  Tables (device columns) must stay
+ * alive while the server runs.  All entry points are thread-safe. */
+typedef struct {
+  int32_t streams;   /* concurrent workers (1..8)                            */
+  int32_t max_batch; /* queries per shared scan (<= ALN_BATCH_MAX; <=0: max) */
+  int32_t plain;     /* 1: no co-batching, operators one query at a time     */
+  int32_t reserved;
+  uint64_t budget_bytes; /* device bytes admitted requests may hold together  */
+  aln_batch_tuning tune;
+} aln_server_config;
+typedef struct {
+  int32_t status; /* ALN_OK or the request's error code */
+  int32_t kind;   /* 0 query, 1 star pipeline           */
+  int32_t batch_size;
+  int32_t reserved;
+  uint64_t batch; /* id of the batch that answered it   */
+  uint64_t ngroups; /* groups (query) or top rows (star) */
+  double queue_ms, run_ms, latency_ms; /* submit -> start -> done */
+} aln_server_result;
+typedef struct {
+  uint64_t submitted, completed, batches, batched_queries, max_concurrent, pending, mem_in_use;
+} aln_server_totals;
+typedef struct aln_server aln_server;
+int aln_server_create(int device, const aln_server_config *cfg, aln_server **out);
+/* Registers a table (the column descriptors are copied; the device data is not). */
+int aln_server_add_table(aln_server *s, const aln_col *cols, int ncols, uint32_t seg_rows,
+                         int32_t *h_table);
+int aln_server_submit(aln_server *s, int32_t table, const aln_batch_query *q, uint64_t *h_id);
+int aln_server_submit_star(aln_server *s, const aln_star_args *args, uint64_t *h_id);
+/* Blocks until the request is done; returns its status (message in aln_last_error). */
+int aln_server_wait(aln_server *s, uint64_t id, aln_server_result *h_res);
+int aln_server_fetch(aln_server *s, uint64_t id, int64_t *h_keys, int64_t *h_sums,
+                     int64_t *h_counts, uint64_t cap);
+int aln_server_fetch_star(aln_server *s, uint64_t id, aln_star_row *h_top, uint32_t cap,
+                          uint32_t *h_ntop);
+/* Forgets a finished request (waits for it first). */
+int aln_server_release(aln_server *s, uint64_t id);
+int aln_server_stats(aln_server *s, aln_server_totals *h_stats);
+/* Finishes every submitted request, then stops the workers. */
+void aln_server_destroy(aln_server *s);
+
 /* ---- synthetic workloads (no reference counterpart: the bench's seeded inputs) --------
  * Counter-based SplitMix64 streams; see datagen.hip for the exact specification.  All
  * output pointers are device int64 arrays and may be NULL to skip a column. */
@@ -275,6 +367,10 @@ int aln_gen_orders(aln_ctx *ctx, uint64_t seed, uint64_t n, uint64_t ndim, int64
 int aln_gen_fact_rows(aln_ctx *ctx, uint64_t seed, uint64_t norders, uint64_t *h_nfact);
 int aln_gen_fact(aln_ctx *ctx, uint64_t seed, uint64_t norders, int64_t *d_okey,
                  int64_t *d_price, int64_t *d_disc, int64_t *d_ship);
+/* Zipf(s) ranks in [0, nkeys) (rank 0 most frequent), by rejection-inversion. */
+int aln_gen_zipf(aln_ctx *ctx, uint64_t seed, uint64_t n, uint64_t nkeys, double s, int64_t *d_out);
+/* Uniform 64-bit keys and, when d_payload is not NULL, payload[i] = i. */
+int aln_gen_sort_input(aln_ctx *ctx, uint64_t seed, uint64_t n, int64_t *d_keys, uint32_t *d_payload);
 /* Dictionary codes in first-seen order (compress_char, cm.cu:1808-1821) for values that
  * index a string table [0, nvals): h_code_of_val[v] = code or -1; d_codes (optional) gets
  * the code of every row. */
