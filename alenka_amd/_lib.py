@@ -160,6 +160,8 @@ SIGNATURES = {
     "aln_server_release": (C.c_int, [P, U64]),
     "aln_server_stats": (C.c_int, [P, C.POINTER(ServerStats)]),
     "aln_server_destroy": (None, [P]),
+    "aln_star_agg_batch": (C.c_int, [P, C.POINTER(StarArgs), C.c_int, C.POINTER(C.POINTER(StarRow)), PU32,
+                                     C.POINTER(StarStats)]),
     "aln_gen_flat": (C.c_int, [P, U64, U64, P, P, P, P]),
     "aln_gen_dim": (C.c_int, [P, U64, U64, P, P]),
     "aln_gen_orders": (C.c_int, [P, U64, U64, U64, P, P, P, P]),

@@ -293,7 +293,7 @@ int batch_create(aln_ctx *c, const aln_col *cols, int ncols, uint32_t seg_rows, 
   };
   // every column must split into the same segments (row counts)
   std::vector<int64_t> cmin(ncols), cmax(ncols);
-  for (int col = ncols - 1; col >= 0; --col) {
+  for (int col = 0; col < ncols; ++col) {  // column 0 first: the others are checked against it
     int rc = range_of(col, &cmin[col], &cmax[col]);
     if (rc) return bail(rc);
     for (uint32_t s = 0; s < nsegs; ++s)

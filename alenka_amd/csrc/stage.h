@@ -14,29 +14,22 @@ constexpr int STG_ITEMS = 16;
 constexpr int STG_TILE = STG_BLOCK * STG_ITEMS;
 constexpr int STG_SLOTS = 2;
 constexpr int STG_MAX_BITS = 31;
-constexpr int STG_SLOT_WORDS = (STG_TILE * STG_MAX_BITS + 31) / 32 + 2;
+// Words a slot can hold: the 16-byte blocks covering one tile's codes at the widest staged
+// width (ceil(4096 * 31 / 32) + 1 words, plus up to 3 words of the first block before them).
+constexpr int STG_SLOT_WORDS = ((STG_TILE * STG_MAX_BITS + 31) / 32 + 1 + 3 + 3) / 4 * 4;
+constexpr int STG_VEC = 8;  // 16-byte loads a thread issues before it stores any
 
 struct StageSlot {
   int64_t base;
   uint32_t bits;
-  uint32_t bit0;  // bit offset of the tile's first code inside the first staged word
+  uint32_t bit0;  // bit offset of the tile's first code inside the slot's staged words
 };
 
-struct TileStage {
+struct __align__(16) TileStage {
   uint32_t words[STG_SLOTS][STG_SLOT_WORDS];
   StageSlot slot[STG_SLOTS];
   int8_t slot_of[ALN_MAX_COLS];
 };
-
-// Copies the words holding rows [r0, r0 + nt) of one segment into lds.
-__device__ __forceinline__ void stage_copy(const uint32_t *__restrict__ words, const aln_seg &sg,
-                                           uint32_t r0, uint32_t nt, uint32_t *lds) {
-  const uint64_t b0 = (uint64_t)r0 * sg.bits;
-  const uint64_t wbeg = b0 >> 5, wend = ((uint64_t)(r0 + nt) * sg.bits + 31) >> 5;
-  const uint32_t *p = words + sg.word_off + wbeg;
-  const uint32_t n = (uint32_t)(wend - wbeg);
-  for (uint32_t i = threadIdx.x; i < n; i += STG_BLOCK) lds[i] = p[i];
-}
 
 __device__ __forceinline__ int64_t stage_get(const uint32_t *lds, const StageSlot &s, uint32_t i) {
   if (s.bits == 0) return s.base;
@@ -48,12 +41,21 @@ __device__ __forceinline__ int64_t stage_get(const uint32_t *lds, const StageSlo
 }
 
 // Stages the listed columns (first STG_SLOTS with at most 31 bits); every thread of the
-// block must call it, followed by __syncthreads().
+// block must call it, followed by __syncthreads().  Each slot receives the aligned 16-byte
+// blocks that cover the tile's codes (an aligned block never crosses a page, so the few
+// words outside the column's range it may carry are readable), copied with up to STG_VEC
+// independent 16-byte loads per thread in flight before the first store: the block keeps
+// tens of KiB of loads outstanding instead of one dword per thread.
 __device__ __forceinline__ void stage_tile(TileStage &ts, const uint32_t *const *words,
                                            const aln_seg *const *segs, const int *want, int nwant,
                                            uint32_t seg, uint32_t r0, uint32_t nt) {
   if (threadIdx.x < ALN_MAX_COLS) ts.slot_of[threadIdx.x] = -1;
   __syncthreads();
+  static_assert(STG_SLOTS == 2, "the copy loop below addresses two slots");
+  // slot sources as scalars: a private array indexed by the slot number would live in
+  // scratch memory
+  const uint4 *src0 = nullptr, *src1 = nullptr;
+  uint32_t n0 = 0, n1 = 0;
   int used = 0;
   for (int k = 0; k < nwant && used < STG_SLOTS; ++k) {
     const int c = want[k];
@@ -62,14 +64,44 @@ __device__ __forceinline__ void stage_tile(TileStage &ts, const uint32_t *const 
     if (dup) continue;
     const aln_seg sg = segs[c][seg];
     if (sg.bits > STG_MAX_BITS) continue;
-    stage_copy(words[c], sg, r0, nt, ts.words[used]);
+    const uint64_t b0 = (uint64_t)r0 * sg.bits;
+    const uint64_t wbeg = b0 >> 5, wend = ((uint64_t)(r0 + nt) * sg.bits + 31) >> 5;
+    const uint32_t *p = words[c] + sg.word_off + wbeg;
+    const uint32_t a = (uint32_t)(((uintptr_t)p >> 2) & 3);  // words of the first block before p
+    const uint32_t nb = sg.bits ? (uint32_t)((a + (wend - wbeg) + 3) >> 2) : 0;
+    if (used == 0) {
+      src0 = (const uint4 *)(p - a);
+      n0 = nb;
+    } else {
+      src1 = (const uint4 *)(p - a);
+      n1 = nb;
+    }
     if (threadIdx.x == 0) {
       ts.slot[used].base = sg.base;
       ts.slot[used].bits = sg.bits;
-      ts.slot[used].bit0 = (uint32_t)(((uint64_t)r0 * sg.bits) & 31);
+      ts.slot[used].bit0 = (uint32_t)(b0 & 31) + 32 * a;
       ts.slot_of[c] = (int8_t)used;
     }
     ++used;
+  }
+  // one index space over the blocks of both slots: v < n0 is slot 0, then slot 1; two
+  // slots of at most STG_SLOT_WORDS / 4 blocks fit one round of STG_VEC loads per thread
+  static_assert(2 * (STG_SLOT_WORDS / 4) <= STG_VEC * STG_BLOCK, "one round of loads covers both slots");
+  uint4 *dst0 = reinterpret_cast<uint4 *>(ts.words[0]), *dst1 = reinterpret_cast<uint4 *>(ts.words[1]);
+  const uint32_t ntot = n0 + n1;
+  uint4 r[STG_VEC];
+#pragma unroll
+  for (int k = 0; k < STG_VEC; ++k) {
+    const uint32_t v = k * STG_BLOCK + threadIdx.x;
+    r[k] = v < n0 ? src0[v] : (v < ntot ? src1[v - n0] : make_uint4(0, 0, 0, 0));
+  }
+#pragma unroll
+  for (int k = 0; k < STG_VEC; ++k) {
+    const uint32_t v = k * STG_BLOCK + threadIdx.x;
+    if (v < n0)
+      dst0[v] = r[k];
+    else if (v < ntot)
+      dst1[v - n0] = r[k];
   }
 }
 

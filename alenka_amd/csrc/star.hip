@@ -20,6 +20,7 @@
 // every range skips predicate evaluation ('A').
 #include <string.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "aln_internal.h"
@@ -173,23 +174,46 @@ __global__ void __launch_bounds__(ST_BLOCK) k_star_mid(ScanDesc d, int kcol, int
   const uint32_t nt = nrows - r0 < (uint32_t)ST_TILE ? nrows - r0 : (uint32_t)ST_TILE;
   star_stage(ts, want, d, mode, fkcol, seg, r0, nt);
   const unsigned lane = __lane_id(), wid = threadIdx.x / 64;
-  uint32_t passmask = 0;
-  uint32_t mine = 0, nprobe = 0;
+  // predicate and foreign key from LDS for all items, then the dimension-bitmap words, the
+  // keys of the joined rows and the mid-bitmap updates, each issued for all items at once
+  uint64_t dc[ST_ITEMS];
+  uint32_t nprobe = 0;
+#pragma unroll
   for (int j = 0; j < ST_ITEMS; ++j) {
-    uint32_t i = j * ST_BLOCK + threadIdx.x;
-    bool p = i < nt && tile_pass(d, ts, mode, seg, r0, i);
-    if (p) {
+    const uint32_t i = j * ST_BLOCK + threadIdx.x;
+    dc[j] = ~0ull;
+    if (i < nt && tile_pass(d, ts, mode, seg, r0, i)) {
       ++nprobe;
-      uint64_t b = (uint64_t)tile_get(ts, d.words, d.segs, fkcol, seg, r0, i) - (uint64_t)dkmin;
-      p = b < drange && ((bitmap[b >> 5] >> (b & 31)) & 1);
+      const uint64_t b = (uint64_t)tile_get(ts, d.words, d.segs, fkcol, seg, r0, i) - (uint64_t)dkmin;
+      if (b < drange) dc[j] = b;
     }
-    if (p && mbits) {
-      uint64_t code = (uint64_t)tile_get(ts, d.words, d.segs, kcol, seg, r0, i) - (uint64_t)mkmin;
-      uint32_t bit = 1u << (code & 31);
-      if (atomicOr(&mbits[code >> 5], bit) & bit) atomicExch(dup, 1u);
-    }
+  }
+  uint32_t dw[ST_ITEMS];
+#pragma unroll
+  for (int j = 0; j < ST_ITEMS; ++j) dw[j] = dc[j] != ~0ull ? bitmap[dc[j] >> 5] : 0u;
+  uint32_t passmask = 0, mine = 0;
+#pragma unroll
+  for (int j = 0; j < ST_ITEMS; ++j) {
+    const bool p = dc[j] != ~0ull && ((dw[j] >> (dc[j] & 31)) & 1);
     passmask |= (uint32_t)p << j;
     mine += p;
+  }
+  int64_t key[ST_ITEMS];
+#pragma unroll
+  for (int j = 0; j < ST_ITEMS; ++j)
+    key[j] = ((passmask >> j) & 1) ? tile_get(ts, d.words, d.segs, kcol, seg, r0, j * ST_BLOCK + threadIdx.x) : 0;
+  if (mbits) {
+    uint32_t old[ST_ITEMS], bit[ST_ITEMS];
+#pragma unroll
+    for (int j = 0; j < ST_ITEMS; ++j) {
+      const uint64_t code = (uint64_t)key[j] - (uint64_t)mkmin;
+      bit[j] = ((passmask >> j) & 1) ? 1u << (code & 31) : 0u;
+      old[j] = bit[j] ? atomicOr(&mbits[code >> 5], bit[j]) : 0u;
+    }
+    bool twice = false;
+#pragma unroll
+    for (int j = 0; j < ST_ITEMS; ++j) twice = twice || (old[j] & bit[j]);
+    if (twice) atomicExch(dup, 1u);
   }
   block_count(nprobe, probed);
   uint32_t wsum = mine;
@@ -210,10 +234,11 @@ __global__ void __launch_bounds__(ST_BLOCK) k_star_mid(ScanDesc d, int kcol, int
     if (lane >= (unsigned)o) x += y;
   }
   pos += x - mine;
+#pragma unroll
   for (int j = 0; j < ST_ITEMS; ++j) {
     if (!((passmask >> j) & 1)) continue;
     uint32_t i = j * ST_BLOCK + threadIdx.x;
-    okey[pos] = tile_get(ts, d.words, d.segs, kcol, seg, r0, i);
+    okey[pos] = key[j];
     og1[pos] = g1col >= 0 ? tile_get(ts, d.words, d.segs, g1col, seg, r0, i) : 0;
     og2[pos] = g2col >= 0 ? tile_get(ts, d.words, d.segs, g2col, seg, r0, i) : 0;
     ++pos;
@@ -359,12 +384,21 @@ __global__ void __launch_bounds__(256) k_bm_block_counts(const uint32_t *__restr
   }
 }
 
+// Rank of a set bit: the block's directory entry plus the bits below it in its 512-bit block,
+// read as four independent 16-byte loads (the bitmap is allocated in whole blocks).
 __device__ __forceinline__ uint64_t bm_rank(const uint32_t *__restrict__ bits,
                                             const uint64_t *__restrict__ rank, uint64_t code) {
-  uint64_t w = code >> 5, blk = code >> 9;
+  const uint64_t blk = code >> 9;
+  const uint4 *q = (const uint4 *)(bits + blk * BM_BLOCK_WORDS);
+  const uint4 q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
+  const uint32_t w[BM_BLOCK_WORDS] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w,
+                                      q2.x, q2.y, q2.z, q2.w, q3.x, q3.y, q3.z, q3.w};
+  const uint32_t wi = (uint32_t)(code >> 5) & (BM_BLOCK_WORDS - 1), sh = (uint32_t)code & 31;
   uint64_t r = rank[blk];
-  for (uint64_t k = blk * BM_BLOCK_WORDS; k < w; ++k) r += __popc(bits[k]);
-  return r + __popc(bits[w] & ((1u << (code & 31)) - 1u));
+#pragma unroll
+  for (uint32_t k = 0; k < (uint32_t)BM_BLOCK_WORDS; ++k)
+    r += __popc(w[k] & (k < wi ? 0xFFFFFFFFu : (k == wi ? (1u << sh) - 1u : 0u)));
+  return r;
 }
 
 // One block per mid tile region (see k_star_mid); entry i moves to position rank(key).
@@ -396,8 +430,14 @@ __global__ void __launch_bounds__(ST_BLOCK) k_star_fact_bm(ScanDesc d, int fkcol
                                                            unsigned int *__restrict__ cnt,
                                                            uint32_t *__restrict__ matched,
                                                            uint32_t *__restrict__ probed) {
+  // After the probe the staged words are dead: the tile's hits (bitmap code, row) are
+  // compacted into the same LDS, in row order, and joined and aggregated from there.
+  static_assert(sizeof(TileStage) >= ST_TILE * (sizeof(uint32_t) + sizeof(uint16_t)),
+                "hit list must fit in the staging area");
   __shared__ TileStage ts;
   __shared__ int want[ALN_MAX_RANGES + 1];
+  __shared__ uint32_t woff[ST_ITEMS][ST_BLOCK / 64];
+  __shared__ uint32_t nhit_s;
   const uint32_t seg = blockIdx.x / d.tps, tile = blockIdx.x % d.tps;
   const uint32_t nrows = d.segs[0][seg].nrows, r0 = tile * ST_TILE;
   const char mode = sd_mode(d, seg);
@@ -410,40 +450,87 @@ __global__ void __launch_bounds__(ST_BLOCK) k_star_fact_bm(ScanDesc d, int fkcol
   }
   const uint32_t nt = nrows - r0 < (uint32_t)ST_TILE ? nrows - r0 : (uint32_t)ST_TILE;
   star_stage(ts, want, d, mode, fkcol, seg, r0, nt);
-  const unsigned lane = __lane_id();
-  uint32_t nmatch = 0, nprobe = 0;
+  const unsigned lane = __lane_id(), wave = threadIdx.x / 64;
+  // 1. predicate and key from LDS for all items, then every bitmap word load at once
+  uint32_t code[ST_ITEMS];
+  uint32_t nprobe = 0;
+#pragma unroll
   for (int j = 0; j < ST_ITEMS; ++j) {
-    uint32_t i = j * ST_BLOCK + threadIdx.x;
-    uint32_t local = r0 + i;
-    uint32_t g = 0xFFFFFFFFu;
-    int64_t v = 0;
+    const uint32_t i = j * ST_BLOCK + threadIdx.x;
+    code[j] = 0xFFFFFFFFu;
     if (i < nt && tile_pass(d, ts, mode, seg, r0, i)) {
       ++nprobe;
-      uint64_t code = (uint64_t)tile_get(ts, d.words, d.segs, fkcol, seg, r0, i) - (uint64_t)mkmin;
-      if (code < mrange && ((bits[code >> 5] >> (code & 31)) & 1)) {
-        g = (uint32_t)bm_rank(bits, rank, code);
-        int64_t x = sd_decode(d, xcol, seg, local), y = sd_decode(d, ycol, seg, local);
-        v = wmul((int64_t)((uint64_t)a0 + (uint64_t)wmul(a1, x)),
-                 (int64_t)((uint64_t)b0 + (uint64_t)wmul(b1, y)));
+      const uint64_t c = (uint64_t)tile_get(ts, d.words, d.segs, fkcol, seg, r0, i) - (uint64_t)mkmin;
+      if (c < mrange) code[j] = (uint32_t)c;  // mrange <= 2^31 on this path
+    }
+  }
+  uint32_t word[ST_ITEMS];
+#pragma unroll
+  for (int j = 0; j < ST_ITEMS; ++j) word[j] = code[j] != 0xFFFFFFFFu ? bits[code[j] >> 5] : 0u;
+  uint32_t hits = 0;
+#pragma unroll
+  for (int j = 0; j < ST_ITEMS; ++j)
+    hits |= (uint32_t)(code[j] != 0xFFFFFFFFu && ((word[j] >> (code[j] & 31)) & 1)) << j;
+  // 2. ordered compaction: rows are item-major, then wave, then lane
+#pragma unroll
+  for (int j = 0; j < ST_ITEMS; ++j) {
+    const uint64_t b = __ballot((hits >> j) & 1);
+    if (lane == 0) woff[j][wave] = (uint32_t)__popcll(b);
+  }
+  __syncthreads();  // every wave is also done reading the staged words
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int j = 0; j < ST_ITEMS; ++j)
+      for (int w = 0; w < ST_BLOCK / 64; ++w) {
+        const uint32_t c = woff[j][w];
+        woff[j][w] = t;
+        t += c;
       }
+    nhit_s = t;
+  }
+  __syncthreads();
+  uint32_t *hcode = reinterpret_cast<uint32_t *>(&ts);
+  uint16_t *hrow = reinterpret_cast<uint16_t *>(hcode + ST_TILE);
+#pragma unroll
+  for (int j = 0; j < ST_ITEMS; ++j) {
+    const bool h = (hits >> j) & 1;
+    const uint64_t b = __ballot(h);
+    if (h) {
+      const uint32_t pos = woff[j][wave] + (uint32_t)__popcll(b & lanemask_lt());
+      hcode[pos] = code[j];
+      hrow[pos] = (uint16_t)(j * ST_BLOCK + threadIdx.x);
+    }
+  }
+  __syncthreads();
+  // 3. group index (rank of the key), payload and a wave run merge of equal groups
+  const uint32_t nh = nhit_s;
+  uint32_t nmatch = 0;
+  for (uint32_t e0 = 0; e0 < nh; e0 += ST_BLOCK) {
+    const uint32_t e = e0 + threadIdx.x;
+    uint32_t g = 0xFFFFFFFFu;
+    int64_t v = 0;
+    if (e < nh) {
+      const uint32_t local = r0 + hrow[e];
+      g = (uint32_t)bm_rank(bits, rank, hcode[e]);
+      const int64_t x = sd_decode(d, xcol, seg, local), y = sd_decode(d, ycol, seg, local);
+      v = wmul((int64_t)((uint64_t)a0 + (uint64_t)wmul(a1, x)), (int64_t)((uint64_t)b0 + (uint64_t)wmul(b1, y)));
     }
     if (__ballot(g != 0xFFFFFFFFu) == 0) continue;
-    uint32_t prev = __shfl_up(g, 1, 64);
+    const uint32_t prev = __shfl_up(g, 1, 64);
     bool f = lane == 0 || prev != g;
     uint32_t c = g != 0xFFFFFFFFu;
     for (int o = 1; o < 64; o <<= 1) {
-      int64_t vu = __shfl_up(v, o, 64);
-      uint32_t cu = __shfl_up(c, o, 64);
-      bool fu = __shfl_up((int)f, o, 64) != 0;
+      const int64_t vu = __shfl_up(v, o, 64);
+      const uint32_t cu = __shfl_up(c, o, 64);
+      const bool fu = __shfl_up((int)f, o, 64) != 0;
       if (lane >= (unsigned)o && !f) {
         v = (int64_t)((uint64_t)v + (uint64_t)vu);
         c += cu;
       }
       if (lane >= (unsigned)o) f = f || fu;
     }
-    uint32_t next = __shfl_down(g, 1, 64);
-    bool tail = g != 0xFFFFFFFFu && (lane == 63 || next != g);
-    if (tail) {
+    const uint32_t next = __shfl_down(g, 1, 64);
+    if (g != 0xFFFFFFFFu && (lane == 63 || next != g)) {
       atomicAdd(&sum[g], (unsigned long long)v);
       atomicAdd(&cnt[g], c);
       nmatch += c;
@@ -603,6 +690,43 @@ static int make_scan(aln_ctx *c, const aln_table_scan &t, int need_col, ScanDesc
   return ALN_OK;
 }
 
+// The first k groups by (sum DESC, g1 ASC, key ASC, g2 ASC): tiles of 1024 candidates are
+// sorted and cut to k, repeatedly, until one tile remains (synchronises).
+static int star_topk(aln_ctx *c, uint64_t m, const unsigned long long *sum, const unsigned int *cnt,
+                     const int64_t *okey, const int64_t *og1, const int64_t *og2, uint32_t k,
+                     aln_star_row *h_top, uint32_t *h_ntop) {
+  const Cand *in = nullptr;
+  uint64_t n = m;
+  while (true) {
+    uint64_t nt = (n + TK_TILE2 - 1) / TK_TILE2;
+    Cand *out;
+    ALN_TRY(ws(c, nt * k, &out));
+    {
+      ProfScope p(c, "star_topk");
+      hipLaunchKernelGGL(k_star_topk_tile, dim3((unsigned)nt), dim3(256), 0, c->stream, in, n, sum, cnt, okey,
+                         og1, og2, k, out);
+    }
+    ALN_TRY(check_launch("star_topk"));
+    in = out;
+    n = nt * k;
+    if (nt == 1) break;
+  }
+  std::vector<Cand> hcand(k);
+  ALN_TRY(copy_to_host(c, hcand.data(), in, sizeof(Cand) * k));
+  uint32_t nt = 0;
+  for (uint32_t i = 0; i < k; ++i) {
+    if (!hcand[i].valid) break;
+    h_top[nt].key = hcand[i].key;
+    h_top[nt].g1 = hcand[i].g1;
+    h_top[nt].g2 = hcand[i].g2;
+    h_top[nt].sum = hcand[i].sum;
+    h_top[nt].count = hcand[i].count;
+    ++nt;
+  }
+  *h_ntop = nt;
+  return ALN_OK;
+}
+
 struct PhaseTimer {
   aln_ctx *c;
   hipEvent_t e[5];
@@ -674,9 +798,9 @@ extern "C" int aln_star_agg(aln_ctx *c, const aln_star_args *a, aln_star_row *h_
   uint64_t mwords = use_bm ? (mrange + 31) / 32 : 0;
   uint64_t mblk = use_bm ? (mwords + BM_BLOCK_WORDS - 1) / BM_BLOCK_WORDS : 0;
   uint32_t *mbits = nullptr;
-  if (use_bm) {
-    ALN_TRY(ws(c, mwords, &mbits));
-    ALN_TRY(fill_u32(c, mbits, mwords, 0));
+  if (use_bm) {  // whole 512-bit blocks: the rank reads a block with four 16-byte loads
+    ALN_TRY(ws(c, mblk * BM_BLOCK_WORDS, &mbits));
+    ALN_TRY(fill_u32(c, mbits, mblk * BM_BLOCK_WORDS, 0));
   }
   uint64_t bwords = (drange + 31) / 32;
   uint32_t *bitmap, *dup;
@@ -791,38 +915,7 @@ extern "C" int aln_star_agg(aln_ctx *c, const aln_star_args *a, aln_star_row *h_
   tm.mark(3);
 
   // top-k
-  uint32_t k = a->limit;
-  if (k && m) {
-    const Cand *in = nullptr;
-    uint64_t n = m;
-    while (true) {
-      uint64_t nt = (n + TK_TILE2 - 1) / TK_TILE2;
-      Cand *out;
-      ALN_TRY(ws(c, nt * k, &out));
-      {
-        ProfScope p(c, "star_topk");
-        hipLaunchKernelGGL(k_star_topk_tile, dim3((unsigned)nt), dim3(256), 0, c->stream, in, n, sum, cnt,
-                           okey, og1, og2, k, out);
-      }
-      ALN_TRY(check_launch("star_topk"));
-      in = out;
-      n = nt * k;
-      if (nt == 1) break;
-    }
-    std::vector<Cand> hcand(k);
-    ALN_TRY(copy_to_host(c, hcand.data(), in, sizeof(Cand) * k));
-    uint32_t nt = 0;
-    for (uint32_t i = 0; i < k; ++i) {
-      if (!hcand[i].valid) break;
-      h_top[nt].key = hcand[i].key;
-      h_top[nt].g1 = hcand[i].g1;
-      h_top[nt].g2 = hcand[i].g2;
-      h_top[nt].sum = hcand[i].sum;
-      h_top[nt].count = hcand[i].count;
-      ++nt;
-    }
-    *h_ntop = nt;
-  }
+  if (a->limit && m) ALN_TRY(star_topk(c, m, sum, cnt, okey, og1, og2, a->limit, h_top, h_ntop));
   tm.mark(4);
   ALN_TRY(copy_to_host(c, hc, ctr, 40));
   uint32_t bdup = 0;
@@ -868,5 +961,557 @@ extern "C" int aln_star_agg(aln_ctx *c, const aln_star_args *a, aln_star_row *h_
   st.ms_fact = tm.ms(2, 3);
   st.ms_topk = tm.ms(3, 4);
   if (h_stats) *h_stats = st;
+  return ALN_OK;
+}
+
+// ---- batched star pipelines ----------------------------------------------------------------
+// Up to ALN_STAR_BATCH_MAX q3-shaped queries over the same three tables, differing in their
+// range predicates and aggregate coefficients (TPC-H q3's substitution parameters), answered
+// with one scan of each table: every tile is staged once and evaluated for every query.
+// Each query keeps its own dimension bitmap, mid bitmap + rank directory and accumulators;
+// the group attributes are written by a second mid scan once the ranks are known.
+namespace aln {
+
+constexpr int SB_MAX = ALN_STAR_BATCH_MAX;
+constexpr int SB_CHUNK = 4;  // items per hit-list pass of the fact kernel (1024 rows)
+
+struct SBRanges {
+  int32_t nr;
+  int32_t col[ALN_MAX_RANGES];
+  int64_t lo[ALN_MAX_RANGES], hi[ALN_MAX_RANGES];
+};
+
+struct SBQuery {
+  SBRanges r[3];  // dimension, mid, fact
+  int64_t a0, a1, b0, b1;
+  uint32_t *dbm;    // dimension bitmap over [dkmin, dkmin + drange)
+  uint32_t *mbits;  // mid bitmap, whole 512-bit blocks
+  uint64_t *rank;   // per-block directory of mbits
+  unsigned long long *sum;
+  unsigned int *cnt;
+  int64_t *okey, *og1, *og2;
+};
+
+struct SBScan {
+  const uint32_t *words[ALN_MAX_COLS];
+  const aln_seg *segs[ALN_MAX_COLS];
+  uint32_t tps;
+  int want[2];  // staged columns
+  int nwant;
+};
+
+__device__ __forceinline__ char sb_mode(const SBScan &t, const SBRanges &r, uint32_t seg) {
+  char m = 'A';
+  for (int k = 0; k < r.nr; ++k) {
+    const aln_seg &s = t.segs[r.col[k]][seg];
+    if (s.vmax < r.lo[k] || s.vmin > r.hi[k]) return 'N';
+    if (!(s.vmin >= r.lo[k] && s.vmax <= r.hi[k])) m = 'R';
+  }
+  return m;
+}
+
+__device__ __forceinline__ bool sb_pass(const SBScan &t, const TileStage &ts, const SBRanges &r, char mode,
+                                        uint32_t seg, uint32_t r0, uint32_t i) {
+  if (mode == 'A') return true;
+  bool p = true;
+  for (int k = 0; k < r.nr; ++k) {
+    const int64_t v = tile_get(ts, t.words, t.segs, r.col[k], seg, r0, i);
+    p = p && v >= r.lo[k] && v <= r.hi[k];
+  }
+  return p;
+}
+
+// Block prologue: zone-map mode of the segment for every query and the tile staged in LDS
+// (skipped when no query reads the segment).  Returns false when the block has no work.
+__device__ __forceinline__ bool sb_prologue(const SBScan &t, const SBQuery *q, int nq, int table,
+                                            TileStage &ts, char *qmode, uint32_t seg, uint32_t r0,
+                                            uint32_t nt) {
+  __shared__ int any;
+  __shared__ int want[2];
+  if (threadIdx.x == 0) any = 0;
+  if (threadIdx.x < 2) want[threadIdx.x] = t.want[threadIdx.x];
+  __syncthreads();
+  if (threadIdx.x < (unsigned)nq) {
+    qmode[threadIdx.x] = sb_mode(t, q[threadIdx.x].r[table], seg);
+    if (qmode[threadIdx.x] != 'N') atomicOr(&any, 1);
+  }
+  __syncthreads();
+  if (!any) return false;
+  stage_tile(ts, t.words, t.segs, want, t.nwant, seg, r0, nt);
+  __syncthreads();
+  return true;
+}
+
+__global__ void __launch_bounds__(ST_BLOCK) k_sb_dim(SBScan t, const SBQuery *__restrict__ q, int nq, int kcol,
+                                                     int64_t kmin, uint64_t range, uint32_t *npass,
+                                                     uint32_t *dup) {
+  __shared__ TileStage ts;
+  __shared__ char qmode[SB_MAX];
+  const uint32_t seg = blockIdx.x / t.tps, tile = blockIdx.x % t.tps;
+  const uint32_t nrows = t.segs[0][seg].nrows, r0 = tile * ST_TILE;
+  const uint32_t nt = r0 < nrows ? (nrows - r0 < (uint32_t)ST_TILE ? nrows - r0 : (uint32_t)ST_TILE) : 0;
+  if (!nt || !sb_prologue(t, q, nq, 0, ts, qmode, seg, r0, nt)) {
+    if (threadIdx.x < (unsigned)nq) npass[(uint64_t)threadIdx.x * gridDim.x + blockIdx.x] = 0;
+    return;
+  }
+  for (int qi = 0; qi < nq; ++qi) {
+    const char mode = qmode[qi];
+    uint32_t cnt = 0;
+    bool twice = false;
+    if (mode != 'N') {
+      const SBQuery &qq = q[qi];
+#pragma unroll 4
+      for (int j = 0; j < ST_ITEMS; ++j) {
+        const uint32_t i = j * ST_BLOCK + threadIdx.x;
+        if (i < nt && sb_pass(t, ts, qq.r[0], mode, seg, r0, i)) {
+          const uint64_t b = (uint64_t)tile_get(ts, t.words, t.segs, kcol, seg, r0, i) - (uint64_t)kmin;
+          if (b < range) {
+            const uint32_t bit = 1u << (b & 31);
+            twice = twice || (atomicOr(&qq.dbm[b >> 5], bit) & bit);
+            ++cnt;
+          }
+        }
+      }
+    }
+    if (twice) atomicExch(&dup[qi], 1u);
+    block_count(cnt, npass + (uint64_t)qi * gridDim.x);
+  }
+}
+
+// Mid scan 1: ranges + dimension-bitmap probe per query; passing keys set the query's mid
+// bitmap.  Counts per query: probes (rows passing the ranges) and passes.
+__global__ void __launch_bounds__(ST_BLOCK) k_sb_mid(SBScan t, const SBQuery *__restrict__ q, int nq, int kcol,
+                                                     int fkcol, int64_t dkmin, uint64_t drange, int64_t mkmin,
+                                                     uint64_t mrange, uint32_t *nprobe_out, uint32_t *npass_out,
+                                                     uint32_t *dup) {
+  __shared__ TileStage ts;
+  __shared__ char qmode[SB_MAX];
+  const uint32_t seg = blockIdx.x / t.tps, tile = blockIdx.x % t.tps;
+  const uint32_t nrows = t.segs[0][seg].nrows, r0 = tile * ST_TILE;
+  const uint32_t nt = r0 < nrows ? (nrows - r0 < (uint32_t)ST_TILE ? nrows - r0 : (uint32_t)ST_TILE) : 0;
+  if (!nt || !sb_prologue(t, q, nq, 1, ts, qmode, seg, r0, nt)) {
+    if (threadIdx.x < (unsigned)nq) {
+      nprobe_out[(uint64_t)threadIdx.x * gridDim.x + blockIdx.x] = 0;
+      npass_out[(uint64_t)threadIdx.x * gridDim.x + blockIdx.x] = 0;
+    }
+    return;
+  }
+  // foreign key (LDS) and key (HBM; only the rows of a tile, coalesced) of every row, once
+  uint64_t fkc[ST_ITEMS];
+  int64_t key[ST_ITEMS];
+#pragma unroll
+  for (int j = 0; j < ST_ITEMS; ++j) {
+    const uint32_t i = j * ST_BLOCK + threadIdx.x;
+    fkc[j] = i < nt ? (uint64_t)tile_get(ts, t.words, t.segs, fkcol, seg, r0, i) - (uint64_t)dkmin : ~0ull;
+    key[j] = i < nt ? tile_get(ts, t.words, t.segs, kcol, seg, r0, i) : 0;
+  }
+  for (int qi = 0; qi < nq; ++qi) {
+    const char mode = qmode[qi];
+    uint32_t np = 0, npass = 0;
+    bool twice = false;
+    if (mode != 'N') {
+      const SBQuery &qq = q[qi];
+      uint32_t w[ST_ITEMS];
+      uint32_t probe = 0;
+#pragma unroll
+      for (int j = 0; j < ST_ITEMS; ++j) {
+        const uint32_t i = j * ST_BLOCK + threadIdx.x;
+        const bool p = i < nt && sb_pass(t, ts, qq.r[1], mode, seg, r0, i);
+        np += p;
+        probe |= (uint32_t)(p && fkc[j] < drange) << j;
+      }
+#pragma unroll
+      for (int j = 0; j < ST_ITEMS; ++j) w[j] = ((probe >> j) & 1) ? qq.dbm[fkc[j] >> 5] : 0u;
+#pragma unroll
+      for (int j = 0; j < ST_ITEMS; ++j)
+        if (((probe >> j) & 1) && ((w[j] >> (fkc[j] & 31)) & 1)) {
+          const uint64_t code = (uint64_t)key[j] - (uint64_t)mkmin;
+          if (code < mrange) {
+            const uint32_t bit = 1u << (code & 31);
+            twice = twice || (atomicOr(&qq.mbits[code >> 5], bit) & bit);
+            ++npass;
+          }
+        }
+    }
+    if (twice) atomicExch(&dup[qi], 1u);
+    block_count(np, nprobe_out + (uint64_t)qi * gridDim.x);
+    block_count(npass, npass_out + (uint64_t)qi * gridDim.x);
+  }
+}
+
+// Mid scan 2: the rows whose key bit is set in a query's mid bitmap are that query's groups;
+// their key and group columns go to position rank(key).
+__global__ void __launch_bounds__(ST_BLOCK) k_sb_attr(SBScan t, const SBQuery *__restrict__ q, int nq, int kcol,
+                                                      int g1col, int g2col, int64_t mkmin, uint64_t mrange) {
+  __shared__ TileStage ts;
+  __shared__ int want[2];
+  const uint32_t seg = blockIdx.x / t.tps, tile = blockIdx.x % t.tps;
+  const uint32_t nrows = t.segs[0][seg].nrows, r0 = tile * ST_TILE;
+  if (r0 >= nrows) return;
+  const uint32_t nt = nrows - r0 < (uint32_t)ST_TILE ? nrows - r0 : (uint32_t)ST_TILE;
+  if (threadIdx.x < 2) want[threadIdx.x] = t.want[threadIdx.x];
+  __syncthreads();
+  stage_tile(ts, t.words, t.segs, want, t.nwant, seg, r0, nt);
+  __syncthreads();
+  // items in chunks (the rank of a hit reads a 64-byte line: a whole tile of hits in flight
+  // would need more registers than a wave has)
+  for (int ch = 0; ch < ST_ITEMS / SB_CHUNK; ++ch) {
+    uint32_t code[SB_CHUNK];
+#pragma unroll
+    for (int u = 0; u < SB_CHUNK; ++u) {
+      const uint32_t i = (ch * SB_CHUNK + u) * ST_BLOCK + threadIdx.x;
+      const uint64_t c = i < nt ? (uint64_t)tile_get(ts, t.words, t.segs, kcol, seg, r0, i) - (uint64_t)mkmin : ~0ull;
+      code[u] = c < mrange ? (uint32_t)c : 0xFFFFFFFFu;
+    }
+    for (int qi = 0; qi < nq; ++qi) {
+      const SBQuery &qq = q[qi];
+      uint32_t w[SB_CHUNK];
+#pragma unroll
+      for (int u = 0; u < SB_CHUNK; ++u) w[u] = code[u] != 0xFFFFFFFFu ? qq.mbits[code[u] >> 5] : 0u;
+#pragma unroll
+      for (int u = 0; u < SB_CHUNK; ++u)
+        if (code[u] != 0xFFFFFFFFu && ((w[u] >> (code[u] & 31)) & 1)) {
+          const uint32_t i = (ch * SB_CHUNK + u) * ST_BLOCK + threadIdx.x;
+          const uint64_t g = bm_rank(qq.mbits, qq.rank, code[u]);
+          qq.okey[g] = (int64_t)((uint64_t)code[u] + (uint64_t)mkmin);
+          qq.og1[g] = g1col >= 0 ? tile_get(ts, t.words, t.segs, g1col, seg, r0, i) : 0;
+          qq.og2[g] = g2col >= 0 ? tile_get(ts, t.words, t.segs, g2col, seg, r0, i) : 0;
+        }
+    }
+  }
+}
+
+// Fact scan: per query, the tile's rows passing its ranges probe its mid bitmap; hits are
+// compacted (row order) into an LDS list 1024 rows at a time and aggregated from there.
+__global__ void __launch_bounds__(ST_BLOCK) k_sb_fact(SBScan t, const SBQuery *__restrict__ q, int nq, int fkcol,
+                                                      int xcol, int ycol, int64_t mkmin, uint64_t mrange,
+                                                      uint32_t *nprobe_out, uint32_t *nmatch_out) {
+  __shared__ TileStage ts;
+  __shared__ char qmode[SB_MAX];
+  __shared__ uint32_t hcode[SB_CHUNK * ST_BLOCK];
+  __shared__ uint16_t hrow[SB_CHUNK * ST_BLOCK];
+  __shared__ uint32_t woff[SB_CHUNK][ST_BLOCK / 64];
+  __shared__ uint32_t nhit_s;
+  const uint32_t seg = blockIdx.x / t.tps, tile = blockIdx.x % t.tps;
+  const uint32_t nrows = t.segs[0][seg].nrows, r0 = tile * ST_TILE;
+  const uint32_t nt = r0 < nrows ? (nrows - r0 < (uint32_t)ST_TILE ? nrows - r0 : (uint32_t)ST_TILE) : 0;
+  if (!nt || !sb_prologue(t, q, nq, 2, ts, qmode, seg, r0, nt)) {
+    if (threadIdx.x < (unsigned)nq) {
+      nprobe_out[(uint64_t)threadIdx.x * gridDim.x + blockIdx.x] = 0;
+      nmatch_out[(uint64_t)threadIdx.x * gridDim.x + blockIdx.x] = 0;
+    }
+    return;
+  }
+  const unsigned lane = __lane_id(), wave = threadIdx.x / 64;
+  uint32_t fkc[ST_ITEMS];
+#pragma unroll
+  for (int j = 0; j < ST_ITEMS; ++j) {
+    const uint32_t i = j * ST_BLOCK + threadIdx.x;
+    const uint64_t c = i < nt ? (uint64_t)tile_get(ts, t.words, t.segs, fkcol, seg, r0, i) - (uint64_t)mkmin : ~0ull;
+    fkc[j] = c < mrange ? (uint32_t)c : 0xFFFFFFFFu;  // mrange <= 2^31 on this path
+  }
+  for (int qi = 0; qi < nq; ++qi) {
+    const char mode = qmode[qi];
+    uint32_t nprobe = 0, nmatch = 0;
+    if (mode != 'N') {
+      const SBQuery &qq = q[qi];
+      for (int ch = 0; ch < ST_ITEMS / SB_CHUNK; ++ch) {
+        uint32_t code[SB_CHUNK], w[SB_CHUNK];
+#pragma unroll
+        for (int u = 0; u < SB_CHUNK; ++u) {
+          const int j = ch * SB_CHUNK + u;
+          const uint32_t i = j * ST_BLOCK + threadIdx.x;
+          const bool p = i < nt && sb_pass(t, ts, qq.r[2], mode, seg, r0, i);
+          nprobe += p;
+          code[u] = p ? fkc[j] : 0xFFFFFFFFu;
+        }
+#pragma unroll
+        for (int u = 0; u < SB_CHUNK; ++u) w[u] = code[u] != 0xFFFFFFFFu ? qq.mbits[code[u] >> 5] : 0u;
+        uint32_t hits = 0;
+#pragma unroll
+        for (int u = 0; u < SB_CHUNK; ++u)
+          hits |= (uint32_t)(code[u] != 0xFFFFFFFFu && ((w[u] >> (code[u] & 31)) & 1)) << u;
+#pragma unroll
+        for (int u = 0; u < SB_CHUNK; ++u) {
+          const uint64_t b = __ballot((hits >> u) & 1);
+          if (lane == 0) woff[u][wave] = (uint32_t)__popcll(b);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+          uint32_t tot = 0;
+          for (int u = 0; u < SB_CHUNK; ++u)
+            for (int v = 0; v < ST_BLOCK / 64; ++v) {
+              const uint32_t c = woff[u][v];
+              woff[u][v] = tot;
+              tot += c;
+            }
+          nhit_s = tot;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < SB_CHUNK; ++u) {
+          const bool h = (hits >> u) & 1;
+          const uint64_t b = __ballot(h);
+          if (h) {
+            const uint32_t pos = woff[u][wave] + (uint32_t)__popcll(b & lanemask_lt());
+            hcode[pos] = code[u];
+            hrow[pos] = (uint16_t)((ch * SB_CHUNK + u) * ST_BLOCK + threadIdx.x);
+          }
+        }
+        __syncthreads();
+        const uint32_t nh = nhit_s;
+        for (uint32_t e0 = 0; e0 < nh; e0 += ST_BLOCK) {
+          const uint32_t e = e0 + threadIdx.x;
+          uint32_t g = 0xFFFFFFFFu;
+          int64_t v = 0;
+          if (e < nh) {
+            const uint32_t local = r0 + hrow[e];
+            g = (uint32_t)bm_rank(qq.mbits, qq.rank, hcode[e]);
+            const int64_t x = decode_row(t.words[xcol], t.segs[xcol][seg], local);
+            const int64_t y = decode_row(t.words[ycol], t.segs[ycol][seg], local);
+            v = wmul((int64_t)((uint64_t)qq.a0 + (uint64_t)wmul(qq.a1, x)),
+                     (int64_t)((uint64_t)qq.b0 + (uint64_t)wmul(qq.b1, y)));
+          }
+          if (__ballot(g != 0xFFFFFFFFu) == 0) continue;
+          const uint32_t prev = __shfl_up(g, 1, 64);
+          bool f = lane == 0 || prev != g;
+          uint32_t c = g != 0xFFFFFFFFu;
+          for (int o = 1; o < 64; o <<= 1) {
+            const int64_t vu = __shfl_up(v, o, 64);
+            const uint32_t cu = __shfl_up(c, o, 64);
+            const bool fu = __shfl_up((int)f, o, 64) != 0;
+            if (lane >= (unsigned)o && !f) {
+              v = (int64_t)((uint64_t)v + (uint64_t)vu);
+              c += cu;
+            }
+            if (lane >= (unsigned)o) f = f || fu;
+          }
+          const uint32_t next = __shfl_down(g, 1, 64);
+          if (g != 0xFFFFFFFFu && (lane == 63 || next != g)) {
+            atomicAdd(&qq.sum[g], (unsigned long long)v);
+            atomicAdd(&qq.cnt[g], c);
+            nmatch += c;
+          }
+        }
+        __syncthreads();  // the list is rewritten by the next chunk
+      }
+    }
+    block_count(nprobe, nprobe_out + (uint64_t)qi * gridDim.x);
+    block_count(nmatch, nmatch_out + (uint64_t)qi * gridDim.x);
+  }
+}
+
+// Union of the range columns of one table over the queries, then `key`; the first two with
+// at most 31 bits are staged by the kernels (the rest are read from HBM).
+static void sb_want(const aln_star_args *a, int nq, int table, int key, SBScan *s) {
+  std::vector<int> w;
+  for (int i = 0; i < nq; ++i) {
+    const aln_table_scan &t = table == 0 ? a[i].dim : table == 1 ? a[i].mid : a[i].fact;
+    for (int k = 0; k < t.nranges; ++k)
+      if (std::find(w.begin(), w.end(), t.ranges[k].col) == w.end()) w.push_back(t.ranges[k].col);
+  }
+  if (key >= 0 && std::find(w.begin(), w.end(), key) == w.end()) w.insert(w.begin() + (w.empty() ? 0 : 1), key);
+  s->nwant = (int)std::min<size_t>(w.size(), 2);
+  for (int i = 0; i < s->nwant; ++i) s->want[i] = w[i];
+}
+
+static int sb_scan(const aln_table_scan &t, SBScan *s, uint64_t *nblocks) {
+  ALN_REQUIRE(t.cols && t.ncols >= 1 && t.ncols <= ALN_MAX_COLS && t.seg_rows > 0,
+              "aln_star_agg_batch: bad table scan");
+  memset(s, 0, sizeof(*s));
+  for (int i = 0; i < t.ncols; ++i) {
+    ALN_REQUIRE(t.cols[i].nsegs == t.cols[0].nsegs, "aln_star_agg_batch: columns differ in segments");
+    s->words[i] = t.cols[i].words;
+    s->segs[i] = t.cols[i].segs;
+  }
+  s->tps = (t.seg_rows + ST_TILE - 1) / ST_TILE;
+  *nblocks = (uint64_t)t.cols[0].nsegs * s->tps;
+  ALN_REQUIRE(*nblocks < (1ull << 31), "aln_star_agg_batch: grid too large");
+  return ALN_OK;
+}
+
+static bool same_table(const aln_table_scan &x, const aln_table_scan &y) {
+  if (x.ncols != y.ncols || x.seg_rows != y.seg_rows || x.seg_modes || y.seg_modes) return false;
+  for (int i = 0; i < x.ncols; ++i)
+    if (x.cols[i].words != y.cols[i].words || x.cols[i].segs != y.cols[i].segs ||
+        x.cols[i].nsegs != y.cols[i].nsegs)
+      return false;
+  return true;
+}
+
+}  // namespace aln
+
+extern "C" int aln_star_agg_batch(aln_ctx *c, const aln_star_args *a, int nq, aln_star_row *const *h_top,
+                                  uint32_t *h_ntop, aln_star_stats *h_stats) {
+  ALN_REQUIRE(c && a && h_ntop && h_top, "aln_star_agg_batch: bad arguments");
+  ALN_REQUIRE(nq >= 1 && nq <= SB_MAX, "aln_star_agg_batch: %d queries (1..%d)", nq, SB_MAX);
+  const aln_star_args &a0 = a[0];
+  for (int i = 0; i < nq; ++i) {
+    const aln_star_args &x = a[i];
+    if (!same_table(x.dim, a0.dim) || !same_table(x.mid, a0.mid) || !same_table(x.fact, a0.fact) ||
+        x.d_key != a0.d_key || x.m_key != a0.m_key || x.m_fk != a0.m_fk || x.m_g1 != a0.m_g1 ||
+        x.m_g2 != a0.m_g2 || x.f_fk != a0.f_fk || x.f_x != a0.f_x || x.f_y != a0.f_y || (x.flags & 3))
+      return set_error(ALN_E_UNSUPPORTED, "aln_star_agg_batch: query %d differs in tables or columns", i);
+    ALN_REQUIRE(x.limit <= TK_TILE2 / 2 && (x.limit == 0 || h_top[i]), "aln_star_agg_batch: query %d limit", i);
+    const aln_table_scan *ts[3] = {&x.dim, &x.mid, &x.fact};
+    for (int k = 0; k < 3; ++k) {
+      ALN_REQUIRE(ts[k]->nranges >= 0 && ts[k]->nranges <= ALN_MAX_RANGES, "aln_star_agg_batch: ranges");
+      for (int r = 0; r < ts[k]->nranges; ++r)
+        ALN_REQUIRE(ts[k]->ranges[r].col >= 0 && ts[k]->ranges[r].col < ts[k]->ncols,
+                    "aln_star_agg_batch: range column");
+    }
+  }
+  ALN_REQUIRE(a0.d_key >= 0 && a0.d_key < a0.dim.ncols && a0.m_key >= 0 && a0.m_key < a0.mid.ncols &&
+                  a0.m_fk >= 0 && a0.m_fk < a0.mid.ncols && a0.m_g1 < a0.mid.ncols && a0.m_g2 < a0.mid.ncols &&
+                  a0.f_fk >= 0 && a0.f_fk < a0.fact.ncols && a0.f_x >= 0 && a0.f_x < a0.fact.ncols &&
+                  a0.f_y >= 0 && a0.f_y < a0.fact.ncols,
+              "aln_star_agg_batch: column slots");
+  ws_reset(c);
+  for (int i = 0; i < nq; ++i) h_ntop[i] = 0;
+  std::vector<aln_star_stats> st(nq);
+  memset(st.data(), 0, sizeof(aln_star_stats) * nq);
+  SBScan sd, sm, sa, sf;
+  uint64_t dblocks, mblocks, fblocks;
+  ALN_TRY(sb_scan(a0.dim, &sd, &dblocks));
+  ALN_TRY(sb_scan(a0.mid, &sm, &mblocks));
+  ALN_TRY(sb_scan(a0.fact, &sf, &fblocks));
+  sa = sm;
+  sb_want(a, nq, 0, a0.d_key, &sd);
+  sb_want(a, nq, 1, a0.m_fk, &sm);
+  sb_want(a, nq, 2, a0.f_fk, &sf);
+  sa.nwant = 0;
+  for (int col : {a0.m_key, a0.m_g1})
+    if (col >= 0 && (sa.nwant == 0 || sa.want[0] != col)) sa.want[sa.nwant++] = col;
+
+  int64_t dkmin, dkmax, mkmin, mkmax;
+  ALN_TRY(col_range(c, a0.dim.cols[a0.d_key], &dkmin, &dkmax));
+  ALN_TRY(col_range(c, a0.mid.cols[a0.m_key], &mkmin, &mkmax));
+  if (a0.dim.cols[0].nsegs == 0 || a0.mid.cols[0].nsegs == 0 || dkmax < dkmin || mkmax < mkmin) {
+    if (h_stats) memcpy(h_stats, st.data(), sizeof(aln_star_stats) * nq);
+    return ALN_OK;
+  }
+  const uint64_t drange = (uint64_t)dkmax - (uint64_t)dkmin + 1;
+  const uint64_t mrange = (uint64_t)mkmax - (uint64_t)mkmin + 1;
+  if (drange == 0 || drange > (1ull << 34) || mrange == 0 || mrange > (1ull << 31))
+    return set_error(ALN_E_UNSUPPORTED, "aln_star_agg_batch: key ranges too wide for bitmaps");
+  const uint64_t dwords = (drange + 31) / 32;
+  const uint64_t mwords = (mrange + 31) / 32, mblk = (mwords + BM_BLOCK_WORDS - 1) / BM_BLOCK_WORDS;
+
+  std::vector<SBQuery> hq(nq);
+  for (int i = 0; i < nq; ++i) {
+    SBQuery &q = hq[i];
+    memset(&q, 0, sizeof(q));
+    const aln_table_scan *ts[3] = {&a[i].dim, &a[i].mid, &a[i].fact};
+    for (int k = 0; k < 3; ++k) {
+      q.r[k].nr = ts[k]->nranges;
+      for (int r = 0; r < ts[k]->nranges; ++r) {
+        q.r[k].col[r] = ts[k]->ranges[r].col;
+        q.r[k].lo[r] = ts[k]->ranges[r].lo;
+        q.r[k].hi[r] = ts[k]->ranges[r].hi;
+      }
+    }
+    q.a0 = a[i].a0;
+    q.a1 = a[i].a1;
+    q.b0 = a[i].b0;
+    q.b1 = a[i].b1;
+    ALN_TRY(ws(c, dwords, &q.dbm));
+    ALN_TRY(fill_u32(c, q.dbm, dwords, 0));
+    ALN_TRY(ws(c, mblk * BM_BLOCK_WORDS, &q.mbits));
+    ALN_TRY(fill_u32(c, q.mbits, mblk * BM_BLOCK_WORDS, 0));
+    ALN_TRY(ws(c, mblk, &q.rank));
+  }
+  SBQuery *dq;
+  uint32_t *dcnt, *mprobe, *mpass, *fprobe, *fmatch, *dup;
+  uint64_t *ctr;
+  ALN_TRY(ws(c, nq, &dq));
+  ALN_TRY(ws(c, dblocks * nq, &dcnt));
+  ALN_TRY(ws(c, mblocks * nq, &mprobe));
+  ALN_TRY(ws(c, mblocks * nq, &mpass));
+  ALN_TRY(ws(c, fblocks * nq, &fprobe));
+  ALN_TRY(ws(c, fblocks * nq, &fmatch));
+  ALN_TRY(ws(c, 2 * SB_MAX, &dup));
+  ALN_TRY(ws(c, 5 * SB_MAX, &ctr));
+  ALN_TRY(fill_u32(c, dup, 2 * SB_MAX, 0));
+  ALN_TRY(fill_u64(c, ctr, 5 * SB_MAX, 0));
+  ALN_TRY(copy_to_device(c, dq, hq.data(), sizeof(SBQuery) * nq));
+  {
+    ProfScope p(c, "star_dim");
+    hipLaunchKernelGGL(k_sb_dim, dim3((unsigned)dblocks), dim3(ST_BLOCK), 0, c->stream, sd, dq, nq, a0.d_key, dkmin,
+                       drange, dcnt, dup);
+  }
+  ALN_TRY(check_launch("sb_dim"));
+  {
+    ProfScope p(c, "star_mid");
+    hipLaunchKernelGGL(k_sb_mid, dim3((unsigned)mblocks), dim3(ST_BLOCK), 0, c->stream, sm, dq, nq, a0.m_key,
+                       a0.m_fk, dkmin, drange, mkmin, mrange, mprobe, mpass, dup + SB_MAX);
+  }
+  ALN_TRY(check_launch("sb_mid"));
+  for (int i = 0; i < nq; ++i) {
+    ALN_TRY(sum_u32(c, dcnt + (uint64_t)i * dblocks, dblocks, ctr + 5 * i + 0));
+    ALN_TRY(sum_u32(c, mprobe + (uint64_t)i * mblocks, mblocks, ctr + 5 * i + 1));
+    ALN_TRY(sum_u32(c, mpass + (uint64_t)i * mblocks, mblocks, ctr + 5 * i + 2));
+  }
+  std::vector<uint64_t> hc(5 * SB_MAX);
+  uint32_t hdup[2 * SB_MAX];
+  ALN_TRY(copy_to_host(c, hc.data(), ctr, 8 * 5 * SB_MAX));
+  ALN_TRY(copy_to_host(c, hdup, dup, 4 * 2 * SB_MAX));
+  for (int i = 0; i < nq; ++i) {
+    if (hdup[i]) return set_error(ALN_E_UNSUPPORTED, "aln_star_agg_batch: duplicate dimension keys");
+    if (hdup[SB_MAX + i]) return set_error(ALN_E_UNSUPPORTED, "aln_star_agg_batch: duplicate mid keys");
+    st[i].dim_pass = hc[5 * i + 0];
+    st[i].mid_probe = hc[5 * i + 1];
+    st[i].mid_pass = hc[5 * i + 2];
+  }
+  // per query: rank directory and group arrays sized to its passing mid rows
+  uint32_t *bcnt;
+  ALN_TRY(ws(c, mblk, &bcnt));
+  uint64_t mmax = 0;
+  for (int i = 0; i < nq; ++i) {
+    SBQuery &q = hq[i];
+    const uint64_t m = st[i].mid_pass ? st[i].mid_pass : 1;
+    mmax = std::max(mmax, m);
+    {
+      ProfScope p(c, "star_build");
+      hipLaunchKernelGGL(k_bm_block_counts, dim3(grid_for(mblk, 256, 65536)), dim3(256), 0, c->stream, q.mbits,
+                         mwords, mblk, bcnt);
+    }
+    ALN_TRY(check_launch("bm_block_counts"));
+    ALN_TRY(exclusive_scan_u32(c, bcnt, mblk, q.rank, nullptr));
+    ALN_TRY(ws(c, m, &q.sum));
+    ALN_TRY(ws(c, m, &q.cnt));
+    ALN_TRY(ws(c, m, &q.okey));
+    ALN_TRY(ws(c, m, &q.og1));
+    ALN_TRY(ws(c, m, &q.og2));
+    ALN_TRY(fill_u64(c, (uint64_t *)q.sum, m, 0));
+    ALN_TRY(fill_u32(c, q.cnt, m, 0));
+  }
+  ALN_TRY(copy_to_device(c, dq, hq.data(), sizeof(SBQuery) * nq));
+  {
+    ProfScope p(c, "star_build");
+    hipLaunchKernelGGL(k_sb_attr, dim3((unsigned)mblocks), dim3(ST_BLOCK), 0, c->stream, sa, dq, nq, a0.m_key,
+                       a0.m_g1, a0.m_g2, mkmin, mrange);
+  }
+  ALN_TRY(check_launch("sb_attr"));
+  {
+    ProfScope p(c, "star_fact");
+    hipLaunchKernelGGL(k_sb_fact, dim3((unsigned)fblocks), dim3(ST_BLOCK), 0, c->stream, sf, dq, nq, a0.f_fk,
+                       a0.f_x, a0.f_y, mkmin, mrange, fprobe, fmatch);
+  }
+  ALN_TRY(check_launch("sb_fact"));
+  for (int i = 0; i < nq; ++i) {
+    ALN_TRY(sum_u32(c, fprobe + (uint64_t)i * fblocks, fblocks, ctr + 5 * i + 3));
+    ALN_TRY(sum_u32(c, fmatch + (uint64_t)i * fblocks, fblocks, ctr + 5 * i + 4));
+  }
+  for (int i = 0; i < nq; ++i)
+    if (a[i].limit && st[i].mid_pass)
+      ALN_TRY(star_topk(c, st[i].mid_pass, hq[i].sum, hq[i].cnt, hq[i].okey, hq[i].og1, hq[i].og2, a[i].limit,
+                        h_top[i], &h_ntop[i]));
+  ALN_TRY(copy_to_host(c, hc.data(), ctr, 8 * 5 * SB_MAX));
+  for (int i = 0; i < nq; ++i) {
+    st[i].fact_probe = hc[5 * i + 3];
+    st[i].fact_pass = hc[5 * i + 4];
+  }
+  ALN_HIP(hipStreamSynchronize(c->stream));
+  if (h_stats) memcpy(h_stats, st.data(), sizeof(aln_star_stats) * nq);
   return ALN_OK;
 }

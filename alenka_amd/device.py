@@ -335,6 +335,29 @@ def star_agg(ctx, dim, mid, fact, spec, limit=10, want_groups=False, groups_cap=
     return rows, g, st
 
 
+def star_agg_batch(ctx, queries, limit=10):
+    """Several star pipelines over the same tables in one pass of each table.  queries:
+    list of (dim, mid, fact, spec) as for star_agg.  Returns a list of (rows, stats)."""
+    n = len(queries)
+    args = (L.StarArgs * n)()
+    keep = []
+    for i, (dm, md, fc, spec) in enumerate(queries):
+        a, k = star_args(dm, md, fc, spec, limit, 0)
+        args[i] = a
+        keep.append(k)
+    tops = [(L.StarRow * max(1, limit))() for _ in range(n)]
+    tp = (C.POINTER(L.StarRow) * n)(*[C.cast(t, C.POINTER(L.StarRow)) for t in tops])
+    ntop = (C.c_uint32 * n)()
+    st = (L.StarStats * n)()
+    L.check(ctx.lib.aln_star_agg_batch(ctx.h, args, n, tp, ntop, st), "aln_star_agg_batch")
+    out = []
+    for i in range(n):
+        rows = [(tops[i][r].key, tops[i][r].g1, tops[i][r].g2, tops[i][r].sum, tops[i][r].count)
+                for r in range(ntop[i])]
+        out.append((rows, st[i]))
+    return out
+
+
 def star_args(dim, mid, fact, spec, limit, flags):
     """aln_star_args for three table scans; returns (args, host arrays to keep alive)."""
     keep = []

@@ -217,6 +217,48 @@ int run_star(Slot *sl, Request *r) {
   return ALN_OK;
 }
 
+// Star requests over the same tables share one scan of each table; when the batched form
+// does not apply (hash join needed, duplicate keys) they run one by one.
+int run_star_batch(Slot *sl, std::vector<Request *> &rs) {
+  if (rs.size() == 1) return run_star(sl, rs[0]);
+  std::vector<aln_star_args> args;
+  std::vector<aln_star_row *> tops;
+  std::vector<uint32_t> ntop(rs.size());
+  std::vector<aln_star_stats> st(rs.size());
+  for (Request *r : rs) {
+    r->top.resize(std::max<uint32_t>(1, r->star.limit));
+    args.push_back(r->star);
+    tops.push_back(r->top.data());
+  }
+  const int rc = aln_star_agg_batch(sl->ctx, args.data(), (int)rs.size(), tops.data(), ntop.data(), st.data());
+  if (rc == ALN_E_UNSUPPORTED) {
+    for (Request *r : rs) ALN_TRY(run_star(sl, r));
+    return ALN_OK;
+  }
+  ALN_TRY(rc);
+  for (size_t i = 0; i < rs.size(); ++i) {
+    rs[i]->top.resize(ntop[i]);
+    rs[i]->stats = st[i];
+  }
+  return ALN_OK;
+}
+
+bool same_scan(const aln_table_scan &x, const aln_table_scan &y) {
+  if (x.ncols != y.ncols || x.seg_rows != y.seg_rows || x.seg_modes || y.seg_modes) return false;
+  for (int i = 0; i < x.ncols; ++i)
+    if (x.cols[i].words != y.cols[i].words || x.cols[i].segs != y.cols[i].segs || x.cols[i].nsegs != y.cols[i].nsegs)
+      return false;
+  return true;
+}
+
+// Two star requests can share scans: same tables and column slots, no group output.
+bool star_compatible(const Request &a, const Request &b) {
+  const aln_star_args &x = a.star, &y = b.star;
+  return same_scan(x.dim, y.dim) && same_scan(x.mid, y.mid) && same_scan(x.fact, y.fact) && x.d_key == y.d_key &&
+         x.m_key == y.m_key && x.m_fk == y.m_fk && x.m_g1 == y.m_g1 && x.m_g2 == y.m_g2 && x.f_fk == y.f_fk &&
+         x.f_x == y.f_x && x.f_y == y.f_y && !(x.flags & 3) && !(y.flags & 3);
+}
+
 // Takes the next batch off the queue (caller holds the lock).  Returns an empty vector
 // when the oldest request must wait for budget.
 std::vector<Request *> take_batch(aln_server *s) {
@@ -240,11 +282,14 @@ std::vector<Request *> take_batch(aln_server *s) {
   s->pending.pop_front();
   out.push_back(h);
   size_t mem = h->mem;
-  const int maxb = s->cfg.plain ? 1 : std::max(1, std::min(s->cfg.max_batch, (int)ALN_BATCH_MAX));
-  if (h->kind == 0 && maxb > 1) {
+  const int cap = h->kind == 0 ? (int)ALN_BATCH_MAX : (int)ALN_STAR_BATCH_MAX;
+  const int maxb = s->cfg.plain ? 1 : std::max(1, std::min(s->cfg.max_batch, cap));
+  if (maxb > 1) {
     for (auto it = s->pending.begin(); it != s->pending.end() && (int)out.size() < maxb;) {
       Request *r = *it;
-      if (r->kind == 0 && r->table == h->table && s->mem_in_use + mem + r->mem <= s->cfg.budget_bytes) {
+      const bool fits = r->kind == h->kind &&
+                        (h->kind == 0 ? r->table == h->table : star_compatible(*h, *r));
+      if (fits && s->mem_in_use + mem + r->mem <= s->cfg.budget_bytes) {
         mem += r->mem;
         out.push_back(r);
         it = s->pending.erase(it);
@@ -285,7 +330,7 @@ void slot_main(aln_server *s, Slot *sl) {
     lk.unlock();
     int rc;
     if (rs[0]->kind == 1)
-      rc = run_star(sl, rs[0]);
+      rc = run_star_batch(sl, rs);
     else if (s->cfg.plain)
       rc = run_plain(s, sl, rs[0]);
     else

@@ -265,10 +265,12 @@ def serve_tables(ctx, D, dim, mid, fact, zc, seed, scales):
     return flat, facts, stars
 
 
-def serve_trace(seed, rate, duration, scales, code_building, nkeys):
+def serve_trace(seed, rate, duration, scales, seg_codes, nkeys):
     """Poisson arrivals of requests drawn uniformly from the four templates (flat filter +
-    GROUP BY; the star pipeline at base and at compressed large scale; filter + GROUP BY
-    on the fact table with Zipf keys), each at a random scale."""
+    GROUP BY; the star pipeline at base and at compressed large scale, with TPC-H q3's
+    substitution parameters drawn per request: a market segment of the five and a date in
+    1995-03-01..31; filter + GROUP BY on the fact table with Zipf keys), each at a random
+    scale."""
     import numpy as np
 
     rng = np.random.default_rng(seed)
@@ -286,11 +288,12 @@ def serve_trace(seed, rate, duration, scales, code_building, nkeys):
         elif tmpl == 3:
             q = batch_queries(int(rng.integers(1 << 30)), 1, nkeys, 90000, 10410001, 2557 + 121)[0]
         else:
-            q = None
+            q = dict(seg=int(seg_codes[rng.integers(0, len(seg_codes))]),
+                     date=19950300 + int(rng.integers(1, 32)))
         out.append((t, tmpl, s, q))
 
 
-def serve_run(D, srv, tids, stars, code_building, trace):
+def serve_run(D, srv, tids, stars, trace):
     """Submits the trace at its arrival times from one client thread while a second
     thread waits the requests in order; returns per-request latencies (ms) and the span."""
     import threading
@@ -321,9 +324,9 @@ def serve_run(D, srv, tids, stars, code_building, trace):
             time.sleep(d)
         if tmpl in (1, 2):
             dm, md, fc = stars[s]
-            ids[i] = srv.submit_star(dict(cols=dm, ranges=[(1, code_building, code_building)]),
-                                     dict(cols=md, ranges=[(2, -(1 << 63), DATE - 1)]),
-                                     dict(cols=fc, ranges=[(3, DATE + 1, (1 << 63) - 1)]), spec, limit=10)
+            ids[i] = srv.submit_star(dict(cols=dm, ranges=[(1, q["seg"], q["seg"])]),
+                                     dict(cols=md, ranges=[(2, -(1 << 63), q["date"] - 1)]),
+                                     dict(cols=fc, ranges=[(3, q["date"] + 1, (1 << 63) - 1)]), spec, limit=10)
         else:
             ids[i] = srv.submit(tids[(tmpl, s)], q)
         ready[i].set()
@@ -331,37 +334,70 @@ def serve_run(D, srv, tids, stars, code_building, trace):
     return lat, status, t_last[0] - t0
 
 
-def serve_leg(ctx, D, dim, mid, fact, zc, code_of, seed, rate, duration, budget_gb):
-    """The served form against serial plain execution on the same Poisson trace."""
+def serve_point(D, kw, budget_gb, flat, facts, stars, scales, seg_codes, seed, rate, duration):
+    """One run of a form over a Poisson trace: achieved q/s and latency percentiles."""
     import numpy as np
+    import torch
+
+    trace = serve_trace(seed, rate, duration, scales, seg_codes, 10_000_000)
+    srv = D.Server(budget_bytes=int(budget_gb) << 30, **kw)
+    try:
+        tids = {}
+        for s in scales:
+            tids[(0, s)] = srv.add_table(flat[s])
+            tids[(3, s)] = srv.add_table(facts[s])
+        torch.cuda.synchronize()
+        lat, status, span = serve_run(D, srv, tids, stars, trace)
+        st = srv.stats()
+    finally:
+        srv.close()
+    return {"offered_qps": rate, "requests": len(trace), "qps": len(trace) / span,
+            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "failed": int((status != 0).sum()), "batches": st["batches"],
+            "batched_queries": st["batched_queries"], "max_concurrent": st["max_concurrent"]}
+
+
+def serve_leg(ctx, D, dim, mid, fact, zc, code_of, seed, rate, duration, budget_gb):
+    """The served form against serial plain execution (one query at a time through the
+    operators) on Poisson traces of the four templates.  Protocol: each form's saturation
+    throughput from a burst; the latency target is the plain form's p99 at 80 % of its
+    saturation; each form's max sustained q/s is the highest offered rate (a sweep around
+    its saturation) whose p99 stays within that target.  Also both forms at `rate`."""
     import torch
 
     scales = [1, 10, 100]
     flat, facts, stars = serve_tables(ctx, D, dim, mid, fact, zc, seed, scales)
-    b = int(code_of[SEGMENTS.index("BUILDING")])
-    trace = serve_trace(seed + 11, rate, duration, scales, b, 10_000_000)
-    out = {"rate_qps": rate, "duration_s": duration, "requests": len(trace), "scales": scales,
-           "budget_gb": budget_gb,
-           "mix": {k: sum(1 for x in trace if x[1] == i) for i, k in enumerate(("flat", "star", "star_large", "fact"))}}
-    for mode, kw in (("served", dict(streams=3, max_batch=64, plain=False)),
-                     ("plain", dict(streams=1, max_batch=1, plain=True))):
-        srv = D.Server(budget_bytes=int(budget_gb) << 30, **kw)
-        try:
-            tids = {}
-            for s in scales:
-                tids[(0, s)] = srv.add_table(flat[s])
-                tids[(3, s)] = srv.add_table(facts[s])
-            torch.cuda.synchronize()
-            lat, status, span = serve_run(D, srv, tids, stars, b, trace)
-            st = srv.stats()
-        finally:
-            srv.close()
-        ok = status == 0
-        out[mode] = {"qps": len(trace) / span, "p50_ms": float(np.percentile(lat, 50)),
-                     "p99_ms": float(np.percentile(lat, 99)), "mean_ms": float(lat.mean()),
-                     "failed": int((~ok).sum()), "batches": st["batches"],
-                     "batched_queries": st["batched_queries"], "max_concurrent": st["max_concurrent"]}
-        log(f"serve {mode}: {out[mode]}")
+    codes = [int(c) for c in code_of if c >= 0]
+    forms = {"served": dict(streams=3, max_batch=64, plain=False),
+             "plain": dict(streams=1, max_batch=1, plain=True)}
+    pt = lambda f, r, d, sd: serve_point(D, forms[f], budget_gb, flat, facts, stars, scales, codes, sd, r, d)
+    out = {"templates": ["flat filter+group-by", "star q3", "star q3 (large)", "fact filter+group-by (Zipf)"],
+           "scales": scales, "budget_gb": budget_gb}
+    sat = {}
+    for f in forms:
+        burst = pt(f, 1e6, 150 / 1e6, seed + 21)  # 150 requests arriving at once
+        sat[f] = burst["qps"]
+        out[f + "_saturation"] = burst
+        log(f"serve {f} saturation: {burst['qps']:.1f} q/s")
+    slo_point = pt("plain", 0.8 * sat["plain"], duration, seed + 22)
+    slo = slo_point["p99_ms"]
+    out["slo"] = {"p99_ms": slo, "from": "plain form at 80% of its saturation", "point": slo_point}
+    log(f"serve SLO p99 {slo:.1f} ms")
+    for f in forms:
+        best = None
+        for k, frac in enumerate((0.5, 0.7, 0.85, 1.0, 1.2)):
+            r = pt(f, frac * sat[f], duration, seed + 30 + k)
+            log(f"serve {f} at {r['offered_qps']:.1f} q/s: {r['qps']:.1f} q/s, p99 {r['p99_ms']:.1f} ms")
+            if r["p99_ms"] <= slo and r["failed"] == 0:
+                best = r
+            else:
+                break
+        out[f + "_at_slo"] = best
+    if out["served_at_slo"] and out["plain_at_slo"]:
+        out["speedup_qps_at_equal_p99"] = out["served_at_slo"]["qps"] / out["plain_at_slo"]["qps"]
+    for f in forms:
+        out[f + "_at_rate"] = pt(f, rate, duration, seed + 11)
+        log(f"serve {f} at {rate} q/s: {out[f + '_at_rate']}")
     del flat, facts, stars
     torch.cuda.empty_cache()
     return out

@@ -264,6 +264,16 @@ typedef struct {
 int aln_star_agg(aln_ctx *ctx, const aln_star_args *args, aln_star_row *h_top, uint32_t *h_ntop,
                  aln_star_row *d_groups, uint64_t groups_cap, aln_star_stats *h_stats);
 
+/* Several star pipelines over the same three tables (same column arrays and slots, no
+ * seg_modes, flags 0) that differ in their ranges, coefficients and limit - TPC-H q3 with
+ * different substitution parameters - answered with one scan of each table.  h_top[i]
+ * receives query i's top rows (a[i].limit of them), h_ntop[i] their number, h_stats[i]
+ * (optional) its counts.  Bitmap joins only: ALN_E_UNSUPPORTED when a key range needs the
+ * hash table or a build side has duplicate keys (run the queries one by one then). */
+#define ALN_STAR_BATCH_MAX 8
+int aln_star_agg_batch(aln_ctx *ctx, const aln_star_args *a, int nq, aln_star_row *const *h_top,
+                       uint32_t *h_ntop, aln_star_stats *h_stats);
+
 /* ---- batched filter + GROUP BY (shared scan) -------------------------------------------
  * Up to ALN_BATCH_MAX queries over one table, each of the shape
  *   F := FILTER T BY pred >= lo AND pred <= hi;            (emit_filter, operators.cu:2306)

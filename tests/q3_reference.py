@@ -27,13 +27,14 @@ def tables(seed, ndim, norders):
                 o_prio=o_prio, l_okey=l_okey, l_price=l_price, l_disc=l_disc, l_ship=l_ship)
 
 
-def run(t, limit=10):
-    building = int(t["code_of"][SEGMENTS.index("BUILDING")])
+def run(t, limit=10, segment="BUILDING", date=DATE):
+    """The q3 script on the oracle; segment and date are q3's substitution parameters."""
+    building = int(t["code_of"][SEGMENTS.index(segment)])
     cf = O.filter_mask([(L.RPN_NAME, 0, 0), (L.RPN_STRING, 0, building), (L.RPN_CMP, 4, 0)],
                        [t["c_seg"]], [0])
-    ofi = O.filter_mask([(L.RPN_NAME, 0, 0), (L.RPN_NUMBER, 0, DATE), (L.RPN_CMP, 1, 0)],
+    ofi = O.filter_mask([(L.RPN_NAME, 0, 0), (L.RPN_NUMBER, 0, date), (L.RPN_CMP, 1, 0)],
                         [t["o_date"]], [0])
-    lf = O.filter_mask([(L.RPN_NAME, 0, 0), (L.RPN_NUMBER, 0, DATE), (L.RPN_CMP, 2, 0)],
+    lf = O.filter_mask([(L.RPN_NAME, 0, 0), (L.RPN_NUMBER, 0, date), (L.RPN_CMP, 2, 0)],
                        [t["l_ship"]], [0])
     c_ids = O.compact(cf)
     o_ids = O.compact(ofi)

@@ -314,3 +314,31 @@ def test_star_q3(ctx, scale, seg_rows, force_hash):
     assert st.dim_pass == stats["dim_pass"] and st.mid_pass == stats["mid_pass"]
     assert st.fact_pass == stats["fact_pass"] and st.groups == stats["groups"]
     assert np.array_equal(host(g), groups)
+
+
+@pytest.mark.parametrize("scale,seg_rows", [(0.02, 4096), (0.1, 65536)])
+def test_star_q3_batch(ctx, scale, seg_rows):
+    """q3 with different substitution parameters (segment, date) answered by one pass of each
+    table: every query equals its own oracle run and its own single-query star_agg."""
+    from alenka_amd import device as D
+
+    ndim, norders = int(150000 * scale), int(1500000 * scale)
+    t = Q3.tables(4321, ndim, norders)
+    dim, mid, fact = encode_q3(ctx, t, seg_rows)
+    params = [("BUILDING", 19950315), ("MACHINERY", 19950301), ("AUTOMOBILE", 19950331),
+              ("BUILDING", 19920201), ("HOUSEHOLD", 19980801)]
+    qs = []
+    for seg_name, date in params:
+        code = int(t["code_of"][Q3.SEGMENTS.index(seg_name)])
+        qs.append((dict(cols=dim, ranges=[(1, code, code)]), dict(cols=mid, ranges=[(2, -(1 << 63), date - 1)]),
+                   dict(cols=fact, ranges=[(3, date + 1, (1 << 63) - 1)]),
+                   dict(d_key=0, m_key=0, m_fk=1, m_g1=2, m_g2=3, f_fk=0, f_x=1, f_y=2, a0=0, a1=1, b0=100, b1=-1)))
+    got = D.star_agg_batch(ctx, qs, limit=10)
+    for (seg_name, date), q, (rows, st) in zip(params, qs, got):
+        top, groups, stats = Q3.run(t, segment=seg_name, date=date)
+        assert rows == top, (seg_name, date)
+        assert st.dim_pass == stats["dim_pass"] and st.mid_pass == stats["mid_pass"]
+        assert st.fact_pass == stats["fact_pass"]
+        alone, _, st1 = D.star_agg(ctx, *q, limit=10)
+        assert alone == rows
+        assert (st1.mid_probe, st1.fact_probe) == (st.mid_probe, st.fact_probe)
