@@ -26,11 +26,17 @@ namespace aln {
 constexpr int BQ_MAX = ALN_BATCH_MAX;
 constexpr int BQ_HOT_MAX = 32;
 
+constexpr int BQ_SLOTS = 4;  // distinct columns a batch's queries read (else the general loop)
+constexpr int BQ_CHUNK = 4;  // rows per thread decoded together
+
 struct BQuery {
   int32_t pred_col, key_col, x_col, y_col;
+  int32_t ps, ks, xs, ys;  // the same columns as slots of BArgs::slot_col
   int64_t lo, hi;
   int64_t div;       // >= 1
   int64_t gbase;     // floor(min(key) / div)
+  uint64_t base;     // gbase * div (wrapping): key - base >= 0 for every key of the column
+  uint64_t magic;    // ceil(2^64 / div) for div >= 2: (key - base) / div of a 32-bit difference
   uint64_t ngroups;  // dense group range
   int64_t a0, a1, b0, b1;
   unsigned long long *sum;  // [ngroups]
@@ -47,13 +53,55 @@ struct BArgs {
   int nstage;
   // columns every query shares (-1: per query)
   int pred_col, key_col, x_col, y_col;
+  // distinct columns of the batch when there are at most BQ_SLOTS (nslots 0: more)
+  int slot_col[BQ_SLOTS];
+  int nslots;
   uint32_t seg_rows, tps;
   uint64_t ntiles;
 };
 
-__device__ __forceinline__ int64_t floordiv(int64_t a, int64_t b) {
-  int64_t q = a / b;
-  return (a % b != 0 && ((a < 0) != (b < 0))) ? q - 1 : q;
+// floor(key / div) - gbase = (key - base) / div with key - base >= 0.  A difference below
+// 2^32 divides by one multiply-high with magic = ceil(2^64 / div): the estimate exceeds
+// x / div by less than x / 2^64 < 1 / div, so its floor is exact (x * div < 2^64).
+__device__ __forceinline__ uint64_t group_of(const BQuery &q, int64_t key) {
+  const uint64_t x = (uint64_t)key - q.base;
+  if (q.div == 1) return x;
+  if (x >> 32) return x / (uint64_t)q.div;
+  return __umul64hi(x, q.magic);
+}
+
+__device__ __forceinline__ int64_t pick(int s, int64_t v0, int64_t v1, int64_t v2, int64_t v3) {
+  return s == 0 ? v0 : s == 1 ? v1 : s == 2 ? v2 : v3;
+}
+
+// Adds (v, 1) of one (query, row) to its group: groups below `hot` go to the block's LDS
+// copy; group 0 - where Zipf ranks concentrate - is first summed over the wave when many
+// lanes hit it, so one lane issues the atomic for all of them.
+__device__ __forceinline__ void bq_add(const BQuery &q, int qi, int hot, uint64_t g, int64_t v, bool on,
+                                       unsigned long long (*hsum)[BQ_HOT_MAX], unsigned int (*hcnt)[BQ_HOT_MAX]) {
+  const uint64_t z = __ballot(on && g == 0);
+  if (__popcll(z) >= 8) {
+    int64_t s = on && g == 0 ? v : 0;
+    for (int o = 32; o > 0; o >>= 1) s = (int64_t)((uint64_t)s + (uint64_t)__shfl_xor(s, o, 64));
+    if (__lane_id() == (unsigned)(__ffsll((unsigned long long)z) - 1)) {
+      if (hot > 0) {
+        atomicAdd(&hsum[qi][0], (unsigned long long)s);
+        atomicAdd(&hcnt[qi][0], (unsigned int)__popcll(z));
+      } else if (q.ngroups) {
+        atomicAdd(&q.sum[0], (unsigned long long)s);
+        atomicAdd(&q.cnt[0], (unsigned int)__popcll(z));
+      }
+    }
+    on = on && g != 0;
+  }
+  if (!on) return;
+  if (g < (uint64_t)hot) {
+    atomicAdd(&hsum[qi][g], (unsigned long long)v);
+    atomicAdd(&hcnt[qi][g], 1u);
+  } else if (g < q.ngroups) {
+    atomicAdd(&q.sum[g], (unsigned long long)v);
+    atomicAdd(&q.cnt[g], 1u);
+  }
 }
 
 __global__ void __launch_bounds__(STG_BLOCK) k_batch(BArgs a) {
@@ -75,41 +123,60 @@ __global__ void __launch_bounds__(STG_BLOCK) k_batch(BArgs a) {
     __syncthreads();  // the previous tile's LDS reads are done
     stage_tile(ts, a.words, a.segs, want, a.nstage, seg, r0, nt);
     __syncthreads();
+    if (a.nslots > 0) {
+      // every distinct column decoded once per row (HBM loads of BQ_CHUNK rows in flight),
+      // then each query's parameters are read once per BQ_CHUNK rows
+      for (int ch = 0; ch < STG_ITEMS / BQ_CHUNK; ++ch) {
+        int64_t v0[BQ_CHUNK], v1[BQ_CHUNK], v2[BQ_CHUNK], v3[BQ_CHUNK];
+        bool ok[BQ_CHUNK];
+#pragma unroll
+        for (int u = 0; u < BQ_CHUNK; ++u) {
+          const uint32_t i = (ch * BQ_CHUNK + u) * STG_BLOCK + threadIdx.x;
+          ok[u] = i < nt;
+          v0[u] = ok[u] ? tile_get(ts, a.words, a.segs, a.slot_col[0], seg, r0, i) : 0;
+          v1[u] = ok[u] && a.nslots > 1 ? tile_get(ts, a.words, a.segs, a.slot_col[1], seg, r0, i) : 0;
+          v2[u] = ok[u] && a.nslots > 2 ? tile_get(ts, a.words, a.segs, a.slot_col[2], seg, r0, i) : 0;
+          v3[u] = ok[u] && a.nslots > 3 ? tile_get(ts, a.words, a.segs, a.slot_col[3], seg, r0, i) : 0;
+        }
+        for (int qi = 0; qi < a.nq; ++qi) {
+          const BQuery &q = a.q[qi];
+#pragma unroll
+          for (int u = 0; u < BQ_CHUNK; ++u) {
+            const int64_t p = pick(q.ps, v0[u], v1[u], v2[u], v3[u]);
+            const bool on = ok[u] && p >= q.lo && p <= q.hi;
+            const int64_t key = pick(q.ks, v0[u], v1[u], v2[u], v3[u]);
+            const int64_t x = pick(q.xs, v0[u], v1[u], v2[u], v3[u]);
+            const int64_t y = pick(q.ys, v0[u], v1[u], v2[u], v3[u]);
+            const uint64_t g = group_of(q, key);
+            const int64_t v = wmul((int64_t)((uint64_t)q.a0 + (uint64_t)wmul(q.a1, x)),
+                                   (int64_t)((uint64_t)q.b0 + (uint64_t)wmul(q.b1, y)));
+            bq_add(q, qi, a.hot, g, v, on, hsum, hcnt);
+          }
+        }
+      }
+      continue;
+    }
     for (int j = 0; j < STG_ITEMS; ++j) {
       const uint32_t i = j * STG_BLOCK + threadIdx.x;
-      if (i >= nt) continue;
-      const int64_t p0 = a.pred_col >= 0 ? tile_get(ts, a.words, a.segs, a.pred_col, seg, r0, i) : 0;
-      const int64_t k0 = a.key_col >= 0 ? tile_get(ts, a.words, a.segs, a.key_col, seg, r0, i) : 0;
-      int64_t x0 = 0, y0 = 0;
-      bool have_xy = false;
+      const bool row = i < nt;
       for (int qi = 0; qi < a.nq; ++qi) {
         const BQuery &q = a.q[qi];
-        const int64_t p = a.pred_col >= 0 ? p0 : tile_get(ts, a.words, a.segs, q.pred_col, seg, r0, i);
-        if (p < q.lo || p > q.hi) continue;
-        const int64_t key = a.key_col >= 0 ? k0 : tile_get(ts, a.words, a.segs, q.key_col, seg, r0, i);
-        int64_t x, y;
-        if (a.x_col >= 0) {
-          if (!have_xy) {
-            x0 = tile_get(ts, a.words, a.segs, a.x_col, seg, r0, i);
-            y0 = tile_get(ts, a.words, a.segs, a.y_col, seg, r0, i);
-            have_xy = true;
+        bool on = false;
+        uint64_t g = 0;
+        int64_t v = 0;
+        if (row) {
+          const int64_t p = tile_get(ts, a.words, a.segs, q.pred_col, seg, r0, i);
+          if (p >= q.lo && p <= q.hi) {
+            const int64_t key = tile_get(ts, a.words, a.segs, q.key_col, seg, r0, i);
+            const int64_t x = tile_get(ts, a.words, a.segs, q.x_col, seg, r0, i);
+            const int64_t y = tile_get(ts, a.words, a.segs, q.y_col, seg, r0, i);
+            g = group_of(q, key);
+            v = wmul((int64_t)((uint64_t)q.a0 + (uint64_t)wmul(q.a1, x)),
+                     (int64_t)((uint64_t)q.b0 + (uint64_t)wmul(q.b1, y)));
+            on = true;
           }
-          x = x0;
-          y = y0;
-        } else {
-          x = tile_get(ts, a.words, a.segs, q.x_col, seg, r0, i);
-          y = tile_get(ts, a.words, a.segs, q.y_col, seg, r0, i);
         }
-        const uint64_t g = (uint64_t)(floordiv(key, q.div) - q.gbase);
-        const int64_t v = wmul((int64_t)((uint64_t)q.a0 + (uint64_t)wmul(q.a1, x)),
-                               (int64_t)((uint64_t)q.b0 + (uint64_t)wmul(q.b1, y)));
-        if (g < (uint64_t)a.hot) {
-          atomicAdd(&hsum[qi][g], (unsigned long long)v);
-          atomicAdd(&hcnt[qi][g], 1u);
-        } else if (g < q.ngroups) {
-          atomicAdd(&q.sum[g], (unsigned long long)v);
-          atomicAdd(&q.cnt[g], 1u);
-        }
+        bq_add(q, qi, a.hot, g, v, on, hsum, hcnt);
       }
     }
   }
@@ -328,6 +395,8 @@ int batch_create(aln_ctx *c, const aln_col *cols, int ncols, uint32_t seg_rows, 
     bq.b0 = q.b0;
     bq.b1 = q.b1;
     bq.gbase = mx < mn ? 0 : floor_div(mn, q.key_div);
+    bq.base = (uint64_t)bq.gbase * (uint64_t)q.key_div;
+    bq.magic = q.key_div >= 2 ? ~0ull / (uint64_t)q.key_div + 1 : 0;
     bq.ngroups = dense_groups(mn, mx, q.key_div);
     if (bq.ngroups > (1ull << 31))
       return bail(set_error(ALN_E_UNSUPPORTED, "aln_batch_create: query %d has %llu dense groups", i,
@@ -345,6 +414,22 @@ int batch_create(aln_ctx *c, const aln_col *cols, int ncols, uint32_t seg_rows, 
     r1 = batch_alloc(b, 4 * (bq.ngroups ? bq.ngroups : 1), &p);
     if (r1) return bail(r1);
     bq.cnt = (unsigned int *)p;
+  }
+  // distinct columns: the staged ones first, then the rest in first-use order
+  std::vector<int> distinct(stage.begin(), stage.end());
+  for (int i = 0; i < nq; ++i)
+    for (int col : {qs[i].pred_col, qs[i].key_col, qs[i].x_col, qs[i].y_col})
+      if (std::find(distinct.begin(), distinct.end(), col) == distinct.end()) distinct.push_back(col);
+  if (distinct.size() <= (size_t)BQ_SLOTS) {
+    b->args.nslots = (int)distinct.size();
+    for (int k = 0; k < BQ_SLOTS; ++k) b->args.slot_col[k] = k < b->args.nslots ? distinct[k] : distinct[0];
+    auto slot = [&](int col) { return (int32_t)(std::find(distinct.begin(), distinct.end(), col) - distinct.begin()); };
+    for (int i = 0; i < nq; ++i) {
+      b->hq[i].ps = slot(qs[i].pred_col);
+      b->hq[i].ks = slot(qs[i].key_col);
+      b->hq[i].xs = slot(qs[i].x_col);
+      b->hq[i].ys = slot(qs[i].y_col);
+    }
   }
   void *p;
   int r2 = batch_alloc(b, sizeof(BQuery) * nq, &p);

@@ -59,3 +59,27 @@ def run(t, limit=10, segment="BUILDING", date=DATE):
     stats = dict(dim_pass=int(cf.sum()), mid_pass=int(len(olc1)), fact_pass=int(len(lrow)),
                  groups=int(len(gk[0])))
     return top, groups, stats
+
+
+def run_cpu_parallel(t, threads=0, limit=10, segment="BUILDING", date=DATE):
+    """The same q3 through oracle/libq3cpu.so (OpenMP, the reference's sort/merge/segmented-
+    reduce algorithms); returns (top rows, threads used)."""
+    import ctypes as C
+    import os
+
+    lib = C.CDLL(os.path.join(os.path.dirname(os.path.abspath(__file__)), "..", "oracle", "libq3cpu.so"))
+    P = C.POINTER(C.c_int64)
+    lib.q3cpu_run.restype = C.c_int
+    lib.q3cpu_run.argtypes = [P, P, C.c_uint64, P, P, P, P, C.c_uint64, P, P, P, P, C.c_uint64,
+                              C.c_int64, C.c_int64, C.c_int, C.c_int, P]
+    lib.q3cpu_threads.restype = C.c_int
+    arr = {k: np.ascontiguousarray(t[k], np.int64) for k in ("c_key", "c_seg", "o_key", "o_cust", "o_date",
+                                                             "o_prio", "l_okey", "l_price", "l_disc", "l_ship")}
+    p = lambda k: arr[k].ctypes.data_as(P)
+    top = np.zeros((max(1, limit), 5), np.int64)
+    code = int(t["code_of"][SEGMENTS.index(segment)])
+    n = lib.q3cpu_run(p("c_key"), p("c_seg"), len(arr["c_key"]), p("o_key"), p("o_cust"), p("o_date"),
+                      p("o_prio"), len(arr["o_key"]), p("l_okey"), p("l_price"), p("l_disc"), p("l_ship"),
+                      len(arr["l_okey"]), code, date, threads, limit, top.ctypes.data_as(P))
+    used = threads if threads > 0 else lib.q3cpu_threads()
+    return [tuple(int(x) for x in top[i]) for i in range(n)], used

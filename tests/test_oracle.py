@@ -298,3 +298,14 @@ def test_order_python_sorted():
     perm = O.order([a, b], [1, 0])
     want = sorted(range(2000), key=lambda i: (-a[i], b[i]))  # Python's sort is stable
     assert list(perm) == want
+
+
+def test_parallel_cpu_q3_matches_oracle():
+    """The OpenMP q3 run used as the benchmark's CPU baseline returns the oracle's rows."""
+    import q3_reference as Q3
+
+    t = Q3.tables(31, 6000, 60000)
+    for seg, date in [("BUILDING", 19950315), ("FURNITURE", 19950310), ("AUTOMOBILE", 19930101)]:
+        want, _, _ = Q3.run(t, segment=seg, date=date)
+        got, used = Q3.run_cpu_parallel(t, 4, segment=seg, date=date)
+        assert got == want and used == 4
