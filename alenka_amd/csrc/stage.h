@@ -16,7 +16,8 @@ constexpr int STG_SLOTS = 2;
 constexpr int STG_MAX_BITS = 31;
 // Words a slot can hold: the 16-byte blocks covering one tile's codes at the widest staged
 // width (ceil(4096 * 31 / 32) + 1 words, plus up to 3 words of the first block before them).
-constexpr int STG_SLOT_WORDS = ((STG_TILE * STG_MAX_BITS + 31) / 32 + 1 + 3 + 3) / 4 * 4;
+// (+4: stage_code reads the word after a code's first word even when the code ends inside it)
+constexpr int STG_SLOT_WORDS = ((STG_TILE * STG_MAX_BITS + 31) / 32 + 1 + 3 + 3) / 4 * 4 + 4;
 constexpr int STG_VEC = 8;  // 16-byte loads a thread issues before it stores any
 
 struct StageSlot {
@@ -38,6 +39,32 @@ __device__ __forceinline__ int64_t stage_get(const uint32_t *lds, const StageSlo
   uint64_t lo = lds[wi];
   if (sh + s.bits > 32) lo |= (uint64_t)lds[wi + 1] << 32;
   return (int64_t)((uint64_t)s.base + ((lo >> sh) & ((1ull << s.bits) - 1)));
+}
+
+// Raw code (value - base) of row i of a staged slot: one funnel shift of two LDS words.
+__device__ __forceinline__ uint32_t stage_code(const uint32_t *lds, const StageSlot &s, uint32_t i) {
+  const uint32_t pos = s.bit0 + i * s.bits;
+  const uint32_t wi = pos >> 5;
+  return __builtin_amdgcn_alignbit(lds[wi + 1], lds[wi], pos & 31) & ((1u << s.bits) - 1u);
+}
+
+// A range lo <= v <= hi on a staged column, restated on the tile's codes: v - base in
+// [clo, clo + cspan], tested as one unsigned compare (code - clo <= cspan); `none` when no
+// code of the tile's width can fall inside.
+struct CodeRange {
+  uint32_t clo, cspan;
+  bool none;
+};
+__device__ __forceinline__ CodeRange code_range(const StageSlot &s, int64_t lo, int64_t hi) {
+  CodeRange r;
+  const uint64_t maxc = s.bits ? (1ull << s.bits) - 1 : 0;
+  const uint64_t L = lo <= s.base ? 0 : (uint64_t)lo - (uint64_t)s.base;
+  const bool below = hi < s.base;
+  const uint64_t H = below ? 0 : ((uint64_t)hi - (uint64_t)s.base < maxc ? (uint64_t)hi - (uint64_t)s.base : maxc);
+  r.none = below || L > H;
+  r.clo = r.none ? 0 : (uint32_t)L;
+  r.cspan = r.none ? 0 : (uint32_t)(H - L);
+  return r;
 }
 
 // Stages the listed columns (first STG_SLOTS with at most 31 bits); every thread of the

@@ -89,6 +89,46 @@ __device__ __forceinline__ void star_stage(TileStage &ts, int *want, const ScanD
   __syncthreads();
 }
 
+// The tile's ranges in code space (ranges on staged columns), built by thread 0 after
+// staging; tile_pass_fast then tests a staged range with one funnel shift and one compare.
+struct TilePred {
+  int8_t slot[ALN_MAX_RANGES];  // staged slot of range k, -1: decode from HBM
+  CodeRange cr[ALN_MAX_RANGES];
+  bool none;                    // a staged range admits no code of the tile
+};
+
+__device__ __forceinline__ void tile_pred_init(TilePred &tp, const ScanDesc &d, const TileStage &ts) {
+  if (threadIdx.x == 0) {
+    tp.none = false;
+    for (int k = 0; k < d.nr; ++k) {
+      const int s = ts.slot_of[d.rcol[k]];
+      tp.slot[k] = (int8_t)s;
+      if (s >= 0) {
+        tp.cr[k] = code_range(ts.slot[s], d.rlo[k], d.rhi[k]);
+        tp.none = tp.none || tp.cr[k].none;
+      }
+    }
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ bool tile_pass_fast(const ScanDesc &d, const TileStage &ts, const TilePred &tp,
+                                               char mode, uint32_t seg, uint32_t r0, uint32_t i) {
+  if (mode == 'A') return true;
+  if (tp.none) return false;
+  bool p = true;
+  for (int k = 0; k < d.nr; ++k) {
+    const int s = tp.slot[k];
+    if (s >= 0) {
+      p = p && (stage_code(ts.words[s], ts.slot[s], i) - tp.cr[k].clo <= tp.cr[k].cspan);
+    } else {
+      const int64_t v = decode_row(d.words[d.rcol[k]], d.segs[d.rcol[k]][seg], r0 + i);
+      p = p && v >= d.rlo[k] && v <= d.rhi[k];
+    }
+  }
+  return p;
+}
+
 __device__ __forceinline__ bool tile_pass(const ScanDesc &d, const TileStage &ts, char mode,
                                           uint32_t seg, uint32_t r0, uint32_t i) {
   if (mode == 'A') return true;
@@ -158,7 +198,7 @@ __global__ void __launch_bounds__(ST_BLOCK) k_star_mid(ScanDesc d, int kcol, int
                                                        uint32_t *__restrict__ mbits, int64_t mkmin,
                                                        uint32_t *dup) {
   // passing rows of tile b are written to entries [b * ST_TILE, b * ST_TILE + counts[b])
-  __shared__ uint32_t wc[ST_BLOCK / 64];
+  __shared__ uint32_t woff[ST_ITEMS][ST_BLOCK / 64];
   __shared__ TileStage ts;
   __shared__ int want[ALN_MAX_RANGES + 1];
   const uint32_t seg = blockIdx.x / d.tps, tile = blockIdx.x % d.tps;
@@ -176,15 +216,20 @@ __global__ void __launch_bounds__(ST_BLOCK) k_star_mid(ScanDesc d, int kcol, int
   const unsigned lane = __lane_id(), wid = threadIdx.x / 64;
   // predicate and foreign key from LDS for all items, then the dimension-bitmap words, the
   // keys of the joined rows and the mid-bitmap updates, each issued for all items at once
+  __shared__ TilePred tp;
+  tile_pred_init(tp, d, ts);
+  const int fs = ts.slot_of[fkcol];
+  const int64_t foff = fs >= 0 ? (int64_t)((uint64_t)ts.slot[fs].base - (uint64_t)dkmin) : 0;
   uint64_t dc[ST_ITEMS];
   uint32_t nprobe = 0;
 #pragma unroll
   for (int j = 0; j < ST_ITEMS; ++j) {
     const uint32_t i = j * ST_BLOCK + threadIdx.x;
     dc[j] = ~0ull;
-    if (i < nt && tile_pass(d, ts, mode, seg, r0, i)) {
+    if (i < nt && tile_pass_fast(d, ts, tp, mode, seg, r0, i)) {
       ++nprobe;
-      const uint64_t b = (uint64_t)tile_get(ts, d.words, d.segs, fkcol, seg, r0, i) - (uint64_t)dkmin;
+      const uint64_t b = fs >= 0 ? (uint64_t)((int64_t)stage_code(ts.words[fs], ts.slot[fs], i) + foff)
+                                 : (uint64_t)tile_get(ts, d.words, d.segs, fkcol, seg, r0, i) - (uint64_t)dkmin;
       if (b < drange) dc[j] = b;
     }
   }
@@ -216,32 +261,35 @@ __global__ void __launch_bounds__(ST_BLOCK) k_star_mid(ScanDesc d, int kcol, int
     if (twice) atomicExch(dup, 1u);
   }
   block_count(nprobe, probed);
-  uint32_t wsum = mine;
-  for (int o = 32; o > 0; o >>= 1) wsum += __shfl_xor(wsum, o, 64);
-  if (lane == 0) wc[wid] = wsum;
+  // passing rows in item-major order (consecutive lanes write consecutive entries)
+#pragma unroll
+  for (int j = 0; j < ST_ITEMS; ++j) {
+    const uint64_t b = __ballot((passmask >> j) & 1);
+    if (lane == 0) woff[j][wid] = (uint32_t)__popcll(b);
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
     uint32_t t = 0;
-    for (int w = 0; w < ST_BLOCK / 64; ++w) t += wc[w];
+    for (int j = 0; j < ST_ITEMS; ++j)
+      for (int w = 0; w < ST_BLOCK / 64; ++w) {
+        const uint32_t c = woff[j][w];
+        woff[j][w] = t;
+        t += c;
+      }
     counts[blockIdx.x] = t;
   }
-  uint64_t pos = (uint64_t)blockIdx.x * ST_TILE;
-  for (int w = 0; w < (int)wid; ++w) pos += wc[w];
-  // order inside the tile: (item, lane) within the wave; fine, output order is free
-  uint32_t x = mine;
-  for (int o = 1; o < 64; o <<= 1) {
-    uint32_t y = __shfl_up(x, o, 64);
-    if (lane >= (unsigned)o) x += y;
-  }
-  pos += x - mine;
+  __syncthreads();
+  const uint64_t base = (uint64_t)blockIdx.x * ST_TILE;
 #pragma unroll
   for (int j = 0; j < ST_ITEMS; ++j) {
-    if (!((passmask >> j) & 1)) continue;
-    uint32_t i = j * ST_BLOCK + threadIdx.x;
+    const bool h = (passmask >> j) & 1;
+    const uint64_t b = __ballot(h);
+    if (!h) continue;
+    const uint32_t i = j * ST_BLOCK + threadIdx.x;
+    const uint64_t pos = base + woff[j][wid] + (uint32_t)__popcll(b & lanemask_lt());
     okey[pos] = key[j];
     og1[pos] = g1col >= 0 ? tile_get(ts, d.words, d.segs, g1col, seg, r0, i) : 0;
     og2[pos] = g2col >= 0 ? tile_get(ts, d.words, d.segs, g2col, seg, r0, i) : 0;
-    ++pos;
   }
 }
 
@@ -450,17 +498,22 @@ __global__ void __launch_bounds__(ST_BLOCK) k_star_fact_bm(ScanDesc d, int fkcol
   }
   const uint32_t nt = nrows - r0 < (uint32_t)ST_TILE ? nrows - r0 : (uint32_t)ST_TILE;
   star_stage(ts, want, d, mode, fkcol, seg, r0, nt);
+  __shared__ TilePred tp;
+  tile_pred_init(tp, d, ts);
   const unsigned lane = __lane_id(), wave = threadIdx.x / 64;
-  // 1. predicate and key from LDS for all items, then every bitmap word load at once
+  // 1. predicate and key from LDS (code space) for all items, then every bitmap word load
+  const int fs = ts.slot_of[fkcol];
+  const int64_t foff = fs >= 0 ? (int64_t)((uint64_t)ts.slot[fs].base - (uint64_t)mkmin) : 0;
   uint32_t code[ST_ITEMS];
   uint32_t nprobe = 0;
 #pragma unroll
   for (int j = 0; j < ST_ITEMS; ++j) {
     const uint32_t i = j * ST_BLOCK + threadIdx.x;
     code[j] = 0xFFFFFFFFu;
-    if (i < nt && tile_pass(d, ts, mode, seg, r0, i)) {
+    if (i < nt && tile_pass_fast(d, ts, tp, mode, seg, r0, i)) {
       ++nprobe;
-      const uint64_t c = (uint64_t)tile_get(ts, d.words, d.segs, fkcol, seg, r0, i) - (uint64_t)mkmin;
+      const uint64_t c = fs >= 0 ? (uint64_t)((int64_t)stage_code(ts.words[fs], ts.slot[fs], i) + foff)
+                                 : (uint64_t)tile_get(ts, d.words, d.segs, fkcol, seg, r0, i) - (uint64_t)mkmin;
       if (c < mrange) code[j] = (uint32_t)c;  // mrange <= 2^31 on this path
     }
   }
@@ -690,11 +743,196 @@ static int make_scan(aln_ctx *c, const aln_table_scan &t, int need_col, ScanDesc
   return ALN_OK;
 }
 
-// The first k groups by (sum DESC, g1 ASC, key ASC, g2 ASC): tiles of 1024 candidates are
-// sorted and cut to k, repeatedly, until one tile remains (synchronises).
+// ---- fast top-k for small limits ------------------------------------------------------
+// The order (sum DESC, g1 ASC, key ASC) is a 128-bit key: hi = sum with its sign bit flipped
+// and complemented (ascending hi = descending sum), lo = (g1 - g1min) << 32 | (key - kmin).
+// Keys are unique per group, so the order is total and g2 never decides.  Every thread keeps
+// its best TS_K candidates sorted in registers (unrolled insertion: static indices only); a
+// block then merges its threads' lists by k rounds of a block-wide minimum over the list
+// heads.  Pass 1 reduces the groups to k per block, pass 2 (one block) to the final k.
+constexpr int TS_K = 10;
+constexpr int TS_BLOCK = 128;
+
+struct TopList {
+  uint64_t hi[TS_K], lo[TS_K];
+  uint32_t j[TS_K];
+};
+
+__device__ __forceinline__ bool ts_less(uint64_t h, uint64_t l, uint64_t H, uint64_t L) {
+  return h < H || (h == H && l < L);
+}
+
+__device__ __forceinline__ void ts_insert(TopList &t, uint64_t h, uint64_t l, uint32_t j) {
+  if (!ts_less(h, l, t.hi[TS_K - 1], t.lo[TS_K - 1])) return;
+#pragma unroll
+  for (int k = TS_K - 1; k > 0; --k) {
+    if (ts_less(h, l, t.hi[k - 1], t.lo[k - 1])) {
+      t.hi[k] = t.hi[k - 1];
+      t.lo[k] = t.lo[k - 1];
+      t.j[k] = t.j[k - 1];
+    } else if (ts_less(h, l, t.hi[k], t.lo[k])) {
+      t.hi[k] = h;
+      t.lo[k] = l;
+      t.j[k] = j;
+    }
+  }
+  if (ts_less(h, l, t.hi[0], t.lo[0])) {
+    t.hi[0] = h;
+    t.lo[0] = l;
+    t.j[0] = j;
+  }
+}
+
+// Candidates: the groups (ihi == nullptr: group j valid when cnt[j] > 0) or the (hi, lo, j)
+// triples of a previous pass.  Writes the block's best k triples (hi = ~0: none).
+__global__ void __launch_bounds__(TS_BLOCK) k_topk_sel(uint64_t n, const unsigned long long *__restrict__ sum,
+                                                       const unsigned int *__restrict__ cnt,
+                                                       const int64_t *__restrict__ okey,
+                                                       const int64_t *__restrict__ og1, int64_t g1min,
+                                                       int64_t kmin, const uint64_t *__restrict__ ihi,
+                                                       const uint64_t *__restrict__ ilo,
+                                                       const uint32_t *__restrict__ ij, uint32_t k,
+                                                       uint64_t *__restrict__ ohi, uint64_t *__restrict__ olo,
+                                                       uint32_t *__restrict__ oj) {
+  __shared__ uint64_t shi[TS_BLOCK * TS_K], slo[TS_BLOCK * TS_K];
+  __shared__ uint32_t sj[TS_BLOCK * TS_K];
+  __shared__ uint64_t whi[TS_BLOCK / 64], wlo[TS_BLOCK / 64];
+  __shared__ uint32_t wt[TS_BLOCK / 64];
+  TopList t;
+#pragma unroll
+  for (int q = 0; q < TS_K; ++q) {
+    t.hi[q] = ~0ull;
+    t.lo[q] = ~0ull;
+    t.j[q] = 0xFFFFFFFFu;
+  }
+  const uint64_t stride = (uint64_t)gridDim.x * TS_BLOCK;
+  if (ihi) {
+    for (uint64_t i = (uint64_t)blockIdx.x * TS_BLOCK + threadIdx.x; i < n; i += stride)
+      ts_insert(t, ihi[i], ilo[i], ij[i]);
+  } else {
+    // four candidates' loads in flight before the first insertion
+    constexpr int U = 4;
+    for (uint64_t i0 = (uint64_t)blockIdx.x * TS_BLOCK + threadIdx.x; i0 < n; i0 += U * stride) {
+      uint64_t h[U], l[U];
+      bool v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const uint64_t i = i0 + u * stride;
+        v[u] = i < n && cnt[i] > 0;
+        h[u] = v[u] ? ~((uint64_t)sum[i] ^ 0x8000000000000000ull) : ~0ull;
+        l[u] = v[u] ? ((uint64_t)(og1[i] - g1min) << 32) | (uint32_t)((uint64_t)okey[i] - (uint64_t)kmin) : ~0ull;
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (v[u]) ts_insert(t, h[u], l[u], (uint32_t)(i0 + u * stride));
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < TS_K; ++q) {
+    shi[threadIdx.x * TS_K + q] = t.hi[q];
+    slo[threadIdx.x * TS_K + q] = t.lo[q];
+    sj[threadIdx.x * TS_K + q] = t.j[q];
+  }
+  __syncthreads();
+  uint32_t head = 0;  // next entry of this thread's list
+  const unsigned lane = __lane_id(), wave = threadIdx.x / 64;
+  for (uint32_t r = 0; r < k; ++r) {
+    uint64_t h = head < TS_K ? shi[threadIdx.x * TS_K + head] : ~0ull;
+    uint64_t l = head < TS_K ? slo[threadIdx.x * TS_K + head] : ~0ull;
+    uint32_t w = threadIdx.x;
+    for (int o = 32; o > 0; o >>= 1) {
+      const uint64_t h2 = __shfl_xor(h, o, 64), l2 = __shfl_xor(l, o, 64);
+      const uint32_t w2 = __shfl_xor(w, o, 64);
+      if (ts_less(h2, l2, h, l) || (h2 == h && l2 == l && w2 < w)) {
+        h = h2;
+        l = l2;
+        w = w2;
+      }
+    }
+    if (lane == 0) {
+      whi[wave] = h;
+      wlo[wave] = l;
+      wt[wave] = w;
+    }
+    __syncthreads();
+    uint64_t bh = whi[0], bl = wlo[0];
+    uint32_t bw = wt[0];
+    for (int v = 1; v < TS_BLOCK / 64; ++v)
+      if (ts_less(whi[v], wlo[v], bh, bl) || (whi[v] == bh && wlo[v] == bl && wt[v] < bw)) {
+        bh = whi[v];
+        bl = wlo[v];
+        bw = wt[v];
+      }
+    const uint64_t o = (uint64_t)blockIdx.x * k + r;
+    if (threadIdx.x == 0) {
+      ohi[o] = bh;
+      olo[o] = bl;
+    }
+    if (threadIdx.x == bw) {  // the winner owns the entry: emit its group, advance its head
+      oj[o] = head < TS_K ? sj[threadIdx.x * TS_K + head] : 0xFFFFFFFFu;
+      if (head < TS_K) ++head;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_topk_emit(const uint32_t *__restrict__ j, uint32_t k,
+                            const unsigned long long *__restrict__ sum, const unsigned int *__restrict__ cnt,
+                            const int64_t *__restrict__ okey, const int64_t *__restrict__ og1,
+                            const int64_t *__restrict__ og2, aln_star_row *__restrict__ out,
+                            uint32_t *__restrict__ nvalid) {
+  const uint32_t i = threadIdx.x;
+  const bool ok = i < k && j[i] != 0xFFFFFFFFu;  // empty entries carry no group
+  if (ok) {
+    const uint32_t g = j[i];
+    aln_star_row r;
+    r.key = okey[g];
+    r.g1 = og1[g];
+    r.g2 = og2[g];
+    r.sum = (int64_t)sum[g];
+    r.count = cnt[g];
+    out[i] = r;
+  }
+  const uint64_t b = __ballot(ok);  // k <= TS_K < 64: one wave; valid entries come first
+  if (i == 0) *nvalid = (uint32_t)__popcll(b);
+}
+
+// The first k groups by (sum DESC, g1 ASC, key ASC, g2 ASC) (synchronises).  Small k with a
+// g1 range below 2^32 take the register selection above; otherwise tiles of 1024
+// candidates are sorted and cut to k, repeatedly, until one tile remains.
 static int star_topk(aln_ctx *c, uint64_t m, const unsigned long long *sum, const unsigned int *cnt,
                      const int64_t *okey, const int64_t *og1, const int64_t *og2, uint32_t k,
-                     aln_star_row *h_top, uint32_t *h_ntop) {
+                     int64_t g1min, int64_t g1max, int64_t kmin, aln_star_row *h_top, uint32_t *h_ntop) {
+  if (k <= (uint32_t)TS_K && g1max >= g1min && (uint64_t)g1max - (uint64_t)g1min < (1ull << 32) &&
+      m < (1ull << 32)) {
+    const uint32_t nb = (uint32_t)std::min<uint64_t>((m + TS_BLOCK * 64 - 1) / (TS_BLOCK * 64), 1024);
+    uint64_t *h1, *l1, *h2, *l2;
+    uint32_t *j1, *j2, *nv;
+    aln_star_row *rows;
+    ALN_TRY(ws(c, (size_t)nb * k, &h1));
+    ALN_TRY(ws(c, (size_t)nb * k, &l1));
+    ALN_TRY(ws(c, (size_t)nb * k, &j1));
+    ALN_TRY(ws(c, k, &h2));
+    ALN_TRY(ws(c, k, &l2));
+    ALN_TRY(ws(c, k, &j2));
+    ALN_TRY(ws(c, 1, &nv));
+    ALN_TRY(ws(c, k, &rows));
+    {
+      ProfScope p(c, "star_topk");
+      hipLaunchKernelGGL(k_topk_sel, dim3(nb), dim3(TS_BLOCK), 0, c->stream, m, sum, cnt, okey, og1, g1min, kmin,
+                         (const uint64_t *)nullptr, (const uint64_t *)nullptr, (const uint32_t *)nullptr, k, h1,
+                         l1, j1);
+      hipLaunchKernelGGL(k_topk_sel, dim3(1), dim3(TS_BLOCK), 0, c->stream, (uint64_t)nb * k, sum, cnt, okey, og1,
+                         g1min, kmin, h1, l1, j1, k, h2, l2, j2);
+      hipLaunchKernelGGL(k_topk_emit, dim3(1), dim3(64), 0, c->stream, j2, k, sum, cnt, okey, og1, og2, rows, nv);
+    }
+    ALN_TRY(check_launch("star_topk"));
+    uint32_t n = 0;
+    ALN_TRY(copy_to_host(c, &n, nv, 4));
+    if (n) ALN_TRY(copy_to_host(c, h_top, rows, sizeof(aln_star_row) * n));
+    *h_ntop = n;
+    return ALN_OK;
+  }
   const Cand *in = nullptr;
   uint64_t n = m;
   while (true) {
@@ -915,7 +1153,11 @@ extern "C" int aln_star_agg(aln_ctx *c, const aln_star_args *a, aln_star_row *h_
   tm.mark(3);
 
   // top-k
-  if (a->limit && m) ALN_TRY(star_topk(c, m, sum, cnt, okey, og1, og2, a->limit, h_top, h_ntop));
+  if (a->limit && m) {
+    int64_t g1min = 0, g1max = 0;
+    if (a->m_g1 >= 0) ALN_TRY(col_range(c, a->mid.cols[a->m_g1], &g1min, &g1max));
+    ALN_TRY(star_topk(c, m, sum, cnt, okey, og1, og2, a->limit, g1min, g1max, mkmin, h_top, h_ntop));
+  }
   tm.mark(4);
   ALN_TRY(copy_to_host(c, hc, ctr, 40));
   uint32_t bdup = 0;
@@ -1010,21 +1252,35 @@ __device__ __forceinline__ char sb_mode(const SBScan &t, const SBRanges &r, uint
   return m;
 }
 
-__device__ __forceinline__ bool sb_pass(const SBScan &t, const TileStage &ts, const SBRanges &r, char mode,
-                                        uint32_t seg, uint32_t r0, uint32_t i) {
+// A query's ranges on one table restated on the staged tile's codes (see TilePred).
+struct SBPred {
+  int8_t slot[ALN_MAX_RANGES];
+  CodeRange cr[ALN_MAX_RANGES];
+  bool none;
+};
+
+__device__ __forceinline__ bool sb_pass_fast(const SBScan &t, const TileStage &ts, const SBRanges &r,
+                                             const SBPred &sp, char mode, uint32_t seg, uint32_t r0, uint32_t i) {
   if (mode == 'A') return true;
+  if (sp.none) return false;
   bool p = true;
   for (int k = 0; k < r.nr; ++k) {
-    const int64_t v = tile_get(ts, t.words, t.segs, r.col[k], seg, r0, i);
-    p = p && v >= r.lo[k] && v <= r.hi[k];
+    const int s = sp.slot[k];
+    if (s >= 0) {
+      p = p && (stage_code(ts.words[s], ts.slot[s], i) - sp.cr[k].clo <= sp.cr[k].cspan);
+    } else {
+      const int64_t v = decode_row(t.words[r.col[k]], t.segs[r.col[k]][seg], r0 + i);
+      p = p && v >= r.lo[k] && v <= r.hi[k];
+    }
   }
   return p;
 }
 
 // Block prologue: zone-map mode of the segment for every query and the tile staged in LDS
-// (skipped when no query reads the segment).  Returns false when the block has no work.
+// (skipped when no query reads the segment), then each query's ranges in code space.
+// Returns false when the block has no work.
 __device__ __forceinline__ bool sb_prologue(const SBScan &t, const SBQuery *q, int nq, int table,
-                                            TileStage &ts, char *qmode, uint32_t seg, uint32_t r0,
+                                            TileStage &ts, char *qmode, SBPred *sp, uint32_t seg, uint32_t r0,
                                             uint32_t nt) {
   __shared__ int any;
   __shared__ int want[2];
@@ -1039,6 +1295,20 @@ __device__ __forceinline__ bool sb_prologue(const SBScan &t, const SBQuery *q, i
   if (!any) return false;
   stage_tile(ts, t.words, t.segs, want, t.nwant, seg, r0, nt);
   __syncthreads();
+  if (threadIdx.x < (unsigned)nq) {
+    const SBRanges &r = q[threadIdx.x].r[table];
+    SBPred &p = sp[threadIdx.x];
+    p.none = false;
+    for (int k = 0; k < r.nr; ++k) {
+      const int s = ts.slot_of[r.col[k]];
+      p.slot[k] = (int8_t)s;
+      if (s >= 0) {
+        p.cr[k] = code_range(ts.slot[s], r.lo[k], r.hi[k]);
+        p.none = p.none || p.cr[k].none;
+      }
+    }
+  }
+  __syncthreads();
   return true;
 }
 
@@ -1047,10 +1317,11 @@ __global__ void __launch_bounds__(ST_BLOCK) k_sb_dim(SBScan t, const SBQuery *__
                                                      uint32_t *dup) {
   __shared__ TileStage ts;
   __shared__ char qmode[SB_MAX];
+  __shared__ SBPred sp[SB_MAX];
   const uint32_t seg = blockIdx.x / t.tps, tile = blockIdx.x % t.tps;
   const uint32_t nrows = t.segs[0][seg].nrows, r0 = tile * ST_TILE;
   const uint32_t nt = r0 < nrows ? (nrows - r0 < (uint32_t)ST_TILE ? nrows - r0 : (uint32_t)ST_TILE) : 0;
-  if (!nt || !sb_prologue(t, q, nq, 0, ts, qmode, seg, r0, nt)) {
+  if (!nt || !sb_prologue(t, q, nq, 0, ts, qmode, sp, seg, r0, nt)) {
     if (threadIdx.x < (unsigned)nq) npass[(uint64_t)threadIdx.x * gridDim.x + blockIdx.x] = 0;
     return;
   }
@@ -1063,7 +1334,7 @@ __global__ void __launch_bounds__(ST_BLOCK) k_sb_dim(SBScan t, const SBQuery *__
 #pragma unroll 4
       for (int j = 0; j < ST_ITEMS; ++j) {
         const uint32_t i = j * ST_BLOCK + threadIdx.x;
-        if (i < nt && sb_pass(t, ts, qq.r[0], mode, seg, r0, i)) {
+        if (i < nt && sb_pass_fast(t, ts, qq.r[0], sp[qi], mode, seg, r0, i)) {
           const uint64_t b = (uint64_t)tile_get(ts, t.words, t.segs, kcol, seg, r0, i) - (uint64_t)kmin;
           if (b < range) {
             const uint32_t bit = 1u << (b & 31);
@@ -1086,10 +1357,11 @@ __global__ void __launch_bounds__(ST_BLOCK) k_sb_mid(SBScan t, const SBQuery *__
                                                      uint32_t *dup) {
   __shared__ TileStage ts;
   __shared__ char qmode[SB_MAX];
+  __shared__ SBPred sp[SB_MAX];
   const uint32_t seg = blockIdx.x / t.tps, tile = blockIdx.x % t.tps;
   const uint32_t nrows = t.segs[0][seg].nrows, r0 = tile * ST_TILE;
   const uint32_t nt = r0 < nrows ? (nrows - r0 < (uint32_t)ST_TILE ? nrows - r0 : (uint32_t)ST_TILE) : 0;
-  if (!nt || !sb_prologue(t, q, nq, 1, ts, qmode, seg, r0, nt)) {
+  if (!nt || !sb_prologue(t, q, nq, 1, ts, qmode, sp, seg, r0, nt)) {
     if (threadIdx.x < (unsigned)nq) {
       nprobe_out[(uint64_t)threadIdx.x * gridDim.x + blockIdx.x] = 0;
       npass_out[(uint64_t)threadIdx.x * gridDim.x + blockIdx.x] = 0;
@@ -1116,7 +1388,7 @@ __global__ void __launch_bounds__(ST_BLOCK) k_sb_mid(SBScan t, const SBQuery *__
 #pragma unroll
       for (int j = 0; j < ST_ITEMS; ++j) {
         const uint32_t i = j * ST_BLOCK + threadIdx.x;
-        const bool p = i < nt && sb_pass(t, ts, qq.r[1], mode, seg, r0, i);
+        const bool p = i < nt && sb_pass_fast(t, ts, qq.r[1], sp[qi], mode, seg, r0, i);
         np += p;
         probe |= (uint32_t)(p && fkc[j] < drange) << j;
       }
@@ -1188,6 +1460,7 @@ __global__ void __launch_bounds__(ST_BLOCK) k_sb_fact(SBScan t, const SBQuery *_
                                                       uint32_t *nprobe_out, uint32_t *nmatch_out) {
   __shared__ TileStage ts;
   __shared__ char qmode[SB_MAX];
+  __shared__ SBPred sp[SB_MAX];
   __shared__ uint32_t hcode[SB_CHUNK * ST_BLOCK];
   __shared__ uint16_t hrow[SB_CHUNK * ST_BLOCK];
   __shared__ uint32_t woff[SB_CHUNK][ST_BLOCK / 64];
@@ -1195,7 +1468,7 @@ __global__ void __launch_bounds__(ST_BLOCK) k_sb_fact(SBScan t, const SBQuery *_
   const uint32_t seg = blockIdx.x / t.tps, tile = blockIdx.x % t.tps;
   const uint32_t nrows = t.segs[0][seg].nrows, r0 = tile * ST_TILE;
   const uint32_t nt = r0 < nrows ? (nrows - r0 < (uint32_t)ST_TILE ? nrows - r0 : (uint32_t)ST_TILE) : 0;
-  if (!nt || !sb_prologue(t, q, nq, 2, ts, qmode, seg, r0, nt)) {
+  if (!nt || !sb_prologue(t, q, nq, 2, ts, qmode, sp, seg, r0, nt)) {
     if (threadIdx.x < (unsigned)nq) {
       nprobe_out[(uint64_t)threadIdx.x * gridDim.x + blockIdx.x] = 0;
       nmatch_out[(uint64_t)threadIdx.x * gridDim.x + blockIdx.x] = 0;
@@ -1221,7 +1494,7 @@ __global__ void __launch_bounds__(ST_BLOCK) k_sb_fact(SBScan t, const SBQuery *_
         for (int u = 0; u < SB_CHUNK; ++u) {
           const int j = ch * SB_CHUNK + u;
           const uint32_t i = j * ST_BLOCK + threadIdx.x;
-          const bool p = i < nt && sb_pass(t, ts, qq.r[2], mode, seg, r0, i);
+          const bool p = i < nt && sb_pass_fast(t, ts, qq.r[2], sp[qi], mode, seg, r0, i);
           nprobe += p;
           code[u] = p ? fkc[j] : 0xFFFFFFFFu;
         }
@@ -1502,10 +1775,12 @@ extern "C" int aln_star_agg_batch(aln_ctx *c, const aln_star_args *a, int nq, al
     ALN_TRY(sum_u32(c, fprobe + (uint64_t)i * fblocks, fblocks, ctr + 5 * i + 3));
     ALN_TRY(sum_u32(c, fmatch + (uint64_t)i * fblocks, fblocks, ctr + 5 * i + 4));
   }
+  int64_t g1min = 0, g1max = 0;
+  if (a0.m_g1 >= 0) ALN_TRY(col_range(c, a0.mid.cols[a0.m_g1], &g1min, &g1max));
   for (int i = 0; i < nq; ++i)
     if (a[i].limit && st[i].mid_pass)
       ALN_TRY(star_topk(c, st[i].mid_pass, hq[i].sum, hq[i].cnt, hq[i].okey, hq[i].og1, hq[i].og2, a[i].limit,
-                        h_top[i], &h_ntop[i]));
+                        g1min, g1max, mkmin, h_top[i], &h_ntop[i]));
   ALN_TRY(copy_to_host(c, hc.data(), ctr, 8 * 5 * SB_MAX));
   for (int i = 0; i < nq; ++i) {
     st[i].fact_probe = hc[5 * i + 3];

@@ -379,30 +379,31 @@ def serve_leg(ctx, D, dim, mid, fact, zc, code_of, seed, rate, duration, budget_
     pt = lambda f, r, d, sd: serve_point(D, forms[f], budget_gb, flat, facts, stars, scales, codes, sd, r, d)
     out = {"templates": ["flat filter+group-by", "star q3", "star q3 (large)", "fact filter+group-by (Zipf)"],
            "scales": scales, "budget_gb": budget_gb}
+    # every point holds at least `nreq` requests so that its p99 is not one request's latency
+    nreq = 300
+    span = lambda r: min(max(duration, nreq / r), 15.0)
     sat = {}
     for f in forms:
-        burst = pt(f, 1e6, 150 / 1e6, seed + 21)  # 150 requests arriving at once
+        burst = pt(f, 1e6, nreq / 1e6, seed + 21)  # nreq requests arriving at once
         sat[f] = burst["qps"]
         out[f + "_saturation"] = burst
         log(f"serve {f} saturation: {burst['qps']:.1f} q/s")
-    slo_point = pt("plain", 0.8 * sat["plain"], duration, seed + 22)
+    slo_point = pt("plain", 0.8 * sat["plain"], span(0.8 * sat["plain"]), seed + 22)
     slo = slo_point["p99_ms"]
     out["slo"] = {"p99_ms": slo, "from": "plain form at 80% of its saturation", "point": slo_point}
     log(f"serve SLO p99 {slo:.1f} ms")
     for f in forms:
         best = None
         for k, frac in enumerate((0.5, 0.7, 0.85, 1.0, 1.2)):
-            r = pt(f, frac * sat[f], duration, seed + 30 + k)
+            r = pt(f, frac * sat[f], span(frac * sat[f]), seed + 30 + k)
             log(f"serve {f} at {r['offered_qps']:.1f} q/s: {r['qps']:.1f} q/s, p99 {r['p99_ms']:.1f} ms")
-            if r["p99_ms"] <= slo and r["failed"] == 0:
+            if r["p99_ms"] <= slo and r["failed"] == 0 and (best is None or r["qps"] > best["qps"]):
                 best = r
-            else:
-                break
         out[f + "_at_slo"] = best
     if out["served_at_slo"] and out["plain_at_slo"]:
         out["speedup_qps_at_equal_p99"] = out["served_at_slo"]["qps"] / out["plain_at_slo"]["qps"]
     for f in forms:
-        out[f + "_at_rate"] = pt(f, rate, duration, seed + 11)
+        out[f + "_at_rate"] = pt(f, rate, span(rate), seed + 11)
         log(f"serve {f} at {rate} q/s: {out[f + '_at_rate']}")
     del flat, facts, stars
     torch.cuda.empty_cache()
@@ -424,6 +425,9 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--sort-rows", type=int, default=100_000_000)
     ap.add_argument("--no-serve", action="store_true", help="skip the served-mix leg")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first run a plain device copy of 1 GiB (the known-size kernel that scales "
+                         "FETCH_SIZE / WRITE_SIZE counters to bytes)")
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-seconds", type=float, default=8.0)
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
@@ -438,6 +442,15 @@ def main():
     from alenka_amd import device as D
 
     ctx = D.Context(0)
+    if args.calibrate_copy:
+        src = torch.empty(1 << 27, dtype=torch.int64, device="cuda")  # 1 GiB
+        src.fill_(1)
+        for _ in range(3):
+            dst = src.clone()
+        torch.cuda.synchronize()
+        del src, dst
+        torch.cuda.empty_cache()
+        log("copy calibration: 3 x clone of 1 GiB")
     dim, mid, fact, code_of, nfact = build_tables(ctx, D, args.seed, args.scale, args.seg_rows)
     D_, M_, F_, spec = q3_spec(code_of, dim, mid, fact)
     run = lambda: D.star_agg(ctx, D_, M_, F_, spec, limit=10)
