@@ -127,6 +127,8 @@ SIGNATURES = {
     "aln_encode_pack": (C.c_int, [P, P, U64, U32, C.POINTER(Seg), P, P]),
     "aln_plain_col": (C.c_int, [P, P, U64, U32, I32, P, C.POINTER(Col)]),
     "aln_decode": (C.c_int, [P, C.POINTER(Col), U32, U64, U64, P]),
+    "aln_segfile_decode": (C.c_int, [P, C.c_char_p, U64, P, U64, PU64, PI64, PI64]),
+    "aln_segfile_encode": (C.c_int, [P, P, U64, C.c_char_p, U64, PU64]),
     "aln_scan_filter": (C.c_int, [P, C.POINTER(Col), C.c_int, U32, C.POINTER(Insn), C.c_int,
                                   C.c_char_p, P, U64, PU64]),
     "aln_gather": (C.c_int, [P, C.POINTER(Col), U32, P, U64, P]),

@@ -18,7 +18,6 @@ constexpr int STG_MAX_BITS = 31;
 // width (ceil(4096 * 31 / 32) + 1 words, plus up to 3 words of the first block before them).
 // (+4: stage_code reads the word after a code's first word even when the code ends inside it)
 constexpr int STG_SLOT_WORDS = ((STG_TILE * STG_MAX_BITS + 31) / 32 + 1 + 3 + 3) / 4 * 4 + 4;
-constexpr int STG_VEC = 8;  // 16-byte loads a thread issues before it stores any
 
 struct StageSlot {
   int64_t base;
@@ -70,12 +69,14 @@ __device__ __forceinline__ CodeRange code_range(const StageSlot &s, int64_t lo, 
 // Stages the listed columns (first STG_SLOTS with at most 31 bits); every thread of the
 // block must call it, followed by __syncthreads().  Each slot receives the aligned 16-byte
 // blocks that cover the tile's codes (an aligned block never crosses a page, so the few
-// words outside the column's range it may carry are readable), copied with up to STG_VEC
-// independent 16-byte loads per thread in flight before the first store: the block keeps
+// words outside the column's range it may carry are readable), copied with all of a thread's
+// independent 16-byte loads (8 with 256 threads) in flight before the first store: the block keeps
 // tens of KiB of loads outstanding instead of one dword per thread.
+template <int BLOCK = STG_BLOCK>
 __device__ __forceinline__ void stage_tile(TileStage &ts, const uint32_t *const *words,
                                            const aln_seg *const *segs, const int *want, int nwant,
                                            uint32_t seg, uint32_t r0, uint32_t nt) {
+  constexpr int VEC = (2 * (STG_SLOT_WORDS / 4) + BLOCK - 1) / BLOCK;  // loads per thread
   if (threadIdx.x < ALN_MAX_COLS) ts.slot_of[threadIdx.x] = -1;
   __syncthreads();
   static_assert(STG_SLOTS == 2, "the copy loop below addresses two slots");
@@ -112,19 +113,19 @@ __device__ __forceinline__ void stage_tile(TileStage &ts, const uint32_t *const 
     ++used;
   }
   // one index space over the blocks of both slots: v < n0 is slot 0, then slot 1; two
-  // slots of at most STG_SLOT_WORDS / 4 blocks fit one round of STG_VEC loads per thread
-  static_assert(2 * (STG_SLOT_WORDS / 4) <= STG_VEC * STG_BLOCK, "one round of loads covers both slots");
+  // slots of at most STG_SLOT_WORDS / 4 blocks fit one round of VEC loads per thread
+  static_assert(2 * (STG_SLOT_WORDS / 4) <= VEC * BLOCK, "one round of loads covers both slots");
   uint4 *dst0 = reinterpret_cast<uint4 *>(ts.words[0]), *dst1 = reinterpret_cast<uint4 *>(ts.words[1]);
   const uint32_t ntot = n0 + n1;
-  uint4 r[STG_VEC];
+  uint4 r[VEC];
 #pragma unroll
-  for (int k = 0; k < STG_VEC; ++k) {
-    const uint32_t v = k * STG_BLOCK + threadIdx.x;
+  for (int k = 0; k < VEC; ++k) {
+    const uint32_t v = k * BLOCK + threadIdx.x;
     r[k] = v < n0 ? src0[v] : (v < ntot ? src1[v - n0] : make_uint4(0, 0, 0, 0));
   }
 #pragma unroll
-  for (int k = 0; k < STG_VEC; ++k) {
-    const uint32_t v = k * STG_BLOCK + threadIdx.x;
+  for (int k = 0; k < VEC; ++k) {
+    const uint32_t v = k * BLOCK + threadIdx.x;
     if (v < n0)
       dst0[v] = r[k];
     else if (v < ntot)

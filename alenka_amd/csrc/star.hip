@@ -28,9 +28,12 @@
 
 namespace aln {
 
-constexpr int ST_BLOCK = 256;
-constexpr int ST_ITEMS = 16;
+// 512 threads x 8 rows per 4096-row tile: eight waves share one tile's 32 KB of staging, so
+// the LDS that caps a CU at four tiles still admits 32 waves.
+constexpr int ST_BLOCK = 512;
+constexpr int ST_ITEMS = 8;
 constexpr int ST_TILE = ST_BLOCK * ST_ITEMS;
+static_assert(ST_TILE == STG_TILE, "a block stages one tile");
 constexpr uint64_t ST_EMPTY = ~0ull;
 constexpr uint32_t ST_SEED = 0x1b873593u;
 
@@ -85,7 +88,7 @@ __device__ __forceinline__ void star_stage(TileStage &ts, int *want, const ScanD
     nwant = n;
   }
   __syncthreads();
-  stage_tile(ts, d.words, d.segs, want, nwant, seg, r0, nt);
+  stage_tile<ST_BLOCK>(ts, d.words, d.segs, want, nwant, seg, r0, nt);
   __syncthreads();
 }
 
@@ -170,10 +173,12 @@ __global__ void __launch_bounds__(ST_BLOCK) k_star_dim(ScanDesc d, int kcol, int
   }
   const uint32_t nt = nrows - r0 < (uint32_t)ST_TILE ? nrows - r0 : (uint32_t)ST_TILE;
   star_stage(ts, want, d, mode, kcol, seg, r0, nt);
+  __shared__ TilePred tp;
+  tile_pred_init(tp, d, ts);
   uint32_t cnt = 0;
   for (int j = 0; j < ST_ITEMS; ++j) {
     uint32_t i = j * ST_BLOCK + threadIdx.x;
-    if (i < nt && tile_pass(d, ts, mode, seg, r0, i)) {
+    if (i < nt && tile_pass_fast(d, ts, tp, mode, seg, r0, i)) {
       uint64_t b = (uint64_t)tile_get(ts, d.words, d.segs, kcol, seg, r0, i) - (uint64_t)kmin;
       if (b < range) {
         uint32_t bit = 1u << (b & 31);
@@ -424,10 +429,12 @@ __global__ void __launch_bounds__(256) k_bm_block_counts(const uint32_t *__restr
                                                         uint32_t *__restrict__ cnt) {
   for (uint64_t b = (uint64_t)blockIdx.x * 256 + threadIdx.x; b < nblk;
        b += (uint64_t)gridDim.x * 256) {
-    uint32_t c = 0;
-    uint64_t w0 = b * BM_BLOCK_WORDS;
-    for (int k = 0; k < BM_BLOCK_WORDS; ++k)
-      if (w0 + k < nwords) c += __popc(bits[w0 + k]);
+    // the bitmap is allocated (and zero-filled) in whole blocks: one 64-byte line
+    const uint4 *q = (const uint4 *)(bits + b * BM_BLOCK_WORDS);
+    const uint4 q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
+    uint32_t c = __popc(q0.x) + __popc(q0.y) + __popc(q0.z) + __popc(q0.w) + __popc(q1.x) + __popc(q1.y) +
+                 __popc(q1.z) + __popc(q1.w) + __popc(q2.x) + __popc(q2.y) + __popc(q2.z) + __popc(q2.w) +
+                 __popc(q3.x) + __popc(q3.y) + __popc(q3.z) + __popc(q3.w);
     cnt[b] = c;
   }
 }
@@ -1293,7 +1300,7 @@ __device__ __forceinline__ bool sb_prologue(const SBScan &t, const SBQuery *q, i
   }
   __syncthreads();
   if (!any) return false;
-  stage_tile(ts, t.words, t.segs, want, t.nwant, seg, r0, nt);
+  stage_tile<ST_BLOCK>(ts, t.words, t.segs, want, t.nwant, seg, r0, nt);
   __syncthreads();
   if (threadIdx.x < (unsigned)nq) {
     const SBRanges &r = q[threadIdx.x].r[table];
@@ -1423,7 +1430,7 @@ __global__ void __launch_bounds__(ST_BLOCK) k_sb_attr(SBScan t, const SBQuery *_
   const uint32_t nt = nrows - r0 < (uint32_t)ST_TILE ? nrows - r0 : (uint32_t)ST_TILE;
   if (threadIdx.x < 2) want[threadIdx.x] = t.want[threadIdx.x];
   __syncthreads();
-  stage_tile(ts, t.words, t.segs, want, t.nwant, seg, r0, nt);
+  stage_tile<ST_BLOCK>(ts, t.words, t.segs, want, t.nwant, seg, r0, nt);
   __syncthreads();
   // items in chunks (the rank of a hit reads a 64-byte line: a whole tile of hits in flight
   // would need more registers than a wave has)

@@ -152,6 +152,29 @@ def _cols_array(cols):
     return arr
 
 
+def segfile_decode(ctx, data: bytes):
+    """One segment file in the reference's format (FOR / FOR-DELTA) -> (int64 device tensor,
+    header lower, header upper)."""
+    rows, lo, hi = C.c_uint64(), C.c_int64(), C.c_int64()
+    rc = ctx.lib.aln_segfile_decode(ctx.h, data, len(data), None, 0, C.byref(rows), C.byref(lo), C.byref(hi))
+    if rc not in (L.ALN_OK, L.ALN_E_CAPACITY):
+        L.check(rc, "aln_segfile_decode")
+    out = ctx.empty(rows.value)
+    L.check(ctx.lib.aln_segfile_decode(ctx.h, data, len(data), _p(out), rows.value, C.byref(rows), C.byref(lo),
+                                       C.byref(hi)), "aln_segfile_decode")
+    return out[: rows.value], lo.value, hi.value
+
+
+def segfile_encode(ctx, vals):
+    """The FOR segment file the reference's pfor_compress writes for these values (bytes)."""
+    n = C.c_uint64()
+    L.check(ctx.lib.aln_segfile_encode(ctx.h, _p(vals), vals.numel(), None, 0, C.byref(n)), "aln_segfile_encode")
+    buf = C.create_string_buffer(n.value)
+    L.check(ctx.lib.aln_segfile_encode(ctx.h, _p(vals), vals.numel(), buf, n.value, C.byref(n)),
+            "aln_segfile_encode")
+    return buf.raw[: n.value]
+
+
 def scan_filter(ctx, cols, prog, seg_modes=None, cap=None):
     """Ascending global row ids of the rows that pass `prog` (compiled predicate)."""
     seg_rows = cols[0].seg_rows

@@ -147,6 +147,20 @@ int aln_plain_col(aln_ctx *ctx, const int64_t *d_vals, uint64_t nrows, uint32_t 
 int aln_decode(aln_ctx *ctx, const aln_col *col, uint32_t seg_rows, uint64_t row_begin,
                uint64_t row_end, int64_t *d_out);
 
+/* ---- the reference's segment files (compress.cu:180-527) --------------------------------
+ * One column segment in the on-disk layout pfor_compress / pfor_delta_compress write:
+ *   [u32 cnt][i64 lower][i64 upper][u32 comp_type][cnt payload bytes]
+ *   [u32 cnt][u32 rec_count][u32 bits][i64 orig_lower][u32 fit_count][i64 start_val]
+ * aln_segfile_decode (pfor_decompress, compress.cu:180-267) reads FOR (comp_type 0) and
+ * FOR-DELTA (comp_type 1) files given as host bytes into int64 values on the device and
+ * returns the header's zone map; aln_segfile_encode writes the FOR form of n device values
+ * (h_file NULL: only *h_len, the file size).  Encode with aln_encode_* to keep the values
+ * resident in this build's bit-packed layout. */
+int aln_segfile_decode(aln_ctx *ctx, const void *h_file, uint64_t len, int64_t *d_out, uint64_t cap,
+                       uint64_t *h_rows, int64_t *h_lower, int64_t *h_upper);
+int aln_segfile_encode(aln_ctx *ctx, const int64_t *d_vals, uint64_t n, void *h_file, uint64_t cap,
+                       uint64_t *h_len);
+
 /* ---- scan: fused decode + predicate + stream compaction (filter_op, cm.cu:3609-3666) --
  * cols: host array of ncols column descriptors sharing one segmentation (seg_rows rows per
  * segment).  seg_modes: host array of nsegs zone-map results ('A','N','R'; NULL = all 'R').
