@@ -30,6 +30,7 @@ namespace aln {
 
 // 512 threads x 8 rows per 4096-row tile: eight waves share one tile's 32 KB of staging, so
 // the LDS that caps a CU at four tiles still admits 32 waves.
+constexpr int ST_FACT_BLOCKS_PER_CU = 2;  // resident blocks of the persistent fact scan
 constexpr int ST_BLOCK = 512;
 constexpr int ST_ITEMS = 8;
 constexpr int ST_TILE = ST_BLOCK * ST_ITEMS;
@@ -600,6 +601,179 @@ __global__ void __launch_bounds__(ST_BLOCK) k_star_fact_bm(ScanDesc d, int fkcol
   block_count(nprobe, probed);
 }
 
+// Persistent form of the fact scan: a grid of a few blocks per CU walks the tiles; each block
+// issues the next tile's staging loads into registers before it computes the current tile,
+// so the HBM latency of staging is hidden behind the compute of the tile before.
+__device__ __forceinline__ bool fact_tile(const ScanDesc &d, uint64_t tile, uint32_t &seg, uint32_t &r0,
+                                          uint32_t &nt, char &mode) {
+  seg = (uint32_t)(tile / d.tps);
+  r0 = (uint32_t)(tile % d.tps) * ST_TILE;
+  const uint32_t nrows = d.segs[0][seg].nrows;
+  if (r0 >= nrows) return false;
+  mode = sd_mode(d, seg);
+  if (mode == 'N') return false;
+  nt = nrows - r0 < (uint32_t)ST_TILE ? nrows - r0 : (uint32_t)ST_TILE;
+  return true;
+}
+
+__global__ void __launch_bounds__(ST_BLOCK) k_star_fact_p(ScanDesc d, int fkcol, int xcol, int ycol, int64_t a0,
+                                                          int64_t a1, int64_t b0, int64_t b1,
+                                                          const uint32_t *__restrict__ bits,
+                                                          const uint64_t *__restrict__ rank, int64_t mkmin,
+                                                          uint64_t mrange, uint64_t ntiles,
+                                                          unsigned long long *__restrict__ sum,
+                                                          unsigned int *__restrict__ cnt,
+                                                          uint32_t *__restrict__ matched,
+                                                          uint32_t *__restrict__ probed) {
+  static_assert(sizeof(TileStage) >= ST_TILE * (sizeof(uint32_t) + sizeof(uint16_t)),
+                "hit list must fit in the staging area");
+  __shared__ TileStage ts;
+  __shared__ TilePred tp;
+  __shared__ int want[2][ALN_MAX_RANGES + 1];
+  __shared__ int nwant[2];
+  __shared__ uint32_t woff[ST_ITEMS][ST_BLOCK / 64];
+  __shared__ uint32_t nhit_s;
+  const unsigned lane = __lane_id(), wave = threadIdx.x / 64;
+  StageLoad<ST_BLOCK> ld;
+  uint32_t nmatch = 0, nprobe = 0;
+  // the want list of a tile: its range columns when the zone map says evaluate, then the key
+  auto plan = [&](int buf, char mode) {
+    if (threadIdx.x == 0) {
+      int n = 0;
+      if (mode == 'R')
+        for (int k = 0; k < d.nr; ++k) want[buf][n++] = d.rcol[k];
+      want[buf][n++] = fkcol;
+      nwant[buf] = n;
+    }
+  };
+  uint64_t tile = blockIdx.x;
+  uint32_t seg = 0, r0 = 0, nt = 0;
+  char mode = 'N';
+  bool act = tile < ntiles && fact_tile(d, tile, seg, r0, nt, mode);
+  if (act) plan(0, mode);
+  __syncthreads();
+  if (act) ld.issue(d.words, d.segs, want[0], nwant[0], seg, r0, nt);
+  int cur = 0;
+  for (; tile < ntiles; tile += gridDim.x) {
+    const uint64_t next = tile + gridDim.x;
+    uint32_t nseg = 0, nr0 = 0, nnt = 0;
+    char nmode = 'N';
+    const bool nact = next < ntiles && fact_tile(d, next, nseg, nr0, nnt, nmode);
+    __syncthreads();  // the block is done with the previous tile's LDS
+    if (act) ld.commit(ts);
+    if (nact) plan(cur ^ 1, nmode);
+    if (act && threadIdx.x == 0) {  // the tile's ranges in code space, from the committed slots
+      tp.none = false;
+      for (int k = 0; k < d.nr; ++k) {
+        const int c = d.rcol[k];
+        const int sl = mode == 'R' ? (c == ld.col0 ? 0 : (c == ld.col1 ? 1 : -1)) : -1;
+        tp.slot[k] = (int8_t)sl;
+        if (sl >= 0) {
+          tp.cr[k] = code_range(sl == 0 ? ld.s0 : ld.s1, d.rlo[k], d.rhi[k]);
+          tp.none = tp.none || tp.cr[k].none;
+        }
+      }
+    }
+    __syncthreads();
+    const int fs = fkcol == ld.col0 ? 0 : (fkcol == ld.col1 ? 1 : -1);
+    const int64_t foff = fs == 0 ? (int64_t)((uint64_t)ld.s0.base - (uint64_t)mkmin)
+                                 : (fs == 1 ? (int64_t)((uint64_t)ld.s1.base - (uint64_t)mkmin) : 0);
+    // probe this tile from LDS before the registers take the next tile's loads
+    uint32_t code[ST_ITEMS];
+#pragma unroll
+    for (int j = 0; j < ST_ITEMS; ++j) {
+      const uint32_t i = j * ST_BLOCK + threadIdx.x;
+      code[j] = 0xFFFFFFFFu;
+      if (act && i < nt && tile_pass_fast(d, ts, tp, mode, seg, r0, i)) {
+        ++nprobe;
+        const uint64_t c = fs >= 0 ? (uint64_t)((int64_t)stage_code(ts.words[fs], ts.slot[fs], i) + foff)
+                                   : (uint64_t)decode_row(d.words[fkcol], d.segs[fkcol][seg], r0 + i) - (uint64_t)mkmin;
+        if (c < mrange) code[j] = (uint32_t)c;
+      }
+    }
+    if (nact) ld.issue(d.words, d.segs, want[cur ^ 1], nwant[cur ^ 1], nseg, nr0, nnt);
+    if (act) {
+      uint32_t word[ST_ITEMS];
+#pragma unroll
+      for (int j = 0; j < ST_ITEMS; ++j) word[j] = code[j] != 0xFFFFFFFFu ? bits[code[j] >> 5] : 0u;
+      uint32_t hits = 0;
+#pragma unroll
+      for (int j = 0; j < ST_ITEMS; ++j)
+        hits |= (uint32_t)(code[j] != 0xFFFFFFFFu && ((word[j] >> (code[j] & 31)) & 1)) << j;
+#pragma unroll
+      for (int j = 0; j < ST_ITEMS; ++j) {
+        const uint64_t b = __ballot((hits >> j) & 1);
+        if (lane == 0) woff[j][wave] = (uint32_t)__popcll(b);
+      }
+      __syncthreads();  // also: the staged words are no longer read
+      if (threadIdx.x < 64) {  // one wave scans the (item, wave) counts
+        const uint32_t n = threadIdx.x < ST_ITEMS * (ST_BLOCK / 64) ? (&woff[0][0])[threadIdx.x] : 0;
+        uint32_t x = n;
+        for (int o = 1; o < 64; o <<= 1) {
+          const uint32_t y = __shfl_up(x, o, 64);
+          if (lane >= (unsigned)o) x += y;
+        }
+        if (threadIdx.x < ST_ITEMS * (ST_BLOCK / 64)) (&woff[0][0])[threadIdx.x] = x - n;
+        if (threadIdx.x == 63) nhit_s = x;
+      }
+      __syncthreads();
+      uint32_t *hcode = reinterpret_cast<uint32_t *>(&ts);
+      uint16_t *hrow = reinterpret_cast<uint16_t *>(hcode + ST_TILE);
+#pragma unroll
+      for (int j = 0; j < ST_ITEMS; ++j) {
+        const bool h = (hits >> j) & 1;
+        const uint64_t b = __ballot(h);
+        if (h) {
+          const uint32_t pos = woff[j][wave] + (uint32_t)__popcll(b & lanemask_lt());
+          hcode[pos] = code[j];
+          hrow[pos] = (uint16_t)(j * ST_BLOCK + threadIdx.x);
+        }
+      }
+      __syncthreads();
+      const uint32_t nh = nhit_s;
+      for (uint32_t e0 = 0; e0 < nh; e0 += ST_BLOCK) {
+        const uint32_t e = e0 + threadIdx.x;
+        uint32_t g = 0xFFFFFFFFu;
+        int64_t v = 0;
+        if (e < nh) {
+          const uint32_t local = r0 + hrow[e];
+          g = (uint32_t)bm_rank(bits, rank, hcode[e]);
+          const int64_t x = sd_decode(d, xcol, seg, local), y = sd_decode(d, ycol, seg, local);
+          v = wmul((int64_t)((uint64_t)a0 + (uint64_t)wmul(a1, x)), (int64_t)((uint64_t)b0 + (uint64_t)wmul(b1, y)));
+        }
+        if (__ballot(g != 0xFFFFFFFFu) == 0) continue;
+        const uint32_t prev = __shfl_up(g, 1, 64);
+        bool f = lane == 0 || prev != g;
+        uint32_t c = g != 0xFFFFFFFFu;
+        for (int o = 1; o < 64; o <<= 1) {
+          const int64_t vu = __shfl_up(v, o, 64);
+          const uint32_t cu = __shfl_up(c, o, 64);
+          const bool fu = __shfl_up((int)f, o, 64) != 0;
+          if (lane >= (unsigned)o && !f) {
+            v = (int64_t)((uint64_t)v + (uint64_t)vu);
+            c += cu;
+          }
+          if (lane >= (unsigned)o) f = f || fu;
+        }
+        const uint32_t nx = __shfl_down(g, 1, 64);
+        if (g != 0xFFFFFFFFu && (lane == 63 || nx != g)) {
+          atomicAdd(&sum[g], (unsigned long long)v);
+          atomicAdd(&cnt[g], c);
+          nmatch += c;
+        }
+      }
+    }
+    act = nact;
+    seg = nseg;
+    r0 = nr0;
+    nt = nnt;
+    mode = nmode;
+    cur ^= 1;
+  }
+  block_count(nmatch, matched);
+  block_count(nprobe, probed);
+}
+
 // ---- T: top-k over groups (sum DESC, g1 ASC, key ASC, g2 ASC) -----------------------------
 struct Cand {
   int64_t sum, g1, key, g2;
@@ -912,32 +1086,39 @@ static int star_topk(aln_ctx *c, uint64_t m, const unsigned long long *sum, cons
                      int64_t g1min, int64_t g1max, int64_t kmin, aln_star_row *h_top, uint32_t *h_ntop) {
   if (k <= (uint32_t)TS_K && g1max >= g1min && (uint64_t)g1max - (uint64_t)g1min < (1ull << 32) &&
       m < (1ull << 32)) {
-    const uint32_t nb = (uint32_t)std::min<uint64_t>((m + TS_BLOCK * 64 - 1) / (TS_BLOCK * 64), 1024);
-    uint64_t *h1, *l1, *h2, *l2;
-    uint32_t *j1, *j2, *nv;
+    // passes of at most ~16 candidates per thread until one block holds the result
+    uint64_t n = m;
+    const uint64_t *ih = nullptr, *il = nullptr;
+    const uint32_t *ij = nullptr;
+    uint32_t *j2 = nullptr, *nv;
     aln_star_row *rows;
-    ALN_TRY(ws(c, (size_t)nb * k, &h1));
-    ALN_TRY(ws(c, (size_t)nb * k, &l1));
-    ALN_TRY(ws(c, (size_t)nb * k, &j1));
-    ALN_TRY(ws(c, k, &h2));
-    ALN_TRY(ws(c, k, &l2));
-    ALN_TRY(ws(c, k, &j2));
     ALN_TRY(ws(c, 1, &nv));
     ALN_TRY(ws(c, k, &rows));
     {
       ProfScope p(c, "star_topk");
-      hipLaunchKernelGGL(k_topk_sel, dim3(nb), dim3(TS_BLOCK), 0, c->stream, m, sum, cnt, okey, og1, g1min, kmin,
-                         (const uint64_t *)nullptr, (const uint64_t *)nullptr, (const uint32_t *)nullptr, k, h1,
-                         l1, j1);
-      hipLaunchKernelGGL(k_topk_sel, dim3(1), dim3(TS_BLOCK), 0, c->stream, (uint64_t)nb * k, sum, cnt, okey, og1,
-                         g1min, kmin, h1, l1, j1, k, h2, l2, j2);
+      while (true) {
+        const uint32_t nb = (uint32_t)std::min<uint64_t>((n + TS_BLOCK * 16 - 1) / (TS_BLOCK * 16), 4096);
+        uint64_t *oh, *ol;
+        uint32_t *oj;
+        ALN_TRY(ws(c, (size_t)nb * k, &oh));
+        ALN_TRY(ws(c, (size_t)nb * k, &ol));
+        ALN_TRY(ws(c, (size_t)nb * k, &oj));
+        hipLaunchKernelGGL(k_topk_sel, dim3(nb), dim3(TS_BLOCK), 0, c->stream, n, sum, cnt, okey, og1, g1min, kmin,
+                           ih, il, ij, k, oh, ol, oj);
+        ih = oh;
+        il = ol;
+        ij = oj;
+        n = (uint64_t)nb * k;
+        if (nb == 1) break;
+      }
+      j2 = const_cast<uint32_t *>(ij);
       hipLaunchKernelGGL(k_topk_emit, dim3(1), dim3(64), 0, c->stream, j2, k, sum, cnt, okey, og1, og2, rows, nv);
     }
     ALN_TRY(check_launch("star_topk"));
-    uint32_t n = 0;
-    ALN_TRY(copy_to_host(c, &n, nv, 4));
-    if (n) ALN_TRY(copy_to_host(c, h_top, rows, sizeof(aln_star_row) * n));
-    *h_ntop = n;
+    uint32_t nvalid = 0;
+    ALN_TRY(copy_to_host(c, &nvalid, nv, 4));
+    if (nvalid) ALN_TRY(copy_to_host(c, h_top, rows, sizeof(aln_star_row) * nvalid));
+    *h_ntop = nvalid;
     return ALN_OK;
   }
   const Cand *in = nullptr;
@@ -1145,17 +1326,23 @@ extern "C" int aln_star_agg(aln_ctx *c, const aln_star_args *a, aln_star_row *h_
   tm.mark(2);
   if (m) {
     ProfScope p(c, "star_fact");
-    if (use_bm)
-      hipLaunchKernelGGL(k_star_fact_bm, dim3((unsigned)fblocks), dim3(ST_BLOCK), 0, c->stream, fd,
-                         a->f_fk, a->f_x, a->f_y, a->a0, a->a1, a->b0, a->b1, mbits, rank, mkmin, mrange,
-                         sum, cnt, fmatch, fprobe);
+    uint64_t fact_counts = fblocks;  // per-block count entries the fact kernel writes
+    if (use_bm) {
+      int cus = 0;
+      if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || cus <= 0)
+        cus = 256;
+      const uint32_t pgrid = (uint32_t)std::min<uint64_t>(fblocks, (uint64_t)cus * ST_FACT_BLOCKS_PER_CU);
+      hipLaunchKernelGGL(k_star_fact_p, dim3(pgrid), dim3(ST_BLOCK), 0, c->stream, fd, a->f_fk, a->f_x, a->f_y,
+                         a->a0, a->a1, a->b0, a->b1, mbits, rank, mkmin, mrange, fblocks, sum, cnt, fmatch, fprobe);
+      fact_counts = pgrid;
+    }
     else
       hipLaunchKernelGGL(k_star_fact, dim3((unsigned)fblocks), dim3(ST_BLOCK), 0, c->stream, fd, a->f_fk,
                          a->f_x, a->f_y, a->a0, a->a1, a->b0, a->b1, table, tcap, mkmin, mrange, sum, cnt,
                          fmatch, fprobe);
     ALN_TRY(check_launch("star_fact"));
-    ALN_TRY(sum_u32(c, fmatch, fblocks, ctr + 3));
-    ALN_TRY(sum_u32(c, fprobe, fblocks, ctr + 4));
+    ALN_TRY(sum_u32(c, fmatch, fact_counts, ctr + 3));
+    ALN_TRY(sum_u32(c, fprobe, fact_counts, ctr + 4));
   }
   tm.mark(3);
 
