@@ -133,83 +133,6 @@ __device__ __forceinline__ void stage_tile(TileStage &ts, const uint32_t *const 
   }
 }
 
-// The same copy split in two, for blocks that walk many tiles: issue() starts the loads of a
-// tile into registers (no LDS access, no barrier), commit() writes them and the slot
-// descriptors into LDS once the block is done with the previous tile.  A block issues tile
-// t+1 right after committing tile t, so the loads fly while tile t is computed.
-template <int BLOCK>
-struct StageLoad {
-  static constexpr int VEC = (2 * (STG_SLOT_WORDS / 4) + BLOCK - 1) / BLOCK;
-  uint4 r[VEC];
-  uint32_t n0, n1;
-  int col0, col1;
-  StageSlot s0, s1;
-
-  __device__ __forceinline__ void issue(const uint32_t *const *words, const aln_seg *const *segs, const int *want,
-                                        int nwant, uint32_t seg, uint32_t r0, uint32_t nt) {
-    const uint4 *src0 = nullptr, *src1 = nullptr;
-    n0 = n1 = 0;
-    col0 = col1 = -1;
-    int used = 0;
-    for (int k = 0; k < nwant && used < STG_SLOTS; ++k) {
-      const int c = want[k];
-      bool dup = false;
-      for (int q = 0; q < k; ++q) dup = dup || want[q] == c;
-      if (dup) continue;
-      const aln_seg sg = segs[c][seg];
-      if (sg.bits > STG_MAX_BITS) continue;
-      const uint64_t b0 = (uint64_t)r0 * sg.bits;
-      const uint64_t wbeg = b0 >> 5, wend = ((uint64_t)(r0 + nt) * sg.bits + 31) >> 5;
-      const uint32_t *p = words[c] + sg.word_off + wbeg;
-      const uint32_t a = (uint32_t)(((uintptr_t)p >> 2) & 3);
-      const uint32_t nb = sg.bits ? (uint32_t)((a + (wend - wbeg) + 3) >> 2) : 0;
-      StageSlot sl;
-      sl.base = sg.base;
-      sl.bits = sg.bits;
-      sl.bit0 = (uint32_t)(b0 & 31) + 32 * a;
-      if (used == 0) {
-        src0 = (const uint4 *)(p - a);
-        n0 = nb;
-        col0 = c;
-        s0 = sl;
-      } else {
-        src1 = (const uint4 *)(p - a);
-        n1 = nb;
-        col1 = c;
-        s1 = sl;
-      }
-      ++used;
-    }
-    static_assert(2 * (STG_SLOT_WORDS / 4) <= VEC * BLOCK, "one round of loads covers both slots");
-    const uint32_t ntot = n0 + n1;
-#pragma unroll
-    for (int k = 0; k < VEC; ++k) {
-      const uint32_t v = k * BLOCK + threadIdx.x;
-      r[k] = v < n0 ? src0[v] : (v < ntot ? src1[v - n0] : make_uint4(0, 0, 0, 0));
-    }
-  }
-
-  // Every thread calls it between two __syncthreads().
-  __device__ __forceinline__ void commit(TileStage &ts) const {
-    if (threadIdx.x < ALN_MAX_COLS)
-      ts.slot_of[threadIdx.x] = threadIdx.x == (unsigned)col0 ? 0 : (threadIdx.x == (unsigned)col1 ? 1 : -1);
-    if (threadIdx.x == 0) {
-      ts.slot[0] = s0;
-      ts.slot[1] = s1;
-    }
-    uint4 *dst0 = reinterpret_cast<uint4 *>(ts.words[0]), *dst1 = reinterpret_cast<uint4 *>(ts.words[1]);
-    const uint32_t ntot = n0 + n1;
-#pragma unroll
-    for (int k = 0; k < VEC; ++k) {
-      const uint32_t v = k * BLOCK + threadIdx.x;
-      if (v < n0)
-        dst0[v] = r[k];
-      else if (v < ntot)
-        dst1[v - n0] = r[k];
-    }
-  }
-};
-
 // Value of column c at row r0 + i of segment seg: from LDS when staged, else from global.
 __device__ __forceinline__ int64_t tile_get(const TileStage &ts, const uint32_t *const *words,
                                             const aln_seg *const *segs, int c, uint32_t seg,

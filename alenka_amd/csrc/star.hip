@@ -30,7 +30,6 @@ namespace aln {
 
 // 512 threads x 8 rows per 4096-row tile: eight waves share one tile's 32 KB of staging, so
 // the LDS that caps a CU at four tiles still admits 32 waves.
-constexpr int ST_FACT_BLOCKS_PER_CU = 2;  // resident blocks of the persistent fact scan
 constexpr int ST_BLOCK = 512;
 constexpr int ST_ITEMS = 8;
 constexpr int ST_TILE = ST_BLOCK * ST_ITEMS;
@@ -601,179 +600,6 @@ __global__ void __launch_bounds__(ST_BLOCK) k_star_fact_bm(ScanDesc d, int fkcol
   block_count(nprobe, probed);
 }
 
-// Persistent form of the fact scan: a grid of a few blocks per CU walks the tiles; each block
-// issues the next tile's staging loads into registers before it computes the current tile,
-// so the HBM latency of staging is hidden behind the compute of the tile before.
-__device__ __forceinline__ bool fact_tile(const ScanDesc &d, uint64_t tile, uint32_t &seg, uint32_t &r0,
-                                          uint32_t &nt, char &mode) {
-  seg = (uint32_t)(tile / d.tps);
-  r0 = (uint32_t)(tile % d.tps) * ST_TILE;
-  const uint32_t nrows = d.segs[0][seg].nrows;
-  if (r0 >= nrows) return false;
-  mode = sd_mode(d, seg);
-  if (mode == 'N') return false;
-  nt = nrows - r0 < (uint32_t)ST_TILE ? nrows - r0 : (uint32_t)ST_TILE;
-  return true;
-}
-
-__global__ void __launch_bounds__(ST_BLOCK) k_star_fact_p(ScanDesc d, int fkcol, int xcol, int ycol, int64_t a0,
-                                                          int64_t a1, int64_t b0, int64_t b1,
-                                                          const uint32_t *__restrict__ bits,
-                                                          const uint64_t *__restrict__ rank, int64_t mkmin,
-                                                          uint64_t mrange, uint64_t ntiles,
-                                                          unsigned long long *__restrict__ sum,
-                                                          unsigned int *__restrict__ cnt,
-                                                          uint32_t *__restrict__ matched,
-                                                          uint32_t *__restrict__ probed) {
-  static_assert(sizeof(TileStage) >= ST_TILE * (sizeof(uint32_t) + sizeof(uint16_t)),
-                "hit list must fit in the staging area");
-  __shared__ TileStage ts;
-  __shared__ TilePred tp;
-  __shared__ int want[2][ALN_MAX_RANGES + 1];
-  __shared__ int nwant[2];
-  __shared__ uint32_t woff[ST_ITEMS][ST_BLOCK / 64];
-  __shared__ uint32_t nhit_s;
-  const unsigned lane = __lane_id(), wave = threadIdx.x / 64;
-  StageLoad<ST_BLOCK> ld;
-  uint32_t nmatch = 0, nprobe = 0;
-  // the want list of a tile: its range columns when the zone map says evaluate, then the key
-  auto plan = [&](int buf, char mode) {
-    if (threadIdx.x == 0) {
-      int n = 0;
-      if (mode == 'R')
-        for (int k = 0; k < d.nr; ++k) want[buf][n++] = d.rcol[k];
-      want[buf][n++] = fkcol;
-      nwant[buf] = n;
-    }
-  };
-  uint64_t tile = blockIdx.x;
-  uint32_t seg = 0, r0 = 0, nt = 0;
-  char mode = 'N';
-  bool act = tile < ntiles && fact_tile(d, tile, seg, r0, nt, mode);
-  if (act) plan(0, mode);
-  __syncthreads();
-  if (act) ld.issue(d.words, d.segs, want[0], nwant[0], seg, r0, nt);
-  int cur = 0;
-  for (; tile < ntiles; tile += gridDim.x) {
-    const uint64_t next = tile + gridDim.x;
-    uint32_t nseg = 0, nr0 = 0, nnt = 0;
-    char nmode = 'N';
-    const bool nact = next < ntiles && fact_tile(d, next, nseg, nr0, nnt, nmode);
-    __syncthreads();  // the block is done with the previous tile's LDS
-    if (act) ld.commit(ts);
-    if (nact) plan(cur ^ 1, nmode);
-    if (act && threadIdx.x == 0) {  // the tile's ranges in code space, from the committed slots
-      tp.none = false;
-      for (int k = 0; k < d.nr; ++k) {
-        const int c = d.rcol[k];
-        const int sl = mode == 'R' ? (c == ld.col0 ? 0 : (c == ld.col1 ? 1 : -1)) : -1;
-        tp.slot[k] = (int8_t)sl;
-        if (sl >= 0) {
-          tp.cr[k] = code_range(sl == 0 ? ld.s0 : ld.s1, d.rlo[k], d.rhi[k]);
-          tp.none = tp.none || tp.cr[k].none;
-        }
-      }
-    }
-    __syncthreads();
-    const int fs = fkcol == ld.col0 ? 0 : (fkcol == ld.col1 ? 1 : -1);
-    const int64_t foff = fs == 0 ? (int64_t)((uint64_t)ld.s0.base - (uint64_t)mkmin)
-                                 : (fs == 1 ? (int64_t)((uint64_t)ld.s1.base - (uint64_t)mkmin) : 0);
-    // probe this tile from LDS before the registers take the next tile's loads
-    uint32_t code[ST_ITEMS];
-#pragma unroll
-    for (int j = 0; j < ST_ITEMS; ++j) {
-      const uint32_t i = j * ST_BLOCK + threadIdx.x;
-      code[j] = 0xFFFFFFFFu;
-      if (act && i < nt && tile_pass_fast(d, ts, tp, mode, seg, r0, i)) {
-        ++nprobe;
-        const uint64_t c = fs >= 0 ? (uint64_t)((int64_t)stage_code(ts.words[fs], ts.slot[fs], i) + foff)
-                                   : (uint64_t)decode_row(d.words[fkcol], d.segs[fkcol][seg], r0 + i) - (uint64_t)mkmin;
-        if (c < mrange) code[j] = (uint32_t)c;
-      }
-    }
-    if (nact) ld.issue(d.words, d.segs, want[cur ^ 1], nwant[cur ^ 1], nseg, nr0, nnt);
-    if (act) {
-      uint32_t word[ST_ITEMS];
-#pragma unroll
-      for (int j = 0; j < ST_ITEMS; ++j) word[j] = code[j] != 0xFFFFFFFFu ? bits[code[j] >> 5] : 0u;
-      uint32_t hits = 0;
-#pragma unroll
-      for (int j = 0; j < ST_ITEMS; ++j)
-        hits |= (uint32_t)(code[j] != 0xFFFFFFFFu && ((word[j] >> (code[j] & 31)) & 1)) << j;
-#pragma unroll
-      for (int j = 0; j < ST_ITEMS; ++j) {
-        const uint64_t b = __ballot((hits >> j) & 1);
-        if (lane == 0) woff[j][wave] = (uint32_t)__popcll(b);
-      }
-      __syncthreads();  // also: the staged words are no longer read
-      if (threadIdx.x < 64) {  // one wave scans the (item, wave) counts
-        const uint32_t n = threadIdx.x < ST_ITEMS * (ST_BLOCK / 64) ? (&woff[0][0])[threadIdx.x] : 0;
-        uint32_t x = n;
-        for (int o = 1; o < 64; o <<= 1) {
-          const uint32_t y = __shfl_up(x, o, 64);
-          if (lane >= (unsigned)o) x += y;
-        }
-        if (threadIdx.x < ST_ITEMS * (ST_BLOCK / 64)) (&woff[0][0])[threadIdx.x] = x - n;
-        if (threadIdx.x == 63) nhit_s = x;
-      }
-      __syncthreads();
-      uint32_t *hcode = reinterpret_cast<uint32_t *>(&ts);
-      uint16_t *hrow = reinterpret_cast<uint16_t *>(hcode + ST_TILE);
-#pragma unroll
-      for (int j = 0; j < ST_ITEMS; ++j) {
-        const bool h = (hits >> j) & 1;
-        const uint64_t b = __ballot(h);
-        if (h) {
-          const uint32_t pos = woff[j][wave] + (uint32_t)__popcll(b & lanemask_lt());
-          hcode[pos] = code[j];
-          hrow[pos] = (uint16_t)(j * ST_BLOCK + threadIdx.x);
-        }
-      }
-      __syncthreads();
-      const uint32_t nh = nhit_s;
-      for (uint32_t e0 = 0; e0 < nh; e0 += ST_BLOCK) {
-        const uint32_t e = e0 + threadIdx.x;
-        uint32_t g = 0xFFFFFFFFu;
-        int64_t v = 0;
-        if (e < nh) {
-          const uint32_t local = r0 + hrow[e];
-          g = (uint32_t)bm_rank(bits, rank, hcode[e]);
-          const int64_t x = sd_decode(d, xcol, seg, local), y = sd_decode(d, ycol, seg, local);
-          v = wmul((int64_t)((uint64_t)a0 + (uint64_t)wmul(a1, x)), (int64_t)((uint64_t)b0 + (uint64_t)wmul(b1, y)));
-        }
-        if (__ballot(g != 0xFFFFFFFFu) == 0) continue;
-        const uint32_t prev = __shfl_up(g, 1, 64);
-        bool f = lane == 0 || prev != g;
-        uint32_t c = g != 0xFFFFFFFFu;
-        for (int o = 1; o < 64; o <<= 1) {
-          const int64_t vu = __shfl_up(v, o, 64);
-          const uint32_t cu = __shfl_up(c, o, 64);
-          const bool fu = __shfl_up((int)f, o, 64) != 0;
-          if (lane >= (unsigned)o && !f) {
-            v = (int64_t)((uint64_t)v + (uint64_t)vu);
-            c += cu;
-          }
-          if (lane >= (unsigned)o) f = f || fu;
-        }
-        const uint32_t nx = __shfl_down(g, 1, 64);
-        if (g != 0xFFFFFFFFu && (lane == 63 || nx != g)) {
-          atomicAdd(&sum[g], (unsigned long long)v);
-          atomicAdd(&cnt[g], c);
-          nmatch += c;
-        }
-      }
-    }
-    act = nact;
-    seg = nseg;
-    r0 = nr0;
-    nt = nnt;
-    mode = nmode;
-    cur ^= 1;
-  }
-  block_count(nmatch, matched);
-  block_count(nprobe, probed);
-}
-
 // ---- T: top-k over groups (sum DESC, g1 ASC, key ASC, g2 ASC) -----------------------------
 struct Cand {
   int64_t sum, g1, key, g2;
@@ -966,7 +792,8 @@ __device__ __forceinline__ void ts_insert(TopList &t, uint64_t h, uint64_t l, ui
 
 // Candidates: the groups (ihi == nullptr: group j valid when cnt[j] > 0) or the (hi, lo, j)
 // triples of a previous pass.  Writes the block's best k triples (hi = ~0: none).
-__global__ void __launch_bounds__(TS_BLOCK) k_topk_sel(uint64_t n, const unsigned long long *__restrict__ sum,
+__global__ void __launch_bounds__(TS_BLOCK) k_topk_sel(uint64_t n, const uint32_t *__restrict__ d_n,
+                                                       const unsigned long long *__restrict__ sum,
                                                        const unsigned int *__restrict__ cnt,
                                                        const int64_t *__restrict__ okey,
                                                        const int64_t *__restrict__ og1, int64_t g1min,
@@ -979,6 +806,7 @@ __global__ void __launch_bounds__(TS_BLOCK) k_topk_sel(uint64_t n, const unsigne
   __shared__ uint32_t sj[TS_BLOCK * TS_K];
   __shared__ uint64_t whi[TS_BLOCK / 64], wlo[TS_BLOCK / 64];
   __shared__ uint32_t wt[TS_BLOCK / 64];
+  if (d_n) n = *d_n < n ? *d_n : n;  // a candidate buffer's fill, capped at its size
   TopList t;
 #pragma unroll
   for (int q = 0; q < TS_K; ++q) {
@@ -1057,6 +885,118 @@ __global__ void __launch_bounds__(TS_BLOCK) k_topk_sel(uint64_t n, const unsigne
   }
 }
 
+// Threshold pass for the top-k: the k-th largest of the blocks' maximum sums is a lower
+// bound of the k-th largest sum overall (each block maximum is a real group), so the groups
+// at or above it hold the answer - usually a few dozen.  Sums compare as sum ^ 2^63.
+constexpr int TB_BLOCK = 256, TB_ITEMS = 16;
+
+__global__ void __launch_bounds__(TB_BLOCK) k_topk_bmax(uint64_t n, const unsigned long long *__restrict__ sum,
+                                                        const unsigned int *__restrict__ cnt,
+                                                        uint64_t *__restrict__ bmax) {
+  __shared__ uint64_t wm[TB_BLOCK / 64];
+  const uint64_t base = (uint64_t)blockIdx.x * TB_BLOCK * TB_ITEMS;
+  uint64_t m = 0;
+#pragma unroll
+  for (int u = 0; u < TB_ITEMS; ++u) {
+    const uint64_t i = base + (uint64_t)u * TB_BLOCK + threadIdx.x;
+    if (i < n && cnt[i] > 0) {
+      const uint64_t o = (uint64_t)sum[i] ^ 0x8000000000000000ull;
+      m = o > m ? o : m;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const uint64_t x = __shfl_xor(m, o, 64);
+    m = x > m ? x : m;
+  }
+  if (__lane_id() == 0) wm[threadIdx.x / 64] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < TB_BLOCK / 64; ++w) m = wm[w] > m ? wm[w] : m;
+    bmax[blockIdx.x] = m;
+  }
+}
+
+// One block: the k-th largest of nb block maxima (k rounds of a block-wide maximum).
+__global__ void __launch_bounds__(TB_BLOCK) k_topk_kth(const uint64_t *__restrict__ bmax, uint32_t nb, uint32_t k,
+                                                       uint64_t *__restrict__ thr) {
+  __shared__ uint64_t wm[TB_BLOCK / 64];
+  __shared__ uint32_t wi[TB_BLOCK / 64];
+  __shared__ uint64_t last;
+  __shared__ uint32_t taken;
+  // each thread's values are bmax[threadIdx.x + t * TB_BLOCK]; a taken value is marked in
+  // its bit of a per-thread mask (at most 64 values per thread: nb <= 64 * TB_BLOCK)
+  uint64_t gone = 0;
+  if (threadIdx.x == 0) last = 0;
+  for (uint32_t r = 0; r < k; ++r) {
+    uint64_t m = 0;
+    uint32_t mi = 0xFFFFFFFFu;
+    for (uint32_t t = 0; t * TB_BLOCK + threadIdx.x < nb; ++t) {
+      const uint64_t v = bmax[t * TB_BLOCK + threadIdx.x];
+      if (!((gone >> t) & 1) && (mi == 0xFFFFFFFFu || v > m)) {
+        m = v;
+        mi = t;
+      }
+    }
+    uint32_t who = mi == 0xFFFFFFFFu ? 0xFFFFFFFFu : threadIdx.x;
+    for (int o = 32; o > 0; o >>= 1) {
+      const uint64_t m2 = __shfl_xor(m, o, 64);
+      const uint32_t w2 = __shfl_xor(who, o, 64);
+      if (w2 != 0xFFFFFFFFu && (who == 0xFFFFFFFFu || m2 > m || (m2 == m && w2 < who))) {
+        m = m2;
+        who = w2;
+      }
+    }
+    if (__lane_id() == 0) {
+      wm[threadIdx.x / 64] = m;
+      wi[threadIdx.x / 64] = who;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint64_t bm = wm[0];
+      uint32_t bw = wi[0];
+      for (int w = 1; w < TB_BLOCK / 64; ++w)
+        if (wi[w] != 0xFFFFFFFFu && (bw == 0xFFFFFFFFu || wm[w] > bm || (wm[w] == bm && wi[w] < bw))) {
+          bm = wm[w];
+          bw = wi[w];
+        }
+      taken = bw;
+      last = bw != 0xFFFFFFFFu ? bm : 0;  // fewer than k maxima: no bound, every group passes
+    }
+    __syncthreads();
+    if (taken == threadIdx.x) gone |= 1ull << mi;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *thr = last;
+}
+
+// The groups with sum >= threshold, as (hi, lo, j) candidates; *count may exceed cap.
+__global__ void __launch_bounds__(TB_BLOCK) k_topk_collect(uint64_t n, const unsigned long long *__restrict__ sum,
+                                                           const unsigned int *__restrict__ cnt,
+                                                           const int64_t *__restrict__ okey,
+                                                           const int64_t *__restrict__ og1, int64_t g1min,
+                                                           int64_t kmin, const uint64_t *__restrict__ thr,
+                                                           uint32_t cap, uint32_t *__restrict__ count,
+                                                           uint64_t *__restrict__ chi, uint64_t *__restrict__ clo,
+                                                           uint32_t *__restrict__ cj) {
+  const uint64_t t = *thr;
+  const uint64_t base = (uint64_t)blockIdx.x * TB_BLOCK * TB_ITEMS;
+#pragma unroll
+  for (int u = 0; u < TB_ITEMS; ++u) {
+    const uint64_t i = base + (uint64_t)u * TB_BLOCK + threadIdx.x;
+    if (i < n && cnt[i] > 0) {
+      const uint64_t o = (uint64_t)sum[i] ^ 0x8000000000000000ull;
+      if (o >= t) {
+        const uint32_t pos = atomicAdd(count, 1u);
+        if (pos < cap) {
+          chi[pos] = ~o;
+          clo[pos] = ((uint64_t)(og1[i] - g1min) << 32) | (uint32_t)((uint64_t)okey[i] - (uint64_t)kmin);
+          cj[pos] = (uint32_t)i;
+        }
+      }
+    }
+  }
+}
+
 __global__ void k_topk_emit(const uint32_t *__restrict__ j, uint32_t k,
                             const unsigned long long *__restrict__ sum, const unsigned int *__restrict__ cnt,
                             const int64_t *__restrict__ okey, const int64_t *__restrict__ og1,
@@ -1086,6 +1026,48 @@ static int star_topk(aln_ctx *c, uint64_t m, const unsigned long long *sum, cons
                      int64_t g1min, int64_t g1max, int64_t kmin, aln_star_row *h_top, uint32_t *h_ntop) {
   if (k <= (uint32_t)TS_K && g1max >= g1min && (uint64_t)g1max - (uint64_t)g1min < (1ull << 32) &&
       m < (1ull << 32)) {
+    // threshold path: block maxima -> k-th largest -> the groups at or above it
+    {
+      const uint64_t nb = (m + TB_BLOCK * TB_ITEMS - 1) / (TB_BLOCK * TB_ITEMS);
+      constexpr uint32_t CAP = 1u << 16;
+      if (nb <= 64ull * TB_BLOCK) {
+        uint64_t *bmax, *thr, *chi, *clo, *oh, *ol;
+        uint32_t *count, *cj, *oj, *nv;
+        aln_star_row *rows;
+        ALN_TRY(ws(c, nb, &bmax));
+        ALN_TRY(ws(c, 1, &thr));
+        ALN_TRY(ws(c, 2, &count));
+        ALN_TRY(ws(c, CAP, &chi));
+        ALN_TRY(ws(c, CAP, &clo));
+        ALN_TRY(ws(c, CAP, &cj));
+        ALN_TRY(ws(c, k, &oh));
+        ALN_TRY(ws(c, k, &ol));
+        ALN_TRY(ws(c, k, &oj));
+        ALN_TRY(ws(c, 1, &nv));
+        ALN_TRY(ws(c, k, &rows));
+        ALN_TRY(fill_u32(c, count, 2, 0));
+        {
+          ProfScope p(c, "star_topk");
+          hipLaunchKernelGGL(k_topk_bmax, dim3((unsigned)nb), dim3(TB_BLOCK), 0, c->stream, m, sum, cnt, bmax);
+          hipLaunchKernelGGL(k_topk_kth, dim3(1), dim3(TB_BLOCK), 0, c->stream, bmax, (uint32_t)nb, k, thr);
+          hipLaunchKernelGGL(k_topk_collect, dim3((unsigned)nb), dim3(TB_BLOCK), 0, c->stream, m, sum, cnt, okey, og1,
+                             g1min, kmin, thr, CAP, count, chi, clo, cj);
+          hipLaunchKernelGGL(k_topk_sel, dim3(1), dim3(TS_BLOCK), 0, c->stream, (uint64_t)CAP, count, sum, cnt, okey,
+                             og1, g1min, kmin, chi, clo, cj, k, oh, ol, oj);
+          hipLaunchKernelGGL(k_topk_emit, dim3(1), dim3(64), 0, c->stream, oj, k, sum, cnt, okey, og1, og2, rows, nv);
+        }
+        ALN_TRY(check_launch("star_topk"));
+        uint32_t hres[3];
+        ALN_TRY(copy_to_host(c, hres, count, 4));
+        ALN_TRY(copy_to_host(c, hres + 1, nv, 4));
+        if (hres[0] <= CAP) {
+          if (hres[1]) ALN_TRY(copy_to_host(c, h_top, rows, sizeof(aln_star_row) * hres[1]));
+          *h_ntop = hres[1];
+          return ALN_OK;
+        }
+        // more groups tie at the threshold than the buffer holds: the passes below
+      }
+    }
     // passes of at most ~16 candidates per thread until one block holds the result
     uint64_t n = m;
     const uint64_t *ih = nullptr, *il = nullptr;
@@ -1103,8 +1085,8 @@ static int star_topk(aln_ctx *c, uint64_t m, const unsigned long long *sum, cons
         ALN_TRY(ws(c, (size_t)nb * k, &oh));
         ALN_TRY(ws(c, (size_t)nb * k, &ol));
         ALN_TRY(ws(c, (size_t)nb * k, &oj));
-        hipLaunchKernelGGL(k_topk_sel, dim3(nb), dim3(TS_BLOCK), 0, c->stream, n, sum, cnt, okey, og1, g1min, kmin,
-                           ih, il, ij, k, oh, ol, oj);
+        hipLaunchKernelGGL(k_topk_sel, dim3(nb), dim3(TS_BLOCK), 0, c->stream, n, (const uint32_t *)nullptr, sum, cnt,
+                           okey, og1, g1min, kmin, ih, il, ij, k, oh, ol, oj);
         ih = oh;
         il = ol;
         ij = oj;
@@ -1326,23 +1308,16 @@ extern "C" int aln_star_agg(aln_ctx *c, const aln_star_args *a, aln_star_row *h_
   tm.mark(2);
   if (m) {
     ProfScope p(c, "star_fact");
-    uint64_t fact_counts = fblocks;  // per-block count entries the fact kernel writes
-    if (use_bm) {
-      int cus = 0;
-      if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || cus <= 0)
-        cus = 256;
-      const uint32_t pgrid = (uint32_t)std::min<uint64_t>(fblocks, (uint64_t)cus * ST_FACT_BLOCKS_PER_CU);
-      hipLaunchKernelGGL(k_star_fact_p, dim3(pgrid), dim3(ST_BLOCK), 0, c->stream, fd, a->f_fk, a->f_x, a->f_y,
-                         a->a0, a->a1, a->b0, a->b1, mbits, rank, mkmin, mrange, fblocks, sum, cnt, fmatch, fprobe);
-      fact_counts = pgrid;
-    }
+    if (use_bm)
+      hipLaunchKernelGGL(k_star_fact_bm, dim3((unsigned)fblocks), dim3(ST_BLOCK), 0, c->stream, fd, a->f_fk, a->f_x,
+                         a->f_y, a->a0, a->a1, a->b0, a->b1, mbits, rank, mkmin, mrange, sum, cnt, fmatch, fprobe);
     else
       hipLaunchKernelGGL(k_star_fact, dim3((unsigned)fblocks), dim3(ST_BLOCK), 0, c->stream, fd, a->f_fk,
                          a->f_x, a->f_y, a->a0, a->a1, a->b0, a->b1, table, tcap, mkmin, mrange, sum, cnt,
                          fmatch, fprobe);
     ALN_TRY(check_launch("star_fact"));
-    ALN_TRY(sum_u32(c, fmatch, fact_counts, ctr + 3));
-    ALN_TRY(sum_u32(c, fprobe, fact_counts, ctr + 4));
+    ALN_TRY(sum_u32(c, fmatch, fblocks, ctr + 3));
+    ALN_TRY(sum_u32(c, fprobe, fblocks, ctr + 4));
   }
   tm.mark(3);
 

@@ -45,6 +45,7 @@ _SIG = {
     "emit_order": (None, [C.c_char_p, C.c_char_p, C.c_int, C.c_int]),
     "emit_store": (None, [C.c_char_p, C.c_char_p, C.c_char_p]),
     "emit_display": (None, [C.c_char_p, C.c_char_p]),
+    "alenkaExecute": (C.c_int, [C.c_char_p]),
 }
 
 

@@ -96,6 +96,14 @@ void emit_order(const char *s, const char *f, int e, int ll);
 void emit_store(const char *s, const char *f, const char *sep);
 void emit_display(const char *s, const char *sep);
 
+/* A script in the reference's statement language (alenkaExecute, alenka.h:31; grammar
+ * bison.y:127-287): FILTER, SELECT [JOIN ... ON] [GROUP BY], ORDER, STORE ... USING,
+ * DISPLAY and DROP TABLE over the registered tables and variables, each statement turned
+ * into the emit_* calls the reference's parser makes.  Returns ALN_OK, or the status of
+ * the first statement that failed (its message in alenka_last_error); other statements
+ * (LOAD, binary STORE, outer joins, ...) return ALN_E_UNSUPPORTED. */
+int alenkaExecute(char *s);
+
 #ifdef __cplusplus
 }
 #endif

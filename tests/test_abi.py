@@ -182,7 +182,7 @@ def test_operator_layer_exports_every_declared_symbol():
     from alenka_amd import ops
 
     text = open(os.path.join(ROOT, "include", "alenka_ops.h")).read()
-    names = sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*((?:alenka|emit|clean|process)_\w+)\s*\(",
+    names = sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*((?:alenka|emit|clean|process)_\w+|alenkaExecute)\s*\(",
                                   text, re.M)))
     lib = L.load()
     for n in names:

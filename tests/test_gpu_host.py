@@ -146,3 +146,52 @@ def test_errors_do_not_exit(ops):
     ops.emit_name("nothing"); ops.emit_number(1); ops.emit_cmp(1); ops.emit_filter("X", "no_such_table")
     assert ops.lib.alenka_status() == L.ALN_E_ARG
     assert b"couldn't find" in ops.lib.alenka_last_error()
+
+
+SCRIPT_Q3 = """
+-- TPC-H q3 written in the reference's statement language
+ord_f := FILTER orders BY o_orderdate < {date};
+cust_f := FILTER customer BY c_mktsegment == "{segment}";
+li_f := FILTER lineitem BY shipdate > {date};
+joined := SELECT o_orderkey AS o_orderkey, o_orderdate AS o_orderdate,
+                 o_shippriority AS o_shippriority, price AS price, discount AS discount
+          FROM li_f JOIN ord_f ON orderkey = o_orderkey
+                    JOIN cust_f ON o_custkey = c_custkey;
+grouped := SELECT o_orderkey AS k, o_orderdate AS d, o_shippriority AS p,
+                  SUM(price * (1 - discount)) AS rev, COUNT(o_orderkey) AS n
+           FROM joined GROUP BY o_orderkey, o_orderdate, o_shippriority;
+ranked := ORDER grouped BY rev DESC, d ASC;
+STORE ranked INTO '{out}' USING ('|') LIMIT 10;
+"""
+
+
+@pytest.mark.parametrize("segment,date", [("BUILDING", 19950315), ("MACHINERY", 19950303)])
+def test_script_text_q3(ops, tmp_path, segment, date):
+    """The same q3 as a script (alenkaExecute): parsed into the parser's emit_* calls."""
+    t = Q3.tables(91, 6000, 60000)
+    code_to_str = {int(c): Q3.SEGMENTS[v] for v, c in enumerate(t["code_of"]) if c >= 0}
+    t["c_seg_str"] = [code_to_str[int(c)] for c in t["c_seg"]]
+    top, groups, stats = Q3.run(t, segment=segment, date=date)
+    seg_rows = 16384
+    ops.table("customer", len(t["c_key"]), seg_rows, [("c_custkey", t["c_key"], 0), ("c_mktsegment", t["c_seg_str"], 10)])
+    ops.table("orders", len(t["o_key"]), seg_rows, [("o_orderkey", t["o_key"], 0), ("o_custkey", t["o_cust"], 0),
+                                                    ("o_orderdate", t["o_date"], 0), ("o_shippriority", t["o_prio"], 0)])
+    ops.table("lineitem", len(t["l_okey"]), seg_rows, [("orderkey", t["l_okey"], 0), ("price", t["l_price"], 2),
+                                                       ("discount", t["l_disc"], 2), ("shipdate", t["l_ship"], 0)])
+    out = str(tmp_path / "q3.txt")
+    rc = ops.alenkaExecute(SCRIPT_Q3.format(date=date, segment=segment, out=out))
+    assert rc == 0, ops.lib.alenka_last_error()
+    assert ops.rows("joined") == stats["fact_pass"]
+    got = np.stack([ops.fetch("grouped", c) for c in ("k", "d", "p", "rev", "n")], axis=1)
+    assert np.array_equal(got, groups)
+    res = [tuple(int(ops.fetch("ranked", c)[i]) for c in ("k", "d", "p", "rev", "n")) for i in range(10)]
+    assert res == top
+    assert len(open(out).read().split("\n")) == 10
+
+
+def test_script_errors(ops):
+    assert ops.alenkaExecute("x := LOAD 'f.tbl' USING ('|') AS (a{1}:int);") == L.ALN_E_UNSUPPORTED
+    assert ops.alenkaExecute("x := FILTER ;") == L.ALN_E_ARG
+    assert b"statement 1" in ops.lib.alenka_last_error()
+    assert ops.alenkaExecute("y := FILTER no_such_table BY a < 1; z := FILTER y BY b > 2;") != 0
+    assert b"couldn't find" in ops.lib.alenka_last_error()
