@@ -143,6 +143,21 @@ __device__ __forceinline__ bool tile_pass(const ScanDesc &d, const TileStage &ts
   return p;
 }
 
+// Exclusive scan, in place, of the ST_ITEMS x (ST_BLOCK / 64) per-(item, wave) counts of a
+// tile (one entry per lane of the first wave); the first wave calls it, *total gets the sum.
+static_assert(ST_ITEMS * (ST_BLOCK / 64) == 64, "one wave scans the tile's counts");
+__device__ __forceinline__ void scan_tile_counts(uint32_t *counts, uint32_t *total) {
+  const unsigned lane = __lane_id();
+  const uint32_t n = counts[lane];
+  uint32_t x = n;
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= (unsigned)o) x += y;
+  }
+  counts[lane] = x - n;
+  if (lane == 63) *total = x;
+}
+
 // ---- D: dimension scan into a key bitmap --------------------------------------------------
 // Per-tile counts go to arrays indexed by block (summed afterwards): a counter shared by
 // every block would serialise its atomics across the XCDs.
@@ -273,16 +288,7 @@ __global__ void __launch_bounds__(ST_BLOCK) k_star_mid(ScanDesc d, int kcol, int
     if (lane == 0) woff[j][wid] = (uint32_t)__popcll(b);
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    uint32_t t = 0;
-    for (int j = 0; j < ST_ITEMS; ++j)
-      for (int w = 0; w < ST_BLOCK / 64; ++w) {
-        const uint32_t c = woff[j][w];
-        woff[j][w] = t;
-        t += c;
-      }
-    counts[blockIdx.x] = t;
-  }
+  if (threadIdx.x < 64) scan_tile_counts(&woff[0][0], &counts[blockIdx.x]);
   __syncthreads();
   const uint64_t base = (uint64_t)blockIdx.x * ST_TILE;
 #pragma unroll
@@ -538,16 +544,7 @@ __global__ void __launch_bounds__(ST_BLOCK) k_star_fact_bm(ScanDesc d, int fkcol
     if (lane == 0) woff[j][wave] = (uint32_t)__popcll(b);
   }
   __syncthreads();  // every wave is also done reading the staged words
-  if (threadIdx.x == 0) {
-    uint32_t t = 0;
-    for (int j = 0; j < ST_ITEMS; ++j)
-      for (int w = 0; w < ST_BLOCK / 64; ++w) {
-        const uint32_t c = woff[j][w];
-        woff[j][w] = t;
-        t += c;
-      }
-    nhit_s = t;
-  }
+  if (threadIdx.x < 64) scan_tile_counts(&woff[0][0], &nhit_s);
   __syncthreads();
   uint32_t *hcode = reinterpret_cast<uint32_t *>(&ts);
   uint16_t *hrow = reinterpret_cast<uint16_t *>(hcode + ST_TILE);
