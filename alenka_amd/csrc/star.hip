@@ -267,18 +267,13 @@ __global__ void __launch_bounds__(ST_BLOCK) k_star_mid(ScanDesc d, int kcol, int
 #pragma unroll
   for (int j = 0; j < ST_ITEMS; ++j)
     key[j] = ((passmask >> j) & 1) ? tile_get(ts, d.words, d.segs, kcol, seg, r0, j * ST_BLOCK + threadIdx.x) : 0;
-  if (mbits) {
-    uint32_t old[ST_ITEMS], bit[ST_ITEMS];
+  if (mbits) {  // no returned values: a duplicate key shows as popcount(mbits) < passing rows
 #pragma unroll
-    for (int j = 0; j < ST_ITEMS; ++j) {
-      const uint64_t code = (uint64_t)key[j] - (uint64_t)mkmin;
-      bit[j] = ((passmask >> j) & 1) ? 1u << (code & 31) : 0u;
-      old[j] = bit[j] ? atomicOr(&mbits[code >> 5], bit[j]) : 0u;
-    }
-    bool twice = false;
-#pragma unroll
-    for (int j = 0; j < ST_ITEMS; ++j) twice = twice || (old[j] & bit[j]);
-    if (twice) atomicExch(dup, 1u);
+    for (int j = 0; j < ST_ITEMS; ++j)
+      if ((passmask >> j) & 1) {
+        const uint64_t code = (uint64_t)key[j] - (uint64_t)mkmin;
+        atomicOr(&mbits[code >> 5], 1u << (code & 31));
+      }
   }
   block_count(nprobe, probed);
   // passing rows in item-major order (consecutive lanes write consecutive entries)
@@ -701,16 +696,61 @@ __global__ void k_star_group_emit(const uint32_t *idx, uint64_t g, const int64_t
   }
 }
 
-// Reads the segment descriptors of one column to the host and returns min/max over them.
-static int col_range(aln_ctx *c, const aln_col &col, int64_t *mn, int64_t *mx) {
-  std::vector<aln_seg> h(col.nsegs);
-  ALN_TRY(copy_to_host(c, h.data(), col.segs, sizeof(aln_seg) * col.nsegs));
-  *mn = INT64_MAX;
-  *mx = INT64_MIN;
-  for (auto &s : h) {
-    if (s.nrows == 0) continue;
-    *mn = s.vmin < *mn ? s.vmin : *mn;
-    *mx = s.vmax > *mx ? s.vmax : *mx;
+// min of vmin / max of vmax over the non-empty segments of each column: one block per column.
+__global__ void k_cols_range(const aln_seg *const *segs, const uint32_t *nsegs, int64_t *out) {
+  __shared__ int64_t smn[256], smx[256];
+  const aln_seg *sg = segs[blockIdx.x];
+  int64_t mn = INT64_MAX, mx = INT64_MIN;
+  for (uint32_t i = threadIdx.x; i < nsegs[blockIdx.x]; i += 256)
+    if (sg[i].nrows) {
+      mn = sg[i].vmin < mn ? sg[i].vmin : mn;
+      mx = sg[i].vmax > mx ? sg[i].vmax : mx;
+    }
+  smn[threadIdx.x] = mn;
+  smx[threadIdx.x] = mx;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < (unsigned)o) {
+      smn[threadIdx.x] = smn[threadIdx.x + o] < smn[threadIdx.x] ? smn[threadIdx.x + o] : smn[threadIdx.x];
+      smx[threadIdx.x] = smx[threadIdx.x + o] > smx[threadIdx.x] ? smx[threadIdx.x + o] : smx[threadIdx.x];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[2 * blockIdx.x] = smn[0];
+    out[2 * blockIdx.x + 1] = smx[0];
+  }
+}
+
+// Zone-map ranges of up to 8 columns with one launch and one copy (nullptr entries: 0, 0).
+static int cols_range(aln_ctx *c, const aln_col *const *cols, int n, int64_t *mn, int64_t *mx) {
+  const aln_seg *hs[8];
+  uint32_t hn[8];
+  int used = 0;
+  for (int i = 0; i < n; ++i)
+    if (cols[i]) {
+      hs[used] = cols[i]->segs;
+      hn[used] = cols[i]->nsegs;
+      ++used;
+    }
+  int64_t res[16];
+  if (used) {
+    const aln_seg **ds;
+    uint32_t *dn;
+    int64_t *dout;
+    ALN_TRY(ws(c, used, &ds));
+    ALN_TRY(ws(c, used, &dn));
+    ALN_TRY(ws(c, 2 * used, &dout));
+    ALN_HIP(hipMemcpyAsync(ds, hs, sizeof(void *) * used, hipMemcpyHostToDevice, c->stream));
+    ALN_HIP(hipMemcpyAsync(dn, hn, 4 * used, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_cols_range, dim3(used), dim3(256), 0, c->stream, ds, dn, dout);
+    ALN_TRY(check_launch("cols_range"));
+    ALN_TRY(copy_to_host(c, res, dout, 16 * used));
+  }
+  for (int i = 0, u = 0; i < n; ++i) {
+    mn[i] = cols[i] ? res[2 * u] : 0;
+    mx[i] = cols[i] ? res[2 * u + 1] : 0;
+    if (cols[i]) ++u;
   }
   return ALN_OK;
 }
@@ -1028,21 +1068,22 @@ static int star_topk(aln_ctx *c, uint64_t m, const unsigned long long *sum, cons
       const uint64_t nb = (m + TB_BLOCK * TB_ITEMS - 1) / (TB_BLOCK * TB_ITEMS);
       constexpr uint32_t CAP = 1u << 16;
       if (nb <= 64ull * TB_BLOCK) {
+        // one result block: [u32 collected][u32 rows valid][aln_star_row rows[k]], read with
+        // a single copy
         uint64_t *bmax, *thr, *chi, *clo, *oh, *ol;
-        uint32_t *count, *cj, *oj, *nv;
-        aln_star_row *rows;
+        uint32_t *cj, *oj, *out;
         ALN_TRY(ws(c, nb, &bmax));
         ALN_TRY(ws(c, 1, &thr));
-        ALN_TRY(ws(c, 2, &count));
         ALN_TRY(ws(c, CAP, &chi));
         ALN_TRY(ws(c, CAP, &clo));
         ALN_TRY(ws(c, CAP, &cj));
         ALN_TRY(ws(c, k, &oh));
         ALN_TRY(ws(c, k, &ol));
         ALN_TRY(ws(c, k, &oj));
-        ALN_TRY(ws(c, 1, &nv));
-        ALN_TRY(ws(c, k, &rows));
-        ALN_TRY(fill_u32(c, count, 2, 0));
+        ALN_TRY(ws(c, 2 + k * sizeof(aln_star_row) / 4, &out));
+        uint32_t *count = out, *nv = out + 1;
+        aln_star_row *rows = reinterpret_cast<aln_star_row *>(out + 2);
+        ALN_TRY(fill_u32(c, out, 2, 0));
         {
           ProfScope p(c, "star_topk");
           hipLaunchKernelGGL(k_topk_bmax, dim3((unsigned)nb), dim3(TB_BLOCK), 0, c->stream, m, sum, cnt, bmax);
@@ -1054,11 +1095,10 @@ static int star_topk(aln_ctx *c, uint64_t m, const unsigned long long *sum, cons
           hipLaunchKernelGGL(k_topk_emit, dim3(1), dim3(64), 0, c->stream, oj, k, sum, cnt, okey, og1, og2, rows, nv);
         }
         ALN_TRY(check_launch("star_topk"));
-        uint32_t hres[3];
-        ALN_TRY(copy_to_host(c, hres, count, 4));
-        ALN_TRY(copy_to_host(c, hres + 1, nv, 4));
+        std::vector<uint32_t> hres(2 + k * sizeof(aln_star_row) / 4);
+        ALN_TRY(copy_to_host(c, hres.data(), out, 4 * hres.size()));
         if (hres[0] <= CAP) {
-          if (hres[1]) ALN_TRY(copy_to_host(c, h_top, rows, sizeof(aln_star_row) * hres[1]));
+          memcpy(h_top, hres.data() + 2, sizeof(aln_star_row) * hres[1]);
           *h_ntop = hres[1];
           return ALN_OK;
         }
@@ -1182,9 +1222,14 @@ extern "C" int aln_star_agg(aln_ctx *c, const aln_star_args *a, aln_star_row *h_
               "aln_star_agg: aggregate columns");
 
   // key ranges from the zone maps
-  int64_t dkmin, dkmax, mkmin, mkmax;
-  ALN_TRY(col_range(c, a->dim.cols[a->d_key], &dkmin, &dkmax));
-  ALN_TRY(col_range(c, a->mid.cols[a->m_key], &mkmin, &mkmax));
+  int64_t dkmin, dkmax, mkmin, mkmax, g1min, g1max;
+  {
+    const aln_col *rc[3] = {&a->dim.cols[a->d_key], &a->mid.cols[a->m_key],
+                            a->m_g1 >= 0 ? &a->mid.cols[a->m_g1] : nullptr};
+    int64_t mn[3], mx[3];
+    ALN_TRY(cols_range(c, rc, 3, mn, mx));
+    dkmin = mn[0], dkmax = mx[0], mkmin = mn[1], mkmax = mx[1], g1min = mn[2], g1max = mx[2];
+  }
   if (a->dim.cols[0].nsegs == 0 || a->mid.cols[0].nsegs == 0 || dkmax < dkmin || mkmax < mkmin) {
     if (h_stats) *h_stats = st;
     return ALN_OK;
@@ -1209,18 +1254,19 @@ extern "C" int aln_star_agg(aln_ctx *c, const aln_star_args *a, aln_star_row *h_
   }
   uint64_t bwords = (drange + 31) / 32;
   uint32_t *bitmap, *dup;
-  uint64_t *ctr;  // device sums: 0 dim pass, 1 mid pass, 2 mid probe, 3 fact pass, 4 fact probe
+  // device counters, read back with one copy each time: 0 dim pass, 1 mid pass, 2 mid probe,
+  // 3 fact pass, 4 fact probe, 5 distinct passing mid keys, 6 duplicate flags (two u32)
+  uint64_t *ctr;
   uint32_t *dcnt, *mcnt, *mprobe, *fmatch, *fprobe;
   ALN_TRY(ws(c, bwords, &bitmap));
-  ALN_TRY(ws(c, 2, &dup));
   ALN_TRY(ws(c, 8, &ctr));
+  dup = reinterpret_cast<uint32_t *>(ctr + 6);
   ALN_TRY(ws(c, dblocks, &dcnt));
   ALN_TRY(ws(c, mblocks, &mcnt));
   ALN_TRY(ws(c, mblocks, &mprobe));
   ALN_TRY(ws(c, fblocks, &fmatch));
   ALN_TRY(ws(c, fblocks, &fprobe));
   ALN_TRY(fill_u32(c, bitmap, bwords, 0));
-  ALN_TRY(fill_u32(c, dup, 2, 0));
   ALN_TRY(fill_u64(c, ctr, 8, 0));
 
   tm.mark(0);
@@ -1247,12 +1293,30 @@ extern "C" int aln_star_agg(aln_ctx *c, const aln_star_args *a, aln_star_row *h_
   ALN_TRY(sum_u32(c, dcnt, dblocks, ctr + 0));
   ALN_TRY(sum_u32(c, mcnt, mblocks, ctr + 1));
   ALN_TRY(sum_u32(c, mprobe, mblocks, ctr + 2));
+  // bitmap join: the rank directory now (its total = distinct passing keys), before the
+  // one synchronisation that reads the counters
+  uint64_t *rank = nullptr;
+  uint64_t distinct = 0;
+  if (use_bm) {
+    uint32_t *bcnt;
+    ALN_TRY(ws(c, mblk, &bcnt));
+    ALN_TRY(ws(c, mblk, &rank));
+    {
+      ProfScope p(c, "star_build");
+      hipLaunchKernelGGL(k_bm_block_counts, dim3(grid_for(mblk, 256, 65536)), dim3(256), 0, c->stream,
+                         mbits, mwords, mblk, bcnt);
+    }
+    ALN_TRY(check_launch("bm_block_counts"));
+    ALN_TRY(sum_u32(c, bcnt, mblk, ctr + 5));
+    ALN_TRY(exclusive_scan_u32(c, bcnt, mblk, rank, nullptr));
+  }
   uint64_t hc[8];
-  uint32_t hdup[2];
-  ALN_TRY(copy_to_host(c, hc, ctr, 24));
-  ALN_TRY(copy_to_host(c, hdup, dup, 8));
+  ALN_TRY(copy_to_host(c, hc, ctr, sizeof(hc)));
+  const uint32_t *hdup = reinterpret_cast<const uint32_t *>(hc + 6);
+  distinct = hc[5];
   if (hdup[0]) return set_error(ALN_E_UNSUPPORTED, "aln_star_agg: duplicate dimension keys");
-  if (hdup[1]) return set_error(ALN_E_UNSUPPORTED, "aln_star_agg: duplicate mid keys");
+  if (hdup[1] || (use_bm && distinct != hc[1]))
+    return set_error(ALN_E_UNSUPPORTED, "aln_star_agg: duplicate mid keys");
   st.dim_pass = hc[0];
   const uint64_t m = hc[1];
   st.mid_pass = m;
@@ -1268,20 +1332,10 @@ extern "C" int aln_star_agg(aln_ctx *c, const aln_star_args *a, aln_star_row *h_
   ALN_TRY(ws(c, m ? m : 1, &og2));
   ALN_TRY(fill_u64(c, (uint64_t *)sum, m, 0));
   ALN_TRY(fill_u32(c, cnt, m, 0));
-  uint64_t *table = nullptr, *rank = nullptr;
+  uint64_t *table = nullptr;
   uint64_t tcap = 0;
   if (use_bm) {
-    // rank directory, then the group attributes in rank (= key) order
-    uint32_t *bcnt;
-    ALN_TRY(ws(c, mblk, &bcnt));
-    ALN_TRY(ws(c, mblk, &rank));
-    {
-      ProfScope p(c, "star_build");
-      hipLaunchKernelGGL(k_bm_block_counts, dim3(grid_for(mblk, 256, 65536)), dim3(256), 0, c->stream,
-                         mbits, mwords, mblk, bcnt);
-    }
-    ALN_TRY(check_launch("bm_block_counts"));
-    ALN_TRY(exclusive_scan_u32(c, bcnt, mblk, rank, nullptr));
+    // the group attributes in rank (= key) order
     if (m) {
       ProfScope p(c, "star_build");
       hipLaunchKernelGGL(k_bm_scatter, dim3((unsigned)mblocks), dim3(256), 0, c->stream, rkey, rg1, rg2,
@@ -1319,16 +1373,10 @@ extern "C" int aln_star_agg(aln_ctx *c, const aln_star_args *a, aln_star_row *h_
   tm.mark(3);
 
   // top-k
-  if (a->limit && m) {
-    int64_t g1min = 0, g1max = 0;
-    if (a->m_g1 >= 0) ALN_TRY(col_range(c, a->mid.cols[a->m_g1], &g1min, &g1max));
-    ALN_TRY(star_topk(c, m, sum, cnt, okey, og1, og2, a->limit, g1min, g1max, mkmin, h_top, h_ntop));
-  }
+  if (a->limit && m) ALN_TRY(star_topk(c, m, sum, cnt, okey, og1, og2, a->limit, g1min, g1max, mkmin, h_top, h_ntop));
   tm.mark(4);
-  ALN_TRY(copy_to_host(c, hc, ctr, 40));
-  uint32_t bdup = 0;
-  ALN_TRY(copy_to_host(c, &bdup, dup + 1, 4));
-  if (bdup) return set_error(ALN_E_UNSUPPORTED, "aln_star_agg: duplicate mid keys");
+  ALN_TRY(copy_to_host(c, hc, ctr, sizeof(hc)));
+  if (hdup[1]) return set_error(ALN_E_UNSUPPORTED, "aln_star_agg: duplicate mid keys");
   st.fact_pass = hc[3];
   st.fact_probe = hc[4];
 
@@ -1822,9 +1870,14 @@ extern "C" int aln_star_agg_batch(aln_ctx *c, const aln_star_args *a, int nq, al
   for (int col : {a0.m_key, a0.m_g1})
     if (col >= 0 && (sa.nwant == 0 || sa.want[0] != col)) sa.want[sa.nwant++] = col;
 
-  int64_t dkmin, dkmax, mkmin, mkmax;
-  ALN_TRY(col_range(c, a0.dim.cols[a0.d_key], &dkmin, &dkmax));
-  ALN_TRY(col_range(c, a0.mid.cols[a0.m_key], &mkmin, &mkmax));
+  int64_t dkmin, dkmax, mkmin, mkmax, g1min, g1max;
+  {
+    const aln_col *rc[3] = {&a0.dim.cols[a0.d_key], &a0.mid.cols[a0.m_key],
+                            a0.m_g1 >= 0 ? &a0.mid.cols[a0.m_g1] : nullptr};
+    int64_t mn[3], mx[3];
+    ALN_TRY(cols_range(c, rc, 3, mn, mx));
+    dkmin = mn[0], dkmax = mx[0], mkmin = mn[1], mkmax = mx[1], g1min = mn[2], g1max = mx[2];
+  }
   if (a0.dim.cols[0].nsegs == 0 || a0.mid.cols[0].nsegs == 0 || dkmax < dkmin || mkmax < mkmin) {
     if (h_stats) memcpy(h_stats, st.data(), sizeof(aln_star_stats) * nq);
     return ALN_OK;
@@ -1941,8 +1994,6 @@ extern "C" int aln_star_agg_batch(aln_ctx *c, const aln_star_args *a, int nq, al
     ALN_TRY(sum_u32(c, fprobe + (uint64_t)i * fblocks, fblocks, ctr + 5 * i + 3));
     ALN_TRY(sum_u32(c, fmatch + (uint64_t)i * fblocks, fblocks, ctr + 5 * i + 4));
   }
-  int64_t g1min = 0, g1max = 0;
-  if (a0.m_g1 >= 0) ALN_TRY(col_range(c, a0.mid.cols[a0.m_g1], &g1min, &g1max));
   for (int i = 0; i < nq; ++i)
     if (a[i].limit && st[i].mid_pass)
       ALN_TRY(star_topk(c, st[i].mid_pass, hq[i].sum, hq[i].cnt, hq[i].okey, hq[i].og1, hq[i].og2, a[i].limit,

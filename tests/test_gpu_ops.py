@@ -342,3 +342,23 @@ def test_star_q3_batch(ctx, scale, seg_rows):
         alone, _, st1 = D.star_agg(ctx, *q, limit=10)
         assert alone == rows
         assert (st1.mid_probe, st1.fact_probe) == (st.mid_probe, st.fact_probe)
+
+
+@pytest.mark.parametrize("force_hash", [False, True])
+def test_star_duplicate_keys_rejected(ctx, force_hash):
+    """A build side with a repeated key is not a star join: the fused pipeline refuses it
+    (ALN_E_UNSUPPORTED) so the caller falls back to the unfused operators."""
+    from alenka_amd import device as D
+
+    t = Q3.tables(55, 3000, 30000)
+    o_key = t["o_key"].copy()
+    passing = np.nonzero(t["o_date"] < Q3.DATE)[0]
+    o_key[passing[1]] = o_key[passing[0]]  # two passing orders share a key
+    t2 = dict(t, o_key=o_key)
+    dim, mid, fact = encode_q3(ctx, t2, 4096)
+    D_, M_, F_, spec = q3_args(t2, dim, mid, fact)
+    # every dimension row passes, so both orders reach the mid bitmap
+    D_ = dict(cols=dim, ranges=[])
+    with pytest.raises(L.AlnError) as e:
+        D.star_agg(ctx, D_, M_, F_, spec, limit=10, force_hash=force_hash)
+    assert e.value.code == L.ALN_E_UNSUPPORTED
