@@ -75,23 +75,6 @@ __device__ __forceinline__ bool sd_pass(const ScanDesc &d, char mode, uint32_t s
   return p;
 }
 
-// Block prologue of every scan kernel: stage the range columns (when the zone map says the
-// tile needs evaluating) and `key` into LDS.
-__device__ __forceinline__ void star_stage(TileStage &ts, int *want, const ScanDesc &d, char mode,
-                                           int key, uint32_t seg, uint32_t r0, uint32_t nt) {
-  __shared__ int nwant;
-  if (threadIdx.x == 0) {
-    int n = 0;
-    if (mode == 'R')
-      for (int k = 0; k < d.nr; ++k) want[n++] = d.rcol[k];
-    if (key >= 0) want[n++] = key;
-    nwant = n;
-  }
-  __syncthreads();
-  stage_tile<ST_BLOCK>(ts, d.words, d.segs, want, nwant, seg, r0, nt);
-  __syncthreads();
-}
-
 // The tile's ranges in code space (ranges on staged columns), built by thread 0 after
 // staging; tile_pass_fast then tests a staged range with one funnel shift and one compare.
 struct TilePred {
@@ -100,15 +83,29 @@ struct TilePred {
   bool none;                    // a staged range admits no code of the tile
 };
 
-__device__ __forceinline__ void tile_pred_init(TilePred &tp, const ScanDesc &d, const TileStage &ts) {
+// Block prologue of every scan kernel: stage the range columns (when the zone map says the
+// tile needs evaluating) and `key` into LDS, and (tp) the ranges in code space.  Thread 0
+// writes the column list and the slot descriptors, so it builds tp without a barrier of its
+// own: two barriers in all (the one inside stage_tile and the last one here).
+__device__ __forceinline__ void star_stage(TileStage &ts, int *want, const ScanDesc &d, char mode,
+                                           int key, uint32_t seg, uint32_t r0, uint32_t nt,
+                                           TilePred *tp = nullptr) {
+  const int nwant = (mode == 'R' ? d.nr : 0) + (key >= 0 ? 1 : 0);
   if (threadIdx.x == 0) {
-    tp.none = false;
+    int n = 0;
+    if (mode == 'R')
+      for (int k = 0; k < d.nr; ++k) want[n++] = d.rcol[k];
+    if (key >= 0) want[n++] = key;
+  }
+  stage_tile<ST_BLOCK>(ts, d.words, d.segs, want, nwant, seg, r0, nt);  // barrier before reading want
+  if (tp && threadIdx.x == 0) {
+    tp->none = false;
     for (int k = 0; k < d.nr; ++k) {
       const int s = ts.slot_of[d.rcol[k]];
-      tp.slot[k] = (int8_t)s;
+      tp->slot[k] = (int8_t)s;
       if (s >= 0) {
-        tp.cr[k] = code_range(ts.slot[s], d.rlo[k], d.rhi[k]);
-        tp.none = tp.none || tp.cr[k].none;
+        tp->cr[k] = code_range(ts.slot[s], d.rlo[k], d.rhi[k]);
+        tp->none = tp->none || tp->cr[k].none;
       }
     }
   }
@@ -161,6 +158,30 @@ __device__ __forceinline__ void scan_tile_counts(uint32_t *counts, uint32_t *tot
 // ---- D: dimension scan into a key bitmap --------------------------------------------------
 // Per-tile counts go to arrays indexed by block (summed afterwards): a counter shared by
 // every block would serialise its atomics across the XCDs.
+// Two per-block counts with one reduction (two barriers).
+__device__ __forceinline__ void block_count2(uint32_t a, uint32_t *out_a, uint32_t b, uint32_t *out_b) {
+  __shared__ uint32_t wa[ST_BLOCK / 64], wb[ST_BLOCK / 64];
+  for (int o = 32; o > 0; o >>= 1) {
+    a += __shfl_xor(a, o, 64);
+    b += __shfl_xor(b, o, 64);
+  }
+  __syncthreads();
+  if (__lane_id() == 0) {
+    wa[threadIdx.x / 64] = a;
+    wb[threadIdx.x / 64] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t ta = 0, tb = 0;
+    for (int w = 0; w < ST_BLOCK / 64; ++w) {
+      ta += wa[w];
+      tb += wb[w];
+    }
+    out_a[blockIdx.x] = ta;
+    out_b[blockIdx.x] = tb;
+  }
+}
+
 __device__ __forceinline__ void block_count(uint32_t v, uint32_t *out) {
   __shared__ uint32_t ws_[ST_BLOCK / 64];
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -187,9 +208,8 @@ __global__ void __launch_bounds__(ST_BLOCK) k_star_dim(ScanDesc d, int kcol, int
     return;
   }
   const uint32_t nt = nrows - r0 < (uint32_t)ST_TILE ? nrows - r0 : (uint32_t)ST_TILE;
-  star_stage(ts, want, d, mode, kcol, seg, r0, nt);
   __shared__ TilePred tp;
-  tile_pred_init(tp, d, ts);
+  star_stage(ts, want, d, mode, kcol, seg, r0, nt, &tp);
   uint32_t cnt = 0;
   for (int j = 0; j < ST_ITEMS; ++j) {
     uint32_t i = j * ST_BLOCK + threadIdx.x;
@@ -232,12 +252,11 @@ __global__ void __launch_bounds__(ST_BLOCK) k_star_mid(ScanDesc d, int kcol, int
     return;
   }
   const uint32_t nt = nrows - r0 < (uint32_t)ST_TILE ? nrows - r0 : (uint32_t)ST_TILE;
-  star_stage(ts, want, d, mode, fkcol, seg, r0, nt);
+  __shared__ TilePred tp;
+  star_stage(ts, want, d, mode, fkcol, seg, r0, nt, &tp);
   const unsigned lane = __lane_id(), wid = threadIdx.x / 64;
   // predicate and foreign key from LDS for all items, then the dimension-bitmap words, the
   // keys of the joined rows and the mid-bitmap updates, each issued for all items at once
-  __shared__ TilePred tp;
-  tile_pred_init(tp, d, ts);
   const int fs = ts.slot_of[fkcol];
   const int64_t foff = fs >= 0 ? (int64_t)((uint64_t)ts.slot[fs].base - (uint64_t)dkmin) : 0;
   uint64_t dc[ST_ITEMS];
@@ -413,8 +432,7 @@ __global__ void __launch_bounds__(ST_BLOCK) k_star_fact(ScanDesc d, int fkcol, i
       nmatch += c;
     }
   }
-  block_count(nmatch, matched);
-  block_count(nprobe, probed);
+  block_count2(nmatch, matched, nprobe, probed);
 }
 
 // ---- bitmap + rank join (dense key ranges) -----------------------------------------------
@@ -505,9 +523,8 @@ __global__ void __launch_bounds__(ST_BLOCK) k_star_fact_bm(ScanDesc d, int fkcol
     return;
   }
   const uint32_t nt = nrows - r0 < (uint32_t)ST_TILE ? nrows - r0 : (uint32_t)ST_TILE;
-  star_stage(ts, want, d, mode, fkcol, seg, r0, nt);
   __shared__ TilePred tp;
-  tile_pred_init(tp, d, ts);
+  star_stage(ts, want, d, mode, fkcol, seg, r0, nt, &tp);
   const unsigned lane = __lane_id(), wave = threadIdx.x / 64;
   // 1. predicate and key from LDS (code space) for all items, then every bitmap word load
   const int fs = ts.slot_of[fkcol];
@@ -588,8 +605,7 @@ __global__ void __launch_bounds__(ST_BLOCK) k_star_fact_bm(ScanDesc d, int fkcol
       nmatch += c;
     }
   }
-  block_count(nmatch, matched);
-  block_count(nprobe, probed);
+  block_count2(nmatch, matched, nprobe, probed);
 }
 
 // ---- T: top-k over groups (sum DESC, g1 ASC, key ASC, g2 ASC) -----------------------------
