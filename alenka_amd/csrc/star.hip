@@ -129,6 +129,43 @@ __device__ __forceinline__ bool tile_pass_fast(const ScanDesc &d, const TileStag
   return p;
 }
 
+// The common tile shape - at most one range, on a staged column, and a staged key - decoded
+// with the slot parameters held in registers (no descriptor reads from LDS per row).
+struct FastTile {
+  const uint32_t *rw, *kw;
+  uint32_t rbit0, rbits, rmask, clo, cspan, kbit0, kbits, kmask;
+  bool range;
+};
+
+__device__ __forceinline__ bool fast_tile(const ScanDesc &d, const TileStage &ts, const TilePred &tp, char mode,
+                                          int keycol, FastTile &f) {
+  const int ks = ts.slot_of[keycol];
+  if (ks < 0 || ts.slot[ks].bits > 31) return false;
+  f.range = mode == 'R';
+  if (f.range) {
+    if (d.nr != 1 || tp.slot[0] < 0 || tp.none) return false;
+    const int rs = tp.slot[0];
+    f.rw = ts.words[rs];
+    f.rbit0 = ts.slot[rs].bit0;
+    f.rbits = ts.slot[rs].bits;
+    f.rmask = (1u << f.rbits) - 1u;
+    f.clo = tp.cr[0].clo;
+    f.cspan = tp.cr[0].cspan;
+  }
+  f.kw = ts.words[ks];
+  f.kbit0 = ts.slot[ks].bit0;
+  f.kbits = ts.slot[ks].bits;
+  f.kmask = (1u << f.kbits) - 1u;
+  return true;
+}
+
+__device__ __forceinline__ uint32_t fast_code(const uint32_t *w, uint32_t bit0, uint32_t bits, uint32_t mask,
+                                              uint32_t i) {
+  const uint32_t pos = bit0 + i * bits;
+  const uint32_t wi = pos >> 5;
+  return __builtin_amdgcn_alignbit(w[wi + 1], w[wi], pos & 31) & mask;
+}
+
 __device__ __forceinline__ bool tile_pass(const ScanDesc &d, const TileStage &ts, char mode,
                                           uint32_t seg, uint32_t r0, uint32_t i) {
   if (mode == 'A') return true;
@@ -261,15 +298,27 @@ __global__ void __launch_bounds__(ST_BLOCK) k_star_mid(ScanDesc d, int kcol, int
   const int64_t foff = fs >= 0 ? (int64_t)((uint64_t)ts.slot[fs].base - (uint64_t)dkmin) : 0;
   uint64_t dc[ST_ITEMS];
   uint32_t nprobe = 0;
+  FastTile ft;
+  if (fast_tile(d, ts, tp, mode, fkcol, ft)) {
 #pragma unroll
-  for (int j = 0; j < ST_ITEMS; ++j) {
-    const uint32_t i = j * ST_BLOCK + threadIdx.x;
-    dc[j] = ~0ull;
-    if (i < nt && tile_pass_fast(d, ts, tp, mode, seg, r0, i)) {
-      ++nprobe;
-      const uint64_t b = fs >= 0 ? (uint64_t)((int64_t)stage_code(ts.words[fs], ts.slot[fs], i) + foff)
-                                 : (uint64_t)tile_get(ts, d.words, d.segs, fkcol, seg, r0, i) - (uint64_t)dkmin;
-      if (b < drange) dc[j] = b;
+    for (int j = 0; j < ST_ITEMS; ++j) {
+      const uint32_t i = j * ST_BLOCK + threadIdx.x;
+      const bool p = i < nt && (!ft.range || fast_code(ft.rw, ft.rbit0, ft.rbits, ft.rmask, i) - ft.clo <= ft.cspan);
+      nprobe += p;
+      const uint64_t b = (uint64_t)((int64_t)fast_code(ft.kw, ft.kbit0, ft.kbits, ft.kmask, i) + foff);
+      dc[j] = p && b < drange ? b : ~0ull;
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < ST_ITEMS; ++j) {
+      const uint32_t i = j * ST_BLOCK + threadIdx.x;
+      dc[j] = ~0ull;
+      if (i < nt && tile_pass_fast(d, ts, tp, mode, seg, r0, i)) {
+        ++nprobe;
+        const uint64_t b = fs >= 0 ? (uint64_t)((int64_t)stage_code(ts.words[fs], ts.slot[fs], i) + foff)
+                                   : (uint64_t)tile_get(ts, d.words, d.segs, fkcol, seg, r0, i) - (uint64_t)dkmin;
+        if (b < drange) dc[j] = b;
+      }
     }
   }
   uint32_t dw[ST_ITEMS];
@@ -531,15 +580,27 @@ __global__ void __launch_bounds__(ST_BLOCK) k_star_fact_bm(ScanDesc d, int fkcol
   const int64_t foff = fs >= 0 ? (int64_t)((uint64_t)ts.slot[fs].base - (uint64_t)mkmin) : 0;
   uint32_t code[ST_ITEMS];
   uint32_t nprobe = 0;
+  FastTile ft;
+  if (fast_tile(d, ts, tp, mode, fkcol, ft)) {
 #pragma unroll
-  for (int j = 0; j < ST_ITEMS; ++j) {
-    const uint32_t i = j * ST_BLOCK + threadIdx.x;
-    code[j] = 0xFFFFFFFFu;
-    if (i < nt && tile_pass_fast(d, ts, tp, mode, seg, r0, i)) {
-      ++nprobe;
-      const uint64_t c = fs >= 0 ? (uint64_t)((int64_t)stage_code(ts.words[fs], ts.slot[fs], i) + foff)
-                                 : (uint64_t)tile_get(ts, d.words, d.segs, fkcol, seg, r0, i) - (uint64_t)mkmin;
-      if (c < mrange) code[j] = (uint32_t)c;  // mrange <= 2^31 on this path
+    for (int j = 0; j < ST_ITEMS; ++j) {
+      const uint32_t i = j * ST_BLOCK + threadIdx.x;
+      const bool p = i < nt && (!ft.range || fast_code(ft.rw, ft.rbit0, ft.rbits, ft.rmask, i) - ft.clo <= ft.cspan);
+      nprobe += p;
+      const uint64_t c = (uint64_t)((int64_t)fast_code(ft.kw, ft.kbit0, ft.kbits, ft.kmask, i) + foff);
+      code[j] = p && c < mrange ? (uint32_t)c : 0xFFFFFFFFu;  // mrange <= 2^31 on this path
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < ST_ITEMS; ++j) {
+      const uint32_t i = j * ST_BLOCK + threadIdx.x;
+      code[j] = 0xFFFFFFFFu;
+      if (i < nt && tile_pass_fast(d, ts, tp, mode, seg, r0, i)) {
+        ++nprobe;
+        const uint64_t c = fs >= 0 ? (uint64_t)((int64_t)stage_code(ts.words[fs], ts.slot[fs], i) + foff)
+                                   : (uint64_t)tile_get(ts, d.words, d.segs, fkcol, seg, r0, i) - (uint64_t)mkmin;
+        if (c < mrange) code[j] = (uint32_t)c;  // mrange <= 2^31 on this path
+      }
     }
   }
   uint32_t word[ST_ITEMS];
