@@ -166,6 +166,32 @@ __device__ __forceinline__ uint32_t fast_code(const uint32_t *w, uint32_t bit0, 
   return __builtin_amdgcn_alignbit(w[wi + 1], w[wi], pos & 31) & mask;
 }
 
+// Straight-line decode of a tile's rows for the fast shape: per row, the range test (RANGE)
+// and the key shifted by `off` in 32 bits; code[j] = key code, or ~0 when the row does not
+// probe (fails the range, lies past the tile, or the key falls outside [0, lim)).  Every
+// read stays inside the slots, which span a whole tile.  Returns the number of probes.
+template <bool RANGE>
+__device__ __forceinline__ uint32_t fast_probe(const FastTile &f, uint32_t nt, uint32_t off, uint32_t lim,
+                                               uint32_t (&code)[ST_ITEMS]) {
+  const uint32_t kp0 = f.kbit0 + threadIdx.x * f.kbits, rp0 = f.rbit0 + threadIdx.x * f.rbits;
+  uint32_t n = 0;
+#pragma unroll
+  for (int j = 0; j < ST_ITEMS; ++j) {
+    const uint32_t i = j * ST_BLOCK + threadIdx.x;
+    bool p = i < nt;
+    if (RANGE) {
+      const uint32_t rp = rp0 + j * (ST_BLOCK * f.rbits), rwi = rp >> 5;
+      const uint32_t rc = __builtin_amdgcn_alignbit(f.rw[rwi + 1], f.rw[rwi], rp & 31) & f.rmask;
+      p = p & (rc - f.clo <= f.cspan);
+    }
+    const uint32_t kp = kp0 + j * (ST_BLOCK * f.kbits), kwi = kp >> 5;
+    const uint32_t c = (__builtin_amdgcn_alignbit(f.kw[kwi + 1], f.kw[kwi], kp & 31) & f.kmask) + off;
+    n += p;
+    code[j] = p & (c < lim) ? c : 0xFFFFFFFFu;
+  }
+  return n;
+}
+
 __device__ __forceinline__ bool tile_pass(const ScanDesc &d, const TileStage &ts, char mode,
                                           uint32_t seg, uint32_t r0, uint32_t i) {
   if (mode == 'A') return true;
@@ -299,15 +325,13 @@ __global__ void __launch_bounds__(ST_BLOCK) k_star_mid(ScanDesc d, int kcol, int
   uint64_t dc[ST_ITEMS];
   uint32_t nprobe = 0;
   FastTile ft;
-  if (fast_tile(d, ts, tp, mode, fkcol, ft)) {
+  if (fast_tile(d, ts, tp, mode, fkcol, ft) && foff >= INT32_MIN && foff <= INT32_MAX && drange <= (1ull << 31)) {
+    // branch-free, 32-bit key shift (see k_star_fact_bm)
+    const uint32_t off = (uint32_t)(int32_t)foff, dr = (uint32_t)drange;
+    uint32_t c32[ST_ITEMS];
+    nprobe = ft.range ? fast_probe<true>(ft, nt, off, dr, c32) : fast_probe<false>(ft, nt, off, dr, c32);
 #pragma unroll
-    for (int j = 0; j < ST_ITEMS; ++j) {
-      const uint32_t i = j * ST_BLOCK + threadIdx.x;
-      const bool p = i < nt && (!ft.range || fast_code(ft.rw, ft.rbit0, ft.rbits, ft.rmask, i) - ft.clo <= ft.cspan);
-      nprobe += p;
-      const uint64_t b = (uint64_t)((int64_t)fast_code(ft.kw, ft.kbit0, ft.kbits, ft.kmask, i) + foff);
-      dc[j] = p && b < drange ? b : ~0ull;
-    }
+    for (int j = 0; j < ST_ITEMS; ++j) dc[j] = c32[j] != 0xFFFFFFFFu ? c32[j] : ~0ull;
   } else {
 #pragma unroll
     for (int j = 0; j < ST_ITEMS; ++j) {
@@ -323,7 +347,7 @@ __global__ void __launch_bounds__(ST_BLOCK) k_star_mid(ScanDesc d, int kcol, int
   }
   uint32_t dw[ST_ITEMS];
 #pragma unroll
-  for (int j = 0; j < ST_ITEMS; ++j) dw[j] = dc[j] != ~0ull ? bitmap[dc[j] >> 5] : 0u;
+  for (int j = 0; j < ST_ITEMS; ++j) dw[j] = bitmap[dc[j] != ~0ull ? dc[j] >> 5 : 0u];
   uint32_t passmask = 0, mine = 0;
 #pragma unroll
   for (int j = 0; j < ST_ITEMS; ++j) {
@@ -581,15 +605,12 @@ __global__ void __launch_bounds__(ST_BLOCK) k_star_fact_bm(ScanDesc d, int fkcol
   uint32_t code[ST_ITEMS];
   uint32_t nprobe = 0;
   FastTile ft;
-  if (fast_tile(d, ts, tp, mode, fkcol, ft)) {
-#pragma unroll
-    for (int j = 0; j < ST_ITEMS; ++j) {
-      const uint32_t i = j * ST_BLOCK + threadIdx.x;
-      const bool p = i < nt && (!ft.range || fast_code(ft.rw, ft.rbit0, ft.rbits, ft.rmask, i) - ft.clo <= ft.cspan);
-      nprobe += p;
-      const uint64_t c = (uint64_t)((int64_t)fast_code(ft.kw, ft.kbit0, ft.kbits, ft.kmask, i) + foff);
-      code[j] = p && c < mrange ? (uint32_t)c : 0xFFFFFFFFu;  // mrange <= 2^31 on this path
-    }
+  if (fast_tile(d, ts, tp, mode, fkcol, ft) && foff >= INT32_MIN && foff <= INT32_MAX) {
+    // branch-free: every row's codes are read (the slots span a whole tile), the tests are
+    // combined with & rather than &&, and the key is shifted in 32 bits - code + foff lies
+    // in (-2^31, 2^32), and any value outside [0, mrange) with mrange <= 2^31 maps to >= 2^31
+    const uint32_t off = (uint32_t)(int32_t)foff, mr = (uint32_t)mrange;
+    nprobe = ft.range ? fast_probe<true>(ft, nt, off, mr, code) : fast_probe<false>(ft, nt, off, mr, code);
   } else {
 #pragma unroll
     for (int j = 0; j < ST_ITEMS; ++j) {
@@ -603,13 +624,14 @@ __global__ void __launch_bounds__(ST_BLOCK) k_star_fact_bm(ScanDesc d, int fkcol
       }
     }
   }
+  // the bitmap words, loaded without branches (rows that do not probe read word 0)
   uint32_t word[ST_ITEMS];
 #pragma unroll
-  for (int j = 0; j < ST_ITEMS; ++j) word[j] = code[j] != 0xFFFFFFFFu ? bits[code[j] >> 5] : 0u;
+  for (int j = 0; j < ST_ITEMS; ++j) word[j] = bits[code[j] != 0xFFFFFFFFu ? code[j] >> 5 : 0u];
   uint32_t hits = 0;
 #pragma unroll
   for (int j = 0; j < ST_ITEMS; ++j)
-    hits |= (uint32_t)(code[j] != 0xFFFFFFFFu && ((word[j] >> (code[j] & 31)) & 1)) << j;
+    hits |= (uint32_t)((code[j] != 0xFFFFFFFFu) & ((word[j] >> (code[j] & 31)) & 1)) << j;
   // 2. ordered compaction: rows are item-major, then wave, then lane
 #pragma unroll
   for (int j = 0; j < ST_ITEMS; ++j) {
