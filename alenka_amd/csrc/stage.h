@@ -54,16 +54,19 @@ struct CodeRange {
   uint32_t clo, cspan;
   bool none;
 };
-__device__ __forceinline__ CodeRange code_range(const StageSlot &s, int64_t lo, int64_t hi) {
+__device__ __forceinline__ CodeRange code_range(int64_t base, uint32_t bits, int64_t lo, int64_t hi) {
   CodeRange r;
-  const uint64_t maxc = s.bits ? (1ull << s.bits) - 1 : 0;
-  const uint64_t L = lo <= s.base ? 0 : (uint64_t)lo - (uint64_t)s.base;
-  const bool below = hi < s.base;
-  const uint64_t H = below ? 0 : ((uint64_t)hi - (uint64_t)s.base < maxc ? (uint64_t)hi - (uint64_t)s.base : maxc);
+  const uint64_t maxc = bits ? (1ull << bits) - 1 : 0;
+  const uint64_t L = lo <= base ? 0 : (uint64_t)lo - (uint64_t)base;
+  const bool below = hi < base;
+  const uint64_t H = below ? 0 : ((uint64_t)hi - (uint64_t)base < maxc ? (uint64_t)hi - (uint64_t)base : maxc);
   r.none = below || L > H;
   r.clo = r.none ? 0 : (uint32_t)L;
   r.cspan = r.none ? 0 : (uint32_t)(H - L);
   return r;
+}
+__device__ __forceinline__ CodeRange code_range(const StageSlot &s, int64_t lo, int64_t hi) {
+  return code_range(s.base, s.bits, lo, hi);
 }
 
 // Stages the listed columns (first STG_SLOTS with at most 31 bits); every thread of the

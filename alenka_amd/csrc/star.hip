@@ -187,7 +187,7 @@ __device__ __forceinline__ uint32_t fast_probe(const FastTile &f, uint32_t nt, u
     const uint32_t kp = kp0 + j * (ST_BLOCK * f.kbits), kwi = kp >> 5;
     const uint32_t c = (__builtin_amdgcn_alignbit(f.kw[kwi + 1], f.kw[kwi], kp & 31) & f.kmask) + off;
     n += p;
-    code[j] = p & (c < lim) ? c : 0xFFFFFFFFu;
+    code[j] = (p & (c < lim)) ? c : 0xFFFFFFFFu;
   }
   return n;
 }
@@ -216,6 +216,255 @@ __device__ __forceinline__ void scan_tile_counts(uint32_t *counts, uint32_t *tot
   }
   counts[lane] = x - n;
   if (lane == 63) *total = x;
+}
+
+// ---- per-segment scan plans ------------------------------------------------------------------
+// What a tile needs from its segment's descriptors and the query's range, resolved once per
+// segment by k_scan_plan.  A block reads its segment's record with scalar loads and derives
+// its staging addresses from it, so no descriptor read sits between the block's start and
+// its first data load, and no thread builds per-tile state for the others behind a barrier.
+// Roles: 0 the range column (staged in mode 'R'), 1 the probe key, 2..4 per kernel.
+constexpr int PL_ROLES = 5;
+constexpr int PL_OWN = 3;                // staged columns per tile
+constexpr int PL_STAGE_WORDS = 9216;     // up to 72 staged bits per row over a 4096-row tile
+constexpr int PL_VEC = (PL_STAGE_WORDS / 4 + ST_BLOCK - 1) / ST_BLOCK;  // 16-byte loads per thread
+constexpr int PL_WAVE_ROWS = 64 * ST_ITEMS;  // rows of a tile one wave owns (item-major)
+
+struct PlanCol {
+  const uint32_t *w;  // the segment's packed words
+  int64_t base;
+  uint32_t bits;
+  int32_t slot;       // LDS word offset of the column's staged blocks, -1: not staged
+};
+
+struct StageSrc {     // one staged column
+  const uint32_t *w;
+  uint32_t bits;      // 0: unused entry
+  int32_t slot;
+};
+
+struct __align__(16) SegPlan {
+  PlanCol col[PL_ROLES];
+  StageSrc st[PL_OWN];
+  uint32_t nrows;
+  uint32_t clo, cspan;  // mode 'R': a row passes when code(col[0]) - clo <= cspan
+  int32_t zmode;        // the zone map's verdict: 'N', 'A' or 'R'
+  int32_t mode;         // 'N' skip, 'A' every row passes, 'R' one staged range, 'G' generic path
+};
+
+// LDS words a staged column of `bits` needs: the 16-byte blocks covering a tile's codes
+// (ceil(4096 * bits / 32) + 1 words plus up to 3 words of the first block before them), and
+// 4 words after them for the funnel-shift read of the last code.
+__host__ __device__ constexpr uint32_t plan_slot_words(uint32_t bits) {
+  return ((ST_TILE * bits + 31) / 32 + 1 + 3 + 3) / 4 * 4 + 4;
+}
+static_assert(plan_slot_words(STG_MAX_BITS) == STG_SLOT_WORDS, "slot sizes agree with stage.h");
+
+// rc1..rc4: the columns of roles 1..4 (-1: none); roles whose bit is set in `stage` are
+// staged when their column is not constant.  kmin: the probe key's code is key - kmin, and
+// the fast path needs base - kmin in 32 bits.  fast = false plans every live segment 'G'.
+__global__ void k_scan_plan(ScanDesc d, int rc1, int rc2, int rc3, int rc4, uint32_t stage, int64_t kmin,
+                            int fast, uint32_t nsegs, SegPlan *__restrict__ out) {
+  const uint32_t seg = blockIdx.x * blockDim.x + threadIdx.x;
+  if (seg >= nsegs) return;
+  SegPlan P;
+  const char zm = sd_mode(d, seg);
+  const int rc[PL_ROLES] = {d.nr == 1 ? d.rcol[0] : -1, rc1, rc2, rc3, rc4};
+  bool generic = !fast || d.nr > 1;
+  uint32_t words = 0, nown = 0;
+#pragma unroll
+  for (int k = 0; k < PL_OWN; ++k) P.st[k] = StageSrc{nullptr, 0u, 0};
+#pragma unroll
+  for (int r = 0; r < PL_ROLES; ++r) {
+    PlanCol &pc = P.col[r];
+    pc = PlanCol{nullptr, 0, 0u, -1};
+    const int c = rc[r];
+    if (c < 0) continue;
+    const aln_seg sg = d.segs[c][seg];
+    pc.w = d.words[c] + sg.word_off;
+    pc.base = sg.base;
+    pc.bits = sg.bits;
+    const bool want = r == 0 ? zm == 'R' : (((stage >> r) & 1) && sg.bits > 0);
+    if (!want) continue;
+    if (sg.bits > (uint32_t)STG_MAX_BITS || sg.bits == 0) {
+      generic = true;
+      continue;
+    }
+    int32_t slot = -1;
+#pragma unroll
+    for (int q = 0; q < r; ++q)
+      if (rc[q] == c && P.col[q].slot >= 0) slot = P.col[q].slot;
+    if (slot < 0) {
+      slot = (int32_t)words;
+      words += plan_slot_words(sg.bits);
+#pragma unroll
+      for (int k = 0; k < PL_OWN; ++k)
+        if (nown == (uint32_t)k) P.st[k] = StageSrc{pc.w, sg.bits, slot};
+      ++nown;
+    }
+    pc.slot = slot;
+  }
+  if (words > (uint32_t)PL_STAGE_WORDS || nown > (uint32_t)PL_OWN) generic = true;
+  // the fast probe reads a staged, non-constant key and shifts it in 32 bits
+  if (rc1 >= 0) {
+    const int64_t off = (int64_t)((uint64_t)P.col[1].base - (uint64_t)kmin);
+    if (P.col[1].slot < 0 || off < INT32_MIN || off > INT32_MAX) generic = true;
+  }
+  P.nrows = d.segs[0][seg].nrows;
+  P.clo = P.cspan = 0;
+  P.zmode = zm;
+  char mode = zm == 'N' ? 'N' : (generic ? 'G' : zm);
+  if (mode == 'R') {
+    const CodeRange cr = code_range(P.col[0].base, P.col[0].bits, d.rlo[0], d.rhi[0]);
+    if (cr.none) mode = 'N';
+    P.clo = cr.clo;
+    P.cspan = cr.cspan;
+  }
+  P.mode = mode;
+  out[seg] = P;
+}
+
+// A pointer read from a plan, marked as global memory: the compiler cannot tell where a
+// pointer loaded from memory points and would otherwise use flat (generic) loads.
+template <class T>
+__device__ __forceinline__ T gload(const T *p) {
+#if __HIP_DEVICE_COMPILE__
+  return *(const __attribute__((address_space(1))) T *)p;
+#else
+  return *p;  // host pass: never executed
+#endif
+}
+// uint4 is a class type whose copy would go through a generic reference: load the words as
+// a plain vector instead
+__device__ __forceinline__ uint4 gload4(const uint4 *p) {
+  typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+  const v4u q = gload(reinterpret_cast<const v4u *>(p));
+  return make_uint4(q.x, q.y, q.z, q.w);
+}
+
+// Reads single rows of a segment's column from global memory without branches: the three
+// word loads always issue (a word a row does not need is read from the row's first word,
+// and a constant column reads `any`, some readable global address), so the loads of several
+// readers go out together.
+struct ColReader {
+  const uint32_t *w;
+  int64_t base;
+  uint32_t bits;
+  uint64_t mask;
+};
+__device__ __forceinline__ ColReader col_reader(const PlanCol &pc, const void *any) {
+  const bool c = pc.bits == 0;
+  return ColReader{c ? static_cast<const uint32_t *>(any) : pc.w, pc.base, pc.bits,
+                   pc.bits >= 64 ? ~0ull : (1ull << pc.bits) - 1};
+}
+__device__ __forceinline__ int64_t col_read(const ColReader &r, uint64_t local) {
+  const uint32_t *w = r.w;
+  const uint64_t pos = local * r.bits, wi = pos >> 5;
+  const uint32_t sh = (uint32_t)(pos & 31);
+  const bool two = sh + r.bits > 32, three = sh + r.bits > 64;
+  uint64_t i1 = two ? wi + 1 : wi, i2 = three ? wi + 2 : wi;
+  // opaque to the optimiser, which would otherwise turn the loads of words that may equal
+  // word 0 back into conditional loads, each behind a branch and a wait
+  asm volatile("" : "+v"(i1), "+v"(i2));
+  const uint32_t w0 = gload(w + wi), w1 = gload(w + i1), w2 = gload(w + i2);
+  const uint64_t v = ((((uint64_t)w1 << 32) | w0) >> sh) | (((uint64_t)w2 << ((64 - sh) & 63)) & (three ? ~0ull : 0ull));
+  return (int64_t)((uint64_t)r.base + (v & r.mask));
+}
+// Value of a column at row `local` of the segment, from global memory.
+__device__ __forceinline__ int64_t plan_get(const PlanCol &pc, uint64_t local) {
+  if (pc.bits == 0) return pc.base;
+  return col_read(col_reader(pc, pc.w), local);
+}
+
+// A staged column of the tile starting at row r0: its LDS words and the bit offset of row r0.
+struct PlanSlot {
+  const uint32_t *w;
+  uint32_t bit0, bits, mask;
+};
+__device__ __forceinline__ PlanSlot plan_slot(const uint32_t *lds, const PlanCol &pc, uint32_t r0) {
+  const uint64_t b0 = (uint64_t)r0 * pc.bits;
+  const uint32_t a = (uint32_t)(((uintptr_t)(pc.w + (b0 >> 5)) >> 2) & 3);
+  return PlanSlot{lds + (pc.slot >= 0 ? pc.slot : 0), (uint32_t)(b0 & 31) + 32 * a, pc.bits,
+                  (1u << pc.bits) - 1u};
+}
+__device__ __forceinline__ uint32_t slot_code(const PlanSlot &s, uint32_t i) {
+  const uint32_t pos = s.bit0 + i * s.bits, wi = pos >> 5;
+  return __builtin_amdgcn_alignbit(s.w[wi + 1], s.w[wi], pos & 31) & s.mask;
+}
+
+// Copies the plan's staged columns for rows [r0, r0 + nt) into LDS: the aligned 16-byte
+// blocks covering each column's codes (an aligned block never crosses a page, so the words
+// outside the column it may carry are readable), all of a thread's loads (at most PL_VEC)
+// issued before its first store.  Every thread calls it; a barrier must follow.
+__device__ __forceinline__ void stage_plan(uint32_t *lds, const SegPlan &P, uint32_t r0, uint32_t nt) {
+  // one index space over the blocks of the staged columns; for column k, block v (global
+  // index) is src[k][v] and goes to LDS block v + dst[k] (pointers kept as pointers, so the
+  // loads stay global-memory loads)
+  const uint4 *src[PL_OWN];
+  int32_t dst[PL_OWN];
+  uint32_t end[PL_OWN];
+  uint32_t tot = 0;
+#pragma unroll
+  for (int k = 0; k < PL_OWN; ++k) {
+    const StageSrc S = P.st[k];
+    uint32_t nb = 0;
+    const uint4 *a4 = reinterpret_cast<const uint4 *>(S.w);
+    if (S.bits) {
+      const uint64_t b0 = (uint64_t)r0 * S.bits;
+      const uint64_t wbeg = b0 >> 5, wend = ((uint64_t)(r0 + nt) * S.bits + 31) >> 5;
+      const uint32_t *p = S.w + wbeg;
+      const uint32_t a = (uint32_t)(((uintptr_t)p >> 2) & 3);
+      a4 = reinterpret_cast<const uint4 *>(p - a);
+      nb = (uint32_t)((a + (wend - wbeg) + 3) >> 2);
+    }
+    src[k] = a4 - tot;
+    dst[k] = S.slot / 4 - (int32_t)tot;
+    tot += nb;
+    end[k] = tot;
+  }
+  uint4 *l4 = reinterpret_cast<uint4 *>(lds);
+  uint4 r[PL_VEC];
+#pragma unroll
+  for (int k = 0; k < PL_VEC; ++k) {
+    const uint32_t v = k * ST_BLOCK + threadIdx.x;
+    const uint4 *s0 = v < end[0] ? src[0] : (v < end[1] ? src[1] : src[2]);
+    r[k] = make_uint4(0, 0, 0, 0);
+    if (v < tot) r[k] = gload4(s0 + v);
+  }
+#pragma unroll
+  for (int k = 0; k < PL_VEC; ++k) {
+    const uint32_t v = k * ST_BLOCK + threadIdx.x;
+    const int32_t d0 = v < end[0] ? dst[0] : (v < end[1] ? dst[1] : dst[2]);
+    if (v < tot) l4[(int32_t)v + d0] = r[k];
+  }
+}
+
+// The fast decoder's view of the range (role 0, mode 'R') and probe-key (role 1) slots.
+__device__ __forceinline__ FastTile plan_fast(const uint32_t *lds, const SegPlan &P, uint32_t r0) {
+  FastTile f;
+  f.range = P.mode == 'R';
+  const PlanSlot k = plan_slot(lds, P.col[1], r0);
+  f.kw = k.w;
+  f.kbit0 = k.bit0;
+  f.kbits = k.bits;
+  f.kmask = k.mask;
+  f.rw = k.w;
+  f.rbit0 = f.rbits = f.rmask = f.clo = f.cspan = 0;
+  if (f.range) {
+    const PlanSlot q = plan_slot(lds, P.col[0], r0);
+    f.rw = q.w;
+    f.rbit0 = q.bit0;
+    f.rbits = q.bits;
+    f.rmask = q.mask;
+    f.clo = P.clo;
+    f.cspan = P.cspan;
+  }
+  return f;
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
 }
 
 // ---- D: dimension scan into a key bitmap --------------------------------------------------
@@ -290,54 +539,61 @@ __global__ void __launch_bounds__(ST_BLOCK) k_star_dim(ScanDesc d, int kcol, int
 }
 
 // ---- M: mid scan, probe the dimension bitmap, compact passing rows ------------------------
-__global__ void __launch_bounds__(ST_BLOCK) k_star_mid(ScanDesc d, int kcol, int fkcol, int g1col,
-                                                       int g2col, const uint32_t *__restrict__ bitmap,
-                                                       int64_t dkmin, uint64_t drange,
-                                                       uint32_t *__restrict__ counts,
+// Plan roles: 0 range, 1 foreign key (probes the dimension bitmap), 2 own key, 3 and 4 the
+// group columns; all but the range are staged when not constant.  The passing rows of wave w
+// of tile b go to region b * 8 + w (PL_WAVE_ROWS entries), counted in counts[b * 8 + w].
+constexpr uint32_t MID_STAGE = (1u << 1) | (1u << 2) | (1u << 3) | (1u << 4);
+
+__global__ void __launch_bounds__(ST_BLOCK, 8) k_star_mid(ScanDesc d, const SegPlan *__restrict__ plan, int kcol,
+                                                       int fkcol, int g1col, int g2col,
+                                                       const uint32_t *__restrict__ bitmap, int64_t dkmin,
+                                                       uint64_t drange, uint32_t *__restrict__ counts,
                                                        uint32_t *__restrict__ probed,
                                                        int64_t *__restrict__ okey,
                                                        int64_t *__restrict__ og1,
                                                        int64_t *__restrict__ og2,
-                                                       uint32_t *__restrict__ mbits, int64_t mkmin,
-                                                       uint32_t *dup) {
-  // passing rows of tile b are written to entries [b * ST_TILE, b * ST_TILE + counts[b])
-  __shared__ uint32_t woff[ST_ITEMS][ST_BLOCK / 64];
-  __shared__ TileStage ts;
+                                                       uint32_t *__restrict__ mbits, int64_t mkmin) {
+  static_assert(sizeof(TileStage) <= PL_STAGE_WORDS * sizeof(uint32_t), "generic staging fits the LDS area");
+  __shared__ __align__(16) uint32_t lds[PL_STAGE_WORDS];
   __shared__ int want[ALN_MAX_RANGES + 1];
+  __shared__ TilePred tp;
   const uint32_t seg = blockIdx.x / d.tps, tile = blockIdx.x % d.tps;
-  const uint32_t nrows = d.segs[0][seg].nrows, r0 = tile * ST_TILE;
-  const char mode = sd_mode(d, seg);
+  const SegPlan &P = plan[seg];
+  const uint32_t r0 = tile * ST_TILE, nrows = P.nrows;
+  const unsigned lane = __lane_id(), wid = threadIdx.x / 64;
+  const uint32_t region = blockIdx.x * (ST_BLOCK / 64) + wid;
+  const int mode = P.mode;
   if (r0 >= nrows || mode == 'N') {
-    if (threadIdx.x == 0) {
-      counts[blockIdx.x] = 0;
-      probed[blockIdx.x] = 0;
+    if (lane == 0) {
+      counts[region] = 0;
+      probed[region] = 0;
     }
     return;
   }
   const uint32_t nt = nrows - r0 < (uint32_t)ST_TILE ? nrows - r0 : (uint32_t)ST_TILE;
-  __shared__ TilePred tp;
-  star_stage(ts, want, d, mode, fkcol, seg, r0, nt, &tp);
-  const unsigned lane = __lane_id(), wid = threadIdx.x / 64;
-  // predicate and foreign key from LDS for all items, then the dimension-bitmap words, the
-  // keys of the joined rows and the mid-bitmap updates, each issued for all items at once
-  const int fs = ts.slot_of[fkcol];
-  const int64_t foff = fs >= 0 ? (int64_t)((uint64_t)ts.slot[fs].base - (uint64_t)dkmin) : 0;
+  TileStage &ts = *reinterpret_cast<TileStage *>(lds);
+  const bool fast = mode != 'G';
   uint64_t dc[ST_ITEMS];
   uint32_t nprobe = 0;
-  FastTile ft;
-  if (fast_tile(d, ts, tp, mode, fkcol, ft) && foff >= INT32_MIN && foff <= INT32_MAX && drange <= (1ull << 31)) {
-    // branch-free, 32-bit key shift (see k_star_fact_bm)
-    const uint32_t off = (uint32_t)(int32_t)foff, dr = (uint32_t)drange;
+  if (fast) {
+    stage_plan(lds, P, r0, nt);
+    __syncthreads();
+    const FastTile ft = plan_fast(lds, P, r0);
+    const uint32_t off = (uint32_t)(int32_t)((uint64_t)P.col[1].base - (uint64_t)dkmin), dr = (uint32_t)drange;
     uint32_t c32[ST_ITEMS];
     nprobe = ft.range ? fast_probe<true>(ft, nt, off, dr, c32) : fast_probe<false>(ft, nt, off, dr, c32);
 #pragma unroll
     for (int j = 0; j < ST_ITEMS; ++j) dc[j] = c32[j] != 0xFFFFFFFFu ? c32[j] : ~0ull;
   } else {
+    const char zm = (char)P.zmode;
+    star_stage(ts, want, d, zm, fkcol, seg, r0, nt, &tp);
+    const int fs = ts.slot_of[fkcol];
+    const int64_t foff = fs >= 0 ? (int64_t)((uint64_t)ts.slot[fs].base - (uint64_t)dkmin) : 0;
 #pragma unroll
     for (int j = 0; j < ST_ITEMS; ++j) {
       const uint32_t i = j * ST_BLOCK + threadIdx.x;
       dc[j] = ~0ull;
-      if (i < nt && tile_pass_fast(d, ts, tp, mode, seg, r0, i)) {
+      if (i < nt && tile_pass_fast(d, ts, tp, zm, seg, r0, i)) {
         ++nprobe;
         const uint64_t b = fs >= 0 ? (uint64_t)((int64_t)stage_code(ts.words[fs], ts.slot[fs], i) + foff)
                                    : (uint64_t)tile_get(ts, d.words, d.segs, fkcol, seg, r0, i) - (uint64_t)dkmin;
@@ -348,46 +604,50 @@ __global__ void __launch_bounds__(ST_BLOCK) k_star_mid(ScanDesc d, int kcol, int
   uint32_t dw[ST_ITEMS];
 #pragma unroll
   for (int j = 0; j < ST_ITEMS; ++j) dw[j] = bitmap[dc[j] != ~0ull ? dc[j] >> 5 : 0u];
-  uint32_t passmask = 0, mine = 0;
-#pragma unroll
-  for (int j = 0; j < ST_ITEMS; ++j) {
-    const bool p = dc[j] != ~0ull && ((dw[j] >> (dc[j] & 31)) & 1);
-    passmask |= (uint32_t)p << j;
-    mine += p;
-  }
-  int64_t key[ST_ITEMS];
+  uint32_t passmask = 0;
 #pragma unroll
   for (int j = 0; j < ST_ITEMS; ++j)
-    key[j] = ((passmask >> j) & 1) ? tile_get(ts, d.words, d.segs, kcol, seg, r0, j * ST_BLOCK + threadIdx.x) : 0;
-  if (mbits) {  // no returned values: a duplicate key shows as popcount(mbits) < passing rows
-#pragma unroll
-    for (int j = 0; j < ST_ITEMS; ++j)
-      if ((passmask >> j) & 1) {
-        const uint64_t code = (uint64_t)key[j] - (uint64_t)mkmin;
-        atomicOr(&mbits[code >> 5], 1u << (code & 31));
-      }
-  }
-  block_count(nprobe, probed);
-  // passing rows in item-major order (consecutive lanes write consecutive entries)
-#pragma unroll
-  for (int j = 0; j < ST_ITEMS; ++j) {
-    const uint64_t b = __ballot((passmask >> j) & 1);
-    if (lane == 0) woff[j][wid] = (uint32_t)__popcll(b);
-  }
-  __syncthreads();
-  if (threadIdx.x < 64) scan_tile_counts(&woff[0][0], &counts[blockIdx.x]);
-  __syncthreads();
-  const uint64_t base = (uint64_t)blockIdx.x * ST_TILE;
+    passmask |= (uint32_t)(dc[j] != ~0ull && ((dw[j] >> (dc[j] & 31)) & 1)) << j;
+  // the passing rows' key and group values: staged (fast) or decoded (generic)
+  const PlanSlot ks = plan_slot(lds, P.col[2], r0), s1 = plan_slot(lds, P.col[3], r0),
+                 s2 = plan_slot(lds, P.col[4], r0);
+  // passing rows of the wave in item-major order (consecutive lanes write consecutive
+  // entries); no barrier: each wave owns its region
+  const uint64_t rbase = (uint64_t)region * PL_WAVE_ROWS;
+  uint32_t nout = 0;
 #pragma unroll
   for (int j = 0; j < ST_ITEMS; ++j) {
     const bool h = (passmask >> j) & 1;
     const uint64_t b = __ballot(h);
-    if (!h) continue;
-    const uint32_t i = j * ST_BLOCK + threadIdx.x;
-    const uint64_t pos = base + woff[j][wid] + (uint32_t)__popcll(b & lanemask_lt());
-    okey[pos] = key[j];
-    og1[pos] = g1col >= 0 ? tile_get(ts, d.words, d.segs, g1col, seg, r0, i) : 0;
-    og2[pos] = g2col >= 0 ? tile_get(ts, d.words, d.segs, g2col, seg, r0, i) : 0;
+    if (h) {
+      const uint32_t i = j * ST_BLOCK + threadIdx.x;
+      int64_t key, v1 = 0, v2 = 0;
+      if (fast) {
+        key = (int64_t)((uint64_t)P.col[2].base + slot_code(ks, i));
+        if (g1col >= 0)
+          v1 = P.col[3].slot >= 0 ? (int64_t)((uint64_t)P.col[3].base + slot_code(s1, i)) : plan_get(P.col[3], r0 + i);
+        if (g2col >= 0)
+          v2 = P.col[4].slot >= 0 ? (int64_t)((uint64_t)P.col[4].base + slot_code(s2, i)) : plan_get(P.col[4], r0 + i);
+      } else {
+        key = tile_get(ts, d.words, d.segs, kcol, seg, r0, i);
+        if (g1col >= 0) v1 = tile_get(ts, d.words, d.segs, g1col, seg, r0, i);
+        if (g2col >= 0) v2 = tile_get(ts, d.words, d.segs, g2col, seg, r0, i);
+      }
+      if (mbits) {  // no returned value: a duplicate key shows as popcount(mbits) < passing rows
+        const uint64_t code = (uint64_t)key - (uint64_t)mkmin;
+        atomicOr(&mbits[code >> 5], 1u << (code & 31));
+      }
+      const uint64_t pos = rbase + nout + (uint32_t)__popcll(b & lanemask_lt());
+      okey[pos] = key;
+      og1[pos] = v1;
+      og2[pos] = v2;
+    }
+    nout += (uint32_t)__popcll(b);
+  }
+  nprobe = wave_sum(nprobe);
+  if (lane == 0) {
+    counts[region] = nout;
+    probed[region] = nprobe;
   }
 }
 
@@ -396,7 +656,7 @@ __device__ __forceinline__ uint64_t st_slot(uint32_t code, uint64_t cap) {
 }
 
 // ---- B: build the join table over the compacted mid rows ----------------------------------
-// One block per mid tile region: entry i of tile b becomes group offs[b] + i; its key and
+// One block per mid wave region: entry i of region b becomes group offs[b] + i; its key and
 // group columns move to dense arrays and the key goes into the table.
 __global__ void __launch_bounds__(256) k_star_build(const int64_t *__restrict__ rkey,
                                                     const int64_t *__restrict__ rg1,
@@ -409,7 +669,7 @@ __global__ void __launch_bounds__(256) k_star_build(const int64_t *__restrict__ 
                                                     int64_t *__restrict__ og2, uint32_t *dup) {
   const uint32_t n = counts[blockIdx.x];
   for (uint32_t i = threadIdx.x; i < n; i += 256) {
-    const uint64_t src = (uint64_t)blockIdx.x * ST_TILE + i, j = offs[blockIdx.x] + i;
+    const uint64_t src = (uint64_t)blockIdx.x * PL_WAVE_ROWS + i, j = offs[blockIdx.x] + i;
     okey[j] = rkey[src];
     og1[j] = rg1[src];
     og2[j] = rg2[src];
@@ -548,7 +808,7 @@ __device__ __forceinline__ uint64_t bm_rank(const uint32_t *__restrict__ bits,
   return r;
 }
 
-// One block per mid tile region (see k_star_mid); entry i moves to position rank(key).
+// One block per mid wave region (see k_star_mid); entry i moves to position rank(key).
 __global__ void __launch_bounds__(256) k_bm_scatter(const int64_t *__restrict__ rkey,
                                                    const int64_t *__restrict__ rg1,
                                                    const int64_t *__restrict__ rg2,
@@ -560,7 +820,7 @@ __global__ void __launch_bounds__(256) k_bm_scatter(const int64_t *__restrict__ 
                                                    int64_t *__restrict__ sg2) {
   const uint32_t n = counts[blockIdx.x];
   for (uint32_t i = threadIdx.x; i < n; i += 256) {
-    const uint64_t src = (uint64_t)blockIdx.x * ST_TILE + i;
+    const uint64_t src = (uint64_t)blockIdx.x * PL_WAVE_ROWS + i;
     const uint64_t j = bm_rank(bits, rank, (uint64_t)rkey[src] - (uint64_t)kmin);
     skey[j] = rkey[src];
     sg1[j] = rg1[src];
@@ -568,8 +828,14 @@ __global__ void __launch_bounds__(256) k_bm_scatter(const int64_t *__restrict__ 
   }
 }
 
-__global__ void __launch_bounds__(ST_BLOCK) k_star_fact_bm(ScanDesc d, int fkcol, int xcol, int ycol,
-                                                           int64_t a0, int64_t a1, int64_t b0,
+// Plan roles: 0 range, 1 foreign key (probes the mid bitmap), 2 and 3 the aggregate's x
+// and y, read from global memory for the few rows that join.  Counts are per wave
+// (index b * 8 + w), so the block needs one barrier, after staging.
+constexpr uint32_t FACT_STAGE = 1u << 1;
+
+// (second bound: eight waves per SIMD, four blocks per CU, i.e. at most 64 VGPRs)
+__global__ void __launch_bounds__(ST_BLOCK, 8) k_star_fact_bm(ScanDesc d, const SegPlan *__restrict__ plan,
+                                                           int fkcol, int64_t a0, int64_t a1, int64_t b0,
                                                            int64_t b1, const uint32_t *__restrict__ bits,
                                                            const uint64_t *__restrict__ rank,
                                                            int64_t mkmin, uint64_t mrange,
@@ -577,46 +843,46 @@ __global__ void __launch_bounds__(ST_BLOCK) k_star_fact_bm(ScanDesc d, int fkcol
                                                            unsigned int *__restrict__ cnt,
                                                            uint32_t *__restrict__ matched,
                                                            uint32_t *__restrict__ probed) {
-  // After the probe the staged words are dead: the tile's hits (bitmap code, row) are
-  // compacted into the same LDS, in row order, and joined and aggregated from there.
-  static_assert(sizeof(TileStage) >= ST_TILE * (sizeof(uint32_t) + sizeof(uint16_t)),
-                "hit list must fit in the staging area");
-  __shared__ TileStage ts;
+  static_assert(sizeof(TileStage) <= PL_STAGE_WORDS * sizeof(uint32_t), "generic staging fits the LDS area");
+  __shared__ __align__(16) uint32_t lds[PL_STAGE_WORDS];
   __shared__ int want[ALN_MAX_RANGES + 1];
-  __shared__ uint32_t woff[ST_ITEMS][ST_BLOCK / 64];
-  __shared__ uint32_t nhit_s;
+  __shared__ TilePred tp;
   const uint32_t seg = blockIdx.x / d.tps, tile = blockIdx.x % d.tps;
-  const uint32_t nrows = d.segs[0][seg].nrows, r0 = tile * ST_TILE;
-  const char mode = sd_mode(d, seg);
+  const SegPlan &P = plan[seg];
+  const uint32_t r0 = tile * ST_TILE, nrows = P.nrows;
+  const unsigned lane = __lane_id();
+  const uint32_t cidx = blockIdx.x * (ST_BLOCK / 64) + threadIdx.x / 64;
+  const int mode = P.mode;
   if (r0 >= nrows || mode == 'N') {
-    if (threadIdx.x == 0) {
-      matched[blockIdx.x] = 0;
-      probed[blockIdx.x] = 0;
+    if (lane == 0) {
+      matched[cidx] = 0;
+      probed[cidx] = 0;
     }
     return;
   }
   const uint32_t nt = nrows - r0 < (uint32_t)ST_TILE ? nrows - r0 : (uint32_t)ST_TILE;
-  __shared__ TilePred tp;
-  star_stage(ts, want, d, mode, fkcol, seg, r0, nt, &tp);
-  const unsigned lane = __lane_id(), wave = threadIdx.x / 64;
-  // 1. predicate and key from LDS (code space) for all items, then every bitmap word load
-  const int fs = ts.slot_of[fkcol];
-  const int64_t foff = fs >= 0 ? (int64_t)((uint64_t)ts.slot[fs].base - (uint64_t)mkmin) : 0;
+  // 1. predicate and key (code space) for all items, then every bitmap word load
   uint32_t code[ST_ITEMS];
   uint32_t nprobe = 0;
-  FastTile ft;
-  if (fast_tile(d, ts, tp, mode, fkcol, ft) && foff >= INT32_MIN && foff <= INT32_MAX) {
-    // branch-free: every row's codes are read (the slots span a whole tile), the tests are
-    // combined with & rather than &&, and the key is shifted in 32 bits - code + foff lies
-    // in (-2^31, 2^32), and any value outside [0, mrange) with mrange <= 2^31 maps to >= 2^31
-    const uint32_t off = (uint32_t)(int32_t)foff, mr = (uint32_t)mrange;
+  if (mode != 'G') {
+    stage_plan(lds, P, r0, nt);
+    __syncthreads();
+    const FastTile ft = plan_fast(lds, P, r0);
+    // code + off lies in (-2^31, 2^32): any value outside [0, mrange), mrange <= 2^31, maps
+    // to >= 2^31 in 32 bits
+    const uint32_t off = (uint32_t)(int32_t)((uint64_t)P.col[1].base - (uint64_t)mkmin), mr = (uint32_t)mrange;
     nprobe = ft.range ? fast_probe<true>(ft, nt, off, mr, code) : fast_probe<false>(ft, nt, off, mr, code);
   } else {
+    TileStage &ts = *reinterpret_cast<TileStage *>(lds);
+    const char zm = (char)P.zmode;
+    star_stage(ts, want, d, zm, fkcol, seg, r0, nt, &tp);
+    const int fs = ts.slot_of[fkcol];
+    const int64_t foff = fs >= 0 ? (int64_t)((uint64_t)ts.slot[fs].base - (uint64_t)mkmin) : 0;
 #pragma unroll
     for (int j = 0; j < ST_ITEMS; ++j) {
       const uint32_t i = j * ST_BLOCK + threadIdx.x;
       code[j] = 0xFFFFFFFFu;
-      if (i < nt && tile_pass_fast(d, ts, tp, mode, seg, r0, i)) {
+      if (i < nt && tile_pass_fast(d, ts, tp, zm, seg, r0, i)) {
         ++nprobe;
         const uint64_t c = fs >= 0 ? (uint64_t)((int64_t)stage_code(ts.words[fs], ts.slot[fs], i) + foff)
                                    : (uint64_t)tile_get(ts, d.words, d.segs, fkcol, seg, r0, i) - (uint64_t)mkmin;
@@ -632,63 +898,31 @@ __global__ void __launch_bounds__(ST_BLOCK) k_star_fact_bm(ScanDesc d, int fkcol
 #pragma unroll
   for (int j = 0; j < ST_ITEMS; ++j)
     hits |= (uint32_t)((code[j] != 0xFFFFFFFFu) & ((word[j] >> (code[j] & 31)) & 1)) << j;
-  // 2. ordered compaction: rows are item-major, then wave, then lane
-#pragma unroll
-  for (int j = 0; j < ST_ITEMS; ++j) {
-    const uint64_t b = __ballot((hits >> j) & 1);
-    if (lane == 0) woff[j][wave] = (uint32_t)__popcll(b);
-  }
-  __syncthreads();  // every wave is also done reading the staged words
-  if (threadIdx.x < 64) scan_tile_counts(&woff[0][0], &nhit_s);
-  __syncthreads();
-  uint32_t *hcode = reinterpret_cast<uint32_t *>(&ts);
-  uint16_t *hrow = reinterpret_cast<uint16_t *>(hcode + ST_TILE);
-#pragma unroll
-  for (int j = 0; j < ST_ITEMS; ++j) {
-    const bool h = (hits >> j) & 1;
-    const uint64_t b = __ballot(h);
-    if (h) {
-      const uint32_t pos = woff[j][wave] + (uint32_t)__popcll(b & lanemask_lt());
-      hcode[pos] = code[j];
-      hrow[pos] = (uint16_t)(j * ST_BLOCK + threadIdx.x);
-    }
-  }
-  __syncthreads();
-  // 3. group index (rank of the key), payload and a wave run merge of equal groups
-  const uint32_t nh = nhit_s;
+  // 2. one hit per lane per round (rarely more than one round): its group index (the key's
+  // rank), payload words and the two atomics, the loads of a round issued together
   uint32_t nmatch = 0;
-  for (uint32_t e0 = 0; e0 < nh; e0 += ST_BLOCK) {
-    const uint32_t e = e0 + threadIdx.x;
-    uint32_t g = 0xFFFFFFFFu;
-    int64_t v = 0;
-    if (e < nh) {
-      const uint32_t local = r0 + hrow[e];
-      g = (uint32_t)bm_rank(bits, rank, hcode[e]);
-      const int64_t x = sd_decode(d, xcol, seg, local), y = sd_decode(d, ycol, seg, local);
-      v = wmul((int64_t)((uint64_t)a0 + (uint64_t)wmul(a1, x)), (int64_t)((uint64_t)b0 + (uint64_t)wmul(b1, y)));
-    }
-    if (__ballot(g != 0xFFFFFFFFu) == 0) continue;
-    const uint32_t prev = __shfl_up(g, 1, 64);
-    bool f = lane == 0 || prev != g;
-    uint32_t c = g != 0xFFFFFFFFu;
-    for (int o = 1; o < 64; o <<= 1) {
-      const int64_t vu = __shfl_up(v, o, 64);
-      const uint32_t cu = __shfl_up(c, o, 64);
-      const bool fu = __shfl_up((int)f, o, 64) != 0;
-      if (lane >= (unsigned)o && !f) {
-        v = (int64_t)((uint64_t)v + (uint64_t)vu);
-        c += cu;
-      }
-      if (lane >= (unsigned)o) f = f || fu;
-    }
-    const uint32_t next = __shfl_down(g, 1, 64);
-    if (g != 0xFFFFFFFFu && (lane == 63 || next != g)) {
-      atomicAdd(&sum[g], (unsigned long long)v);
-      atomicAdd(&cnt[g], c);
-      nmatch += c;
-    }
+  const ColReader rx = col_reader(P.col[2], plan), ry = col_reader(P.col[3], plan);
+  while (hits) {
+    const int jh = __builtin_ctz(hits);
+    hits &= hits - 1;
+    uint32_t c = 0;
+#pragma unroll
+    for (int j = 0; j < ST_ITEMS; ++j) c = j == jh ? code[j] : c;
+    const uint32_t local = r0 + (uint32_t)jh * ST_BLOCK + threadIdx.x;
+    const uint32_t g = (uint32_t)bm_rank(bits, rank, c);
+    const int64_t x = col_read(rx, local), y = col_read(ry, local);
+    const int64_t v =
+        wmul((int64_t)((uint64_t)a0 + (uint64_t)wmul(a1, x)), (int64_t)((uint64_t)b0 + (uint64_t)wmul(b1, y)));
+    atomicAdd(&sum[g], (unsigned long long)v);
+    atomicAdd(&cnt[g], 1u);
+    ++nmatch;
   }
-  block_count2(nmatch, matched, nprobe, probed);
+  nmatch = wave_sum(nmatch);
+  nprobe = wave_sum(nprobe);
+  if (lane == 0) {
+    matched[cidx] = nmatch;
+    probed[cidx] = nprobe;
+  }
 }
 
 // ---- T: top-k over groups (sum DESC, g1 ASC, key ASC, g2 ASC) -----------------------------
@@ -1305,6 +1539,7 @@ extern "C" int aln_star_agg(aln_ctx *c, const aln_star_args *a, aln_star_row *h_
   ALN_REQUIRE(c && a && h_ntop, "aln_star_agg: bad arguments");
   ALN_REQUIRE(a->limit <= TK_TILE2 / 2, "aln_star_agg: limit %u above %d", a->limit, TK_TILE2 / 2);
   ALN_REQUIRE(a->limit == 0 || h_top, "aln_star_agg: no top-k buffer");
+  ALN_REQUIRE(a->d_key >= 0 && a->m_key >= 0 && a->m_fk >= 0 && a->f_fk >= 0, "aln_star_agg: key columns");
   ws_reset(c);
   *h_ntop = 0;
   aln_star_stats st;
@@ -1361,12 +1596,28 @@ extern "C" int aln_star_agg(aln_ctx *c, const aln_star_args *a, aln_star_row *h_
   ALN_TRY(ws(c, 8, &ctr));
   dup = reinterpret_cast<uint32_t *>(ctr + 6);
   ALN_TRY(ws(c, dblocks, &dcnt));
-  ALN_TRY(ws(c, mblocks, &mcnt));
-  ALN_TRY(ws(c, mblocks, &mprobe));
-  ALN_TRY(ws(c, fblocks, &fmatch));
-  ALN_TRY(ws(c, fblocks, &fprobe));
+  const uint64_t mregions = mblocks * (ST_BLOCK / 64), fwaves = fblocks * (ST_BLOCK / 64);
+  ALN_TRY(ws(c, mregions, &mcnt));
+  ALN_TRY(ws(c, mregions, &mprobe));
+  ALN_TRY(ws(c, fwaves, &fmatch));
+  ALN_TRY(ws(c, fwaves, &fprobe));
   ALN_TRY(fill_u32(c, bitmap, bwords, 0));
   ALN_TRY(fill_u64(c, ctr, 8, 0));
+  // per-segment plans of the mid and fact scans
+  SegPlan *mplan, *fplan = nullptr;
+  {
+    const uint32_t ms = a->mid.cols[0].nsegs, fs = a->fact.cols[0].nsegs;
+    ALN_TRY(ws(c, ms, &mplan));
+    hipLaunchKernelGGL(k_scan_plan, dim3((ms + 127) / 128), dim3(128), 0, c->stream, md, a->m_fk, a->m_key, a->m_g1,
+                       a->m_g2, MID_STAGE, dkmin, (int)(drange <= (1ull << 31)), ms, mplan);
+    ALN_TRY(check_launch("scan_plan"));
+    if (use_bm && fs) {
+      ALN_TRY(ws(c, fs, &fplan));
+      hipLaunchKernelGGL(k_scan_plan, dim3((fs + 127) / 128), dim3(128), 0, c->stream, fd, a->f_fk, a->f_x, a->f_y,
+                         -1, FACT_STAGE, mkmin, 1, fs, fplan);
+      ALN_TRY(check_launch("scan_plan"));
+    }
+  }
 
   tm.mark(0);
   {
@@ -1377,21 +1628,22 @@ extern "C" int aln_star_agg(aln_ctx *c, const aln_star_args *a, aln_star_row *h_
   ALN_TRY(check_launch("star_dim"));
   tm.mark(1);
 
-  // mid scan: passing rows of tile b land in entries [b * ST_TILE, ...) of the region arrays
+  // mid scan: passing rows of wave region b land in entries [b * PL_WAVE_ROWS, ...) of the
+  // region arrays
   int64_t *rkey, *rg1, *rg2;
-  ALN_TRY(ws(c, mblocks * ST_TILE, &rkey));
-  ALN_TRY(ws(c, mblocks * ST_TILE, &rg1));
-  ALN_TRY(ws(c, mblocks * ST_TILE, &rg2));
+  ALN_TRY(ws(c, mregions * PL_WAVE_ROWS, &rkey));
+  ALN_TRY(ws(c, mregions * PL_WAVE_ROWS, &rg1));
+  ALN_TRY(ws(c, mregions * PL_WAVE_ROWS, &rg2));
   {
     ProfScope p(c, "star_mid");
-    hipLaunchKernelGGL(k_star_mid, dim3((unsigned)mblocks), dim3(ST_BLOCK), 0, c->stream, md, a->m_key,
+    hipLaunchKernelGGL(k_star_mid, dim3((unsigned)mblocks), dim3(ST_BLOCK), 0, c->stream, md, mplan, a->m_key,
                        a->m_fk, a->m_g1, a->m_g2, bitmap, dkmin, drange, mcnt, mprobe, rkey, rg1, rg2,
-                       mbits, mkmin, dup + 1);
+                       mbits, mkmin);
   }
   ALN_TRY(check_launch("star_mid"));
   ALN_TRY(sum_u32(c, dcnt, dblocks, ctr + 0));
-  ALN_TRY(sum_u32(c, mcnt, mblocks, ctr + 1));
-  ALN_TRY(sum_u32(c, mprobe, mblocks, ctr + 2));
+  ALN_TRY(sum_u32(c, mcnt, mregions, ctr + 1));
+  ALN_TRY(sum_u32(c, mprobe, mregions, ctr + 2));
   // bitmap join: the rank directory now (its total = distinct passing keys), before the
   // one synchronisation that reads the counters
   uint64_t *rank = nullptr;
@@ -1437,37 +1689,38 @@ extern "C" int aln_star_agg(aln_ctx *c, const aln_star_args *a, aln_star_row *h_
     // the group attributes in rank (= key) order
     if (m) {
       ProfScope p(c, "star_build");
-      hipLaunchKernelGGL(k_bm_scatter, dim3((unsigned)mblocks), dim3(256), 0, c->stream, rkey, rg1, rg2,
+      hipLaunchKernelGGL(k_bm_scatter, dim3((unsigned)mregions), dim3(256), 0, c->stream, rkey, rg1, rg2,
                          mcnt, mbits, rank, mkmin, okey, og1, og2);
     }
     ALN_TRY(check_launch("bm_scatter"));
   } else {
     uint64_t *moff;
-    ALN_TRY(ws(c, mblocks, &moff));
-    ALN_TRY(exclusive_scan_u32(c, mcnt, mblocks, moff, nullptr));
+    ALN_TRY(ws(c, mregions, &moff));
+    ALN_TRY(exclusive_scan_u32(c, mcnt, mregions, moff, nullptr));
     tcap = m + m / 2 + 1024;
     ALN_TRY(ws(c, tcap, &table));
     ALN_TRY(fill_u64(c, table, tcap, ST_EMPTY));
     if (m) {
       ProfScope p(c, "star_build");
-      hipLaunchKernelGGL(k_star_build, dim3((unsigned)mblocks), dim3(256), 0, c->stream, rkey, rg1, rg2,
+      hipLaunchKernelGGL(k_star_build, dim3((unsigned)mregions), dim3(256), 0, c->stream, rkey, rg1, rg2,
                          mcnt, moff, mkmin, table, tcap, okey, og1, og2, dup + 1);
     }
     ALN_TRY(check_launch("star_build"));
   }
   tm.mark(2);
-  if (m) {
+  if (m && fblocks) {
     ProfScope p(c, "star_fact");
     if (use_bm)
-      hipLaunchKernelGGL(k_star_fact_bm, dim3((unsigned)fblocks), dim3(ST_BLOCK), 0, c->stream, fd, a->f_fk, a->f_x,
-                         a->f_y, a->a0, a->a1, a->b0, a->b1, mbits, rank, mkmin, mrange, sum, cnt, fmatch, fprobe);
-    else
+      hipLaunchKernelGGL(k_star_fact_bm, dim3((unsigned)fblocks), dim3(ST_BLOCK), 0, c->stream, fd, fplan, a->f_fk,
+                         a->a0, a->a1, a->b0, a->b1, mbits, rank, mkmin, mrange, sum, cnt, fmatch, fprobe);
+    else  // per-block counts
       hipLaunchKernelGGL(k_star_fact, dim3((unsigned)fblocks), dim3(ST_BLOCK), 0, c->stream, fd, a->f_fk,
                          a->f_x, a->f_y, a->a0, a->a1, a->b0, a->b1, table, tcap, mkmin, mrange, sum, cnt,
                          fmatch, fprobe);
     ALN_TRY(check_launch("star_fact"));
-    ALN_TRY(sum_u32(c, fmatch, fblocks, ctr + 3));
-    ALN_TRY(sum_u32(c, fprobe, fblocks, ctr + 4));
+    const uint64_t nfc = use_bm ? fwaves : fblocks;
+    ALN_TRY(sum_u32(c, fmatch, nfc, ctr + 3));
+    ALN_TRY(sum_u32(c, fprobe, nfc, ctr + 4));
   }
   tm.mark(3);
 
