@@ -168,25 +168,29 @@ __device__ __forceinline__ uint32_t fast_code(const uint32_t *w, uint32_t bit0, 
 
 // Straight-line decode of a tile's rows for the fast shape: per row, the range test (RANGE)
 // and the key shifted by `off` in 32 bits; code[j] = key code, or ~0 when the row does not
-// probe (fails the range, lies past the tile, or the key falls outside [0, lim)).  Every
-// read stays inside the slots, which span a whole tile.  Returns the number of probes.
+// probe (fails the range, lies past the tile, or the key falls outside [0, lim)).  Row
+// j * 512 + t sits 512 * bits bits (16 * bits words) after row (j - 1) * 512 + t, so a
+// thread's rows share one funnel-shift amount and step through LDS by a constant.  Every
+// read stays inside the slots, which span a whole tile.  Returns the wave's probe count.
 template <bool RANGE>
 __device__ __forceinline__ uint32_t fast_probe(const FastTile &f, uint32_t nt, uint32_t off, uint32_t lim,
                                                uint32_t (&code)[ST_ITEMS]) {
   const uint32_t kp0 = f.kbit0 + threadIdx.x * f.kbits, rp0 = f.rbit0 + threadIdx.x * f.rbits;
+  const uint32_t *kw = f.kw + (kp0 >> 5), *rw = f.rw + (rp0 >> 5);
+  const uint32_t ksh = kp0 & 31, rsh = rp0 & 31, kstep = 16 * f.kbits, rstep = 16 * f.rbits;
   uint32_t n = 0;
 #pragma unroll
   for (int j = 0; j < ST_ITEMS; ++j) {
     const uint32_t i = j * ST_BLOCK + threadIdx.x;
     bool p = i < nt;
     if (RANGE) {
-      const uint32_t rp = rp0 + j * (ST_BLOCK * f.rbits), rwi = rp >> 5;
-      const uint32_t rc = __builtin_amdgcn_alignbit(f.rw[rwi + 1], f.rw[rwi], rp & 31) & f.rmask;
+      const uint32_t *q = rw + j * rstep;
+      const uint32_t rc = __builtin_amdgcn_alignbit(q[1], q[0], rsh) & f.rmask;
       p = p & (rc - f.clo <= f.cspan);
     }
-    const uint32_t kp = kp0 + j * (ST_BLOCK * f.kbits), kwi = kp >> 5;
-    const uint32_t c = (__builtin_amdgcn_alignbit(f.kw[kwi + 1], f.kw[kwi], kp & 31) & f.kmask) + off;
-    n += p;
+    const uint32_t *q = kw + j * kstep;
+    const uint32_t c = (__builtin_amdgcn_alignbit(q[1], q[0], ksh) & f.kmask) + off;
+    n += (uint32_t)__popcll(__builtin_amdgcn_ballot_w64(p));
     code[j] = (p & (c < lim)) ? c : 0xFFFFFFFFu;
   }
   return n;
@@ -394,49 +398,39 @@ __device__ __forceinline__ uint32_t slot_code(const PlanSlot &s, uint32_t i) {
 
 // Copies the plan's staged columns for rows [r0, r0 + nt) into LDS: the aligned 16-byte
 // blocks covering each column's codes (an aligned block never crosses a page, so the words
-// outside the column it may carry are readable), all of a thread's loads (at most PL_VEC)
-// issued before its first store.  Every thread calls it; a barrier must follow.
+// outside the column it may carry are readable), all of a thread's loads (PL_CVEC per
+// column) issued before its first store.  Every thread calls it; a barrier must follow.
+constexpr int PL_CVEC = (STG_SLOT_WORDS / 4 + ST_BLOCK - 1) / ST_BLOCK;  // loads per column
 __device__ __forceinline__ void stage_plan(uint32_t *lds, const SegPlan &P, uint32_t r0, uint32_t nt) {
-  // one index space over the blocks of the staged columns; for column k, block v (global
-  // index) is src[k][v] and goes to LDS block v + dst[k] (pointers kept as pointers, so the
-  // loads stay global-memory loads)
   const uint4 *src[PL_OWN];
-  int32_t dst[PL_OWN];
-  uint32_t end[PL_OWN];
-  uint32_t tot = 0;
+  uint32_t nb[PL_OWN];
 #pragma unroll
   for (int k = 0; k < PL_OWN; ++k) {
     const StageSrc S = P.st[k];
-    uint32_t nb = 0;
-    const uint4 *a4 = reinterpret_cast<const uint4 *>(S.w);
-    if (S.bits) {
-      const uint64_t b0 = (uint64_t)r0 * S.bits;
-      const uint64_t wbeg = b0 >> 5, wend = ((uint64_t)(r0 + nt) * S.bits + 31) >> 5;
-      const uint32_t *p = S.w + wbeg;
-      const uint32_t a = (uint32_t)(((uintptr_t)p >> 2) & 3);
-      a4 = reinterpret_cast<const uint4 *>(p - a);
-      nb = (uint32_t)((a + (wend - wbeg) + 3) >> 2);
+    const uint64_t b0 = (uint64_t)r0 * S.bits;
+    const uint64_t wbeg = b0 >> 5, wend = ((uint64_t)(r0 + nt) * S.bits + 31) >> 5;
+    const uint32_t *p = S.w + wbeg;
+    const uint32_t a = (uint32_t)(((uintptr_t)p >> 2) & 3);
+    src[k] = reinterpret_cast<const uint4 *>(p - a);
+    nb[k] = S.bits ? (uint32_t)((a + (wend - wbeg) + 3) >> 2) : 0u;
+  }
+  uint4 r[PL_OWN][PL_CVEC];
+#pragma unroll
+  for (int k = 0; k < PL_OWN; ++k)
+#pragma unroll
+    for (int t = 0; t < PL_CVEC; ++t) {
+      const uint32_t v = t * ST_BLOCK + threadIdx.x;
+      r[k][t] = make_uint4(0, 0, 0, 0);
+      if (v < nb[k]) r[k][t] = gload4(src[k] + v);
     }
-    src[k] = a4 - tot;
-    dst[k] = S.slot / 4 - (int32_t)tot;
-    tot += nb;
-    end[k] = tot;
-  }
   uint4 *l4 = reinterpret_cast<uint4 *>(lds);
-  uint4 r[PL_VEC];
 #pragma unroll
-  for (int k = 0; k < PL_VEC; ++k) {
-    const uint32_t v = k * ST_BLOCK + threadIdx.x;
-    const uint4 *s0 = v < end[0] ? src[0] : (v < end[1] ? src[1] : src[2]);
-    r[k] = make_uint4(0, 0, 0, 0);
-    if (v < tot) r[k] = gload4(s0 + v);
-  }
+  for (int k = 0; k < PL_OWN; ++k)
 #pragma unroll
-  for (int k = 0; k < PL_VEC; ++k) {
-    const uint32_t v = k * ST_BLOCK + threadIdx.x;
-    const int32_t d0 = v < end[0] ? dst[0] : (v < end[1] ? dst[1] : dst[2]);
-    if (v < tot) l4[(int32_t)v + d0] = r[k];
-  }
+    for (int t = 0; t < PL_CVEC; ++t) {
+      const uint32_t v = t * ST_BLOCK + threadIdx.x;
+      if (v < nb[k]) l4[P.st[k].slot / 4 + (int32_t)v] = r[k][t];
+    }
 }
 
 // The fast decoder's view of the range (role 0, mode 'R') and probe-key (role 1) slots.
@@ -584,7 +578,7 @@ __global__ void __launch_bounds__(ST_BLOCK, 8) k_star_mid(ScanDesc d, const SegP
     nprobe = ft.range ? fast_probe<true>(ft, nt, off, dr, c32) : fast_probe<false>(ft, nt, off, dr, c32);
 #pragma unroll
     for (int j = 0; j < ST_ITEMS; ++j) dc[j] = c32[j] != 0xFFFFFFFFu ? c32[j] : ~0ull;
-  } else {
+  } else {  // per-lane count, summed over the wave below
     const char zm = (char)P.zmode;
     star_stage(ts, want, d, zm, fkcol, seg, r0, nt, &tp);
     const int fs = ts.slot_of[fkcol];
@@ -600,6 +594,7 @@ __global__ void __launch_bounds__(ST_BLOCK, 8) k_star_mid(ScanDesc d, const SegP
         if (b < drange) dc[j] = b;
       }
     }
+    nprobe = wave_sum(nprobe);
   }
   uint32_t dw[ST_ITEMS];
 #pragma unroll
@@ -644,7 +639,6 @@ __global__ void __launch_bounds__(ST_BLOCK, 8) k_star_mid(ScanDesc d, const SegP
     }
     nout += (uint32_t)__popcll(b);
   }
-  nprobe = wave_sum(nprobe);
   if (lane == 0) {
     counts[region] = nout;
     probed[region] = nprobe;
@@ -656,9 +650,9 @@ __device__ __forceinline__ uint64_t st_slot(uint32_t code, uint64_t cap) {
 }
 
 // ---- B: build the join table over the compacted mid rows ----------------------------------
-// One block per mid wave region: entry i of region b becomes group offs[b] + i; its key and
-// group columns move to dense arrays and the key goes into the table.
-__global__ void __launch_bounds__(256) k_star_build(const int64_t *__restrict__ rkey,
+// One block per mid tile, one wave per wave region: entry i of region b becomes group
+// offs[b] + i; its key and group columns move to dense arrays and the key goes into the table.
+__global__ void __launch_bounds__(ST_BLOCK) k_star_build(const int64_t *__restrict__ rkey,
                                                     const int64_t *__restrict__ rg1,
                                                     const int64_t *__restrict__ rg2,
                                                     const uint32_t *__restrict__ counts,
@@ -667,9 +661,10 @@ __global__ void __launch_bounds__(256) k_star_build(const int64_t *__restrict__ 
                                                     int64_t *__restrict__ okey,
                                                     int64_t *__restrict__ og1,
                                                     int64_t *__restrict__ og2, uint32_t *dup) {
-  const uint32_t n = counts[blockIdx.x];
-  for (uint32_t i = threadIdx.x; i < n; i += 256) {
-    const uint64_t src = (uint64_t)blockIdx.x * PL_WAVE_ROWS + i, j = offs[blockIdx.x] + i;
+  const uint32_t region = blockIdx.x * (ST_BLOCK / 64) + threadIdx.x / 64;
+  const uint32_t n = counts[region];
+  for (uint32_t i = __lane_id(); i < n; i += 64) {
+    const uint64_t src = (uint64_t)region * PL_WAVE_ROWS + i, j = offs[region] + i;
     okey[j] = rkey[src];
     og1[j] = rg1[src];
     og2[j] = rg2[src];
@@ -808,20 +803,50 @@ __device__ __forceinline__ uint64_t bm_rank(const uint32_t *__restrict__ bits,
   return r;
 }
 
-// One block per mid wave region (see k_star_mid); entry i moves to position rank(key).
-__global__ void __launch_bounds__(256) k_bm_scatter(const int64_t *__restrict__ rkey,
-                                                   const int64_t *__restrict__ rg1,
-                                                   const int64_t *__restrict__ rg2,
-                                                   const uint32_t *__restrict__ counts,
-                                                   const uint32_t *__restrict__ bits,
-                                                   const uint64_t *__restrict__ rank, int64_t kmin,
-                                                   int64_t *__restrict__ skey,
-                                                   int64_t *__restrict__ sg1,
-                                                   int64_t *__restrict__ sg2) {
-  const uint32_t n = counts[blockIdx.x];
-  for (uint32_t i = threadIdx.x; i < n; i += 256) {
-    const uint64_t src = (uint64_t)blockIdx.x * PL_WAVE_ROWS + i;
-    const uint64_t j = bm_rank(bits, rank, (uint64_t)rkey[src] - (uint64_t)kmin);
+// Rank of the first bit of every bitmap word (the bits set in the words before it): its
+// 512-bit block's directory entry plus the preceding words of the block.  One thread per
+// block, 64 bytes in and out; ranks fit 32 bits (at most 2^31 keys).
+__global__ void __launch_bounds__(256) k_bm_word_ranks(const uint32_t *__restrict__ bits,
+                                                      const uint64_t *__restrict__ rank, uint64_t nblk,
+                                                      uint32_t *__restrict__ wrank) {
+  for (uint64_t b = (uint64_t)blockIdx.x * 256 + threadIdx.x; b < nblk; b += (uint64_t)gridDim.x * 256) {
+    const uint4 *q = (const uint4 *)(bits + b * BM_BLOCK_WORDS);
+    uint4 *o = (uint4 *)(wrank + b * BM_BLOCK_WORDS);
+    uint32_t r = (uint32_t)rank[b];
+#pragma unroll
+    for (int k = 0; k < BM_BLOCK_WORDS / 4; ++k) {
+      const uint4 w = q[k];
+      uint4 t;
+      t.x = r;
+      r += __popc(w.x);
+      t.y = r;
+      r += __popc(w.y);
+      t.z = r;
+      r += __popc(w.z);
+      t.w = r;
+      r += __popc(w.w);
+      o[k] = t;
+    }
+  }
+}
+
+// One block per mid tile, one wave per wave region (see k_star_mid); entry i moves to
+// position rank(key).
+__global__ void __launch_bounds__(ST_BLOCK) k_bm_scatter(const int64_t *__restrict__ rkey,
+                                                        const int64_t *__restrict__ rg1,
+                                                        const int64_t *__restrict__ rg2,
+                                                        const uint32_t *__restrict__ counts,
+                                                        const uint32_t *__restrict__ bits,
+                                                        const uint32_t *__restrict__ wrank, int64_t kmin,
+                                                        int64_t *__restrict__ skey,
+                                                        int64_t *__restrict__ sg1,
+                                                        int64_t *__restrict__ sg2) {
+  const uint32_t region = blockIdx.x * (ST_BLOCK / 64) + threadIdx.x / 64;
+  const uint32_t n = counts[region];
+  for (uint32_t i = __lane_id(); i < n; i += 64) {
+    const uint64_t src = (uint64_t)region * PL_WAVE_ROWS + i;
+    const uint64_t code = (uint64_t)rkey[src] - (uint64_t)kmin;
+    const uint64_t j = wrank[code >> 5] + (uint32_t)__popc(bits[code >> 5] & ((1u << (code & 31)) - 1u));
     skey[j] = rkey[src];
     sg1[j] = rg1[src];
     sg2[j] = rg2[src];
@@ -837,7 +862,7 @@ constexpr uint32_t FACT_STAGE = 1u << 1;
 __global__ void __launch_bounds__(ST_BLOCK, 8) k_star_fact_bm(ScanDesc d, const SegPlan *__restrict__ plan,
                                                            int fkcol, int64_t a0, int64_t a1, int64_t b0,
                                                            int64_t b1, const uint32_t *__restrict__ bits,
-                                                           const uint64_t *__restrict__ rank,
+                                                           const uint32_t *__restrict__ wrank,
                                                            int64_t mkmin, uint64_t mrange,
                                                            unsigned long long *__restrict__ sum,
                                                            unsigned int *__restrict__ cnt,
@@ -889,36 +914,40 @@ __global__ void __launch_bounds__(ST_BLOCK, 8) k_star_fact_bm(ScanDesc d, const 
         if (c < mrange) code[j] = (uint32_t)c;  // mrange <= 2^31 on this path
       }
     }
+    nprobe = wave_sum(nprobe);
   }
   // the bitmap words, loaded without branches (rows that do not probe read word 0)
   uint32_t word[ST_ITEMS];
 #pragma unroll
   for (int j = 0; j < ST_ITEMS; ++j) word[j] = bits[code[j] != 0xFFFFFFFFu ? code[j] >> 5 : 0u];
-  uint32_t hits = 0;
+  uint32_t hits = 0, nmatch = 0;
 #pragma unroll
-  for (int j = 0; j < ST_ITEMS; ++j)
-    hits |= (uint32_t)((code[j] != 0xFFFFFFFFu) & ((word[j] >> (code[j] & 31)) & 1)) << j;
-  // 2. one hit per lane per round (rarely more than one round): its group index (the key's
-  // rank), payload words and the two atomics, the loads of a round issued together
-  uint32_t nmatch = 0;
+  for (int j = 0; j < ST_ITEMS; ++j) {
+    const bool h = (code[j] != 0xFFFFFFFFu) & ((word[j] >> (code[j] & 31)) & 1);
+    hits |= (uint32_t)h << j;
+    nmatch += (uint32_t)__popcll(__builtin_amdgcn_ballot_w64(h));
+  }
+  // 2. one hit per lane per round (rarely more than one round): its group index (the rank
+  // of the key: the word's rank plus the set bits below the key's bit), payload words and
+  // the two atomics, the loads of a round issued together
   const ColReader rx = col_reader(P.col[2], plan), ry = col_reader(P.col[3], plan);
   while (hits) {
     const int jh = __builtin_ctz(hits);
     hits &= hits - 1;
-    uint32_t c = 0;
+    uint32_t c = 0, w = 0;
 #pragma unroll
-    for (int j = 0; j < ST_ITEMS; ++j) c = j == jh ? code[j] : c;
+    for (int j = 0; j < ST_ITEMS; ++j) {
+      c = j == jh ? code[j] : c;
+      w = j == jh ? word[j] : w;
+    }
     const uint32_t local = r0 + (uint32_t)jh * ST_BLOCK + threadIdx.x;
-    const uint32_t g = (uint32_t)bm_rank(bits, rank, c);
+    const uint32_t g = wrank[c >> 5] + (uint32_t)__popc(w & ((1u << (c & 31)) - 1u));
     const int64_t x = col_read(rx, local), y = col_read(ry, local);
     const int64_t v =
         wmul((int64_t)((uint64_t)a0 + (uint64_t)wmul(a1, x)), (int64_t)((uint64_t)b0 + (uint64_t)wmul(b1, y)));
     atomicAdd(&sum[g], (unsigned long long)v);
     atomicAdd(&cnt[g], 1u);
-    ++nmatch;
   }
-  nmatch = wave_sum(nmatch);
-  nprobe = wave_sum(nprobe);
   if (lane == 0) {
     matched[cidx] = nmatch;
     probed[cidx] = nprobe;
@@ -1647,6 +1676,7 @@ extern "C" int aln_star_agg(aln_ctx *c, const aln_star_args *a, aln_star_row *h_
   // bitmap join: the rank directory now (its total = distinct passing keys), before the
   // one synchronisation that reads the counters
   uint64_t *rank = nullptr;
+  uint32_t *wrank = nullptr;
   uint64_t distinct = 0;
   if (use_bm) {
     uint32_t *bcnt;
@@ -1660,6 +1690,13 @@ extern "C" int aln_star_agg(aln_ctx *c, const aln_star_args *a, aln_star_row *h_
     ALN_TRY(check_launch("bm_block_counts"));
     ALN_TRY(sum_u32(c, bcnt, mblk, ctr + 5));
     ALN_TRY(exclusive_scan_u32(c, bcnt, mblk, rank, nullptr));
+    ALN_TRY(ws(c, mblk * BM_BLOCK_WORDS, &wrank));
+    {
+      ProfScope p(c, "star_build");
+      hipLaunchKernelGGL(k_bm_word_ranks, dim3(grid_for(mblk, 256, 65536)), dim3(256), 0, c->stream, mbits, rank,
+                         mblk, wrank);
+    }
+    ALN_TRY(check_launch("bm_word_ranks"));
   }
   uint64_t hc[8];
   ALN_TRY(copy_to_host(c, hc, ctr, sizeof(hc)));
@@ -1689,8 +1726,8 @@ extern "C" int aln_star_agg(aln_ctx *c, const aln_star_args *a, aln_star_row *h_
     // the group attributes in rank (= key) order
     if (m) {
       ProfScope p(c, "star_build");
-      hipLaunchKernelGGL(k_bm_scatter, dim3((unsigned)mregions), dim3(256), 0, c->stream, rkey, rg1, rg2,
-                         mcnt, mbits, rank, mkmin, okey, og1, og2);
+      hipLaunchKernelGGL(k_bm_scatter, dim3((unsigned)mblocks), dim3(ST_BLOCK), 0, c->stream, rkey, rg1, rg2,
+                         mcnt, mbits, wrank, mkmin, okey, og1, og2);
     }
     ALN_TRY(check_launch("bm_scatter"));
   } else {
@@ -1702,7 +1739,7 @@ extern "C" int aln_star_agg(aln_ctx *c, const aln_star_args *a, aln_star_row *h_
     ALN_TRY(fill_u64(c, table, tcap, ST_EMPTY));
     if (m) {
       ProfScope p(c, "star_build");
-      hipLaunchKernelGGL(k_star_build, dim3((unsigned)mregions), dim3(256), 0, c->stream, rkey, rg1, rg2,
+      hipLaunchKernelGGL(k_star_build, dim3((unsigned)mblocks), dim3(ST_BLOCK), 0, c->stream, rkey, rg1, rg2,
                          mcnt, moff, mkmin, table, tcap, okey, og1, og2, dup + 1);
     }
     ALN_TRY(check_launch("star_build"));
@@ -1712,7 +1749,7 @@ extern "C" int aln_star_agg(aln_ctx *c, const aln_star_args *a, aln_star_row *h_
     ProfScope p(c, "star_fact");
     if (use_bm)
       hipLaunchKernelGGL(k_star_fact_bm, dim3((unsigned)fblocks), dim3(ST_BLOCK), 0, c->stream, fd, fplan, a->f_fk,
-                         a->a0, a->a1, a->b0, a->b1, mbits, rank, mkmin, mrange, sum, cnt, fmatch, fprobe);
+                         a->a0, a->a1, a->b0, a->b1, mbits, wrank, mkmin, mrange, sum, cnt, fmatch, fprobe);
     else  // per-block counts
       hipLaunchKernelGGL(k_star_fact, dim3((unsigned)fblocks), dim3(ST_BLOCK), 0, c->stream, fd, a->f_fk,
                          a->f_x, a->f_y, a->a0, a->a1, a->b0, a->b1, table, tcap, mkmin, mrange, sum, cnt,
