@@ -533,19 +533,17 @@ __global__ void __launch_bounds__(ST_BLOCK) k_star_dim(ScanDesc d, int kcol, int
 }
 
 // ---- M: mid scan, probe the dimension bitmap, compact passing rows ------------------------
-// Plan roles: 0 range, 1 foreign key (probes the dimension bitmap), 2 own key, 3 and 4 the
-// group columns; all but the range are staged when not constant.  The passing rows of wave w
-// of tile b go to region b * 8 + w (PL_WAVE_ROWS entries), counted in counts[b * 8 + w].
-constexpr uint32_t MID_STAGE = (1u << 1) | (1u << 2) | (1u << 3) | (1u << 4);
+// Plan roles: 0 range, 1 foreign key (probes the dimension bitmap, staged), 2 own key
+// (staged), 3 and 4 the group columns (decoded later, for the groups that need them, from
+// the row).  Each passing row leaves {key - mkmin, row}: wave w of tile b writes region
+// b * 8 + w (PL_WAVE_ROWS entries) and its count to counts[b * 8 + w].
+constexpr uint32_t MID_STAGE = (1u << 1) | (1u << 2);
 
 __global__ void __launch_bounds__(ST_BLOCK, 8) k_star_mid(ScanDesc d, const SegPlan *__restrict__ plan, int kcol,
-                                                       int fkcol, int g1col, int g2col,
-                                                       const uint32_t *__restrict__ bitmap, int64_t dkmin,
-                                                       uint64_t drange, uint32_t *__restrict__ counts,
-                                                       uint32_t *__restrict__ probed,
-                                                       int64_t *__restrict__ okey,
-                                                       int64_t *__restrict__ og1,
-                                                       int64_t *__restrict__ og2,
+                                                       int fkcol, const uint32_t *__restrict__ bitmap,
+                                                       int64_t dkmin, uint64_t drange,
+                                                       uint32_t *__restrict__ counts,
+                                                       uint32_t *__restrict__ probed, uint2 *__restrict__ rent,
                                                        uint32_t *__restrict__ mbits, int64_t mkmin) {
   static_assert(sizeof(TileStage) <= PL_STAGE_WORDS * sizeof(uint32_t), "generic staging fits the LDS area");
   __shared__ __align__(16) uint32_t lds[PL_STAGE_WORDS];
@@ -603,9 +601,9 @@ __global__ void __launch_bounds__(ST_BLOCK, 8) k_star_mid(ScanDesc d, const SegP
 #pragma unroll
   for (int j = 0; j < ST_ITEMS; ++j)
     passmask |= (uint32_t)(dc[j] != ~0ull && ((dw[j] >> (dc[j] & 31)) & 1)) << j;
-  // the passing rows' key and group values: staged (fast) or decoded (generic)
-  const PlanSlot ks = plan_slot(lds, P.col[2], r0), s1 = plan_slot(lds, P.col[3], r0),
-                 s2 = plan_slot(lds, P.col[4], r0);
+  // the passing rows' keys: staged (fast) or decoded (generic)
+  const PlanSlot ks = plan_slot(lds, P.col[2], r0);
+  const uint32_t row0 = seg * d.seg_rows + r0;
   // passing rows of the wave in item-major order (consecutive lanes write consecutive
   // entries); no barrier: each wave owns its region
   const uint64_t rbase = (uint64_t)region * PL_WAVE_ROWS;
@@ -616,26 +614,12 @@ __global__ void __launch_bounds__(ST_BLOCK, 8) k_star_mid(ScanDesc d, const SegP
     const uint64_t b = __ballot(h);
     if (h) {
       const uint32_t i = j * ST_BLOCK + threadIdx.x;
-      int64_t key, v1 = 0, v2 = 0;
-      if (fast) {
-        key = (int64_t)((uint64_t)P.col[2].base + slot_code(ks, i));
-        if (g1col >= 0)
-          v1 = P.col[3].slot >= 0 ? (int64_t)((uint64_t)P.col[3].base + slot_code(s1, i)) : plan_get(P.col[3], r0 + i);
-        if (g2col >= 0)
-          v2 = P.col[4].slot >= 0 ? (int64_t)((uint64_t)P.col[4].base + slot_code(s2, i)) : plan_get(P.col[4], r0 + i);
-      } else {
-        key = tile_get(ts, d.words, d.segs, kcol, seg, r0, i);
-        if (g1col >= 0) v1 = tile_get(ts, d.words, d.segs, g1col, seg, r0, i);
-        if (g2col >= 0) v2 = tile_get(ts, d.words, d.segs, g2col, seg, r0, i);
-      }
-      if (mbits) {  // no returned value: a duplicate key shows as popcount(mbits) < passing rows
-        const uint64_t code = (uint64_t)key - (uint64_t)mkmin;
+      const int64_t key = fast ? (int64_t)((uint64_t)P.col[2].base + slot_code(ks, i))
+                               : tile_get(ts, d.words, d.segs, kcol, seg, r0, i);
+      const uint32_t code = (uint32_t)((uint64_t)key - (uint64_t)mkmin);  // key range < 2^32
+      if (mbits)  // no returned value: a duplicate key shows as popcount(mbits) < passing rows
         atomicOr(&mbits[code >> 5], 1u << (code & 31));
-      }
-      const uint64_t pos = rbase + nout + (uint32_t)__popcll(b & lanemask_lt());
-      okey[pos] = key;
-      og1[pos] = v1;
-      og2[pos] = v2;
+      rent[rbase + nout + (uint32_t)__popcll(b & lanemask_lt())] = make_uint2(code, row0 + i);
     }
     nout += (uint32_t)__popcll(b);
   }
@@ -651,24 +635,18 @@ __device__ __forceinline__ uint64_t st_slot(uint32_t code, uint64_t cap) {
 
 // ---- B: build the join table over the compacted mid rows ----------------------------------
 // One block per mid tile, one wave per wave region: entry i of region b becomes group
-// offs[b] + i; its key and group columns move to dense arrays and the key goes into the table.
-__global__ void __launch_bounds__(ST_BLOCK) k_star_build(const int64_t *__restrict__ rkey,
-                                                    const int64_t *__restrict__ rg1,
-                                                    const int64_t *__restrict__ rg2,
+// offs[b] + i; its mid row goes to grow[group] and its key into the table.
+__global__ void __launch_bounds__(ST_BLOCK) k_star_build(const uint2 *__restrict__ rent,
                                                     const uint32_t *__restrict__ counts,
-                                                    const uint64_t *__restrict__ offs,
-                                                    int64_t kmin, uint64_t *table, uint64_t cap,
-                                                    int64_t *__restrict__ okey,
-                                                    int64_t *__restrict__ og1,
-                                                    int64_t *__restrict__ og2, uint32_t *dup) {
+                                                    const uint64_t *__restrict__ offs, uint64_t *table,
+                                                    uint64_t cap, uint32_t *__restrict__ grow, uint32_t *dup) {
   const uint32_t region = blockIdx.x * (ST_BLOCK / 64) + threadIdx.x / 64;
   const uint32_t n = counts[region];
   for (uint32_t i = __lane_id(); i < n; i += 64) {
-    const uint64_t src = (uint64_t)region * PL_WAVE_ROWS + i, j = offs[region] + i;
-    okey[j] = rkey[src];
-    og1[j] = rg1[src];
-    og2[j] = rg2[src];
-    uint32_t code = (uint32_t)((uint64_t)rkey[src] - (uint64_t)kmin);
+    const uint64_t j = offs[region] + i;
+    const uint2 e = rent[(uint64_t)region * PL_WAVE_ROWS + i];
+    grow[j] = e.y;
+    const uint32_t code = e.x;
     uint64_t entry = ((uint64_t)j << 32) | code;
     uint64_t h = st_slot(code, cap);
     for (uint64_t probes = 0; probes < cap; ++probes) {
@@ -832,24 +810,17 @@ __global__ void __launch_bounds__(256) k_bm_word_ranks(const uint32_t *__restric
 
 // One block per mid tile, one wave per wave region (see k_star_mid); entry i moves to
 // position rank(key).
-__global__ void __launch_bounds__(ST_BLOCK) k_bm_scatter(const int64_t *__restrict__ rkey,
-                                                        const int64_t *__restrict__ rg1,
-                                                        const int64_t *__restrict__ rg2,
+__global__ void __launch_bounds__(ST_BLOCK) k_bm_scatter(const uint2 *__restrict__ rent,
                                                         const uint32_t *__restrict__ counts,
                                                         const uint32_t *__restrict__ bits,
-                                                        const uint32_t *__restrict__ wrank, int64_t kmin,
-                                                        int64_t *__restrict__ skey,
-                                                        int64_t *__restrict__ sg1,
-                                                        int64_t *__restrict__ sg2) {
+                                                        const uint32_t *__restrict__ wrank,
+                                                        uint32_t *__restrict__ grow) {
   const uint32_t region = blockIdx.x * (ST_BLOCK / 64) + threadIdx.x / 64;
   const uint32_t n = counts[region];
   for (uint32_t i = __lane_id(); i < n; i += 64) {
-    const uint64_t src = (uint64_t)region * PL_WAVE_ROWS + i;
-    const uint64_t code = (uint64_t)rkey[src] - (uint64_t)kmin;
-    const uint64_t j = wrank[code >> 5] + (uint32_t)__popc(bits[code >> 5] & ((1u << (code & 31)) - 1u));
-    skey[j] = rkey[src];
-    sg1[j] = rg1[src];
-    sg2[j] = rg2[src];
+    const uint2 e = rent[(uint64_t)region * PL_WAVE_ROWS + i];
+    const uint32_t code = e.x;
+    grow[wrank[code >> 5] + (uint32_t)__popc(bits[code >> 5] & ((1u << (code & 31)) - 1u))] = e.y;
   }
 }
 
@@ -955,6 +926,24 @@ __global__ void __launch_bounds__(ST_BLOCK, 8) k_star_fact_bm(ScanDesc d, const 
 }
 
 // ---- T: top-k over groups (sum DESC, g1 ASC, key ASC, g2 ASC) -----------------------------
+// A group's (key, g1, g2): arrays indexed by group, or (row != nullptr) decoded from the
+// group's mid row row[j] through the mid scan's plan (roles 2, 3, 4; a role without a column
+// reads 0), for the few groups that need them.
+struct GroupAttr {
+  const int64_t *key, *g1, *g2;
+  const uint32_t *row;
+  const SegPlan *plan;
+  uint32_t seg_rows;
+};
+__device__ __forceinline__ int64_t ga_get(const GroupAttr &a, uint64_t j, int which) {
+  if (!a.row) return which == 0 ? a.key[j] : (which == 1 ? a.g1[j] : a.g2[j]);
+  const uint32_t r = a.row[j], seg = r / a.seg_rows;
+  return plan_get(a.plan[seg].col[2 + which], r - seg * a.seg_rows);
+}
+__device__ __forceinline__ int64_t ga_key(const GroupAttr &a, uint64_t j) { return ga_get(a, j, 0); }
+__device__ __forceinline__ int64_t ga_g1(const GroupAttr &a, uint64_t j) { return ga_get(a, j, 1); }
+__device__ __forceinline__ int64_t ga_g2(const GroupAttr &a, uint64_t j) { return ga_get(a, j, 2); }
+
 struct Cand {
   int64_t sum, g1, key, g2;
   uint32_t j, valid;
@@ -975,10 +964,7 @@ constexpr int TK_TILE2 = 1024;
 __global__ void __launch_bounds__(256) k_star_topk_tile(const Cand *__restrict__ in, uint64_t n,
                                                         const unsigned long long *__restrict__ sum,
                                                         const unsigned int *__restrict__ cnt,
-                                                        const int64_t *__restrict__ okey,
-                                                        const int64_t *__restrict__ og1,
-                                                        const int64_t *__restrict__ og2,
-                                                        uint32_t k, Cand *__restrict__ out) {
+                                                        GroupAttr ga, uint32_t k, Cand *__restrict__ out) {
   __shared__ Cand s[TK_TILE2];
   uint64_t base = (uint64_t)blockIdx.x * TK_TILE2;
   for (int t = threadIdx.x; t < TK_TILE2; t += 256) {
@@ -989,9 +975,9 @@ __global__ void __launch_bounds__(256) k_star_topk_tile(const Cand *__restrict__
       else c.valid = 0;
     } else if (i < n && cnt[i] > 0) {
       c.sum = (int64_t)sum[i];
-      c.g1 = og1[i];
-      c.key = okey[i];
-      c.g2 = og2[i];
+      c.g1 = ga_g1(ga, i);
+      c.key = ga_key(ga, i);
+      c.g2 = ga_g2(ga, i);
       c.j = (uint32_t)i;
       c.valid = 1;
       c.count = cnt[i];
@@ -1031,27 +1017,26 @@ __global__ void k_star_group_flags(const unsigned int *cnt, uint64_t n, uint32_t
 }
 
 __global__ void k_star_group_codes(const uint32_t *flag, const uint64_t *pos, uint64_t n,
-                                   const int64_t *okey, int64_t kmin, uint64_t *code,
+                                   GroupAttr ga, int64_t kmin, uint64_t *code,
                                    uint32_t *idx) {
   for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * 256)
     if (flag[i]) {
-      code[pos[i]] = (uint64_t)okey[i] - (uint64_t)kmin;
+      code[pos[i]] = (uint64_t)ga_key(ga, i) - (uint64_t)kmin;
       idx[pos[i]] = (uint32_t)i;
     }
 }
 
-__global__ void k_star_group_emit(const uint32_t *idx, uint64_t g, const int64_t *okey,
-                                  const int64_t *og1, const int64_t *og2,
+__global__ void k_star_group_emit(const uint32_t *idx, uint64_t g, GroupAttr ga,
                                   const unsigned long long *sum, const unsigned int *cnt,
                                   aln_star_row *out) {
   for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < g;
        i += (uint64_t)gridDim.x * 256) {
     uint32_t j = idx[i];
     aln_star_row r;
-    r.key = okey[j];
-    r.g1 = og1[j];
-    r.g2 = og2[j];
+    r.key = ga_key(ga, j);
+    r.g1 = ga_g1(ga, j);
+    r.g2 = ga_g2(ga, j);
     r.sum = (int64_t)sum[j];
     r.count = cnt[j];
     out[i] = r;
@@ -1193,10 +1178,9 @@ __device__ __forceinline__ void ts_insert(TopList &t, uint64_t h, uint64_t l, ui
 // triples of a previous pass.  Writes the block's best k triples (hi = ~0: none).
 __global__ void __launch_bounds__(TS_BLOCK) k_topk_sel(uint64_t n, const uint32_t *__restrict__ d_n,
                                                        const unsigned long long *__restrict__ sum,
-                                                       const unsigned int *__restrict__ cnt,
-                                                       const int64_t *__restrict__ okey,
-                                                       const int64_t *__restrict__ og1, int64_t g1min,
-                                                       int64_t kmin, const uint64_t *__restrict__ ihi,
+                                                       const unsigned int *__restrict__ cnt, GroupAttr ga,
+                                                       int64_t g1min, int64_t kmin,
+                                                       const uint64_t *__restrict__ ihi,
                                                        const uint64_t *__restrict__ ilo,
                                                        const uint32_t *__restrict__ ij, uint32_t k,
                                                        uint64_t *__restrict__ ohi, uint64_t *__restrict__ olo,
@@ -1228,7 +1212,8 @@ __global__ void __launch_bounds__(TS_BLOCK) k_topk_sel(uint64_t n, const uint32_
         const uint64_t i = i0 + u * stride;
         v[u] = i < n && cnt[i] > 0;
         h[u] = v[u] ? ~((uint64_t)sum[i] ^ 0x8000000000000000ull) : ~0ull;
-        l[u] = v[u] ? ((uint64_t)(og1[i] - g1min) << 32) | (uint32_t)((uint64_t)okey[i] - (uint64_t)kmin) : ~0ull;
+        l[u] = v[u] ? ((uint64_t)(ga_g1(ga, i) - g1min) << 32) | (uint32_t)((uint64_t)ga_key(ga, i) - (uint64_t)kmin)
+                    : ~0ull;
       }
 #pragma unroll
       for (int u = 0; u < U; ++u)
@@ -1370,10 +1355,9 @@ __global__ void __launch_bounds__(TB_BLOCK) k_topk_kth(const uint64_t *__restric
 
 // The groups with sum >= threshold, as (hi, lo, j) candidates; *count may exceed cap.
 __global__ void __launch_bounds__(TB_BLOCK) k_topk_collect(uint64_t n, const unsigned long long *__restrict__ sum,
-                                                           const unsigned int *__restrict__ cnt,
-                                                           const int64_t *__restrict__ okey,
-                                                           const int64_t *__restrict__ og1, int64_t g1min,
-                                                           int64_t kmin, const uint64_t *__restrict__ thr,
+                                                           const unsigned int *__restrict__ cnt, GroupAttr ga,
+                                                           int64_t g1min, int64_t kmin,
+                                                           const uint64_t *__restrict__ thr,
                                                            uint32_t cap, uint32_t *__restrict__ count,
                                                            uint64_t *__restrict__ chi, uint64_t *__restrict__ clo,
                                                            uint32_t *__restrict__ cj) {
@@ -1388,7 +1372,7 @@ __global__ void __launch_bounds__(TB_BLOCK) k_topk_collect(uint64_t n, const uns
         const uint32_t pos = atomicAdd(count, 1u);
         if (pos < cap) {
           chi[pos] = ~o;
-          clo[pos] = ((uint64_t)(og1[i] - g1min) << 32) | (uint32_t)((uint64_t)okey[i] - (uint64_t)kmin);
+          clo[pos] = ((uint64_t)(ga_g1(ga, i) - g1min) << 32) | (uint32_t)((uint64_t)ga_key(ga, i) - (uint64_t)kmin);
           cj[pos] = (uint32_t)i;
         }
       }
@@ -1398,17 +1382,15 @@ __global__ void __launch_bounds__(TB_BLOCK) k_topk_collect(uint64_t n, const uns
 
 __global__ void k_topk_emit(const uint32_t *__restrict__ j, uint32_t k,
                             const unsigned long long *__restrict__ sum, const unsigned int *__restrict__ cnt,
-                            const int64_t *__restrict__ okey, const int64_t *__restrict__ og1,
-                            const int64_t *__restrict__ og2, aln_star_row *__restrict__ out,
-                            uint32_t *__restrict__ nvalid) {
+                            GroupAttr ga, aln_star_row *__restrict__ out, uint32_t *__restrict__ nvalid) {
   const uint32_t i = threadIdx.x;
   const bool ok = i < k && j[i] != 0xFFFFFFFFu;  // empty entries carry no group
   if (ok) {
     const uint32_t g = j[i];
     aln_star_row r;
-    r.key = okey[g];
-    r.g1 = og1[g];
-    r.g2 = og2[g];
+    r.key = ga_key(ga, g);
+    r.g1 = ga_g1(ga, g);
+    r.g2 = ga_g2(ga, g);
     r.sum = (int64_t)sum[g];
     r.count = cnt[g];
     out[i] = r;
@@ -1421,8 +1403,8 @@ __global__ void k_topk_emit(const uint32_t *__restrict__ j, uint32_t k,
 // g1 range below 2^32 take the register selection above; otherwise tiles of 1024
 // candidates are sorted and cut to k, repeatedly, until one tile remains.
 static int star_topk(aln_ctx *c, uint64_t m, const unsigned long long *sum, const unsigned int *cnt,
-                     const int64_t *okey, const int64_t *og1, const int64_t *og2, uint32_t k,
-                     int64_t g1min, int64_t g1max, int64_t kmin, aln_star_row *h_top, uint32_t *h_ntop) {
+                     const GroupAttr &ga, uint32_t k, int64_t g1min, int64_t g1max, int64_t kmin,
+                     aln_star_row *h_top, uint32_t *h_ntop) {
   if (k <= (uint32_t)TS_K && g1max >= g1min && (uint64_t)g1max - (uint64_t)g1min < (1ull << 32) &&
       m < (1ull << 32)) {
     // threshold path: block maxima -> k-th largest -> the groups at or above it
@@ -1450,11 +1432,11 @@ static int star_topk(aln_ctx *c, uint64_t m, const unsigned long long *sum, cons
           ProfScope p(c, "star_topk");
           hipLaunchKernelGGL(k_topk_bmax, dim3((unsigned)nb), dim3(TB_BLOCK), 0, c->stream, m, sum, cnt, bmax);
           hipLaunchKernelGGL(k_topk_kth, dim3(1), dim3(TB_BLOCK), 0, c->stream, bmax, (uint32_t)nb, k, thr);
-          hipLaunchKernelGGL(k_topk_collect, dim3((unsigned)nb), dim3(TB_BLOCK), 0, c->stream, m, sum, cnt, okey, og1,
-                             g1min, kmin, thr, CAP, count, chi, clo, cj);
-          hipLaunchKernelGGL(k_topk_sel, dim3(1), dim3(TS_BLOCK), 0, c->stream, (uint64_t)CAP, count, sum, cnt, okey,
-                             og1, g1min, kmin, chi, clo, cj, k, oh, ol, oj);
-          hipLaunchKernelGGL(k_topk_emit, dim3(1), dim3(64), 0, c->stream, oj, k, sum, cnt, okey, og1, og2, rows, nv);
+          hipLaunchKernelGGL(k_topk_collect, dim3((unsigned)nb), dim3(TB_BLOCK), 0, c->stream, m, sum, cnt, ga, g1min,
+                             kmin, thr, CAP, count, chi, clo, cj);
+          hipLaunchKernelGGL(k_topk_sel, dim3(1), dim3(TS_BLOCK), 0, c->stream, (uint64_t)CAP, count, sum, cnt, ga,
+                             g1min, kmin, chi, clo, cj, k, oh, ol, oj);
+          hipLaunchKernelGGL(k_topk_emit, dim3(1), dim3(64), 0, c->stream, oj, k, sum, cnt, ga, rows, nv);
         }
         ALN_TRY(check_launch("star_topk"));
         std::vector<uint32_t> hres(2 + k * sizeof(aln_star_row) / 4);
@@ -1485,7 +1467,7 @@ static int star_topk(aln_ctx *c, uint64_t m, const unsigned long long *sum, cons
         ALN_TRY(ws(c, (size_t)nb * k, &ol));
         ALN_TRY(ws(c, (size_t)nb * k, &oj));
         hipLaunchKernelGGL(k_topk_sel, dim3(nb), dim3(TS_BLOCK), 0, c->stream, n, (const uint32_t *)nullptr, sum, cnt,
-                           okey, og1, g1min, kmin, ih, il, ij, k, oh, ol, oj);
+                           ga, g1min, kmin, ih, il, ij, k, oh, ol, oj);
         ih = oh;
         il = ol;
         ij = oj;
@@ -1493,7 +1475,7 @@ static int star_topk(aln_ctx *c, uint64_t m, const unsigned long long *sum, cons
         if (nb == 1) break;
       }
       j2 = const_cast<uint32_t *>(ij);
-      hipLaunchKernelGGL(k_topk_emit, dim3(1), dim3(64), 0, c->stream, j2, k, sum, cnt, okey, og1, og2, rows, nv);
+      hipLaunchKernelGGL(k_topk_emit, dim3(1), dim3(64), 0, c->stream, j2, k, sum, cnt, ga, rows, nv);
     }
     ALN_TRY(check_launch("star_topk"));
     uint32_t nvalid = 0;
@@ -1510,8 +1492,8 @@ static int star_topk(aln_ctx *c, uint64_t m, const unsigned long long *sum, cons
     ALN_TRY(ws(c, nt * k, &out));
     {
       ProfScope p(c, "star_topk");
-      hipLaunchKernelGGL(k_star_topk_tile, dim3((unsigned)nt), dim3(256), 0, c->stream, in, n, sum, cnt, okey,
-                         og1, og2, k, out);
+      hipLaunchKernelGGL(k_star_topk_tile, dim3((unsigned)nt), dim3(256), 0, c->stream, in, n, sum, cnt, ga, k,
+                         out);
     }
     ALN_TRY(check_launch("star_topk"));
     in = out;
@@ -1580,6 +1562,8 @@ extern "C" int aln_star_agg(aln_ctx *c, const aln_star_args *a, aln_star_row *h_
   ALN_TRY(make_scan(c, a->dim, a->d_key, &dd, &dblocks));
   ALN_TRY(make_scan(c, a->mid, a->m_key > a->m_fk ? a->m_key : a->m_fk, &md, &mblocks));
   ALN_REQUIRE(a->m_g1 < a->mid.ncols && a->m_g2 < a->mid.ncols, "aln_star_agg: group columns");
+  ALN_REQUIRE((uint64_t)a->mid.cols[0].nsegs * a->mid.seg_rows <= 0xFFFFFFFFull,
+              "aln_star_agg: mid table above 2^32 rows");
   ALN_TRY(make_scan(c, a->fact, a->f_fk, &fd, &fblocks));
   ALN_REQUIRE(a->f_x >= 0 && a->f_x < a->fact.ncols && a->f_y >= 0 && a->f_y < a->fact.ncols,
               "aln_star_agg: aggregate columns");
@@ -1659,15 +1643,12 @@ extern "C" int aln_star_agg(aln_ctx *c, const aln_star_args *a, aln_star_row *h_
 
   // mid scan: passing rows of wave region b land in entries [b * PL_WAVE_ROWS, ...) of the
   // region arrays
-  int64_t *rkey, *rg1, *rg2;
-  ALN_TRY(ws(c, mregions * PL_WAVE_ROWS, &rkey));
-  ALN_TRY(ws(c, mregions * PL_WAVE_ROWS, &rg1));
-  ALN_TRY(ws(c, mregions * PL_WAVE_ROWS, &rg2));
+  uint2 *rent;
+  ALN_TRY(ws(c, mregions * PL_WAVE_ROWS, &rent));
   {
     ProfScope p(c, "star_mid");
     hipLaunchKernelGGL(k_star_mid, dim3((unsigned)mblocks), dim3(ST_BLOCK), 0, c->stream, md, mplan, a->m_key,
-                       a->m_fk, a->m_g1, a->m_g2, bitmap, dkmin, drange, mcnt, mprobe, rkey, rg1, rg2,
-                       mbits, mkmin);
+                       a->m_fk, bitmap, dkmin, drange, mcnt, mprobe, rent, mbits, mkmin);
   }
   ALN_TRY(check_launch("star_mid"));
   ALN_TRY(sum_u32(c, dcnt, dblocks, ctr + 0));
@@ -1711,13 +1692,11 @@ extern "C" int aln_star_agg(aln_ctx *c, const aln_star_args *a, aln_star_row *h_
   st.mid_probe = hc[2];
 
   unsigned long long *sum;
-  unsigned int *cnt;
-  int64_t *okey, *og1, *og2;
+  unsigned int *cnt, *grow;
   ALN_TRY(ws(c, m ? m : 1, &sum));
   ALN_TRY(ws(c, m ? m : 1, &cnt));
-  ALN_TRY(ws(c, m ? m : 1, &okey));
-  ALN_TRY(ws(c, m ? m : 1, &og1));
-  ALN_TRY(ws(c, m ? m : 1, &og2));
+  ALN_TRY(ws(c, m ? m : 1, &grow));
+  const GroupAttr ga{nullptr, nullptr, nullptr, grow, mplan, a->mid.seg_rows};
   ALN_TRY(fill_u64(c, (uint64_t *)sum, m, 0));
   ALN_TRY(fill_u32(c, cnt, m, 0));
   uint64_t *table = nullptr;
@@ -1726,8 +1705,8 @@ extern "C" int aln_star_agg(aln_ctx *c, const aln_star_args *a, aln_star_row *h_
     // the group attributes in rank (= key) order
     if (m) {
       ProfScope p(c, "star_build");
-      hipLaunchKernelGGL(k_bm_scatter, dim3((unsigned)mblocks), dim3(ST_BLOCK), 0, c->stream, rkey, rg1, rg2,
-                         mcnt, mbits, wrank, mkmin, okey, og1, og2);
+      hipLaunchKernelGGL(k_bm_scatter, dim3((unsigned)mblocks), dim3(ST_BLOCK), 0, c->stream, rent, mcnt, mbits,
+                         wrank, grow);
     }
     ALN_TRY(check_launch("bm_scatter"));
   } else {
@@ -1739,8 +1718,8 @@ extern "C" int aln_star_agg(aln_ctx *c, const aln_star_args *a, aln_star_row *h_
     ALN_TRY(fill_u64(c, table, tcap, ST_EMPTY));
     if (m) {
       ProfScope p(c, "star_build");
-      hipLaunchKernelGGL(k_star_build, dim3((unsigned)mblocks), dim3(ST_BLOCK), 0, c->stream, rkey, rg1, rg2,
-                         mcnt, moff, mkmin, table, tcap, okey, og1, og2, dup + 1);
+      hipLaunchKernelGGL(k_star_build, dim3((unsigned)mblocks), dim3(ST_BLOCK), 0, c->stream, rent, mcnt, moff,
+                         table, tcap, grow, dup + 1);
     }
     ALN_TRY(check_launch("star_build"));
   }
@@ -1762,7 +1741,7 @@ extern "C" int aln_star_agg(aln_ctx *c, const aln_star_args *a, aln_star_row *h_
   tm.mark(3);
 
   // top-k
-  if (a->limit && m) ALN_TRY(star_topk(c, m, sum, cnt, okey, og1, og2, a->limit, g1min, g1max, mkmin, h_top, h_ntop));
+  if (a->limit && m) ALN_TRY(star_topk(c, m, sum, cnt, ga, a->limit, g1min, g1max, mkmin, h_top, h_ntop));
   tm.mark(4);
   ALN_TRY(copy_to_host(c, hc, ctr, sizeof(hc)));
   if (hdup[1]) return set_error(ALN_E_UNSUPPORTED, "aln_star_agg: duplicate mid keys");
@@ -1791,11 +1770,11 @@ extern "C" int aln_star_agg(aln_ctx *c, const aln_star_args *a, aln_star_row *h_
       ALN_TRY(ws(c, g, &code));
       ALN_TRY(ws(c, g, &idx));
       hipLaunchKernelGGL(k_star_group_codes, dim3(grid_for(m, 256, 65536)), dim3(256), 0, c->stream, flag,
-                         pos, m, okey, mkmin, code, idx);
+                         pos, m, ga, mkmin, code, idx);
       ALN_TRY(check_launch("star_group_codes"));
       if (!use_bm) ALN_TRY(radix_sort_pairs(c, code, idx, g, bits_for_range_u(mrange - 1)));
-      hipLaunchKernelGGL(k_star_group_emit, dim3(grid_for(g, 256, 65536)), dim3(256), 0, c->stream, idx, g,
-                         okey, og1, og2, sum, cnt, d_groups);
+      hipLaunchKernelGGL(k_star_group_emit, dim3(grid_for(g, 256, 65536)), dim3(256), 0, c->stream, idx, g, ga,
+                         sum, cnt, d_groups);
       ALN_TRY(check_launch("star_group_emit"));
       ALN_HIP(hipStreamSynchronize(c->stream));
     }
@@ -2385,8 +2364,9 @@ extern "C" int aln_star_agg_batch(aln_ctx *c, const aln_star_args *a, int nq, al
   }
   for (int i = 0; i < nq; ++i)
     if (a[i].limit && st[i].mid_pass)
-      ALN_TRY(star_topk(c, st[i].mid_pass, hq[i].sum, hq[i].cnt, hq[i].okey, hq[i].og1, hq[i].og2, a[i].limit,
-                        g1min, g1max, mkmin, h_top[i], &h_ntop[i]));
+      ALN_TRY(star_topk(c, st[i].mid_pass, hq[i].sum, hq[i].cnt,
+                        GroupAttr{hq[i].okey, hq[i].og1, hq[i].og2, nullptr, nullptr, 0}, a[i].limit, g1min, g1max,
+                        mkmin, h_top[i], &h_ntop[i]));
   ALN_TRY(copy_to_host(c, hc.data(), ctr, 8 * 5 * SB_MAX));
   for (int i = 0; i < nq; ++i) {
     st[i].fact_probe = hc[5 * i + 3];
