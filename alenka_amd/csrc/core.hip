@@ -284,6 +284,38 @@ int sum_u32(aln_ctx *c, const uint32_t *d_in, uint64_t n, uint64_t *d_out) {
   return check_launch("sum_u32");
 }
 
+// Several sums in one launch: block (x, y) adds a strided share of array y and adds its
+// total to *out[y] (at most 512 atomics per output; the caller zeroes the outputs).
+struct SumSet {
+  const uint32_t *in[SUM_MULTI_MAX];
+  uint64_t n[SUM_MULTI_MAX];
+  uint64_t *out[SUM_MULTI_MAX];
+};
+__global__ void __launch_bounds__(256) k_sum_multi(SumSet s) {
+  const uint32_t *in = s.in[blockIdx.y];
+  const uint64_t n = s.n[blockIdx.y];
+  uint64_t v = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) v += in[i];
+  uint64_t tot;
+  (void)block_excl_scan<uint64_t>(v, &tot);
+  if (threadIdx.x == 0 && tot) atomicAdd((unsigned long long *)s.out[blockIdx.y], (unsigned long long)tot);
+}
+
+int sum_u32_multi(aln_ctx *c, int k, const uint32_t *const *in, const uint64_t *n, uint64_t *const *out) {
+  ALN_REQUIRE(k >= 1 && k <= SUM_MULTI_MAX, "sum_u32_multi: %d arrays", k);
+  SumSet s;
+  uint64_t most = 0;
+  for (int i = 0; i < k; ++i) {
+    s.in[i] = in[i];
+    s.n[i] = n[i];
+    s.out[i] = out[i];
+    most = n[i] > most ? n[i] : most;
+  }
+  const unsigned nb = grid_for(most ? most : 1, 256 * 8, 512);
+  hipLaunchKernelGGL(k_sum_multi, dim3(nb, k), dim3(256), 0, c->stream, s);
+  return check_launch("sum_u32_multi");
+}
+
 // Min/max of an int64 array: per-block partials, then one block folds them (no atomics on
 // a shared address).
 __global__ void __launch_bounds__(256) k_minmax_part(const int64_t *__restrict__ d, uint64_t n,

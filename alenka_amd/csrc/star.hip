@@ -749,19 +749,44 @@ __global__ void __launch_bounds__(ST_BLOCK) k_star_fact(ScanDesc d, int fkcol, i
 // are clustered on the key touch a few contiguous lines instead of random slots.
 constexpr int BM_BLOCK_WORDS = 16;  // 512 bits
 
+// Set bits per 512-bit block (cnt) and, when rel != nullptr, a rank9 sub-directory (Vigna,
+// "Broadword implementation of rank/select queries", 2008): for the block's 64-bit words
+// i = 1..7, the set bits of words 0..i-1 as a 9-bit field at bit 9 (i - 1) of rel[b].  With
+// the exclusive scan of cnt, rank(code) = scan[b] + field(word) + popcount of the bits below
+// code in its 64-bit word: three 8-byte reads.
 __global__ void __launch_bounds__(256) k_bm_block_counts(const uint32_t *__restrict__ bits,
                                                         uint64_t nwords, uint64_t nblk,
-                                                        uint32_t *__restrict__ cnt) {
+                                                        uint32_t *__restrict__ cnt,
+                                                        uint64_t *__restrict__ rel = nullptr) {
   for (uint64_t b = (uint64_t)blockIdx.x * 256 + threadIdx.x; b < nblk;
        b += (uint64_t)gridDim.x * 256) {
     // the bitmap is allocated (and zero-filled) in whole blocks: one 64-byte line
     const uint4 *q = (const uint4 *)(bits + b * BM_BLOCK_WORDS);
     const uint4 q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
-    uint32_t c = __popc(q0.x) + __popc(q0.y) + __popc(q0.z) + __popc(q0.w) + __popc(q1.x) + __popc(q1.y) +
-                 __popc(q1.z) + __popc(q1.w) + __popc(q2.x) + __popc(q2.y) + __popc(q2.z) + __popc(q2.w) +
-                 __popc(q3.x) + __popc(q3.y) + __popc(q3.z) + __popc(q3.w);
+    const uint32_t p[8] = {__popc(q0.x) + __popc(q0.y), __popc(q0.z) + __popc(q0.w), __popc(q1.x) + __popc(q1.y),
+                           __popc(q1.z) + __popc(q1.w), __popc(q2.x) + __popc(q2.y), __popc(q2.z) + __popc(q2.w),
+                           __popc(q3.x) + __popc(q3.y), __popc(q3.z) + __popc(q3.w)};
+    uint32_t c = p[0];
+    uint64_t r = 0;
+#pragma unroll
+    for (int i = 1; i < 8; ++i) {
+      r |= (uint64_t)c << (9 * (i - 1));
+      c += p[i];
+    }
     cnt[b] = c;
+    if (rel) rel[b] = r;
   }
+}
+
+// rank9 lookup (see k_bm_block_counts): the number of set bits below `code`.
+__device__ __forceinline__ uint32_t bm_rank9(const uint32_t *__restrict__ bits, const uint64_t *__restrict__ rank,
+                                             const uint64_t *__restrict__ rel, uint32_t code) {
+  const uint32_t blk = code >> 9, w = (code >> 6) & 7;
+  const uint64_t word = reinterpret_cast<const uint64_t *>(bits)[code >> 6], r = rel[blk];
+  // word 0 shifts by 63: bit 63 of rel is never set (7 fields of 9 bits), so the field is 0;
+  // selecting the shift rather than the value keeps the load of rel unconditional
+  const uint32_t sub = (uint32_t)(r >> (w ? 9 * w - 9 : 63)) & 511u;
+  return (uint32_t)rank[blk] + sub + (uint32_t)__popcll(word & ((1ull << (code & 63)) - 1ull));
 }
 
 // Rank of a set bit: the block's directory entry plus the bits below it in its 512-bit block,
@@ -781,46 +806,19 @@ __device__ __forceinline__ uint64_t bm_rank(const uint32_t *__restrict__ bits,
   return r;
 }
 
-// Rank of the first bit of every bitmap word (the bits set in the words before it): its
-// 512-bit block's directory entry plus the preceding words of the block.  One thread per
-// block, 64 bytes in and out; ranks fit 32 bits (at most 2^31 keys).
-__global__ void __launch_bounds__(256) k_bm_word_ranks(const uint32_t *__restrict__ bits,
-                                                      const uint64_t *__restrict__ rank, uint64_t nblk,
-                                                      uint32_t *__restrict__ wrank) {
-  for (uint64_t b = (uint64_t)blockIdx.x * 256 + threadIdx.x; b < nblk; b += (uint64_t)gridDim.x * 256) {
-    const uint4 *q = (const uint4 *)(bits + b * BM_BLOCK_WORDS);
-    uint4 *o = (uint4 *)(wrank + b * BM_BLOCK_WORDS);
-    uint32_t r = (uint32_t)rank[b];
-#pragma unroll
-    for (int k = 0; k < BM_BLOCK_WORDS / 4; ++k) {
-      const uint4 w = q[k];
-      uint4 t;
-      t.x = r;
-      r += __popc(w.x);
-      t.y = r;
-      r += __popc(w.y);
-      t.z = r;
-      r += __popc(w.z);
-      t.w = r;
-      r += __popc(w.w);
-      o[k] = t;
-    }
-  }
-}
-
 // One block per mid tile, one wave per wave region (see k_star_mid); entry i moves to
 // position rank(key).
 __global__ void __launch_bounds__(ST_BLOCK) k_bm_scatter(const uint2 *__restrict__ rent,
                                                         const uint32_t *__restrict__ counts,
                                                         const uint32_t *__restrict__ bits,
-                                                        const uint32_t *__restrict__ wrank,
+                                                        const uint64_t *__restrict__ rank,
+                                                        const uint64_t *__restrict__ rel,
                                                         uint32_t *__restrict__ grow) {
   const uint32_t region = blockIdx.x * (ST_BLOCK / 64) + threadIdx.x / 64;
   const uint32_t n = counts[region];
   for (uint32_t i = __lane_id(); i < n; i += 64) {
     const uint2 e = rent[(uint64_t)region * PL_WAVE_ROWS + i];
-    const uint32_t code = e.x;
-    grow[wrank[code >> 5] + (uint32_t)__popc(bits[code >> 5] & ((1u << (code & 31)) - 1u))] = e.y;
+    grow[bm_rank9(bits, rank, rel, e.x)] = e.y;
   }
 }
 
@@ -833,7 +831,8 @@ constexpr uint32_t FACT_STAGE = 1u << 1;
 __global__ void __launch_bounds__(ST_BLOCK, 8) k_star_fact_bm(ScanDesc d, const SegPlan *__restrict__ plan,
                                                            int fkcol, int64_t a0, int64_t a1, int64_t b0,
                                                            int64_t b1, const uint32_t *__restrict__ bits,
-                                                           const uint32_t *__restrict__ wrank,
+                                                           const uint64_t *__restrict__ rank,
+                                                           const uint64_t *__restrict__ rel,
                                                            int64_t mkmin, uint64_t mrange,
                                                            unsigned long long *__restrict__ sum,
                                                            unsigned int *__restrict__ cnt,
@@ -899,20 +898,16 @@ __global__ void __launch_bounds__(ST_BLOCK, 8) k_star_fact_bm(ScanDesc d, const 
     nmatch += (uint32_t)__popcll(__builtin_amdgcn_ballot_w64(h));
   }
   // 2. one hit per lane per round (rarely more than one round): its group index (the rank
-  // of the key: the word's rank plus the set bits below the key's bit), payload words and
-  // the two atomics, the loads of a round issued together
+  // of the key), payload words and the two atomics, the loads of a round issued together
   const ColReader rx = col_reader(P.col[2], plan), ry = col_reader(P.col[3], plan);
   while (hits) {
     const int jh = __builtin_ctz(hits);
     hits &= hits - 1;
-    uint32_t c = 0, w = 0;
+    uint32_t c = 0;
 #pragma unroll
-    for (int j = 0; j < ST_ITEMS; ++j) {
-      c = j == jh ? code[j] : c;
-      w = j == jh ? word[j] : w;
-    }
+    for (int j = 0; j < ST_ITEMS; ++j) c = j == jh ? code[j] : c;
     const uint32_t local = r0 + (uint32_t)jh * ST_BLOCK + threadIdx.x;
-    const uint32_t g = wrank[c >> 5] + (uint32_t)__popc(w & ((1u << (c & 31)) - 1u));
+    const uint32_t g = bm_rank9(bits, rank, rel, c);
     const int64_t x = col_read(rx, local), y = col_read(ry, local);
     const int64_t v =
         wmul((int64_t)((uint64_t)a0 + (uint64_t)wmul(a1, x)), (int64_t)((uint64_t)b0 + (uint64_t)wmul(b1, y)));
@@ -1044,11 +1039,15 @@ __global__ void k_star_group_emit(const uint32_t *idx, uint64_t g, GroupAttr ga,
 }
 
 // min of vmin / max of vmax over the non-empty segments of each column: one block per column.
-__global__ void k_cols_range(const aln_seg *const *segs, const uint32_t *nsegs, int64_t *out) {
+struct ColsRange {  // passed by value: no host-to-device copy before the launch
+  const aln_seg *segs[8];
+  uint32_t nsegs[8];
+};
+__global__ void k_cols_range(ColsRange cr, int64_t *out) {
   __shared__ int64_t smn[256], smx[256];
-  const aln_seg *sg = segs[blockIdx.x];
+  const aln_seg *sg = cr.segs[blockIdx.x];
   int64_t mn = INT64_MAX, mx = INT64_MIN;
-  for (uint32_t i = threadIdx.x; i < nsegs[blockIdx.x]; i += 256)
+  for (uint32_t i = threadIdx.x; i < cr.nsegs[blockIdx.x]; i += 256)
     if (sg[i].nrows) {
       mn = sg[i].vmin < mn ? sg[i].vmin : mn;
       mx = sg[i].vmax > mx ? sg[i].vmax : mx;
@@ -1071,26 +1070,20 @@ __global__ void k_cols_range(const aln_seg *const *segs, const uint32_t *nsegs, 
 
 // Zone-map ranges of up to 8 columns with one launch and one copy (nullptr entries: 0, 0).
 static int cols_range(aln_ctx *c, const aln_col *const *cols, int n, int64_t *mn, int64_t *mx) {
-  const aln_seg *hs[8];
-  uint32_t hn[8];
+  ALN_REQUIRE(n <= 8, "cols_range: %d columns", n);
+  ColsRange cr;
   int used = 0;
   for (int i = 0; i < n; ++i)
     if (cols[i]) {
-      hs[used] = cols[i]->segs;
-      hn[used] = cols[i]->nsegs;
+      cr.segs[used] = cols[i]->segs;
+      cr.nsegs[used] = cols[i]->nsegs;
       ++used;
     }
   int64_t res[16];
   if (used) {
-    const aln_seg **ds;
-    uint32_t *dn;
     int64_t *dout;
-    ALN_TRY(ws(c, used, &ds));
-    ALN_TRY(ws(c, used, &dn));
     ALN_TRY(ws(c, 2 * used, &dout));
-    ALN_HIP(hipMemcpyAsync(ds, hs, sizeof(void *) * used, hipMemcpyHostToDevice, c->stream));
-    ALN_HIP(hipMemcpyAsync(dn, hn, 4 * used, hipMemcpyHostToDevice, c->stream));
-    hipLaunchKernelGGL(k_cols_range, dim3(used), dim3(256), 0, c->stream, ds, dn, dout);
+    hipLaunchKernelGGL(k_cols_range, dim3(used), dim3(256), 0, c->stream, cr, dout);
     ALN_TRY(check_launch("cols_range"));
     ALN_TRY(copy_to_host(c, res, dout, 16 * used));
   }
@@ -1300,32 +1293,35 @@ __global__ void __launch_bounds__(TB_BLOCK) k_topk_bmax(uint64_t n, const unsign
   }
 }
 
-// One block: the k-th largest of nb block maxima (k rounds of a block-wide maximum).
+// One block: the k-th largest of nb block maxima (k <= TS_K).  Each thread keeps its best
+// TS_K values sorted in registers (one read of bmax), then k rounds of a block-wide maximum
+// over the list heads, the winner popping its head.  0 stands for "none": with fewer than k
+// maxima the result is 0 and every group passes.
 __global__ void __launch_bounds__(TB_BLOCK) k_topk_kth(const uint64_t *__restrict__ bmax, uint32_t nb, uint32_t k,
                                                        uint64_t *__restrict__ thr) {
   __shared__ uint64_t wm[TB_BLOCK / 64];
   __shared__ uint32_t wi[TB_BLOCK / 64];
-  __shared__ uint64_t last;
   __shared__ uint32_t taken;
-  // each thread's values are bmax[threadIdx.x + t * TB_BLOCK]; a taken value is marked in
-  // its bit of a per-thread mask (at most 64 values per thread: nb <= 64 * TB_BLOCK)
-  uint64_t gone = 0;
-  if (threadIdx.x == 0) last = 0;
-  for (uint32_t r = 0; r < k; ++r) {
-    uint64_t m = 0;
-    uint32_t mi = 0xFFFFFFFFu;
-    for (uint32_t t = 0; t * TB_BLOCK + threadIdx.x < nb; ++t) {
-      const uint64_t v = bmax[t * TB_BLOCK + threadIdx.x];
-      if (!((gone >> t) & 1) && (mi == 0xFFFFFFFFu || v > m)) {
-        m = v;
-        mi = t;
-      }
+  uint64_t l[TS_K];
+#pragma unroll
+  for (int q = 0; q < TS_K; ++q) l[q] = 0;
+  for (uint32_t t = threadIdx.x; t < nb; t += TB_BLOCK) {
+    uint64_t v = bmax[t];
+#pragma unroll
+    for (int q = 0; q < TS_K; ++q) {  // insertion into the descending list
+      const uint64_t hi = v > l[q] ? v : l[q], lo = v > l[q] ? l[q] : v;
+      l[q] = hi;
+      v = lo;
     }
-    uint32_t who = mi == 0xFFFFFFFFu ? 0xFFFFFFFFu : threadIdx.x;
+  }
+  uint64_t last = 0;
+  for (uint32_t r = 0; r < k; ++r) {
+    uint64_t m = l[0];
+    uint32_t who = threadIdx.x;
     for (int o = 32; o > 0; o >>= 1) {
       const uint64_t m2 = __shfl_xor(m, o, 64);
       const uint32_t w2 = __shfl_xor(who, o, 64);
-      if (w2 != 0xFFFFFFFFu && (who == 0xFFFFFFFFu || m2 > m || (m2 == m && w2 < who))) {
+      if (m2 > m || (m2 == m && w2 < who)) {
         m = m2;
         who = w2;
       }
@@ -1339,15 +1335,20 @@ __global__ void __launch_bounds__(TB_BLOCK) k_topk_kth(const uint64_t *__restric
       uint64_t bm = wm[0];
       uint32_t bw = wi[0];
       for (int w = 1; w < TB_BLOCK / 64; ++w)
-        if (wi[w] != 0xFFFFFFFFu && (bw == 0xFFFFFFFFu || wm[w] > bm || (wm[w] == bm && wi[w] < bw))) {
+        if (wm[w] > bm || (wm[w] == bm && wi[w] < bw)) {
           bm = wm[w];
           bw = wi[w];
         }
       taken = bw;
-      last = bw != 0xFFFFFFFFu ? bm : 0;  // fewer than k maxima: no bound, every group passes
+      wm[0] = bm;
     }
     __syncthreads();
-    if (taken == threadIdx.x) gone |= 1ull << mi;
+    last = wm[0];
+    if (taken == threadIdx.x) {
+#pragma unroll
+      for (int q = 0; q + 1 < TS_K; ++q) l[q] = l[q + 1];
+      l[TS_K - 1] = 0;
+    }
     __syncthreads();
   }
   if (threadIdx.x == 0) *thr = last;
@@ -1651,33 +1652,28 @@ extern "C" int aln_star_agg(aln_ctx *c, const aln_star_args *a, aln_star_row *h_
                        a->m_fk, bitmap, dkmin, drange, mcnt, mprobe, rent, mbits, mkmin);
   }
   ALN_TRY(check_launch("star_mid"));
-  ALN_TRY(sum_u32(c, dcnt, dblocks, ctr + 0));
-  ALN_TRY(sum_u32(c, mcnt, mregions, ctr + 1));
-  ALN_TRY(sum_u32(c, mprobe, mregions, ctr + 2));
   // bitmap join: the rank directory now (its total = distinct passing keys), before the
   // one synchronisation that reads the counters
-  uint64_t *rank = nullptr;
-  uint32_t *wrank = nullptr;
+  uint64_t *rank = nullptr, *rel = nullptr;
+  uint32_t *bcnt = nullptr;
   uint64_t distinct = 0;
   if (use_bm) {
-    uint32_t *bcnt;
     ALN_TRY(ws(c, mblk, &bcnt));
     ALN_TRY(ws(c, mblk, &rank));
+    ALN_TRY(ws(c, mblk, &rel));
     {
       ProfScope p(c, "star_build");
       hipLaunchKernelGGL(k_bm_block_counts, dim3(grid_for(mblk, 256, 65536)), dim3(256), 0, c->stream,
-                         mbits, mwords, mblk, bcnt);
+                         mbits, mwords, mblk, bcnt, rel);
     }
     ALN_TRY(check_launch("bm_block_counts"));
-    ALN_TRY(sum_u32(c, bcnt, mblk, ctr + 5));
     ALN_TRY(exclusive_scan_u32(c, bcnt, mblk, rank, nullptr));
-    ALN_TRY(ws(c, mblk * BM_BLOCK_WORDS, &wrank));
-    {
-      ProfScope p(c, "star_build");
-      hipLaunchKernelGGL(k_bm_word_ranks, dim3(grid_for(mblk, 256, 65536)), dim3(256), 0, c->stream, mbits, rank,
-                         mblk, wrank);
-    }
-    ALN_TRY(check_launch("bm_word_ranks"));
+  }
+  {  // the counters: dim pass, mid pass, mid probe and (bitmap join) distinct mid keys
+    const uint32_t *in[4] = {dcnt, mcnt, mprobe, bcnt};
+    const uint64_t n[4] = {dblocks, mregions, mregions, mblk};
+    uint64_t *out[4] = {ctr + 0, ctr + 1, ctr + 2, ctr + 5};
+    ALN_TRY(sum_u32_multi(c, use_bm ? 4 : 3, in, n, out));
   }
   uint64_t hc[8];
   ALN_TRY(copy_to_host(c, hc, ctr, sizeof(hc)));
@@ -1706,7 +1702,7 @@ extern "C" int aln_star_agg(aln_ctx *c, const aln_star_args *a, aln_star_row *h_
     if (m) {
       ProfScope p(c, "star_build");
       hipLaunchKernelGGL(k_bm_scatter, dim3((unsigned)mblocks), dim3(ST_BLOCK), 0, c->stream, rent, mcnt, mbits,
-                         wrank, grow);
+                         rank, rel, grow);
     }
     ALN_TRY(check_launch("bm_scatter"));
   } else {
@@ -1728,15 +1724,17 @@ extern "C" int aln_star_agg(aln_ctx *c, const aln_star_args *a, aln_star_row *h_
     ProfScope p(c, "star_fact");
     if (use_bm)
       hipLaunchKernelGGL(k_star_fact_bm, dim3((unsigned)fblocks), dim3(ST_BLOCK), 0, c->stream, fd, fplan, a->f_fk,
-                         a->a0, a->a1, a->b0, a->b1, mbits, wrank, mkmin, mrange, sum, cnt, fmatch, fprobe);
+                         a->a0, a->a1, a->b0, a->b1, mbits, rank, rel, mkmin, mrange, sum, cnt, fmatch, fprobe);
     else  // per-block counts
       hipLaunchKernelGGL(k_star_fact, dim3((unsigned)fblocks), dim3(ST_BLOCK), 0, c->stream, fd, a->f_fk,
                          a->f_x, a->f_y, a->a0, a->a1, a->b0, a->b1, table, tcap, mkmin, mrange, sum, cnt,
                          fmatch, fprobe);
     ALN_TRY(check_launch("star_fact"));
     const uint64_t nfc = use_bm ? fwaves : fblocks;
-    ALN_TRY(sum_u32(c, fmatch, nfc, ctr + 3));
-    ALN_TRY(sum_u32(c, fprobe, nfc, ctr + 4));
+    const uint32_t *in[2] = {fmatch, fprobe};
+    const uint64_t n[2] = {nfc, nfc};
+    uint64_t *out[2] = {ctr + 3, ctr + 4};
+    ALN_TRY(sum_u32_multi(c, 2, in, n, out));
   }
   tm.mark(3);
 
