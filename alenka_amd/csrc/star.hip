@@ -266,9 +266,10 @@ static_assert(plan_slot_words(STG_MAX_BITS) == STG_SLOT_WORDS, "slot sizes agree
 
 // rc1..rc4: the columns of roles 1..4 (-1: none); roles whose bit is set in `stage` are
 // staged when their column is not constant.  kmin: the probe key's code is key - kmin, and
-// the fast path needs base - kmin in 32 bits.  fast = false plans every live segment 'G'.
+// the fast path needs base - kmin in 32 bits.  fast = false plans every live segment 'G';
+// a segment needing more than max_own (<= PL_OWN) staged columns is 'G' too.
 __global__ void k_scan_plan(ScanDesc d, int rc1, int rc2, int rc3, int rc4, uint32_t stage, int64_t kmin,
-                            int fast, uint32_t nsegs, SegPlan *__restrict__ out) {
+                            int fast, uint32_t max_own, uint32_t nsegs, SegPlan *__restrict__ out) {
   const uint32_t seg = blockIdx.x * blockDim.x + threadIdx.x;
   if (seg >= nsegs) return;
   SegPlan P;
@@ -308,7 +309,7 @@ __global__ void k_scan_plan(ScanDesc d, int rc1, int rc2, int rc3, int rc4, uint
     }
     pc.slot = slot;
   }
-  if (words > (uint32_t)PL_STAGE_WORDS || nown > (uint32_t)PL_OWN) generic = true;
+  if (words > (uint32_t)PL_STAGE_WORDS || nown > max_own) generic = true;
   // the fast probe reads a staged, non-constant key and shifts it in 32 bits
   if (rc1 >= 0) {
     const int64_t off = (int64_t)((uint64_t)P.col[1].base - (uint64_t)kmin);
@@ -396,41 +397,53 @@ __device__ __forceinline__ uint32_t slot_code(const PlanSlot &s, uint32_t i) {
   return __builtin_amdgcn_alignbit(s.w[wi + 1], s.w[wi], pos & 31) & s.mask;
 }
 
-// Copies the plan's staged columns for rows [r0, r0 + nt) into LDS: the aligned 16-byte
-// blocks covering each column's codes (an aligned block never crosses a page, so the words
-// outside the column it may carry are readable), all of a thread's loads (PL_CVEC per
-// column) issued before its first store.  Every thread calls it; a barrier must follow.
+// The plan's staged columns for rows [r0, r0 + nt), in two steps: issue() loads the aligned
+// 16-byte blocks covering each column's codes into registers (an aligned block never
+// crosses a page, so the words outside the column it may carry are readable), all of a
+// thread's loads (PL_CVEC per column) in flight together; store() copies them into LDS.
+// Between the two a block can do other work - decode the tile staged before.  Every thread
+// calls both; a barrier must follow store().  NOWN: staged columns the plan may hold.
 constexpr int PL_CVEC = (STG_SLOT_WORDS / 4 + ST_BLOCK - 1) / ST_BLOCK;  // loads per column
-__device__ __forceinline__ void stage_plan(uint32_t *lds, const SegPlan &P, uint32_t r0, uint32_t nt) {
-  const uint4 *src[PL_OWN];
-  uint32_t nb[PL_OWN];
+template <int NOWN>
+struct StageRegs {
+  uint4 r[NOWN][PL_CVEC];
+  uint32_t nb[NOWN];
+  int32_t slot[NOWN];
+  __device__ __forceinline__ void issue(const SegPlan &P, uint32_t r0, uint32_t nt) {
 #pragma unroll
-  for (int k = 0; k < PL_OWN; ++k) {
-    const StageSrc S = P.st[k];
-    const uint64_t b0 = (uint64_t)r0 * S.bits;
-    const uint64_t wbeg = b0 >> 5, wend = ((uint64_t)(r0 + nt) * S.bits + 31) >> 5;
-    const uint32_t *p = S.w + wbeg;
-    const uint32_t a = (uint32_t)(((uintptr_t)p >> 2) & 3);
-    src[k] = reinterpret_cast<const uint4 *>(p - a);
-    nb[k] = S.bits ? (uint32_t)((a + (wend - wbeg) + 3) >> 2) : 0u;
+    for (int k = 0; k < NOWN; ++k) {
+      const StageSrc S = P.st[k];
+      const uint64_t b0 = (uint64_t)r0 * S.bits;
+      const uint64_t wbeg = b0 >> 5, wend = ((uint64_t)(r0 + nt) * S.bits + 31) >> 5;
+      const uint32_t *p = S.w + wbeg;
+      const uint32_t a = (uint32_t)(((uintptr_t)p >> 2) & 3);
+      const uint4 *src = reinterpret_cast<const uint4 *>(p - a);
+      nb[k] = S.bits ? (uint32_t)((a + (wend - wbeg) + 3) >> 2) : 0u;
+      slot[k] = S.slot / 4;
+#pragma unroll
+      for (int t = 0; t < PL_CVEC; ++t) {
+        const uint32_t v = t * ST_BLOCK + threadIdx.x;
+        r[k][t] = make_uint4(0, 0, 0, 0);
+        if (v < nb[k]) r[k][t] = gload4(src + v);
+      }
+    }
   }
-  uint4 r[PL_OWN][PL_CVEC];
+  __device__ __forceinline__ void store(uint32_t *lds) const {
+    uint4 *l4 = reinterpret_cast<uint4 *>(lds);
 #pragma unroll
-  for (int k = 0; k < PL_OWN; ++k)
+    for (int k = 0; k < NOWN; ++k)
 #pragma unroll
-    for (int t = 0; t < PL_CVEC; ++t) {
-      const uint32_t v = t * ST_BLOCK + threadIdx.x;
-      r[k][t] = make_uint4(0, 0, 0, 0);
-      if (v < nb[k]) r[k][t] = gload4(src[k] + v);
-    }
-  uint4 *l4 = reinterpret_cast<uint4 *>(lds);
-#pragma unroll
-  for (int k = 0; k < PL_OWN; ++k)
-#pragma unroll
-    for (int t = 0; t < PL_CVEC; ++t) {
-      const uint32_t v = t * ST_BLOCK + threadIdx.x;
-      if (v < nb[k]) l4[P.st[k].slot / 4 + (int32_t)v] = r[k][t];
-    }
+      for (int t = 0; t < PL_CVEC; ++t) {
+        const uint32_t v = t * ST_BLOCK + threadIdx.x;
+        if (v < nb[k]) l4[slot[k] + (int32_t)v] = r[k][t];
+      }
+  }
+};
+template <int NOWN = PL_OWN>
+__device__ __forceinline__ void stage_plan(uint32_t *lds, const SegPlan &P, uint32_t r0, uint32_t nt) {
+  StageRegs<NOWN> s;
+  s.issue(P, r0, nt);
+  s.store(lds);
 }
 
 // The fast decoder's view of the range (role 0, mode 'R') and probe-key (role 1) slots.
@@ -826,70 +839,66 @@ __global__ void __launch_bounds__(ST_BLOCK) k_bm_scatter(const uint2 *__restrict
 // and y, read from global memory for the few rows that join.  Counts are per wave
 // (index b * 8 + w), so the block needs one barrier, after staging.
 constexpr uint32_t FACT_STAGE = 1u << 1;
+constexpr int FACT_OWN = 2;  // staged columns: the range and the key
 
-// (second bound: eight waves per SIMD, four blocks per CU, i.e. at most 64 VGPRs)
-__global__ void __launch_bounds__(ST_BLOCK, 8) k_star_fact_bm(ScanDesc d, const SegPlan *__restrict__ plan,
-                                                           int fkcol, int64_t a0, int64_t a1, int64_t b0,
-                                                           int64_t b1, const uint32_t *__restrict__ bits,
-                                                           const uint64_t *__restrict__ rank,
-                                                           const uint64_t *__restrict__ rel,
-                                                           int64_t mkmin, uint64_t mrange,
-                                                           unsigned long long *__restrict__ sum,
-                                                           unsigned int *__restrict__ cnt,
-                                                           uint32_t *__restrict__ matched,
-                                                           uint32_t *__restrict__ probed) {
-  static_assert(sizeof(TileStage) <= PL_STAGE_WORDS * sizeof(uint32_t), "generic staging fits the LDS area");
-  __shared__ __align__(16) uint32_t lds[PL_STAGE_WORDS];
-  __shared__ int want[ALN_MAX_RANGES + 1];
-  __shared__ TilePred tp;
-  const uint32_t seg = blockIdx.x / d.tps, tile = blockIdx.x % d.tps;
-  const SegPlan &P = plan[seg];
-  const uint32_t r0 = tile * ST_TILE, nrows = P.nrows;
-  const unsigned lane = __lane_id();
-  const uint32_t cidx = blockIdx.x * (ST_BLOCK / 64) + threadIdx.x / 64;
-  const int mode = P.mode;
-  if (r0 >= nrows || mode == 'N') {
-    if (lane == 0) {
-      matched[cidx] = 0;
-      probed[cidx] = 0;
-    }
-    return;
-  }
-  const uint32_t nt = nrows - r0 < (uint32_t)ST_TILE ? nrows - r0 : (uint32_t)ST_TILE;
-  // 1. predicate and key (code space) for all items, then every bitmap word load
-  uint32_t code[ST_ITEMS];
+// The fact scan's per-query constants.
+struct FactArgs {
+  int fkcol;
+  int64_t a0, a1, b0, b1;
+  const uint32_t *bits;  // mid bitmap and its rank9 directory
+  const uint64_t *rank, *rel;
+  int64_t mkmin;
+  uint64_t mrange;
+  unsigned long long *sum;
+  unsigned int *cnt;
+  uint32_t *matched, *probed;  // per wave: index tile * 8 + wave
+};
+
+// Predicate and key of a staged fast tile (after the barrier that follows staging), as codes
+// into the mid bitmap (~0: no probe); returns the wave's probe count.
+__device__ __forceinline__ uint32_t fact_fast_codes(const uint32_t *lds, const SegPlan &P, const FactArgs &A,
+                                                    uint32_t r0, uint32_t nt, uint32_t (&code)[ST_ITEMS]) {
+  const FastTile ft = plan_fast(lds, P, r0);
+  // code + off lies in (-2^31, 2^32): any value outside [0, mrange), mrange <= 2^31, maps to
+  // >= 2^31 in 32 bits
+  const uint32_t off = (uint32_t)(int32_t)((uint64_t)P.col[1].base - (uint64_t)A.mkmin), mr = (uint32_t)A.mrange;
+  return ft.range ? fast_probe<true>(ft, nt, off, mr, code) : fast_probe<false>(ft, nt, off, mr, code);
+}
+
+// The same for a tile the plan could not take the fast way ('G'): generic staging (with its
+// own barriers) and decoding; returns the wave's probe count.
+__device__ __forceinline__ uint32_t fact_generic_codes(uint32_t *lds, int *want, TilePred &tp, const ScanDesc &d,
+                                                       const SegPlan &P, const FactArgs &A, uint32_t seg,
+                                                       uint32_t r0, uint32_t nt, uint32_t (&code)[ST_ITEMS]) {
+  TileStage &ts = *reinterpret_cast<TileStage *>(lds);
+  const char zm = (char)P.zmode;
+  star_stage(ts, want, d, zm, A.fkcol, seg, r0, nt, &tp);
+  const int fs = ts.slot_of[A.fkcol];
+  const int64_t foff = fs >= 0 ? (int64_t)((uint64_t)ts.slot[fs].base - (uint64_t)A.mkmin) : 0;
   uint32_t nprobe = 0;
-  if (mode != 'G') {
-    stage_plan(lds, P, r0, nt);
-    __syncthreads();
-    const FastTile ft = plan_fast(lds, P, r0);
-    // code + off lies in (-2^31, 2^32): any value outside [0, mrange), mrange <= 2^31, maps
-    // to >= 2^31 in 32 bits
-    const uint32_t off = (uint32_t)(int32_t)((uint64_t)P.col[1].base - (uint64_t)mkmin), mr = (uint32_t)mrange;
-    nprobe = ft.range ? fast_probe<true>(ft, nt, off, mr, code) : fast_probe<false>(ft, nt, off, mr, code);
-  } else {
-    TileStage &ts = *reinterpret_cast<TileStage *>(lds);
-    const char zm = (char)P.zmode;
-    star_stage(ts, want, d, zm, fkcol, seg, r0, nt, &tp);
-    const int fs = ts.slot_of[fkcol];
-    const int64_t foff = fs >= 0 ? (int64_t)((uint64_t)ts.slot[fs].base - (uint64_t)mkmin) : 0;
 #pragma unroll
-    for (int j = 0; j < ST_ITEMS; ++j) {
-      const uint32_t i = j * ST_BLOCK + threadIdx.x;
-      code[j] = 0xFFFFFFFFu;
-      if (i < nt && tile_pass_fast(d, ts, tp, zm, seg, r0, i)) {
-        ++nprobe;
-        const uint64_t c = fs >= 0 ? (uint64_t)((int64_t)stage_code(ts.words[fs], ts.slot[fs], i) + foff)
-                                   : (uint64_t)tile_get(ts, d.words, d.segs, fkcol, seg, r0, i) - (uint64_t)mkmin;
-        if (c < mrange) code[j] = (uint32_t)c;  // mrange <= 2^31 on this path
-      }
+  for (int j = 0; j < ST_ITEMS; ++j) {
+    const uint32_t i = j * ST_BLOCK + threadIdx.x;
+    code[j] = 0xFFFFFFFFu;
+    if (i < nt && tile_pass_fast(d, ts, tp, zm, seg, r0, i)) {
+      ++nprobe;
+      const uint64_t c = fs >= 0 ? (uint64_t)((int64_t)stage_code(ts.words[fs], ts.slot[fs], i) + foff)
+                                 : (uint64_t)tile_get(ts, d.words, d.segs, A.fkcol, seg, r0, i) - (uint64_t)A.mkmin;
+      if (c < A.mrange) code[j] = (uint32_t)c;  // mrange <= 2^31 on this path
     }
-    nprobe = wave_sum(nprobe);
   }
+  return wave_sum(nprobe);
+}
+
+// Probe of the mid bitmap for a tile's codes, then one hit per lane per round (rarely more
+// than one round): its group index (the rank of the key), payload words and the two
+// atomics, the loads of a round issued together; lane 0 writes the wave's counts.
+__device__ __forceinline__ void fact_probe_hits(const SegPlan &P, const void *any, const FactArgs &A, uint32_t r0,
+                                                const uint32_t (&code)[ST_ITEMS], uint32_t nprobe, uint32_t cidx) {
   // the bitmap words, loaded without branches (rows that do not probe read word 0)
   uint32_t word[ST_ITEMS];
 #pragma unroll
-  for (int j = 0; j < ST_ITEMS; ++j) word[j] = bits[code[j] != 0xFFFFFFFFu ? code[j] >> 5 : 0u];
+  for (int j = 0; j < ST_ITEMS; ++j) word[j] = A.bits[code[j] != 0xFFFFFFFFu ? code[j] >> 5 : 0u];
   uint32_t hits = 0, nmatch = 0;
 #pragma unroll
   for (int j = 0; j < ST_ITEMS; ++j) {
@@ -897,9 +906,7 @@ __global__ void __launch_bounds__(ST_BLOCK, 8) k_star_fact_bm(ScanDesc d, const 
     hits |= (uint32_t)h << j;
     nmatch += (uint32_t)__popcll(__builtin_amdgcn_ballot_w64(h));
   }
-  // 2. one hit per lane per round (rarely more than one round): its group index (the rank
-  // of the key), payload words and the two atomics, the loads of a round issued together
-  const ColReader rx = col_reader(P.col[2], plan), ry = col_reader(P.col[3], plan);
+  const ColReader rx = col_reader(P.col[2], any), ry = col_reader(P.col[3], any);
   while (hits) {
     const int jh = __builtin_ctz(hits);
     hits &= hits - 1;
@@ -907,16 +914,106 @@ __global__ void __launch_bounds__(ST_BLOCK, 8) k_star_fact_bm(ScanDesc d, const 
 #pragma unroll
     for (int j = 0; j < ST_ITEMS; ++j) c = j == jh ? code[j] : c;
     const uint32_t local = r0 + (uint32_t)jh * ST_BLOCK + threadIdx.x;
-    const uint32_t g = bm_rank9(bits, rank, rel, c);
+    const uint32_t g = bm_rank9(A.bits, A.rank, A.rel, c);
     const int64_t x = col_read(rx, local), y = col_read(ry, local);
-    const int64_t v =
-        wmul((int64_t)((uint64_t)a0 + (uint64_t)wmul(a1, x)), (int64_t)((uint64_t)b0 + (uint64_t)wmul(b1, y)));
-    atomicAdd(&sum[g], (unsigned long long)v);
-    atomicAdd(&cnt[g], 1u);
+    const int64_t v = wmul((int64_t)((uint64_t)A.a0 + (uint64_t)wmul(A.a1, x)),
+                           (int64_t)((uint64_t)A.b0 + (uint64_t)wmul(A.b1, y)));
+    atomicAdd(&A.sum[g], (unsigned long long)v);
+    atomicAdd(&A.cnt[g], 1u);
   }
-  if (lane == 0) {
-    matched[cidx] = nmatch;
-    probed[cidx] = nprobe;
+  if (__lane_id() == 0) {
+    A.matched[cidx] = nmatch;
+    A.probed[cidx] = nprobe;
+  }
+}
+
+// One block per tile.  (Second bound: eight waves per SIMD, four blocks per CU, i.e. at
+// most 64 VGPRs.)
+__global__ void __launch_bounds__(ST_BLOCK, 8) k_star_fact_bm(ScanDesc d, const SegPlan *__restrict__ plan,
+                                                           FactArgs A) {
+  static_assert(sizeof(TileStage) <= PL_STAGE_WORDS * sizeof(uint32_t), "generic staging fits the LDS area");
+  __shared__ __align__(16) uint32_t lds[PL_STAGE_WORDS];
+  __shared__ int want[ALN_MAX_RANGES + 1];
+  __shared__ TilePred tp;
+  const uint32_t seg = blockIdx.x / d.tps, tile = blockIdx.x % d.tps;
+  const SegPlan &P = plan[seg];
+  const uint32_t r0 = tile * ST_TILE, nrows = P.nrows;
+  const uint32_t cidx = blockIdx.x * (ST_BLOCK / 64) + threadIdx.x / 64;
+  const int mode = P.mode;
+  if (r0 >= nrows || mode == 'N') {
+    if (__lane_id() == 0) {
+      A.matched[cidx] = 0;
+      A.probed[cidx] = 0;
+    }
+    return;
+  }
+  const uint32_t nt = nrows - r0 < (uint32_t)ST_TILE ? nrows - r0 : (uint32_t)ST_TILE;
+  uint32_t code[ST_ITEMS];
+  uint32_t nprobe;
+  if (mode != 'G') {
+    stage_plan<FACT_OWN>(lds, P, r0, nt);
+    __syncthreads();
+    nprobe = fact_fast_codes(lds, P, A, r0, nt, code);
+  } else {
+    nprobe = fact_generic_codes(lds, want, tp, d, P, A, seg, r0, nt, code);
+  }
+  fact_probe_hits(P, plan, A, r0, code, nprobe, cidx);
+}
+
+// Persistent form: block b takes tiles b, b + G, b + 2G, ... (G = gridDim.x, the blocks the
+// device holds at once) and keeps the next fast tile's staging loads in registers while it
+// decodes, probes and aggregates the current one, so a block's HBM reads overlap its own
+// compute instead of waiting at the start of every tile.
+constexpr int FACT_PP_BLOCKS_PER_CU = 3;  // the prefetch registers need up to 80 VGPRs
+__global__ void __launch_bounds__(ST_BLOCK, 2 * FACT_PP_BLOCKS_PER_CU) k_star_fact_pp(ScanDesc d, const SegPlan *__restrict__ plan,
+                                                           uint32_t ntiles, FactArgs A) {
+  __shared__ __align__(16) uint32_t lds[PL_STAGE_WORDS];
+  __shared__ int want[ALN_MAX_RANGES + 1];
+  __shared__ TilePred tp;
+  // a tile is fast when its segment's plan is 'A' or 'R' and the tile holds rows
+  auto fast_tile_at = [&](uint32_t t, uint32_t &r0, uint32_t &nt) -> bool {
+    const SegPlan &Q = plan[t / d.tps];
+    r0 = (t % d.tps) * ST_TILE;
+    nt = Q.nrows > r0 ? (Q.nrows - r0 < (uint32_t)ST_TILE ? Q.nrows - r0 : (uint32_t)ST_TILE) : 0u;
+    return nt && (Q.mode == 'A' || Q.mode == 'R');
+  };
+  StageRegs<FACT_OWN> sr;
+  uint32_t t = blockIdx.x, r0n, ntn;
+  bool have = t < ntiles && fast_tile_at(t, r0n, ntn);
+  if (have) sr.issue(plan[t / d.tps], r0n, ntn);
+  for (; t < ntiles; t += gridDim.x) {
+    const uint32_t seg = t / d.tps;
+    const SegPlan &P = plan[seg];
+    const uint32_t r0 = (t % d.tps) * ST_TILE, nrows = P.nrows;
+    const uint32_t cidx = t * (ST_BLOCK / 64) + threadIdx.x / 64;
+    const int mode = P.mode;
+    const uint32_t tn = t + gridDim.x;
+    if (r0 >= nrows || mode == 'N') {
+      if (__lane_id() == 0) {
+        A.matched[cidx] = 0;
+        A.probed[cidx] = 0;
+      }
+      have = tn < ntiles && fast_tile_at(tn, r0n, ntn);
+      if (have) sr.issue(plan[tn / d.tps], r0n, ntn);
+      continue;
+    }
+    const uint32_t nt = nrows - r0 < (uint32_t)ST_TILE ? nrows - r0 : (uint32_t)ST_TILE;
+    uint32_t code[ST_ITEMS];
+    uint32_t nprobe;
+    __syncthreads();  // the previous tile's reads of LDS are done
+    if (mode != 'G') {
+      if (!have) sr.issue(P, r0, nt);
+      sr.store(lds);
+      __syncthreads();
+      have = tn < ntiles && fast_tile_at(tn, r0n, ntn);
+      if (have) sr.issue(plan[tn / d.tps], r0n, ntn);  // in flight during this tile's work
+      nprobe = fact_fast_codes(lds, P, A, r0, nt, code);
+    } else {
+      nprobe = fact_generic_codes(lds, want, tp, d, P, A, seg, r0, nt, code);
+      have = tn < ntiles && fast_tile_at(tn, r0n, ntn);
+      if (have) sr.issue(plan[tn / d.tps], r0n, ntn);
+    }
+    fact_probe_hits(P, plan, A, r0, code, nprobe, cidx);
   }
 }
 
@@ -1623,12 +1720,12 @@ extern "C" int aln_star_agg(aln_ctx *c, const aln_star_args *a, aln_star_row *h_
     const uint32_t ms = a->mid.cols[0].nsegs, fs = a->fact.cols[0].nsegs;
     ALN_TRY(ws(c, ms, &mplan));
     hipLaunchKernelGGL(k_scan_plan, dim3((ms + 127) / 128), dim3(128), 0, c->stream, md, a->m_fk, a->m_key, a->m_g1,
-                       a->m_g2, MID_STAGE, dkmin, (int)(drange <= (1ull << 31)), ms, mplan);
+                       a->m_g2, MID_STAGE, dkmin, (int)(drange <= (1ull << 31)), (uint32_t)PL_OWN, ms, mplan);
     ALN_TRY(check_launch("scan_plan"));
     if (use_bm && fs) {
       ALN_TRY(ws(c, fs, &fplan));
       hipLaunchKernelGGL(k_scan_plan, dim3((fs + 127) / 128), dim3(128), 0, c->stream, fd, a->f_fk, a->f_x, a->f_y,
-                         -1, FACT_STAGE, mkmin, 1, fs, fplan);
+                         -1, FACT_STAGE, mkmin, 1, (uint32_t)FACT_OWN, fs, fplan);
       ALN_TRY(check_launch("scan_plan"));
     }
   }
@@ -1722,10 +1819,18 @@ extern "C" int aln_star_agg(aln_ctx *c, const aln_star_args *a, aln_star_row *h_
   tm.mark(2);
   if (m && fblocks) {
     ProfScope p(c, "star_fact");
-    if (use_bm)
-      hipLaunchKernelGGL(k_star_fact_bm, dim3((unsigned)fblocks), dim3(ST_BLOCK), 0, c->stream, fd, fplan, a->f_fk,
-                         a->a0, a->a1, a->b0, a->b1, mbits, rank, rel, mkmin, mrange, sum, cnt, fmatch, fprobe);
-    else  // per-block counts
+    if (use_bm) {
+      const FactArgs fa{a->f_fk, a->a0, a->a1, a->b0, a->b1, mbits, rank, rel, mkmin, mrange, sum, cnt, fmatch, fprobe};
+      if (a->flags & 4) {  // persistent blocks with the next tile's loads in flight
+        int cus = 0;
+        ALN_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+        const uint64_t g = std::min<uint64_t>(fblocks, (uint64_t)cus * FACT_PP_BLOCKS_PER_CU);
+        hipLaunchKernelGGL(k_star_fact_pp, dim3((unsigned)g), dim3(ST_BLOCK), 0, c->stream, fd, fplan,
+                           (uint32_t)fblocks, fa);
+      } else {
+        hipLaunchKernelGGL(k_star_fact_bm, dim3((unsigned)fblocks), dim3(ST_BLOCK), 0, c->stream, fd, fplan, fa);
+      }
+    } else  // per-block counts
       hipLaunchKernelGGL(k_star_fact, dim3((unsigned)fblocks), dim3(ST_BLOCK), 0, c->stream, fd, a->f_fk,
                          a->f_x, a->f_y, a->a0, a->a1, a->b0, a->b1, table, tcap, mkmin, mrange, sum, cnt,
                          fmatch, fprobe);

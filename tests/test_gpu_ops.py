@@ -299,9 +299,9 @@ def q3_args(t, dim, mid, fact):
     return D_, M_, F_, spec
 
 
-@pytest.mark.parametrize("force_hash", [False, True])
+@pytest.mark.parametrize("join", ["bitmap", "hash", "bitmap-persistent"])
 @pytest.mark.parametrize("scale,seg_rows", [(0.02, 4096), (0.1, 65536)])
-def test_star_q3(ctx, scale, seg_rows, force_hash):
+def test_star_q3(ctx, scale, seg_rows, join):
     from alenka_amd import device as D
 
     ndim, norders = int(150000 * scale), int(1500000 * scale)
@@ -309,7 +309,8 @@ def test_star_q3(ctx, scale, seg_rows, force_hash):
     top, groups, stats = Q3.run(t)
     dim, mid, fact = encode_q3(ctx, t, seg_rows)
     D_, M_, F_, spec = q3_args(t, dim, mid, fact)
-    rows, g, st = D.star_agg(ctx, D_, M_, F_, spec, limit=10, want_groups=True, force_hash=force_hash)
+    rows, g, st = D.star_agg(ctx, D_, M_, F_, spec, limit=10, want_groups=True, force_hash=join == "hash",
+                             persistent=join == "bitmap-persistent")
     assert rows == top
     assert st.dim_pass == stats["dim_pass"] and st.mid_pass == stats["mid_pass"]
     assert st.fact_pass == stats["fact_pass"] and st.groups == stats["groups"]
