@@ -401,8 +401,8 @@ __device__ __forceinline__ uint32_t slot_code(const PlanSlot &s, uint32_t i) {
 // 16-byte blocks covering each column's codes into registers (an aligned block never
 // crosses a page, so the words outside the column it may carry are readable), all of a
 // thread's loads (PL_CVEC per column) in flight together; store() copies them into LDS.
-// Between the two a block can do other work - decode the tile staged before.  Every thread
-// calls both; a barrier must follow store().  NOWN: staged columns the plan may hold.
+// Every thread calls both; a barrier must follow store().  NOWN: staged columns the plan
+// may hold.
 constexpr int PL_CVEC = (STG_SLOT_WORDS / 4 + ST_BLOCK - 1) / ST_BLOCK;  // loads per column
 template <int NOWN>
 struct StageRegs {
@@ -648,17 +648,22 @@ __device__ __forceinline__ uint64_t st_slot(uint32_t code, uint64_t cap) {
 
 // ---- B: build the join table over the compacted mid rows ----------------------------------
 // One block per mid tile, one wave per wave region: entry i of region b becomes group
-// offs[b] + i; its mid row goes to grow[group] and its key into the table.
+// offs[b] + i; its mid row goes to grow[group], its key into the table, and its
+// accumulators are zeroed.
 __global__ void __launch_bounds__(ST_BLOCK) k_star_build(const uint2 *__restrict__ rent,
                                                     const uint32_t *__restrict__ counts,
                                                     const uint64_t *__restrict__ offs, uint64_t *table,
-                                                    uint64_t cap, uint32_t *__restrict__ grow, uint32_t *dup) {
+                                                    uint64_t cap, uint32_t *__restrict__ grow,
+                                                    unsigned long long *__restrict__ sum,
+                                                    unsigned int *__restrict__ cnt, uint32_t *dup) {
   const uint32_t region = blockIdx.x * (ST_BLOCK / 64) + threadIdx.x / 64;
   const uint32_t n = counts[region];
   for (uint32_t i = __lane_id(); i < n; i += 64) {
     const uint64_t j = offs[region] + i;
     const uint2 e = rent[(uint64_t)region * PL_WAVE_ROWS + i];
     grow[j] = e.y;
+    sum[j] = 0;  // every group index below the passing-row count is written exactly once
+    cnt[j] = 0;
     const uint32_t code = e.x;
     uint64_t entry = ((uint64_t)j << 32) | code;
     uint64_t h = st_slot(code, cap);
@@ -819,19 +824,47 @@ __device__ __forceinline__ uint64_t bm_rank(const uint32_t *__restrict__ bits,
   return r;
 }
 
-// One block per mid tile, one wave per wave region (see k_star_mid); entry i moves to
-// position rank(key).
-__global__ void __launch_bounds__(ST_BLOCK) k_bm_scatter(const uint2 *__restrict__ rent,
-                                                        const uint32_t *__restrict__ counts,
-                                                        const uint32_t *__restrict__ bits,
-                                                        const uint64_t *__restrict__ rank,
-                                                        const uint64_t *__restrict__ rel,
-                                                        uint32_t *__restrict__ grow) {
-  const uint32_t region = blockIdx.x * (ST_BLOCK / 64) + threadIdx.x / 64;
-  const uint32_t n = counts[region];
-  for (uint32_t i = __lane_id(); i < n; i += 64) {
-    const uint2 e = rent[(uint64_t)region * PL_WAVE_ROWS + i];
-    grow[bm_rank9(bits, rank, rel, e.x)] = e.y;
+// One wave per mid tile: the tile's 8 wave regions (see k_star_mid) are taken as one list,
+// each lane holding up to SC_Q entries with all their reads in flight together.  Entry
+// {code, row} makes row the mid row of group rank(code); the group's accumulators are
+// zeroed here too (every group index below the passing-key count is written exactly once).
+constexpr int SC_Q = 8;
+__global__ void __launch_bounds__(64) k_bm_scatter(const uint2 *__restrict__ rent, const uint32_t *__restrict__ counts,
+                                                  const uint32_t *__restrict__ bits,
+                                                  const uint64_t *__restrict__ rank,
+                                                  const uint64_t *__restrict__ rel, uint32_t *__restrict__ grow,
+                                                  unsigned long long *__restrict__ sum,
+                                                  unsigned int *__restrict__ cnt) {
+  constexpr int R = ST_BLOCK / 64;  // regions per tile
+  const uint64_t tile = blockIdx.x;
+  uint32_t off[R + 1];
+  off[0] = 0;
+#pragma unroll
+  for (int r = 0; r < R; ++r) off[r + 1] = off[r] + counts[tile * R + r];
+  const uint32_t total = off[R];
+  for (uint32_t base = 0; base < total; base += 64 * SC_Q) {
+    uint2 e[SC_Q];
+#pragma unroll
+    for (int q = 0; q < SC_Q; ++q) {
+      const uint32_t idx = base + q * 64 + threadIdx.x;
+      uint32_t r = 0;
+#pragma unroll
+      for (int k = 1; k < R; ++k) r += idx >= off[k];
+      const uint2 v = rent[(tile * R + r) * PL_WAVE_ROWS + (idx < total ? idx - off[r] : 0u)];  // in bounds
+      e[q] = idx < total ? v : make_uint2(0xFFFFFFFFu, 0u);
+    }
+    // every lane computes a rank for every q (an empty entry uses code 0) so that the
+    // directory reads of all q are in flight together; only real entries store
+    uint32_t g[SC_Q];
+#pragma unroll
+    for (int q = 0; q < SC_Q; ++q) g[q] = bm_rank9(bits, rank, rel, e[q].x != 0xFFFFFFFFu ? e[q].x : 0u);
+#pragma unroll
+    for (int q = 0; q < SC_Q; ++q)
+      if (e[q].x != 0xFFFFFFFFu) {
+        grow[g[q]] = e[q].y;
+        sum[g[q]] = 0;
+        cnt[g[q]] = 0;
+      }
   }
 }
 
@@ -852,7 +885,11 @@ struct FactArgs {
   unsigned long long *sum;
   unsigned int *cnt;
   uint32_t *matched, *probed;  // per wave: index tile * 8 + wave
+  uint2 *hitbuf;               // per wave FACT_HIT_CAP slots {code, row in segment}, or null
 };
+// A wave's first FACT_HIT_CAP joined rows are left in its slots for k_fact_resolve (the
+// scan does not wait on their payload reads); the rest (rare) are resolved in place.
+constexpr uint32_t FACT_HIT_CAP = 16;
 
 // Predicate and key of a staged fast tile (after the barrier that follows staging), as codes
 // into the mid bitmap (~0: no probe); returns the wave's probe count.
@@ -903,8 +940,18 @@ __device__ __forceinline__ void fact_probe_hits(const SegPlan &P, const void *an
 #pragma unroll
   for (int j = 0; j < ST_ITEMS; ++j) {
     const bool h = (code[j] != 0xFFFFFFFFu) & ((word[j] >> (code[j] & 31)) & 1);
-    hits |= (uint32_t)h << j;
-    nmatch += (uint32_t)__popcll(__builtin_amdgcn_ballot_w64(h));
+    const uint64_t b = __builtin_amdgcn_ballot_w64(h);
+    if (A.hitbuf) {  // slot of this row among the wave's joined rows; the first slots are deferred
+      const uint32_t pos = nmatch + (uint32_t)__popcll(b & lanemask_lt());
+      if (h && pos < FACT_HIT_CAP) {
+        A.hitbuf[(uint64_t)cidx * FACT_HIT_CAP + pos] = make_uint2(code[j], r0 + j * ST_BLOCK + threadIdx.x);
+      } else {
+        hits |= (uint32_t)h << j;
+      }
+    } else {
+      hits |= (uint32_t)h << j;
+    }
+    nmatch += (uint32_t)__popcll(b);
   }
   const ColReader rx = col_reader(P.col[2], any), ry = col_reader(P.col[3], any);
   while (hits) {
@@ -924,6 +971,55 @@ __device__ __forceinline__ void fact_probe_hits(const SegPlan &P, const void *an
   if (__lane_id() == 0) {
     A.matched[cidx] = nmatch;
     A.probed[cidx] = nprobe;
+  }
+}
+
+// The deferred joined rows: one thread per slot of every fact wave (slot s of wave w holds
+// a row when s < min(matched[w], FACT_HIT_CAP)): group index = rank of the key, the payload
+// columns of the row, then SUM and COUNT.  All slots' reads are independent, so the payload's
+// HBM latency overlaps across the whole grid instead of stalling the scan.  A fact wave's
+// slots hold its joined rows in row order, and a group's rows are adjacent (the fact table
+// is clustered on the key), so runs of equal group index in adjacent lanes are summed with
+// a segmented wave scan first: one atomic pair per run.
+__global__ void __launch_bounds__(256) k_fact_resolve(const uint2 *__restrict__ hitbuf, uint64_t nwaves,
+                                                      uint32_t tps, const SegPlan *__restrict__ plan, FactArgs A) {
+  const uint64_t t = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  const uint64_t w = t / FACT_HIT_CAP;
+  if (w >= nwaves) return;  // whole waves (nwaves * FACT_HIT_CAP is a multiple of 64)
+  // no early exit: the slot's count, entry and segment plan are read together, then the
+  // rank and payload words together (an unused slot reads key code 0 and row 0 instead of
+  // its stale entry, so every read stays in bounds), and only used slots add
+  const uint32_t s = (uint32_t)(t % FACT_HIT_CAP), n = A.matched[w];
+  const uint2 e = hitbuf[t];
+  // a wave's 64 slots belong to fact waves of one tile: its segment (and plan) is uniform
+  static_assert((FACT_HIT_CAP * (ST_BLOCK / 64)) % 64 == 0, "a resolve wave stays inside one fact tile");
+  const uint32_t tile = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(t / (FACT_HIT_CAP * (ST_BLOCK / 64))));
+  const SegPlan &P = plan[tile / tps];
+  const ColReader rx = col_reader(P.col[2], plan), ry = col_reader(P.col[3], plan);
+  const bool used = s < n;
+  const uint32_t code = used ? e.x : 0u, row = used ? e.y : 0u;
+  const uint32_t g = used ? bm_rank9(A.bits, A.rank, A.rel, code) : 0xFFFFFFFFu;
+  const int64_t x = col_read(rx, row), y = col_read(ry, row);
+  int64_t v = wmul((int64_t)((uint64_t)A.a0 + (uint64_t)wmul(A.a1, x)), (int64_t)((uint64_t)A.b0 + (uint64_t)wmul(A.b1, y)));
+  // segmented inclusive scan over runs of equal g (unused slots, g = ~0, form their own runs)
+  const unsigned lane = __lane_id();
+  uint32_t c = used;
+  const uint32_t prev = __shfl_up(g, 1, 64);  // every lane takes part (not inside a short circuit)
+  bool f = lane == 0 || prev != g;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int64_t vu = __shfl_up(v, o, 64);
+    const uint32_t cu = __shfl_up(c, o, 64);
+    const bool fu = __shfl_up((int)f, o, 64) != 0;
+    if (lane >= (unsigned)o && !f) {
+      v = (int64_t)((uint64_t)v + (uint64_t)vu);
+      c += cu;
+    }
+    if (lane >= (unsigned)o) f = f || fu;
+  }
+  const uint32_t next = __shfl_down(g, 1, 64);
+  if (used && (lane == 63 || next != g)) {
+    atomicAdd(&A.sum[g], (unsigned long long)v);
+    atomicAdd(&A.cnt[g], c);
   }
 }
 
@@ -958,63 +1054,6 @@ __global__ void __launch_bounds__(ST_BLOCK, 8) k_star_fact_bm(ScanDesc d, const 
     nprobe = fact_generic_codes(lds, want, tp, d, P, A, seg, r0, nt, code);
   }
   fact_probe_hits(P, plan, A, r0, code, nprobe, cidx);
-}
-
-// Persistent form: block b takes tiles b, b + G, b + 2G, ... (G = gridDim.x, the blocks the
-// device holds at once) and keeps the next fast tile's staging loads in registers while it
-// decodes, probes and aggregates the current one, so a block's HBM reads overlap its own
-// compute instead of waiting at the start of every tile.
-constexpr int FACT_PP_BLOCKS_PER_CU = 3;  // the prefetch registers need up to 80 VGPRs
-__global__ void __launch_bounds__(ST_BLOCK, 2 * FACT_PP_BLOCKS_PER_CU) k_star_fact_pp(ScanDesc d, const SegPlan *__restrict__ plan,
-                                                           uint32_t ntiles, FactArgs A) {
-  __shared__ __align__(16) uint32_t lds[PL_STAGE_WORDS];
-  __shared__ int want[ALN_MAX_RANGES + 1];
-  __shared__ TilePred tp;
-  // a tile is fast when its segment's plan is 'A' or 'R' and the tile holds rows
-  auto fast_tile_at = [&](uint32_t t, uint32_t &r0, uint32_t &nt) -> bool {
-    const SegPlan &Q = plan[t / d.tps];
-    r0 = (t % d.tps) * ST_TILE;
-    nt = Q.nrows > r0 ? (Q.nrows - r0 < (uint32_t)ST_TILE ? Q.nrows - r0 : (uint32_t)ST_TILE) : 0u;
-    return nt && (Q.mode == 'A' || Q.mode == 'R');
-  };
-  StageRegs<FACT_OWN> sr;
-  uint32_t t = blockIdx.x, r0n, ntn;
-  bool have = t < ntiles && fast_tile_at(t, r0n, ntn);
-  if (have) sr.issue(plan[t / d.tps], r0n, ntn);
-  for (; t < ntiles; t += gridDim.x) {
-    const uint32_t seg = t / d.tps;
-    const SegPlan &P = plan[seg];
-    const uint32_t r0 = (t % d.tps) * ST_TILE, nrows = P.nrows;
-    const uint32_t cidx = t * (ST_BLOCK / 64) + threadIdx.x / 64;
-    const int mode = P.mode;
-    const uint32_t tn = t + gridDim.x;
-    if (r0 >= nrows || mode == 'N') {
-      if (__lane_id() == 0) {
-        A.matched[cidx] = 0;
-        A.probed[cidx] = 0;
-      }
-      have = tn < ntiles && fast_tile_at(tn, r0n, ntn);
-      if (have) sr.issue(plan[tn / d.tps], r0n, ntn);
-      continue;
-    }
-    const uint32_t nt = nrows - r0 < (uint32_t)ST_TILE ? nrows - r0 : (uint32_t)ST_TILE;
-    uint32_t code[ST_ITEMS];
-    uint32_t nprobe;
-    __syncthreads();  // the previous tile's reads of LDS are done
-    if (mode != 'G') {
-      if (!have) sr.issue(P, r0, nt);
-      sr.store(lds);
-      __syncthreads();
-      have = tn < ntiles && fast_tile_at(tn, r0n, ntn);
-      if (have) sr.issue(plan[tn / d.tps], r0n, ntn);  // in flight during this tile's work
-      nprobe = fact_fast_codes(lds, P, A, r0, nt, code);
-    } else {
-      nprobe = fact_generic_codes(lds, want, tp, d, P, A, seg, r0, nt, code);
-      have = tn < ntiles && fast_tile_at(tn, r0n, ntn);
-      if (have) sr.issue(plan[tn / d.tps], r0n, ntn);
-    }
-    fact_probe_hits(P, plan, A, r0, code, nprobe, cidx);
-  }
 }
 
 // ---- T: top-k over groups (sum DESC, g1 ASC, key ASC, g2 ASC) -----------------------------
@@ -1790,16 +1829,14 @@ extern "C" int aln_star_agg(aln_ctx *c, const aln_star_args *a, aln_star_row *h_
   ALN_TRY(ws(c, m ? m : 1, &cnt));
   ALN_TRY(ws(c, m ? m : 1, &grow));
   const GroupAttr ga{nullptr, nullptr, nullptr, grow, mplan, a->mid.seg_rows};
-  ALN_TRY(fill_u64(c, (uint64_t *)sum, m, 0));
-  ALN_TRY(fill_u32(c, cnt, m, 0));
   uint64_t *table = nullptr;
   uint64_t tcap = 0;
   if (use_bm) {
     // the group attributes in rank (= key) order
     if (m) {
       ProfScope p(c, "star_build");
-      hipLaunchKernelGGL(k_bm_scatter, dim3((unsigned)mblocks), dim3(ST_BLOCK), 0, c->stream, rent, mcnt, mbits,
-                         rank, rel, grow);
+      hipLaunchKernelGGL(k_bm_scatter, dim3((unsigned)mblocks), dim3(64), 0, c->stream, rent, mcnt, mbits, rank, rel,
+                         grow, sum, cnt);
     }
     ALN_TRY(check_launch("bm_scatter"));
   } else {
@@ -1812,29 +1849,31 @@ extern "C" int aln_star_agg(aln_ctx *c, const aln_star_args *a, aln_star_row *h_
     if (m) {
       ProfScope p(c, "star_build");
       hipLaunchKernelGGL(k_star_build, dim3((unsigned)mblocks), dim3(ST_BLOCK), 0, c->stream, rent, mcnt, moff,
-                         table, tcap, grow, dup + 1);
+                         table, tcap, grow, sum, cnt, dup + 1);
     }
     ALN_TRY(check_launch("star_build"));
   }
   tm.mark(2);
   if (m && fblocks) {
-    ProfScope p(c, "star_fact");
-    if (use_bm) {
-      const FactArgs fa{a->f_fk, a->a0, a->a1, a->b0, a->b1, mbits, rank, rel, mkmin, mrange, sum, cnt, fmatch, fprobe};
-      if (a->flags & 4) {  // persistent blocks with the next tile's loads in flight
-        int cus = 0;
-        ALN_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
-        const uint64_t g = std::min<uint64_t>(fblocks, (uint64_t)cus * FACT_PP_BLOCKS_PER_CU);
-        hipLaunchKernelGGL(k_star_fact_pp, dim3((unsigned)g), dim3(ST_BLOCK), 0, c->stream, fd, fplan,
-                           (uint32_t)fblocks, fa);
-      } else {
+    {
+      ProfScope p(c, "star_fact");  // the scan and, for the bitmap join, the deferred rows
+      if (use_bm) {
+        uint2 *hitbuf;
+        ALN_TRY(ws(c, fwaves * FACT_HIT_CAP, &hitbuf));
+        const FactArgs fa{a->f_fk, a->a0, a->a1, a->b0, a->b1, mbits, rank, rel, mkmin, mrange, sum, cnt,
+                          fmatch, fprobe, hitbuf};
         hipLaunchKernelGGL(k_star_fact_bm, dim3((unsigned)fblocks), dim3(ST_BLOCK), 0, c->stream, fd, fplan, fa);
+        ALN_TRY(check_launch("star_fact"));
+        const uint64_t nslot = fwaves * FACT_HIT_CAP;
+        hipLaunchKernelGGL(k_fact_resolve, dim3((unsigned)((nslot + 255) / 256)), dim3(256), 0, c->stream, hitbuf,
+                           fwaves, fd.tps, fplan, fa);
+      } else {  // per-block counts
+        hipLaunchKernelGGL(k_star_fact, dim3((unsigned)fblocks), dim3(ST_BLOCK), 0, c->stream, fd, a->f_fk,
+                           a->f_x, a->f_y, a->a0, a->a1, a->b0, a->b1, table, tcap, mkmin, mrange, sum, cnt,
+                           fmatch, fprobe);
       }
-    } else  // per-block counts
-      hipLaunchKernelGGL(k_star_fact, dim3((unsigned)fblocks), dim3(ST_BLOCK), 0, c->stream, fd, a->f_fk,
-                         a->f_x, a->f_y, a->a0, a->a1, a->b0, a->b1, table, tcap, mkmin, mrange, sum, cnt,
-                         fmatch, fprobe);
-    ALN_TRY(check_launch("star_fact"));
+      ALN_TRY(check_launch("star_fact"));
+    }
     const uint64_t nfc = use_bm ? fwaves : fblocks;
     const uint32_t *in[2] = {fmatch, fprobe};
     const uint64_t n[2] = {nfc, nfc};

@@ -336,12 +336,11 @@ def dict_first_seen(ctx, vals, nvals):
 
 
 def star_agg(ctx, dim, mid, fact, spec, limit=10, want_groups=False, groups_cap=None,
-             force_hash=False, persistent=False):
+             force_hash=False):
     """Fused filter-join-join-aggregate-top-k.  dim/mid/fact: dicts with 'cols' (list of
     Column), 'ranges' (list of (col, lo, hi)) and optional 'modes'.  spec: dict of the
     column slots d_key, m_key, m_fk, m_g1, m_g2, f_fk, f_x, f_y and a0, a1, b0, b1."""
-    a, keep = star_args(dim, mid, fact, spec, limit,
-                        (1 if want_groups else 0) | (2 if force_hash else 0) | (4 if persistent else 0))
+    a, keep = star_args(dim, mid, fact, spec, limit, (1 if want_groups else 0) | (2 if force_hash else 0))
     top = (L.StarRow * max(1, limit))()
     ntop = C.c_uint32()
     st = L.StarStats()

@@ -425,8 +425,6 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--sort-rows", type=int, default=100_000_000)
     ap.add_argument("--no-serve", action="store_true", help="skip the served-mix leg")
-    ap.add_argument("--fact-persistent", type=int, default=0,
-                    help="1: fact scan in persistent blocks that prefetch their next tile")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run a plain device copy of 1 GiB (the known-size kernel that scales "
                          "FETCH_SIZE / WRITE_SIZE counters to bytes)")
@@ -455,7 +453,7 @@ def main():
         log("copy calibration: 3 x clone of 1 GiB")
     dim, mid, fact, code_of, nfact = build_tables(ctx, D, args.seed, args.scale, args.seg_rows)
     D_, M_, F_, spec = q3_spec(code_of, dim, mid, fact)
-    run = lambda: D.star_agg(ctx, D_, M_, F_, spec, limit=10, persistent=bool(args.fact_persistent))
+    run = lambda: D.star_agg(ctx, D_, M_, F_, spec, limit=10)
 
     for i in range(args.warmup):
         rows, _, st = run()

@@ -254,9 +254,7 @@ typedef struct {
   uint32_t limit;
   uint32_t flags; /* bit 0: also return every group (d_groups);
                      bit 1: join mid -> fact through the MurmurHash64A open-addressing table
-                     even when the mid key range is dense enough for the bitmap + rank index;
-                     bit 2: fact scan in persistent blocks that prefetch their next tile
-                     (bitmap join only; same results, a scheduling choice) */
+                     even when the mid key range is dense enough for the bitmap + rank index */
 } aln_star_args;
 typedef struct {
   int64_t key, g1, g2, sum, count;

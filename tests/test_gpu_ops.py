@@ -299,7 +299,7 @@ def q3_args(t, dim, mid, fact):
     return D_, M_, F_, spec
 
 
-@pytest.mark.parametrize("join", ["bitmap", "hash", "bitmap-persistent"])
+@pytest.mark.parametrize("join", ["bitmap", "hash"])
 @pytest.mark.parametrize("scale,seg_rows", [(0.02, 4096), (0.1, 65536)])
 def test_star_q3(ctx, scale, seg_rows, join):
     from alenka_amd import device as D
@@ -309,8 +309,7 @@ def test_star_q3(ctx, scale, seg_rows, join):
     top, groups, stats = Q3.run(t)
     dim, mid, fact = encode_q3(ctx, t, seg_rows)
     D_, M_, F_, spec = q3_args(t, dim, mid, fact)
-    rows, g, st = D.star_agg(ctx, D_, M_, F_, spec, limit=10, want_groups=True, force_hash=join == "hash",
-                             persistent=join == "bitmap-persistent")
+    rows, g, st = D.star_agg(ctx, D_, M_, F_, spec, limit=10, want_groups=True, force_hash=join == "hash")
     assert rows == top
     assert st.dim_pass == stats["dim_pass"] and st.mid_pass == stats["mid_pass"]
     assert st.fact_pass == stats["fact_pass"] and st.groups == stats["groups"]
