@@ -505,6 +505,15 @@ def main():
                                args.serve_seconds, args.serve_budget_gb)
         del zc
 
+    # HBM bytes per fact phase, from the two counter passes (FETCH_SIZE, WRITE_SIZE) of this
+    # workload, scaled by a plain copy's counts; the counters cannot be read from inside a run
+    traffic = traffic_src = None
+    tf = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "star_fact_phase_600M_hbm_traffic.json")
+    if (args.scale, args.seg_rows, args.seed) == (100.0, 1000000, 2024) and os.path.exists(tf):
+        with open(tf) as f:
+            traffic = json.load(f)["traffic_bytes"]
+        traffic_src = "profiles/star_fact_phase_600M_hbm_traffic.json (rocprofv3 FETCH_SIZE/WRITE_SIZE, same workload)"
+
     value = nfact / (ms_per_step / 1000.0)
     out = {
         "metric": METRIC,
@@ -531,8 +540,9 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
-            "kernel": "star_fact",
+            "traffic": traffic,
+            "traffic_source": traffic_src,
+            "kernel": "star_fact phase: k_star_fact_bm + k_fact_resolve",
             "kernel_avg_ms": kernels["star_fact"]["avg_ms"],
             "algorithmic_bytes": b_fact,
         },
