@@ -94,9 +94,11 @@ int exclusive_scan_u64(aln_ctx *c, const uint64_t *d_in, uint64_t n, uint64_t *d
                        uint64_t *h_total);
 int minmax_i64(aln_ctx *c, const int64_t *d, uint64_t n, int64_t *h_min, int64_t *h_max);
 int sum_u32(aln_ctx *c, const uint32_t *d_in, uint64_t n, uint64_t *d_out);  // no sync
-// *out[i] += sum of in[i][0, n[i]) for up to SUM_MULTI_MAX arrays, one launch (no sync)
+// *out[i] += sum of in[i][0, n[i]) for up to SUM_MULTI_MAX arrays, one launch (no sync);
+// for the arrays whose bit is set in popc, the number of set bits of the words instead
 constexpr int SUM_MULTI_MAX = 8;
-int sum_u32_multi(aln_ctx *c, int k, const uint32_t *const *in, const uint64_t *n, uint64_t *const *out);
+int sum_u32_multi(aln_ctx *c, int k, const uint32_t *const *in, const uint64_t *n, uint64_t *const *out,
+                  uint32_t popc = 0);
 int radix_sort_pairs(aln_ctx *c, uint64_t *d_keys, uint32_t *d_vals, uint64_t n,
                      uint32_t bits);
 // Batched queries (batch.hip): a batch whose device tables are carved from a caller-owned

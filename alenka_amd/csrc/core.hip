@@ -285,25 +285,37 @@ int sum_u32(aln_ctx *c, const uint32_t *d_in, uint64_t n, uint64_t *d_out) {
 }
 
 // Several sums in one launch: block (x, y) adds a strided share of array y and adds its
-// total to *out[y] (at most 512 atomics per output; the caller zeroes the outputs).
+// total to *out[y] (at most 128 atomics per output; the caller zeroes the outputs).
 struct SumSet {
   const uint32_t *in[SUM_MULTI_MAX];
   uint64_t n[SUM_MULTI_MAX];
   uint64_t *out[SUM_MULTI_MAX];
+  uint32_t popc;  // bit i: add the set bits of array i's words instead of their values
 };
 __global__ void __launch_bounds__(256) k_sum_multi(SumSet s) {
   const uint32_t *in = s.in[blockIdx.y];
   const uint64_t n = s.n[blockIdx.y];
+  const bool bits = (s.popc >> blockIdx.y) & 1;
+  // eight loads in flight per thread and round
   uint64_t v = 0;
-  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) v += in[i];
+  const uint64_t stride = (uint64_t)gridDim.x * 256;
+  for (uint64_t i0 = (uint64_t)blockIdx.x * 256 + threadIdx.x; i0 < n; i0 += 8 * stride) {
+    uint32_t x[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) x[u] = i0 + u * stride < n ? in[i0 + u * stride] : 0u;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v += bits ? (uint32_t)__popc(x[u]) : x[u];
+  }
   uint64_t tot;
   (void)block_excl_scan<uint64_t>(v, &tot);
   if (threadIdx.x == 0 && tot) atomicAdd((unsigned long long *)s.out[blockIdx.y], (unsigned long long)tot);
 }
 
-int sum_u32_multi(aln_ctx *c, int k, const uint32_t *const *in, const uint64_t *n, uint64_t *const *out) {
+int sum_u32_multi(aln_ctx *c, int k, const uint32_t *const *in, const uint64_t *n, uint64_t *const *out,
+                  uint32_t popc) {
   ALN_REQUIRE(k >= 1 && k <= SUM_MULTI_MAX, "sum_u32_multi: %d arrays", k);
   SumSet s;
+  s.popc = popc;
   uint64_t most = 0;
   for (int i = 0; i < k; ++i) {
     s.in[i] = in[i];
@@ -311,7 +323,9 @@ int sum_u32_multi(aln_ctx *c, int k, const uint32_t *const *in, const uint64_t *
     s.out[i] = out[i];
     most = n[i] > most ? n[i] : most;
   }
-  const unsigned nb = grid_for(most ? most : 1, 256 * 8, 512);
+  // few blocks (the outputs usually share a cache line, whose atomics serialise), each
+  // thread with eight loads in flight
+  const unsigned nb = grid_for(most ? most : 1, 256 * 32, 128);
   hipLaunchKernelGGL(k_sum_multi, dim3(nb, k), dim3(256), 0, c->stream, s);
   return check_launch("sum_u32_multi");
 }

@@ -514,35 +514,74 @@ __device__ __forceinline__ void block_count(uint32_t v, uint32_t *out) {
   }
 }
 
-__global__ void __launch_bounds__(ST_BLOCK) k_star_dim(ScanDesc d, int kcol, int64_t kmin,
-                                                       uint64_t range, uint32_t *bitmap,
-                                                       uint32_t *npass, uint32_t *dup) {
-  __shared__ TileStage ts;
+// Plan roles: 0 range, 1 key (staged).  Passing keys set their bit without a returned value;
+// a key seen twice shows as popcount(bitmap) < passing rows.  Counts are per wave:
+// npass[b * 8 + w].
+constexpr uint32_t DIM_STAGE = 1u << 1;
+constexpr int DIM_OWN = 2;
+
+__global__ void __launch_bounds__(ST_BLOCK, 8) k_star_dim(ScanDesc d, const SegPlan *__restrict__ plan, int kcol,
+                                                          int64_t kmin, uint64_t range, uint32_t *__restrict__ bitmap,
+                                                          uint32_t *__restrict__ npass) {
+  static_assert(sizeof(TileStage) <= PL_STAGE_WORDS * sizeof(uint32_t), "generic staging fits the LDS area");
+  __shared__ __align__(16) uint32_t lds[PL_STAGE_WORDS];
   __shared__ int want[ALN_MAX_RANGES + 1];
+  __shared__ TilePred tp;
   const uint32_t seg = blockIdx.x / d.tps, tile = blockIdx.x % d.tps;
-  const uint32_t nrows = d.segs[0][seg].nrows, r0 = tile * ST_TILE;
-  const char mode = sd_mode(d, seg);
+  const SegPlan &P = plan[seg];
+  const uint32_t r0 = tile * ST_TILE, nrows = P.nrows;
+  const uint32_t region = blockIdx.x * (ST_BLOCK / 64) + threadIdx.x / 64;
+  const int mode = P.mode;
   if (r0 >= nrows || mode == 'N') {
-    if (threadIdx.x == 0) npass[blockIdx.x] = 0;
+    if (__lane_id() == 0) npass[region] = 0;
     return;
   }
   const uint32_t nt = nrows - r0 < (uint32_t)ST_TILE ? nrows - r0 : (uint32_t)ST_TILE;
-  __shared__ TilePred tp;
-  star_stage(ts, want, d, mode, kcol, seg, r0, nt, &tp);
-  uint32_t cnt = 0;
-  for (int j = 0; j < ST_ITEMS; ++j) {
-    uint32_t i = j * ST_BLOCK + threadIdx.x;
-    if (i < nt && tile_pass_fast(d, ts, tp, mode, seg, r0, i)) {
-      uint64_t b = (uint64_t)tile_get(ts, d.words, d.segs, kcol, seg, r0, i) - (uint64_t)kmin;
-      if (b < range) {
-        uint32_t bit = 1u << (b & 31);
-        uint32_t old = atomicOr(&bitmap[b >> 5], bit);
-        if (old & bit) atomicExch(dup, 1u);
-        ++cnt;
+  uint32_t n = 0;
+  if (mode != 'G') {
+    stage_plan<DIM_OWN>(lds, P, r0, nt);
+    __syncthreads();
+    const FastTile ft = plan_fast(lds, P, r0);
+    const uint32_t off = (uint32_t)(int32_t)((uint64_t)P.col[1].base - (uint64_t)kmin), lim = (uint32_t)range;
+    uint32_t code[ST_ITEMS];
+    (void)(ft.range ? fast_probe<true>(ft, nt, off, lim, code) : fast_probe<false>(ft, nt, off, lim, code));
+    // keys of adjacent rows are often adjacent: lanes setting bits of the same word OR them
+    // together first (segmented scan over runs of equal word index), one atomic per run
+    const unsigned lane = __lane_id();
+#pragma unroll
+    for (int j = 0; j < ST_ITEMS; ++j) {
+      const bool v = code[j] != 0xFFFFFFFFu;
+      n += (uint32_t)__popcll(__builtin_amdgcn_ballot_w64(v));
+      const uint32_t wi = v ? code[j] >> 5 : 0xFFFFFFFFu;
+      uint32_t bitsv = v ? 1u << (code[j] & 31) : 0u;
+      const uint32_t prev = __shfl_up(wi, 1, 64);
+      bool f = lane == 0 || prev != wi;
+      for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t bu = __shfl_up(bitsv, o, 64);
+        const bool fu = __shfl_up((int)f, o, 64) != 0;
+        if (lane >= (unsigned)o && !f) bitsv |= bu;
+        if (lane >= (unsigned)o) f = f || fu;
+      }
+      const uint32_t next = __shfl_down(wi, 1, 64);
+      if (v && (lane == 63 || next != wi)) atomicOr(&bitmap[wi], bitsv);
+    }
+  } else {
+    TileStage &ts = *reinterpret_cast<TileStage *>(lds);
+    const char zm = (char)P.zmode;
+    star_stage(ts, want, d, zm, kcol, seg, r0, nt, &tp);
+    for (int j = 0; j < ST_ITEMS; ++j) {
+      const uint32_t i = j * ST_BLOCK + threadIdx.x;
+      if (i < nt && tile_pass_fast(d, ts, tp, zm, seg, r0, i)) {
+        const uint64_t b = (uint64_t)tile_get(ts, d.words, d.segs, kcol, seg, r0, i) - (uint64_t)kmin;
+        if (b < range) {
+          atomicOr(&bitmap[b >> 5], 1u << (b & 31));
+          ++n;
+        }
       }
     }
+    n = wave_sum(n);
   }
-  block_count(cnt, npass);
+  if (__lane_id() == 0) npass[region] = n;
 }
 
 // ---- M: mid scan, probe the dimension bitmap, compact passing rows ------------------------
@@ -1739,13 +1778,15 @@ extern "C" int aln_star_agg(aln_ctx *c, const aln_star_args *a, aln_star_row *h_
   uint64_t bwords = (drange + 31) / 32;
   uint32_t *bitmap, *dup;
   // device counters, read back with one copy each time: 0 dim pass, 1 mid pass, 2 mid probe,
-  // 3 fact pass, 4 fact probe, 5 distinct passing mid keys, 6 duplicate flags (two u32)
+  // 3 fact pass, 4 fact probe, 5 distinct passing mid keys, 6 duplicate flags (two u32; the
+  // hash join's build sets the second), 7 distinct passing dimension keys
   uint64_t *ctr;
   uint32_t *dcnt, *mcnt, *mprobe, *fmatch, *fprobe;
   ALN_TRY(ws(c, bwords, &bitmap));
   ALN_TRY(ws(c, 8, &ctr));
   dup = reinterpret_cast<uint32_t *>(ctr + 6);
-  ALN_TRY(ws(c, dblocks, &dcnt));
+  const uint64_t dregions = dblocks * (ST_BLOCK / 64);
+  ALN_TRY(ws(c, dregions, &dcnt));
   const uint64_t mregions = mblocks * (ST_BLOCK / 64), fwaves = fblocks * (ST_BLOCK / 64);
   ALN_TRY(ws(c, mregions, &mcnt));
   ALN_TRY(ws(c, mregions, &mprobe));
@@ -1753,10 +1794,14 @@ extern "C" int aln_star_agg(aln_ctx *c, const aln_star_args *a, aln_star_row *h_
   ALN_TRY(ws(c, fwaves, &fprobe));
   ALN_TRY(fill_u32(c, bitmap, bwords, 0));
   ALN_TRY(fill_u64(c, ctr, 8, 0));
-  // per-segment plans of the mid and fact scans
-  SegPlan *mplan, *fplan = nullptr;
+  // per-segment plans of the three scans
+  SegPlan *dplan, *mplan, *fplan = nullptr;
   {
-    const uint32_t ms = a->mid.cols[0].nsegs, fs = a->fact.cols[0].nsegs;
+    const uint32_t ds = a->dim.cols[0].nsegs, ms = a->mid.cols[0].nsegs, fs = a->fact.cols[0].nsegs;
+    ALN_TRY(ws(c, ds, &dplan));
+    hipLaunchKernelGGL(k_scan_plan, dim3((ds + 127) / 128), dim3(128), 0, c->stream, dd, a->d_key, -1, -1, -1,
+                       DIM_STAGE, dkmin, (int)(drange <= (1ull << 31)), (uint32_t)DIM_OWN, ds, dplan);
+    ALN_TRY(check_launch("scan_plan"));
     ALN_TRY(ws(c, ms, &mplan));
     hipLaunchKernelGGL(k_scan_plan, dim3((ms + 127) / 128), dim3(128), 0, c->stream, md, a->m_fk, a->m_key, a->m_g1,
                        a->m_g2, MID_STAGE, dkmin, (int)(drange <= (1ull << 31)), (uint32_t)PL_OWN, ms, mplan);
@@ -1772,8 +1817,8 @@ extern "C" int aln_star_agg(aln_ctx *c, const aln_star_args *a, aln_star_row *h_
   tm.mark(0);
   {
     ProfScope p(c, "star_dim");
-    hipLaunchKernelGGL(k_star_dim, dim3((unsigned)dblocks), dim3(ST_BLOCK), 0, c->stream, dd, a->d_key,
-                       dkmin, drange, bitmap, dcnt, dup + 0);
+    hipLaunchKernelGGL(k_star_dim, dim3((unsigned)dblocks), dim3(ST_BLOCK), 0, c->stream, dd, dplan, a->d_key,
+                       dkmin, drange, bitmap, dcnt);
   }
   ALN_TRY(check_launch("star_dim"));
   tm.mark(1);
@@ -1805,17 +1850,18 @@ extern "C" int aln_star_agg(aln_ctx *c, const aln_star_args *a, aln_star_row *h_
     ALN_TRY(check_launch("bm_block_counts"));
     ALN_TRY(exclusive_scan_u32(c, bcnt, mblk, rank, nullptr));
   }
-  {  // the counters: dim pass, mid pass, mid probe and (bitmap join) distinct mid keys
-    const uint32_t *in[4] = {dcnt, mcnt, mprobe, bcnt};
-    const uint64_t n[4] = {dblocks, mregions, mregions, mblk};
-    uint64_t *out[4] = {ctr + 0, ctr + 1, ctr + 2, ctr + 5};
-    ALN_TRY(sum_u32_multi(c, use_bm ? 4 : 3, in, n, out));
+  {  // the counters: dim pass, mid pass, mid probe, distinct dim keys (set bits of the
+     // dimension bitmap) and (bitmap join) distinct mid keys
+    const uint32_t *in[5] = {dcnt, mcnt, mprobe, bitmap, bcnt};
+    const uint64_t n[5] = {dregions, mregions, mregions, bwords, mblk};
+    uint64_t *out[5] = {ctr + 0, ctr + 1, ctr + 2, ctr + 7, ctr + 5};
+    ALN_TRY(sum_u32_multi(c, use_bm ? 5 : 4, in, n, out, 1u << 3));
   }
   uint64_t hc[8];
   ALN_TRY(copy_to_host(c, hc, ctr, sizeof(hc)));
   const uint32_t *hdup = reinterpret_cast<const uint32_t *>(hc + 6);
   distinct = hc[5];
-  if (hdup[0]) return set_error(ALN_E_UNSUPPORTED, "aln_star_agg: duplicate dimension keys");
+  if (hc[7] != hc[0]) return set_error(ALN_E_UNSUPPORTED, "aln_star_agg: duplicate dimension keys");
   if (hdup[1] || (use_bm && distinct != hc[1]))
     return set_error(ALN_E_UNSUPPORTED, "aln_star_agg: duplicate mid keys");
   st.dim_pass = hc[0];

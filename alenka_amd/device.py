@@ -358,6 +358,30 @@ def star_agg(ctx, dim, mid, fact, spec, limit=10, want_groups=False, groups_cap=
     return rows, g, st
 
 
+class StarQuery:
+    """A prepared star_agg call: the argument block is built once and each run() is one
+    C call (for repeated execution of the same query, e.g. a benchmark loop)."""
+
+    def __init__(self, ctx, dim, mid, fact, spec, limit=10):
+        self.ctx = ctx
+        self.args, self._keep = star_args(dim, mid, fact, spec, limit, 0)
+        self.limit = limit
+        self.top = (L.StarRow * max(1, limit))()
+        self.ntop = C.c_uint32()
+        self.st = L.StarStats()
+        self._fn = ctx.lib.aln_star_agg
+        self._a = C.byref(self.args)
+        self._nt = C.byref(self.ntop)
+        self._s = C.byref(self.st)
+
+    def run(self):
+        """Returns (rows, None, stats) as star_agg does (no group output)."""
+        L.check(self._fn(self.ctx.h, self._a, self.top, self._nt, None, 0, self._s), "aln_star_agg")
+        rows = [(self.top[i].key, self.top[i].g1, self.top[i].g2, self.top[i].sum, self.top[i].count)
+                for i in range(self.ntop.value)]
+        return rows, None, self.st
+
+
 def star_agg_batch(ctx, queries, limit=10):
     """Several star pipelines over the same tables in one pass of each table.  queries:
     list of (dim, mid, fact, spec) as for star_agg.  Returns a list of (rows, stats)."""

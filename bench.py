@@ -453,7 +453,7 @@ def main():
         log("copy calibration: 3 x clone of 1 GiB")
     dim, mid, fact, code_of, nfact = build_tables(ctx, D, args.seed, args.scale, args.seg_rows)
     D_, M_, F_, spec = q3_spec(code_of, dim, mid, fact)
-    run = lambda: D.star_agg(ctx, D_, M_, F_, spec, limit=10)
+    run = D.StarQuery(ctx, D_, M_, F_, spec, limit=10).run
 
     for i in range(args.warmup):
         rows, _, st = run()
