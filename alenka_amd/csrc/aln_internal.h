@@ -46,9 +46,15 @@ struct aln_ctx {
   };
   std::vector<Chunk> chunks;
   size_t cur = 0, off = 0;
-  // pinned host staging for small readbacks
+  // pinned host staging for readbacks
   void *pinned = nullptr;
   size_t pinned_size = 0;
+  // small readbacks: a kernel writes the bytes and then a sequence number into coherent
+  // host memory, and the host polls the number (no stream synchronisation)
+  void *rb = nullptr;
+  uint32_t *rb_dev = nullptr;
+  uint32_t rb_seq = 0;
+  bool rb_failed = false;
   // launch timing
   bool profiling = false;
   struct Pending {

@@ -4,6 +4,9 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <atomic>
+#include <chrono>
+#include <thread>
 #include <vector>
 
 #include "aln_internal.h"
@@ -80,8 +83,70 @@ int pinned(aln_ctx *c, size_t bytes, void **out) {
   return ALN_OK;
 }
 
+// Small readbacks without a stream synchronisation: one block copies the words into
+// coherent (fine-grained) host memory, makes them visible system-wide, then stores `seq`
+// after them; the host polls that word.  The wait on a blocking stream synchronisation
+// (the host thread's wake-up after the GPU finishes) is what this avoids.
+constexpr size_t RB_BYTES = 65536;
+__global__ void __launch_bounds__(256) k_readback(const uint32_t *__restrict__ src, uint32_t nwords,
+                                                 uint32_t *__restrict__ dst, uint32_t *flag, uint32_t seq) {
+  for (uint32_t i = threadIdx.x; i < nwords; i += 256) dst[i] = src[i];
+  __threadfence_system();
+  __syncthreads();
+  if (threadIdx.x == 0) __hip_atomic_store(flag, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+static bool readback_ready(aln_ctx *c) {
+  if (c->rb) return true;
+  if (c->rb_failed) return false;
+  void *p = nullptr;
+  uint32_t *dp = nullptr;
+  if (hipHostMalloc(&p, RB_BYTES + 64, hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess) {
+    (void)hipGetLastError();
+    c->rb_failed = true;
+    return false;
+  }
+  if (hipHostGetDevicePointer((void **)&dp, p, 0) != hipSuccess) {
+    (void)hipGetLastError();
+    (void)hipHostFree(p);
+    c->rb_failed = true;
+    return false;
+  }
+  memset(p, 0, RB_BYTES + 64);
+  c->rb = p;
+  c->rb_dev = dp;
+  return true;
+}
+
+static int readback_small(aln_ctx *c, void *h, const void *d, size_t bytes) {
+  uint32_t seq = ++c->rb_seq;
+  if (seq == 0) seq = ++c->rb_seq;  // 0 is the buffer's initial value
+  volatile uint32_t *flag = reinterpret_cast<volatile uint32_t *>(static_cast<char *>(c->rb) + RB_BYTES);
+  hipLaunchKernelGGL(k_readback, dim3(1), dim3(256), 0, c->stream, static_cast<const uint32_t *>(d),
+                     (uint32_t)((bytes + 3) / 4), c->rb_dev, c->rb_dev + RB_BYTES / 4, seq);
+  ALN_TRY(check_launch("readback"));
+  // poll; after a while give the thread up between polls, and after 200 ms fall back to a
+  // stream synchronisation (which also reports a failed launch)
+  const auto t0 = std::chrono::steady_clock::now();
+  for (uint64_t spins = 0; *flag != seq; ++spins) {
+    if ((spins & 1023) == 1023) {
+      const double us = std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
+      if (us > 200000.0) {
+        ALN_HIP(hipStreamSynchronize(c->stream));
+        if (*flag != seq) return set_error(ALN_E_HIP, "readback: the copy kernel did not complete");
+        break;
+      }
+      if (us > 50.0) std::this_thread::yield();
+    }
+  }
+  std::atomic_thread_fence(std::memory_order_acquire);
+  memcpy(h, c->rb, bytes);
+  return ALN_OK;
+}
+
 int copy_to_host(aln_ctx *c, void *h, const void *d, size_t bytes) {
   if (bytes == 0) return ALN_OK;
+  if (bytes <= RB_BYTES && ((uintptr_t)d & 3) == 0 && readback_ready(c)) return readback_small(c, h, d, bytes);
   void *p;
   ALN_TRY(pinned(c, bytes, &p));
   ALN_HIP(hipMemcpyAsync(p, d, bytes, hipMemcpyDeviceToHost, c->stream));
@@ -450,6 +515,7 @@ void aln_destroy(aln_ctx *c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (auto &ch : c->chunks) (void)hipFree(ch.ptr);
   if (c->pinned) (void)hipHostFree(c->pinned);
+  if (c->rb) (void)hipHostFree(c->rb);
   for (auto &p : c->pending) {
     (void)hipEventDestroy(p.a);
     (void)hipEventDestroy(p.b);

@@ -1578,9 +1578,12 @@ __global__ void k_topk_emit(const uint32_t *__restrict__ j, uint32_t k,
 // The first k groups by (sum DESC, g1 ASC, key ASC, g2 ASC) (synchronises).  Small k with a
 // g1 range below 2^32 take the register selection above; otherwise tiles of 1024
 // candidates are sorted and cut to k, repeatedly, until one tile remains.
+// With ctr (nctr device counters followed by room for the result block), the threshold
+// path reads the counters and its result with one copy into h_ctr and sets *ctr_read.
 static int star_topk(aln_ctx *c, uint64_t m, const unsigned long long *sum, const unsigned int *cnt,
                      const GroupAttr &ga, uint32_t k, int64_t g1min, int64_t g1max, int64_t kmin,
-                     aln_star_row *h_top, uint32_t *h_ntop) {
+                     aln_star_row *h_top, uint32_t *h_ntop, uint64_t *ctr = nullptr, uint32_t nctr = 0,
+                     uint64_t *h_ctr = nullptr, bool *ctr_read = nullptr) {
   if (k <= (uint32_t)TS_K && g1max >= g1min && (uint64_t)g1max - (uint64_t)g1min < (1ull << 32) &&
       m < (1ull << 32)) {
     // threshold path: block maxima -> k-th largest -> the groups at or above it
@@ -1600,7 +1603,10 @@ static int star_topk(aln_ctx *c, uint64_t m, const unsigned long long *sum, cons
         ALN_TRY(ws(c, k, &oh));
         ALN_TRY(ws(c, k, &ol));
         ALN_TRY(ws(c, k, &oj));
-        ALN_TRY(ws(c, 2 + k * sizeof(aln_star_row) / 4, &out));
+        if (ctr)
+          out = reinterpret_cast<uint32_t *>(ctr + nctr);
+        else
+          ALN_TRY(ws(c, 2 + k * sizeof(aln_star_row) / 4, &out));
         uint32_t *count = out, *nv = out + 1;
         aln_star_row *rows = reinterpret_cast<aln_star_row *>(out + 2);
         ALN_TRY(fill_u32(c, out, 2, 0));
@@ -1616,7 +1622,15 @@ static int star_topk(aln_ctx *c, uint64_t m, const unsigned long long *sum, cons
         }
         ALN_TRY(check_launch("star_topk"));
         std::vector<uint32_t> hres(2 + k * sizeof(aln_star_row) / 4);
-        ALN_TRY(copy_to_host(c, hres.data(), out, 4 * hres.size()));
+        if (ctr) {  // the counters and the result with one copy
+          std::vector<uint64_t> hb(nctr + (4 * hres.size() + 7) / 8);
+          ALN_TRY(copy_to_host(c, hb.data(), ctr, 8 * hb.size()));
+          memcpy(h_ctr, hb.data(), 8 * nctr);
+          memcpy(hres.data(), hb.data() + nctr, 4 * hres.size());
+          *ctr_read = true;
+        } else {
+          ALN_TRY(copy_to_host(c, hres.data(), out, 4 * hres.size()));
+        }
         if (hres[0] <= CAP) {
           memcpy(h_top, hres.data() + 2, sizeof(aln_star_row) * hres[1]);
           *h_ntop = hres[1];
@@ -1783,7 +1797,8 @@ extern "C" int aln_star_agg(aln_ctx *c, const aln_star_args *a, aln_star_row *h_
   uint64_t *ctr;
   uint32_t *dcnt, *mcnt, *mprobe, *fmatch, *fprobe;
   ALN_TRY(ws(c, bwords, &bitmap));
-  ALN_TRY(ws(c, 8, &ctr));
+  // (room after the 8 counters for the top-k result block: 2 u32 + limit rows)
+  ALN_TRY(ws(c, 8 + (8 + (uint64_t)a->limit * sizeof(aln_star_row) + 7) / 8, &ctr));
   dup = reinterpret_cast<uint32_t *>(ctr + 6);
   const uint64_t dregions = dblocks * (ST_BLOCK / 64);
   ALN_TRY(ws(c, dregions, &dcnt));
@@ -1929,9 +1944,11 @@ extern "C" int aln_star_agg(aln_ctx *c, const aln_star_args *a, aln_star_row *h_
   tm.mark(3);
 
   // top-k
-  if (a->limit && m) ALN_TRY(star_topk(c, m, sum, cnt, ga, a->limit, g1min, g1max, mkmin, h_top, h_ntop));
+  bool ctr_read = false;
+  if (a->limit && m)
+    ALN_TRY(star_topk(c, m, sum, cnt, ga, a->limit, g1min, g1max, mkmin, h_top, h_ntop, ctr, 8, hc, &ctr_read));
   tm.mark(4);
-  ALN_TRY(copy_to_host(c, hc, ctr, sizeof(hc)));
+  if (!ctr_read) ALN_TRY(copy_to_host(c, hc, ctr, sizeof(hc)));
   if (hdup[1]) return set_error(ALN_E_UNSUPPORTED, "aln_star_agg: duplicate mid keys");
   st.fact_pass = hc[3];
   st.fact_probe = hc[4];
