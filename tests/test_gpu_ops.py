@@ -321,6 +321,70 @@ def test_star_q3(ctx, scale, seg_rows, join):
     assert np.array_equal(host(g), groups)
 
 
+def star_expected(t, dr, mr, fr, limit=10):
+    """Oracle answer of a star query with range lists (col, lo, hi) on the q3 tables: the same
+    sort-merge joins and sort-based group-by as the q3 reference run, SUM(price*(100-disc))."""
+    cols = dict(d=[t["c_key"], t["c_seg"]], m=[t["o_key"], t["o_cust"], t["o_date"], t["o_prio"]],
+                f=[t["l_okey"], t["l_price"], t["l_disc"], t["l_ship"]])
+
+    def ids(k, ranges):
+        m = np.ones(len(cols[k][0]), bool)
+        for c, lo, hi in ranges:
+            m &= (cols[k][c] >= lo) & (cols[k][c] <= hi)
+        return O.compact(m)
+
+    c_ids, o_ids, l_ids = ids("d", dr), ids("m", mr), ids("f", fr)
+    pl, _ = O.join_sortmerge(t["o_cust"][o_ids], t["c_key"][c_ids])
+    olc1 = o_ids[pl]
+    ql, qr = O.join_sortmerge(t["l_okey"][l_ids], t["o_key"][olc1])
+    lrow, orow = l_ids[ql], olc1[qr]
+    rev = t["l_price"][lrow] * (100 - t["l_disc"][lrow])
+    gk, gv = O.groupby([t["o_key"][orow], t["o_date"][orow], t["o_prio"][orow]], [rev, rev],
+                       [L.AGG_SUM, L.AGG_COUNT])
+    perm = O.order([gv[0], gk[1]], [1, 0])
+    top = [(int(gk[0][i]), int(gk[1][i]), int(gk[2][i]), int(gv[0][i]), int(gv[1][i])) for i in perm[:limit]]
+    groups = np.stack([gk[0], gk[1], gk[2], gv[0], gv[1]], axis=1) if len(gk[0]) else np.zeros((0, 5), np.int64)
+    return top, groups, len(c_ids), len(olc1), len(lrow)
+
+
+@pytest.mark.parametrize("join", ["bitmap", "hash"])
+@pytest.mark.parametrize("case", ["wide", "multi", "zone"])
+def test_star_scan_shapes(ctx, case, join):
+    """Segments the scans' fast decoder cannot take (codes wider than 31 bits, several ranges
+    on a table) run the generic staged decoder, and zone maps skip ('N') or pass ('A') whole
+    segments; every shape gives the oracle's answer."""
+    from alenka_amd import device as D
+
+    t = Q3.tables(77, 3000, 30000)
+    seg_rows = 4096
+    enc = lambda a, d=0, fb=0: D.Column.encode(ctx, dev(ctx, a), seg_rows, decimals=d, force_bits=fb)
+    wide = case == "wide"
+    dim = [enc(t["c_key"], fb=36 if wide else 0), enc(t["c_seg"])]
+    mid = [enc(t["o_key"], fb=40 if wide else 0), enc(t["o_cust"], fb=36 if wide else 0), enc(t["o_date"]),
+           enc(t["o_prio"])]
+    fact = [enc(t["l_okey"], fb=40 if wide else 0), enc(t["l_price"], 2), enc(t["l_disc"], 2), enc(t["l_ship"])]
+    building = int(t["code_of"][Q3.SEGMENTS.index("BUILDING")])
+    lo, hi = -(1 << 63), (1 << 63) - 1
+    if case == "multi":
+        dr = [(1, building, building), (0, lo, 2400)]
+        mr = [(2, lo, Q3.DATE - 1), (2, 19930101, hi)]
+        fr = [(3, Q3.DATE + 1, hi), (2, 2, 8)]
+    elif case == "zone":  # fact and mid ranges on the (clustered) order key itself
+        kmid = int(np.median(t["o_key"]))
+        dr = [(1, building, building)]
+        mr = [(0, lo, kmid)]
+        fr = [(0, int(t["l_okey"][len(t["l_okey"]) // 5]), kmid)]
+    else:
+        dr, mr, fr = [(1, building, building)], [(2, lo, Q3.DATE - 1)], [(3, Q3.DATE + 1, hi)]
+    top, groups, dpass, mpass, fpass = star_expected(t, dr, mr, fr)
+    spec = dict(d_key=0, m_key=0, m_fk=1, m_g1=2, m_g2=3, f_fk=0, f_x=1, f_y=2, a0=0, a1=1, b0=100, b1=-1)
+    rows, g, st = D.star_agg(ctx, dict(cols=dim, ranges=dr), dict(cols=mid, ranges=mr), dict(cols=fact, ranges=fr),
+                             spec, limit=10, want_groups=True, force_hash=join == "hash")
+    assert rows == top
+    assert (st.dim_pass, st.mid_pass, st.fact_pass) == (dpass, mpass, fpass)
+    assert np.array_equal(host(g), groups)
+
+
 @pytest.mark.parametrize("scale,seg_rows", [(0.02, 4096), (0.1, 65536)])
 def test_star_q3_batch(ctx, scale, seg_rows):
     """q3 with different substitution parameters (segment, date) answered by one pass of each
