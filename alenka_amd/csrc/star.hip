@@ -264,18 +264,26 @@ __host__ __device__ constexpr uint32_t plan_slot_words(uint32_t bits) {
 }
 static_assert(plan_slot_words(STG_MAX_BITS) == STG_SLOT_WORDS, "slot sizes agree with stage.h");
 
-// rc1..rc4: the columns of roles 1..4 (-1: none); roles whose bit is set in `stage` are
-// staged when their column is not constant.  kmin: the probe key's code is key - kmin, and
-// the fast path needs base - kmin in 32 bits.  fast = false plans every live segment 'G';
-// a segment needing more than max_own (<= PL_OWN) staged columns is 'G' too.
-__global__ void k_scan_plan(ScanDesc d, int rc1, int rc2, int rc3, int rc4, uint32_t stage, int64_t kmin,
-                            int fast, uint32_t max_own, uint32_t nsegs, SegPlan *__restrict__ out) {
-  const uint32_t seg = blockIdx.x * blockDim.x + threadIdx.x;
-  if (seg >= nsegs) return;
+// One table's plan request.  rc[1..4]: the columns of roles 1..4 (-1: none); roles whose bit
+// is set in `stage` are staged when their column is not constant.  kmin: the probe key's
+// code is key - kmin, and the fast path needs base - kmin in 32 bits.  fast = 0 plans every
+// live segment 'G'; a segment needing more than max_own (<= PL_OWN) staged columns is 'G'.
+struct PlanJob {
+  int rc[PL_ROLES];
+  uint32_t stage;
+  int64_t kmin;
+  int fast;
+  uint32_t max_own, nsegs;
+  SegPlan *out;
+};
+
+__device__ void plan_segment(const ScanDesc &d, const PlanJob &j, uint32_t seg) {
   SegPlan P;
   const char zm = sd_mode(d, seg);
-  const int rc[PL_ROLES] = {d.nr == 1 ? d.rcol[0] : -1, rc1, rc2, rc3, rc4};
-  bool generic = !fast || d.nr > 1;
+  const int rc1 = j.rc[1], stage = (int)j.stage;
+  const int64_t kmin = j.kmin;
+  const int rc[PL_ROLES] = {d.nr == 1 ? d.rcol[0] : -1, j.rc[1], j.rc[2], j.rc[3], j.rc[4]};
+  bool generic = !j.fast || d.nr > 1;
   uint32_t words = 0, nown = 0;
 #pragma unroll
   for (int k = 0; k < PL_OWN; ++k) P.st[k] = StageSrc{nullptr, 0u, 0};
@@ -309,7 +317,7 @@ __global__ void k_scan_plan(ScanDesc d, int rc1, int rc2, int rc3, int rc4, uint
     }
     pc.slot = slot;
   }
-  if (words > (uint32_t)PL_STAGE_WORDS || nown > max_own) generic = true;
+  if (words > (uint32_t)PL_STAGE_WORDS || nown > j.max_own) generic = true;
   // the fast probe reads a staged, non-constant key and shifts it in 32 bits
   if (rc1 >= 0) {
     const int64_t off = (int64_t)((uint64_t)P.col[1].base - (uint64_t)kmin);
@@ -326,7 +334,35 @@ __global__ void k_scan_plan(ScanDesc d, int rc1, int rc2, int rc3, int rc4, uint
     P.cspan = cr.cspan;
   }
   P.mode = mode;
-  out[seg] = P;
+  j.out[seg] = P;
+}
+
+// Zero fills of up to 4 word arrays, with the plans, in one launch.
+struct FillJob {
+  uint32_t *p[4];
+  uint64_t n[4];
+};
+
+// Per-query setup of the star pipeline in one launch: the first plan_blocks blocks plan the
+// segments of the three scans (one thread per segment), the others zero the listed arrays.
+__global__ void __launch_bounds__(128) k_star_setup(ScanDesc d0, ScanDesc d1, ScanDesc d2, PlanJob j0, PlanJob j1,
+                                                    PlanJob j2, uint32_t plan_blocks, FillJob f) {
+  if (blockIdx.x < plan_blocks) {
+    uint32_t i = blockIdx.x * 128 + threadIdx.x;
+    if (i < j0.nsegs) {
+      plan_segment(d0, j0, i);
+    } else if ((i -= j0.nsegs) < j1.nsegs) {
+      plan_segment(d1, j1, i);
+    } else if ((i -= j1.nsegs) < j2.nsegs) {
+      plan_segment(d2, j2, i);
+    }
+    return;
+  }
+  const uint64_t t0 = (uint64_t)(blockIdx.x - plan_blocks) * 128 + threadIdx.x;
+  const uint64_t stride = (uint64_t)(gridDim.x - plan_blocks) * 128;
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    for (uint64_t i = t0; i < f.n[k]; i += stride) f.p[k][i] = 0;
 }
 
 // A pointer read from a plan, marked as global memory: the compiler cannot tell where a
@@ -1785,10 +1821,8 @@ extern "C" int aln_star_agg(aln_ctx *c, const aln_star_args *a, aln_star_row *h_
   uint64_t mwords = use_bm ? (mrange + 31) / 32 : 0;
   uint64_t mblk = use_bm ? (mwords + BM_BLOCK_WORDS - 1) / BM_BLOCK_WORDS : 0;
   uint32_t *mbits = nullptr;
-  if (use_bm) {  // whole 512-bit blocks: the rank reads a block with four 16-byte loads
+  if (use_bm)  // whole 512-bit blocks (zeroed by the setup launch below)
     ALN_TRY(ws(c, mblk * BM_BLOCK_WORDS, &mbits));
-    ALN_TRY(fill_u32(c, mbits, mblk * BM_BLOCK_WORDS, 0));
-  }
   uint64_t bwords = (drange + 31) / 32;
   uint32_t *bitmap, *dup;
   // device counters, read back with one copy each time: 0 dim pass, 1 mid pass, 2 mid probe,
@@ -1807,26 +1841,25 @@ extern "C" int aln_star_agg(aln_ctx *c, const aln_star_args *a, aln_star_row *h_
   ALN_TRY(ws(c, mregions, &mprobe));
   ALN_TRY(ws(c, fwaves, &fmatch));
   ALN_TRY(ws(c, fwaves, &fprobe));
-  ALN_TRY(fill_u32(c, bitmap, bwords, 0));
-  ALN_TRY(fill_u64(c, ctr, 8, 0));
-  // per-segment plans of the three scans
+  // one launch: per-segment plans of the three scans, and zeroed bitmaps and counters
   SegPlan *dplan, *mplan, *fplan = nullptr;
   {
-    const uint32_t ds = a->dim.cols[0].nsegs, ms = a->mid.cols[0].nsegs, fs = a->fact.cols[0].nsegs;
+    const uint32_t ds = a->dim.cols[0].nsegs, ms = a->mid.cols[0].nsegs;
+    const uint32_t fs = use_bm ? a->fact.cols[0].nsegs : 0u;
+    const int dfast = (int)(drange <= (1ull << 31));
     ALN_TRY(ws(c, ds, &dplan));
-    hipLaunchKernelGGL(k_scan_plan, dim3((ds + 127) / 128), dim3(128), 0, c->stream, dd, a->d_key, -1, -1, -1,
-                       DIM_STAGE, dkmin, (int)(drange <= (1ull << 31)), (uint32_t)DIM_OWN, ds, dplan);
-    ALN_TRY(check_launch("scan_plan"));
     ALN_TRY(ws(c, ms, &mplan));
-    hipLaunchKernelGGL(k_scan_plan, dim3((ms + 127) / 128), dim3(128), 0, c->stream, md, a->m_fk, a->m_key, a->m_g1,
-                       a->m_g2, MID_STAGE, dkmin, (int)(drange <= (1ull << 31)), (uint32_t)PL_OWN, ms, mplan);
-    ALN_TRY(check_launch("scan_plan"));
-    if (use_bm && fs) {
-      ALN_TRY(ws(c, fs, &fplan));
-      hipLaunchKernelGGL(k_scan_plan, dim3((fs + 127) / 128), dim3(128), 0, c->stream, fd, a->f_fk, a->f_x, a->f_y,
-                         -1, FACT_STAGE, mkmin, 1, (uint32_t)FACT_OWN, fs, fplan);
-      ALN_TRY(check_launch("scan_plan"));
-    }
+    ALN_TRY(ws(c, fs ? fs : 1, &fplan));
+    const PlanJob jd{{-1, a->d_key, -1, -1, -1}, DIM_STAGE, dkmin, dfast, (uint32_t)DIM_OWN, ds, dplan};
+    const PlanJob jm{{-1, a->m_fk, a->m_key, a->m_g1, a->m_g2}, MID_STAGE, dkmin, dfast, (uint32_t)PL_OWN, ms, mplan};
+    const PlanJob jf{{-1, a->f_fk, a->f_x, a->f_y, -1}, FACT_STAGE, mkmin, 1, (uint32_t)FACT_OWN, fs, fplan};
+    const FillJob fj{{bitmap, reinterpret_cast<uint32_t *>(ctr), mbits, nullptr},
+                     {bwords, 16, use_bm ? mblk * BM_BLOCK_WORDS : 0, 0}};
+    const uint64_t most = std::max<uint64_t>(bwords, use_bm ? mblk * BM_BLOCK_WORDS : 0);
+    const uint32_t pb = (ds + ms + fs + 127) / 128, fb = (uint32_t)std::min<uint64_t>((most + 1023) / 1024, 16384);
+    hipLaunchKernelGGL(k_star_setup, dim3(pb + (fb ? fb : 1)), dim3(128), 0, c->stream, dd, md, fd, jd, jm, jf, pb,
+                       fj);
+    ALN_TRY(check_launch("star_setup"));
   }
 
   tm.mark(0);
