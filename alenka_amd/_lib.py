@@ -17,8 +17,8 @@ ALN_E_UNSUPPORTED = -4
 ALN_E_CAPACITY = -5
 
 RPN_NAME, RPN_NUMBER, RPN_STRING, RPN_CMP, RPN_AND, RPN_OR = 1, 2, 3, 4, 5, 6
-RPN_ADD, RPN_MINUS, RPN_MUL, RPN_DIV = 7, 8, 9, 10
-I_COL, I_CONST, I_ADD, I_SUB, I_MUL, I_DIV, I_CMP, I_AND, I_OR = range(1, 10)
+RPN_ADD, RPN_MINUS, RPN_MUL, RPN_DIV, RPN_CODESET = 7, 8, 9, 10, 11
+I_COL, I_CONST, I_ADD, I_SUB, I_MUL, I_DIV, I_CMP, I_AND, I_OR, I_INSET = range(1, 11)
 AGG_COUNT, AGG_SUM, AGG_MIN, AGG_MAX = 0, 2, 4, 5
 ORDER_KEYS, ORDER_HASH, ORDER_NONE = 0, 1, 2
 MAX_INSN, MAX_STACK, MAX_COLS, MAX_KEYS, MAX_AGGS, MAX_RANGES = 64, 16, 16, 4, 8, 8

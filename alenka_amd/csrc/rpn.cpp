@@ -9,6 +9,7 @@
 // so the device evaluates each row in one pass with the reference's results.
 #include <string.h>
 
+#include <utility>
 #include <vector>
 
 #include "aln_internal.h"
@@ -16,7 +17,7 @@
 namespace {
 
 using Frag = std::vector<aln_insn>;
-enum { T_NAME = 1, T_NUMBER = 2, T_STRING = 3 };
+enum { T_NAME = 1, T_NUMBER = 2, T_STRING = 3, T_SET = 4 };
 
 int pw10(uint32_t p, int64_t *out) {
   static const int64_t t[10] = {1,      10,      100,      1000,      10000,
@@ -123,6 +124,7 @@ struct Compiler {
   int mode = 0;  // 0: predicate (filter.cu), 1: expression (select.cu)
   std::vector<int> type, value;
   std::vector<int64_t> nums, strs;
+  std::vector<std::pair<int64_t, int32_t>> sets;  // LIKE operands: bit set address, codes
   std::vector<uint32_t> prec;
   std::vector<Frag> vecs, bools;
   int err = ALN_OK;
@@ -249,6 +251,21 @@ struct Compiler {
   void cmp(int code) {
     int s1 = pop(type), s2 = pop(type);
     if (err) return;
+    if (code == 7 || s1 == T_SET || s2 == T_SET) {
+      // LIKE (filter.cu:452-499): the pattern was matched against the column's dictionary
+      // when the query was bound; the row test is membership of the row's code in that set
+      if (code != 7 || !((s1 == T_SET && s2 == T_NAME) || (s1 == T_NAME && s2 == T_SET)))
+        return fail(ALN_E_UNSUPPORTED, "LIKE needs a char column and a pattern");
+      int v1 = pop(value), v2 = pop(value);
+      int col = (s1 == T_NAME) ? v1 : v2;
+      int sv = (s1 == T_SET) ? v1 : v2;
+      if (col < 0) return fail(ALN_E_UNSUPPORTED, "LIKE on a computed value");
+      if (sv > -2 || -2 - sv >= (int)sets.size()) return fail(ALN_E_ARG, "LIKE operand missing");
+      const auto &st = sets[-2 - sv];
+      bools.push_back(Frag{I(ALN_I_COL, col, 1), I(ALN_I_INSET, st.second, st.first)});
+      type.push_back(T_NAME);
+      return;
+    }
     if (code < 1 || code > 6) return fail(ALN_E_UNSUPPORTED, "comparison code outside 1..6");
     if (s1 == T_NUMBER && s2 == T_NUMBER) {
       int64_t n1 = pop(nums), n2 = pop(nums);
@@ -335,6 +352,15 @@ struct Compiler {
           value.push_back(-2 - (int)(strs.size() - 1));
           type.push_back(T_STRING);
           break;
+        case ALN_RPN_CODESET:
+          if (o.arg < 0 || (o.arg > 0 && o.val == 0)) {
+            fail(ALN_E_ARG, "LIKE operand without a bit set");
+            break;
+          }
+          sets.emplace_back(o.val, o.arg);
+          value.push_back(-2 - (int)(sets.size() - 1));
+          type.push_back(T_SET);
+          break;
         case ALN_RPN_ADD:
         case ALN_RPN_MINUS:
         case ALN_RPN_MUL:
@@ -372,7 +398,7 @@ int stack_depth(const Frag &f) {
   for (auto &x : f) {
     if (x.op == ALN_I_COL || x.op == ALN_I_CONST)
       ++d;
-    else
+    else if (x.op != ALN_I_INSET)
       --d;
     if (d > mx) mx = d;
     if (d < 1) return -1;
@@ -509,7 +535,8 @@ extern "C" int aln_zone_map_check(const aln_rpn *rpn, int nrpn, const int64_t *s
       type.push_back(T_NUMBER);
       nums.push_back(o.val);
       prec.push_back((uint32_t)o.arg);
-    } else if (o.type == ALN_RPN_STRING) {
+    } else if (o.type == ALN_RPN_STRING || o.type == ALN_RPN_CODESET) {
+      // a char comparison or LIKE is evaluated on every segment (zone_map.cu:1018-1044)
       type.push_back(T_STRING);
       val.push_back(-1);
     } else if (o.type == ALN_RPN_CMP) {

@@ -33,6 +33,10 @@ __device__ __forceinline__ int64_t vm_run(const aln_insn *__restrict__ prog, int
     } else if (op == ALN_I_CONST) {
       stk[sp * VM_BLOCK] = prog[i].k;
       ++sp;
+    } else if (op == ALN_I_INSET) {
+      const int64_t v = stk[(sp - 1) * VM_BLOCK];
+      const uint32_t *set = reinterpret_cast<const uint32_t *>(prog[i].k);
+      stk[(sp - 1) * VM_BLOCK] = (v >= 0 && v < prog[i].a) ? (set[v >> 5] >> (v & 31)) & 1u : 0;
     } else {
       int64_t r = stk[(sp - 1) * VM_BLOCK], l = stk[(sp - 2) * VM_BLOCK], v;
       switch (op) {

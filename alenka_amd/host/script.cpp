@@ -234,7 +234,7 @@ struct Parser {
         if (op.sub == 12) break;
         emit_cmp(op.sub);
         return true;
-      case T_LIKE: break;
+      case T_LIKE: emit_cmp(7); return true;
       case T_PUNCT:
         if (op.c == '+') { emit_add(); return true; }
         if (op.c == '-') { emit_minus(); return true; }

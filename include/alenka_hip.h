@@ -72,7 +72,11 @@ enum {
   ALN_RPN_ADD = 7,
   ALN_RPN_MINUS = 8,
   ALN_RPN_MUL = 9,
-  ALN_RPN_DIV = 10
+  ALN_RPN_DIV = 10,
+  ALN_RPN_CODESET = 11 /* LIKE operand (filter.cu:452-499): val = device address of a bit set
+                          over the column's dictionary codes (bit c set when entry c matches
+                          the pattern), arg = number of codes it covers; compared with the
+                          column by CMP 7 */
 };
 typedef struct {
   int32_t type;
@@ -90,7 +94,9 @@ enum {
   ALN_I_DIV = 6, /* l / r, truncating; r == 0 yields 0        */
   ALN_I_CMP = 7, /* a = comparison code, push (l op r)        */
   ALN_I_AND = 8,
-  ALN_I_OR = 9
+  ALN_I_OR = 9,
+  ALN_I_INSET = 10 /* v = pop, push (0 <= v < a and bit v of the uint32 bit set at device
+                      address k is set) */
 };
 typedef struct {
   int32_t op;

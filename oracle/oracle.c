@@ -21,6 +21,9 @@
  *   orc_groupby_*        - brute-force dictionary aggregation (Python) + ordering rules.
  *   orc_join_sortmerge   - nested-loop multiset + the documented output order.
  *   orc_order            - Python's stable sorted() with a composite key.
+ *   orc_like             - Python's re.fullmatch with '%' -> '.*' and '?' -> '[^.]' on
+ *                          values and patterns too short for the restart counter to bind,
+ *                          plus hand-derived cases where it does.
  */
 #include "oracle.h"
 
@@ -186,6 +189,45 @@ void orc_dict_first_seen(const int64_t *vals, uint64_t n, int64_t nvals, int64_t
     if (code_of_val[v] < 0) code_of_val[v] = next++;
     codes[i] = code_of_val[v];
   }
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* LIKE, restated from the reference's gpu_regex functor (filter.cu:67-129): '%' matches
+ * any run (backtracking to the last '%'), '?' any character except '.', anything else
+ * itself.  Kept as the reference has it: the position counter j is not reset when the
+ * match restarts after a '%', and a '%' at the end of the value is only skipped after the
+ * loop.  The value is read as at most len bytes, stopping at a NUL.                        */
+int orc_like(const char *s0, uint32_t len, const char *pat0) {
+  int star = 0;
+  uint32_t j = 0;
+  const char *s, *p, *str = s0, *pat = pat0;
+restart:
+  for (s = str, p = pat; j < len && *s; ++s, ++p, ++j) {
+    switch (*p) {
+      case '?':
+        if (*s == '.') goto mismatch;
+        break;
+      case '%':
+        star = 1;
+        str = s;
+        pat = p;
+        do {
+          ++pat;
+        } while (*pat == '%');
+        if (!*pat) return 1;
+        goto restart;
+      default:
+        if (*s != *p) goto mismatch;
+        break;
+    }
+  }
+  while (*p == '%') ++p;
+  return !*p;
+mismatch:
+  if (!star) return 0;
+  str++;
+  j++;
+  goto restart;
 }
 
 /* ------------------------------------------------------------------------------------ */

@@ -38,6 +38,9 @@ void orc_gen_fact(uint64_t seed, uint64_t norders, int64_t *l_okey, int64_t *l_p
                   int64_t *l_disc, int64_t *l_ship);
 void orc_dict_first_seen(const int64_t *vals, uint64_t n, int64_t nvals, int64_t *code_of_val,
                          int64_t *codes);
+/* LIKE on one fixed-width char value (filter.cu:67-129): 1 when `pat` matches the first
+ * `len` bytes of `s` (NUL-padded) */
+int orc_like(const char *s, uint32_t len, const char *pat);
 
 /* codecs */
 uint32_t orc_bits_for_range(int64_t lo, int64_t hi);

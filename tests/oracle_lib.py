@@ -35,6 +35,8 @@ def load():
         _o.orc_groupby_sorted.restype = C.c_uint64
         _o.orc_groupby_segmented.restype = C.c_uint64
         _o.orc_join_sortmerge.restype = C.c_uint64
+        _o.orc_like.restype = C.c_int
+        _o.orc_like.argtypes = [C.c_char_p, C.c_uint32, C.c_char_p]
     return _o
 
 
@@ -79,6 +81,13 @@ def gen_fact(seed, norders):
     cols = [np.empty(n, np.int64) for _ in range(4)]
     o.orc_gen_fact(C.c_uint64(seed), C.c_uint64(norders), *[_ptr(c) for c in cols])
     return cols
+
+
+def like(value, width, pattern):
+    """LIKE on one char[width] value (NUL-padded to width) - filter.cu:67-129."""
+    v = value.encode() if isinstance(value, str) else bytes(value)
+    buf = v[:width].ljust(width, b"\0") + b"\0"
+    return bool(load().orc_like(buf, width, pattern.encode()))
 
 
 def dict_first_seen(vals, nvals):

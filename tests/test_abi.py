@@ -170,6 +170,27 @@ def test_errors_are_status_codes():
         L.compile_rpn([(NAME, 5, 0), (NUM, 0, 1), (CMP, 1, 0)], [0])  # unknown column slot
 
 
+def test_like_compiles_to_a_code_set_test():
+    """column LIKE pattern arrives as NAME, CODESET (the dictionary codes the pattern matches,
+    as a device bit set), CMP 7 and becomes one set-membership instruction; the zone map
+    evaluates such a segment ('R'), as the reference does for every char comparison."""
+    SET = L.RPN_CODESET
+    prog, _ = L.compile_rpn([(NAME, 1, 0), (SET, 150, 0x7F0000001000), (CMP, 7, 0), (NAME, 0, 0),
+                             (NUM, 0, 15), (CMP, 4, 0), (AND, 0, 0)], [0, 0])
+    assert prog == [(L.I_COL, 1, 1), (L.I_INSET, 150, 0x7F0000001000), (L.I_COL, 0, 1),
+                    (L.I_CONST, 0, 15), (L.I_CMP, 4, 0), (L.I_AND, 0, 0)]
+    assert L.zone_map_check([(NAME, 0, 0), (SET, 150, 0x1000), (CMP, 7, 0)], [0], [9], [0]) == "R"
+    for bad in ([(NAME, 0, 0), (STR, 0, 3), (CMP, 7, 0)],      # pattern not matched to codes
+                [(NAME, 0, 0), (SET, 150, 0x1000), (CMP, 4, 0)],  # a set compared with ==
+                [(NUM, 0, 1), (SET, 150, 0x1000), (CMP, 7, 0)]):  # LIKE on a number
+        with pytest.raises(L.AlnError) as e:
+            L.compile_rpn(bad, [0])
+        assert e.value.code == L.ALN_E_UNSUPPORTED
+    with pytest.raises(L.AlnError) as e:
+        L.compile_rpn([(NAME, 0, 0), (SET, 150, 0), (CMP, 7, 0)], [0])  # no bit set
+    assert e.value.code == L.ALN_E_ARG
+
+
 def test_host_murmur_matches_oracle():
     r = np.random.default_rng(14)
     for n in range(0, 40):

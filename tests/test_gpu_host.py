@@ -195,3 +195,54 @@ def test_script_errors(ops):
     assert b"statement 1" in ops.lib.alenka_last_error()
     assert ops.alenkaExecute("y := FILTER no_such_table BY a < 1; z := FILTER y BY b > 2;") != 0
     assert b"couldn't find" in ops.lib.alenka_last_error()
+
+
+# ---- LIKE over char columns (filter.cu:67-129 matched per dictionary entry, 452-499) ----------
+P_TYPES = [f"{a} {b} {c}" for a in ("STANDARD", "SMALL", "MEDIUM", "LARGE", "ECONOMY", "PROMO")
+           for b in ("ANODIZED", "BURNISHED", "PLATED", "POLISHED", "BRUSHED")
+           for c in ("TIN", "NICKEL", "BRASS", "STEEL", "COPPER")]
+
+
+@pytest.fixture(scope="module")
+def part_table(ops):
+    rng = np.random.default_rng(11)
+    n = 60000
+    key = np.arange(n, dtype=np.int64) * 3 + 1
+    ty = [P_TYPES[i] for i in rng.integers(0, len(P_TYPES), n)]
+    size = rng.integers(1, 51, n).astype(np.int64)
+    return key, ty, size
+
+
+@pytest.mark.parametrize("pattern", ["%BRASS", "PROMO%", "%POLISHED%", "?MALL%", "%", "LARGE BRUSHED STEEL",
+                                     "%Z%", "ST%ED%TIN", "%O?%L"])
+def test_script_filter_like(ops, part_table, pattern):
+    """FILTER ... BY p_type LIKE pattern AND p_size == 15 (the shape of the reference's q2)."""
+    key, ty, size = part_table
+    ops.table("part", len(key), 8192, [("p_partkey", key, 0), ("p_type", ty, 25), ("p_size", size, 0)])
+    rc = ops.alenkaExecute(f'PF := FILTER part BY p_type LIKE "{pattern}" AND p_size == 15;')
+    assert rc == 0, ops.lib.alenka_last_error()
+    match = {t: O.like(t, 25, pattern) for t in set(ty)}
+    want = np.array([match[t] for t in ty]) & (size == 15)
+    assert ops.rows("PF") == int(want.sum())
+    assert np.array_equal(ops.fetch("PF", "p_partkey"), key[want])
+
+
+def test_emit_like_or_and_position_budget(ops):
+    """emit_* calls for `v LIKE "%ING" OR v LIKE "a?c"` on a char[8] column whose values fill
+    the width: the matcher's position budget (the column width) rejects BUILDING."""
+    vals = ["BUILDING", "aaab", "a.c", "abc", "RING", "xING", "", "ING.ING."]
+    rng = np.random.default_rng(3)
+    pick = rng.integers(0, len(vals), 20000)
+    col = [vals[i] for i in pick]
+    ids = np.arange(len(col), dtype=np.int64)
+    ops.table("V", len(col), 4096, [("id", ids, 0), ("v", col, 8)])
+    ops.emit_name("v"); ops.emit_string(b"%ING"); ops.emit_cmp(7)
+    ops.emit_name("v"); ops.emit_string(b"a?c"); ops.emit_cmp(7)
+    ops.emit_or()
+    ops.emit_filter("V1", "V")
+    ops.check(None, "filter")
+    match = [O.like(v, 8, "%ING") or O.like(v, 8, "a?c") for v in vals]
+    assert match == [False, False, False, True, True, True, False, False]
+    want = np.array([match[i] for i in pick])
+    assert ops.rows("V1") == int(want.sum())
+    assert np.array_equal(ops.fetch("V1", "id"), ids[want])

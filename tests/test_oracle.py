@@ -111,6 +111,50 @@ def test_dict_first_seen():
     assert list(codes) == [0, 0, 1, 2, 1, 3]
 
 
+# ---- LIKE (filter.cu:67-129) ------------------------------------------------------------------
+def _like_regex(pattern):
+    import re
+
+    out = []
+    for ch in pattern:
+        out.append(".*" if ch == "%" else "[^.]" if ch == "?" else re.escape(ch))
+    return re.compile("".join(out), re.S)
+
+
+def test_like_matches_wildcard_regex():
+    """With a width far above the matcher's work (at most (#'%' + 1) * (len + 1)^2 steps),
+    the position budget never binds and LIKE is the wildcard language: '%' = any run,
+    '?' = one character other than '.'."""
+    rng = np.random.default_rng(7)
+    for _ in range(4000):
+        v = "".join(rng.choice(list("ab."), size=rng.integers(0, 8)))
+        p = "".join(rng.choice(list("ab.?%"), size=rng.integers(0, 6)))
+        want = _like_regex(p).fullmatch(v) is not None
+        assert O.like(v, 4096, p) == want, (v, p)
+
+
+@pytest.mark.parametrize("value,width,pattern,want", [
+    # traced by hand: the position counter j (filter.cu:83) is shared by every restart after
+    # a '%' and bounded by the column width, so late restarts run out of budget
+    ("BUILDING", 8, "%ING", False),   # j reaches 8 on the 'N' of the last attempt
+    ("BUILDING", 10, "%ING", True),   # a wider column leaves room for the 'G'
+    ("aaab", 4, "%ab", False),        # third attempt starts with j == 4
+    ("aaab", 5, "%ab", False),        # third attempt matches 'a' with j == 4, then j == 5
+    ("aaab", 6, "%ab", True),
+    ("ab", 2, "%b", True),            # second attempt matches 'b' with j == 1
+    ("BUILDING", 8, "BUILD%", True),  # a trailing '%' run matches at once
+    ("BUILDING", 8, "BUILDING%%", True),  # the value ends first; trailing '%' are skipped
+    ("a.c", 3, "a?c", False),         # '?' never matches '.'
+    ("abc", 3, "a?c", True),
+    ("", 4, "%", True),
+    ("", 4, "", True),
+    ("abc", 3, "", False),            # pattern exhausted before the value: mismatch
+    ("abcdef", 3, "abc", True),       # only the first width bytes are the value
+])
+def test_like_position_budget(value, width, pattern, want):
+    assert O.like(value, width, pattern) == want
+
+
 # ---- codecs -------------------------------------------------------------------------------
 def test_pack_hand_example():
     # codes 5, 2, 7 at 3 bits, LSB first: 101 | 010 | 111 -> 0b111_010_101
