@@ -258,3 +258,113 @@ extern "C" int aln_hash_join(aln_ctx *c, const int64_t *d_build_keys, uint64_t n
   ALN_HIP(hipStreamSynchronize(c->stream));
   return ALN_OK;
 }
+
+namespace aln {
+
+// ---- semi / anti reduction ---------------------------------------------------------------
+// The reference narrows a join's pairs to one side's rows with sort + unique (semi) or a
+// set difference against 0..n-1 (anti), operators.cu:1357-1399 and 1473-1480.  Here the
+// rows a side contributes are marked in a bitmap (one bit per row, atomicOr), and one
+// compaction pass writes the set (semi) or clear (anti) rows in ascending order: one
+// thread per 32-row word, 8192 rows per block.
+constexpr int JR_BLOCK = 256;
+
+__global__ void __launch_bounds__(256) k_rows_mark(const uint32_t *__restrict__ idx, uint64_t n,
+                                                   uint64_t nrows, uint32_t *bm) {
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
+    const uint32_t r = idx[i];
+    if (r < nrows) atomicOr(&bm[r >> 5], 1u << (r & 31));
+  }
+}
+
+// Rows of word w that are kept: its marked rows (semi) or unmarked rows (anti), clipped to
+// the row count.
+__device__ __forceinline__ uint32_t kept_rows(const uint32_t *__restrict__ bm, uint64_t w, uint64_t nrows,
+                                              int anti) {
+  if (w >= (nrows + 31) >> 5) return 0;
+  uint32_t x = bm[w];
+  if (anti) x = ~x;
+  const uint64_t rem = nrows - (w << 5);
+  if (rem < 32) x &= (1u << rem) - 1u;
+  return x;
+}
+
+__global__ void __launch_bounds__(JR_BLOCK) k_rows_count(const uint32_t *__restrict__ bm, uint64_t nrows,
+                                                         int anti, uint32_t *counts) {
+  __shared__ uint32_t wsum[JR_BLOCK / 64];
+  const uint64_t w = (uint64_t)blockIdx.x * JR_BLOCK + threadIdx.x;
+  uint32_t n = __popc(kept_rows(bm, w, nrows, anti));
+  for (int o = 32; o > 0; o >>= 1) n += __shfl_xor(n, o, 64);
+  if (__lane_id() == 0) wsum[threadIdx.x / 64] = n;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t s = 0;
+    for (int k = 0; k < JR_BLOCK / 64; ++k) s += wsum[k];
+    counts[blockIdx.x] = s;
+  }
+}
+
+__global__ void __launch_bounds__(JR_BLOCK) k_rows_write(const uint32_t *__restrict__ bm, uint64_t nrows,
+                                                         int anti, const uint64_t *__restrict__ off,
+                                                         uint32_t *__restrict__ out) {
+  __shared__ uint32_t wsum[JR_BLOCK / 64];
+  const uint64_t w = (uint64_t)blockIdx.x * JR_BLOCK + threadIdx.x;
+  uint32_t x = kept_rows(bm, w, nrows, anti);
+  const uint32_t n = __popc(x);
+  const unsigned lane = __lane_id(), wid = threadIdx.x / 64;
+  uint32_t incl = n;
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(incl, o, 64);
+    if (lane >= (unsigned)o) incl += y;
+  }
+  if (lane == 63) wsum[wid] = incl;
+  __syncthreads();
+  uint32_t before = 0;
+  for (unsigned k = 0; k < wid; ++k) before += wsum[k];
+  uint64_t pos = off[blockIdx.x] + before + incl - n;
+  while (x) {
+    out[pos++] = (uint32_t)((w << 5) + (uint32_t)(__ffs(x) - 1));
+    x &= x - 1;
+  }
+}
+
+}  // namespace aln
+
+extern "C" int aln_join_rows(aln_ctx *c, const uint32_t *d_idx, uint64_t nidx, uint64_t nrows, int anti,
+                             uint32_t *d_rows, uint64_t cap, uint64_t *h_count) {
+  using namespace aln;
+  ALN_REQUIRE(c && h_count, "aln_join_rows: bad arguments");
+  ALN_REQUIRE(nrows < 0xFFFFFFFFull, "aln_join_rows: more than 2^32-1 rows");
+  ALN_REQUIRE(nidx == 0 || d_idx, "aln_join_rows: NULL row indices");
+  ws_reset(c);
+  *h_count = 0;
+  if (nrows == 0) return ALN_OK;
+  const uint64_t nw = (nrows + 31) / 32, nblocks = (nw + JR_BLOCK - 1) / JR_BLOCK;
+  uint32_t *bm, *counts;
+  uint64_t *off;
+  ALN_TRY(ws(c, nw, &bm));
+  ALN_TRY(ws(c, nblocks, &counts));
+  ALN_TRY(ws(c, nblocks, &off));
+  ALN_TRY(fill_u32(c, bm, nw, 0));
+  if (nidx) {
+    hipLaunchKernelGGL(k_rows_mark, dim3(grid_for(nidx, 256, 65536)), dim3(256), 0, c->stream, d_idx, nidx,
+                       nrows, bm);
+    ALN_TRY(check_launch("join_rows_mark"));
+  }
+  hipLaunchKernelGGL(k_rows_count, dim3((unsigned)nblocks), dim3(JR_BLOCK), 0, c->stream, (const uint32_t *)bm,
+                     nrows, anti ? 1 : 0, counts);
+  ALN_TRY(check_launch("join_rows_count"));
+  uint64_t total = 0;
+  ALN_TRY(exclusive_scan_u32(c, counts, nblocks, off, &total));
+  *h_count = total;
+  if (total > cap)
+    return set_error(ALN_E_CAPACITY, "aln_join_rows: %llu rows, capacity %llu", (unsigned long long)total,
+                     (unsigned long long)cap);
+  if (total == 0) return ALN_OK;
+  ALN_REQUIRE(d_rows, "aln_join_rows: no output buffer");
+  hipLaunchKernelGGL(k_rows_write, dim3((unsigned)nblocks), dim3(JR_BLOCK), 0, c->stream, (const uint32_t *)bm,
+                     nrows, anti ? 1 : 0, (const uint64_t *)off, d_rows);
+  ALN_TRY(check_launch("join_rows_write"));
+  ALN_HIP(hipStreamSynchronize(c->stream));
+  return ALN_OK;
+}

@@ -231,6 +231,17 @@ def hash_join(ctx, build_keys, probe_keys, cap=None):
         return pi[: cnt.value], bi[: cnt.value]
 
 
+def join_rows(ctx, idx, nrows, anti, cap=None):
+    """Rows 0..nrows-1 that occur in idx (anti=False) or do not (anti=True), ascending
+    (int32 tensor)."""
+    cap = max(nrows, 1) if cap is None else cap
+    out = ctx.empty(max(cap, 1), torch.int32)
+    cnt = C.c_uint64()
+    L.check(ctx.lib.aln_join_rows(ctx.h, _p(idx), idx.numel(), nrows, int(anti), _p(out), cap, C.byref(cnt)),
+            "aln_join_rows")
+    return out[: cnt.value]
+
+
 def groupby(ctx, keys, vals, aggs, order=L.ORDER_KEYS, seed=100, cap=None):
     """Returns (out_keys list, out_vals list) trimmed to the group count."""
     n = keys[0].numel()

@@ -89,6 +89,7 @@ int g_device = 0;
 std::map<std::string, std::shared_ptr<Set>> varNames;
 std::vector<Tok> g_tok;
 std::vector<std::string> g_join;  // op_join
+std::vector<char> g_join_kind;    // join_type: 'I' inner, '1'/'2' left/right semi, '3'/'4' anti
 int g_limit = 0;
 bool g_has_limit = false;
 bool g_join_bad = false;  // an unsupported join kind was queued for this statement
@@ -516,6 +517,7 @@ void finish_statement() {
   g_join_bad = false;
   g_tok.clear();
   g_join.clear();
+  g_join_kind.clear();
   g_has_limit = false;
   g_limit = 0;
 }
@@ -725,8 +727,9 @@ void emit_sel_name(const char *name) { push("SEL", name); }
 void emit_var_asc(const char *name) { push("ORDER", name, 0); }
 void emit_var_desc(const char *name) { push("ORDER", name, 1); }
 void emit_join_tab(const char *name, char tp) {
-  if (tp != 'I') g_join_bad = true;
+  if (tp != 'I' && tp != '1' && tp != '2' && tp != '3' && tp != '4') g_join_bad = true;
   g_join.push_back(name);
+  g_join_kind.push_back(tp);
 }
 void emit_limit(int val) {
   g_limit = val;
@@ -942,33 +945,46 @@ struct Cond {
   std::vector<std::pair<std::string, std::string>> eqs;  // a = b, in order
 };
 
-// Splits the condition tokens (NAME NAME JOIN, combined with AND) into one condition per
-// join, in parse order.
+// Splits the condition tokens into one condition per join, in parse order, counting them as
+// emit_and / emit_eq do (operators.cu:153-172): the ANDs seen since the last condition give
+// the number of extra column pairs of the next one, whose equalities then close it.  '=' has
+// the grammar's lowest precedence (bison.y:42-49), so `ON a = b AND c = d` arrives as
+// a b c AND d = =, and a condition's names pair up in text order: (a, b), (c, d).
 static int parse_conditions(const std::vector<Tok> &t, std::vector<Cond> *out) {
-  std::vector<Cond> stack;
   std::vector<std::string> names;
+  std::vector<int> extra;  // extra column pairs of each condition
+  int and_cnt = 0, misses = 0;
   for (auto &k : t) {
     if (k.type == "NAME") {
       names.push_back(k.value);
-    } else if (k.type == "JOIN") {
-      if (names.size() < 2) return fail(ALN_E_ARG, "Join : bad condition");
-      Cond c;
-      std::string b = names.back();
-      names.pop_back();
-      std::string a = names.back();
-      names.pop_back();
-      c.eqs.push_back({a, b});
-      stack.push_back(c);
     } else if (k.type == "AND") {
-      if (stack.size() < 2) return fail(ALN_E_ARG, "Join : bad condition");
-      Cond c2 = stack.back();
-      stack.pop_back();
-      stack.back().eqs.insert(stack.back().eqs.end(), c2.eqs.begin(), c2.eqs.end());
+      ++and_cnt;
+    } else if (k.type == "JOIN") {
+      if (misses == 0) {
+        extra.push_back(and_cnt);
+        misses = and_cnt;
+        and_cnt = 0;
+      } else {
+        --misses;
+      }
     } else {
       return fail(ALN_E_UNSUPPORTED, "Join : operation %s in a join condition", k.type.c_str());
     }
   }
-  *out = stack;
+  if (and_cnt || misses) return fail(ALN_E_ARG, "Join : bad condition");
+  size_t p = 0;
+  for (int x : extra) {
+    // the reference compares the third and fourth names for every extra pair
+    // (operators.cu:1288-1296), so it checks only one extra equality
+    if (x > 1) return fail(ALN_E_UNSUPPORTED, "Join : more than two column pairs in one condition");
+    Cond c;
+    for (int q = 0; q <= x; ++q, p += 2) {
+      if (p + 2 > names.size()) return fail(ALN_E_ARG, "Join : bad condition");
+      c.eqs.push_back({names[p], names[p + 1]});
+    }
+    out->push_back(c);
+  }
+  if (p != names.size()) return fail(ALN_E_ARG, "Join : bad condition");
   return ALN_OK;
 }
 
@@ -985,9 +1001,14 @@ static std::vector<std::string> all_cols(const std::string &var) {
   return base->names;
 }
 
-// left JOIN right ON cond -> result `name` with the columns of both sides in `keep`.
+static int semi_anti(const std::string &name, char kind, const View &lv, const View &rv,
+                     const std::vector<std::pair<std::string, std::string>> &eqs, DBufP pi, DBufP bi,
+                     uint64_t np, const std::set<std::string> &lneed, const std::set<std::string> &rneed);
+
+// left JOIN right ON cond -> result `name` with the columns of both sides in `keep`; for the
+// semi and anti kinds, the columns of the side that is kept.
 static int join_pair(const std::string &name, const std::string &left, const std::string &right,
-                     const Cond &cond, const std::set<std::string> &keep) {
+                     const Cond &cond, const std::set<std::string> &keep, char kind) {
   // orient each equality: the left column belongs to the left variable
   std::vector<std::pair<std::string, std::string>> eqs;
   for (auto &e : cond.eqs) {
@@ -1032,6 +1053,7 @@ static int join_pair(const std::string &name, const std::string &left, const std
     ALN(rc, "join");
     break;
   }
+  if (kind != 'I') return semi_anti(name, kind, lv, rv, eqs, pi, bi, np, lneed, rneed);
   // gather every kept column through the pairs
   auto gather_all = [&](const View &v, const std::set<std::string> &cols, const DBufP &idx, uint64_t n,
                         std::map<std::string, Col> *dst) -> int {
@@ -1160,6 +1182,121 @@ static int join_pair(const std::string &name, const std::string &left, const std
   return ALN_OK;
 }
 
+// A column of `n` rows gathered from view `v` through the row ids `idx`, as a plain column.
+static int gather_plain(const View &v, const std::string &cn, const DBufP &idx, uint64_t n, Col *out) {
+  const Col &sc = v.cols.at(cn);
+  DBufP vals = dalloc(8 * (n ? n : 1));
+  if (!vals) return fail(ALN_E_NOMEM, "device allocation failed");
+  aln_col d = sc.desc();
+  if (n) ALN(aln_gather(g_ctx, &d, v.seg_rows, (const uint32_t *)idx->p, n, (int64_t *)vals->p), "join gather");
+  TRY(make_plain(vals, n, (uint32_t)(n ? n : 1), sc.decimals, out));
+  out->type = sc.type;
+  out->dict = sc.dict;
+  out->width = sc.width;
+  return ALN_OK;
+}
+
+// LEFT/RIGHT SEMI and ANTI joins (operators.cu:1357-1399, 1473-1530): the inner join's pairs,
+// narrowed by the extra equalities, reduced to the rows of one side that take part (semi) or
+// do not (anti), each once.  The left kinds keep the left's row order (the reference works
+// per left segment, in row order within each); the right kinds return the right's rows in
+// the order of its stable sort by the join key, the order the reference's right side is in
+// when it gathers them.  Only the kept side's columns are in the result: the reference
+// gathers the other side's through indices that no longer line up with the rows.
+static int semi_anti(const std::string &name, char kind, const View &lv, const View &rv,
+                     const std::vector<std::pair<std::string, std::string>> &eqs, DBufP pi, DBufP bi,
+                     uint64_t np, const std::set<std::string> &lneed, const std::set<std::string> &rneed) {
+  if (eqs.size() > 1 && np) {
+    // the pairs where every extra equality holds (operators.cu:1288-1348)
+    std::vector<Col> tmp;
+    std::vector<aln_col> cols;
+    std::vector<aln_rpn> rpn;
+    std::vector<int32_t> dec;
+    for (size_t e = 1; e < eqs.size(); ++e) {
+      Col a, b;
+      TRY(gather_plain(lv, eqs[e].first, pi, np, &a));
+      TRY(gather_plain(rv, eqs[e].second, bi, np, &b));
+      if (a.type == 2 || b.type == 2) return fail(ALN_E_UNSUPPORTED, "Join : string join keys");
+      const int s = (int)cols.size();
+      tmp.push_back(a);
+      tmp.push_back(b);
+      cols.push_back(a.desc());
+      cols.push_back(b.desc());
+      dec.push_back(0);
+      dec.push_back(0);
+      rpn.push_back(aln_rpn{ALN_RPN_NAME, s, 0});
+      rpn.push_back(aln_rpn{ALN_RPN_NAME, s + 1, 0});
+      rpn.push_back(aln_rpn{ALN_RPN_CMP, 4, 0});
+      if (e > 1) rpn.push_back(aln_rpn{ALN_RPN_AND, 0, 0});
+    }
+    aln_insn prog[ALN_MAX_INSN];
+    int nprog = 0;
+    ALN(aln_compile_rpn(rpn.data(), (int)rpn.size(), dec.data(), (int)dec.size(), 0, prog, &nprog, nullptr),
+        "join condition");
+    DBufP sel = dalloc(4 * np);
+    if (!sel) return fail(ALN_E_NOMEM, "device allocation failed");
+    uint64_t k = 0;
+    ALN(aln_scan_filter(g_ctx, cols.data(), (int)cols.size(), (uint32_t)np, prog, nprog, nullptr,
+                        (uint32_t *)sel->p, np, &k),
+        "join condition");
+    std::vector<uint32_t> hs(k), hp(np), hb(np);
+    if ((k && hipMemcpy(hs.data(), sel->p, 4 * k, hipMemcpyDeviceToHost) != hipSuccess) ||
+        hipMemcpy(hp.data(), pi->p, 4 * np, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(hb.data(), bi->p, 4 * np, hipMemcpyDeviceToHost) != hipSuccess)
+      return fail(ALN_E_HIP, "device to host copy failed");
+    for (uint64_t j = 0; j < k; ++j) {
+      hp[j] = hp[hs[j]];
+      hb[j] = hb[hs[j]];
+    }
+    if (k && (hipMemcpy(pi->p, hp.data(), 4 * k, hipMemcpyHostToDevice) != hipSuccess ||
+              hipMemcpy(bi->p, hb.data(), 4 * k, hipMemcpyHostToDevice) != hipSuccess))
+      return fail(ALN_E_HIP, "host to device copy failed");
+    np = k;
+  }
+  const bool left_side = kind == '1' || kind == '3';
+  const bool anti = kind == '3' || kind == '4';
+  const View &v = left_side ? lv : rv;
+  DBufP rows = dalloc(4 * (v.nrows ? v.nrows : 1));
+  if (!rows) return fail(ALN_E_NOMEM, "device allocation failed");
+  uint64_t n = 0;
+  ALN(aln_join_rows(g_ctx, (const uint32_t *)(left_side ? pi : bi)->p, np, v.nrows, anti ? 1 : 0,
+                    (uint32_t *)rows->p, v.nrows, &n),
+      "join");
+  auto out = std::make_shared<Set>();
+  out->not_compressed = true;
+  out->nrows = n;
+  out->seg_rows = (uint32_t)(n ? n : 1);
+  for (auto &cn : left_side ? lneed : rneed) {
+    Col c;
+    TRY(gather_plain(v, cn, rows, n, &c));
+    out->names.push_back(cn);
+    out->cols[cn] = c;
+  }
+  if (!left_side && n > 1) {
+    // the right side's order: stable by its join key
+    const int64_t *ks[1] = {(const int64_t *)out->cols[eqs[0].second].words->p};
+    int32_t desc[1] = {0};
+    DBufP dperm = dalloc(4 * n);
+    if (!dperm) return fail(ALN_E_NOMEM, "device allocation failed");
+    ALN(aln_sort_keys(g_ctx, ks, desc, 1, n, (uint32_t *)dperm->p), "join order");
+    for (auto &kv : out->cols) {
+      DBufP dst = dalloc(8 * n);
+      if (!dst) return fail(ALN_E_NOMEM, "device allocation failed");
+      ALN(aln_apply_perm(g_ctx, (const int64_t *)kv.second.words->p, (const uint32_t *)dperm->p, n,
+                         (int64_t *)dst->p),
+          "join order");
+      Col c;
+      TRY(make_plain(dst, n, (uint32_t)n, kv.second.decimals, &c));
+      c.type = kv.second.type;
+      c.dict = kv.second.dict;
+      c.width = kv.second.width;
+      kv.second = c;
+    }
+  }
+  varNames[name] = out;
+  return ALN_OK;
+}
+
 static int do_join(const char *s, const char *j1) {
   std::vector<SelItem> items;
   std::vector<Tok> rest;
@@ -1191,7 +1328,7 @@ static int do_join(const char *s, const char *j1) {
       left = i < nt ? g_join[i] : std::string(j1);
       right = prev;
     }
-    TRY(join_pair(name, left, right, conds[nt - i], keep));
+    TRY(join_pair(name, left, right, conds[nt - i], keep, g_join_kind[i - 1]));
     if (!prev.empty()) varNames.erase(prev);
     prev = name;
   }
@@ -1218,7 +1355,7 @@ void emit_join(const char *s, const char *j1, int grp, int start_seg, int end_se
   (void)end_seg;
   g_status = ALN_OK;
   if (g_join_bad) {
-    fail(ALN_E_UNSUPPORTED, "Join : only inner joins are supported");
+    fail(ALN_E_UNSUPPORTED, "Join : only inner, semi and anti joins are supported");
   } else {
     if (!find_var(j1 ? j1 : ""))
       fail(ALN_E_ARG, "Join : couldn't find variable %s", j1 ? j1 : "");

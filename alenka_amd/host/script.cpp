@@ -417,16 +417,42 @@ struct Parser {
       // joins: the grammar's join_list is right-recursive, so the tables are handed over
       // innermost (last) first, after all the ON expressions
       std::vector<std::string> joins;
+      std::vector<std::string> kinds;  // the words before JOIN: "", "LEFT SEMI", "RIGHT", ...
       while (kw("JOIN") || kw("LEFT") || kw("RIGHT") || kw("OUTER")) {
-        if (!kw("JOIN")) {
-          error("only inner joins are supported by this build");
-          return false;
+        std::string k;
+        while (!kw("JOIN")) {
+          if (!(kw("LEFT") || kw("RIGHT") || kw("OUTER") || kw("SEMI") || kw("ANTI"))) {
+            error("JOIN expected");
+            return false;
+          }
+          k += (k.empty() ? "" : " ") + cur.s;
+          if (!advance()) return false;
         }
         std::string u;
         if (!advance() || !name(u) || !expect_kw("ON") || !expr(1)) return false;
         joins.push_back(u);
+        kinds.push_back(k);
       }
-      for (size_t k = joins.size(); k-- > 0;) emit_join_tab(joins[k].c_str(), 'I');
+      // join kinds as bison.y:260-275 queues them; a join followed by another one takes the
+      // grammar's second set of rules, which queue LEFT as 'L' and RIGHT SEMI as 'R'
+      for (size_t k = joins.size(); k-- > 0;) {
+        const bool last = k + 1 == joins.size();
+        const std::string &w = kinds[k];
+        char tp;
+        if (w.empty()) tp = 'I';
+        else if (w == "LEFT ANTI") tp = '3';
+        else if (w == "RIGHT ANTI") tp = '4';
+        else if (w == "LEFT SEMI") tp = '1';
+        else if (w == "RIGHT SEMI") tp = last ? '2' : 'R';
+        else if (w == "LEFT") tp = last ? 'S' : 'L';
+        else if (w == "RIGHT") tp = 'R';
+        else if (w == "OUTER") tp = 'O';
+        else {
+          error("unknown join kind %s", w.c_str());
+          return false;
+        }
+        emit_join_tab(joins[k].c_str(), tp);
+      }
       int grp = 0;
       if (kw("GROUP")) {
         if (!advance() || !expect_kw("BY")) return false;

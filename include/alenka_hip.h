@@ -21,6 +21,7 @@
  *   emit_multijoin + mgpu::inner_join operators.cu:940-1582  -> aln_hash_join
  *   CudaSet::GroupBy + select() segreduce cm.cu:910-956, select.cu:481-580 -> aln_groupby
  *   add()/count_avg() merge.cu:92-496 (hash-ordered merge)   -> aln_groupby(order=ALN_ORDER_HASH)
+ *   semi/anti joins: unique / set_difference operators.cu:1357-1399 -> aln_join_rows
  *   update_permutation/apply_permutation sorts.cu:17-125     -> aln_sort_keys, aln_apply_perm
  *   emit_order + Store(limit) operators.cu:1641-1798, cm.cu:1493-1498 -> aln_topk
  *   MurmurHash64A     MurmurHash2_64.cu:63-104               -> aln_murmur64a, aln_hash_rows
@@ -192,6 +193,15 @@ int aln_apply_perm(aln_ctx *ctx, const int64_t *d_in, const uint32_t *d_perm, ui
 int aln_hash_join(aln_ctx *ctx, const int64_t *d_build_keys, uint64_t nbuild,
                   const int64_t *d_probe_keys, uint64_t nprobe, uint32_t *d_probe_idx,
                   uint32_t *d_build_idx, uint64_t cap, uint64_t *h_npairs);
+
+/* Semi / anti reduction of a join (replaces sort + unique and set_difference in
+ * operators.cu:1357-1399 and 1473-1480): d_idx holds the row indices (< nrows) one side
+ * contributes to the join's pairs, duplicates allowed.  Writes the rows of that side that
+ * occur in d_idx (anti == 0: LEFT/RIGHT SEMI) or that do not (anti != 0: LEFT/RIGHT ANTI),
+ * each once, in ascending order.  *h_count receives the row count; ALN_E_CAPACITY when it
+ * exceeds cap. */
+int aln_join_rows(aln_ctx *ctx, const uint32_t *d_idx, uint64_t nidx, uint64_t nrows, int anti,
+                  uint32_t *d_rows, uint64_t cap, uint64_t *h_count);
 
 /* ---- group-by SUM/COUNT/MIN/MAX ------------------------------------------------------ */
 enum { ALN_AGG_COUNT = 0, ALN_AGG_SUM = 2, ALN_AGG_MIN = 4, ALN_AGG_MAX = 5 };

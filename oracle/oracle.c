@@ -20,6 +20,8 @@
  *                          quirks.
  *   orc_groupby_*        - brute-force dictionary aggregation (Python) + ordering rules.
  *   orc_join_sortmerge   - nested-loop multiset + the documented output order.
+ *   orc_semi_anti        - Python set membership over all key pairs (brute force), and the
+ *                          stable key order of the right side.
  *   orc_order            - Python's stable sorted() with a composite key.
  *   orc_like             - Python's re.fullmatch with '%' -> '.*' and '?' -> '[^.]' on
  *                          values and patterns too short for the restart counter to bind,
@@ -1011,6 +1013,67 @@ uint64_t orc_join_sortmerge(const int64_t *lkeys, uint64_t nl, const int64_t *rk
   }
   free(rp);
   return k;
+}
+
+/* Semi and anti joins, restated from operators.cu:1357-1399 and 1473-1530.  The pairs of
+ * the sort-merge join on the first key, narrowed to those where the other nkeys-1 keys are
+ * equal too (operators.cu:1288-1348); then
+ *   '1' LEFT SEMI   sort + unique of the pairs' left rows;
+ *   '3' LEFT ANTI   0..nl-1 minus those rows (set difference);
+ *   '2' RIGHT SEMI  sort + unique of the pairs' right positions, a position being the
+ *                   right row's rank in the right side's stable sort by its first key;
+ *   '4' RIGHT ANTI  0..nr-1 minus those positions;
+ * right positions are written as right rows.  Writes the rows to out (capacity nl or nr)
+ * and returns their count. */
+static int cmp_u32(const void *a, const void *b) {
+  uint32_t x = *(const uint32_t *)a, y = *(const uint32_t *)b;
+  return x < y ? -1 : x > y;
+}
+
+uint64_t orc_semi_anti(const int64_t *const *lkeys, const int64_t *const *rkeys, int nkeys, uint64_t nl,
+                       uint64_t nr, const uint64_t *lseg_bounds, int nlsegs, char kind, uint32_t *out) {
+  uint64_t np = orc_join_sortmerge(lkeys[0], nl, rkeys[0], nr, lseg_bounds, nlsegs, NULL, NULL, 0);
+  uint32_t *pl = (uint32_t *)malloc(4 * (np ? np : 1)), *pr = (uint32_t *)malloc(4 * (np ? np : 1));
+  orc_join_sortmerge(lkeys[0], nl, rkeys[0], nr, lseg_bounds, nlsegs, pl, pr, np);
+  uint64_t k = 0;
+  for (uint64_t i = 0; i < np; ++i) {
+    int eq = 1;
+    for (int c = 1; c < nkeys; ++c) eq = eq && lkeys[c][pl[i]] == rkeys[c][pr[i]];
+    if (eq) { pl[k] = pl[i]; pr[k] = pr[i]; ++k; }
+  }
+  const int left = kind == '1' || kind == '3';
+  const uint64_t n = left ? nl : nr;
+  uint32_t *rp = NULL, *v = left ? pl : pr;
+  if (!left) {  /* right rows -> positions in the stable key order */
+    rp = (uint32_t *)malloc(4 * (nr ? nr : 1));
+    uint32_t *pos = (uint32_t *)malloc(4 * (nr ? nr : 1));
+    const int64_t *rk[1] = {rkeys[0]};
+    orc_order(rk, NULL, 1, nr, rp);
+    for (uint64_t i = 0; i < nr; ++i) pos[rp[i]] = (uint32_t)i;
+    for (uint64_t i = 0; i < k; ++i) v[i] = pos[v[i]];
+    free(pos);
+  }
+  qsort(v, k, 4, cmp_u32);
+  uint64_t u = 0;
+  for (uint64_t i = 0; i < k; ++i)
+    if (u == 0 || v[u - 1] != v[i]) v[u++] = v[i];
+  uint64_t m = 0;
+  if (kind == '1' || kind == '2') {
+    for (uint64_t i = 0; i < u; ++i) out[m++] = v[i];
+  } else {
+    uint64_t j = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+      while (j < u && v[j] < i) ++j;
+      if (j < u && v[j] == i) continue;
+      out[m++] = (uint32_t)i;
+    }
+  }
+  if (!left)
+    for (uint64_t i = 0; i < m; ++i) out[i] = rp[out[i]];
+  free(rp);
+  free(pl);
+  free(pr);
+  return m;
 }
 
 /* Kahan-compensated FP64 sum (the FP64 oracle for SUM aggregates). */

@@ -38,6 +38,10 @@ void orc_gen_fact(uint64_t seed, uint64_t norders, int64_t *l_okey, int64_t *l_p
                   int64_t *l_disc, int64_t *l_ship);
 void orc_dict_first_seen(const int64_t *vals, uint64_t n, int64_t nvals, int64_t *code_of_val,
                          int64_t *codes);
+/* Semi ('1' left, '2' right) and anti ('3' left, '4' right) joins on nkeys equal keys
+ * (operators.cu:1357-1399, 1473-1530); returns the kept rows of that side in out */
+uint64_t orc_semi_anti(const int64_t *const *lkeys, const int64_t *const *rkeys, int nkeys, uint64_t nl,
+                       uint64_t nr, const uint64_t *lseg_bounds, int nlsegs, char kind, uint32_t *out);
 /* LIKE on one fixed-width char value (filter.cu:67-129): 1 when `pat` matches the first
  * `len` bytes of `s` (NUL-padded) */
 int orc_like(const char *s, uint32_t len, const char *pat);

@@ -206,6 +206,24 @@ def join_sortmerge(lkeys, rkeys, lseg_bounds=None):
     return ol[:need], orr[:need]
 
 
+def semi_anti(lkeys, rkeys, kind, lseg_bounds=None):
+    """kind '1' LEFT SEMI, '2' RIGHT SEMI, '3' LEFT ANTI, '4' RIGHT ANTI; lkeys/rkeys: lists
+    of equally long key columns (first = the sort-merge key).  Returns the kept rows."""
+    lk = [np.ascontiguousarray(k, np.int64) for k in lkeys]
+    rk = [np.ascontiguousarray(k, np.int64) for k in rkeys]
+    nl, nr = len(lk[0]), len(rk[0])
+    if lseg_bounds is None:
+        sb, ns = np.array([0, nl], np.uint64), 1
+    else:
+        sb, ns = np.ascontiguousarray(lseg_bounds, np.uint64), len(lseg_bounds) - 1
+    out = np.empty(max(1, nl, nr), np.uint32)
+    o = load()
+    o.orc_semi_anti.restype = C.c_uint64
+    n = o.orc_semi_anti(_ptrs(lk), _ptrs(rk), len(lk), C.c_uint64(nl), C.c_uint64(nr), _ptr(sb), ns,
+                        C.c_char(kind.encode()), _ptr(out))
+    return out[:n]
+
+
 def order(keys, desc):
     keys = [np.ascontiguousarray(k, np.int64) for k in keys]
     n = len(keys[0])

@@ -246,3 +246,98 @@ def test_emit_like_or_and_position_budget(ops):
     want = np.array([match[i] for i in pick])
     assert ops.rows("V1") == int(want.sum())
     assert np.array_equal(ops.fetch("V1", "id"), ids[want])
+
+
+# ---- semi / anti joins (operators.cu:1357-1399, 1473-1530) -------------------------------------
+def semi_anti_tables(ops, seed=17):
+    r = np.random.default_rng(seed)
+    nc, no = 3000, 40000
+    c_key = r.permutation(np.arange(1, nc + 1)).astype(np.int64)
+    c_nat = r.integers(0, 25, nc).astype(np.int64)
+    c_prio = r.integers(0, 5, nc).astype(np.int64)
+    o_key = np.arange(no, dtype=np.int64) * 7 + 3
+    o_cust = r.integers(1, 4500, no).astype(np.int64)  # a third match no customer
+    o_prio = r.integers(0, 5, no).astype(np.int64)
+    ops.table("customer", nc, 1024, [("c_custkey", c_key, 0), ("c_nationkey", c_nat, 0), ("c_prio", c_prio, 0)])
+    ops.table("orders", no, 16384, [("o_orderkey", o_key, 0), ("o_custkey", o_cust, 0), ("o_prio", o_prio, 0)])
+    return dict(c_key=c_key, c_nat=c_nat, c_prio=c_prio, o_key=o_key, o_cust=o_cust, o_prio=o_prio)
+
+
+@pytest.mark.parametrize("words,kind", [("LEFT SEMI", "1"), ("LEFT ANTI", "3"), ("RIGHT SEMI", "2"),
+                                        ("RIGHT ANTI", "4")])
+@pytest.mark.parametrize("two_keys", [False, True])
+def test_script_semi_anti_join(ops, words, kind, two_keys):
+    t = semi_anti_tables(ops)
+    cond = "o_custkey = c_custkey" + (" AND o_prio = c_prio" if two_keys else "")
+    left = kind in "13"
+    sel = "o_orderkey AS k, o_custkey AS c" if left else "c_custkey AS k, c_nationkey AS c"
+    script = (f"C := FILTER customer BY c_nationkey < 20;\n"
+              f"S := SELECT {sel} FROM orders {words} JOIN C ON {cond};")
+    assert ops.alenkaExecute(script) == 0, ops.lib.alenka_last_error()
+    keep = t["c_nat"] < 20
+    ck, cp, cn = t["c_key"][keep], t["c_prio"][keep], t["c_nat"][keep]
+    lk = [t["o_cust"]] + ([t["o_prio"]] if two_keys else [])
+    rk = [ck] + ([cp] if two_keys else [])
+    rows = O.semi_anti(lk, rk, kind)
+    assert ops.rows("S") == len(rows)
+    if left:
+        assert np.array_equal(ops.fetch("S", "k"), t["o_key"][rows])
+        assert np.array_equal(ops.fetch("S", "c"), t["o_cust"][rows])
+    else:
+        assert np.array_equal(ops.fetch("S", "k"), ck[rows])
+        assert np.array_equal(ops.fetch("S", "c"), cn[rows])
+
+
+def test_script_inner_join_two_keys(ops):
+    """`ON a = b AND c = d` (parsed as a = ((b AND c) = d), bison.y:42-49): pairs of the first
+    key's join where the second keys are equal too, in the pair order (operators.cu:1288-1348)."""
+    t = semi_anti_tables(ops, seed=29)
+    rc = ops.alenkaExecute("S := SELECT o_orderkey AS k, c_nationkey AS n FROM orders "
+                           "JOIN customer ON o_custkey = c_custkey AND o_prio = c_prio;")
+    assert rc == 0, ops.lib.alenka_last_error()
+    no = len(t["o_key"])
+    bounds = list(range(0, no, 16384)) + [no]
+    ol, orr = O.join_sortmerge(t["o_cust"], t["c_key"], bounds)
+    keep = t["o_prio"][ol] == t["c_prio"][orr]
+    assert ops.rows("S") == int(keep.sum())
+    assert np.array_equal(ops.fetch("S", "k"), t["o_key"][ol[keep]])
+    assert np.array_equal(ops.fetch("S", "n"), t["c_nat"][orr[keep]])
+    rc = ops.alenkaExecute("S := SELECT o_orderkey AS k FROM orders JOIN customer "
+                           "ON o_custkey = c_custkey AND o_prio = c_prio AND o_orderkey = c_nationkey;")
+    assert rc == L.ALN_E_UNSUPPORTED
+
+
+@pytest.mark.parametrize("script", [False, True])
+def test_anti_join_in_a_chain(ops, script):
+    """`nation JOIN customer ON ... LEFT ANTI JOIN orders ON ...`: the parser hands over the
+    innermost join first (bison.y:260-275), so customer LEFT ANTI JOIN orders runs first and
+    nation is joined with the customers that have no order."""
+    t = semi_anti_tables(ops, seed=5)
+    nat = np.arange(0, 25, dtype=np.int64)
+    ops.table("nation", 25, 64, [("n_nationkey", nat, 0), ("n_regionkey", nat % 5, 0)])
+    if script:
+        rc = ops.alenkaExecute("S := SELECT c_custkey AS k, n_regionkey AS rg FROM nation "
+                               "JOIN customer ON n_nationkey = c_nationkey "
+                               "LEFT ANTI JOIN orders ON c_custkey = o_custkey;")
+        assert rc == 0, ops.lib.alenka_last_error()
+    else:
+        ops.emit_name("c_custkey"); ops.emit_sel_name("k")
+        ops.emit_name("n_regionkey"); ops.emit_sel_name("rg")
+        ops.emit_name("n_nationkey"); ops.emit_name("c_nationkey"); ops.emit_eq()
+        ops.emit_name("c_custkey"); ops.emit_name("o_custkey"); ops.emit_eq()
+        ops.emit_join_tab(b"orders", b"3")
+        ops.emit_join_tab(b"customer", b"I")
+        ops.emit_join("S", "nation", 1, 0, -1)
+        ops.check(None, "join")
+    anti = O.semi_anti([t["c_key"]], [t["o_cust"]], "3")
+    ln, rc_ = O.join_sortmerge(nat, t["c_nat"][anti])
+    assert ops.rows("S") == len(ln)
+    assert np.array_equal(ops.fetch("S", "k"), t["c_key"][anti][rc_])
+    assert np.array_equal(ops.fetch("S", "rg"), (nat % 5)[ln])
+
+
+def test_unsupported_join_kinds(ops):
+    semi_anti_tables(ops)
+    for words in ("LEFT", "RIGHT", "OUTER"):
+        rc = ops.alenkaExecute(f"S := SELECT o_orderkey AS k FROM orders {words} JOIN customer ON o_custkey = c_custkey;")
+        assert rc == L.ALN_E_UNSUPPORTED, words

@@ -234,6 +234,26 @@ def test_hash_join(ctx, dups):
     assert sorted(zip(ol.tolist(), orr.tolist())) == sorted(zip(pi.tolist(), bi.tolist()))
 
 
+@pytest.mark.parametrize("nrows,nidx", [(1, 0), (31, 5), (32, 64), (100003, 250000), (5000, 0)])
+def test_join_rows(ctx, nrows, nidx):
+    """Semi/anti reduction: rows that occur among a join side's pair indices (with
+    duplicates), or do not, in ascending order - the oracle's sort + unique and set
+    difference (operators.cu:1357-1399)."""
+    from alenka_amd import device as D
+
+    r = np.random.default_rng(nrows + nidx)
+    idx = r.integers(0, nrows, nidx).astype(np.int32)
+    t = torch.as_tensor(idx if nidx else np.zeros(1, np.int32)).to(ctx.device)[:nidx]
+    semi = np.unique(idx)
+    anti = np.setdiff1d(np.arange(nrows), semi)
+    assert np.array_equal(host(D.join_rows(ctx, t, nrows, False)), semi.astype(np.int32))
+    assert np.array_equal(host(D.join_rows(ctx, t, nrows, True)), anti.astype(np.int32))
+    if len(anti) > 1:
+        with pytest.raises(L.AlnError) as e:
+            D.join_rows(ctx, t, nrows, True, cap=len(anti) - 1)
+        assert e.value.code == L.ALN_E_CAPACITY
+
+
 # ---- sort / top-k ------------------------------------------------------------------------
 @pytest.mark.parametrize("wide", [False, True])
 def test_sort_and_topk(ctx, wide):

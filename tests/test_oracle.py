@@ -336,6 +336,27 @@ def test_join_nested_loop():
         assert seq == sorted(seq)
 
 
+@pytest.mark.parametrize("nkeys", [1, 2])
+def test_semi_anti_brute_force(nkeys):
+    """Semi = the rows of a side with some equal-keyed row on the other side, anti = the
+    rest; left rows in row order, right rows in the stable order of the right's first key."""
+    r = np.random.default_rng(21 + nkeys)
+    for trial in range(30):
+        nl, nr = int(r.integers(0, 60)), int(r.integers(0, 60))
+        lk = [r.integers(0, 12, nl) for _ in range(nkeys)]
+        rk = [r.integers(0, 12, nr) for _ in range(nkeys)]
+        ltup = [tuple(int(k[i]) for k in lk) for i in range(nl)]
+        rtup = [tuple(int(k[i]) for k in rk) for i in range(nr)]
+        rset, lset = set(rtup), set(ltup)
+        bounds = [0, nl // 3, nl] if trial % 2 else None
+        want_l = [i for i in range(nl) if ltup[i] in rset]
+        want_r_rows = sorted(range(nr), key=lambda i: (int(rk[0][i]), i))
+        assert list(O.semi_anti(lk, rk, "1", bounds)) == want_l
+        assert list(O.semi_anti(lk, rk, "3", bounds)) == [i for i in range(nl) if ltup[i] not in rset]
+        assert list(O.semi_anti(lk, rk, "2", bounds)) == [i for i in want_r_rows if rtup[i] in lset]
+        assert list(O.semi_anti(lk, rk, "4", bounds)) == [i for i in want_r_rows if rtup[i] not in lset]
+
+
 def test_order_python_sorted():
     r = np.random.default_rng(9)
     a, b = r.integers(0, 10, 2000), r.integers(0, 5, 2000)
