@@ -136,6 +136,7 @@ SIGNATURES = {
     "aln_apply_perm": (C.c_int, [P, P, P, U64, P]),
     "aln_hash_join": (C.c_int, [P, P, U64, P, U64, P, P, U64, PU64]),
     "aln_join_rows": (C.c_int, [P, P, U64, U64, C.c_int, P, U64, PU64]),
+    "aln_dict_lookup": (C.c_int, [P, P, U64, P, U64, P]),
     "aln_groupby": (C.c_int, [P, C.POINTER(P), C.c_int, C.POINTER(P), PI32, C.c_int, U64,
                               C.c_int, U32, C.POINTER(P), C.POINTER(P), U64, PU64]),
     "aln_sort_keys": (C.c_int, [P, C.POINTER(P), PI32, C.c_int, U64, P]),

@@ -122,6 +122,16 @@ __global__ void __launch_bounds__(256) k_apply_perm(const int64_t *__restrict__ 
     out[i] = in[perm[i]];
 }
 
+// out[i] = table[codes[i]], -1 for a code outside the table.
+__global__ void __launch_bounds__(256) k_dict_lookup(const int64_t *__restrict__ codes, uint64_t n,
+                                                     const int64_t *__restrict__ table, uint64_t ntable,
+                                                     int64_t *__restrict__ out) {
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
+    const int64_t c = codes[i];
+    out[i] = (c >= 0 && (uint64_t)c < ntable) ? table[c] : -1;
+  }
+}
+
 // ---- fused scan: decode + predicate + compaction -------------------------------------------
 constexpr int SC_BLOCK = VM_BLOCK;  // 256
 constexpr int SC_ITEMS = 16;
@@ -458,6 +468,16 @@ int aln_apply_perm(aln_ctx *c, const int64_t *d_in, const uint32_t *d_perm, uint
   hipLaunchKernelGGL(k_apply_perm, dim3(grid_for(n, 256, 65536)), dim3(256), 0, c->stream, d_in,
                      d_perm, n, d_out);
   return check_launch("apply_perm");
+}
+
+int aln_dict_lookup(aln_ctx *c, const int64_t *d_codes, uint64_t n, const int64_t *d_table, uint64_t ntable,
+                    int64_t *d_out) {
+  ALN_REQUIRE(c && (n == 0 || (d_codes && d_out)) && (ntable == 0 || d_table), "aln_dict_lookup: bad arguments");
+  if (!n) return ALN_OK;
+  ProfScope p(c, "dict_lookup");
+  hipLaunchKernelGGL(k_dict_lookup, dim3(grid_for(n, 256, 65536)), dim3(256), 0, c->stream, d_codes, n, d_table,
+                     ntable, d_out);
+  return check_launch("dict_lookup");
 }
 
 int aln_scan_filter(aln_ctx *c, const aln_col *cols, int ncols, uint32_t seg_rows,

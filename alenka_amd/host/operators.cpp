@@ -1389,14 +1389,46 @@ static int do_order(const char *s, const char *f) {
   const uint64_t n = v.nrows;
   std::vector<DBufP> kb;
   std::vector<const int64_t *> kp;
-  for (auto &k : keys) {
+  std::vector<int32_t> kdesc;
+  for (size_t i = 0; i < keys.size(); ++i) {
+    const Col &kc = v.cols[keys[i]];
     DBufP d;
-    TRY(plain_copy(v.cols[k], n, v.seg_rows, &d));
+    TRY(plain_copy(kc, n, v.seg_rows, &d));
+    if (kc.dict) {
+      // char key: str_sort handles widths 1..101 and 255 and leaves the order as it is for
+      // any other width (strings_sort_device.cu:125-137), so such a key does not sort
+      if (!((kc.width >= 1 && kc.width <= 101) || kc.width == 255)) continue;
+      // rows sort by their string's rank in the dictionary's byte order (the strings are
+      // NUL-padded to the width and compared as unsigned bytes, strings_type.h:46-115)
+      const auto &dict = *kc.dict;
+      std::vector<uint32_t> ord(dict.size());
+      for (size_t j = 0; j < ord.size(); ++j) ord[j] = (uint32_t)j;
+      std::sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t b) { return dict[a] < dict[b]; });
+      std::vector<int64_t> rank(dict.size());
+      for (size_t j = 0; j < ord.size(); ++j) rank[ord[j]] = (int64_t)j;
+      DBufP table = dalloc(8 * (rank.size() ? rank.size() : 1));
+      DBufP r = dalloc(8 * (n ? n : 1));
+      if (!table || !r) return fail(ALN_E_NOMEM, "device allocation failed");
+      if (!rank.empty() && hipMemcpy(table->p, rank.data(), 8 * rank.size(), hipMemcpyHostToDevice) != hipSuccess)
+        return fail(ALN_E_HIP, "host to device copy failed");
+      ALN(aln_dict_lookup(g_ctx, (const int64_t *)d->p, n, (const int64_t *)table->p, rank.size(), (int64_t *)r->p),
+          "order");
+      ALN(aln_sync(g_ctx), "order");
+      d = r;
+    }
     kb.push_back(d);
     kp.push_back((const int64_t *)d->p);
+    kdesc.push_back(desc[i]);
   }
   DBufP perm = dalloc(4 * (n ? n : 1));
-  if (n) ALN(aln_sort_keys(g_ctx, kp.data(), desc.data(), (int)keys.size(), n, (uint32_t *)perm->p), "order");
+  if (n && kp.empty()) {
+    std::vector<uint32_t> id(n);
+    for (uint64_t i = 0; i < n; ++i) id[i] = (uint32_t)i;
+    if (hipMemcpy(perm->p, id.data(), 4 * n, hipMemcpyHostToDevice) != hipSuccess)
+      return fail(ALN_E_HIP, "host to device copy failed");
+  } else if (n) {
+    ALN(aln_sort_keys(g_ctx, kp.data(), kdesc.data(), (int)kp.size(), n, (uint32_t *)perm->p), "order");
+  }
   auto out = std::make_shared<Set>();
   out->not_compressed = true;
   out->nrows = n;

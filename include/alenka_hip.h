@@ -186,6 +186,12 @@ int aln_eval(aln_ctx *ctx, const aln_col *cols, int ncols, uint32_t seg_rows,
 /* d_out[i] = d_in[d_perm[i]] (apply_permutation, sorts.cu:86-125). */
 int aln_apply_perm(aln_ctx *ctx, const int64_t *d_in, const uint32_t *d_perm, uint64_t n,
                    int64_t *d_out);
+/* d_out[i] = d_table[d_codes[i]] (-1 for a code outside 0..ntable-1).  ORDER BY on a char
+ * column maps each row's dictionary code to the rank of its string in byte order and sorts
+ * the ranks; it replaces the per-row string fetch and string sort of
+ * update_char_permutation + str_sort (cm.cu:4218-4260, strings_sort_device.cu:125-137). */
+int aln_dict_lookup(aln_ctx *ctx, const int64_t *d_codes, uint64_t n, const int64_t *d_table,
+                    uint64_t ntable, int64_t *d_out);
 
 /* ---- hash join (replaces the sort-merge join of operators.cu:1117-1284) -------------
  * Inner equi-join on int64 keys.  Pairs are written in probe-row order and, within one

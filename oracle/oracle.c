@@ -20,6 +20,8 @@
  *                          quirks.
  *   orc_groupby_*        - brute-force dictionary aggregation (Python) + ordering rules.
  *   orc_join_sortmerge   - nested-loop multiset + the documented output order.
+ *   orc_str_sort         - Python's stable sorted() on the values' bytes, and the
+ *                          widths the reference leaves unsorted.
  *   orc_semi_anti        - Python set membership over all key pairs (brute force), and the
  *                          stable key order of the right side.
  *   orc_order            - Python's stable sorted() with a composite key.
@@ -1013,6 +1015,39 @@ uint64_t orc_join_sortmerge(const int64_t *lkeys, uint64_t nl, const int64_t *rk
   }
   free(rp);
   return k;
+}
+
+/* One string key of ORDER BY, restated from str_sort (strings_sort_device.cu:21-31, 125-137)
+ * and Str::comparison (strings_type.h:46-115): perm (row ids, in their current order) is
+ * stably sorted by the rows' width-byte values vals[row * width ..], compared as unsigned
+ * bytes, descending when desc.  Widths the reference has no sort for (above 101, other than
+ * 255) leave perm unchanged. */
+static const char *g_str_vals;
+static uint32_t g_str_width;
+static int g_str_desc;
+static int cmp_str_rows(const void *a, const void *b) {
+  const uint32_t x = ((const uint32_t *)a)[0], y = ((const uint32_t *)b)[0];
+  int r = memcmp(g_str_vals + (uint64_t)x * g_str_width, g_str_vals + (uint64_t)y * g_str_width, g_str_width);
+  if (g_str_desc) r = -r;
+  if (r) return r;
+  /* the second word is the input position: ties keep their order (a stable sort) */
+  const uint32_t px = ((const uint32_t *)a)[1], py = ((const uint32_t *)b)[1];
+  return px < py ? -1 : px > py;
+}
+
+void orc_str_sort(const char *vals, uint32_t width, uint64_t n, int desc, uint32_t *perm) {
+  if (!((width >= 1 && width <= 101) || width == 255)) return;
+  uint32_t *pairs = (uint32_t *)malloc(8 * (n ? n : 1));
+  for (uint64_t i = 0; i < n; ++i) {
+    pairs[2 * i] = perm[i];
+    pairs[2 * i + 1] = (uint32_t)i;
+  }
+  g_str_vals = vals;
+  g_str_width = width;
+  g_str_desc = desc;
+  qsort(pairs, n, 8, cmp_str_rows);
+  for (uint64_t i = 0; i < n; ++i) perm[i] = pairs[2 * i];
+  free(pairs);
 }
 
 /* Semi and anti joins, restated from operators.cu:1357-1399 and 1473-1530.  The pairs of

@@ -38,6 +38,9 @@ void orc_gen_fact(uint64_t seed, uint64_t norders, int64_t *l_okey, int64_t *l_p
                   int64_t *l_disc, int64_t *l_ship);
 void orc_dict_first_seen(const int64_t *vals, uint64_t n, int64_t nvals, int64_t *code_of_val,
                          int64_t *codes);
+/* One char[width] ORDER BY key (strings_sort_device.cu:125-137): stable sort of the row ids
+ * in perm by their values, unsigned byte order, descending when desc */
+void orc_str_sort(const char *vals, uint32_t width, uint64_t n, int desc, uint32_t *perm);
 /* Semi ('1' left, '2' right) and anti ('3' left, '4' right) joins on nkeys equal keys
  * (operators.cu:1357-1399, 1473-1530); returns the kept rows of that side in out */
 uint64_t orc_semi_anti(const int64_t *const *lkeys, const int64_t *const *rkeys, int nkeys, uint64_t nl,

@@ -206,6 +206,14 @@ def join_sortmerge(lkeys, rkeys, lseg_bounds=None):
     return ol[:need], orr[:need]
 
 
+def str_sort(values, width, perm, desc=False):
+    """Stable sort of the row ids `perm` by char[width] values (str list), unsigned byte order."""
+    buf = b"".join(v.encode()[:width].ljust(width, b"\0") for v in values)
+    p = np.ascontiguousarray(perm, np.uint32).copy()
+    load().orc_str_sort(buf, C.c_uint32(width), C.c_uint64(len(p)), int(desc), _ptr(p))
+    return p
+
+
 def semi_anti(lkeys, rkeys, kind, lseg_bounds=None):
     """kind '1' LEFT SEMI, '2' RIGHT SEMI, '3' LEFT ANTI, '4' RIGHT ANTI; lkeys/rkeys: lists
     of equally long key columns (first = the sort-merge key).  Returns the kept rows."""

@@ -341,3 +341,36 @@ def test_unsupported_join_kinds(ops):
     for words in ("LEFT", "RIGHT", "OUTER"):
         rc = ops.alenkaExecute(f"S := SELECT o_orderkey AS k FROM orders {words} JOIN customer ON o_custkey = c_custkey;")
         assert rc == L.ALN_E_UNSUPPORTED, words
+
+
+# ---- ORDER BY on char columns (update_char_permutation cm.cu:4218-4260, str_sort
+# strings_sort_device.cu:125-137) ---------------------------------------------------------------
+def reference_order(keys, n):
+    """The reference's ORDER BY: one stable sort per key, the last key first
+    (operators.cu:1754-1762); keys are (values, width or None, desc)."""
+    perm = np.arange(n, dtype=np.uint32)
+    for vals, width, desc in reversed(keys):
+        if width is None:
+            perm = perm[O.order([np.asarray(vals)[perm]], [int(desc)])]
+        else:
+            perm = O.str_sort(vals, width, perm, desc)
+    return perm
+
+
+@pytest.mark.parametrize("width", [10, 4, 120])
+def test_script_order_by_char(ops, width):
+    r = np.random.default_rng(width)
+    words = ["PROMO", "AUTOMOBILE", "BUILDING", "", "MACHINERY", "A", "Z", "HOUSEHOLD", "FURNITURE", "AB"]
+    n = 5000
+    name = [words[i][:width] for i in r.integers(0, len(words), n)]
+    grp = r.integers(0, 7, n).astype(np.int64)
+    ids = np.arange(n, dtype=np.int64)
+    ops.table("T", n, 1024, [("id", ids, 0), ("name", name, width), ("grp", grp, 0)])
+    assert ops.alenkaExecute("O1 := ORDER T BY name ASC, grp DESC;\nO2 := ORDER T BY grp ASC, name DESC;") == 0, \
+        ops.lib.alenka_last_error()
+    want1 = reference_order([(name, width, False), (grp, None, True)], n)
+    want2 = reference_order([(grp, None, False), (name, width, True)], n)
+    assert np.array_equal(ops.fetch("O1", "id"), ids[want1])
+    assert np.array_equal(ops.fetch("O2", "id"), ids[want2])
+    if width == 120:  # no string sort at this width: the int key alone orders the rows
+        assert np.array_equal(want2, np.argsort(grp, kind="stable"))

@@ -357,6 +357,26 @@ def test_semi_anti_brute_force(nkeys):
         assert list(O.semi_anti(lk, rk, "4", bounds)) == [i for i in want_r_rows if rtup[i] not in lset]
 
 
+@pytest.mark.parametrize("width", [1, 3, 8, 10, 101, 255])
+def test_str_sort_python_sorted(width):
+    """char ORDER BY key: Python's stable sorted() on the NUL-padded bytes (unsigned), from a
+    shuffled starting order; descending keeps ties in their order too."""
+    r = np.random.default_rng(width)
+    alpha = ["", "A", "AB", "ABC", "B", "a", "\x7f", "\u00e9", "Z Z", "AB C"]
+    vals = [alpha[i] for i in r.integers(0, len(alpha), 300)]
+    start = r.permutation(300).astype(np.uint32)
+    padded = [v.encode()[:width].ljust(width, b"\0") for v in vals]
+    asc = sorted(start.tolist(), key=lambda i: padded[i])
+    assert list(O.str_sort(vals, width, start)) == asc
+    neg = sorted(start.tolist(), key=lambda i: bytes(255 - b for b in padded[i]))
+    assert list(O.str_sort(vals, width, start, desc=True)) == neg
+
+
+def test_str_sort_unsupported_width_keeps_order():
+    perm = np.array([3, 1, 2, 0], np.uint32)
+    assert list(O.str_sort(["d", "c", "b", "a"], 120, perm)) == [3, 1, 2, 0]
+
+
 def test_order_python_sorted():
     r = np.random.default_rng(9)
     a, b = r.integers(0, 10, 2000), r.integers(0, 5, 2000)
