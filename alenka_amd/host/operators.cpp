@@ -80,7 +80,17 @@ struct Set {
   DBufP sel;  // uint32 row ids of the source
   uint64_t nsel = 0;
   bool sel_ready = false;
+  // a table read from the reference's segment files keeps those files' segment boundaries
+  // (first row of each segment, then nrows; empty: every segment has seg_rows rows), which
+  // decide the group-by merge order and the join's pair order
+  std::vector<uint64_t> seg_starts;
+  std::vector<std::string> sorted_fields, presorted_fields;  // the .sort / .presort lists
   uint32_t nsegs() const { return (uint32_t)((nrows + seg_rows - 1) / seg_rows); }
+  uint32_t file_segs() const { return seg_starts.empty() ? nsegs() : (uint32_t)(seg_starts.size() - 1); }
+  int64_t seg_of(uint64_t row) const {
+    if (seg_starts.empty()) return (int64_t)(row / seg_rows);
+    return (int64_t)(std::upper_bound(seg_starts.begin(), seg_starts.end(), row) - seg_starts.begin()) - 1;
+  }
 };
 
 // ---- session state (the reference's globals, cm.h:66-117, operators.cu:20-50) -------------
@@ -124,9 +134,23 @@ int ensure_ctx() {
   return ALN_OK;
 }
 
+// data_dict (cm.h, load_col_data / save_col_data cm.cu:4057-4114): table -> column ->
+// {col_type 0 int/decimal/timestamp, 1 float, 2 char; col_length decimal digits, UINT_MAX for a
+// timestamp, or the char width}
+struct ColData {
+  uint32_t type = 0, length = 0;
+};
+std::map<std::string, std::map<std::string, ColData>> g_data_dict;
+int load_binary(const std::string &name, const std::string &file);
+
+// A variable by name; a table that is not a variable yet but is in data_dict is read from its
+// segment files first, as load_vars (operators.cu:2708-2747) loads every table a script uses.
 std::shared_ptr<Set> find_var(const std::string &n) {
   auto it = varNames.find(n);
-  return it == varNames.end() ? nullptr : it->second;
+  if (it != varNames.end()) return it->second;
+  if (!n.empty() && g_data_dict.count(n) && ensure_ctx() == ALN_OK && load_binary(n, n) == ALN_OK)
+    return varNames[n];
+  return nullptr;
 }
 
 // ---- column construction ---------------------------------------------------------------
@@ -378,7 +402,7 @@ int make_view(const std::string &name, const std::set<std::string> &need, View *
     }
     v->nrows = a->nrows;
     v->seg_rows = a->seg_rows;
-    v->cycles = a->not_compressed ? 1 : a->nsegs();
+    v->cycles = a->not_compressed ? 1 : a->file_segs();
     v->rowids = nullptr;
     return ALN_OK;
   }
@@ -387,7 +411,7 @@ int make_view(const std::string &name, const std::set<std::string> &need, View *
   const uint64_t k = a->nsel;
   v->nrows = k;
   v->seg_rows = (uint32_t)(k ? (k < 0xFFFFFFFFull ? k : 0xFFFFFFFFull) : 1);
-  v->cycles = src->not_compressed ? 1 : src->nsegs();
+  v->cycles = src->not_compressed ? 1 : src->file_segs();
   for (auto &n : need) {
     if (!src->cols.count(n)) return fail(ALN_E_ARG, "Couldn't find column %s", n.c_str());
     const Col &sc = src->cols[n];
@@ -1095,9 +1119,9 @@ static int join_pair(const std::string &name, const std::string &left, const std
     }
     auto lsrc = find_var(left);
     auto lbase = lsrc->filtered ? find_var(lsrc->source) : lsrc;
-    const uint64_t lseg_rows = lbase->not_compressed ? (lbase->nrows ? lbase->nrows : 1) : lbase->seg_rows;
-    std::vector<int64_t> seg(n);
-    for (uint64_t i = 0; i < n; ++i) seg[i] = lrow[hp[i]] / (int64_t)lseg_rows;
+    std::vector<int64_t> seg(n, 0);
+    if (!lbase->not_compressed)
+      for (uint64_t i = 0; i < n; ++i) seg[i] = lbase->seg_of((uint64_t)lrow[hp[i]]);
     DBufP dseg = dalloc(8 * n);
     if (hipMemcpy(dseg->p, seg.data(), 8 * n, hipMemcpyHostToDevice) != hipSuccess)
       return fail(ALN_E_HIP, "host to device copy failed");
@@ -1522,3 +1546,305 @@ void emit_display(const char *s, const char *sep) {
 }
 
 }  // extern "C"
+
+// ---- binary tables: the reference's on-disk format ------------------------------------------
+// <f>.<col>.header       u64 rows, u32 segments, u32 most rows in a segment, u32 largest
+//                        payload (writeHeader, cm.cu:1251-1263)
+// <f>.<col>.<seg>        int / decimal segment, FOR or FOR-DELTA (compress.cu:388-399, 511-522)
+// <f>.<col>              char dictionary: width-byte NUL-padded entries, code = entry index
+// <f>.<col>.<seg>.idx    char codes of a segment, FOR; <f>.<col>.<seg>.hash: u32 rows, then
+//                        MurmurHash64A(entry, width, hash_seed) / 2 per row (compress_char,
+//                        cm.cu:1765-1844)
+// <f>.sort, <f>.presort  u32 count, then (u32 length, name) per sorted column (cm.cu:1279-1324)
+namespace {
+
+bool read_file(const std::string &path, std::vector<char> *out) {
+  FILE *fp = fopen(path.c_str(), "rb");
+  if (!fp) return false;
+  out->clear();
+  char buf[1 << 16];
+  size_t got;
+  while ((got = fread(buf, 1, sizeof(buf), fp)) > 0) out->insert(out->end(), buf, buf + got);
+  fclose(fp);
+  return true;
+}
+
+bool write_file(const std::string &path, const void *data, size_t len) {
+  FILE *fp = fopen(path.c_str(), "wb");
+  if (!fp) return false;
+  const bool ok = len == 0 || fwrite(data, 1, len, fp) == len;
+  return fclose(fp) == 0 && ok;
+}
+
+int read_name_list(const std::string &path, std::vector<std::string> *out) {
+  std::vector<char> b;
+  out->clear();
+  if (!read_file(path, &b)) return ALN_OK;  // no such list
+  uint32_t n = 0;
+  if (b.size() < 4) return fail(ALN_E_ARG, "%s is truncated", path.c_str());
+  memcpy(&n, b.data(), 4);
+  size_t p = 4;
+  for (uint32_t i = 0; i < n; ++i) {
+    uint32_t len = 0;
+    if (p + 4 > b.size()) return fail(ALN_E_ARG, "%s is truncated", path.c_str());
+    memcpy(&len, b.data() + p, 4);
+    p += 4;
+    if (p + len > b.size()) return fail(ALN_E_ARG, "%s is truncated", path.c_str());
+    out->emplace_back(b.data() + p, len);
+    p += len;
+  }
+  return ALN_OK;
+}
+
+// emit_load_binary (operators.cu:2518-2562) with CudaSet::initialize (cm.cu:2849-2962): the
+// columns data_dict lists for `file`, in its order; the row and segment counts from the first
+// column's header; every segment decoded on the device (aln_segfile_decode) and re-encoded
+// into this build's bit-packed layout with the files' largest segment as seg_rows, the file
+// segment boundaries kept.
+int load_binary(const std::string &name, const std::string &file) {
+  auto dd = g_data_dict.find(file);
+  if (dd == g_data_dict.end() || dd->second.empty())
+    return fail(ALN_E_ARG, "Couldn't find %s in the data dictionary", file.c_str());
+  const std::string first = dd->second.begin()->first;
+  std::vector<char> hb;
+  if (!read_file(file + "." + first + ".header", &hb) || hb.size() < 16)
+    return fail(ALN_E_ARG, "Couldn't open file %s.%s.header", file.c_str(), first.c_str());
+  uint64_t total = 0;
+  uint32_t nseg = 0, max_recs = 0;
+  memcpy(&total, hb.data(), 8);
+  memcpy(&nseg, hb.data() + 8, 4);
+  memcpy(&max_recs, hb.data() + 12, 4);
+  auto s = std::make_shared<Set>();
+  s->nrows = total;
+  s->seg_rows = max_recs ? max_recs : 1;
+  TRY(read_name_list(file + ".sort", &s->sorted_fields));
+  TRY(read_name_list(file + ".presort", &s->presorted_fields));
+  DBufP vals = dalloc(8 * (total ? total : 1));
+  if (!vals) return fail(ALN_E_NOMEM, "device allocation failed");
+  for (auto &kv : dd->second) {
+    const std::string &cn = kv.first;
+    const ColData cd = kv.second;
+    if (cd.type == 1) return fail(ALN_E_UNSUPPORTED, "float column %s", cn.c_str());
+    if (cd.type == 2 && cd.length == 0) return fail(ALN_E_ARG, "char column %s without a width", cn.c_str());
+    // the segments, back to back on the device
+    std::vector<uint64_t> starts;
+    uint64_t off = 0;
+    std::vector<char> fb;
+    for (uint32_t g = 0; g < nseg; ++g) {
+      const std::string path = file + "." + cn + "." + std::to_string(g) + (cd.type == 2 ? ".idx" : "");
+      if (!read_file(path, &fb)) return fail(ALN_E_ARG, "Error opening %s file", path.c_str());
+      uint64_t rows = 0;
+      int64_t lo = 0, hi = 0;
+      ALN(aln_segfile_decode(g_ctx, fb.data(), fb.size(), (int64_t *)vals->p + off, total - off, &rows, &lo, &hi),
+          path.c_str());
+      starts.push_back(off);
+      off += rows;
+    }
+    if (off != total)
+      return fail(ALN_E_ARG, "%s: segments hold %llu rows, header %llu", cn.c_str(), (unsigned long long)off,
+                  (unsigned long long)total);
+    starts.push_back(total);
+    if (s->seg_starts.empty())
+      s->seg_starts = starts;
+    else if (s->seg_starts != starts)
+      return fail(ALN_E_ARG, "column %s is segmented differently", cn.c_str());
+    ALN(aln_sync(g_ctx), "load");
+    const int decimals = (cd.type == 2 || cd.length == 0xFFFFFFFFu) ? 0 : (int)cd.length;
+    Col c;
+    TRY(make_packed((const int64_t *)vals->p, total, s->seg_rows, decimals, &c));
+    if (cd.type == 2) {
+      std::vector<char> db;
+      if (!read_file(file + "." + cn, &db)) return fail(ALN_E_ARG, "Couldn't open file %s.%s", file.c_str(), cn.c_str());
+      auto dict = std::make_shared<std::vector<std::string>>();
+      for (size_t e = 0; e + cd.length <= db.size(); e += cd.length)
+        dict->emplace_back(db.data() + e, strnlen(db.data() + e, cd.length));
+      c.type = 2;
+      c.dict = dict;
+      c.width = cd.length;
+    }
+    s->names.push_back(cn);
+    s->cols[cn] = c;
+  }
+  varNames[name] = s;
+  return ALN_OK;
+}
+
+// rows per stored segment: -l (bison.y:299-308), 10^9 unless set
+uint64_t g_process_count = 1000000000ull;
+
+// emit_store_binary (operators.cu:2446-2516) for a variable held on the device: data_dict is
+// updated (CudaSet::Store, cm.cu:1685-1701) and each column is written in segments of
+// g_process_count rows the way the text-source path of CudaSet::compress writes them
+// (cm.cu:1034-1240): FOR segment files, the char dictionary rebuilt in first-seen order of the
+// stored rows with .idx and .hash files, a header per column; .sort and .presort are removed.
+int store_binary(const std::string &var, const std::string &file) {
+  auto names = all_cols(var);
+  View v;
+  TRY(make_view(var, std::set<std::string>(names.begin(), names.end()), &v));
+  const uint64_t n = v.nrows, pc = g_process_count;
+  if (n > 0xFFFFFFFFull * pc) return fail(ALN_E_UNSUPPORTED, "Store : too many rows");
+  const uint32_t nseg = (uint32_t)((n + pc - 1) / pc);
+  const uint32_t most = (uint32_t)std::min<uint64_t>(n, pc);
+  std::map<std::string, ColData> cols;
+  std::vector<char> fb;
+  for (auto &cn : names) {
+    const Col &c = v.cols[cn];
+    DBufP d;
+    TRY(plain_copy(c, n, v.seg_rows, &d));
+    ColData cd;
+    cd.type = c.dict ? 2 : 0;
+    cd.length = c.dict ? c.width : (uint32_t)c.decimals;
+    if (c.dict) {
+      // the stored rows' own dictionary, first-seen order (compress_char, cm.cu:1808-1821)
+      std::vector<int64_t> h(n);
+      if (n && hipMemcpy(h.data(), d->p, 8 * n, hipMemcpyDeviceToHost) != hipSuccess)
+        return fail(ALN_E_HIP, "device to host copy failed");
+      std::map<int64_t, int64_t> recode;
+      std::vector<char> dfile, hfile;
+      for (uint64_t i = 0; i < n; ++i) {
+        auto it = recode.find(h[i]);
+        if (it == recode.end()) {
+          it = recode.emplace(h[i], (int64_t)recode.size()).first;
+          std::string e = (h[i] >= 0 && (size_t)h[i] < c.dict->size()) ? (*c.dict)[(size_t)h[i]] : std::string();
+          e.resize(c.width, '\0');
+          dfile.insert(dfile.end(), e.begin(), e.end());
+        }
+        h[i] = it->second;
+      }
+      if (!write_file(file + "." + cn, dfile.data(), dfile.size()))
+        return fail(ALN_E_ARG, "Could not write %s.%s", file.c_str(), cn.c_str());
+      if (n && hipMemcpy(d->p, h.data(), 8 * n, hipMemcpyHostToDevice) != hipSuccess)
+        return fail(ALN_E_HIP, "host to device copy failed");
+      for (uint32_t g = 0; g < nseg; ++g) {
+        const uint64_t r0 = (uint64_t)g * pc, r = std::min<uint64_t>(pc, n - r0);
+        const uint32_t r32 = (uint32_t)r;
+        hfile.assign(4 + 8 * r, 0);
+        memcpy(hfile.data(), &r32, 4);
+        for (uint64_t i = 0; i < r; ++i) {
+          const char *e = dfile.data() + (size_t)h[r0 + i] * c.width;
+          const uint64_t hv = aln_murmur64a(e, (int)c.width, g_hash_seed) / 2;
+          memcpy(hfile.data() + 4 + 8 * i, &hv, 8);
+        }
+        const std::string hp = file + "." + cn + "." + std::to_string(g) + ".hash";
+        if (!write_file(hp, hfile.data(), hfile.size())) return fail(ALN_E_ARG, "Could not write %s", hp.c_str());
+      }
+    }
+    uint32_t largest = 0;
+    for (uint32_t g = 0; g < nseg; ++g) {
+      const uint64_t r0 = (uint64_t)g * pc, r = std::min<uint64_t>(pc, n - r0);
+      uint64_t len = 0;
+      ALN(aln_segfile_encode(g_ctx, (const int64_t *)d->p + r0, r, nullptr, 0, &len), "store");
+      fb.resize(len);
+      ALN(aln_segfile_encode(g_ctx, (const int64_t *)d->p + r0, r, fb.data(), fb.size(), &len), "store");
+      uint32_t cnt = 0;
+      memcpy(&cnt, fb.data(), 4);
+      largest = std::max(largest, cnt);
+      const std::string sp = file + "." + cn + "." + std::to_string(g) + (c.dict ? ".idx" : "");
+      if (!write_file(sp, fb.data(), fb.size())) return fail(ALN_E_ARG, "Could not write %s", sp.c_str());
+    }
+    char hdr[20];
+    memcpy(hdr, &n, 8);
+    memcpy(hdr + 8, &nseg, 4);
+    memcpy(hdr + 12, &most, 4);
+    memcpy(hdr + 16, &largest, 4);
+    if (!write_file(file + "." + cn + ".header", hdr, sizeof(hdr)))
+      return fail(ALN_E_ARG, "Could not write %s.%s.header", file.c_str(), cn.c_str());
+    cols[cn] = cd;
+  }
+  g_data_dict[file] = cols;
+  remove((file + ".sort").c_str());
+  remove((file + ".presort").c_str());
+  return ALN_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void emit_load_binary(const char *s, const char *f, int d) {
+  (void)d;
+  g_status = ALN_OK;
+  if (ensure_ctx() == ALN_OK) {
+    if (!s || !f)
+      fail(ALN_E_ARG, "emit_load_binary: bad arguments");
+    else
+      load_binary(s, f);
+  }
+  finish_statement();
+}
+
+void emit_store_binary(const char *s, const char *f, int append) {
+  g_status = ALN_OK;
+  if (append)
+    fail(ALN_E_UNSUPPORTED, "Store : APPEND is not supported");
+  else if (!f || !find_var(s ? s : ""))
+    fail(ALN_E_ARG, "Store : couldn't find variable %s", s ? s : "");
+  else
+    store_binary(s, f);
+  finish_statement();
+}
+
+void alenka_set_process_count(uint64_t rows) { g_process_count = rows ? rows : 1; }
+
+int alenka_load_col_data(const char *file) {
+  g_status = ALN_OK;
+  std::vector<char> b;
+  if (!file || !read_file(file, &b)) return fail(ALN_E_ARG, "Couldn't open data dictionary");
+  size_t p = 0;
+  auto rd = [&](void *dst, size_t len) {
+    if (p + len > b.size()) return false;
+    memcpy(dst, b.data() + p, len);
+    p += len;
+    return true;
+  };
+  auto rds = [&](std::string *out) {
+    uint64_t len = 0;
+    if (!rd(&len, 8) || p + len > b.size()) return false;
+    out->assign(b.data() + p, (size_t)len);
+    p += (size_t)len;
+    return true;
+  };
+  uint64_t ntab = 0;
+  if (!rd(&ntab, 8)) return fail(ALN_E_ARG, "data dictionary %s is truncated", file);
+  for (uint64_t t = 0; t < ntab; ++t) {
+    std::string tab;
+    uint64_t ncol = 0;
+    if (!rds(&tab) || !rd(&ncol, 8)) return fail(ALN_E_ARG, "data dictionary %s is truncated", file);
+    for (uint64_t c = 0; c < ncol; ++c) {
+      std::string col;
+      ColData cd;
+      if (!rds(&col) || !rd(&cd.type, 4) || !rd(&cd.length, 4))
+        return fail(ALN_E_ARG, "data dictionary %s is truncated", file);
+      g_data_dict[tab][col] = cd;
+    }
+  }
+  return ALN_OK;
+}
+
+int alenka_save_col_data(const char *file) {
+  g_status = ALN_OK;
+  std::vector<char> b;
+  auto put = [&](const void *src, size_t len) { b.insert(b.end(), (const char *)src, (const char *)src + len); };
+  auto put_str = [&](const std::string &str) {
+    const uint64_t len = str.size();
+    put(&len, 8);
+    put(str.data(), str.size());
+  };
+  const uint64_t ntab = g_data_dict.size();
+  put(&ntab, 8);
+  for (auto &t : g_data_dict) {
+    put_str(t.first);
+    const uint64_t ncol = t.second.size();
+    put(&ncol, 8);
+    for (auto &c : t.second) {
+      put_str(c.first);
+      put(&c.second.type, 4);
+      put(&c.second.length, 4);
+    }
+  }
+  if (!file || !write_file(file, b.data(), b.size())) return fail(ALN_E_ARG, "Could not write the data dictionary");
+  return ALN_OK;
+}
+
+}  // extern "C"
+

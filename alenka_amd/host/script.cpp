@@ -8,9 +8,10 @@
 //   V := SELECT expr AS name, ... FROM T [JOIN U ON expr ...] [GROUP BY expr, ...];
 //   V := ORDER T BY name [ASC|DESC], ...;
 //   STORE V INTO 'file' USING ('sep') [LIMIT n];   DISPLAY V USING ('sep') [LIMIT n];
+//   STORE V INTO 'file' [LIMIT n] BINARY;   (tables in the data dictionary load by name)
 //   DROP TABLE V;
-// Others (LOAD, binary STORE, DESCRIBE, INSERT, DELETE, SHOW, CREATE INDEX, outer and semi
-// joins) fail with ALN_E_UNSUPPORTED.  Operator precedence is the grammar's (bison.y:41-57):
+// Others (text LOAD, APPEND, sorted binary STORE, DESCRIBE, INSERT, DELETE, SHOW, CREATE
+// INDEX, outer joins) fail with ALN_E_UNSUPPORTED.  Operator precedence is the grammar's (bison.y:41-57):
 // '=' binds loosest, then '<>', OR, XOR, AND, LIKE, NOT, comparisons, + -, * / %.  Numbers
 // are lexed as the reference lexes them: "-?[0-9]+" is one token, so "a-1" is NAME INTNUM.
 #include <ctype.h>
@@ -350,8 +351,19 @@ struct Parser {
       file = cur.s;
       if (!advance()) return false;
       if (!kw("USING")) {
-        error("binary STORE is not supported by this build");
-        return false;
+        // STORE V INTO 'f' [APPEND] [LIMIT n] BINARY [sort_def] (bison.y:148-151)
+        bool append = false;
+        if (kw("APPEND")) {
+          append = true;
+          if (!advance()) return false;
+        }
+        if (!opt_limit() || !expect_kw("BINARY")) return false;
+        if (kw("SORT") || kw("PRESORTED")) {
+          error("binary STORE with a sort order is not supported by this build");
+          return false;
+        }
+        emit_store_binary(v.c_str(), file.c_str(), append ? 1 : 0);
+        return true;
       }
       if (!using_sep(sep) || !opt_limit()) return false;
       emit_store(v.c_str(), file.c_str(), sep.c_str());

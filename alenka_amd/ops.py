@@ -45,6 +45,11 @@ _SIG = {
     "emit_order": (None, [C.c_char_p, C.c_char_p, C.c_int, C.c_int]),
     "emit_store": (None, [C.c_char_p, C.c_char_p, C.c_char_p]),
     "emit_display": (None, [C.c_char_p, C.c_char_p]),
+    "emit_store_binary": (None, [C.c_char_p, C.c_char_p, C.c_int]),
+    "emit_load_binary": (None, [C.c_char_p, C.c_char_p, C.c_int]),
+    "alenka_set_process_count": (None, [C.c_uint64]),
+    "alenka_load_col_data": (C.c_int, [C.c_char_p]),
+    "alenka_save_col_data": (C.c_int, [C.c_char_p]),
     "alenkaExecute": (C.c_int, [C.c_char_p]),
 }
 

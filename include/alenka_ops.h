@@ -10,16 +10,18 @@
  *   emit_select  operators.h:55  (operators.cu:1801-2084)  projection / GROUP BY
  *   emit_join    operators.h:56  (operators.cu:797-928)    inner equi-join chain
  *   emit_order   operators.h:53  (operators.cu:1641-1798)  stable multi-key ORDER BY
- *   emit_store   operators.h:47  (operators.cu:2410-2443)  text output with LIMIT
+ *   emit_store   operators.h:48  (operators.cu:2410-2443)  text output with LIMIT
  *   emit_display operators.h:59  (operators.cu:2446-2479)  text output to stdout
+ *   emit_store_binary / emit_load_binary (operators.cu:2446-2562)  the segment files
  *
  * Errors: the reference reports through process_error (operators.cu:2752-2765), which
  * exits the process for severity > 1.  Here process_error records the message, every
  * statement call leaves its status in alenka_status(), and nothing exits.
  *
- * Tables are registered from host or device arrays (the reference loads them from its
- * segment files, emit_load_binary operators.cu:2518-2562; reading that file format is not
- * part of this library).  Loaded int/decimal columns are stored frame-of-reference +
+ * Tables are registered from host or device arrays, or read from the reference's segment
+ * files (emit_load_binary operators.cu:2518-2562, and implicitly for every table the loaded
+ * data dictionary lists, as load_vars operators.cu:2708-2747 does).  Int/decimal columns are
+ * stored frame-of-reference +
  * bit-packed in segments of seg_rows rows with min/max zone maps; char(n) columns are
  * dictionary-encoded in first-seen order (compress_char, cm.cu:1765-1844).
  */
@@ -38,6 +40,13 @@ int alenka_close(void);                  /* frees every variable and the device 
 int alenka_status(void);                 /* status of the last statement (ALN_OK = 0)     */
 const char *alenka_last_error(void);
 void alenka_set_hash_seed(uint32_t seed); /* hash_seed, 100 in the file driver (bison.y:304) */
+/* process_count: rows per segment that emit_store_binary writes (-l, bison.y:299-308; 10^9) */
+void alenka_set_process_count(uint64_t rows);
+/* load_col_data / save_col_data (cm.cu:4057-4114): read or write the data dictionary file
+ * (table -> column -> type, length) the segment files are described by; the reference reads
+ * "data.dictionary" at start-up (bison.y:333) and writes it after a binary STORE. */
+int alenka_load_col_data(const char *file);
+int alenka_save_col_data(const char *file);
 
 /* ---- tables ---------------------------------------------------------------------------- */
 int alenka_table_create(const char *name, uint64_t nrows, uint32_t seg_rows);
@@ -95,6 +104,14 @@ void emit_join(const char *s, const char *j1, int grp, int start_seg, int end_se
 void emit_order(const char *s, const char *f, int e, int ll);
 void emit_store(const char *s, const char *f, const char *sep);
 void emit_display(const char *s, const char *sep);
+/* emit_store_binary (operators.h:49, operators.cu:2446-2516): the variable's columns as the
+ * reference's segment files <f>.<col>.<seg> (char: dictionary <f>.<col>, .idx and .hash per
+ * segment) with a .header per column, and its entry in the data dictionary (save it with
+ * alenka_save_col_data).  append != 0 is ALN_E_UNSUPPORTED. */
+void emit_store_binary(const char *s, const char *f, int append);
+/* emit_load_binary (operators.h:47, operators.cu:2518-2562): variable s from the segment files
+ * with prefix f, described by the data dictionary's entry for f; .sort/.presort are read. */
+void emit_load_binary(const char *s, const char *f, int d);
 
 /* A script in the reference's statement language (alenkaExecute, alenka.h:31; grammar
  * bison.y:127-287): FILTER, SELECT [JOIN ... ON] [GROUP BY], ORDER, STORE ... USING,

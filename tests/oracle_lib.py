@@ -128,6 +128,92 @@ def pfor_decode(buf: bytes, cap):
     return out[:n]
 
 
+# ---- the reference's binary table files (written here as the reference writes them) ----------
+def data_dictionary_bytes(dd):
+    """save_col_data (cm.cu:4057-4080): u64 table count; per table (sorted) u64 name length,
+    name, u64 column count; per column (sorted) u64 name length, name, u32 type, u32 length."""
+    import struct
+
+    out = [struct.pack("<Q", len(dd))]
+    for t in sorted(dd):
+        out += [struct.pack("<Q", len(t)), t.encode(), struct.pack("<Q", len(dd[t]))]
+        for c in sorted(dd[t]):
+            typ, length = dd[t][c]
+            out += [struct.pack("<Q", len(c)), c.encode(), struct.pack("<II", typ, length)]
+    return b"".join(out)
+
+
+def parse_data_dictionary(b):
+    """load_col_data (cm.cu:4082-4114): the inverse of data_dictionary_bytes."""
+    import struct
+
+    p, out = 8, {}
+    for _ in range(struct.unpack_from("<Q", b)[0]):
+        ln = struct.unpack_from("<Q", b, p)[0]
+        t = b[p + 8:p + 8 + ln].decode()
+        p += 8 + ln
+        nc = struct.unpack_from("<Q", b, p)[0]
+        p += 8
+        out[t] = {}
+        for _ in range(nc):
+            ln = struct.unpack_from("<Q", b, p)[0]
+            c = b[p + 8:p + 8 + ln].decode()
+            p += 8 + ln
+            out[t][c] = struct.unpack_from("<II", b, p)
+            p += 8
+    assert p == len(b)
+    return out
+
+
+def header_bytes(total, nseg, max_recs, max_payload):
+    """writeHeader (cm.cu:1251-1263): u64 rows, u32 segments, u32 most rows, u32 largest payload."""
+    import struct
+
+    return struct.pack("<QIII", total, nseg, max_recs, max_payload)
+
+
+def write_binary_table(prefix, columns, seg_sizes, hash_seed=100):
+    """Files of one table as the reference's text loader leaves them (CudaSet::compress,
+    cm.cu:1034-1240, compress_char 1765-1844): columns are (name, kind, values, arg) with kind
+    "int" (arg: True for FOR-DELTA segments), "decimal" (arg: digits) or "char" (values: str
+    list, arg: width).  Returns the data dictionary entry {column: (type, length)}."""
+    import struct
+
+    bounds = np.concatenate([[0], np.cumsum(seg_sizes)]).astype(np.int64)
+    entry = {}
+    for name, kind, vals, arg in columns:
+        base = f"{prefix}.{name}"
+        largest = 0
+        if kind == "char":
+            width = arg
+            code_of, dict_bytes, codes = {}, [], []
+            for v in vals:
+                b = v.encode()[:width].ljust(width, b"\0")
+                if b not in code_of:
+                    code_of[b] = len(code_of)
+                    dict_bytes.append(b)
+                codes.append(code_of[b])
+            open(base, "wb").write(b"".join(dict_bytes))
+            codes = np.array(codes, np.int64)
+            for g in range(len(seg_sizes)):
+                seg = codes[bounds[g]:bounds[g + 1]]
+                f = pfor_encode(seg)
+                largest = max(largest, struct.unpack_from("<I", f)[0])
+                open(f"{base}.{g}.idx", "wb").write(f)
+                hs = [murmur64a(dict_bytes[c], hash_seed) // 2 for c in seg]
+                open(f"{base}.{g}.hash", "wb").write(struct.pack("<I", len(seg)) + struct.pack(f"<{len(hs)}Q", *hs))
+            entry[name] = (2, width)
+        else:
+            v = np.ascontiguousarray(vals, np.int64)
+            for g in range(len(seg_sizes)):
+                f = pfor_encode(v[bounds[g]:bounds[g + 1]], delta=(kind == "int" and bool(arg)))
+                largest = max(largest, struct.unpack_from("<I", f)[0])
+                open(f"{base}.{g}", "wb").write(f)
+            entry[name] = (0, arg if kind == "decimal" else 0)
+        open(f"{base}.header", "wb").write(header_bytes(int(bounds[-1]), len(seg_sizes), int(max(seg_sizes)), largest))
+    return entry
+
+
 def filter_mask(ops, cols, col_dec):
     cols = [np.ascontiguousarray(c, np.int64) for c in cols]
     n = len(cols[0]) if cols else 0

@@ -190,6 +190,18 @@ def test_pfor_file_layout():
     assert struct.unpack_from("<I", b2, 24 + 8 + 8)[0] == 32
 
 
+def test_binary_table_layouts():
+    """data.dictionary (save_col_data, cm.cu:4057-4080) and .header (writeHeader,
+    cm.cu:1251-1263), byte by byte."""
+    dd = O.data_dictionary_bytes({"t": {"b": (2, 10), "a": (0, 2)}})
+    assert dd == (struct.pack("<Q", 1) + struct.pack("<Q", 1) + b"t" + struct.pack("<Q", 2)
+                  + struct.pack("<Q", 1) + b"a" + struct.pack("<II", 0, 2)
+                  + struct.pack("<Q", 1) + b"b" + struct.pack("<II", 2, 10))
+    assert O.parse_data_dictionary(dd) == {"t": {"a": (0, 2), "b": (2, 10)}}
+    h = O.header_bytes(3000, 3, 1300, 2600)
+    assert len(h) == 20 and struct.unpack("<QIII", h) == (3000, 3, 1300, 2600)
+
+
 def test_pfor_delta_round_trip():
     r = np.random.default_rng(2)
     v = np.cumsum(r.integers(0, 50, 1000)).astype(np.int64) + 7
