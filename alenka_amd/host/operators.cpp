@@ -181,6 +181,13 @@ int make_plain(const DBufP &vals, uint64_t n, uint32_t seg_rows, int decimals, C
 int make_packed(const int64_t *d_vals, uint64_t n, uint32_t seg_rows, int decimals, Col *out) {
   uint32_t nsegs = (uint32_t)((n + seg_rows - 1) / seg_rows);
   out->hsegs.resize(nsegs);
+  if (nsegs == 0) {  // an empty column: no segments to encode
+    out->words = dalloc(8);
+    out->dsegs = dalloc(sizeof(aln_seg));
+    if (!out->words || !out->dsegs) return fail(ALN_E_NOMEM, "device allocation failed");
+    out->decimals = decimals;
+    return ALN_OK;
+  }
   uint64_t words = 0;
   ALN(aln_encode_plan(g_ctx, d_vals, n, seg_rows, 0, out->hsegs.data(), &words), "aln_encode_plan");
   out->words = dalloc(words * 4);

@@ -157,3 +157,18 @@ def test_binary_errors(ops, tmp_path):
     assert ops.lib.alenka_load_col_data(str(tmp_path / "missing").encode()) == L.ALN_E_ARG
     ops.emit_store_binary("X", str(tmp_path / "z"), 1)
     assert ops.lib.alenka_status() == L.ALN_E_UNSUPPORTED
+
+
+def test_empty_table_store_and_load(ops, tmp_path):
+    """An empty result stores as zero segments and loads back as an empty table."""
+    ops.table("E0", 4, 4, [("id", np.arange(4, dtype=np.int64), 0), ("name", ["a", "b", "c", "d"], 3)])
+    prefix = str(tmp_path / "empty")
+    assert ops.alenkaExecute(f"E1 := FILTER E0 BY id > 10;\nSTORE E1 INTO '{prefix}' BINARY;") == 0, \
+        ops.lib.alenka_last_error()
+    assert struct.unpack("<QIII", open(prefix + ".id.header", "rb").read()) == (0, 0, 0, 0)
+    ops.emit_load_binary("E2", prefix, 0)
+    ops.check(None, "emit_load_binary")
+    assert ops.rows("E2") == 0
+    assert ops.alenkaExecute("E3 := FILTER E2 BY name LIKE \"%\";\nE4 := ORDER E2 BY name ASC;") == 0, \
+        ops.lib.alenka_last_error()
+    assert ops.rows("E3") == 0 and ops.rows("E4") == 0

@@ -336,6 +336,19 @@ def test_anti_join_in_a_chain(ops, script):
     assert np.array_equal(ops.fetch("S", "rg"), (nat % 5)[ln])
 
 
+def test_semi_anti_with_an_empty_side(ops):
+    """No pairs: semi joins are empty, anti joins keep every row of their side."""
+    t = semi_anti_tables(ops, seed=3)
+    script = ("E := FILTER customer BY c_nationkey > 100;\n"
+              "S1 := SELECT o_orderkey AS k FROM orders LEFT SEMI JOIN E ON o_custkey = c_custkey;\n"
+              "S3 := SELECT o_orderkey AS k FROM orders LEFT ANTI JOIN E ON o_custkey = c_custkey;\n"
+              "S4 := SELECT c_custkey AS k FROM customer RIGHT ANTI JOIN E ON c_custkey = c_custkey;")
+    assert ops.alenkaExecute(script) == 0, ops.lib.alenka_last_error()
+    assert ops.rows("S1") == 0
+    assert np.array_equal(ops.fetch("S3", "k"), t["o_key"])
+    assert ops.rows("S4") == 0
+
+
 def test_unsupported_join_kinds(ops):
     semi_anti_tables(ops)
     for words in ("LEFT", "RIGHT", "OUTER"):
