@@ -1049,52 +1049,78 @@ __device__ __forceinline__ void fact_probe_hits(const SegPlan &P, const void *an
   }
 }
 
-// The deferred joined rows: one thread per slot of every fact wave (slot s of wave w holds
-// a row when s < min(matched[w], FACT_HIT_CAP)): group index = rank of the key, the payload
-// columns of the row, then SUM and COUNT.  All slots' reads are independent, so the payload's
-// HBM latency overlaps across the whole grid instead of stalling the scan.  A fact wave's
-// slots hold its joined rows in row order, and a group's rows are adjacent (the fact table
-// is clustered on the key), so runs of equal group index in adjacent lanes are summed with
-// a segmented wave scan first: one atomic pair per run.
-__global__ void __launch_bounds__(256) k_fact_resolve(const uint2 *__restrict__ hitbuf, uint64_t nwaves,
-                                                      uint32_t tps, const SegPlan *__restrict__ plan, FactArgs A) {
-  const uint64_t t = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-  const uint64_t w = t / FACT_HIT_CAP;
-  if (w >= nwaves) return;  // whole waves (nwaves * FACT_HIT_CAP is a multiple of 64)
-  // no early exit: the slot's count, entry and segment plan are read together, then the
-  // rank and payload words together (an unused slot reads key code 0 and row 0 instead of
-  // its stale entry, so every read stays in bounds), and only used slots add
-  const uint32_t s = (uint32_t)(t % FACT_HIT_CAP), n = A.matched[w];
-  const uint2 e = hitbuf[t];
-  // a wave's 64 slots belong to fact waves of one tile: its segment (and plan) is uniform
-  static_assert((FACT_HIT_CAP * (ST_BLOCK / 64)) % 64 == 0, "a resolve wave stays inside one fact tile");
-  const uint32_t tile = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(t / (FACT_HIT_CAP * (ST_BLOCK / 64))));
-  const SegPlan &P = plan[tile / tps];
-  const ColReader rx = col_reader(P.col[2], plan), ry = col_reader(P.col[3], plan);
-  const bool used = s < n;
-  const uint32_t code = used ? e.x : 0u, row = used ? e.y : 0u;
-  const uint32_t g = used ? bm_rank9(A.bits, A.rank, A.rel, code) : 0xFFFFFFFFu;
-  const int64_t x = col_read(rx, row), y = col_read(ry, row);
-  int64_t v = wmul((int64_t)((uint64_t)A.a0 + (uint64_t)wmul(A.a1, x)), (int64_t)((uint64_t)A.b0 + (uint64_t)wmul(A.b1, y)));
-  // segmented inclusive scan over runs of equal g (unused slots, g = ~0, form their own runs)
-  const unsigned lane = __lane_id();
-  uint32_t c = used;
-  const uint32_t prev = __shfl_up(g, 1, 64);  // every lane takes part (not inside a short circuit)
-  bool f = lane == 0 || prev != g;
-  for (int o = 1; o < 64; o <<= 1) {
-    const int64_t vu = __shfl_up(v, o, 64);
-    const uint32_t cu = __shfl_up(c, o, 64);
-    const bool fu = __shfl_up((int)f, o, 64) != 0;
-    if (lane >= (unsigned)o && !f) {
-      v = (int64_t)((uint64_t)v + (uint64_t)vu);
-      c += cu;
-    }
-    if (lane >= (unsigned)o) f = f || fu;
+// The deferred joined rows: one wave per RS_TILES adjacent fact tiles of a segment.  Their
+// deferred rows (the first min(matched[w], FACT_HIT_CAP) slots of each fact wave w) are
+// packed onto the lanes, 64 at a time (a group of tiles rarely has more): group index =
+// rank of the key, the payload columns of the row, then SUM and COUNT.  Every lane's reads
+// are independent, so the payload's HBM latency overlaps across the whole grid instead of
+// stalling the scan, and a lane idles only past the group's last row.  The rows of one fact
+// wave come in row order, and a group's rows are adjacent (the fact table is clustered on
+// the key), so runs of equal group index in adjacent lanes are summed with a segmented wave
+// scan first: one atomic pair per run.  (One tile per wave: with three, the kernel took
+// 130 us against 125 us on the 600M-row q3, its time being the rows' atomics and reads.)
+constexpr uint32_t RS_TILES = 1;
+__global__ void __launch_bounds__(64) k_fact_resolve(const uint2 *__restrict__ hitbuf, uint64_t nsegs,
+                                                     uint32_t tps, const SegPlan *__restrict__ plan, FactArgs A) {
+  constexpr int NW = ST_BLOCK / 64;  // fact waves per tile
+  constexpr int NK = NW * RS_TILES;  // fact waves per resolve wave
+  const uint32_t gps = (tps + RS_TILES - 1) / RS_TILES;
+  const uint64_t seg = blockIdx.x / gps;
+  if (seg >= nsegs) return;
+  const uint32_t t0 = (blockIdx.x % gps) * RS_TILES;
+  const uint32_t nt = tps - t0 < RS_TILES ? tps - t0 : RS_TILES;  // tiles of this group
+  const uint64_t w0 = (seg * tps + t0) * NW;                      // its first fact wave
+  // deferred rows before each fact wave of the group (wave-uniform loads)
+  uint32_t pre[NK + 1];
+  pre[0] = 0;
+#pragma unroll
+  for (int k = 0; k < NK; ++k) {
+    const uint32_t m = (uint32_t)k < nt * NW ? A.matched[w0 + k] : 0u;
+    pre[k + 1] = pre[k] + (m < FACT_HIT_CAP ? m : FACT_HIT_CAP);
   }
-  const uint32_t next = __shfl_down(g, 1, 64);
-  if (used && (lane == 63 || next != g)) {
-    atomicAdd(&A.sum[g], (unsigned long long)v);
-    atomicAdd(&A.cnt[g], c);
+  const uint32_t total = pre[NK];
+  if (total == 0) return;
+  const SegPlan &P = plan[seg];
+  const ColReader rx = col_reader(P.col[2], plan), ry = col_reader(P.col[3], plan);
+  const unsigned lane = __lane_id();
+  for (uint32_t b0 = 0; b0 < total; b0 += 64) {
+    // lane q of the round takes deferred row q: slot q - pre[k] of the fact wave k with
+    // pre[k] <= q < pre[k + 1]; lanes past the last row read slot 0 of wave 0 (in bounds)
+    // and add nothing
+    const uint32_t q = b0 + lane;
+    const bool used = q < total;
+    uint32_t k = 0, first = 0;
+#pragma unroll
+    for (int kk = 1; kk < NK; ++kk)
+      if (q >= pre[kk]) {
+        k = (uint32_t)kk;
+        first = pre[kk];
+      }
+    const uint2 e = hitbuf[used ? (w0 + k) * FACT_HIT_CAP + (q - first) : w0 * FACT_HIT_CAP];
+    const uint32_t code = used ? e.x : 0u, row = used ? e.y : 0u;
+    const uint32_t g = used ? bm_rank9(A.bits, A.rank, A.rel, code) : 0xFFFFFFFFu;
+    const int64_t x = col_read(rx, row), y = col_read(ry, row);
+    int64_t v = wmul((int64_t)((uint64_t)A.a0 + (uint64_t)wmul(A.a1, x)),
+                     (int64_t)((uint64_t)A.b0 + (uint64_t)wmul(A.b1, y)));
+    // segmented inclusive scan over runs of equal g (unused lanes, g = ~0, form their own run)
+    uint32_t c = used;
+    const uint32_t prev = __shfl_up(g, 1, 64);  // every lane takes part (not inside a short circuit)
+    bool f = lane == 0 || prev != g;
+    for (int o = 1; o < 64; o <<= 1) {
+      const int64_t vu = __shfl_up(v, o, 64);
+      const uint32_t cu = __shfl_up(c, o, 64);
+      const bool fu = __shfl_up((int)f, o, 64) != 0;
+      if (lane >= (unsigned)o && !f) {
+        v = (int64_t)((uint64_t)v + (uint64_t)vu);
+        c += cu;
+      }
+      if (lane >= (unsigned)o) f = f || fu;
+    }
+    const uint32_t next = __shfl_down(g, 1, 64);
+    if (used && (lane == 63 || next != g)) {
+      atomicAdd(&A.sum[g], (unsigned long long)v);
+      atomicAdd(&A.cnt[g], c);
+    }
   }
 }
 
@@ -1958,9 +1984,9 @@ extern "C" int aln_star_agg(aln_ctx *c, const aln_star_args *a, aln_star_row *h_
                           fmatch, fprobe, hitbuf};
         hipLaunchKernelGGL(k_star_fact_bm, dim3((unsigned)fblocks), dim3(ST_BLOCK), 0, c->stream, fd, fplan, fa);
         ALN_TRY(check_launch("star_fact"));
-        const uint64_t nslot = fwaves * FACT_HIT_CAP;
-        hipLaunchKernelGGL(k_fact_resolve, dim3((unsigned)((nslot + 255) / 256)), dim3(256), 0, c->stream, hitbuf,
-                           fwaves, fd.tps, fplan, fa);
+        const uint64_t fsegs = fblocks / fd.tps, rwaves = fsegs * ((fd.tps + RS_TILES - 1) / RS_TILES);
+        hipLaunchKernelGGL(k_fact_resolve, dim3((unsigned)rwaves), dim3(64), 0, c->stream, hitbuf, fsegs, fd.tps,
+                           fplan, fa);
       } else {  // per-block counts
         hipLaunchKernelGGL(k_star_fact, dim3((unsigned)fblocks), dim3(ST_BLOCK), 0, c->stream, fd, a->f_fk,
                            a->f_x, a->f_y, a->a0, a->a1, a->b0, a->b1, table, tcap, mkmin, mrange, sum, cnt,
