@@ -579,28 +579,34 @@ __global__ void __launch_bounds__(ST_BLOCK, 8) k_star_dim(ScanDesc d, const SegP
     __syncthreads();
     const FastTile ft = plan_fast(lds, P, r0);
     const uint32_t off = (uint32_t)(int32_t)((uint64_t)P.col[1].base - (uint64_t)kmin), lim = (uint32_t)range;
-    uint32_t code[ST_ITEMS];
-    (void)(ft.range ? fast_probe<true>(ft, nt, off, lim, code) : fast_probe<false>(ft, nt, off, lim, code));
-    // keys of adjacent rows are often adjacent: lanes setting bits of the same word OR them
-    // together first (segmented scan over runs of equal word index), one atomic per run
-    const unsigned lane = __lane_id();
+    // each thread takes ST_ITEMS consecutive rows: keys of adjacent rows are often adjacent
+    // (a dimension table is usually stored in key order), so a thread ORs the bits of its
+    // rows that fall in one bitmap word itself and sets them with one atomic per word
+    uint32_t cw = 0xFFFFFFFFu, cb = 0;
 #pragma unroll
-    for (int j = 0; j < ST_ITEMS; ++j) {
-      const bool v = code[j] != 0xFFFFFFFFu;
-      n += (uint32_t)__popcll(__builtin_amdgcn_ballot_w64(v));
-      const uint32_t wi = v ? code[j] >> 5 : 0xFFFFFFFFu;
-      uint32_t bitsv = v ? 1u << (code[j] & 31) : 0u;
-      const uint32_t prev = __shfl_up(wi, 1, 64);
-      bool f = lane == 0 || prev != wi;
-      for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t bu = __shfl_up(bitsv, o, 64);
-        const bool fu = __shfl_up((int)f, o, 64) != 0;
-        if (lane >= (unsigned)o && !f) bitsv |= bu;
-        if (lane >= (unsigned)o) f = f || fu;
+    for (int u = 0; u < ST_ITEMS; ++u) {
+      const uint32_t i = threadIdx.x * ST_ITEMS + u;
+      bool p = i < nt;
+      if (ft.range) {
+        const uint32_t rp = ft.rbit0 + i * ft.rbits;
+        const uint32_t *q = ft.rw + (rp >> 5);
+        p = p & (((__builtin_amdgcn_alignbit(q[1], q[0], rp & 31) & ft.rmask) - ft.clo) <= ft.cspan);
       }
-      const uint32_t next = __shfl_down(wi, 1, 64);
-      if (v && (lane == 63 || next != wi)) atomicOr(&bitmap[wi], bitsv);
+      const uint32_t kp = ft.kbit0 + i * ft.kbits;
+      const uint32_t *q = ft.kw + (kp >> 5);
+      const uint32_t c = (__builtin_amdgcn_alignbit(q[1], q[0], kp & 31) & ft.kmask) + off;
+      const bool v = p & (c < lim);
+      n += (uint32_t)__popcll(__builtin_amdgcn_ballot_w64(v));
+      if (v) {
+        if ((c >> 5) != cw) {
+          if (cb) atomicOr(&bitmap[cw], cb);
+          cw = c >> 5;
+          cb = 0;
+        }
+        cb |= 1u << (c & 31);
+      }
     }
+    if (cb) atomicOr(&bitmap[cw], cb);
   } else {
     TileStage &ts = *reinterpret_cast<TileStage *>(lds);
     const char zm = (char)P.zmode;
