@@ -690,7 +690,10 @@ __global__ void __launch_bounds__(ST_BLOCK, 8) k_star_mid(ScanDesc d, const SegP
   }
   uint32_t dw[ST_ITEMS];
 #pragma unroll
-  for (int j = 0; j < ST_ITEMS; ++j) dw[j] = bitmap[dc[j] != ~0ull ? dc[j] >> 5 : 0u];
+  for (int j = 0; j < ST_ITEMS; ++j) {  // only probing lanes load (fewer addresses per load)
+    dw[j] = 0;
+    if (dc[j] != ~0ull) dw[j] = gload(bitmap + (dc[j] >> 5));
+  }
   uint32_t passmask = 0;
 #pragma unroll
   for (int j = 0; j < ST_ITEMS; ++j)
@@ -1013,10 +1016,13 @@ __device__ __forceinline__ uint32_t fact_generic_codes(uint32_t *lds, int *want,
 // atomics, the loads of a round issued together; lane 0 writes the wave's counts.
 __device__ __forceinline__ void fact_probe_hits(const SegPlan &P, const void *any, const FactArgs &A, uint32_t r0,
                                                 const uint32_t (&code)[ST_ITEMS], uint32_t nprobe, uint32_t cidx) {
-  // the bitmap words, loaded without branches (rows that do not probe read word 0)
+  // the bitmap words, loaded by the probing lanes only (fewer addresses per load)
   uint32_t word[ST_ITEMS];
 #pragma unroll
-  for (int j = 0; j < ST_ITEMS; ++j) word[j] = A.bits[code[j] != 0xFFFFFFFFu ? code[j] >> 5 : 0u];
+  for (int j = 0; j < ST_ITEMS; ++j) {
+    word[j] = 0;
+    if (code[j] != 0xFFFFFFFFu) word[j] = gload(A.bits + (code[j] >> 5));
+  }
   uint32_t hits = 0, nmatch = 0;
 #pragma unroll
   for (int j = 0; j < ST_ITEMS; ++j) {
@@ -1091,8 +1097,7 @@ __global__ void __launch_bounds__(64) k_fact_resolve(const uint2 *__restrict__ h
   const unsigned lane = __lane_id();
   for (uint32_t b0 = 0; b0 < total; b0 += 64) {
     // lane q of the round takes deferred row q: slot q - pre[k] of the fact wave k with
-    // pre[k] <= q < pre[k + 1]; lanes past the last row read slot 0 of wave 0 (in bounds)
-    // and add nothing
+    // pre[k] <= q < pre[k + 1]; lanes past the last row load nothing and add nothing
     const uint32_t q = b0 + lane;
     const bool used = q < total;
     uint32_t k = 0, first = 0;
@@ -1102,10 +1107,17 @@ __global__ void __launch_bounds__(64) k_fact_resolve(const uint2 *__restrict__ h
         k = (uint32_t)kk;
         first = pre[kk];
       }
-    const uint2 e = hitbuf[used ? (w0 + k) * FACT_HIT_CAP + (q - first) : w0 * FACT_HIT_CAP];
-    const uint32_t code = used ? e.x : 0u, row = used ? e.y : 0u;
-    const uint32_t g = used ? bm_rank9(A.bits, A.rank, A.rel, code) : 0xFFFFFFFFu;
-    const int64_t x = col_read(rx, row), y = col_read(ry, row);
+    uint2 e = make_uint2(0u, 0u);
+    if (used) e = hitbuf[(w0 + k) * FACT_HIT_CAP + (q - first)];
+    const uint32_t code = e.x, row = e.y;
+    // only the lanes holding a row load (their rank and payload reads are issued together)
+    uint32_t g = 0xFFFFFFFFu;
+    int64_t x = 0, y = 0;
+    if (used) {
+      g = bm_rank9(A.bits, A.rank, A.rel, code);
+      x = col_read(rx, row);
+      y = col_read(ry, row);
+    }
     int64_t v = wmul((int64_t)((uint64_t)A.a0 + (uint64_t)wmul(A.a1, x)),
                      (int64_t)((uint64_t)A.b0 + (uint64_t)wmul(A.b1, y)));
     // segmented inclusive scan over runs of equal g (unused lanes, g = ~0, form their own run)
