@@ -1554,25 +1554,38 @@ __global__ void __launch_bounds__(TB_BLOCK) k_topk_bmax(uint64_t n, const unsign
 // maxima the result is 0 and every group passes.
 __global__ void __launch_bounds__(TB_BLOCK) k_topk_kth(const uint64_t *__restrict__ bmax, uint32_t nb, uint32_t k,
                                                        uint64_t *__restrict__ thr) {
-  __shared__ uint64_t wm[TB_BLOCK / 64];
-  __shared__ uint32_t wi[TB_BLOCK / 64];
-  __shared__ uint32_t taken;
+  __shared__ uint64_t wtop[TB_BLOCK / 64][TS_K + 1];  // + 1: a merged wave's list ends in 0
   uint64_t l[TS_K];
 #pragma unroll
   for (int q = 0; q < TS_K; ++q) l[q] = 0;
-  for (uint32_t t = threadIdx.x; t < nb; t += TB_BLOCK) {
-    uint64_t v = bmax[t];
+  // eight maxima's loads in flight before the first insertion (one round trip per eight
+  // instead of one per value)
+  constexpr int U = 8;
+  for (uint32_t t0 = threadIdx.x; t0 < nb; t0 += U * TB_BLOCK) {
+    uint64_t vs[U];
 #pragma unroll
-    for (int q = 0; q < TS_K; ++q) {  // insertion into the descending list
-      const uint64_t hi = v > l[q] ? v : l[q], lo = v > l[q] ? l[q] : v;
-      l[q] = hi;
-      v = lo;
+    for (int u = 0; u < U; ++u) {
+      const uint32_t t = t0 + u * TB_BLOCK;
+      vs[u] = 0;
+      if (t < nb) vs[u] = bmax[t];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      uint64_t v = vs[u];
+#pragma unroll
+      for (int q = 0; q < TS_K; ++q) {  // insertion into the descending list
+        const uint64_t hi = v > l[q] ? v : l[q], lo = v > l[q] ? l[q] : v;
+        l[q] = hi;
+        v = lo;
+      }
     }
   }
-  uint64_t last = 0;
+  // each wave: the k largest of its lanes' lists, by k rounds of a wave maximum (shuffles
+  // only, no block barrier); then one thread merges the waves' descending lists
+  const unsigned lane = __lane_id(), wv = threadIdx.x / 64;
   for (uint32_t r = 0; r < k; ++r) {
     uint64_t m = l[0];
-    uint32_t who = threadIdx.x;
+    uint32_t who = lane;
     for (int o = 32; o > 0; o >>= 1) {
       const uint64_t m2 = __shfl_xor(m, o, 64);
       const uint32_t w2 = __shfl_xor(who, o, 64);
@@ -1581,32 +1594,37 @@ __global__ void __launch_bounds__(TB_BLOCK) k_topk_kth(const uint64_t *__restric
         who = w2;
       }
     }
-    if (__lane_id() == 0) {
-      wm[threadIdx.x / 64] = m;
-      wi[threadIdx.x / 64] = who;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      uint64_t bm = wm[0];
-      uint32_t bw = wi[0];
-      for (int w = 1; w < TB_BLOCK / 64; ++w)
-        if (wm[w] > bm || (wm[w] == bm && wi[w] < bw)) {
-          bm = wm[w];
-          bw = wi[w];
-        }
-      taken = bw;
-      wm[0] = bm;
-    }
-    __syncthreads();
-    last = wm[0];
-    if (taken == threadIdx.x) {
+    if (lane == 0) wtop[wv][r] = m;
+    if (lane == who) {
 #pragma unroll
       for (int q = 0; q + 1 < TS_K; ++q) l[q] = l[q + 1];
       l[TS_K - 1] = 0;
     }
-    __syncthreads();
   }
-  if (threadIdx.x == 0) *thr = last;
+  if (lane == 0) wtop[wv][k] = 0;  // end of the wave's list
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t pos[TB_BLOCK / 64];
+#pragma unroll
+    for (int w = 0; w < TB_BLOCK / 64; ++w) pos[w] = 0;
+    uint64_t last = 0;
+    for (uint32_t r = 0; r < k; ++r) {
+      uint64_t best = 0;
+      int bw = 0;
+#pragma unroll
+      for (int w = 0; w < TB_BLOCK / 64; ++w) {
+        const uint64_t v = wtop[w][pos[w]];
+        if (w == 0 || v > best) {
+          best = v;
+          bw = w;
+        }
+      }
+      last = best;
+#pragma unroll
+      for (int w = 0; w < TB_BLOCK / 64; ++w) pos[w] += (uint32_t)(w == bw);
+    }
+    *thr = last;
+  }
 }
 
 // The groups with sum >= threshold, as (hi, lo, j) candidates; *count may exceed cap.
