@@ -137,6 +137,7 @@ SIGNATURES = {
     "aln_hash_join": (C.c_int, [P, P, U64, P, U64, P, P, U64, PU64]),
     "aln_join_rows": (C.c_int, [P, P, U64, U64, C.c_int, P, U64, PU64]),
     "aln_dict_lookup": (C.c_int, [P, P, U64, P, U64, P]),
+    "aln_like_dict": (C.c_int, [P, P, U64, C.c_uint32, C.c_char_p, P]),
     "aln_groupby": (C.c_int, [P, C.POINTER(P), C.c_int, C.POINTER(P), PI32, C.c_int, U64,
                               C.c_int, U32, C.POINTER(P), C.POINTER(P), U64, PU64]),
     "aln_sort_keys": (C.c_int, [P, C.POINTER(P), PI32, C.c_int, U64, P]),

@@ -132,6 +132,61 @@ __global__ void __launch_bounds__(256) k_dict_lookup(const int64_t *__restrict__
   }
 }
 
+// LIKE on one dictionary entry (the reference's gpu_regex, filter.cu:67-129): '%' matches any
+// run, retrying from the character after the last attempt's start on a mismatch; '?' any
+// character but '.'; others themselves.  The entry is `len` bytes, NUL-padded, and the
+// position counter is shared by all attempts and bounded by len, as in the reference.
+__device__ bool like_entry(const char *value, uint32_t len, const char *pattern) {
+  bool after_pct = false;
+  uint32_t used = 0;
+  const char *from = value, *pat = pattern;
+  for (;;) {
+    const char *s = from, *p = pat;
+    int stop = 0;  // 0: the value or the budget ran out, 1: '%' consumed, 2: mismatch
+    for (; used < len && *s; ++s, ++p, ++used) {
+      if (*p == '%') {
+        after_pct = true;
+        from = s;
+        pat = p;
+        while (*++pat == '%') {
+        }
+        if (!*pat) return true;
+        stop = 1;
+        break;
+      }
+      if (*p == '?' ? *s == '.' : *s != *p) {
+        stop = 2;
+        break;
+      }
+    }
+    if (stop == 1) continue;
+    if (stop == 2) {
+      if (!after_pct) return false;
+      ++from;
+      ++used;
+      continue;
+    }
+    while (*p == '%') ++p;
+    return !*p;
+  }
+}
+
+// One thread per entry of a fixed-width dictionary, matched where it lies: the matcher reads
+// a value character only while its position counter is below the width, and the counter is
+// never behind the character's offset, so no read leaves the entry.  A wave covers 64
+// entries, so its two bit-set words are written whole from one ballot.
+constexpr uint32_t LIKE_MAX_WIDTH = 256;
+__global__ void __launch_bounds__(64) k_like_dict(const char *__restrict__ entries, uint64_t n, uint32_t width,
+                                                  const char *__restrict__ pattern, uint32_t *__restrict__ bits) {
+  const uint64_t i = (uint64_t)blockIdx.x * 64 + threadIdx.x;
+  bool m = false;
+  if (i < n) m = like_entry(entries + i * width, width, pattern);
+  const uint64_t b = __builtin_amdgcn_ballot_w64(m);
+  const uint64_t w = (uint64_t)blockIdx.x * 2;
+  if (threadIdx.x == 0) bits[w] = (uint32_t)b;
+  if (threadIdx.x == 32) bits[w + 1] = (uint32_t)(b >> 32);
+}
+
 // ---- fused scan: decode + predicate + compaction -------------------------------------------
 constexpr int SC_BLOCK = VM_BLOCK;  // 256
 constexpr int SC_ITEMS = 16;
@@ -468,6 +523,25 @@ int aln_apply_perm(aln_ctx *c, const int64_t *d_in, const uint32_t *d_perm, uint
   hipLaunchKernelGGL(k_apply_perm, dim3(grid_for(n, 256, 65536)), dim3(256), 0, c->stream, d_in,
                      d_perm, n, d_out);
   return check_launch("apply_perm");
+}
+
+int aln_like_dict(aln_ctx *c, const char *d_entries, uint64_t n, uint32_t width, const char *h_pattern,
+                  uint32_t *d_bits) {
+  ALN_REQUIRE(c && h_pattern && (n == 0 || (d_entries && d_bits)), "aln_like_dict: bad arguments");
+  ALN_REQUIRE(width >= 1 && width <= LIKE_MAX_WIDTH, "aln_like_dict: width %u outside 1..%u", width, LIKE_MAX_WIDTH);
+  if (!n) return ALN_OK;
+  ALN_REQUIRE(n < (1ull << 37), "aln_like_dict: %llu entries", (unsigned long long)n);
+  ws_reset(c);
+  const size_t plen = strlen(h_pattern) + 1;
+  char *dp;
+  ALN_TRY(ws(c, plen, &dp));
+  ALN_TRY(copy_to_device(c, dp, h_pattern, plen));
+  ProfScope p(c, "like_dict");
+  hipLaunchKernelGGL(k_like_dict, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, c->stream, d_entries, n, width,
+                     (const char *)dp, d_bits);
+  ALN_TRY(check_launch("like_dict"));
+  ALN_HIP(hipStreamSynchronize(c->stream));
+  return ALN_OK;
 }
 
 int aln_dict_lookup(aln_ctx *c, const int64_t *d_codes, uint64_t n, const int64_t *d_table, uint64_t ntable,

@@ -231,6 +231,19 @@ def hash_join(ctx, build_keys, probe_keys, cap=None):
         return pi[: cnt.value], bi[: cnt.value]
 
 
+def like_dict(ctx, entries, width, pattern):
+    """Bit set (numpy uint32 words) of the entries (str list, NUL-padded to width bytes)
+    that match the LIKE pattern."""
+    import numpy as np
+
+    buf = b"".join(e.encode()[:width].ljust(width, b"\0") for e in entries) or b"\0"
+    d = torch.frombuffer(bytearray(buf), dtype=torch.uint8).to(ctx.device)
+    n = len(entries)
+    words = ctx.empty((n + 63) // 64 * 2 + 1, torch.int32)
+    L.check(ctx.lib.aln_like_dict(ctx.h, _p(d), n, width, pattern.encode(), _p(words)), "aln_like_dict")
+    return words.cpu().numpy().view(np.uint32)
+
+
 def join_rows(ctx, idx, nrows, anti, cap=None):
     """Rows 0..nrows-1 that occur in idx (anti=False) or do not (anti=True), ascending
     (int32 tensor)."""

@@ -209,77 +209,41 @@ struct View {
   DBufP rowids;         // int64: source row id of each row (for join ordering)
 };
 
-// LIKE on one dictionary entry, as the reference's gpu_regex functor matches it
-// (filter.cu:67-129): '%' matches any run of characters, retrying from the character after
-// the start of the last attempt on a mismatch; '?' matches any character but '.'; every
-// other pattern character matches itself.  The entry is read from a NUL-padded buffer of
-// `len` bytes and the position counter is shared by all attempts, so an attempt that
-// restarts late can run out of the `len` budget before the entry ends - as in the reference.
-bool like_match(const char *value, uint32_t len, const char *pattern) {
-  bool after_pct = false;
-  uint32_t used = 0;
-  const char *from = value, *pat = pattern;
-  enum { END, PERCENT, MISMATCH };
-  for (;;) {
-    const char *s = from, *p = pat;
-    int stop = END;  // END: the value or the budget ran out
-    for (; used < len && *s; ++s, ++p, ++used) {
-      if (*p == '%') {
-        after_pct = true;
-        from = s;
-        pat = p;
-        while (*++pat == '%') {
-        }
-        if (!*pat) return true;
-        stop = PERCENT;  // restart here with the pattern after the '%' run
-        break;
-      }
-      if (*p == '?' ? *s == '.' : *s != *p) {
-        stop = MISMATCH;
-        break;
-      }
-    }
-    if (stop == PERCENT) continue;
-    if (stop == MISMATCH) {
-      if (!after_pct) return false;
-      ++from;
-      ++used;
-      continue;
-    }
-    while (*p == '%') ++p;
-    return !*p;
-  }
-}
-
 // Bit sets of the dictionary codes a LIKE pattern matches, cached per dictionary and
 // pattern (the reference keeps the matching codes per column in map_res, filter.cu:452-499).
+// The dictionary goes to the device once, as width-byte NUL-padded entries (the layout of
+// the reference's dictionary file), and aln_like_dict matches every entry there.
 struct LikeCache {
   std::weak_ptr<std::vector<std::string>> dict;
+  DBufP entries;
+  uint32_t width = 0;
   std::map<std::string, DBufP> sets;
 };
 std::map<const void *, LikeCache> g_like;
 
 int like_codes(const Col &c, const std::string &pattern, aln_rpn *r) {
   const auto &d = *c.dict;
+  if (d.size() > 0x7FFFFFFFu) return fail(ALN_E_UNSUPPORTED, "dictionary too large for LIKE");
+  if (c.width < 1 || c.width > 256) return fail(ALN_E_UNSUPPORTED, "LIKE on a char(%u) column", c.width);
   auto &e = g_like[(const void *)c.dict.get()];
-  if (e.dict.expired() || e.dict.lock() != c.dict) {
+  if (e.dict.expired() || e.dict.lock() != c.dict || e.width != c.width) {
     e.dict = c.dict;
+    e.width = c.width;
     e.sets.clear();
+    std::vector<char> h(d.size() * c.width + 1, 0);
+    for (size_t i = 0; i < d.size(); ++i)
+      memcpy(h.data() + i * c.width, d[i].data(), std::min<size_t>(d[i].size(), c.width));
+    e.entries = dalloc(h.size());
+    if (!e.entries) return fail(ALN_E_NOMEM, "device allocation failed");
+    if (hipMemcpy(e.entries->p, h.data(), h.size(), hipMemcpyHostToDevice) != hipSuccess)
+      return fail(ALN_E_HIP, "copy of a dictionary failed");
   }
   DBufP &set = e.sets[pattern];
-  if (d.size() > 0x7FFFFFFFu) return fail(ALN_E_UNSUPPORTED, "dictionary too large for LIKE");
   if (!set) {
-    std::vector<uint32_t> bits((d.size() + 31) / 32 + 1, 0);
-    std::vector<char> buf(c.width + 1);
-    for (size_t code = 0; code < d.size(); ++code) {
-      std::fill(buf.begin(), buf.end(), 0);
-      memcpy(buf.data(), d[code].data(), std::min<size_t>(d[code].size(), c.width));
-      if (like_match(buf.data(), c.width, pattern.c_str())) bits[code >> 5] |= 1u << (code & 31);
-    }
-    set = dalloc(4 * bits.size());
+    set = dalloc(4 * ((d.size() + 63) / 64 * 2 + 1));
     if (!set) return fail(ALN_E_NOMEM, "device allocation failed");
-    if (hipMemcpy(set->p, bits.data(), 4 * bits.size(), hipMemcpyHostToDevice) != hipSuccess)
-      return fail(ALN_E_HIP, "copy of a LIKE code set failed");
+    ALN(aln_like_dict(g_ctx, (const char *)e.entries->p, d.size(), c.width, pattern.c_str(), (uint32_t *)set->p),
+        "LIKE");
   }
   r->type = ALN_RPN_CODESET;
   r->arg = (int32_t)d.size();

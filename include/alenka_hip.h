@@ -186,6 +186,12 @@ int aln_eval(aln_ctx *ctx, const aln_col *cols, int ncols, uint32_t seg_rows,
 /* d_out[i] = d_in[d_perm[i]] (apply_permutation, sorts.cu:86-125). */
 int aln_apply_perm(aln_ctx *ctx, const int64_t *d_in, const uint32_t *d_perm, uint64_t n,
                    int64_t *d_out);
+/* LIKE over a dictionary (gpu_regex over the dictionary file, filter.cu:67-129, 452-485):
+ * d_entries holds n NUL-padded entries of width bytes (1..256); bit i of the uint32 bit set
+ * d_bits is set when entry i matches h_pattern ('%' any run, '?' one character but '.').
+ * d_bits must hold ceil(n / 64) * 2 words. */
+int aln_like_dict(aln_ctx *ctx, const char *d_entries, uint64_t n, uint32_t width, const char *h_pattern,
+                  uint32_t *d_bits);
 /* d_out[i] = d_table[d_codes[i]] (-1 for a code outside 0..ntable-1).  ORDER BY on a char
  * column maps each row's dictionary code to the rank of its string in byte order and sorts
  * the ranks; it replaces the per-row string fetch and string sort of

@@ -234,6 +234,24 @@ def test_hash_join(ctx, dups):
     assert sorted(zip(ol.tolist(), orr.tolist())) == sorted(zip(pi.tolist(), bi.tolist()))
 
 
+@pytest.mark.parametrize("width", [3, 8, 25])
+def test_like_dict_matches_oracle(ctx, width):
+    """LIKE over a dictionary on the device against the oracle's restatement of gpu_regex
+    (filter.cu:67-129), with entries that fill the width (where the position budget binds)."""
+    from alenka_amd import device as D
+
+    r = np.random.default_rng(width)
+    alpha = list("ab.")
+    entries = ["".join(r.choice(alpha, size=int(r.integers(0, width + 1)))) for _ in range(777)]
+    entries += ["BUILDING"[:width], "aaab"[:width], ""]
+    pats = ["%", "", "a%", "%b", "%a%b%", "a?b", "?%", "%.%", "%ab", "a.b%", "%%a", "b?%?a"]
+    for pat in pats:
+        bits = D.like_dict(ctx, entries, width, pat)
+        got = [(int(bits[i >> 5]) >> (i & 31)) & 1 == 1 for i in range(len(entries))]
+        want = [O.like(e, width, pat) for e in entries]
+        assert got == want, pat
+
+
 @pytest.mark.parametrize("nrows,nidx", [(1, 0), (31, 5), (32, 64), (100003, 250000), (5000, 0)])
 def test_join_rows(ctx, nrows, nidx):
     """Semi/anti reduction: rows that occur among a join side's pair indices (with
