@@ -192,9 +192,11 @@ constexpr int SC_BLOCK = VM_BLOCK;  // 256
 constexpr int SC_ITEMS = 16;
 constexpr int SC_TILE = SC_BLOCK * SC_ITEMS;
 
+// One term of a conjunction the scan's fast path evaluates on column col: lo <= v <= hi, or
+// (set != 0) membership of v in the bit set at device address lo over codes 0..hi-1.
 struct RangeC {
   int32_t col;
-  int32_t pad;
+  int32_t set;
   int64_t lo, hi;
 };
 
@@ -233,7 +235,12 @@ __global__ void __launch_bounds__(SC_BLOCK) k_scan(const ColTable *__restrict__ 
         for (int k = 0; k < nrg; ++k) {
           const int c = rg[k].col;
           int64_t v = decode_row(ct->words[c], ct->segs[c][seg], local);
-          pass = pass && v >= rg[k].lo && v <= rg[k].hi;
+          if (rg[k].set) {
+            const uint32_t *bs = reinterpret_cast<const uint32_t *>(rg[k].lo);
+            pass = pass && v >= 0 && v < rg[k].hi && ((bs[v >> 5] >> (v & 31)) & 1u);
+          } else {
+            pass = pass && v >= rg[k].lo && v <= rg[k].hi;
+          }
         }
       } else {
         auto load = [&](int c) -> int64_t { return decode_row(ct->words[c], ct->segs[c][seg], local); };
@@ -268,12 +275,14 @@ __global__ void __launch_bounds__(SC_BLOCK) k_scan(const ColTable *__restrict__ 
   }
 }
 
-// Recognise a conjunction of "column OP constant" terms (multiplier 1) and turn it into
-// one closed interval per column.  Returns false when the program has another shape.
+// Recognise a conjunction of "column OP constant" terms (multiplier 1) and code-set tests
+// (LIKE) and turn it into one closed interval per column plus the set tests.  Returns false
+// when the program has another shape.
 static bool as_intervals(const aln_insn *p, int n, std::vector<RangeC> &out, bool *empty) {
   *empty = false;
   int64_t lo[ALN_MAX_COLS], hi[ALN_MAX_COLS];
   bool used[ALN_MAX_COLS] = {false};
+  std::vector<RangeC> sets;
   int i = 0;
   int pending = 0;  // predicates on the evaluation stack
   while (i < n) {
@@ -281,6 +290,12 @@ static bool as_intervals(const aln_insn *p, int n, std::vector<RangeC> &out, boo
       if (pending < 2) return false;
       --pending;
       ++i;
+      continue;
+    }
+    if (i + 1 < n && p[i].op == ALN_I_COL && p[i].k == 1 && p[i + 1].op == ALN_I_INSET) {
+      sets.push_back(RangeC{p[i].a, 1, p[i + 1].k, (int64_t)p[i + 1].a});
+      ++pending;
+      i += 2;
       continue;
     }
     if (i + 2 >= n) return false;
@@ -319,6 +334,7 @@ static bool as_intervals(const aln_insn *p, int n, std::vector<RangeC> &out, boo
       if (lo[c] > hi[c]) *empty = true;
       out.push_back(RangeC{c, 0, lo[c], hi[c]});
     }
+  out.insert(out.end(), sets.begin(), sets.end());
   return true;
 }
 

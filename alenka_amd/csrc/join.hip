@@ -269,11 +269,31 @@ namespace aln {
 // thread per 32-row word, 8192 rows per block.
 constexpr int JR_BLOCK = 256;
 
+// Each thread takes 8 consecutive indices: a join's pairs come in probe-row order, so they
+// are mostly rows of one bitmap word, ORed locally and set with one atomic per word.
 __global__ void __launch_bounds__(256) k_rows_mark(const uint32_t *__restrict__ idx, uint64_t n,
                                                    uint64_t nrows, uint32_t *bm) {
-  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
-    const uint32_t r = idx[i];
-    if (r < nrows) atomicOr(&bm[r >> 5], 1u << (r & 31));
+  const uint64_t nchunks = (n + 7) / 8;
+  for (uint64_t t = (uint64_t)blockIdx.x * 256 + threadIdx.x; t < nchunks; t += (uint64_t)gridDim.x * 256) {
+    uint32_t r[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const uint64_t i = t * 8 + u;
+      r[u] = 0xFFFFFFFFu;
+      if (i < n) r[u] = idx[i];
+    }
+    uint32_t cw = 0xFFFFFFFFu, cb = 0;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      if (r[u] >= nrows) continue;  // past the end, or not a row
+      if ((r[u] >> 5) != cw) {
+        if (cb) atomicOr(&bm[cw], cb);
+        cw = r[u] >> 5;
+        cb = 0;
+      }
+      cb |= 1u << (r[u] & 31);
+    }
+    if (cb) atomicOr(&bm[cw], cb);
   }
 }
 
@@ -347,8 +367,8 @@ extern "C" int aln_join_rows(aln_ctx *c, const uint32_t *d_idx, uint64_t nidx, u
   ALN_TRY(ws(c, nblocks, &off));
   ALN_TRY(fill_u32(c, bm, nw, 0));
   if (nidx) {
-    hipLaunchKernelGGL(k_rows_mark, dim3(grid_for(nidx, 256, 65536)), dim3(256), 0, c->stream, d_idx, nidx,
-                       nrows, bm);
+    hipLaunchKernelGGL(k_rows_mark, dim3(grid_for((nidx + 7) / 8, 256, 65536)), dim3(256), 0, c->stream, d_idx,
+                       nidx, nrows, bm);
     ALN_TRY(check_launch("join_rows_mark"));
   }
   hipLaunchKernelGGL(k_rows_count, dim3((unsigned)nblocks), dim3(JR_BLOCK), 0, c->stream, (const uint32_t *)bm,

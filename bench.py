@@ -410,6 +410,89 @@ def serve_leg(ctx, D, dim, mid, fact, zc, code_of, seed, rate, duration, budget_
     return out
 
 
+P_TYPES = [f"{a} {b} {c}" for a in ("STANDARD", "SMALL", "MEDIUM", "LARGE", "ECONOMY", "PROMO")
+           for b in ("ANODIZED", "BURNISHED", "PLATED", "POLISHED", "BRUSHED")
+           for c in ("TIN", "NICKEL", "BRASS", "STEEL", "COPPER")]
+
+
+def timed(fn, reps):
+    """Median wall time (s) of fn() over reps runs, synchronised on both sides."""
+    import torch
+
+    out, ts = None, []
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = fn()
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t0)
+    ts.sort()
+    return ts[len(ts) // 2], out
+
+
+def widened_rows(ctx, D, seed, n=100_000_000, reps=5):
+    """The operators around the hot path on 100M-row synthetic inputs, each checked and timed
+    (inputs resident in HBM; medians of `reps` runs, one synchronising C call each):
+    LIKE over a char(25) column (dictionary match + set-membership scan), a semi join's row
+    reduction, the char ORDER BY key map, and a segment file read into HBM."""
+    import numpy as np
+    import torch
+
+    from alenka_amd import _lib as L
+
+    out = {}
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    # LIKE '%BRASS' over dictionary codes of 150 part types (the shape of q2's filter)
+    codes = torch.randint(0, len(P_TYPES), (n,), generator=g, device="cuda", dtype=torch.int64)
+    col = D.Column.encode(ctx, codes, 1_000_000)
+    words = D.like_dict(ctx, P_TYPES, 25, "%BRASS")
+    dset = torch.as_tensor(words.view(np.int32)).to("cuda")
+    prog, _ = L.compile_rpn([(L.RPN_NAME, 0, 0), (L.RPN_CODESET, len(P_TYPES), dset.data_ptr()), (L.RPN_CMP, 7, 0)], [0])
+    t, rows = timed(lambda: D.scan_filter(ctx, [col], prog), reps)
+    want = int(sum(torch.bincount(codes, minlength=len(P_TYPES))[i].item()
+                   for i, p in enumerate(P_TYPES) if p.endswith("BRASS")))
+    assert rows.numel() == want, "LIKE scan disagrees with the expected count"
+    packed = int(col.words.numel()) * 4
+    out["like_filter"] = {"rows": n, "passing": want, "ms": t * 1e3, "rows_per_s": n / t,
+                          "algorithmic_gbs": (packed + 4 * want) / t / 1e9,
+                          "bytes": "packed codes + 4 B per passing row id"}
+    del col, rows
+    # semi join reduction: 100M probe rows, 10M unique build keys, pairs -> distinct probe rows
+    probe = torch.randint(0, 20_000_000, (n,), generator=g, device="cuda", dtype=torch.int64)
+    build = torch.randperm(20_000_000, generator=g, device="cuda")[:10_000_000].to(torch.int64)
+    tj, (pi, bi) = timed(lambda: D.hash_join(ctx, build, probe), 3)
+    tr, semi = timed(lambda: D.join_rows(ctx, pi, n, False), reps)
+    assert semi.numel() == pi.numel(), "unique build keys: every matched probe row once"
+    out["semi_join"] = {"probe_rows": n, "build_rows": 10_000_000, "pairs": int(pi.numel()),
+                        "hash_join_ms": tj * 1e3, "row_reduction_ms": tr * 1e3,
+                        "rows_per_s": n / (tj + tr),
+                        "reduction_gbs": (4 * pi.numel() + n / 8 + 4 * semi.numel()) / tr / 1e9}
+    del probe, build, pi, bi, semi
+    # char ORDER BY key map: dictionary rank per row
+    ranks = torch.as_tensor(np.argsort(np.argsort(np.array(P_TYPES, dtype=object), kind="stable"),
+                                       kind="stable").astype(np.int64)).to("cuda")
+    rk = ctx.empty(n)
+
+    def lookup():
+        L.check(ctx.lib.aln_dict_lookup(ctx.h, codes.data_ptr(), n, ranks.data_ptr(), ranks.numel(), rk.data_ptr()),
+                "aln_dict_lookup")
+        return rk
+
+    tl, r = timed(lookup, reps)
+    assert torch.equal(r[:n], ranks[codes]), "rank map disagrees"
+    out["char_order_key"] = {"rows": n, "ms": tl * 1e3, "rows_per_s": n / tl, "algorithmic_gbs": 16 * n / tl / 1e9}
+    del rk, r
+    # a segment file (the reference's FOR layout, 32-bit codes) read into HBM
+    vals = torch.randint(0, 1 << 30, (n // 4,), generator=g, device="cuda", dtype=torch.int64)
+    blob = D.segfile_encode(ctx, vals)
+    ts, (dec, lo, hi) = timed(lambda: D.segfile_decode(ctx, blob), 3)
+    assert torch.equal(dec, vals), "segment file round trip"
+    out["segment_file_read"] = {"rows": n // 4, "file_bytes": len(blob), "ms": ts * 1e3, "rows_per_s": (n // 4) / ts,
+                                "gbs_pcie_inclusive": len(blob) / ts / 1e9,
+                                "note": "host file bytes copied over PCIe, then decoded on the device"}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -425,6 +508,7 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--sort-rows", type=int, default=100_000_000)
     ap.add_argument("--no-serve", action="store_true", help="skip the served-mix leg")
+    ap.add_argument("--no-widened", action="store_true", help="skip the operators around the hot path")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run a plain device copy of 1 GiB (the known-size kernel that scales "
                          "FETCH_SIZE / WRITE_SIZE counters to bytes)")
@@ -504,6 +588,10 @@ def main():
             served = serve_leg(ctx, D, dim, mid, fact, zc, code_of, args.seed, args.serve_rate,
                                args.serve_seconds, args.serve_budget_gb)
         del zc
+    widened = None
+    if not args.no_widened:
+        widened = widened_rows(ctx, D, args.seed)
+        log("widened: " + ", ".join(f"{k} {v.get('ms', v.get('row_reduction_ms', 0)):.2f} ms" for k, v in widened.items()))
 
     # HBM bytes per fact phase, from the two counter passes (FETCH_SIZE, WRITE_SIZE) of this
     # workload, scaled by a plain copy's counts; the counters cannot be read from inside a run
@@ -560,6 +648,7 @@ def main():
         "cpu_baseline": cpu,
         "batched": batched,
         "served": served,
+        "widened": widened,
     }
     print(json.dumps(out), flush=True)
     ctx.close()
