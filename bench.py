@@ -590,8 +590,13 @@ def main():
         del zc
     widened = None
     if not args.no_widened:
-        widened = widened_rows(ctx, D, args.seed)
-        log("widened: " + ", ".join(f"{k} {v.get('ms', v.get('row_reduction_ms', 0)):.2f} ms" for k, v in widened.items()))
+        try:
+            widened = widened_rows(ctx, D, args.seed)
+            log("widened: " + ", ".join(f"{k} {v.get('ms', v.get('row_reduction_ms', 0)):.2f} ms"
+                                        for k, v in widened.items()))
+        except Exception as e:  # a failed check is reported in the line, not lost with it
+            widened = {"error": f"{type(e).__name__}: {e}"}
+            log(f"widened: {widened['error']}")
 
     # HBM bytes per fact phase, from the two counter passes (FETCH_SIZE, WRITE_SIZE) of this
     # workload, scaled by a plain copy's counts; the counters cannot be read from inside a run
