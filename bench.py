@@ -449,8 +449,8 @@ def widened_rows(ctx, D, seed, n=100_000_000, reps=5):
     dset = torch.as_tensor(words.view(np.int32)).to("cuda")
     prog, _ = L.compile_rpn([(L.RPN_NAME, 0, 0), (L.RPN_CODESET, len(P_TYPES), dset.data_ptr()), (L.RPN_CMP, 7, 0)], [0])
     t, rows = timed(lambda: D.scan_filter(ctx, [col], prog), reps)
-    want = int(sum(torch.bincount(codes, minlength=len(P_TYPES))[i].item()
-                   for i, p in enumerate(P_TYPES) if p.endswith("BRASS")))
+    per_code = torch.bincount(codes, minlength=len(P_TYPES)).cpu().numpy()
+    want = int(sum(int(per_code[i]) for i, p in enumerate(P_TYPES) if p.endswith("BRASS")))
     assert rows.numel() == want, "LIKE scan disagrees with the expected count"
     packed = int(col.words.numel()) * 4
     out["like_filter"] = {"rows": n, "passing": want, "ms": t * 1e3, "rows_per_s": n / t,
