@@ -970,6 +970,7 @@ struct FactArgs {
   unsigned int *cnt;
   uint32_t *matched, *probed;  // per wave: index tile * 8 + wave
   uint2 *hitbuf;               // per wave FACT_HIT_CAP slots {code, row in segment}, or null
+  uint32_t *held;              // bit g set once group g holds a row (or null)
 };
 // A wave's first FACT_HIT_CAP joined rows are left in its slots for k_fact_resolve (the
 // scan does not wait on their payload reads); the rest (rare) are resolved in place.
@@ -1054,6 +1055,7 @@ __device__ __forceinline__ void fact_probe_hits(const SegPlan &P, const void *an
                            (int64_t)((uint64_t)A.b0 + (uint64_t)wmul(A.b1, y)));
     atomicAdd(&A.sum[g], (unsigned long long)v);
     atomicAdd(&A.cnt[g], 1u);
+    if (A.held) atomicOr(&A.held[g >> 5], 1u << (g & 31));
   }
   if (__lane_id() == 0) {
     A.matched[cidx] = nmatch;
@@ -1138,6 +1140,7 @@ __global__ void __launch_bounds__(64) k_fact_resolve(const uint2 *__restrict__ h
     if (used && (lane == 63 || next != g)) {
       atomicAdd(&A.sum[g], (unsigned long long)v);
       atomicAdd(&A.cnt[g], c);
+      if (A.held) atomicOr(&A.held[g >> 5], 1u << (g & 31));
     }
   }
 }
@@ -1654,6 +1657,62 @@ __global__ void __launch_bounds__(TB_BLOCK) k_topk_collect(uint64_t n, const uns
   }
 }
 
+constexpr uint32_t TL_BLOCKS = 1024;
+// The same two passes over the bit set of the groups that hold rows (one bit per group,
+// ~5 % set): a thread per 32 groups reads one word and the sums of its set bits only.
+__global__ void __launch_bounds__(TB_BLOCK) k_topk_bmax_held(const uint32_t *__restrict__ held, uint64_t nw,
+                                                             const unsigned long long *__restrict__ sum,
+                                                             uint64_t *__restrict__ bmax) {
+  __shared__ uint64_t wm[TB_BLOCK / 64];
+  uint64_t m = 0;
+  for (uint64_t w = (uint64_t)blockIdx.x * TB_BLOCK + threadIdx.x; w < nw; w += (uint64_t)gridDim.x * TB_BLOCK) {
+    uint32_t b = held[w];
+    while (b) {
+      const uint64_t g = w * 32 + (uint64_t)__builtin_ctz(b);
+      b &= b - 1;
+      const uint64_t o = (uint64_t)sum[g] ^ 0x8000000000000000ull;
+      m = o > m ? o : m;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const uint64_t x = __shfl_xor(m, o, 64);
+    m = x > m ? x : m;
+  }
+  if (__lane_id() == 0) wm[threadIdx.x / 64] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < TB_BLOCK / 64; ++w) m = wm[w] > m ? wm[w] : m;
+    bmax[blockIdx.x] = m;
+  }
+}
+
+__global__ void __launch_bounds__(TB_BLOCK) k_topk_collect_held(const uint32_t *__restrict__ held, uint64_t nw,
+                                                                const unsigned long long *__restrict__ sum,
+                                                                GroupAttr ga, int64_t g1min, int64_t kmin,
+                                                                const uint64_t *__restrict__ thr, uint32_t cap,
+                                                                uint32_t *__restrict__ count,
+                                                                uint64_t *__restrict__ chi,
+                                                                uint64_t *__restrict__ clo,
+                                                                uint32_t *__restrict__ cj) {
+  const uint64_t t = *thr;
+  for (uint64_t w = (uint64_t)blockIdx.x * TB_BLOCK + threadIdx.x; w < nw; w += (uint64_t)gridDim.x * TB_BLOCK) {
+    uint32_t b = held[w];
+    while (b) {
+      const uint64_t g = w * 32 + (uint64_t)__builtin_ctz(b);
+      b &= b - 1;
+      const uint64_t o = (uint64_t)sum[g] ^ 0x8000000000000000ull;
+      if (o >= t) {
+        const uint32_t pos = atomicAdd(count, 1u);
+        if (pos < cap) {
+          chi[pos] = ~o;
+          clo[pos] = ((uint64_t)(ga_g1(ga, g) - g1min) << 32) | (uint32_t)((uint64_t)ga_key(ga, g) - (uint64_t)kmin);
+          cj[pos] = (uint32_t)g;
+        }
+      }
+    }
+  }
+}
+
 __global__ void k_topk_emit(const uint32_t *__restrict__ j, uint32_t k,
                             const unsigned long long *__restrict__ sum, const unsigned int *__restrict__ cnt,
                             GroupAttr ga, aln_star_row *__restrict__ out, uint32_t *__restrict__ nvalid) {
@@ -1681,12 +1740,15 @@ __global__ void k_topk_emit(const uint32_t *__restrict__ j, uint32_t k,
 static int star_topk(aln_ctx *c, uint64_t m, const unsigned long long *sum, const unsigned int *cnt,
                      const GroupAttr &ga, uint32_t k, int64_t g1min, int64_t g1max, int64_t kmin,
                      aln_star_row *h_top, uint32_t *h_ntop, uint64_t *ctr = nullptr, uint32_t nctr = 0,
-                     uint64_t *h_ctr = nullptr, bool *ctr_read = nullptr) {
+                     uint64_t *h_ctr = nullptr, bool *ctr_read = nullptr, const uint32_t *held = nullptr) {
   if (k <= (uint32_t)TS_K && g1max >= g1min && (uint64_t)g1max - (uint64_t)g1min < (1ull << 32) &&
       m < (1ull << 32)) {
-    // threshold path: block maxima -> k-th largest -> the groups at or above it
+    // threshold path: block maxima -> k-th largest -> the groups at or above it (over the
+    // bit set of groups that hold rows when there is one, else over every group slot)
     {
-      const uint64_t nb = (m + TB_BLOCK * TB_ITEMS - 1) / (TB_BLOCK * TB_ITEMS);
+      const uint64_t hw = (m + 31) / 32;
+      const uint64_t nb = held ? std::min<uint64_t>(TL_BLOCKS, (hw + TB_BLOCK - 1) / TB_BLOCK)
+                               : (m + TB_BLOCK * TB_ITEMS - 1) / (TB_BLOCK * TB_ITEMS);
       constexpr uint32_t CAP = 1u << 16;
       if (nb <= 64ull * TB_BLOCK) {
         // one result block: [u32 collected][u32 rows valid][aln_star_row rows[k]], read with
@@ -1710,10 +1772,18 @@ static int star_topk(aln_ctx *c, uint64_t m, const unsigned long long *sum, cons
         ALN_TRY(fill_u32(c, out, 2, 0));
         {
           ProfScope p(c, "star_topk");
-          hipLaunchKernelGGL(k_topk_bmax, dim3((unsigned)nb), dim3(TB_BLOCK), 0, c->stream, m, sum, cnt, bmax);
+          if (held)
+            hipLaunchKernelGGL(k_topk_bmax_held, dim3((unsigned)nb), dim3(TB_BLOCK), 0, c->stream, held, hw, sum,
+                               bmax);
+          else
+            hipLaunchKernelGGL(k_topk_bmax, dim3((unsigned)nb), dim3(TB_BLOCK), 0, c->stream, m, sum, cnt, bmax);
           hipLaunchKernelGGL(k_topk_kth, dim3(1), dim3(TB_BLOCK), 0, c->stream, bmax, (uint32_t)nb, k, thr);
-          hipLaunchKernelGGL(k_topk_collect, dim3((unsigned)nb), dim3(TB_BLOCK), 0, c->stream, m, sum, cnt, ga, g1min,
-                             kmin, thr, CAP, count, chi, clo, cj);
+          if (held)
+            hipLaunchKernelGGL(k_topk_collect_held, dim3((unsigned)nb), dim3(TB_BLOCK), 0, c->stream, held, hw, sum, ga,
+                               g1min, kmin, thr, CAP, count, chi, clo, cj);
+          else
+            hipLaunchKernelGGL(k_topk_collect, dim3((unsigned)nb), dim3(TB_BLOCK), 0, c->stream, m, sum, cnt, ga,
+                               g1min, kmin, thr, CAP, count, chi, clo, cj);
           hipLaunchKernelGGL(k_topk_sel, dim3(1), dim3(TS_BLOCK), 0, c->stream, (uint64_t)CAP, count, sum, cnt, ga,
                              g1min, kmin, chi, clo, cj, k, oh, ol, oj);
           hipLaunchKernelGGL(k_topk_emit, dim3(1), dim3(64), 0, c->stream, oj, k, sum, cnt, ga, rows, nv);
@@ -1903,6 +1973,7 @@ extern "C" int aln_star_agg(aln_ctx *c, const aln_star_args *a, aln_star_row *h_
   ALN_TRY(ws(c, mregions, &mprobe));
   ALN_TRY(ws(c, fwaves, &fmatch));
   ALN_TRY(ws(c, fwaves, &fprobe));
+  uint32_t *held = nullptr;  // bit set of the groups that hold rows (bitmap join)
   // one launch: per-segment plans of the three scans, and zeroed bitmaps and counters
   SegPlan *dplan, *mplan, *fplan = nullptr;
   {
@@ -2016,8 +2087,10 @@ extern "C" int aln_star_agg(aln_ctx *c, const aln_star_args *a, aln_star_row *h_
       if (use_bm) {
         uint2 *hitbuf;
         ALN_TRY(ws(c, fwaves * FACT_HIT_CAP, &hitbuf));
+        ALN_TRY(ws(c, (m + 31) / 32, &held));
+        ALN_TRY(fill_u32(c, held, (m + 31) / 32, 0));
         const FactArgs fa{a->f_fk, a->a0, a->a1, a->b0, a->b1, mbits, rank, rel, mkmin, mrange, sum, cnt,
-                          fmatch, fprobe, hitbuf};
+                          fmatch, fprobe, hitbuf, held};
         hipLaunchKernelGGL(k_star_fact_bm, dim3((unsigned)fblocks), dim3(ST_BLOCK), 0, c->stream, fd, fplan, fa);
         ALN_TRY(check_launch("star_fact"));
         const uint64_t fsegs = fblocks / fd.tps, rwaves = fsegs * ((fd.tps + RS_TILES - 1) / RS_TILES);
@@ -2041,7 +2114,8 @@ extern "C" int aln_star_agg(aln_ctx *c, const aln_star_args *a, aln_star_row *h_
   // top-k
   bool ctr_read = false;
   if (a->limit && m)
-    ALN_TRY(star_topk(c, m, sum, cnt, ga, a->limit, g1min, g1max, mkmin, h_top, h_ntop, ctr, 8, hc, &ctr_read));
+    ALN_TRY(star_topk(c, m, sum, cnt, ga, a->limit, g1min, g1max, mkmin, h_top, h_ntop, ctr, 8, hc, &ctr_read,
+                      held));
   tm.mark(4);
   if (!ctr_read) ALN_TRY(copy_to_host(c, hc, ctr, sizeof(hc)));
   if (hdup[1]) return set_error(ALN_E_UNSUPPORTED, "aln_star_agg: duplicate mid keys");
