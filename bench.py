@@ -12,8 +12,10 @@ min/max zone maps, resident in HBM.  One step is the TPC-H q3 pipeline:
   priority) SUM(price*(1-discount)), COUNT; ORDER BY revenue DESC, orderdate ASC LIMIT 10.
 
 Prints one JSON line: fact rows/s over the whole step (inputs already in HBM), the
-fraction of HBM peak the dominant kernel (the fact scan) reaches on algorithmic bytes,
-and a CPU baseline (the CPU oracle, one core, on a bounded sample).
+fraction of HBM peak the dominant kernel (the fact phase) reaches on the bytes it must
+move, the same q3 on the host cores over the same tables (rocThrust's OpenMP backend),
+whose answer checks the GPU's, and the 1M-row flat and 6M-row star configurations timed
+on both.
 """
 import argparse
 import json
@@ -35,26 +37,35 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def build_tables(ctx, D, seed, scale, seg_rows):
+def build_tables(ctx, D, seed, scale, seg_rows, host=None):
+    """The three q3 tables, encoded in HBM.  With a dict `host`, the raw int64 columns are
+    also copied to host memory under the oracle's names (the CPU check's input)."""
     import torch
 
     ndim, norders = int(150000 * scale), int(1500000 * scale)
     t0 = time.time()
+    keep = (lambda name, col: host.__setitem__(name, col.cpu().numpy())) if host is not None else (lambda *a: None)
     c_key, c_seg_idx = D.gen_dim(ctx, seed, ndim)
     code_of, c_seg = D.dict_first_seen(ctx, c_seg_idx, len(SEGMENTS))
     del c_seg_idx
+    keep("c_key", c_key)
+    keep("c_seg", c_seg)
     dim = [D.Column.encode(ctx, c_key, seg_rows), D.Column.encode(ctx, c_seg, seg_rows)]
     del c_key, c_seg
     mid = []
-    for i, col in enumerate(D.gen_orders(ctx, seed, norders, ndim)):
+    for name, col in zip(("o_key", "o_cust", "o_date", "o_prio"), D.gen_orders(ctx, seed, norders, ndim)):
+        keep(name, col)
         mid.append(D.Column.encode(ctx, col, seg_rows))
         del col
     fact = []
     okey, price, disc, ship = D.gen_fact(ctx, seed, norders)
     nfact = okey.numel()
-    for col, dec in ((okey, 0), (price, 2), (disc, 2), (ship, 0)):
+    for name, col, dec in (("l_okey", okey, 0), ("l_price", price, 2), ("l_disc", disc, 2), ("l_ship", ship, 0)):
+        keep(name, col)
         fact.append(D.Column.encode(ctx, col, seg_rows, decimals=dec))
     del okey, price, disc, ship, col
+    if host is not None:
+        host["code_of"] = list(code_of)
     torch.cuda.empty_cache()
     log(f"tables: dim {ndim}, orders {norders}, fact {nfact} rows; built in {time.time() - t0:.1f}s")
     return dim, mid, fact, code_of, nfact
@@ -89,35 +100,130 @@ def scan_bytes(t, used_cols):
     return sum(t["cols"][c].packed_bytes(segs) for c in used_cols), len(segs)
 
 
-def cpu_baseline(sample_scale, seed):
-    """The q3 script on the host cores (oracle/libq3cpu.so: the reference's filter /
-    sort-merge join / sorted group-by / stable ORDER BY algorithms with OpenMP, checked
-    against the single-threaded oracle) on a bounded sample of the same workload."""
+def star_sources_hash():
+    """sha256 over the sources the star kernels are compiled from."""
+    import hashlib
+
+    h = hashlib.sha256()
+    for f in ("star.hip", "stage.h", "aln_internal.h"):
+        with open(os.path.join(ROOT, "alenka_amd", "csrc", f), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()
+
+
+def cpu_info():
+    model = platform.processor() or platform.machine()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    return model, threads
+
+
+def median_time(fn, runs, budget_s=None):
+    """1 warm-up call (its result is returned), then `runs` timed calls (fewer if budget_s
+    seconds are used up); returns (median seconds, runs timed, warm-up result)."""
+    first = fn()
+    ts = []
+    t_end = None if budget_s is None else time.time() + budget_s
+    while len(ts) < runs and (t_end is None or time.time() < t_end or not ts):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    ts.sort()
+    return ts[len(ts) // 2], len(ts), first
+
+
+def cpu_q3(t, threads, runs, budget_s=None):
+    """q3 on the host cores through oracle/libq3cpu.so (the reference's Thrust algorithms on
+    rocThrust's OpenMP backend): (median s, runs, (top rows, pass counts))."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import q3_reference as Q3
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
-    t = Q3.tables(seed, int(150000 * sample_scale), int(1500000 * sample_scale))
-    nfact = len(t["l_okey"])
-    want, _, _ = Q3.run(t)
-    got, used = Q3.run_cpu_parallel(t, threads)
-    assert got == want, "CPU baseline disagrees with the oracle"
-    times = []
-    t_end = time.time() + 10.0
-    while time.time() < t_end or len(times) < 3:
-        t0 = time.perf_counter()
-        Q3.run_cpu_parallel(t, threads)
-        times.append(time.perf_counter() - t0)
-        if len(times) >= 25:
-            break
-    times.sort()
-    med = times[len(times) // 2]
-    return dict(value=nfact / med, unit="fact rows/s", cores=used, kind="port",
-                sample=(f"q3 pipeline on dim {int(150000 * sample_scale)} / orders "
-                        f"{int(1500000 * sample_scale)} / fact {nfact} rows (scale {sample_scale} of "
-                        f"the 6M-row base), median of {len(times)} runs, oracle/libq3cpu.so "
-                        f"(OpenMP, {used} threads; result checked against the single-threaded "
-                        f"oracle) on {platform.processor() or platform.machine()}"))
+    def go():
+        rows, _, st = Q3.run_cpu_parallel(t, threads, with_stats=True)
+        return rows, st
+
+    return median_time(go, runs, budget_s)
+
+
+def flat_gpu_query(ctx, D, L, cols, date_lt=DATE):
+    """The flat query on the GPU through the per-statement operators: scan + compaction of
+    date < date_lt, key and SUM(price*(1-discount)) of the passing rows, GROUP BY key in key
+    order (one segment)."""
+    prog = [(L.I_COL, 1, 1), (L.I_CONST, 0, date_lt), (L.I_CMP, 1, 0)]
+    rows = D.scan_filter(ctx, cols, prog)
+    key = D.eval_rows(ctx, cols, [(L.I_COL, 0, 1)], rows)
+    rev = D.eval_rows(ctx, cols, [(L.I_COL, 2, 1), (L.I_CONST, 0, 100), (L.I_COL, 3, 1), (L.I_SUB, 0, 0),
+                                  (L.I_MUL, 0, 0)], rows)
+    return D.groupby(ctx, [key], [rev, rev], [L.AGG_SUM, L.AGG_COUNT], order=L.ORDER_KEYS)
+
+
+def check_star(ctx, D, D_, M_, F_, spec, rows, cpu_rows, cpu_stats):
+    """The GPU's top-10 (from the timed runs) and one more run's pass and group counts
+    against the CPU port's on the same tables."""
+    _, g, st = D.star_agg(ctx, D_, M_, F_, spec, limit=10, want_groups=True,
+                          groups_cap=max(16, 4 * cpu_stats["groups"]))
+    del g
+    return bool(rows == cpu_rows and (st.dim_pass, st.mid_pass, st.fact_pass, st.groups) == (
+        cpu_stats["dim_pass"], cpu_stats["mid_pass"], cpu_stats["fact_pass"], cpu_stats["groups"]))
+
+
+def config0_leg(ctx, D, seed, reps, cpu_threads, cpu_runs):
+    """Flat filter + GROUP BY over a 1M-row table in one segment: GPU (operators, inputs
+    in HBM) and CPU (Thrust/OpenMP port), results checked against each other."""
+    import numpy as np
+    import torch
+
+    from alenka_amd import _lib as L
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import q3_reference as Q3
+
+    n = 1_000_000
+    raw = D.gen_flat(ctx, seed, n)
+    cols = [D.Column.encode(ctx, c, n, decimals=d) for c, d in zip(raw, (0, 0, 2, 2))]
+    host = [c.cpu().numpy() for c in raw]
+    del raw
+    t_gpu, (gk, gv) = timed(lambda: flat_gpu_query(ctx, D, L, cols), reps)
+    out = {"rows": n, "query": "WHERE date < 19950315 GROUP BY key: SUM(price*(1-discount)), COUNT(*)",
+           "groups": int(gk[0].numel()),
+           "gpu": {"ms": t_gpu * 1e3, "rows_per_s": n / t_gpu,
+                   "path": "aln_scan_filter -> aln_eval x2 -> aln_groupby (key order)"}}
+    if cpu_threads:
+        t_cpu, nr, (ck, cs, cc) = median_time(lambda: Q3.run_flat_cpu(host, threads=cpu_threads), cpu_runs)
+        out["cpu"] = {"ms": t_cpu * 1e3, "rows_per_s": n / t_cpu, "threads": cpu_threads, "runs": nr}
+        out["checked"] = bool(np.array_equal(gk[0].cpu().numpy(), ck) and np.array_equal(gv[0].cpu().numpy(), cs)
+                              and np.array_equal(gv[1].cpu().numpy(), cc))
+    del cols, gk, gv
+    torch.cuda.empty_cache()
+    return out
+
+
+def config1_leg(ctx, D, seed, seg_rows, steps, cpu_threads, cpu_runs):
+    """The q3 star pipeline at the 6M-row base size (the same generator, scale 1): GPU
+    (aln_star_agg, compressed columns in HBM) and CPU (Thrust/OpenMP port), checked."""
+    import torch
+
+    host = {}
+    dim, mid, fact, code_of, nfact = build_tables(ctx, D, seed, 1.0, seg_rows, host=host)
+    D_, M_, F_, spec = q3_spec(code_of, dim, mid, fact)
+    q = D.StarQuery(ctx, D_, M_, F_, spec, limit=10)
+    t_gpu, (rows, _, _) = timed(q.run, steps)
+    out = {"fact_rows": nfact, "orders_rows": mid[0].nrows, "dim_rows": dim[0].nrows, "seg_rows": seg_rows,
+           "gpu": {"ms": t_gpu * 1e3, "fact_rows_per_s": nfact / t_gpu, "path": "aln_star_agg"}}
+    if cpu_threads:
+        t_cpu, nr, (crows, cst) = cpu_q3(host, cpu_threads, cpu_runs)
+        out["cpu"] = {"ms": t_cpu * 1e3, "fact_rows_per_s": nfact / t_cpu, "threads": cpu_threads, "runs": nr}
+        out["checked"] = check_star(ctx, D, D_, M_, F_, spec, rows, crows, cst)
+    del dim, mid, fact, q
+    torch.cuda.empty_cache()
+    return out
 
 
 def batch_queries(seed, nq, nkeys, price_lo, price_span, ndays):
@@ -502,7 +608,10 @@ def main():
     ap.add_argument("--seg-rows", type=int, default=1000000)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample-scale", type=float, default=2.0)
+    ap.add_argument("--cpu-runs-full", type=int, default=3,
+                    help="timed CPU runs of the full-size q3 (after the run that checks the GPU's answer)")
+    ap.add_argument("--cpu-budget-s", type=float, default=60.0, help="time cap on those runs (at least one)")
+    ap.add_argument("--no-configs", action="store_true", help="skip the 1M flat and 6M star configurations")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-queries leg")
     ap.add_argument("--batch-queries", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=5)
@@ -535,7 +644,9 @@ def main():
         del src, dst
         torch.cuda.empty_cache()
         log("copy calibration: 3 x clone of 1 GiB")
-    dim, mid, fact, code_of, nfact = build_tables(ctx, D, args.seed, args.scale, args.seg_rows)
+    model, threads = cpu_info()
+    host = None if args.no_cpu_baseline else {}
+    dim, mid, fact, code_of, nfact = build_tables(ctx, D, args.seed, args.scale, args.seg_rows, host=host)
     D_, M_, F_, spec = q3_spec(code_of, dim, mid, fact)
     run = D.StarQuery(ctx, D_, M_, F_, spec, limit=10).run
 
@@ -551,7 +662,7 @@ def main():
     ms_per_step = elapsed * 1000.0 / args.steps
     log(f"timed {args.steps} steps: {ms_per_step:.3f} ms/step")
 
-    # dominant kernel: the fact scan, timed with HIP events on the context stream
+    # dominant kernel: the fact phase, timed with HIP events on the context stream
     ctx.profile(True)
     ctx.profile_reset()
     nprof = max(3, min(10, args.steps))
@@ -564,20 +675,54 @@ def main():
     ctx.profile(False)
     t_fact = kernels["star_fact"]["avg_ms"] / 1000.0
 
-    # algorithmic bytes (counts measured in this run)
-    fact_scan, fact_segs = scan_bytes(F_, [spec["f_fk"], spec["f_x"], spec["f_y"], 3])
+    # the same q3 on the host cores over the same tables: its answer checks the GPU's, its
+    # time is the full-size CPU baseline
+    full = None
+    groups = None
+    if host is not None:
+        log(f"cpu q3 over the full tables ({threads} threads) ...")
+        t_cpu, nr, (crows, cst) = cpu_q3(host, threads, args.cpu_runs_full, budget_s=args.cpu_budget_s)
+        checked = check_star(ctx, D, D_, M_, F_, spec, rows, crows, cst)
+        groups = cst["groups"]
+        full = {"ms": t_cpu * 1e3, "fact_rows_per_s": nfact / t_cpu, "threads": threads, "runs": nr,
+                "checked": checked, "cpu_top1": list(crows[0]) if crows else None, "cpu_counts": cst}
+        log(f"cpu q3: {t_cpu * 1e3:.0f} ms, checked {checked}")
+        del host
+        assert checked, "the GPU's q3 answer differs from the CPU port's on the same tables"
+
+    # Algorithmic bytes of the fact phase: what it must move from HBM whatever the kernel -
+    # the packed words of the columns read for every row of the scanned segments (join key
+    # and predicate columns), one 4-byte word per payload column per joined row, the join
+    # index once (mid bitmap over the key range + its rank directory, 16 B per 512 bits) and
+    # each touched group's SUM (8 B) and COUNT (4 B) read and written.
+    fact_cols = sorted({spec["f_fk"]} | {c for c, _, _ in F_["ranges"]})
+    fact_scan, fact_segs = scan_bytes(F_, fact_cols)
+    kmin = min(mid[0].hsegs[i].vmin for i in range(mid[0].nsegs))
+    kmax = max(mid[0].hsegs[i].vmax for i in range(mid[0].nsegs))
+    bitmap_bytes = (kmax - kmin + 1 + 31) // 32 * 4
+    join_index = bitmap_bytes + (kmax - kmin + 1 + 511) // 512 * 16
+    touched = groups if groups is not None else st.fact_pass
+    b_fact = fact_scan + 4 * 2 * st.fact_pass + join_index + 2 * 12 * touched
+    achieved = b_fact / t_fact / 1e9 if t_fact > 0 else 0.0
+    # the survey's per-unit counts for the whole step (8 B per probe row, whatever serves it)
+    all_fact_scan, _ = scan_bytes(F_, [spec["f_fk"], spec["f_x"], spec["f_y"], 3])
     mid_scan, mid_segs = scan_bytes(M_, [0, 1, 2, 3])
     dim_scan, dim_segs = scan_bytes(D_, [0, 1])
-    b_fact = fact_scan + 8 * st.fact_probe + 16 * st.fact_pass
-    achieved = b_fact / t_fact / 1e9 if t_fact > 0 else 0.0
-    b_pipeline = (dim_scan + mid_scan + fact_scan + 8 * st.dim_pass + 8 * st.mid_pass
+    b_pipeline = (dim_scan + mid_scan + all_fact_scan + 8 * st.dim_pass + 8 * st.mid_pass
                   + 8 * st.mid_probe + 8 * st.fact_probe + 16 * st.fact_pass + 10 * 40)
     pipeline_gbs = b_pipeline / (ms_per_step / 1000.0) / 1e9
 
-    cpu = None
-    if not args.no_cpu_baseline:
-        log("cpu baseline ...")
-        cpu = cpu_baseline(args.cpu_sample_scale, args.seed)
+    configs = {"2": {"fact_rows": nfact, "gpu": {"ms": ms_per_step, "fact_rows_per_s": nfact / (ms_per_step / 1e3),
+                                                  "path": "aln_star_agg"}}}
+    if full is not None:
+        configs["2"]["cpu"] = {k: full[k] for k in ("ms", "fact_rows_per_s", "threads", "runs")}
+        configs["2"]["checked"] = full["checked"]
+    if not args.no_configs:
+        cthreads = 0 if args.no_cpu_baseline else threads
+        configs["0"] = config0_leg(ctx, D, args.seed, 20, cthreads, 5)
+        log(f"config 0: {configs['0']}")
+        configs["1"] = config1_leg(ctx, D, args.seed, args.seg_rows, 20, cthreads, 5)
+        log(f"config 1: {configs['1']}")
 
     batched = served = None
     if not args.no_batch:
@@ -599,13 +744,20 @@ def main():
             log(f"widened: {widened['error']}")
 
     # HBM bytes per fact phase, from the two counter passes (FETCH_SIZE, WRITE_SIZE) of this
-    # workload, scaled by a plain copy's counts; the counters cannot be read from inside a run
+    # workload, scaled by a plain copy's counts.  The counters cannot be read from inside a
+    # run, so the figure comes from the committed profile, and only while the kernels' sources
+    # hash to what that profile was measured on (otherwise it is stale and left out).
     traffic = traffic_src = None
-    tf = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "star_fact_phase_600M_hbm_traffic.json")
+    tf = os.path.join(ROOT, "profiles", "star_fact_phase_600M_hbm_traffic.json")
     if (args.scale, args.seg_rows, args.seed) == (100.0, 1000000, 2024) and os.path.exists(tf):
         with open(tf) as f:
-            traffic = json.load(f)["traffic_bytes"]
-        traffic_src = "profiles/star_fact_phase_600M_hbm_traffic.json (rocprofv3 FETCH_SIZE/WRITE_SIZE, same workload)"
+            prof = json.load(f)
+        if prof.get("sources_sha256") == star_sources_hash():
+            traffic = prof["traffic_bytes"]
+            traffic_src = ("profiles/star_fact_phase_600M_hbm_traffic.json (rocprofv3 FETCH_SIZE/WRITE_SIZE of "
+                           "this workload, kernel sources sha256 " + prof["sources_sha256"][:12] + ")")
+        else:
+            traffic_src = "stale: the committed counter profile was taken on other kernel sources"
 
     value = nfact / (ms_per_step / 1000.0)
     out = {
@@ -638,9 +790,15 @@ def main():
             "kernel": "star_fact phase: k_star_fact_bm + k_fact_resolve",
             "kernel_avg_ms": kernels["star_fact"]["avg_ms"],
             "algorithmic_bytes": b_fact,
+            "bytes_definition": ("packed words of the fact columns read for every row of the scanned segments "
+                                 "(join key, shipdate) + 4 B per payload column per joined row + the join index "
+                                 "once (mid bitmap + rank directory) + 12 B SUM/COUNT read and written per group"),
+            "fact_segments_scanned": fact_segs,
         },
         "pipeline": {
             "algorithmic_bytes": b_pipeline,
+            "bytes_definition": ("per-unit counts for the whole step: packed bytes of every referenced column + 8 B "
+                                 "per build row + 8 B per probe row (whatever serves it) + 16 B per aggregated row"),
             "gbs": pipeline_gbs,
             "frac_of_peak": pipeline_gbs / HBM_PEAK_GBS,
             # per step: every launch under the phase's name (build and top-k are several)
@@ -650,7 +808,14 @@ def main():
             "segments_scanned": dict(dim=dim_segs, mid=mid_segs, fact=fact_segs),
             "top1": list(rows[0]) if rows else None,
         },
-        "cpu_baseline": cpu,
+        "checked": None if full is None else full["checked"],
+        "cpu_baseline": None if full is None else {
+            "value": full["fact_rows_per_s"], "unit": "fact rows/s", "cores": full["threads"], "kind": "port",
+            "sample": (f"the whole workload (fact {nfact} rows): q3 through oracle/libq3cpu.so, the "
+                       f"reference's Thrust algorithms on rocThrust's OpenMP backend, {full['threads']} threads on "
+                       f"{model}; median of {full['runs']} runs after one that checked the GPU's top-10 and "
+                       f"pass counts")},
+        "configs": configs,
         "batched": batched,
         "served": served,
         "widened": widened,

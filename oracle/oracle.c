@@ -7,7 +7,8 @@
  * Parity pins (what keeps each restatement honest; see tests/test_oracle.py):
  *   orc_murmur64a        - the reference's own MurmurHash2_64.cu compiled by
  *                          oracle/build_ref.sh into oracle/_ref (bit-equal on random keys
- *                          of every length 0..64), and hand-derived vectors in tests/golden.
+ *                          of every length 0..64), and the vectors in tests/golden, which
+                          tests/golden/make_murmur_vectors.py wrote from that same build.
  *   orc_day_to_ymd       - Python datetime over every day of the generator's range.
  *   orc_pack/unpack      - round trip + a hand-laid 3-bit example.
  *   orc_pfor_*           - byte layout vs compress.cu:388-399/511-522 fields read back by
@@ -25,6 +26,11 @@
  *   orc_semi_anti        - Python set membership over all key pairs (brute force), and the
  *                          stable key order of the right side.
  *   orc_order            - Python's stable sorted() with a composite key.
+ *   orc_compact          - numpy's flatnonzero(mask) + row0 on random masks of every
+ *                          density, the empty mask, an all-true mask and last-row-only.
+ *   orc_kahan_sum_f64    - math.fsum (exactly rounded) within Kahan's error bound
+ *                          2u*sum|x|; inputs where a naive sum is off by far more than that
+ *                          bound, and cancelling sums whose exact value is known.
  *   orc_like             - Python's re.fullmatch with '%' -> '.*' and '?' -> '[^.]' on
  *                          values and patterns too short for the restart counter to bind,
  *                          plus hand-derived cases where it does.

@@ -61,25 +61,53 @@ def run(t, limit=10, segment="BUILDING", date=DATE):
     return top, groups, stats
 
 
-def run_cpu_parallel(t, threads=0, limit=10, segment="BUILDING", date=DATE):
-    """The same q3 through oracle/libq3cpu.so (OpenMP, the reference's sort/merge/segmented-
-    reduce algorithms); returns (top rows, threads used)."""
+def _cpu_lib():
     import ctypes as C
     import os
 
     lib = C.CDLL(os.path.join(os.path.dirname(os.path.abspath(__file__)), "..", "oracle", "libq3cpu.so"))
     P = C.POINTER(C.c_int64)
-    lib.q3cpu_run.restype = C.c_int
-    lib.q3cpu_run.argtypes = [P, P, C.c_uint64, P, P, P, P, C.c_uint64, P, P, P, P, C.c_uint64,
-                              C.c_int64, C.c_int64, C.c_int, C.c_int, P]
+    lib.q3cpu_run_stats.restype = C.c_int
+    lib.q3cpu_run_stats.argtypes = [P, P, C.c_uint64, P, P, P, P, C.c_uint64, P, P, P, P, C.c_uint64,
+                                    C.c_int64, C.c_int64, C.c_int, C.c_int, P, P]
+    lib.flatcpu_run.restype = C.c_int64
+    lib.flatcpu_run.argtypes = [P, P, P, P, C.c_uint64, C.c_int64, C.c_int, P, P, P, C.c_uint64]
     lib.q3cpu_threads.restype = C.c_int
+    return lib, P
+
+
+def run_cpu_parallel(t, threads=0, limit=10, segment="BUILDING", date=DATE, with_stats=False):
+    """The same q3 through oracle/libq3cpu.so (the reference's Thrust algorithms on the OpenMP
+    backend); returns (top rows, threads used) and, with_stats, the pass counts as run()
+    names them."""
+    lib, P = _cpu_lib()
     arr = {k: np.ascontiguousarray(t[k], np.int64) for k in ("c_key", "c_seg", "o_key", "o_cust", "o_date",
                                                              "o_prio", "l_okey", "l_price", "l_disc", "l_ship")}
     p = lambda k: arr[k].ctypes.data_as(P)
     top = np.zeros((max(1, limit), 5), np.int64)
+    st = np.zeros(4, np.int64)
     code = int(t["code_of"][SEGMENTS.index(segment)])
-    n = lib.q3cpu_run(p("c_key"), p("c_seg"), len(arr["c_key"]), p("o_key"), p("o_cust"), p("o_date"),
-                      p("o_prio"), len(arr["o_key"]), p("l_okey"), p("l_price"), p("l_disc"), p("l_ship"),
-                      len(arr["l_okey"]), code, date, threads, limit, top.ctypes.data_as(P))
+    n = lib.q3cpu_run_stats(p("c_key"), p("c_seg"), len(arr["c_key"]), p("o_key"), p("o_cust"), p("o_date"),
+                            p("o_prio"), len(arr["o_key"]), p("l_okey"), p("l_price"), p("l_disc"), p("l_ship"),
+                            len(arr["l_okey"]), code, date, threads, limit, top.ctypes.data_as(P),
+                            st.ctypes.data_as(P))
     used = threads if threads > 0 else lib.q3cpu_threads()
-    return [tuple(int(x) for x in top[i]) for i in range(n)], used
+    rows = [tuple(int(x) for x in top[i]) for i in range(n)]
+    if with_stats:
+        return rows, used, dict(dim_pass=int(st[0]), mid_pass=int(st[1]), fact_pass=int(st[2]), groups=int(st[3]))
+    return rows, used
+
+
+def run_flat_cpu(cols, date_lt=DATE, threads=0, want=True):
+    """The flat query (WHERE date < date_lt GROUP BY key, SUM(price*(1-discount)), COUNT(*))
+    through oracle/libq3cpu.so; cols = [key, date, price, disc].  Returns (keys, sums,
+    counts) in key order, or only the group count when want is False."""
+    lib, P = _cpu_lib()
+    a = [np.ascontiguousarray(c, np.int64) for c in cols]
+    n = len(a[0])
+    if not want:
+        return lib.flatcpu_run(*[x.ctypes.data_as(P) for x in a], n, date_lt, threads, None, None, None, 0)
+    out = [np.zeros(max(1, n), np.int64) for _ in range(3)]
+    g = lib.flatcpu_run(*[x.ctypes.data_as(P) for x in a], n, date_lt, threads,
+                        *[o.ctypes.data_as(P) for o in out], n)
+    return tuple(o[:g] for o in out)

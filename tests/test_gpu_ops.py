@@ -473,3 +473,64 @@ def test_star_duplicate_keys_rejected(ctx, force_hash):
     with pytest.raises(L.AlnError) as e:
         D.star_agg(ctx, D_, M_, F_, spec, limit=10, force_hash=force_hash)
     assert e.value.code == L.ALN_E_UNSUPPORTED
+
+
+# ---- the benchmark's segment layout: 1,000,000-row segments ------------------------------
+@pytest.mark.parametrize("join", ["bitmap", "hash"])
+def test_star_q3_million_row_segments(ctx, join):
+    """q3 at the 6M-row base size (dimension 150k, orders 1.5M, fact ~6M rows) in the
+    benchmark's 1,000,000-row segments: six fact segments, each 244 full 4096-row tiles and
+    a 576-row tail tile, and the resolve grid built from that shape; groups, pass counts and
+    top-10 against the oracle, then three runs of a prepared query as the benchmark loop
+    makes them."""
+    from alenka_amd import device as D
+
+    t = Q3.tables(2024, 150000, 1500000)
+    top, groups, stats = Q3.run(t)
+    assert len(t["l_okey"]) > 5_000_000 and stats["fact_pass"] > 10_000
+    dim, mid, fact = encode_q3(ctx, t, 1_000_000)
+    assert fact[0].nsegs == 6 and all(fact[0].hsegs[s].nrows % 4096 == 576 for s in range(5))
+    D_, M_, F_, spec = q3_args(t, dim, mid, fact)
+    rows, g, st = D.star_agg(ctx, D_, M_, F_, spec, limit=10, want_groups=True, force_hash=join == "hash")
+    assert rows == top
+    assert (st.dim_pass, st.mid_pass, st.fact_pass, st.groups) == (
+        stats["dim_pass"], stats["mid_pass"], stats["fact_pass"], stats["groups"])
+    assert np.array_equal(host(g), groups)
+    if join == "bitmap":
+        q = D.StarQuery(ctx, D_, M_, F_, spec, limit=10)
+        for _ in range(3):
+            r2, _, st2 = q.run()
+            assert r2 == top and (st2.mid_pass, st2.fact_pass) == (stats["mid_pass"], stats["fact_pass"])
+
+
+@pytest.mark.parametrize("name,ops", [PREDICATES[0], PREDICATES[1], PREDICATES[2], PREDICATES[6]])
+@pytest.mark.parametrize("zone_maps", [False, True])
+def test_scan_filter_million_row_segments(ctx, name, ops, zone_maps):
+    """aln_scan_filter over 3.5 segments of 1,000,000 rows (the benchmark's segment size):
+    exact row ids against the oracle's compaction, with and without zone-map skipping."""
+    from alenka_amd import device as D
+
+    n, seg_rows = 3_500_000, 1_000_000
+    cols = O.gen_flat(12, n)
+    cols[1] = np.sort(cols[1])  # date sorted, so zone maps give A / N / R segments
+    dec = [0, 0, 2, 2]
+    dcols = [D.Column.encode(ctx, dev(ctx, c), seg_rows, decimals=d) for c, d in zip(cols, dec)]
+    assert dcols[0].nsegs == 4
+    prog, _ = L.compile_rpn(ops, dec)
+    modes = None
+    if zone_maps:
+        modes = [L.zone_map_check(ops, [c.hsegs[s].vmin for c in dcols], [c.hsegs[s].vmax for c in dcols], dec)
+                 for s in range(4)]
+    got = host(D.scan_filter(ctx, dcols, prog, modes)).astype(np.uint32)
+    parts = []
+    for s in range(4):
+        lo, hi = s * seg_rows, min(n, (s + 1) * seg_rows)
+        m = "R" if modes is None else modes[s]
+        if m == "A":
+            parts.append(np.arange(lo, hi, dtype=np.uint32))
+        elif m == "R":
+            parts.append(O.compact(O.filter_mask(ops, [c[lo:hi] for c in cols], dec), lo))
+    want = np.concatenate(parts)
+    assert np.array_equal(got, want)
+    if modes is None:  # the whole-table mask gives the same ids
+        assert np.array_equal(got, O.compact(O.filter_mask(ops, cols, dec)))

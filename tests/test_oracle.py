@@ -398,11 +398,91 @@ def test_order_python_sorted():
 
 
 def test_parallel_cpu_q3_matches_oracle():
-    """The OpenMP q3 run used as the benchmark's CPU baseline returns the oracle's rows."""
+    """The Thrust/OpenMP q3 run used as the benchmark's CPU baseline (and as the checker of
+    the full-size GPU answer) returns the oracle's rows and pass counts."""
     import q3_reference as Q3
 
     t = Q3.tables(31, 6000, 60000)
     for seg, date in [("BUILDING", 19950315), ("FURNITURE", 19950310), ("AUTOMOBILE", 19930101)]:
-        want, _, _ = Q3.run(t, segment=seg, date=date)
-        got, used = Q3.run_cpu_parallel(t, 4, segment=seg, date=date)
-        assert got == want and used == 4
+        want, _, stats = Q3.run(t, segment=seg, date=date)
+        got, used, st = Q3.run_cpu_parallel(t, 4, segment=seg, date=date, with_stats=True)
+        assert got == want and used == 4 and st == stats
+
+
+def test_parallel_cpu_flat_matches_oracle():
+    """The flat filter + GROUP BY CPU baseline returns the oracle's groups in key order."""
+    import q3_reference as Q3
+
+    cols = O.gen_flat(5, 200_000)
+    sel = cols[1] < Q3.DATE
+    rev = cols[2][sel] * (100 - cols[3][sel])
+    wk, wv = O.groupby([cols[0][sel]], [rev, rev], [L.AGG_SUM, L.AGG_COUNT])
+    k, s, c = Q3.run_flat_cpu(cols, threads=4)
+    assert np.array_equal(k, wk[0]) and np.array_equal(s, wv[0]) and np.array_equal(c, wv[1])
+    assert Q3.run_flat_cpu(cols, want=False) == len(wk[0])
+
+
+# ---- compaction (count + copy_if, cm.cu:3648-3650) ----------------------------------------
+@pytest.mark.parametrize("density", [0.0, 0.003, 0.5, 0.97, 1.0])
+def test_compact_matches_flatnonzero(density):
+    r = np.random.default_rng(int(density * 1000) + 5)
+    for n in (1, 63, 64, 65, 4095, 4096, 100_003):
+        mask = r.random(n) < density
+        for row0 in (0, 1, 1_000_000, (1 << 32) - n - 1):
+            got = O.compact(mask, row0)
+            want = (np.flatnonzero(mask) + row0).astype(np.uint32)
+            assert got.dtype == np.uint32 and np.array_equal(got, want), (n, row0)
+
+
+def test_compact_edge_masks():
+    assert len(O.compact(np.zeros(0, np.uint8), 7)) == 0
+    assert len(O.compact(np.zeros(1000, np.uint8), 7)) == 0
+    assert np.array_equal(O.compact(np.ones(1000, np.uint8), 7), np.arange(7, 1007, dtype=np.uint32))
+    last = np.zeros(1000, np.uint8)
+    last[-1] = 1
+    assert list(O.compact(last, 0)) == [999]  # the reference's star_join drops this row
+    first = np.zeros(1000, np.uint8)
+    first[0] = 1
+    assert list(O.compact(first, 3)) == [3]
+
+
+# ---- FP64 Kahan sum: the checker of FP64 aggregates ----------------------------------------
+def test_kahan_sum_within_error_bound_of_fsum():
+    import math
+
+    u = 2.0 ** -53
+    r = np.random.default_rng(17)
+    cases = [
+        r.uniform(-1, 1, 10_000) * 10.0 ** r.integers(-8, 9, 10_000),  # mixed magnitudes
+        np.concatenate([[1e16], r.uniform(0, 1, 5000), [-1e16]]),  # cancellation
+        r.uniform(900, 105000, 100_000).round(2) * (1 - r.integers(0, 11, 100_000) / 100),
+    ]
+    for v in cases:
+        got, want = O.kahan_sum(v), math.fsum(v)
+        bound = (2 * u + len(v) * u * u) * float(np.abs(v).sum())
+        assert abs(got - want) <= bound, (got, want, bound)
+
+
+def test_kahan_sum_beats_naive_where_naive_fails():
+    import math
+
+    # 1 + 10^6 * 1e-16: a running double sum never leaves 1.0, the compensated one does
+    v = np.concatenate([[1.0], np.full(1_000_000, 1e-16)])
+    naive = 0.0
+    for x in v[:2000]:
+        naive += x
+    assert naive == 1.0  # the naive sum stalls
+    got, want = O.kahan_sum(v), math.fsum(v)
+    assert abs(want - (1.0 + 1e-10)) < 1e-20
+    assert abs(got - want) <= 4 * 2.0 ** -53 * want
+    assert abs(got - 1.0) > 9e-11  # compensation kept the small terms
+
+
+def test_kahan_sum_exact_cases():
+    assert O.kahan_sum(np.zeros(0)) == 0.0
+    assert O.kahan_sum(np.arange(1, 100_001, dtype=np.float64)) == 100_000 * 100_001 / 2
+    r = np.random.default_rng(3)
+    x = r.normal(size=5000) * 1e8
+    v = np.concatenate([x, -x, [0.125]])
+    r.shuffle(v)
+    assert abs(O.kahan_sum(v) - 0.125) <= 2 * 2.0 ** -53 * float(np.abs(v).sum())
