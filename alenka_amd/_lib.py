@@ -15,6 +15,7 @@ ALN_E_HIP = -2
 ALN_E_NOMEM = -3
 ALN_E_UNSUPPORTED = -4
 ALN_E_CAPACITY = -5
+SCAN_NOALLOC_MAX_TILES = 262144  # ALN_SCAN_NOALLOC_MAX_TILES: tiles of 4096 rows
 
 RPN_NAME, RPN_NUMBER, RPN_STRING, RPN_CMP, RPN_AND, RPN_OR = 1, 2, 3, 4, 5, 6
 RPN_ADD, RPN_MINUS, RPN_MUL, RPN_DIV, RPN_CODESET = 7, 8, 9, 10, 11
@@ -131,6 +132,8 @@ SIGNATURES = {
     "aln_segfile_encode": (C.c_int, [P, P, U64, C.c_char_p, U64, PU64]),
     "aln_scan_filter": (C.c_int, [P, C.POINTER(Col), C.c_int, U32, C.POINTER(Insn), C.c_int,
                                   C.c_char_p, P, U64, PU64]),
+    "aln_scan_filter_noalloc": (C.c_int, [P, C.POINTER(Col), C.c_int, U32, C.POINTER(Insn), C.c_int,
+                                          C.c_char_p, P, U64, PU64]),
     "aln_gather": (C.c_int, [P, C.POINTER(Col), U32, P, U64, P]),
     "aln_eval": (C.c_int, [P, C.POINTER(Col), C.c_int, U32, C.POINTER(Insn), C.c_int, P, U64, P]),
     "aln_apply_perm": (C.c_int, [P, P, P, U64, P]),

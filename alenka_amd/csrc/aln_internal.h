@@ -98,6 +98,14 @@ int exclusive_scan_u32(aln_ctx *c, const uint32_t *d_in, uint64_t n, uint64_t *d
                        uint64_t *h_total);
 int exclusive_scan_u64(aln_ctx *c, const uint64_t *d_in, uint64_t n, uint64_t *d_out,
                        uint64_t *h_total);
+// The same scan on caller-provided temporaries: d_sums holds scan_tiles(n) entries, d_total
+// one; enqueued on s, no readback.
+uint64_t scan_tiles(uint64_t n);
+int exclusive_scan_u32_buf(hipStream_t s, const uint32_t *d_in, uint64_t n, uint64_t *d_out,
+                           uint64_t *d_sums, uint64_t *d_total);
+// Looks up this library's fixed device scratch (aln_scan_filter_noalloc) on `device`, which
+// must be current; once per device, later calls return the recorded status.
+int scan_scratch_attach(int device);
 int minmax_i64(aln_ctx *c, const int64_t *d, uint64_t n, int64_t *h_min, int64_t *h_max);
 int sum_u32(aln_ctx *c, const uint32_t *d_in, uint64_t n, uint64_t *d_out);  // no sync
 // *out[i] += sum of in[i][0, n[i]) for up to SUM_MULTI_MAX arrays, one launch (no sync);

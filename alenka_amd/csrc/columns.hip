@@ -5,6 +5,8 @@
 // compaction (filter_op, cm.cu:3609-3666; filter, filter.cu:133-643).
 #include <string.h>
 
+#include <atomic>
+#include <mutex>
 #include <vector>
 
 #include "aln_internal.h"
@@ -278,11 +280,15 @@ __global__ void __launch_bounds__(SC_BLOCK) k_scan(const ColTable *__restrict__ 
 // Recognise a conjunction of "column OP constant" terms (multiplier 1) and code-set tests
 // (LIKE) and turn it into one closed interval per column plus the set tests.  Returns false
 // when the program has another shape.
-static bool as_intervals(const aln_insn *p, int n, std::vector<RangeC> &out, bool *empty) {
+// At most one interval per column plus one set test per two instructions.
+constexpr int MAX_RANGES_C = ALN_MAX_COLS + ALN_MAX_INSN / 2;
+static bool as_intervals(const aln_insn *p, int n, RangeC *out, int *nout, bool *empty) {
   *empty = false;
+  *nout = 0;
   int64_t lo[ALN_MAX_COLS], hi[ALN_MAX_COLS];
   bool used[ALN_MAX_COLS] = {false};
-  std::vector<RangeC> sets;
+  RangeC sets[ALN_MAX_INSN / 2];
+  int nsets = 0;
   int i = 0;
   int pending = 0;  // predicates on the evaluation stack
   while (i < n) {
@@ -293,7 +299,8 @@ static bool as_intervals(const aln_insn *p, int n, std::vector<RangeC> &out, boo
       continue;
     }
     if (i + 1 < n && p[i].op == ALN_I_COL && p[i].k == 1 && p[i + 1].op == ALN_I_INSET) {
-      sets.push_back(RangeC{p[i].a, 1, p[i + 1].k, (int64_t)p[i + 1].a});
+      if (nsets == ALN_MAX_INSN / 2 || p[i].a < 0 || p[i].a >= ALN_MAX_COLS) return false;
+      sets[nsets++] = RangeC{p[i].a, 1, p[i + 1].k, (int64_t)p[i + 1].a};
       ++pending;
       i += 2;
       continue;
@@ -303,6 +310,7 @@ static bool as_intervals(const aln_insn *p, int n, std::vector<RangeC> &out, boo
           p[i + 2].op == ALN_I_CMP))
       return false;
     int c = p[i].a;
+    if (c < 0 || c >= ALN_MAX_COLS) return false;
     int64_t v = p[i + 1].k, l = INT64_MIN, h = INT64_MAX;
     switch (p[i + 2].a) {
       case 1:
@@ -328,36 +336,32 @@ static bool as_intervals(const aln_insn *p, int n, std::vector<RangeC> &out, boo
     i += 3;
   }
   if (pending != 1) return false;
-  out.clear();
+  int k = 0;
   for (int c = 0; c < ALN_MAX_COLS; ++c)
     if (used[c]) {
       if (lo[c] > hi[c]) *empty = true;
-      out.push_back(RangeC{c, 0, lo[c], hi[c]});
+      out[k++] = RangeC{c, 0, lo[c], hi[c]};
     }
-  out.insert(out.end(), sets.begin(), sets.end());
+  for (int j = 0; j < nsets; ++j) out[k++] = sets[j];
+  *nout = k;
   return true;
 }
 
-int upload_cols(aln_ctx *c, const aln_col *cols, int ncols, ColTable **out) {
+static int fill_cols(const aln_col *cols, int ncols, ColTable *h) {
   ALN_REQUIRE(ncols >= 1 && ncols <= ALN_MAX_COLS, "column count %d outside 1..%d", ncols,
               ALN_MAX_COLS);
-  ColTable h;
-  memset(&h, 0, sizeof(h));
+  memset(h, 0, sizeof(*h));
   for (int i = 0; i < ncols; ++i) {
     ALN_REQUIRE(cols[i].words || cols[i].nsegs == 0, "column %d has no words", i);
     ALN_REQUIRE(cols[i].segs, "column %d has no segment descriptors", i);
     ALN_REQUIRE(cols[i].nsegs == cols[0].nsegs, "columns differ in segment count");
-    h.words[i] = cols[i].words;
-    h.segs[i] = cols[i].segs;
+    h->words[i] = cols[i].words;
+    h->segs[i] = cols[i].segs;
   }
-  ColTable *d;
-  ALN_TRY(ws(c, 1, &d));
-  ALN_HIP(hipMemcpyAsync(d, &h, sizeof(h), hipMemcpyHostToDevice, c->stream));
-  *out = d;
   return ALN_OK;
 }
 
-int upload_prog(aln_ctx *c, const aln_insn *prog, int nprog, aln_insn **out) {
+static int check_prog(const aln_insn *prog, int nprog) {
   ALN_REQUIRE(prog && nprog >= 1 && nprog <= ALN_MAX_INSN, "program length %d", nprog);
   int depth = 0, mx = 0;
   for (int i = 0; i < nprog; ++i) {
@@ -371,6 +375,21 @@ int upload_prog(aln_ctx *c, const aln_insn *prog, int nprog, aln_insn **out) {
     mx = depth > mx ? depth : mx;
   }
   ALN_REQUIRE(depth == 1 && mx <= ALN_MAX_STACK, "program leaves %d values (max depth %d)", depth, mx);
+  return ALN_OK;
+}
+
+int upload_cols(aln_ctx *c, const aln_col *cols, int ncols, ColTable **out) {
+  ColTable h;
+  ALN_TRY(fill_cols(cols, ncols, &h));
+  ColTable *d;
+  ALN_TRY(ws(c, 1, &d));
+  ALN_HIP(hipMemcpyAsync(d, &h, sizeof(h), hipMemcpyHostToDevice, c->stream));
+  *out = d;
+  return ALN_OK;
+}
+
+int upload_prog(aln_ctx *c, const aln_insn *prog, int nprog, aln_insn **out) {
+  ALN_TRY(check_prog(prog, nprog));
   aln_insn *d;
   ALN_TRY(ws(c, nprog, &d));
   ALN_HIP(hipMemcpyAsync(d, prog, sizeof(aln_insn) * nprog, hipMemcpyHostToDevice, c->stream));
@@ -382,6 +401,67 @@ static int check_prog_cols(const aln_insn *prog, int nprog, int ncols) {
   for (int i = 0; i < nprog; ++i)
     if (prog[i].op == ALN_I_COL && prog[i].a >= ncols)
       return set_error(ALN_E_ARG, "program reads column %d of %d", prog[i].a, ncols);
+  return ALN_OK;
+}
+
+// ---- fixed scratch of the allocation-free scan -----------------------------------------------
+// aln_scan_filter_noalloc keeps its temporaries in this array instead of the context's
+// workspace: the column table, program and intervals it uploads, the per-segment zone-map
+// verdicts, the per-tile counts, their exclusive scan and the scan's partial sums and total.
+// Every byte a call reads is written earlier in the same call, so nothing is cleared.  Each
+// GPU holds its own copy for the life of the process; host code reaches it only through the
+// address hipGetSymbolAddress returns on that device.
+constexpr uint64_t NA_TILES = ALN_SCAN_NOALLOC_MAX_TILES;
+constexpr size_t NA_CT = 0;                                               // ColTable
+constexpr size_t NA_PROG = 256;                                           // ALN_MAX_INSN insns
+constexpr size_t NA_RANGES = NA_PROG + sizeof(aln_insn) * ALN_MAX_INSN;   // MAX_RANGES_C
+constexpr size_t NA_TOTAL = 2560;                                         // u64 scan total
+constexpr size_t NA_MODES = 4096;                                         // u8 per segment
+constexpr size_t NA_COUNTS = NA_MODES + NA_TILES;                         // u32 per tile
+constexpr size_t NA_OFFS = NA_COUNTS + 4 * NA_TILES;                      // u64 per tile
+constexpr size_t NA_SUMS = NA_OFFS + 8 * NA_TILES;                        // u64 per scan tile
+constexpr size_t NA_BYTES = NA_SUMS + 8 * ((NA_TILES + 4095) / 4096 + 32);
+static_assert(sizeof(ColTable) <= NA_PROG, "column table overlaps the program");
+static_assert(NA_RANGES + sizeof(RangeC) * MAX_RANGES_C <= NA_TOTAL, "intervals overlap the total");
+static_assert(NA_TILES % 256 == 0, "scratch regions lose their alignment");
+
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+constexpr int NA_MAX_DEVICES = 64;
+static std::mutex g_na_mu;
+static std::atomic<char *> g_na_ptr[NA_MAX_DEVICES];
+
+int scan_scratch_attach(int device) {
+  ALN_REQUIRE(device >= 0 && device < NA_MAX_DEVICES, "device %d outside 0..%d", device,
+              NA_MAX_DEVICES - 1);
+  std::lock_guard<std::mutex> lock(g_na_mu);
+  if (g_na_ptr[device].load()) return ALN_OK;
+  void *p = nullptr;
+  hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    (void)hipGetLastError();
+    return set_error(ALN_E_HIP, "hipGetSymbolAddress of the scan scratch on device %d failed: %s", device,
+                     hipGetErrorString(e));
+  }
+  g_na_ptr[device].store(static_cast<char *>(p));
+  return ALN_OK;
+}
+
+// The scratch address on the context's device, looked up now (with that device current for
+// the call) if the context's creation could not.
+static int scan_scratch(aln_ctx *c, char **out) {
+  ALN_REQUIRE(c->device >= 0 && c->device < NA_MAX_DEVICES, "device %d", c->device);
+  char *p = g_na_ptr[c->device].load();
+  if (!p) {
+    int cur = -1;
+    ALN_HIP(hipGetDevice(&cur));
+    if (cur != c->device) ALN_HIP(hipSetDevice(c->device));
+    int r = scan_scratch_attach(c->device);
+    if (cur != c->device) ALN_HIP(hipSetDevice(cur));
+    ALN_TRY(r);
+    p = g_na_ptr[c->device].load();
+  }
+  *out = p;
   return ALN_OK;
 }
 
@@ -584,15 +664,15 @@ int aln_scan_filter(aln_ctx *c, const aln_col *cols, int ncols, uint32_t seg_row
   aln_insn *dp;
   ALN_TRY(upload_cols(c, cols, ncols, &ct));
   ALN_TRY(upload_prog(c, prog, nprog, &dp));
-  std::vector<RangeC> rgs;
+  RangeC rgs[MAX_RANGES_C];
+  int nrg = 0;
   bool empty = false;
-  bool fast = as_intervals(prog, nprog, rgs, &empty);
+  bool fast = as_intervals(prog, nprog, rgs, &nrg, &empty);
   if (fast && empty) return ALN_OK;
   RangeC *drg = nullptr;
-  if (fast && !rgs.empty()) {
-    ALN_TRY(ws(c, rgs.size(), &drg));
-    ALN_HIP(hipMemcpyAsync(drg, rgs.data(), sizeof(RangeC) * rgs.size(), hipMemcpyHostToDevice,
-                           c->stream));
+  if (fast && nrg > 0) {
+    ALN_TRY(ws(c, nrg, &drg));
+    ALN_HIP(hipMemcpyAsync(drg, rgs, sizeof(RangeC) * nrg, hipMemcpyHostToDevice, c->stream));
   }
   char *dmodes = nullptr;
   if (seg_modes) {
@@ -610,7 +690,7 @@ int aln_scan_filter(aln_ctx *c, const aln_col *cols, int ncols, uint32_t seg_row
     ProfScope p(c, "scan_count");
     if (fast)
       hipLaunchKernelGGL((k_scan<0, true>), dim3((unsigned)nblocks), dim3(SC_BLOCK), 0, c->stream,
-                         ct, dp, nprog, drg, (int)rgs.size(), seg_rows, tps, dmodes, counts,
+                         ct, dp, nprog, drg, nrg, seg_rows, tps, dmodes, counts,
                          (const uint64_t *)nullptr, (uint32_t *)nullptr);
     else
       hipLaunchKernelGGL((k_scan<0, false>), dim3((unsigned)nblocks), dim3(SC_BLOCK), 0, c->stream,
@@ -629,12 +709,80 @@ int aln_scan_filter(aln_ctx *c, const aln_col *cols, int ncols, uint32_t seg_row
     ProfScope p(c, "scan_write");
     if (fast)
       hipLaunchKernelGGL((k_scan<1, true>), dim3((unsigned)nblocks), dim3(SC_BLOCK), 0, c->stream,
-                         ct, dp, nprog, drg, (int)rgs.size(), seg_rows, tps, dmodes, counts, offs,
+                         ct, dp, nprog, drg, nrg, seg_rows, tps, dmodes, counts, offs,
                          d_row_ids);
     else
       hipLaunchKernelGGL((k_scan<1, false>), dim3((unsigned)nblocks), dim3(SC_BLOCK), 0, c->stream,
                          ct, dp, nprog, drg, 0, seg_rows, tps, dmodes, counts, offs, d_row_ids);
   }
+  ALN_TRY(check_launch("scan_write"));
+  ALN_HIP(hipStreamSynchronize(c->stream));
+  return ALN_OK;
+}
+
+int aln_scan_filter_noalloc(aln_ctx *c, const aln_col *cols, int ncols, uint32_t seg_rows,
+                            const aln_insn *prog, int nprog, const char *seg_modes,
+                            uint32_t *d_row_ids, uint64_t cap, uint64_t *h_count) {
+  ALN_REQUIRE(c && cols && h_count && seg_rows > 0, "aln_scan_filter_noalloc: bad arguments");
+  ALN_TRY(check_prog_cols(prog, nprog, ncols));
+  ColTable hct;
+  ALN_TRY(fill_cols(cols, ncols, &hct));
+  ALN_TRY(check_prog(prog, nprog));
+  const uint32_t nsegs = cols[0].nsegs;
+  ALN_REQUIRE((uint64_t)nsegs * seg_rows <= (1ull << 32), "aln_scan_filter_noalloc: more than 2^32 rows");
+  *h_count = 0;
+  if (nsegs == 0) return ALN_OK;
+  const uint32_t tps = (seg_rows + SC_TILE - 1) / SC_TILE;
+  const uint64_t nblocks = (uint64_t)nsegs * tps;
+  if (nblocks > NA_TILES)
+    return set_error(ALN_E_ARG,
+                     "aln_scan_filter_noalloc: %u segments of %u rows are %llu tiles of %d rows; the fixed "
+                     "scratch holds %llu tiles (use aln_scan_filter)",
+                     nsegs, seg_rows, (unsigned long long)nblocks, SC_TILE, (unsigned long long)NA_TILES);
+  RangeC rgs[MAX_RANGES_C];
+  int nrg = 0;
+  bool empty = false;
+  const bool fast = as_intervals(prog, nprog, rgs, &nrg, &empty);
+  if (fast && empty) return ALN_OK;
+  char *base;
+  ALN_TRY(scan_scratch(c, &base));
+  ColTable *ct = reinterpret_cast<ColTable *>(base + NA_CT);
+  aln_insn *dp = reinterpret_cast<aln_insn *>(base + NA_PROG);
+  RangeC *drg = fast && nrg > 0 ? reinterpret_cast<RangeC *>(base + NA_RANGES) : nullptr;
+  uint64_t *tot = reinterpret_cast<uint64_t *>(base + NA_TOTAL);
+  char *dmodes = seg_modes ? base + NA_MODES : nullptr;
+  uint32_t *counts = reinterpret_cast<uint32_t *>(base + NA_COUNTS);
+  uint64_t *offs = reinterpret_cast<uint64_t *>(base + NA_OFFS);
+  uint64_t *sums = reinterpret_cast<uint64_t *>(base + NA_SUMS);
+  ALN_HIP(hipMemcpyAsync(ct, &hct, sizeof(hct), hipMemcpyHostToDevice, c->stream));
+  ALN_HIP(hipMemcpyAsync(dp, prog, sizeof(aln_insn) * nprog, hipMemcpyHostToDevice, c->stream));
+  if (drg) ALN_HIP(hipMemcpyAsync(drg, rgs, sizeof(RangeC) * nrg, hipMemcpyHostToDevice, c->stream));
+  if (dmodes) ALN_HIP(hipMemcpyAsync(dmodes, seg_modes, nsegs, hipMemcpyHostToDevice, c->stream));
+  if (fast)
+    hipLaunchKernelGGL((k_scan<0, true>), dim3((unsigned)nblocks), dim3(SC_BLOCK), 0, c->stream, ct, dp,
+                       nprog, drg, nrg, seg_rows, tps, dmodes, counts, (const uint64_t *)nullptr,
+                       (uint32_t *)nullptr);
+  else
+    hipLaunchKernelGGL((k_scan<0, false>), dim3((unsigned)nblocks), dim3(SC_BLOCK), 0, c->stream, ct, dp,
+                       nprog, drg, 0, seg_rows, tps, dmodes, counts, (const uint64_t *)nullptr,
+                       (uint32_t *)nullptr);
+  ALN_TRY(check_launch("scan_count"));
+  ALN_TRY(exclusive_scan_u32_buf(c->stream, counts, nblocks, offs, sums, tot));
+  uint64_t total = 0;
+  ALN_HIP(hipMemcpyAsync(&total, tot, sizeof(total), hipMemcpyDeviceToHost, c->stream));
+  ALN_HIP(hipStreamSynchronize(c->stream));
+  *h_count = total;
+  if (total > cap)
+    return set_error(ALN_E_CAPACITY, "aln_scan_filter_noalloc: %llu rows pass, capacity %llu",
+                     (unsigned long long)total, (unsigned long long)cap);
+  if (total == 0) return ALN_OK;
+  ALN_REQUIRE(d_row_ids, "aln_scan_filter_noalloc: no output buffer");
+  if (fast)
+    hipLaunchKernelGGL((k_scan<1, true>), dim3((unsigned)nblocks), dim3(SC_BLOCK), 0, c->stream, ct, dp,
+                       nprog, drg, nrg, seg_rows, tps, dmodes, counts, offs, d_row_ids);
+  else
+    hipLaunchKernelGGL((k_scan<1, false>), dim3((unsigned)nblocks), dim3(SC_BLOCK), 0, c->stream, ct, dp,
+                       nprog, drg, 0, seg_rows, tps, dmodes, counts, offs, d_row_ids);
   ALN_TRY(check_launch("scan_write"));
   ALN_HIP(hipStreamSynchronize(c->stream));
   return ALN_OK;

@@ -291,22 +291,35 @@ __global__ void __launch_bounds__(SCAN_BLOCK) k_tile_scan(const T *__restrict__ 
 }
 
 template <typename T>
+static int exclusive_scan_launch(hipStream_t s, const T *d_in, uint64_t n, uint64_t *d_out,
+                                 uint64_t *sums, uint64_t *tot) {
+  const uint64_t nt = scan_tiles(n);
+  hipLaunchKernelGGL(k_tile_sums<T>, dim3((unsigned)nt), dim3(SCAN_BLOCK), 0, s, d_in, n, sums);
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SCAN_BLOCK), 0, s, sums, nt, tot);
+  hipLaunchKernelGGL(k_tile_scan<T>, dim3((unsigned)nt), dim3(SCAN_BLOCK), 0, s, d_in, n, sums, d_out);
+  return check_launch("exclusive_scan");
+}
+
+uint64_t scan_tiles(uint64_t n) {
+  const uint64_t nt = (n + SCAN_TILE - 1) / SCAN_TILE;
+  return nt == 0 ? 1 : nt;
+}
+
+int exclusive_scan_u32_buf(hipStream_t s, const uint32_t *d_in, uint64_t n, uint64_t *d_out,
+                           uint64_t *d_sums, uint64_t *d_total) {
+  return exclusive_scan_launch<uint32_t>(s, d_in, n, d_out, d_sums, d_total);
+}
+
+template <typename T>
 static int exclusive_scan_impl(aln_ctx *c, const T *d_in, uint64_t n, uint64_t *d_out,
                                uint64_t *h_total) {
-  uint64_t nt = (n + SCAN_TILE - 1) / SCAN_TILE;
-  if (nt == 0) nt = 1;
   uint64_t *sums, *tot;
-  ALN_TRY(ws(c, nt, &sums));
+  ALN_TRY(ws(c, scan_tiles(n), &sums));
   ALN_TRY(ws(c, 1, &tot));
   {
     ProfScope p(c, "scan");
-    hipLaunchKernelGGL(k_tile_sums<T>, dim3((unsigned)nt), dim3(SCAN_BLOCK), 0, c->stream, d_in,
-                       n, sums);
-    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SCAN_BLOCK), 0, c->stream, sums, nt, tot);
-    hipLaunchKernelGGL(k_tile_scan<T>, dim3((unsigned)nt), dim3(SCAN_BLOCK), 0, c->stream,
-                       d_in, n, sums, d_out);
+    ALN_TRY(exclusive_scan_launch<T>(c->stream, d_in, n, d_out, sums, tot));
   }
-  ALN_TRY(check_launch("exclusive_scan"));
   if (h_total) ALN_TRY(copy_to_host(c, h_total, tot, 8));
   return ALN_OK;
 }
@@ -505,6 +518,9 @@ int aln_create(int device, void *stream, aln_ctx **out) {
   aln_ctx *c = new aln_ctx();
   c->device = device;
   c->stream = (hipStream_t)stream;
+  // the allocation-free scan's scratch address on this device (current here); a failure
+  // is left for that entry point's first call, which looks it up again and reports it
+  if (aln::scan_scratch_attach(device) != ALN_OK) (void)hipGetLastError();
   *out = c;
   return ALN_OK;
 }

@@ -176,6 +176,21 @@ int aln_scan_filter(aln_ctx *ctx, const aln_col *cols, int ncols, uint32_t seg_r
                     const aln_insn *prog, int nprog, const char *seg_modes, uint32_t *d_row_ids,
                     uint64_t cap, uint64_t *h_count);
 
+/* aln_scan_filter without any allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (work under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments, results and errors as
+ * aln_scan_filter, but its temporaries live in a fixed device array of the library (about
+ * 3.3 MB per GPU, held until the process ends) instead of the context's workspace, and it
+ * calls no hipMalloc, hipFree or other allocating routine.  Limit: the scan covers at most
+ * ALN_SCAN_NOALLOC_MAX_TILES tiles of 4096 rows (nsegs * ceil(seg_rows / 4096) tiles, e.g.
+ * 1.07e9 rows in full segments); a larger table returns ALN_E_ARG.  Every call on one GPU
+ * shares that GPU's array: calls that overlap on one GPU (several host threads or streams)
+ * race and must be serialised by the caller. */
+#define ALN_SCAN_NOALLOC_MAX_TILES 262144
+int aln_scan_filter_noalloc(aln_ctx *ctx, const aln_col *cols, int ncols, uint32_t seg_rows,
+                            const aln_insn *prog, int nprog, const char *seg_modes,
+                            uint32_t *d_row_ids, uint64_t cap, uint64_t *h_count);
+
 /* Gather decoded values of rows d_rows[0..n) (d_rows NULL = rows 0..n-1). */
 int aln_gather(aln_ctx *ctx, const aln_col *col, uint32_t seg_rows, const uint32_t *d_rows,
                uint64_t n, int64_t *d_out);
