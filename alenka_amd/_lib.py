@@ -134,6 +134,7 @@ SIGNATURES = {
                                   C.c_char_p, P, U64, PU64]),
     "aln_scan_filter_noalloc": (C.c_int, [P, C.POINTER(Col), C.c_int, U32, C.POINTER(Insn), C.c_int,
                                           C.c_char_p, P, U64, PU64]),
+    "aln_row_segments": (C.c_int, [P, P, U64, P, P, U32, U32, P]),
     "aln_gather": (C.c_int, [P, C.POINTER(Col), U32, P, U64, P]),
     "aln_eval": (C.c_int, [P, C.POINTER(Col), C.c_int, U32, C.POINTER(Insn), C.c_int, P, U64, P]),
     "aln_apply_perm": (C.c_int, [P, P, P, U64, P]),

@@ -404,6 +404,29 @@ static int check_prog_cols(const aln_insn *prog, int nprog, int ncols) {
   return ALN_OK;
 }
 
+// Segment of the source row behind each index: r = rows ? rows[idx[i]] : idx[i]; the last
+// segment start <= r when starts are given (uneven file segments), else r / seg_rows.
+__global__ void __launch_bounds__(256) k_row_segments(const uint32_t *__restrict__ idx, uint64_t n,
+                                                      const uint32_t *__restrict__ rows,
+                                                      const uint64_t *__restrict__ starts, uint32_t nstarts,
+                                                      uint32_t seg_rows, int64_t *__restrict__ out) {
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
+    const uint64_t r = rows ? rows[idx[i]] : idx[i];
+    int64_t seg;
+    if (starts) {  // upper bound of r in starts[0, nstarts), minus one
+      uint32_t lo = 0, hi = nstarts;
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (starts[mid] <= r) lo = mid + 1; else hi = mid;
+      }
+      seg = (int64_t)lo - 1;
+    } else {
+      seg = (int64_t)(r / seg_rows);
+    }
+    out[i] = seg;
+  }
+}
+
 // ---- fixed scratch of the allocation-free scan -----------------------------------------------
 // aln_scan_filter_noalloc keeps its temporaries in this array instead of the context's
 // workspace: the column table, program and intervals it uploads, the per-segment zone-map
@@ -718,6 +741,16 @@ int aln_scan_filter(aln_ctx *c, const aln_col *cols, int ncols, uint32_t seg_row
   ALN_TRY(check_launch("scan_write"));
   ALN_HIP(hipStreamSynchronize(c->stream));
   return ALN_OK;
+}
+
+int aln_row_segments(aln_ctx *c, const uint32_t *d_idx, uint64_t n, const uint32_t *d_rows,
+                     const uint64_t *d_starts, uint32_t nstarts, uint32_t seg_rows, int64_t *d_out) {
+  ALN_REQUIRE(c && (n == 0 || (d_idx && d_out)), "aln_row_segments: bad arguments");
+  ALN_REQUIRE(d_starts ? nstarts > 0 : seg_rows > 0, "aln_row_segments: no segmentation");
+  if (n == 0) return ALN_OK;
+  hipLaunchKernelGGL(k_row_segments, dim3(grid_for(n, 256, 65536)), dim3(256), 0, c->stream, d_idx, n, d_rows,
+                     d_starts, nstarts, seg_rows, d_out);
+  return check_launch("row_segments");
 }
 
 int aln_scan_filter_noalloc(aln_ctx *c, const aln_col *cols, int ncols, uint32_t seg_rows,

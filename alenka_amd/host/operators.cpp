@@ -85,12 +85,60 @@ struct Set {
   // decide the group-by merge order and the join's pair order
   std::vector<uint64_t> seg_starts;
   std::vector<std::string> sorted_fields, presorted_fields;  // the .sort / .presort lists
+  // a statement whose work waits for a consumer (JOIN, GROUP BY over a join, ORDER BY of that
+  // GROUP BY); until it runs, the set has no columns
+  std::shared_ptr<struct Deferred> deferred;
   uint32_t nsegs() const { return (uint32_t)((nrows + seg_rows - 1) / seg_rows); }
   uint32_t file_segs() const { return seg_starts.empty() ? nsegs() : (uint32_t)(seg_starts.size() - 1); }
   int64_t seg_of(uint64_t row) const {
     if (seg_starts.empty()) return (int64_t)(row / seg_rows);
     return (int64_t)(std::upper_bound(seg_starts.begin(), seg_starts.end(), row) - seg_starts.begin()) - 1;
   }
+};
+
+// ---- deferred statements ---------------------------------------------------------------------
+// The reference defers a FILTER until a consumer reads it (emit_filter, operators.cu:2306-2408).
+// A JOIN of three tables in the star shape, a GROUP BY over such a join and an ORDER BY of that
+// GROUP BY are deferred here the same way, so that a STORE/DISPLAY ... LIMIT k of the ORDER BY
+// can run the whole chain as one fused pipeline (aln_star_agg); every other consumer runs the
+// statements one by one, as the reference does, when it first reads the set.
+struct StarTable {
+  std::shared_ptr<Set> base;      // the stored table the scan reads
+  std::vector<std::string> cols;  // its column slots
+  std::vector<aln_range> ranges;  // the table's filter as ANDed closed ranges
+  int slot(const std::string &n) {
+    auto it = std::find(cols.begin(), cols.end(), n);
+    if (it != cols.end()) return (int)(it - cols.begin());
+    cols.push_back(n);
+    return (int)cols.size() - 1;
+  }
+};
+struct StarOut {  // one output column of the GROUP BY, in select-list order
+  std::string alias;
+  int field = 0;  // 0 key, 1 g1, 2 g2, 3 sum, 4 count (aln_star_row order)
+  int decimals = 0, type = 0;
+  std::shared_ptr<std::vector<std::string>> dict;
+  uint32_t width = 0;
+};
+struct StarPlan {
+  StarTable dim, mid, fact;
+  int d_key = 0, m_key = 0, m_fk = 0, m_g1 = 0, m_g2 = 0, f_fk = 0, f_x = 0, f_y = 0;
+  int64_t a0 = 0, a1 = 0, b0 = 0, b1 = 0;
+  std::vector<StarOut> out;
+};
+struct Deferred {
+  enum { JOIN = 1, STAR_GROUP = 2, ORDER = 3 };
+  int kind = 0;
+  std::vector<Tok> tok;            // the statement's queue
+  std::vector<std::string> join;   // JOIN: op_join, innermost first
+  std::vector<char> join_kind;
+  std::string src;                 // JOIN: the left variable (j1); else the FROM variable
+  int grp_cnt = 0;                 // STAR_GROUP
+  // the variables the statement reads, as they were when it was issued (a later statement
+  // may assign their names again before this one runs)
+  std::vector<std::pair<std::string, std::shared_ptr<Set>>> pins;
+  std::shared_ptr<StarPlan> star;  // STAR_GROUP, and an ORDER BY of one
+  std::vector<std::pair<std::string, int>> order_keys;  // ORDER: (column, desc)
 };
 
 // ---- session state (the reference's globals, cm.h:66-117, operators.cu:20-50) -------------
@@ -107,7 +155,19 @@ uint32_t g_hash_seed = 100;
 int g_status = ALN_OK;
 char g_err[512] = "";
 
+// When a deferred statement fails inside the statement that consumes it, its error is the
+// consumer's error (not the "couldn't find" that follows from it).
+int g_deferred_rc = ALN_OK;
+char g_deferred_err[512] = "";
+// how the last deferred chain ran (alenka_last_path)
+std::string g_path = "";
+
 int fail(int code, const char *fmt, ...) {
+  if (g_deferred_rc != ALN_OK) {
+    snprintf(g_err, sizeof(g_err), "%s", g_deferred_err);
+    g_status = g_deferred_rc;
+    return g_deferred_rc;
+  }
   va_list ap;
   va_start(ap, fmt);
   vsnprintf(g_err, sizeof(g_err), fmt, ap);
@@ -145,12 +205,24 @@ int load_binary(const std::string &name, const std::string &file);
 
 // A variable by name; a table that is not a variable yet but is in data_dict is read from its
 // segment files first, as load_vars (operators.cu:2708-2747) loads every table a script uses.
-std::shared_ptr<Set> find_var(const std::string &n) {
+std::shared_ptr<Set> peek_var(const std::string &n) {
   auto it = varNames.find(n);
   if (it != varNames.end()) return it->second;
   if (!n.empty() && g_data_dict.count(n) && ensure_ctx() == ALN_OK && load_binary(n, n) == ALN_OK)
     return varNames[n];
   return nullptr;
+}
+int run_deferred(const std::shared_ptr<Set> &s);
+bool defer_join(const char *s, const char *j1);
+bool defer_star_group(const char *s, const char *f, int grp_cnt, const std::shared_ptr<Set> &src);
+bool defer_order(const char *s, const char *f, const std::shared_ptr<Set> &src);
+int store_star_topk(const std::shared_ptr<Set> &v, FILE *fp, const char *sep, int limit, bool *done);
+// A variable by name with its deferred statement (if any) run first: every consumer but the
+// fused pipeline's reads sets this way.
+std::shared_ptr<Set> find_var(const std::string &n) {
+  auto s = peek_var(n);
+  if (s && s->deferred && run_deferred(s) != ALN_OK) return nullptr;
+  return s;
 }
 
 // ---- column construction ---------------------------------------------------------------
@@ -206,7 +278,7 @@ struct View {
   uint64_t nrows = 0;
   uint32_t seg_rows = 1;
   uint32_t cycles = 1;  // the reference's segment loop count (group-by merge order)
-  DBufP rowids;         // int64: source row id of each row (for join ordering)
+  DBufP rowids;         // uint32: source row id of each row, NULL = row i (join ordering)
 };
 
 // Bit sets of the dictionary codes a LIKE pattern matches, cached per dictionary and
@@ -398,17 +470,8 @@ int make_view(const std::string &name, const std::set<std::string> &need, View *
     c.width = sc.width;
     v->cols[n] = c;
   }
-  // source row ids as int64 (join output order needs the left segment)
-  v->rowids = dalloc(8 * (k ? k : 1));
-  if (!v->rowids) return fail(ALN_E_NOMEM, "device allocation failed");
-  if (k) {
-    std::vector<uint32_t> h(k);
-    if (hipMemcpy(h.data(), a->sel->p, 4 * k, hipMemcpyDeviceToHost) != hipSuccess)
-      return fail(ALN_E_HIP, "copy of row ids failed");
-    std::vector<int64_t> h64(h.begin(), h.end());
-    if (hipMemcpy(v->rowids->p, h64.data(), 8 * k, hipMemcpyHostToDevice) != hipSuccess)
-      return fail(ALN_E_HIP, "copy of row ids failed");
-  }
+  // the source row of each row (join output order needs the left segment)
+  v->rowids = a->sel;
   return ALN_OK;
 }
 
@@ -509,6 +572,7 @@ std::set<std::string> names_in(const std::vector<Tok> &t) {
 }
 
 void finish_statement() {
+  g_deferred_rc = ALN_OK;
   g_join_bad = false;
   g_tok.clear();
   g_join.clear();
@@ -928,10 +992,13 @@ static int do_select(const char *s, const char *f, int grp_cnt) {
 
 void emit_select(const char *s, const char *f, int grp_cnt) {
   g_status = ALN_OK;
-  if (!find_var(f ? f : ""))
+  auto src = peek_var(f ? f : "");
+  if (!src) {
     fail(ALN_E_ARG, "Couldn't find(1) %s", f ? f : "");
-  else
-    do_select(s, f, grp_cnt);
+  } else if (!(grp_cnt > 0 && src->deferred && src->deferred->kind == Deferred::JOIN &&
+               defer_star_group(s, f, grp_cnt, src))) {
+    if (find_var(f)) do_select(s, f, grp_cnt);
+  }
   finish_statement();
 }
 
@@ -1077,25 +1144,29 @@ static int join_pair(const std::string &name, const std::string &left, const std
   uint64_t n = np;
   std::vector<uint32_t> perm;
   if (n > 1) {
-    // left source row of each pair, and its segment
-    std::vector<uint32_t> hp(n);
-    if (hipMemcpy(hp.data(), pi->p, 4 * n, hipMemcpyDeviceToHost) != hipSuccess)
-      return fail(ALN_E_HIP, "device to host copy failed");
-    std::vector<int64_t> lrow(lv.nrows);
-    if (lv.rowids) {
-      if (lv.nrows && hipMemcpy(lrow.data(), lv.rowids->p, 8 * lv.nrows, hipMemcpyDeviceToHost) != hipSuccess)
-        return fail(ALN_E_HIP, "device to host copy failed");
-    } else {
-      for (uint64_t i = 0; i < lv.nrows; ++i) lrow[i] = (int64_t)i;
-    }
+    // the left source segment of each pair, on the device
     auto lsrc = find_var(left);
     auto lbase = lsrc->filtered ? find_var(lsrc->source) : lsrc;
-    std::vector<int64_t> seg(n, 0);
-    if (!lbase->not_compressed)
-      for (uint64_t i = 0; i < n; ++i) seg[i] = lbase->seg_of((uint64_t)lrow[hp[i]]);
     DBufP dseg = dalloc(8 * n);
-    if (hipMemcpy(dseg->p, seg.data(), 8 * n, hipMemcpyHostToDevice) != hipSuccess)
-      return fail(ALN_E_HIP, "host to device copy failed");
+    if (!dseg) return fail(ALN_E_NOMEM, "device allocation failed");
+    if (lbase->not_compressed) {  // an operator result is one segment
+      ALN(aln_row_segments(g_ctx, (const uint32_t *)pi->p, n, nullptr, nullptr, 0, 0xFFFFFFFFu,
+                           (int64_t *)dseg->p),
+          "join order");
+    } else {
+      DBufP starts;
+      if (!lbase->seg_starts.empty()) {
+        starts = dalloc(8 * lbase->seg_starts.size());
+        if (!starts) return fail(ALN_E_NOMEM, "device allocation failed");
+        if (hipMemcpy(starts->p, lbase->seg_starts.data(), 8 * lbase->seg_starts.size(), hipMemcpyHostToDevice) !=
+            hipSuccess)
+          return fail(ALN_E_HIP, "host to device copy failed");
+      }
+      ALN(aln_row_segments(g_ctx, (const uint32_t *)pi->p, n, lv.rowids ? (const uint32_t *)lv.rowids->p : nullptr,
+                           starts ? (const uint64_t *)starts->p : nullptr, (uint32_t)lbase->seg_starts.size(),
+                           lbase->seg_rows, (int64_t *)dseg->p),
+          "join order");
+    }
     DBufP key;
     TRY(plain_copy(lc[eqs[0].first], n, (uint32_t)n, &key));
     const int64_t *ks[2] = {(const int64_t *)dseg->p, (const int64_t *)key->p};
@@ -1352,9 +1423,9 @@ void emit_join(const char *s, const char *j1, int grp, int start_seg, int end_se
   if (g_join_bad) {
     fail(ALN_E_UNSUPPORTED, "Join : only inner, semi and anti joins are supported");
   } else {
-    if (!find_var(j1 ? j1 : ""))
+    if (!peek_var(j1 ? j1 : ""))
       fail(ALN_E_ARG, "Join : couldn't find variable %s", j1 ? j1 : "");
-    else
+    else if (!defer_join(s, j1) && find_var(j1))
       do_join(s, j1);
   }
   finish_statement();
@@ -1449,26 +1520,21 @@ void emit_order(const char *s, const char *f, int e, int ll) {
   (void)e;
   (void)ll;
   g_status = ALN_OK;
-  if (!find_var(f ? f : ""))
+  auto src = peek_var(f ? f : "");
+  if (!src)
     fail(ALN_E_ARG, "Order : couldn't find variable %s", f ? f : "");
-  else
+  else if (!(src->deferred && src->deferred->kind == Deferred::STAR_GROUP && defer_order(s, f, src)) &&
+           find_var(f))
     do_order(s, f);
   finish_statement();
 }
 
 // ---- emit_store / emit_display: CudaSet::Store text output (cm.cu:1479-1762) ----------------
-static int store_text(const char *var, FILE *fp, const char *sep, int limit) {
-  auto names = all_cols(var);
-  View v;
-  TRY(make_view(var, std::set<std::string>(names.begin(), names.end()), &v));
-  uint64_t m = (limit != 0 && (uint64_t)limit < v.nrows) ? (uint64_t)limit : v.nrows;
-  std::vector<std::vector<int64_t>> data(names.size());
-  for (size_t j = 0; j < names.size(); ++j) {
-    TRY(fetch_col(v.cols[names[j]], v.nrows, v.seg_rows, &data[j]));
-  }
+static void print_rows(FILE *fp, const char *sep, const std::vector<const Col *> &cols,
+                       const std::vector<std::vector<int64_t>> &data, uint64_t m) {
   for (uint64_t i = 0; i < m; ++i) {
-    for (size_t j = 0; j < names.size(); ++j) {
-      const Col &c = v.cols[names[j]];
+    for (size_t j = 0; j < cols.size(); ++j) {
+      const Col &c = *cols[j];
       int64_t x = data[j][i];
       if (c.dict) {
         const std::string &sv = (*c.dict)[(size_t)x];
@@ -1485,20 +1551,38 @@ static int store_text(const char *var, FILE *fp, const char *sep, int limit) {
     }
     if (i != m - 1) fputs("\n", fp);
   }
+}
+
+static int store_text(const char *var, FILE *fp, const char *sep, int limit) {
+  auto names = all_cols(var);
+  View v;
+  TRY(make_view(var, std::set<std::string>(names.begin(), names.end()), &v));
+  uint64_t m = (limit != 0 && (uint64_t)limit < v.nrows) ? (uint64_t)limit : v.nrows;
+  std::vector<std::vector<int64_t>> data(names.size());
+  std::vector<const Col *> cols;
+  for (size_t j = 0; j < names.size(); ++j) {  // the printed rows only
+    TRY(fetch_col(v.cols[names[j]], m, v.seg_rows, &data[j]));
+    cols.push_back(&v.cols[names[j]]);
+  }
+  print_rows(fp, sep, cols, data, m);
   return ALN_OK;
 }
 
 void emit_store(const char *s, const char *f, const char *sep) {
   g_status = ALN_OK;
   int limit = g_has_limit ? g_limit : 0;
-  if (!find_var(s ? s : "")) {
+  auto v = peek_var(s ? s : "");
+  if (!v) {
     fail(ALN_E_ARG, "Store : couldn't find variable %s", s ? s : "");
   } else {
     FILE *fp = fopen(f, "w");
     if (!fp) {
       fail(ALN_E_ARG, "Could not open file %s", f);
     } else {
-      store_text(s, fp, sep, limit);
+      bool done = false;
+      if (store_star_topk(v, fp, sep, limit, &done) == ALN_OK && !done) {
+        if (find_var(s)) store_text(s, fp, sep, limit);
+      }
       fclose(fp);
     }
   }
@@ -1508,15 +1592,695 @@ void emit_store(const char *s, const char *f, const char *sep) {
 void emit_display(const char *s, const char *sep) {
   g_status = ALN_OK;
   int limit = g_has_limit ? g_limit : 0;
-  if (!find_var(s ? s : ""))
+  auto v = peek_var(s ? s : "");
+  if (!v) {
     fail(ALN_E_ARG, "Display : couldn't find variable %s", s ? s : "");
-  else
-    store_text(s, stdout, sep, limit);
+  } else {
+    bool done = false;
+    if (store_star_topk(v, stdout, sep, limit, &done) == ALN_OK && !done) {
+      if (find_var(s)) store_text(s, stdout, sep, limit);
+    }
+  }
   fflush(stdout);
   finish_statement();
 }
 
+const char *alenka_last_path(void) { return g_path.c_str(); }
+
 }  // extern "C"
+
+// ---- deferred statements: the star pipeline behind emit_join / emit_select / emit_order ------
+namespace {
+
+uint64_t g_tmp_seq = 0;
+
+std::shared_ptr<Set> pinned(const Deferred &d, const std::string &n) {
+  for (auto &p : d.pins)
+    if (p.first == n) return p.second;
+  return nullptr;
+}
+
+void pin(Deferred *d, const std::string &n, const std::shared_ptr<Set> &v) {
+  if (!pinned(*d, n)) d->pins.push_back({n, v});
+}
+
+// The stored table a star scan reads for variable v: v itself, or the source of a filter
+// (pinned under its name); NULL when v is the result of another statement.
+std::shared_ptr<Set> star_base(const Deferred &d, const std::shared_ptr<Set> &v) {
+  if (!v || v->deferred) return nullptr;
+  if (!v->filtered) return v;
+  auto b = pinned(d, v->source);
+  if (!b || b->deferred || b->filtered) return nullptr;
+  return b;
+}
+
+// A filter as ANDed closed ranges on the table's columns: the compiled program must be a
+// conjunction of `column OP constant` terms on unscaled columns (the shapes the scan's
+// interval path takes).  False for any other predicate.
+bool star_ranges(const Set &var, StarTable *t) {
+  if (!var.filtered) return true;
+  std::map<std::string, int> slot;
+  std::vector<std::string> names;
+  for (auto &k : var.pred)
+    if (k.type == "NAME" && !slot.count(k.value)) {
+      if (!t->base->cols.count(k.value)) return false;
+      slot[k.value] = (int)names.size();
+      names.push_back(k.value);
+    }
+  if (names.empty() || names.size() > ALN_MAX_COLS) return false;
+  std::vector<aln_rpn> rpn;
+  if (resolve_string_codes(*t->base, var.pred, &rpn, slot) != ALN_OK) return false;
+  std::vector<int32_t> dec;
+  for (auto &n : names) dec.push_back(t->base->cols[n].decimals);
+  aln_insn prog[ALN_MAX_INSN];
+  int np = 0;
+  if (aln_compile_rpn(rpn.data(), (int)rpn.size(), dec.data(), (int)dec.size(), 0, prog, &np, nullptr) != ALN_OK)
+    return false;
+  std::vector<int64_t> lo(names.size(), INT64_MIN), hi(names.size(), INT64_MAX);
+  std::vector<bool> used(names.size(), false);
+  int pending = 0;
+  for (int i = 0; i < np;) {
+    if (prog[i].op == ALN_I_AND) {
+      if (pending < 2) return false;
+      --pending;
+      ++i;
+      continue;
+    }
+    if (i + 2 >= np || prog[i].op != ALN_I_COL || prog[i].k != 1 || prog[i + 1].op != ALN_I_CONST ||
+        prog[i + 2].op != ALN_I_CMP)
+      return false;
+    const int c = prog[i].a;
+    const int64_t v = prog[i + 1].k;
+    int64_t l = INT64_MIN, h = INT64_MAX;
+    switch (prog[i + 2].a) {
+      case 1: if (v == INT64_MIN) return false; h = v - 1; break;
+      case 2: if (v == INT64_MAX) return false; l = v + 1; break;
+      case 4: l = h = v; break;
+      case 5: h = v; break;
+      case 6: l = v; break;
+      default: return false;
+    }
+    if (c < 0 || c >= (int)names.size()) return false;
+    used[c] = true;
+    lo[c] = std::max(lo[c], l);
+    hi[c] = std::min(hi[c], h);
+    ++pending;
+    i += 3;
+  }
+  if (pending != 1) return false;
+  for (size_t c = 0; c < names.size(); ++c) {
+    if (!used[c]) continue;
+    if (lo[c] > hi[c]) return false;  // an empty filter: the statements run one by one
+    aln_range r;
+    memset(&r, 0, sizeof(r));
+    r.col = t->slot(names[c]);
+    r.lo = lo[c];
+    r.hi = hi[c];
+    t->ranges.push_back(r);
+  }
+  return t->ranges.size() <= ALN_MAX_RANGES;
+}
+
+// SUM's expression as (a0 + a1*x) * (b0 + b1*y): each value on the evaluation stack is an
+// affine form in at most one column, or (only as the final value) the product of two.
+// Integer arithmetic wraps (mod 2^64) here as in the kernels, so the expanded form gives the
+// interpreted program's exact results.
+bool sum_form(const aln_insn *prog, int np, int *cx, int64_t *a0, int64_t *a1, int *cy, int64_t *b0,
+              int64_t *b1) {
+  struct Aff {
+    int col = -1;
+    uint64_t a = 0, c = 0;
+  };
+  struct Val {
+    bool prod = false;
+    Aff l, r;
+  };
+  std::vector<Val> st;
+  for (int i = 0; i < np; ++i) {
+    const aln_insn &in = prog[i];
+    if (in.op == ALN_I_COL || in.op == ALN_I_CONST) {
+      Val v;
+      if (in.op == ALN_I_COL) {
+        v.l.col = in.a;
+        v.l.a = (uint64_t)in.k;
+      } else {
+        v.l.c = (uint64_t)in.k;
+      }
+      st.push_back(v);
+      continue;
+    }
+    if (st.size() < 2) return false;
+    Val r = st.back();
+    st.pop_back();
+    Val l = st.back();
+    st.pop_back();
+    if (l.prod || r.prod) return false;
+    Val o;
+    if (in.op == ALN_I_ADD || in.op == ALN_I_SUB) {
+      if (l.l.col >= 0 && r.l.col >= 0 && l.l.col != r.l.col) return false;
+      o.l.col = l.l.col >= 0 ? l.l.col : r.l.col;
+      o.l.a = in.op == ALN_I_ADD ? l.l.a + r.l.a : l.l.a - r.l.a;
+      o.l.c = in.op == ALN_I_ADD ? l.l.c + r.l.c : l.l.c - r.l.c;
+    } else if (in.op == ALN_I_MUL) {
+      if (l.l.col < 0 || r.l.col < 0) {
+        const Aff &k = l.l.col < 0 ? l.l : r.l, &x = l.l.col < 0 ? r.l : l.l;
+        o.l.col = x.col;
+        o.l.a = x.a * k.c;
+        o.l.c = x.c * k.c;
+      } else {
+        o.prod = true;
+        o.l = l.l;
+        o.r = r.l;
+      }
+    } else {
+      return false;
+    }
+    st.push_back(o);
+  }
+  if (st.size() != 1) return false;
+  const Val &v = st[0];
+  Aff x = v.l, y;
+  if (v.prod) {
+    y = v.r;
+  } else {
+    y.c = 1;
+  }
+  *cx = x.col;
+  *a0 = (int64_t)x.c;
+  *a1 = (int64_t)x.a;
+  *cy = y.col;
+  *b0 = (int64_t)y.c;
+  *b1 = (int64_t)y.a;
+  return true;
+}
+
+// Plans SELECT ... GROUP BY over a deferred join as one aln_star_agg call: the join must be
+// fact JOIN mid ON f_fk = m_key JOIN dim ON m_fk = d_key (inner, one column pair each, the
+// three inputs stored tables or filters of them with interval predicates), the GROUP BY's
+// first column the mid table's join key and the others mid columns, and the select list
+// the group columns, COUNTs and at most one SUM of an expression of the form
+// (a0 + a1*x) * (b0 + b1*y) over fact columns.  False (and no error) for any other shape.
+bool plan_star(const Deferred &dj, const std::vector<Tok> &sel_tok, int grp_cnt, StarPlan *p) {
+  std::vector<SelItem> jitems;
+  std::vector<Tok> jrest;
+  parse_select(dj.tok, &jitems, &jrest);
+  std::vector<Cond> conds;
+  if (parse_conditions(jrest, &conds) != ALN_OK) return false;
+  if (dj.join.size() != 2 || conds.size() != 2) return false;
+  for (char k : dj.join_kind)
+    if (k != 'I') return false;
+  for (auto &c : conds)
+    if (c.eqs.size() != 1) return false;
+  auto vd = pinned(dj, dj.join[0]), vm = pinned(dj, dj.join[1]), vf = pinned(dj, dj.src);
+  auto bd = star_base(dj, vd), bm = star_base(dj, vm), bf = star_base(dj, vf);
+  if (!bd || !bm || !bf) return false;
+  auto has = [](const std::shared_ptr<Set> &b, const std::string &c) { return b->cols.count(c) > 0; };
+  // the first join: mid (left) with dim (right) on the last condition (operators.cu:832-860)
+  std::string mfk, dkey;
+  {
+    const auto &e = conds[1].eqs[0];
+    if (has(bm, e.first) && has(bd, e.second)) {
+      mfk = e.first;
+      dkey = e.second;
+    } else if (has(bm, e.second) && has(bd, e.first)) {
+      mfk = e.second;
+      dkey = e.first;
+    } else {
+      return false;
+    }
+  }
+  std::set<std::string> keep;
+  std::map<std::string, std::string> alias;  // join output name -> source column
+  for (auto &it : jitems) {
+    if (!it.agg.empty() || it.expr.size() != 1 || it.expr[0].type != "NAME") return false;
+    keep.insert(it.expr[0].value);
+    alias[it.alias] = it.expr[0].value;
+  }
+  for (auto &c : conds)
+    for (auto &e : c.eqs) {
+      keep.insert(e.first);
+      keep.insert(e.second);
+    }
+  auto j1has = [&](const std::string &c) { return keep.count(c) && (has(bm, c) || has(bd, c)); };
+  // the second join: fact (left) with the first join's result on the first condition
+  std::string ffk, mkey;
+  {
+    const auto &e = conds[0].eqs[0];
+    if (has(bf, e.first) && j1has(e.second)) {
+      ffk = e.first;
+      mkey = e.second;
+    } else if (has(bf, e.second) && j1has(e.first)) {
+      ffk = e.second;
+      mkey = e.first;
+    } else {
+      return false;
+    }
+    if (!has(bm, mkey)) return false;  // the fact table joins the dimension directly
+  }
+  for (const std::string *k : {&mfk, &dkey, &ffk, &mkey}) {
+    const auto &b = k == &dkey ? bd : k == &ffk ? bf : bm;
+    if (b->cols.at(*k).type == 2) return false;  // string keys: the join's own error
+  }
+  // a join output column comes from the fact table, else the mid, else the dimension table
+  auto provider = [&](const std::string &a, std::string *col) -> int {
+    auto it = alias.find(a);
+    if (it == alias.end()) return -1;
+    *col = it->second;
+    return has(bf, *col) ? 2 : has(bm, *col) ? 1 : has(bd, *col) ? 0 : -1;
+  };
+  std::vector<SelItem> items;
+  std::vector<Tok> rest;
+  parse_select(sel_tok, &items, &rest);
+  if (items.empty() || grp_cnt < 1 || grp_cnt > 3) return false;
+  std::vector<std::string> groups;
+  for (auto &k : rest)
+    if (k.type == "NAME") groups.push_back(k.value);
+  if ((int)groups.size() < grp_cnt) return false;
+  groups.erase(groups.begin(), groups.end() - grp_cnt);
+  std::vector<std::string> gcol(grp_cnt);
+  for (int i = 0; i < grp_cnt; ++i)
+    if (provider(groups[i], &gcol[i]) != 1) return false;
+  if (gcol[0] != mkey) return false;
+  p->dim.base = bd;
+  p->mid.base = bm;
+  p->fact.base = bf;
+  p->d_key = p->dim.slot(dkey);
+  p->m_key = p->mid.slot(mkey);
+  p->m_fk = p->mid.slot(mfk);
+  p->m_g1 = p->mid.slot(grp_cnt > 1 ? gcol[1] : mkey);
+  p->m_g2 = p->mid.slot(grp_cnt > 2 ? gcol[2] : grp_cnt > 1 ? gcol[1] : mkey);
+  p->f_fk = p->fact.slot(ffk);
+  p->f_x = p->f_y = p->f_fk;
+  bool have_sum = false;
+  for (auto &it : items) {
+    StarOut o;
+    o.alias = it.alias;
+    if (it.agg.empty()) {
+      if (it.expr.size() != 1 || it.expr[0].type != "NAME") return false;
+      auto g = std::find(groups.begin(), groups.end(), it.expr[0].value);
+      if (g == groups.end()) return false;
+      o.field = (int)(g - groups.begin());
+      const Col &sc = bm->cols.at(gcol[o.field]);
+      o.decimals = sc.decimals;
+      o.type = sc.type;
+      o.dict = sc.dict;
+      o.width = sc.width;
+    } else if (it.agg == "COUNT") {
+      for (auto &k : it.expr) {
+        std::string c;
+        if (k.type == "NAME" && provider(k.value, &c) < 0) return false;
+      }
+      o.field = 4;
+    } else if (it.agg == "SUM") {
+      if (have_sum) return false;
+      have_sum = true;
+      std::map<std::string, int> slot;
+      std::vector<std::string> names;  // fact columns, in first-use order
+      Set view;
+      for (auto &k : it.expr)
+        if (k.type == "NAME" && !slot.count(k.value)) {
+          std::string c;
+          if (provider(k.value, &c) != 2) return false;
+          slot[k.value] = (int)names.size();
+          names.push_back(c);
+          view.cols[k.value] = bf->cols.at(c);
+        }
+      if (names.empty()) return false;
+      std::vector<aln_rpn> rpn;
+      if (resolve_string_codes(view, it.expr, &rpn, slot) != ALN_OK) return false;
+      std::vector<int32_t> dec;
+      for (auto &n : names) dec.push_back(bf->cols.at(n).decimals);
+      aln_insn prog[ALN_MAX_INSN];
+      int np = 0;
+      int32_t prec = 0;
+      if (aln_compile_rpn(rpn.data(), (int)rpn.size(), dec.data(), (int)dec.size(), 1, prog, &np, &prec) != ALN_OK)
+        return false;
+      int cx = -1, cy = -1;
+      if (!sum_form(prog, np, &cx, &p->a0, &p->a1, &cy, &p->b0, &p->b1)) return false;
+      if (cx >= 0) p->f_x = p->fact.slot(names[cx]);
+      if (cy >= 0) p->f_y = p->fact.slot(names[cy]);
+      o.field = 3;
+      o.decimals = prec;
+    } else {
+      return false;  // AVG, MIN, MAX
+    }
+    p->out.push_back(o);
+  }
+  return star_ranges(*vd, &p->dim) && star_ranges(*vm, &p->mid) && star_ranges(*vf, &p->fact);
+}
+
+// One aln_star_agg call of a plan: the first `limit` rows of ORDER BY sum DESC, g1 ASC, key ASC
+// into *top, or (limit 0) every group in key order, one int64 array per field, into *fields.
+// ALN_E_UNSUPPORTED / ALN_E_ARG (shapes the fused kernels do not take) come back without an
+// error being recorded; the caller then runs the statements one by one.
+int run_star(const StarPlan &p, uint32_t limit, std::vector<aln_star_row> *top, std::vector<DBufP> *fields,
+             uint64_t *ngroups) {
+  auto cols_of = [](const StarTable &t, std::vector<aln_col> *v) {
+    for (auto &n : t.cols) v->push_back(t.base->cols.at(n).desc());
+  };
+  std::vector<aln_col> dc, mc, fc;
+  cols_of(p.dim, &dc);
+  cols_of(p.mid, &mc);
+  cols_of(p.fact, &fc);
+  aln_star_args a;
+  memset(&a, 0, sizeof(a));
+  auto scan = [](const StarTable &t, const std::vector<aln_col> &c) {
+    aln_table_scan s;
+    memset(&s, 0, sizeof(s));
+    s.cols = c.data();
+    s.ncols = (int32_t)c.size();
+    s.seg_rows = t.base->seg_rows;
+    s.seg_modes = nullptr;
+    s.ranges = t.ranges.data();
+    s.nranges = (int32_t)t.ranges.size();
+    return s;
+  };
+  a.dim = scan(p.dim, dc);
+  a.mid = scan(p.mid, mc);
+  a.fact = scan(p.fact, fc);
+  a.d_key = p.d_key;
+  a.m_key = p.m_key;
+  a.m_fk = p.m_fk;
+  a.m_g1 = p.m_g1;
+  a.m_g2 = p.m_g2;
+  a.f_fk = p.f_fk;
+  a.f_x = p.f_x;
+  a.f_y = p.f_y;
+  a.a0 = p.a0;
+  a.a1 = p.a1;
+  a.b0 = p.b0;
+  a.b1 = p.b1;
+  auto soft = [](int rc) { return rc == ALN_E_UNSUPPORTED || rc == ALN_E_ARG; };
+  if (limit) {
+    a.limit = limit;
+    top->assign(limit, aln_star_row());
+    uint32_t nt = 0;
+    int rc = aln_star_agg(g_ctx, &a, top->data(), &nt, nullptr, 0, nullptr);
+    if (soft(rc)) return rc;
+    ALN(rc, "star pipeline");
+    top->resize(nt);
+    return ALN_OK;
+  }
+  a.flags = 1;
+  uint64_t cap = std::max<uint64_t>(1, std::min<uint64_t>(p.fact.base->nrows, 1u << 20));
+  for (;;) {
+    DBufP rows = dalloc(sizeof(aln_star_row) * cap);
+    if (!rows) return fail(ALN_E_NOMEM, "device allocation failed");
+    aln_star_stats st;
+    memset(&st, 0, sizeof(st));
+    uint32_t nt = 0;
+    int rc = aln_star_agg(g_ctx, &a, nullptr, &nt, (aln_star_row *)rows->p, cap, &st);
+    if (rc == ALN_E_CAPACITY && st.groups > cap) {
+      cap = st.groups;
+      continue;
+    }
+    if (soft(rc)) return rc;
+    ALN(rc, "star pipeline");
+    const uint64_t g = st.groups;
+    fields->clear();
+    for (int f = 0; f < 5; ++f) {  // row layout -> one array per field
+      DBufP d = dalloc(8 * (g ? g : 1));
+      if (!d) return fail(ALN_E_NOMEM, "device allocation failed");
+      if (g && hipMemcpy2D(d->p, 8, (const char *)rows->p + 8 * f, sizeof(aln_star_row), 8, g,
+                           hipMemcpyDeviceToDevice) != hipSuccess)
+        return fail(ALN_E_HIP, "device copy failed");
+      fields->push_back(d);
+    }
+    *ngroups = g;
+    return ALN_OK;
+  }
+}
+
+// The GROUP BY's result set from the fused pipeline: every group in key order (the order a
+// one-segment GROUP BY over the join's rows gives, cm.cu:910-956).
+int star_groups(const StarPlan &p, const std::string &name) {
+  std::vector<DBufP> fields;
+  uint64_t g = 0;
+  TRY(run_star(p, 0, nullptr, &fields, &g));
+  auto out = std::make_shared<Set>();
+  out->not_compressed = true;
+  out->nrows = g;
+  out->seg_rows = (uint32_t)(g ? g : 1);
+  for (auto &o : p.out) {
+    Col c;
+    TRY(make_plain(fields[o.field], g, out->seg_rows, o.decimals, &c));
+    c.type = o.type;
+    c.dict = o.dict;
+    c.width = o.width;
+    out->names.push_back(o.alias);
+    out->cols[o.alias] = c;
+  }
+  varNames[name] = out;
+  g_path = "aln_star_agg groups";
+  return ALN_OK;
+}
+
+}  // namespace
+
+namespace {
+
+// Runs a deferred statement in place: the consumer's own statement state is set aside, the
+// variables the statement read are put back under their names while it runs, and its result
+// replaces the set's contents.  On failure the set stays deferred and the error becomes the
+// consumer's.
+int run_deferred(const std::shared_ptr<Set> &s) {
+  std::shared_ptr<Deferred> d = s->deferred;
+  if (!d) return ALN_OK;
+  s->deferred = nullptr;
+  const auto tok = g_tok;
+  const auto join = g_join;
+  const auto jk = g_join_kind;
+  const int lim = g_limit;
+  const bool hl = g_has_limit, jb = g_join_bad;
+  std::vector<std::pair<std::string, std::shared_ptr<Set>>> saved;
+  for (auto &pn : d->pins) {
+    auto it = varNames.find(pn.first);
+    saved.push_back({pn.first, it == varNames.end() ? nullptr : it->second});
+    varNames[pn.first] = pn.second;
+  }
+  const std::string tmp = std::string("\x02") + std::to_string(++g_tmp_seq);
+  g_tok = d->tok;
+  g_join = d->join;
+  g_join_kind = d->join_kind;
+  g_limit = 0;
+  g_has_limit = false;
+  g_join_bad = false;
+  int rc = ALN_OK;
+  if (d->kind == Deferred::JOIN) {
+    rc = do_join(tmp.c_str(), d->src.c_str());
+    if (rc == ALN_OK) g_path = "unfused";
+  } else if (d->kind == Deferred::STAR_GROUP) {
+    rc = star_groups(*d->star, tmp);
+    if (rc == ALN_E_UNSUPPORTED || rc == ALN_E_ARG) {
+      g_tok = d->tok;
+      rc = find_var(d->src) ? do_select(tmp.c_str(), d->src.c_str(), d->grp_cnt) : g_status;
+      if (rc == ALN_OK) g_path = "unfused";
+    }
+  } else {
+    rc = find_var(d->src) ? do_order(tmp.c_str(), d->src.c_str()) : g_status;
+  }
+  for (auto it = saved.rbegin(); it != saved.rend(); ++it) {
+    if (it->second)
+      varNames[it->first] = it->second;
+    else
+      varNames.erase(it->first);
+  }
+  g_tok = tok;
+  g_join = join;
+  g_join_kind = jk;
+  g_limit = lim;
+  g_has_limit = hl;
+  g_join_bad = jb;
+  std::shared_ptr<Set> r;
+  if (rc == ALN_OK) {
+    auto it = varNames.find(tmp);
+    if (it != varNames.end()) r = it->second;
+    if (!r) rc = fail(ALN_E_ARG, "deferred statement left no result");
+  }
+  // its intermediate variables (do_join names them after the statement)
+  for (auto it = varNames.begin(); it != varNames.end();) {
+    if (it->first.compare(0, tmp.size(), tmp) == 0)
+      it = varNames.erase(it);
+    else
+      ++it;
+  }
+  if (rc != ALN_OK) {
+    s->deferred = d;
+    if (g_deferred_rc == ALN_OK) {
+      g_deferred_rc = rc == ALN_OK ? ALN_E_ARG : rc;
+      snprintf(g_deferred_err, sizeof(g_deferred_err), "%s", g_err);
+    }
+    return rc;
+  }
+  *s = *r;
+  return ALN_OK;
+}
+
+bool defer_join(const char *s, const char *j1) {
+  if (g_join.size() != 2 || g_join_bad) return false;
+  for (char k : g_join_kind)
+    if (k != 'I') return false;
+  auto d = std::make_shared<Deferred>();
+  d->kind = Deferred::JOIN;
+  d->tok = g_tok;
+  d->join = g_join;
+  d->join_kind = g_join_kind;
+  d->src = j1;
+  for (const std::string &n : {std::string(j1), g_join[0], g_join[1]}) {
+    auto v = peek_var(n);
+    if (!v || v->deferred) return false;
+    pin(d.get(), n, v);
+    if (v->filtered) {
+      auto b = peek_var(v->source);
+      if (!b || b->deferred || b->filtered) return false;
+      pin(d.get(), v->source, b);
+    }
+  }
+  // the statement must be one the star plan can take apart (its columns and conditions
+  // resolve); anything else runs now, with the reference's errors
+  const int st = g_status;
+  std::string err = g_err;
+  std::vector<SelItem> items;
+  std::vector<Tok> rest;
+  parse_select(g_tok, &items, &rest);
+  std::vector<Cond> conds;
+  const bool ok = parse_conditions(rest, &conds) == ALN_OK && conds.size() == 2 && conds[0].eqs.size() == 1 &&
+                  conds[1].eqs.size() == 1 && !items.empty();
+  g_status = st;
+  snprintf(g_err, sizeof(g_err), "%s", err.c_str());
+  if (!ok) return false;
+  auto bd = star_base(*d, pinned(*d, g_join[0])), bm = star_base(*d, pinned(*d, g_join[1])),
+       bf = star_base(*d, pinned(*d, j1));
+  if (!bd || !bm || !bf) return false;
+  for (auto &it : items) {
+    if (!it.agg.empty() || it.expr.size() != 1 || it.expr[0].type != "NAME") return false;
+    const std::string &c = it.expr[0].value;
+    if (!bd->cols.count(c) && !bm->cols.count(c) && !bf->cols.count(c)) return false;
+  }
+  for (auto &c : conds)
+    for (auto &e : c.eqs)
+      for (const std::string *n : {&e.first, &e.second}) {
+        auto in = [&](const std::shared_ptr<Set> &b) { return b->cols.count(*n) && b->cols.at(*n).type != 2; };
+        if (!in(bd) && !in(bm) && !in(bf)) return false;
+      }
+  auto b = std::make_shared<Set>();
+  b->deferred = d;
+  varNames[s] = b;
+  return true;
+}
+
+bool defer_star_group(const char *s, const char *f, int grp_cnt, const std::shared_ptr<Set> &src) {
+  auto plan = std::make_shared<StarPlan>();
+  const int st = g_status;
+  std::string err = g_err;
+  const bool ok = plan_star(*src->deferred, g_tok, grp_cnt, plan.get());
+  g_status = st;
+  snprintf(g_err, sizeof(g_err), "%s", err.c_str());
+  if (!ok) return false;
+  auto d = std::make_shared<Deferred>();
+  d->kind = Deferred::STAR_GROUP;
+  d->tok = g_tok;
+  d->src = f;
+  d->grp_cnt = grp_cnt;
+  d->star = plan;
+  pin(d.get(), f, src);
+  auto b = std::make_shared<Set>();
+  b->deferred = d;
+  varNames[s] = b;
+  return true;
+}
+
+bool defer_order(const char *s, const char *f, const std::shared_ptr<Set> &src) {
+  auto d = std::make_shared<Deferred>();
+  d->kind = Deferred::ORDER;
+  d->tok = g_tok;
+  d->src = f;
+  d->star = src->deferred->star;
+  for (auto &k : g_tok) {
+    if (k.type == "NAME")
+      d->order_keys.push_back({k.value, 0});
+    else if (k.type == "ORDER")
+      d->order_keys.push_back({k.value, (int)k.num});
+  }
+  if (d->order_keys.empty() || d->order_keys.size() > ALN_MAX_KEYS) return false;
+  for (auto &k : d->order_keys) {
+    bool found = false;
+    for (auto &o : d->star->out) found = found || o.alias == k.first;
+    if (!found) return false;  // the statement runs now, with the reference's error
+  }
+  pin(d.get(), f, src);
+  auto b = std::make_shared<Set>();
+  b->deferred = d;
+  varNames[s] = b;
+  return true;
+}
+
+// STORE / DISPLAY ... LIMIT k of a deferred ORDER BY over a deferred star GROUP BY: when the
+// ORDER BY's keys give the fused top-k's order (SUM DESC, then the first group column after
+// the key ASC, then the key: a one-segment GROUP BY returns groups in key order and the sort
+// is stable, so the key breaks the remaining ties), the first k rows come from one
+// aln_star_agg call.  *done is false when the chain has another shape or the fused kernels
+// refuse it; the caller then runs the statements one by one.
+int store_star_topk(const std::shared_ptr<Set> &v, FILE *fp, const char *sep, int limit, bool *done) {
+  *done = false;
+  if (!v->deferred || v->deferred->kind != Deferred::ORDER || !v->deferred->star) return ALN_OK;
+  if (limit <= 0 || limit > 512) return ALN_OK;
+  const Deferred &d = *v->deferred;
+  const StarPlan &p = *d.star;
+  // the output column each alias names (the last item of that name, as do_select stores it)
+  std::map<std::string, const StarOut *> by_alias;
+  for (auto &o : p.out) by_alias[o.alias] = &o;
+  const bool g1_is_key = p.m_g1 == p.m_key;
+  std::vector<std::pair<int, int>> want = {{3, 1}};
+  if (!g1_is_key) want.push_back({1, 0});
+  want.push_back({0, 0});
+  std::vector<std::pair<int, int>> got;
+  for (auto &k : d.order_keys) {
+    const StarOut *o = by_alias.at(k.first);
+    int field = o->field;
+    if (field == 1 && g1_is_key) field = 0;
+    if (field == 2 && p.m_g2 == p.m_key) field = 0;
+    if (field == 2 && p.m_g2 == p.m_g1) field = g1_is_key ? 0 : 1;
+    if (field == 1 && o->dict) return ALN_OK;  // char keys sort by their strings
+    bool seen = false;
+    for (auto &g : got) seen = seen || g.first == field;
+    if (seen) continue;
+    got.push_back({field, k.second});
+    if (field == 0) break;  // the key is unique: later keys change nothing
+  }
+  if (got.empty() || got.back().first != 0) got.push_back({0, 0});
+  if (got != want) return ALN_OK;
+  std::vector<aln_star_row> top;
+  int rc = run_star(p, (uint32_t)limit, &top, nullptr, nullptr);
+  if (rc == ALN_E_UNSUPPORTED || rc == ALN_E_ARG) return ALN_OK;
+  TRY(rc);
+  // the GROUP BY's columns in select-list order, each alias printing its last item's column
+  std::vector<Col> metas;
+  std::vector<std::vector<int64_t>> data;
+  for (auto &o : p.out) {
+    const StarOut &src = *by_alias.at(o.alias);
+    Col c;
+    c.decimals = src.decimals;
+    c.type = src.type;
+    c.dict = src.dict;
+    c.width = src.width;
+    metas.push_back(c);
+    std::vector<int64_t> col;
+    for (auto &r : top) {
+      const int64_t f[5] = {r.key, r.g1, r.g2, r.sum, r.count};
+      col.push_back(f[src.field]);
+    }
+    data.push_back(col);
+  }
+  std::vector<const Col *> cols;
+  for (auto &c : metas) cols.push_back(&c);
+  print_rows(fp, sep, cols, data, top.size());
+  g_path = "aln_star_agg top-k";
+  *done = true;
+  return ALN_OK;
+}
+
+}  // namespace
 
 // ---- binary tables: the reference's on-disk format ------------------------------------------
 // <f>.<col>.header       u64 rows, u32 segments, u32 most rows in a segment, u32 largest

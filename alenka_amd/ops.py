@@ -24,6 +24,7 @@ _SIG = {
     "alenka_fetch_int": (C.c_int, [C.c_char_p, C.c_char_p, C.c_void_p, C.c_uint64]),
     "alenka_fetch_char": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_uint64, C.c_uint32]),
     "alenka_drop": (C.c_int, [C.c_char_p]),
+    "alenka_last_path": (C.c_char_p, []),
     "emit_name": (None, [C.c_char_p]),
     "emit_number": (None, [C.c_int64]),
     "emit_decimal": (None, [C.c_char_p]),

@@ -226,6 +226,94 @@ def config1_leg(ctx, D, seed, seg_rows, steps, cpu_threads, cpu_runs):
     return out
 
 
+API_Q3 = """ord_f := FILTER orders BY o_orderdate < {date};
+cust_f := FILTER customer BY c_mktsegment == "BUILDING";
+li_f := FILTER lineitem BY shipdate > {date};
+joined := SELECT o_orderkey AS o_orderkey, o_orderdate AS o_orderdate, o_shippriority AS o_shippriority,
+                 price AS price, discount AS discount
+          FROM li_f JOIN ord_f ON orderkey = o_orderkey JOIN cust_f ON o_custkey = c_custkey;
+grouped := SELECT o_orderkey AS k, o_orderdate AS d, o_shippriority AS p,
+                  SUM(price * (1 - discount)) AS rev, COUNT(o_orderkey) AS n
+           FROM joined GROUP BY o_orderkey, o_orderdate, o_shippriority;
+ranked := ORDER grouped BY rev DESC, d ASC;
+STORE ranked INTO '{out}' USING ('|') LIMIT 10;
+"""
+
+
+def api_leg(ctx, D, seed, scale, seg_rows, steps, warmup, want_rows, star_ms):
+    """The same q3 through the reference's operator API: the three tables registered with
+    alenka_table_add_*, and each step one alenkaExecute of the q3 script text (parse, three
+    FILTERs, the JOIN, GROUP BY, ORDER BY, STORE ... LIMIT 10 into a file).  The deferred
+    statements run as one aln_star_agg top-k at the STORE; the file must hold the fused
+    step's rows."""
+    import ctypes as C
+    import tempfile
+
+    import numpy as np
+    import torch
+
+    from alenka_amd.ops import Ops
+
+    ops = Ops(0)
+    t0 = time.time()
+    ndim, norders = int(150000 * scale), int(1500000 * scale)
+
+    def add(table, name, col, dec=0):
+        ops.check(ops.lib.alenka_table_add_int_device(table.encode(), name.encode(), C.c_void_p(col.data_ptr()),
+                                                      dec, 1), name)
+
+    key, seg_idx = D.gen_dim(ctx, seed, ndim)
+    ops.check(ops.lib.alenka_table_create(b"customer", ndim, seg_rows), "customer")
+    add("customer", "c_custkey", key)
+    names = np.array([s.encode().ljust(10, b"\0") for s in SEGMENTS], dtype="S10")
+    chars = names[seg_idx.cpu().numpy()].tobytes()
+    ops.check(ops.lib.alenka_table_add_char(b"customer", b"c_mktsegment", chars, 10), "c_mktsegment")
+    del key, seg_idx, chars
+    ocols = D.gen_orders(ctx, seed, norders, ndim)
+    ops.check(ops.lib.alenka_table_create(b"orders", norders, seg_rows), "orders")
+    for name, col in zip(("o_orderkey", "o_custkey", "o_orderdate", "o_shippriority"), ocols):
+        add("orders", name, col)
+    del ocols, col
+    nfact = D.gen_fact_rows(ctx, seed, norders)
+    ops.check(ops.lib.alenka_table_create(b"lineitem", nfact, seg_rows), "lineitem")
+    for i, (name, dec) in enumerate((("orderkey", 0), ("price", 2), ("discount", 2), ("shipdate", 0))):
+        col = D.gen_fact(ctx, seed, norders, cols=tuple(j == i for j in range(4)))[i]
+        add("lineitem", name, col, dec)
+        del col
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    log(f"api: tables registered through alenka_table_add_* in {time.time() - t0:.1f}s")
+    out = os.path.join(tempfile.gettempdir(), f"alenka_q3_{os.getpid()}.txt")
+    script = API_Q3.format(date=DATE, out=out)
+
+    def step():
+        rc = ops.alenkaExecute(script)
+        if rc != 0:
+            raise RuntimeError(f"alenkaExecute: {ops.lib.alenka_last_error().decode()}")
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t1) * 1000.0 / steps
+    path = ops.lib.alenka_last_path().decode()
+    with open(out) as f:
+        text = f.read()
+    os.remove(out)
+    want = []
+    for k, d, p, rev, n in want_rows:
+        frac = str(rev).rjust(5, "0")
+        want.append(f"{k}|{d}|{p}|{frac[:-4]}.{frac[-4:]}|{n}|")
+    ops.close()
+    torch.cuda.empty_cache()
+    return {"ms": ms, "fact_rows_per_s": nfact / (ms / 1000.0), "vs_fused_step": ms / star_ms, "path": path,
+            "checked": text == "\n".join(want), "steps": steps,
+            "route": "alenkaExecute(q3 script) -> emit_filter x3, emit_join, emit_select, emit_order, emit_store"}
+
+
 def batch_queries(seed, nq, nkeys, price_lo, price_span, ndays):
     """nq filter + GROUP BY queries over the fact table [okey, price, disc, ship, zipf]:
     selectivity log-uniform over [1%, 90%] (a price range, or every 4th query a ship-date
@@ -612,6 +700,7 @@ def main():
                     help="timed CPU runs of the full-size q3 (after the run that checks the GPU's answer)")
     ap.add_argument("--cpu-budget-s", type=float, default=60.0, help="time cap on those runs (at least one)")
     ap.add_argument("--no-configs", action="store_true", help="skip the 1M flat and 6M star configurations")
+    ap.add_argument("--no-api", action="store_true", help="skip q3 through the operator API (alenkaExecute)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-queries leg")
     ap.add_argument("--batch-queries", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=5)
@@ -711,6 +800,12 @@ def main():
     b_pipeline = (dim_scan + mid_scan + all_fact_scan + 8 * st.dim_pass + 8 * st.mid_pass
                   + 8 * st.mid_probe + 8 * st.fact_probe + 16 * st.fact_pass + 10 * 40)
     pipeline_gbs = b_pipeline / (ms_per_step / 1000.0) / 1e9
+
+    api = None
+    if not args.no_api:
+        api = api_leg(ctx, D, args.seed, args.scale, args.seg_rows, args.steps, args.warmup, rows, ms_per_step)
+        log(f"api: q3 through alenkaExecute {api['ms']:.3f} ms/step ({api['vs_fused_step']:.2f}x the fused "
+            f"step), path {api['path']}, checked {api['checked']}")
 
     configs = {"2": {"fact_rows": nfact, "gpu": {"ms": ms_per_step, "fact_rows_per_s": nfact / (ms_per_step / 1e3),
                                                   "path": "aln_star_agg"}}}
@@ -815,6 +910,7 @@ def main():
                        f"reference's Thrust algorithms on rocThrust's OpenMP backend, {full['threads']} threads on "
                        f"{model}; median of {full['runs']} runs after one that checked the GPU's top-10 and "
                        f"pass counts")},
+        "api": api,
         "configs": configs,
         "batched": batched,
         "served": served,

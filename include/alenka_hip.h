@@ -191,6 +191,14 @@ int aln_scan_filter_noalloc(aln_ctx *ctx, const aln_col *cols, int ncols, uint32
                             const aln_insn *prog, int nprog, const char *seg_modes,
                             uint32_t *d_row_ids, uint64_t cap, uint64_t *h_count);
 
+/* Segment of the source row behind each of n indices (the order key of a join's pairs: the
+ * reference joins left segment by left segment, operators.cu:1168-1240): d_out[i] = segment
+ * of row r, r = d_rows ? d_rows[d_idx[i]] : d_idx[i] (d_rows: the selection a filtered view
+ * reads through); the segment is the index of the last of nstarts ascending first rows
+ * d_starts that is <= r (uneven file segments), or r / seg_rows when d_starts is NULL. */
+int aln_row_segments(aln_ctx *ctx, const uint32_t *d_idx, uint64_t n, const uint32_t *d_rows,
+                     const uint64_t *d_starts, uint32_t nstarts, uint32_t seg_rows, int64_t *d_out);
+
 /* Gather decoded values of rows d_rows[0..n) (d_rows NULL = rows 0..n-1). */
 int aln_gather(aln_ctx *ctx, const aln_col *col, uint32_t seg_rows, const uint32_t *d_rows,
                uint64_t n, int64_t *d_out);

@@ -104,6 +104,18 @@ void emit_join(const char *s, const char *j1, int grp, int start_seg, int end_se
 void emit_order(const char *s, const char *f, int e, int ll);
 void emit_store(const char *s, const char *f, const char *sep);
 void emit_display(const char *s, const char *sep);
+/* Deferred statements: like emit_filter (operators.cu:2306-2408), a JOIN of three tables in
+ * the star shape (fact JOIN mid ON f = m_key JOIN dim ON m_fk = d_key, inner, one column pair
+ * each), a GROUP BY over such a join and an ORDER BY of that GROUP BY record their work and
+ * run it when a consumer first reads the variable.  STORE / DISPLAY ... LIMIT k of such an
+ * ORDER BY, whose keys give ORDER BY SUM DESC, <group column> ASC, runs the whole chain as
+ * one fused pipeline (aln_star_agg's top-k); any other consumer runs the statements one by
+ * one, a GROUP BY over the join through aln_star_agg's group output when its shape allows,
+ * and the unfused operators otherwise (or when the fused kernels refuse the data, e.g.
+ * duplicate build keys).  Results are the same either way.  alenka_last_path() names the
+ * path the last deferred chain took: "aln_star_agg top-k", "aln_star_agg groups" or
+ * "unfused" (no reference counterpart: a diagnostic). */
+const char *alenka_last_path(void);
 /* emit_store_binary (operators.h:49, operators.cu:2446-2516): the variable's columns as the
  * reference's segment files <f>.<col>.<seg> (char: dictionary <f>.<col>, .idx and .hash per
  * segment) with a .header per column, and its entry in the data dictionary (save it with
