@@ -18,9 +18,11 @@ ALN_E_CAPACITY = -5
 SCAN_NOALLOC_MAX_TILES = 262144  # ALN_SCAN_NOALLOC_MAX_TILES: tiles of 4096 rows
 
 RPN_NAME, RPN_NUMBER, RPN_STRING, RPN_CMP, RPN_AND, RPN_OR = 1, 2, 3, 4, 5, 6
-RPN_ADD, RPN_MINUS, RPN_MUL, RPN_DIV, RPN_CODESET = 7, 8, 9, 10, 11
+RPN_ADD, RPN_MINUS, RPN_MUL, RPN_DIV, RPN_CODESET, RPN_FLOAT = 7, 8, 9, 10, 11, 12
 I_COL, I_CONST, I_ADD, I_SUB, I_MUL, I_DIV, I_CMP, I_AND, I_OR, I_INSET = range(1, 11)
-AGG_COUNT, AGG_SUM, AGG_MIN, AGG_MAX = 0, 2, 4, 5
+I_I2F, I_FADD, I_FSUB, I_FMUL, I_FDIV, I_FCMP = range(11, 17)
+DEC_F64 = -1  # col_dec of an f64 column slot
+AGG_COUNT, AGG_SUM, AGG_MIN, AGG_MAX, AGG_FSUM, AGG_FMIN, AGG_FMAX = 0, 2, 4, 5, 6, 7, 8
 ORDER_KEYS, ORDER_HASH, ORDER_NONE = 0, 1, 2
 MAX_INSN, MAX_STACK, MAX_COLS, MAX_KEYS, MAX_AGGS, MAX_RANGES = 64, 16, 16, 4, 8, 8
 
@@ -135,6 +137,8 @@ SIGNATURES = {
     "aln_scan_filter_noalloc": (C.c_int, [P, C.POINTER(Col), C.c_int, U32, C.POINTER(Insn), C.c_int,
                                           C.c_char_p, P, U64, PU64]),
     "aln_row_segments": (C.c_int, [P, P, U64, P, P, U32, U32, P]),
+    "aln_f64_order_keys": (C.c_int, [P, P, U64, P]),
+    "aln_f64_unorder_keys": (C.c_int, [P, P, U64, P]),
     "aln_gather": (C.c_int, [P, C.POINTER(Col), U32, P, U64, P]),
     "aln_eval": (C.c_int, [P, C.POINTER(Col), C.c_int, U32, C.POINTER(Insn), C.c_int, P, U64, P]),
     "aln_apply_perm": (C.c_int, [P, P, P, U64, P]),

@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <string.h>
 
 #include <map>
 #include <string>
@@ -209,6 +210,42 @@ __device__ __forceinline__ int cmp_code(int code, int64_t l, int64_t r) {
     case 6: return l >= r;
     default: return 0;
   }
+}
+
+// f64 values travel as their IEEE-754 bits in int64 slots.
+__host__ __device__ __forceinline__ int64_t f64_bits(double x) {
+  int64_t b;
+  memcpy(&b, &x, 8);
+  return b;
+}
+__host__ __device__ __forceinline__ double bits_f64(int64_t b) {
+  double x;
+  memcpy(&x, &b, 8);
+  return x;
+}
+// The EPSILON comparisons of CudaSet's float overloads (f_less, f_greater, f_equal_to,
+// f_greater_equal_to, f_less_equal, f_not_equal_to; cm.h:17, cm.cu:81-134): x is the left
+// operand, y the right.
+constexpr double ALN_EPSILON = 1.0E-8;
+__host__ __device__ __forceinline__ int fcmp_code(int code, double x, double y) {
+  const bool eq = ((x - y) < ALN_EPSILON) && ((x - y) > -ALN_EPSILON);
+  switch (code) {
+    case 1: return (y - x) > ALN_EPSILON;
+    case 2: return (x - y) > ALN_EPSILON;
+    case 3: return ((x - y) > ALN_EPSILON) || ((x - y) < -ALN_EPSILON);
+    case 4: return eq;
+    case 5: return ((y - x) > ALN_EPSILON) || eq;
+    case 6: return ((x - y) > ALN_EPSILON) || eq;
+    default: return 0;
+  }
+}
+// Order-preserving int64 of a double's bits (-0.0 as +0.0).
+__host__ __device__ __forceinline__ int64_t f64_order_key(int64_t b) {
+  if (b == INT64_MIN) b = 0;  // -0.0
+  return b < 0 ? (int64_t)((uint64_t)b ^ 0x7FFFFFFFFFFFFFFFull) : b;
+}
+__host__ __device__ __forceinline__ int64_t f64_unorder_key(int64_t k) {
+  return k < 0 ? (int64_t)((uint64_t)k ^ 0x7FFFFFFFFFFFFFFFull) : k;
 }
 
 __device__ __forceinline__ uint64_t lanemask_lt() {

@@ -366,11 +366,12 @@ static int check_prog(const aln_insn *prog, int nprog) {
   int depth = 0, mx = 0;
   for (int i = 0; i < nprog; ++i) {
     int op = prog[i].op;
-    ALN_REQUIRE(op >= ALN_I_COL && op <= ALN_I_INSET, "bad instruction %d", op);
+    ALN_REQUIRE(op >= ALN_I_COL && op <= ALN_I_FCMP, "bad instruction %d", op);
     if (op == ALN_I_COL) ALN_REQUIRE(prog[i].a >= 0 && prog[i].a < ALN_MAX_COLS, "bad column %d", prog[i].a);
     if (op == ALN_I_INSET)
       ALN_REQUIRE(prog[i].a >= 0 && (prog[i].a == 0 || prog[i].k != 0), "set test without a bit set");
-    depth += (op == ALN_I_COL || op == ALN_I_CONST) ? 1 : op == ALN_I_INSET ? 0 : -1;
+    if (op == ALN_I_I2F) ALN_REQUIRE((prog[i].a == 0 || prog[i].a == 1) && depth > prog[i].a, "bad conversion");
+    depth += (op == ALN_I_COL || op == ALN_I_CONST) ? 1 : (op == ALN_I_INSET || op == ALN_I_I2F) ? 0 : -1;
     ALN_REQUIRE(depth >= 1, "program underflows its stack");
     mx = depth > mx ? depth : mx;
   }

@@ -42,7 +42,43 @@ struct GbArgs {
 __device__ __forceinline__ int64_t agg_combine(int a, int64_t x, int64_t y) {
   if (a == ALN_AGG_MIN) return x < y ? x : y;
   if (a == ALN_AGG_MAX) return x > y ? x : y;
+  if (a == ALN_AGG_FSUM) return f64_bits(bits_f64(x) + bits_f64(y));
+  if (a == ALN_AGG_FMIN) return bits_f64(y) < bits_f64(x) ? y : x;
+  if (a == ALN_AGG_FMAX) return bits_f64(y) > bits_f64(x) ? y : x;
   return (int64_t)((uint64_t)x + (uint64_t)y);  // SUM, COUNT
+}
+
+// The starting value of an aggregate's accumulator.
+__host__ __device__ __forceinline__ int64_t agg_identity(int a) {
+  switch (a) {
+    case ALN_AGG_MIN: return INT64_MAX;
+    case ALN_AGG_MAX: return INT64_MIN;
+    case ALN_AGG_FMIN: return 0x7FF0000000000000ll;                // +inf
+    case ALN_AGG_FMAX: return (int64_t)0xFFF0000000000000ull;      // -inf
+    default: return 0;                                             // 0, 0.0
+  }
+}
+
+// Adds x into the accumulator at t.
+__device__ __forceinline__ void agg_atomic(int a, int64_t *t, int64_t x) {
+  if (a == ALN_AGG_MIN) {
+    atomicMin((long long *)t, (long long)x);
+  } else if (a == ALN_AGG_MAX) {
+    atomicMax((long long *)t, (long long)x);
+  } else if (a == ALN_AGG_FSUM) {
+    atomicAdd(reinterpret_cast<double *>(t), bits_f64(x));
+  } else if (a == ALN_AGG_FMIN || a == ALN_AGG_FMAX) {
+    unsigned long long *p = reinterpret_cast<unsigned long long *>(t);
+    unsigned long long old = *p, assumed;
+    do {
+      assumed = old;
+      const int64_t nv = agg_combine(a, (int64_t)assumed, x);
+      if (nv == (int64_t)assumed) break;
+      old = atomicCAS(p, assumed, (unsigned long long)nv);
+    } while (old != assumed);
+  } else {
+    atomicAdd((unsigned long long *)t, (unsigned long long)x);
+  }
 }
 
 __device__ __forceinline__ uint64_t murmur64a_tuple(const int64_t *v, int nk, uint32_t seed) {
@@ -66,10 +102,7 @@ __global__ void __launch_bounds__(256) k_gb_init(uint64_t *tkey, GbArgs a, uint6
   for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < cap;
        i += (uint64_t)gridDim.x * 256) {
     tkey[i] = GB_EMPTY;
-    for (int v = 0; v < a.nv; ++v) {
-      int ag = a.aggs[v];
-      a.tab[v][i] = ag == ALN_AGG_MIN ? INT64_MAX : ag == ALN_AGG_MAX ? INT64_MIN : 0;
-    }
+    for (int v = 0; v < a.nv; ++v) a.tab[v][i] = agg_identity(a.aggs[v]);
   }
 }
 
@@ -123,16 +156,7 @@ __global__ void __launch_bounds__(256) k_gb_insert(GbArgs a, uint64_t n, uint64_
         }
       }
       if (h != GB_EMPTY)
-        for (int v = 0; v < a.nv; ++v) {
-          int ag = a.aggs[v];
-          int64_t *t = &a.tab[v][h];
-          if (ag == ALN_AGG_MIN)
-            atomicMin((long long *)t, (long long)val[v]);
-          else if (ag == ALN_AGG_MAX)
-            atomicMax((long long *)t, (long long)val[v]);
-          else
-            atomicAdd((unsigned long long *)t, (unsigned long long)val[v]);
-        }
+        for (int v = 0; v < a.nv; ++v) agg_atomic(a.aggs[v], &a.tab[v][h], val[v]);
     }
   }
 }
@@ -202,14 +226,8 @@ __global__ void k_gb_sorted_agg(GbArgs a, GbOut o, const uint32_t *perm, const u
     if (head[i])
       for (int k = 0; k < a.nk; ++k) o.keys[k][g] = a.keys[k][r];
     for (int v = 0; v < a.nv; ++v) {
-      int ag = a.aggs[v];
-      int64_t x = ag == ALN_AGG_COUNT ? 1 : a.vals[v][r];
-      if (ag == ALN_AGG_MIN)
-        atomicMin((long long *)&o.vals[v][g], (long long)x);
-      else if (ag == ALN_AGG_MAX)
-        atomicMax((long long *)&o.vals[v][g], (long long)x);
-      else
-        atomicAdd((unsigned long long *)&o.vals[v][g], (unsigned long long)x);
+      const int ag = a.aggs[v];
+      agg_atomic(ag, &o.vals[v][g], ag == ALN_AGG_COUNT ? 1 : a.vals[v][r]);
     }
   }
 }
@@ -250,6 +268,11 @@ __global__ void k_hash_rows(GbArgs a, uint64_t n, uint32_t seed, uint64_t *out) 
   }
 }
 
+__global__ void k_f64_keys(const int64_t *in, uint64_t n, int back, int64_t *out) {
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256)
+    out[i] = back ? f64_unorder_key(in[i]) : f64_order_key(in[i]);
+}
+
 static int groupby_sorted(aln_ctx *c, GbArgs &a, uint64_t n, int order, uint32_t seed, GbOut &o,
                           uint64_t cap, uint64_t *h_ngroups) {
   uint32_t *perm, *head;
@@ -269,10 +292,7 @@ static int groupby_sorted(aln_ctx *c, GbArgs &a, uint64_t n, int order, uint32_t
   if (g > cap)
     return set_error(ALN_E_CAPACITY, "aln_groupby: %llu groups, capacity %llu",
                      (unsigned long long)g, (unsigned long long)cap);
-  for (int v = 0; v < a.nv; ++v) {
-    int ag = a.aggs[v];
-    ALN_TRY(fill_i64(c, o.vals[v], g, ag == ALN_AGG_MIN ? INT64_MAX : ag == ALN_AGG_MAX ? INT64_MIN : 0));
-  }
+  for (int v = 0; v < a.nv; ++v) ALN_TRY(fill_i64(c, o.vals[v], g, agg_identity(a.aggs[v])));
   {
     ProfScope p(c, "groupby_sorted_agg");
     hipLaunchKernelGGL(k_gb_sorted_agg, dim3(grid_for(n, 256, 65536)), dim3(256), 0, c->stream, a,
@@ -340,7 +360,8 @@ int aln_groupby(aln_ctx *c, const int64_t *const *d_keys, int nkeys, const int64
   }
   for (int v = 0; v < nvals; ++v) {
     ALN_REQUIRE(aggs[v] == ALN_AGG_COUNT || aggs[v] == ALN_AGG_SUM || aggs[v] == ALN_AGG_MIN ||
-                    aggs[v] == ALN_AGG_MAX,
+                    aggs[v] == ALN_AGG_MAX || aggs[v] == ALN_AGG_FSUM || aggs[v] == ALN_AGG_FMIN ||
+                    aggs[v] == ALN_AGG_FMAX,
                 "aln_groupby: aggregate %d", aggs[v]);
     ALN_REQUIRE(aggs[v] == ALN_AGG_COUNT || d_vals[v], "aln_groupby: value %d is NULL", v);
     a.vals[v] = d_vals[v];
@@ -430,6 +451,20 @@ int aln_groupby(aln_ctx *c, const int64_t *const *d_keys, int nkeys, const int64
   ALN_TRY(check_launch("gb_emit"));
   ALN_HIP(hipStreamSynchronize(c->stream));
   return ALN_OK;
+}
+
+int aln_f64_order_keys(aln_ctx *c, const int64_t *d_in, uint64_t n, int64_t *d_out) {
+  ALN_REQUIRE(c && (n == 0 || (d_in && d_out)), "aln_f64_order_keys: bad arguments");
+  if (n == 0) return ALN_OK;
+  hipLaunchKernelGGL(k_f64_keys, dim3(grid_for(n, 256, 65536)), dim3(256), 0, c->stream, d_in, n, 0, d_out);
+  return check_launch("f64_order_keys");
+}
+
+int aln_f64_unorder_keys(aln_ctx *c, const int64_t *d_in, uint64_t n, int64_t *d_out) {
+  ALN_REQUIRE(c && (n == 0 || (d_in && d_out)), "aln_f64_unorder_keys: bad arguments");
+  if (n == 0) return ALN_OK;
+  hipLaunchKernelGGL(k_f64_keys, dim3(grid_for(n, 256, 65536)), dim3(256), 0, c->stream, d_in, n, 1, d_out);
+  return check_launch("f64_unorder_keys");
 }
 
 int aln_hash_rows(aln_ctx *c, const int64_t *const *d_keys, int nkeys, uint64_t n, uint32_t seed,

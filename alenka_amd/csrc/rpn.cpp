@@ -16,8 +16,10 @@
 
 namespace {
 
+using aln::bits_f64;
+using aln::f64_bits;
 using Frag = std::vector<aln_insn>;
-enum { T_NAME = 1, T_NUMBER = 2, T_STRING = 3, T_SET = 4 };
+enum { T_NAME = 1, T_NUMBER = 2, T_STRING = 3, T_SET = 4, T_FLOAT = 5 };
 
 int pw10(uint32_t p, int64_t *out) {
   static const int64_t t[10] = {1,      10,      100,      1000,      10000,
@@ -127,6 +129,8 @@ struct Compiler {
   std::vector<std::pair<int64_t, int32_t>> sets;  // LIKE operands: bit set address, codes
   std::vector<uint32_t> prec;
   std::vector<Frag> vecs, bools;
+  std::vector<bool> vfloat;   // per vecs entry: the vector holds doubles
+  std::vector<double> fnums;  // FLOAT operands
   int err = ALN_OK;
   const char *why = "";
 
@@ -143,7 +147,108 @@ struct Compiler {
   }
   Frag get_vec(int v) {
     if (v >= 0) return Frag{I(ALN_I_COL, v, 1)};
+    if (!vfloat.empty()) vfloat.pop_back();
     return pop(vecs);
+  }
+  void push_vec(const Frag &f, bool fl) {
+    vecs.push_back(f);
+    vfloat.push_back(fl);
+  }
+  bool float_col(int v) const { return v >= 0 && v < ncols && dec[v] == ALN_DEC_F64; }
+
+  // ---- double precision: CudaSet's float overloads (cm.cu:2178-2437, 2440-2773) ----------
+  // Whether either of the top two operands is a double (a FLOAT, an f64 column or a value
+  // computed from one), read without popping.
+  bool float_operands() const {
+    const size_t nt = type.size();
+    if (nt < 2) return false;
+    size_t vi = value.size(), ci = vecs.size();
+    for (int k = 0; k < 2; ++k) {
+      const int t = type[nt - 1 - k];
+      if (t == T_FLOAT) return true;
+      if (t == T_NAME || t == T_STRING || t == T_SET) {
+        if (vi == 0) return false;
+        const int v = value[--vi];
+        if (t != T_NAME) continue;
+        if (v >= 0) {
+          if (float_col(v)) return true;
+        } else if (ci > 0 && vfloat[--ci]) {
+          return true;
+        }
+      }
+    }
+    return false;
+  }
+  struct FOp {
+    Frag f;
+    bool is_const = false;
+    double c = 0;
+  };
+  // One operand as a double: an int column by a plain cast of its stored integer
+  // (long_to_float_type, cm.cu:137-144), a NUMBER by its decimal value.
+  FOp fpop(int t) {
+    FOp o;
+    if (t == T_FLOAT) {
+      o.is_const = true;
+      o.c = pop(fnums);
+    } else if (t == T_NUMBER) {
+      const int64_t n = pop(nums);
+      const uint32_t p = pop(prec);
+      if (p > 18) {
+        fail(ALN_E_UNSUPPORTED, "decimal scale above 10^18");
+        return o;
+      }
+      double d = 1;
+      for (uint32_t i = 0; i < p; ++i) d *= 10;
+      o.is_const = true;
+      o.c = (double)n / d;
+    } else if (t == T_NAME) {
+      const int v = pop(value);
+      if (v >= 0) {
+        o.f = Frag{I(ALN_I_COL, v, 1)};
+        if (!float_col(v)) o.f.push_back(I(ALN_I_I2F, 0, 0));
+      } else {
+        if (vecs.empty()) {
+          fail(ALN_E_UNSUPPORTED, "a comparison result used as a value");
+          return o;
+        }
+        const bool fl = vfloat.back();
+        o.f = get_vec(v);
+        (void)pop(prec);
+        if (!fl) o.f.push_back(I(ALN_I_I2F, 0, 0));
+      }
+      return o;
+    } else {
+      fail(ALN_E_UNSUPPORTED, "a string operand in double arithmetic");
+      return o;
+    }
+    o.f = Frag{I(ALN_I_CONST, 0, f64_bits(o.c))};
+    return o;
+  }
+  void farith(int op) {
+    const int s1 = pop(type), s2 = pop(type);
+    FOp r = fpop(s1), l = fpop(s2);
+    if (err) return;
+    if (l.is_const && r.is_const) {  // folded, as the reference folds NUMBER NUMBER
+      double v = op == ALN_RPN_ADD ? l.c + r.c : op == ALN_RPN_MINUS ? l.c - r.c
+               : op == ALN_RPN_MUL ? l.c * r.c : l.c / r.c;
+      fnums.push_back(v);
+      type.push_back(T_FLOAT);
+      return;
+    }
+    const int io = op == ALN_RPN_ADD ? ALN_I_FADD : op == ALN_RPN_MINUS ? ALN_I_FSUB
+                 : op == ALN_RPN_MUL ? ALN_I_FMUL : ALN_I_FDIV;
+    prec.push_back(0);
+    type.push_back(T_NAME);
+    value.push_back(-1);
+    push_vec(cat(l.f, r.f, I(io, 0, 0)), true);
+  }
+  void fcmp(int code) {
+    const int s1 = pop(type), s2 = pop(type);
+    FOp r = fpop(s1), l = fpop(s2);
+    if (err) return;
+    type.push_back(T_NAME);
+    bools.push_back(cat(l.f, r.f, I(ALN_I_FCMP, code, 0)));
   }
   uint32_t get_decimals(int v) {
     if (v >= 0) return (uint32_t)dec[v];
@@ -186,6 +291,7 @@ struct Compiler {
   }
 
   void arith(int op) {
+    if (float_operands()) return farith(op);
     int s1 = pop(type), s2 = pop(type);
     if (err) return;
     if (s1 == T_NUMBER && s2 == T_NUMBER) {
@@ -199,7 +305,7 @@ struct Compiler {
       if (p2) n2 = wmul(n2, b);
       type.push_back(T_NAME);
       value.push_back(-1);
-      vecs.push_back(Frag{I(ALN_I_CONST, 0, arith_fold(op, n1, n2))});
+      push_vec(Frag{I(ALN_I_CONST, 0, arith_fold(op, n1, n2))}, false);
     } else if (s1 == T_NAME && s2 == T_NUMBER) {
       int v1 = pop(value);
       int64_t n1 = pop(nums);
@@ -211,7 +317,7 @@ struct Compiler {
       prec.push_back(pres);
       type.push_back(T_NAME);
       value.push_back(-1);
-      vecs.push_back(mode == 0 ? op_col_const(t, n1, op, 1, p1, p2) : op_col_const(t, n1, op, 1, p2, p1));
+      push_vec(mode == 0 ? op_col_const(t, n1, op, 1, p1, p2) : op_col_const(t, n1, op, 1, p2, p1), false);
     } else if (s1 == T_NUMBER && s2 == T_NAME) {
       int64_t n1 = pop(nums);
       int v2 = pop(value);
@@ -222,7 +328,7 @@ struct Compiler {
       prec.push_back(pres);
       type.push_back(T_NAME);
       value.push_back(-1);
-      vecs.push_back(op_col_const(t, n1, op, 0, p2, p1));
+      push_vec(op_col_const(t, n1, op, 0, p2, p1), false);
     } else if (s1 == T_NAME && s2 == T_NAME) {
       int v1 = pop(value), v2 = pop(value);
       Frag t1 = get_vec(v1), t = get_vec(v2);
@@ -231,7 +337,7 @@ struct Compiler {
       prec.push_back(pres);
       type.push_back(T_NAME);
       value.push_back(-1);
-      vecs.push_back(op_col_col(t, t1, op, p2, p1));
+      push_vec(op_col_col(t, t1, op, p2, p1), false);
     } else {
       fail(ALN_E_UNSUPPORTED, "arithmetic on a string operand");
     }
@@ -249,6 +355,7 @@ struct Compiler {
   }
 
   void cmp(int code) {
+    if (code >= 1 && code <= 6 && float_operands()) return fcmp(code);
     int s1 = pop(type), s2 = pop(type);
     if (err) return;
     if (code == 7 || s1 == T_SET || s2 == T_SET) {
@@ -267,6 +374,7 @@ struct Compiler {
       return;
     }
     if (code < 1 || code > 6) return fail(ALN_E_UNSUPPORTED, "comparison code outside 1..6");
+    if (s1 == T_FLOAT || s2 == T_FLOAT) return fail(ALN_E_UNSUPPORTED, "FLOAT compared with a string");
     if (s1 == T_NUMBER && s2 == T_NUMBER) {
       int64_t n1 = pop(nums), n2 = pop(nums);
       type.push_back(T_NAME);
@@ -347,6 +455,10 @@ struct Compiler {
           type.push_back(T_NUMBER);
           prec.push_back((uint32_t)o.arg);
           break;
+        case ALN_RPN_FLOAT:
+          fnums.push_back(bits_f64(o.val));
+          type.push_back(T_FLOAT);
+          break;
         case ALN_RPN_STRING:
           strs.push_back(o.val);
           value.push_back(-2 - (int)(strs.size() - 1));
@@ -398,7 +510,7 @@ int stack_depth(const Frag &f) {
   for (auto &x : f) {
     if (x.op == ALN_I_COL || x.op == ALN_I_CONST)
       ++d;
-    else if (x.op != ALN_I_INSET)
+    else if (x.op != ALN_I_INSET && x.op != ALN_I_I2F)
       --d;
     if (d > mx) mx = d;
     if (d < 1) return -1;
@@ -426,7 +538,10 @@ extern "C" int aln_compile_rpn(const aln_rpn *rpn, int nrpn, const int32_t *col_
     ALN_REQUIRE(!c.bools.empty(), "aln_compile_rpn: predicate yields no boolean");
     res = c.bools.back();
   } else {
-    if (!c.type.empty() && c.type.back() == T_NUMBER && !c.nums.empty()) {
+    if (!c.type.empty() && c.type.back() == T_FLOAT && !c.fnums.empty()) {
+      res = Frag{I(ALN_I_CONST, 0, f64_bits(c.fnums.back()))};
+      prec = ALN_DEC_F64;
+    } else if (!c.type.empty() && c.type.back() == T_NUMBER && !c.nums.empty()) {
       res = Frag{I(ALN_I_CONST, 0, c.nums.back())};
       prec = c.prec.empty() ? 0 : (int32_t)c.prec.back();
     } else if (!c.value.empty() && c.value.back() >= 0) {
@@ -434,7 +549,7 @@ extern "C" int aln_compile_rpn(const aln_rpn *rpn, int nrpn, const int32_t *col_
       prec = col_dec[c.value.back()];
     } else if (!c.vecs.empty()) {
       res = c.vecs.back();
-      prec = c.prec.empty() ? 0 : (int32_t)c.prec.back();
+      prec = c.vfloat.back() ? ALN_DEC_F64 : c.prec.empty() ? 0 : (int32_t)c.prec.back();
     } else {
       return set_error(ALN_E_ARG, "aln_compile_rpn: expression yields no value");
     }
@@ -509,7 +624,10 @@ extern "C" int aln_zone_map_check(const aln_rpn *rpn, int nrpn, const int64_t *s
     return ALN_OK;
   }
   for (int i = 0; i < nrpn; ++i) {
-    if (rpn[i].type >= ALN_RPN_ADD && rpn[i].type <= ALN_RPN_DIV) {
+    // arithmetic, FLOAT operands and f64 columns: evaluated on every segment
+    if ((rpn[i].type >= ALN_RPN_ADD && rpn[i].type <= ALN_RPN_DIV) || rpn[i].type == ALN_RPN_FLOAT ||
+        (rpn[i].type == ALN_RPN_NAME && rpn[i].arg >= 0 && rpn[i].arg < ncols && col_dec &&
+         col_dec[rpn[i].arg] == ALN_DEC_F64)) {
       *out = 'R';
       return ALN_OK;
     }

@@ -37,6 +37,21 @@ __device__ __forceinline__ int64_t vm_run(const aln_insn *__restrict__ prog, int
       const int64_t v = stk[(sp - 1) * VM_BLOCK];
       const uint32_t *set = reinterpret_cast<const uint32_t *>(prog[i].k);
       stk[(sp - 1) * VM_BLOCK] = (v >= 0 && v < prog[i].a) ? (set[v >> 5] >> (v & 31)) & 1u : 0;
+    } else if (op == ALN_I_I2F) {
+      int64_t *e = &stk[(sp - 1 - prog[i].a) * VM_BLOCK];
+      *e = f64_bits((double)*e);
+    } else if (op >= ALN_I_FADD) {
+      const double r = bits_f64(stk[(sp - 1) * VM_BLOCK]), l = bits_f64(stk[(sp - 2) * VM_BLOCK]);
+      int64_t v;
+      switch (op) {
+        case ALN_I_FADD: v = f64_bits(l + r); break;
+        case ALN_I_FSUB: v = f64_bits(l - r); break;
+        case ALN_I_FMUL: v = f64_bits(l * r); break;
+        case ALN_I_FDIV: v = f64_bits(l / r); break;
+        default: v = fcmp_code(prog[i].a, l, r); break;
+      }
+      stk[(sp - 2) * VM_BLOCK] = v;
+      --sp;
     } else {
       int64_t r = stk[(sp - 1) * VM_BLOCK], l = stk[(sp - 2) * VM_BLOCK], v;
       switch (op) {

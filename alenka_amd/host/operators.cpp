@@ -43,7 +43,7 @@ DBufP dalloc(size_t bytes) {
 }
 
 struct Col {
-  int type = 0;       // 0 int/decimal, 2 char (dictionary code)
+  int type = 0;       // 0 int/decimal, 1 f64 (the doubles' bits, plain), 2 char (dictionary code)
   int decimals = 0;   // decimal_zeroes
   DBufP words;        // packed words, or the int64 values of a plain column
   DBufP dsegs;        // device aln_seg[nsegs]
@@ -59,6 +59,9 @@ struct Col {
     return c;
   }
 };
+
+// The compiler's col_dec entry of a column: its decimal scale, or ALN_DEC_F64 for doubles.
+inline int32_t col_dec(const Col &c) { return c.type == 1 ? ALN_DEC_F64 : c.decimals; }
 
 struct Tok {
   std::string type;  // NAME NUMBER STRING CMP AND OR ADD MINUS MUL DIV JOIN COUNT SUM AVG MIN MAX SEL ORDER
@@ -343,6 +346,9 @@ int resolve_string_codes(const Set &src, const std::vector<Tok> &t, std::vector<
       r.type = ALN_RPN_NUMBER;
       r.val = k.num;
       r.arg = (int32_t)k.prec;
+    } else if (k.type == "FLOAT") {
+      r.type = ALN_RPN_FLOAT;
+      r.val = k.num;  // the double's bits
     } else if (k.type == "STRING") {
       r.type = ALN_RPN_STRING;
       r.val = -1;
@@ -404,7 +410,7 @@ int run_filter(Set &f) {
   std::vector<int32_t> dec;
   for (auto &n : names) {
     cols.push_back(src->cols[n].desc());
-    dec.push_back(src->cols[n].decimals);
+    dec.push_back(col_dec(src->cols[n]));
   }
   aln_insn prog[ALN_MAX_INSN];
   int nprog = 0;
@@ -539,15 +545,15 @@ int eval_expr(const View &v, const std::vector<Tok> &expr, DBufP *out, int *prec
   std::vector<int32_t> dec;
   for (auto &n : names) {
     cols.push_back(v.cols.at(n).desc());
-    dec.push_back(v.cols.at(n).decimals);
+    dec.push_back(col_dec(v.cols.at(n)));
   }
   *out = dalloc(8 * (v.nrows ? v.nrows : 1));
   if (!*out) return fail(ALN_E_NOMEM, "device allocation failed");
   if (names.empty()) {  // a constant
-    if (rpn.size() != 1 || rpn[0].type != ALN_RPN_NUMBER)
+    if (rpn.size() != 1 || (rpn[0].type != ALN_RPN_NUMBER && rpn[0].type != ALN_RPN_FLOAT))
       return fail(ALN_E_UNSUPPORTED, "expression without columns");
     std::vector<int64_t> h(v.nrows, rpn[0].val);
-    *prec = rpn[0].arg;
+    *prec = rpn[0].type == ALN_RPN_FLOAT ? ALN_DEC_F64 : rpn[0].arg;
     if (v.nrows && hipMemcpy((*out)->p, h.data(), 8 * v.nrows, hipMemcpyHostToDevice) != hipSuccess)
       return fail(ALN_E_HIP, "host to device copy failed");
     return ALN_OK;
@@ -649,6 +655,12 @@ int alenka_table_add_int_device(const char *table, const char *col, const int64_
   return add_int(table, col, d_vals, true, decimals, compress);
 }
 
+int alenka_table_add_float(const char *table, const char *col, const double *h_vals) {
+  int rc = add_int(table, col, reinterpret_cast<const int64_t *>(h_vals), false, 0, 0);
+  if (rc == ALN_OK) find_var(table)->cols[col].type = 1;
+  return rc;
+}
+
 int alenka_table_add_char(const char *table, const char *col, const char *h_chars, uint32_t width) {
   g_status = ALN_OK;
   TRY(ensure_ctx());
@@ -718,6 +730,18 @@ int alenka_fetch_int(const char *var, const char *col, int64_t *h_out, uint64_t 
   return ALN_OK;
 }
 
+int alenka_fetch_f64(const char *var, const char *col, double *h_out, uint64_t cap) {
+  g_status = ALN_OK;
+  View v;
+  TRY(make_view(var ? var : "", {col ? col : ""}, &v));
+  if (v.cols[col].type != 1) return fail(ALN_E_ARG, "alenka_fetch_f64: %s is not a float column", col);
+  if (v.nrows > cap) return fail(ALN_E_CAPACITY, "alenka_fetch_f64: %llu rows", (unsigned long long)v.nrows);
+  std::vector<int64_t> h;
+  TRY(fetch_col(v.cols[col], v.nrows, v.seg_rows, &h));
+  if (!h.empty()) memcpy(h_out, h.data(), 8 * h.size());
+  return ALN_OK;
+}
+
 int alenka_fetch_char(const char *var, const char *col, char *h_out, uint64_t cap_rows, uint32_t width) {
   g_status = ALN_OK;
   View v;
@@ -768,6 +792,12 @@ void emit_string(const char *str) {
   std::string s(str);
   if (!s.empty() && s[0] == '"') s = s.substr(1, s.size() >= 2 ? s.size() - 2 : 0);
   push("STRING", s);
+}
+// emit_float (operators.cu:114-118): the double travels as its bits
+void emit_float(double val) {
+  int64_t b;
+  memcpy(&b, &val, 8);
+  push("FLOAT", "", b);
 }
 void emit_cmp(int val) { push("CMP", "", val); }
 void emit_and(void) { push("AND"); }
@@ -851,7 +881,8 @@ static int do_select(const char *s, const char *f, int grp_cnt) {
       int prec = 0;
       TRY(eval_expr(v, it.expr, &vals, &prec));
       Col c;
-      TRY(make_plain(vals, v.nrows, (uint32_t)(v.nrows ? v.nrows : 1), prec, &c));
+      TRY(make_plain(vals, v.nrows, (uint32_t)(v.nrows ? v.nrows : 1), prec == ALN_DEC_F64 ? 0 : prec, &c));
+      if (prec == ALN_DEC_F64) c.type = 1;
       if (it.expr.size() == 1 && it.expr[0].type == "NAME") {
         const Col &srcc = v.cols[it.expr[0].value];
         c.type = srcc.type;
@@ -868,10 +899,17 @@ static int do_select(const char *s, const char *f, int grp_cnt) {
     return ALN_OK;
   }
 
-  // keys: the GROUP BY columns (a constant key when there is none: one output row)
+  // one source segment: key order (order_inplace + GroupBy); several: the hash-ordered
+  // merge of add()/count_avg() (merge.cu:92-496)
+  const int order = (grp_cnt > 0 && v.cycles > 1) ? ALN_ORDER_HASH : ALN_ORDER_KEYS;
+  // keys: the GROUP BY columns (a constant key when there is none: one output row).  f64
+  // keys group by value: in key order they are sorted as doubles (order-preserving keys,
+  // mapped back afterwards), in hash order the hash covers the doubles' 8 bytes as the
+  // reference's merge does (merge.cu:131-133).
   std::vector<DBufP> keybufs;
   std::vector<const int64_t *> keys;
   std::vector<int> keydec;
+  std::vector<bool> keyf64;
   if (grp_cnt == 0) {
     DBufP z = dalloc(8 * (v.nrows ? v.nrows : 1));
     if (!z) return fail(ALN_E_NOMEM, "device allocation failed");
@@ -879,14 +917,19 @@ static int do_select(const char *s, const char *f, int grp_cnt) {
     keybufs.push_back(z);
     keys.push_back((const int64_t *)z->p);
     keydec.push_back(0);
+    keyf64.push_back(false);
   } else {
     if (grp_cnt > ALN_MAX_KEYS) return fail(ALN_E_UNSUPPORTED, "GROUP BY on more than %d columns", ALN_MAX_KEYS);
     for (auto &g : groups) {
       DBufP d;
       TRY(plain_copy(v.cols[g], v.nrows, v.seg_rows, &d));
+      const bool f64 = v.cols[g].type == 1;
+      if (f64 && order == ALN_ORDER_KEYS && v.nrows)
+        ALN(aln_f64_order_keys(g_ctx, (const int64_t *)d->p, v.nrows, (int64_t *)d->p), "group by");
       keybufs.push_back(d);
       keys.push_back((const int64_t *)d->p);
       keydec.push_back(v.cols[g].decimals);
+      keyf64.push_back(f64);
     }
   }
   // aggregate inputs
@@ -895,6 +938,7 @@ static int do_select(const char *s, const char *f, int grp_cnt) {
     int key = -1;  // key index
     int agg = -1, agg2 = -1;
     int prec = 0;
+    bool f64 = false;  // the value is a double
   };
   std::vector<Out> outs;
   std::vector<DBufP> valbufs;
@@ -911,13 +955,19 @@ static int do_select(const char *s, const char *f, int grp_cnt) {
       o.kind = 0;
       o.key = (int)(g - groups.begin());
       o.prec = keydec[o.key];
+      o.f64 = keyf64[o.key];
     } else {
       DBufP vb;
       int prec = 0;
       TRY(eval_expr(v, it.expr, &vb, &prec));
       valbufs.push_back(vb);
-      o.prec = it.agg == "COUNT" ? 0 : prec;
-      int code = it.agg == "COUNT" ? ALN_AGG_COUNT : it.agg == "MIN" ? ALN_AGG_MIN : it.agg == "MAX" ? ALN_AGG_MAX : ALN_AGG_SUM;
+      const bool f64 = prec == ALN_DEC_F64 && it.agg != "COUNT";
+      o.f64 = f64;
+      o.prec = it.agg == "COUNT" || f64 ? 0 : prec;
+      int code = it.agg == "COUNT" ? ALN_AGG_COUNT
+               : it.agg == "MIN"   ? (f64 ? ALN_AGG_FMIN : ALN_AGG_MIN)
+               : it.agg == "MAX"   ? (f64 ? ALN_AGG_FMAX : ALN_AGG_MAX)
+                                   : (f64 ? ALN_AGG_FSUM : ALN_AGG_SUM);
       o.kind = it.agg == "AVG" ? 2 : 1;
       o.agg = (int)aggs.size();
       vals.push_back((const int64_t *)vb->p);
@@ -942,9 +992,6 @@ static int do_select(const char *s, const char *f, int grp_cnt) {
     ovb.push_back(dalloc(8 * cap));
     ov.push_back((int64_t *)ovb.back()->p);
   }
-  // one source segment: key order (order_inplace + GroupBy); several: the hash-ordered
-  // merge of add()/count_avg() (merge.cu:92-496)
-  const int order = (grp_cnt > 0 && v.cycles > 1) ? ALN_ORDER_HASH : ALN_ORDER_KEYS;
   uint64_t ng = 0;
   if (v.nrows) {
     ALN(aln_groupby(g_ctx, keys.data(), (int)keys.size(), vals.data(), aggs.data(), (int)aggs.size(),
@@ -960,6 +1007,10 @@ static int do_select(const char *s, const char *f, int grp_cnt) {
     const Out &o = outs[i];
     Col c;
     if (o.kind == 0) {
+      if (o.f64 && order == ALN_ORDER_KEYS && ng)
+        ALN(aln_f64_unorder_keys(g_ctx, okb[o.key]->p ? (const int64_t *)okb[o.key]->p : nullptr, ng,
+                                 (int64_t *)okb[o.key]->p),
+            "group by");
       TRY(make_plain(okb[o.key], ng, (uint32_t)(ng ? ng : 1), o.prec, &c));
       const Col &srcc = v.cols[groups[o.key]];
       c.type = srcc.type;
@@ -967,6 +1018,25 @@ static int do_select(const char *s, const char *f, int grp_cnt) {
       c.width = srcc.width;
     } else if (o.kind == 1) {
       TRY(make_plain(ovb[o.agg], ng, (uint32_t)(ng ? ng : 1), o.prec, &c));
+      if (o.f64) c.type = 1;
+    } else if (o.f64) {
+      // AVG of doubles: sum / count (float_avg, merge.cu:31-36)
+      std::vector<int64_t> sum(ng), cnt(ng);
+      if (hipMemcpy(sum.data(), ovb[o.agg]->p, 8 * ng, hipMemcpyDeviceToHost) != hipSuccess ||
+          hipMemcpy(cnt.data(), ovb[o.agg2]->p, 8 * ng, hipMemcpyDeviceToHost) != hipSuccess)
+        return fail(ALN_E_HIP, "device to host copy failed");
+      for (uint64_t g = 0; g < ng; ++g) {
+        double x;
+        memcpy(&x, &sum[g], 8);
+        x /= (double)cnt[g];
+        memcpy(&sum[g], &x, 8);
+      }
+      DBufP d = dalloc(8 * (ng ? ng : 1));
+      if (!d) return fail(ALN_E_NOMEM, "device allocation failed");
+      if (ng && hipMemcpy(d->p, sum.data(), 8 * ng, hipMemcpyHostToDevice) != hipSuccess)
+        return fail(ALN_E_HIP, "host to device copy failed");
+      TRY(make_plain(d, ng, (uint32_t)(ng ? ng : 1), 0, &c));
+      c.type = 1;
     } else {
       // count_avg (merge.cu:446-460): scale 2 or less gains two digits, (sum*100)/count
       std::vector<int64_t> sum(ng), cnt(ng);
@@ -1096,6 +1166,10 @@ static int join_pair(const std::string &name, const std::string &left, const std
   for (auto &e : eqs) {
     if (lv.cols[e.first].type == 2 || rv.cols[e.second].type == 2)
       return fail(ALN_E_UNSUPPORTED, "Join : string join keys (operators.cu:1030-1033)");
+    // the reference joins float keys only when both are decimal floats (operators.cu:1031),
+    // which are ints here
+    if (lv.cols[e.first].type == 1 || rv.cols[e.second].type == 1)
+      return fail(ALN_E_UNSUPPORTED, "Join : float join keys (operators.cu:1030-1033)");
   }
   DBufP lk, rk;
   TRY(plain_copy(lv.cols[eqs[0].first], lv.nrows, lv.seg_rows, &lk));
@@ -1283,6 +1357,7 @@ static int semi_anti(const std::string &name, char kind, const View &lv, const V
       TRY(gather_plain(lv, eqs[e].first, pi, np, &a));
       TRY(gather_plain(rv, eqs[e].second, bi, np, &b));
       if (a.type == 2 || b.type == 2) return fail(ALN_E_UNSUPPORTED, "Join : string join keys");
+      if (a.type == 1 || b.type == 1) return fail(ALN_E_UNSUPPORTED, "Join : float join keys");
       const int s = (int)cols.size();
       tmp.push_back(a);
       tmp.push_back(b);
@@ -1460,6 +1535,10 @@ static int do_order(const char *s, const char *f) {
     const Col &kc = v.cols[keys[i]];
     DBufP d;
     TRY(plain_copy(kc, n, v.seg_rows, &d));
+    if (kc.type == 1 && n) {
+      // f64 key: doubles sort by value (update_permutation on d_columns_float, sorts.cu:17-86)
+      ALN(aln_f64_order_keys(g_ctx, (const int64_t *)d->p, n, (int64_t *)d->p), "order");
+    }
     if (kc.dict) {
       // char key: str_sort handles widths 1..101 and 255 and leaves the order as it is for
       // any other width (strings_sort_device.cu:125-137), so such a key does not sort
@@ -1539,6 +1618,10 @@ static void print_rows(FILE *fp, const char *sep, const std::vector<const Col *>
       if (c.dict) {
         const std::string &sv = (*c.dict)[(size_t)x];
         fprintf(fp, "%.*s", (int)c.width, sv.c_str());
+      } else if (c.type == 1) {  // float: two digits (cm.cu:1569)
+        double d;
+        memcpy(&d, &x, 8);
+        fprintf(fp, "%.2f", d);
       } else if (c.decimals) {
         std::string str = std::to_string(x);
         while (str.length() <= (size_t)c.decimals) str = '0' + str;
@@ -1651,7 +1734,7 @@ bool star_ranges(const Set &var, StarTable *t) {
   std::vector<aln_rpn> rpn;
   if (resolve_string_codes(*t->base, var.pred, &rpn, slot) != ALN_OK) return false;
   std::vector<int32_t> dec;
-  for (auto &n : names) dec.push_back(t->base->cols[n].decimals);
+  for (auto &n : names) dec.push_back(col_dec(t->base->cols[n]));
   aln_insn prog[ALN_MAX_INSN];
   int np = 0;
   if (aln_compile_rpn(rpn.data(), (int)rpn.size(), dec.data(), (int)dec.size(), 0, prog, &np, nullptr) != ALN_OK)
@@ -1839,7 +1922,7 @@ bool plan_star(const Deferred &dj, const std::vector<Tok> &sel_tok, int grp_cnt,
   }
   for (const std::string *k : {&mfk, &dkey, &ffk, &mkey}) {
     const auto &b = k == &dkey ? bd : k == &ffk ? bf : bm;
-    if (b->cols.at(*k).type == 2) return false;  // string keys: the join's own error
+    if (b->cols.at(*k).type != 0) return false;  // string / float keys: the join's own error
   }
   // a join output column comes from the fact table, else the mid, else the dimension table
   auto provider = [&](const std::string &a, std::string *col) -> int {
@@ -1859,7 +1942,7 @@ bool plan_star(const Deferred &dj, const std::vector<Tok> &sel_tok, int grp_cnt,
   groups.erase(groups.begin(), groups.end() - grp_cnt);
   std::vector<std::string> gcol(grp_cnt);
   for (int i = 0; i < grp_cnt; ++i)
-    if (provider(groups[i], &gcol[i]) != 1) return false;
+    if (provider(groups[i], &gcol[i]) != 1 || bm->cols.at(gcol[i]).type == 1) return false;
   if (gcol[0] != mkey) return false;
   p->dim.base = bd;
   p->mid.base = bm;
@@ -1909,7 +1992,7 @@ bool plan_star(const Deferred &dj, const std::vector<Tok> &sel_tok, int grp_cnt,
       std::vector<aln_rpn> rpn;
       if (resolve_string_codes(view, it.expr, &rpn, slot) != ALN_OK) return false;
       std::vector<int32_t> dec;
-      for (auto &n : names) dec.push_back(bf->cols.at(n).decimals);
+      for (auto &n : names) dec.push_back(col_dec(bf->cols.at(n)));
       aln_insn prog[ALN_MAX_INSN];
       int np = 0;
       int32_t prec = 0;
@@ -2160,7 +2243,7 @@ bool defer_join(const char *s, const char *j1) {
   for (auto &c : conds)
     for (auto &e : c.eqs)
       for (const std::string *n : {&e.first, &e.second}) {
-        auto in = [&](const std::shared_ptr<Set> &b) { return b->cols.count(*n) && b->cols.at(*n).type != 2; };
+        auto in = [&](const std::shared_ptr<Set> &b) { return b->cols.count(*n) && b->cols.at(*n).type == 0; };
         if (!in(bd) && !in(bm) && !in(bf)) return false;
       }
   auto b = std::make_shared<Set>();
@@ -2359,8 +2442,61 @@ int load_binary(const std::string &name, const std::string &file) {
   for (auto &kv : dd->second) {
     const std::string &cn = kv.first;
     const ColData cd = kv.second;
-    if (cd.type == 1) return fail(ALN_E_UNSUPPORTED, "float column %s", cn.c_str());
     if (cd.type == 2 && cd.length == 0) return fail(ALN_E_ARG, "char column %s without a width", cn.c_str());
+    if (cd.type == 1) {
+      // float segments: [u32 cnt][cnt doubles][u32 comp_type 3] (cm.cu:1150-1180), or a
+      // decimal float stored as value*100 in a FOR file, read back as x / 100.0 (long_to_float,
+      // cm.h:300-307; cm.cu:767-787)
+      std::vector<int64_t> h;
+      std::vector<uint64_t> starts;
+      std::vector<char> fb;
+      for (uint32_t g = 0; g < nseg; ++g) {
+        const std::string path = file + "." + cn + "." + std::to_string(g);
+        if (!read_file(path, &fb) || fb.size() < 8) return fail(ALN_E_ARG, "Error opening %s file", path.c_str());
+        uint32_t cnt = 0, ct = 0;
+        memcpy(&cnt, fb.data(), 4);
+        memcpy(&ct, fb.data() + fb.size() - 4, 4);
+        starts.push_back(h.size());
+        if (ct == 3 && fb.size() == 8 + 8ull * cnt) {
+          const size_t at = h.size();
+          h.resize(at + cnt);
+          memcpy(h.data() + at, fb.data() + 4, 8ull * cnt);
+        } else {
+          uint64_t rows = 0;
+          int64_t lo = 0, hi = 0;
+          ALN(aln_segfile_decode(g_ctx, fb.data(), fb.size(), (int64_t *)vals->p, total, &rows, &lo, &hi),
+              path.c_str());
+          ALN(aln_sync(g_ctx), "load");
+          std::vector<int64_t> iv(rows);
+          if (rows && hipMemcpy(iv.data(), vals->p, 8 * rows, hipMemcpyDeviceToHost) != hipSuccess)
+            return fail(ALN_E_HIP, "device to host copy failed");
+          for (int64_t x : iv) {
+            const double d = (double)x / 100.0;
+            int64_t b;
+            memcpy(&b, &d, 8);
+            h.push_back(b);
+          }
+        }
+      }
+      if (h.size() != total)
+        return fail(ALN_E_ARG, "%s: segments hold %llu rows, header %llu", cn.c_str(), (unsigned long long)h.size(),
+                    (unsigned long long)total);
+      starts.push_back(total);
+      if (s->seg_starts.empty())
+        s->seg_starts = starts;
+      else if (s->seg_starts != starts)
+        return fail(ALN_E_ARG, "column %s is segmented differently", cn.c_str());
+      DBufP fv = dalloc(8 * (total ? total : 1));
+      if (!fv) return fail(ALN_E_NOMEM, "device allocation failed");
+      if (total && hipMemcpy(fv->p, h.data(), 8 * total, hipMemcpyHostToDevice) != hipSuccess)
+        return fail(ALN_E_HIP, "host to device copy failed");
+      Col c;
+      TRY(make_plain(fv, total, s->seg_rows, 0, &c));
+      c.type = 1;
+      s->names.push_back(cn);
+      s->cols[cn] = c;
+      continue;
+    }
     // the segments, back to back on the device
     std::vector<uint64_t> starts;
     uint64_t off = 0;
@@ -2427,8 +2563,35 @@ int store_binary(const std::string &var, const std::string &file) {
     DBufP d;
     TRY(plain_copy(c, n, v.seg_rows, &d));
     ColData cd;
-    cd.type = c.dict ? 2 : 0;
-    cd.length = c.dict ? c.width : (uint32_t)c.decimals;
+    cd.type = c.dict ? 2 : c.type == 1 ? 1 : 0;
+    cd.length = c.dict ? c.width : c.type == 1 ? 0 : (uint32_t)c.decimals;
+    if (c.type == 1) {
+      // doubles, uncompressed: [u32 cnt][cnt doubles][u32 comp_type 3] (cm.cu:1150-1180)
+      std::vector<int64_t> h(n);
+      if (n && hipMemcpy(h.data(), d->p, 8 * n, hipMemcpyDeviceToHost) != hipSuccess)
+        return fail(ALN_E_HIP, "device to host copy failed");
+      uint32_t largest = 0;
+      for (uint32_t g = 0; g < nseg; ++g) {
+        const uint64_t r0 = (uint64_t)g * pc;
+        const uint32_t r = (uint32_t)std::min<uint64_t>(pc, n - r0), ct = 3;
+        fb.assign(8 + 8ull * r, 0);
+        memcpy(fb.data(), &r, 4);
+        memcpy(fb.data() + 4, h.data() + r0, 8ull * r);
+        memcpy(fb.data() + 4 + 8ull * r, &ct, 4);
+        largest = std::max(largest, 8 * r);
+        const std::string sp = file + "." + cn + "." + std::to_string(g);
+        if (!write_file(sp, fb.data(), fb.size())) return fail(ALN_E_ARG, "Could not write %s", sp.c_str());
+      }
+      char hdr[20];
+      memcpy(hdr, &n, 8);
+      memcpy(hdr + 8, &nseg, 4);
+      memcpy(hdr + 12, &most, 4);
+      memcpy(hdr + 16, &largest, 4);
+      if (!write_file(file + "." + cn + ".header", hdr, sizeof(hdr)))
+        return fail(ALN_E_ARG, "Could not write %s.%s.header", file.c_str(), cn.c_str());
+      cols[cn] = cd;
+      continue;
+    }
     if (c.dict) {
       // the stored rows' own dictionary, first-seen order (compress_char, cm.cu:1808-1821)
       std::vector<int64_t> h(n);

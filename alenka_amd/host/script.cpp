@@ -28,6 +28,47 @@
 
 namespace {
 
+// The value of an APPROXNUM literal as the reference's lexer computes it (atoff, atof.cu:16-90,
+// Tom Van Baak's published fast atof): the mantissa digits accumulated in doubles, the
+// fraction digit by digit over growing powers of ten, the exponent as a product of 1e50, 1e8
+// and 10 factors (capped at 308), then one division or multiplication.  This is not
+// correctly rounded, and the literal's value decides which rows a comparison keeps, so the
+// same steps are taken.
+double approx_value(const char *p) {
+  double sign = 1.0, value = 0.0, scale = 1.0;
+  if (*p == '-') {
+    sign = -1.0;
+    ++p;
+  } else if (*p == '+') {
+    ++p;
+  }
+  for (; *p >= '0' && *p <= '9'; ++p) value = value * 10.0 + (*p - '0');
+  if (*p == '.') {
+    double pw = 10.0;
+    for (++p; *p >= '0' && *p <= '9'; ++p) {
+      value += (*p - '0') / pw;
+      pw *= 10.0;
+    }
+  }
+  bool neg = false;
+  if (*p == 'e' || *p == 'E') {
+    ++p;
+    if (*p == '-') {
+      neg = true;
+      ++p;
+    } else if (*p == '+') {
+      ++p;
+    }
+    unsigned e = 0;
+    for (; *p >= '0' && *p <= '9'; ++p) e = e * 10 + (unsigned)(*p - '0');
+    if (e > 308) e = 308;
+    for (; e >= 50; e -= 50) scale *= 1E50;
+    for (; e >= 8; e -= 8) scale *= 1E8;
+    for (; e > 0; --e) scale *= 10.0;
+  }
+  return sign * (neg ? value / scale : value * scale);
+}
+
 enum Tok {
   T_END, T_NAME, T_INT, T_DEC, T_FLOAT, T_STR, T_FILE, T_ASSIGN, T_EQUAL, T_NONEQ, T_CMP, T_OR, T_XOR,
   T_AND, T_NOT, T_LIKE, T_PUNCT, T_KW
@@ -270,8 +311,8 @@ struct Parser {
       return advance();
     }
     if (cur.t == T_FLOAT) {
-      error("float literals are not supported by this build");
-      return false;
+      emit_float(approx_value(cur.s.c_str()));
+      return advance();
     }
     if (cur.t == T_KW && (cur.s == "COUNT" || cur.s == "SUM" || cur.s == "AVG" || cur.s == "MIN" || cur.s == "MAX")) {
       const std::string f = cur.s;

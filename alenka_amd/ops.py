@@ -25,6 +25,9 @@ _SIG = {
     "alenka_fetch_char": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_uint64, C.c_uint32]),
     "alenka_drop": (C.c_int, [C.c_char_p]),
     "alenka_last_path": (C.c_char_p, []),
+    "alenka_table_add_float": (C.c_int, [C.c_char_p, C.c_char_p, C.c_void_p]),
+    "alenka_fetch_f64": (C.c_int, [C.c_char_p, C.c_char_p, C.c_void_p, C.c_uint64]),
+    "emit_float": (None, [C.c_double]),
     "emit_name": (None, [C.c_char_p]),
     "emit_number": (None, [C.c_int64]),
     "emit_decimal": (None, [C.c_char_p]),
@@ -87,6 +90,10 @@ class Ops:
             if isinstance(vals, (list, tuple)) and vals and isinstance(vals[0], str):
                 buf = b"".join(s.encode().ljust(extra, b"\0")[:extra] for s in vals)
                 self.check(self.lib.alenka_table_add_char(name.encode(), col.encode(), buf, extra), col)
+            elif extra == "f64":
+                a = np.ascontiguousarray(vals, np.float64)
+                self.check(self.lib.alenka_table_add_float(name.encode(), col.encode(),
+                                                           a.ctypes.data_as(C.c_void_p)), col)
             else:
                 a = np.ascontiguousarray(vals, np.int64)
                 self.check(self.lib.alenka_table_add_int(name.encode(), col.encode(),
@@ -102,6 +109,13 @@ class Ops:
         out = np.empty(max(1, n), np.int64)
         self.check(self.lib.alenka_fetch_int(var.encode(), col.encode(), out.ctypes.data_as(C.c_void_p), n),
                    "fetch")
+        return out[:n]
+
+    def fetch_f64(self, var, col):
+        n = self.rows(var)
+        out = np.empty(max(1, n), np.float64)
+        self.check(self.lib.alenka_fetch_f64(var.encode(), col.encode(), out.ctypes.data_as(C.c_void_p), n),
+                   "fetch_f64")
         return out[:n]
 
     def columns(self, var):

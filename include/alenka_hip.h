@@ -74,11 +74,22 @@ enum {
   ALN_RPN_MINUS = 8,
   ALN_RPN_MUL = 9,
   ALN_RPN_DIV = 10,
-  ALN_RPN_CODESET = 11 /* LIKE operand (filter.cu:452-499): val = device address of a bit set
+  ALN_RPN_CODESET = 11, /* LIKE operand (filter.cu:452-499): val = device address of a bit set
                           over the column's dictionary codes (bit c set when entry c matches
                           the pattern), arg = number of codes it covers; compared with the
                           column by CMP 7 */
+  ALN_RPN_FLOAT = 12   /* emit_float (operators.cu:114-118): val = the IEEE-754 bits of a
+                          double */
 };
+/* col_dec entry of an f64 (type 1, d_columns_float) column slot: its values are the bits of
+ * doubles.  An operation with an f64 operand (an f64 column, a FLOAT, or a value computed
+ * from one) is evaluated in double precision, as CudaSet's float overloads do
+ * (cm.cu:2178-2437, 2440-2773): an int column is converted by a plain cast of its stored
+ * integer (long_to_float_type, cm.cu:137-144), a NUMBER by its decimal value (digits /
+ * 10^precision), arithmetic is IEEE double, comparisons use the EPSILON (1e-8) functors
+ * (cm.h:17, cm.cu:81-134) with the left operand deeper in the stack.  A float expression
+ * compiles to a program whose result holds double bits; its out_prec is ALN_DEC_F64. */
+#define ALN_DEC_F64 (-1)
 typedef struct {
   int32_t type;
   int32_t arg;
@@ -96,8 +107,15 @@ enum {
   ALN_I_CMP = 7, /* a = comparison code, push (l op r)        */
   ALN_I_AND = 8,
   ALN_I_OR = 9,
-  ALN_I_INSET = 10 /* v = pop, push (0 <= v < a and bit v of the uint32 bit set at device
+  ALN_I_INSET = 10, /* v = pop, push (0 <= v < a and bit v of the uint32 bit set at device
                       address k is set) */
+  /* double precision (values are IEEE-754 bits in the int64 stack slots) */
+  ALN_I_I2F = 11,  /* the entry a deep (0 = top, 1 = below it) from int64 to double */
+  ALN_I_FADD = 12, /* r = pop, l = pop, push l + r (double) */
+  ALN_I_FSUB = 13,
+  ALN_I_FMUL = 14,
+  ALN_I_FDIV = 15,
+  ALN_I_FCMP = 16  /* a = comparison code, push the EPSILON comparison of l with r */
 };
 typedef struct {
   int32_t op;
@@ -239,7 +257,10 @@ int aln_join_rows(aln_ctx *ctx, const uint32_t *d_idx, uint64_t nidx, uint64_t n
                   uint32_t *d_rows, uint64_t cap, uint64_t *h_count);
 
 /* ---- group-by SUM/COUNT/MIN/MAX ------------------------------------------------------ */
-enum { ALN_AGG_COUNT = 0, ALN_AGG_SUM = 2, ALN_AGG_MIN = 4, ALN_AGG_MAX = 5 };
+/* F* aggregates take and return f64 bits (SUM in double precision, accumulated in no fixed
+ * order: within 1e-12 relative of a Kahan sum for sums without cancellation). */
+enum { ALN_AGG_COUNT = 0, ALN_AGG_SUM = 2, ALN_AGG_MIN = 4, ALN_AGG_MAX = 5, ALN_AGG_FSUM = 6,
+       ALN_AGG_FMIN = 7, ALN_AGG_FMAX = 8 };
 enum {
   ALN_ORDER_KEYS = 0, /* ascending key tuple: one-segment reference output (cm.cu:910-956) */
   ALN_ORDER_HASH = 1, /* ascending MurmurHash64A(keys, 8*nkeys, seed): multi-segment merge
@@ -262,6 +283,13 @@ int aln_sort_keys(aln_ctx *ctx, const int64_t *const *d_keys, const int32_t *des
 /* First k rows of that stable order (ORDER BY ... LIMIT k). */
 int aln_topk(aln_ctx *ctx, const int64_t *const *d_keys, const int32_t *desc, int nkeys,
              uint64_t n, uint32_t k, uint32_t *d_out_idx, uint32_t *h_nout);
+/* Order-preserving int64 of each double (f64 bits in d_in): a < b as doubles iff
+ * out(a) < out(b) as int64, with -0.0 taken as +0.0 (equal doubles, equal keys); NaNs
+ * order after +inf (sign bit clear) or before -inf (set).  ORDER BY / GROUP BY on f64
+ * columns (update_permutation on d_columns_float, sorts.cu:17-86) sort these keys;
+ * aln_f64_unorder_keys maps them back to the doubles' bits. */
+int aln_f64_order_keys(aln_ctx *ctx, const int64_t *d_in, uint64_t n, int64_t *d_out);
+int aln_f64_unorder_keys(aln_ctx *ctx, const int64_t *d_in, uint64_t n, int64_t *d_out);
 /* Stable radix sort of (u64 key, u32 value) pairs by the low `bits` bits of key. */
 int aln_radix_sort_pairs(aln_ctx *ctx, uint64_t *d_keys, uint32_t *d_vals, uint64_t n,
                          uint32_t bits);

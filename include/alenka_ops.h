@@ -57,6 +57,9 @@ int alenka_table_add_int(const char *table, const char *col, const int64_t *h_va
 /* the same from a device int64 array (copied) */
 int alenka_table_add_int_device(const char *table, const char *col, const int64_t *d_vals,
                                 int32_t decimals, int compress);
+/* f64 (type 1, d_columns_float) column from host doubles, kept uncompressed as the
+ * reference stores non-decimal floats (comp_type 3, cm.cu:1150-1180) */
+int alenka_table_add_float(const char *table, const char *col, const double *h_vals);
 /* char(width) column from nrows fixed-width, NUL-padded host strings */
 int alenka_table_add_char(const char *table, const char *col, const char *h_chars,
                           uint32_t width);
@@ -67,6 +70,7 @@ int alenka_column_info(const char *var, int i, char *buf, int len, int *h_type, 
 /* decoded values of an int/decimal column (or dictionary codes of a char column) */
 int alenka_fetch_int(const char *var, const char *col, int64_t *h_out, uint64_t cap);
 /* fixed-width strings of a char column, width bytes per row */
+int alenka_fetch_f64(const char *var, const char *col, double *h_out, uint64_t cap);
 int alenka_fetch_char(const char *var, const char *col, char *h_out, uint64_t cap_rows,
                       uint32_t width);
 int alenka_drop(const char *var);
@@ -75,6 +79,7 @@ int alenka_drop(const char *var);
 void emit_name(const char *name);
 void emit_number(int64_t val);
 void emit_decimal(const char *str);
+void emit_float(double val); /* operators.h:39: an APPROXNUM (1.5E3) literal */
 void emit_string(const char *str);
 void emit_cmp(int val);
 void emit_and(void);

@@ -375,7 +375,7 @@ int64_t orc_pfor_decode(const uint8_t *buf, uint64_t len, int64_t *out, uint64_t
  *   - NUMBER CMP NAME scales by the raw precisions, not their difference (filter.cu:598);
  *   - constant folding NUMBER NUMBER MINUS/DIV uses the top of stack as left operand.   */
 
-enum { T_NAME = 1, T_NUMBER = 2, T_STRING = 3 };
+enum { T_NAME = 1, T_NUMBER = 2, T_STRING = 3, T_FLOAT = 5 };
 #define SMAX 64
 
 typedef struct {
@@ -389,6 +389,8 @@ typedef struct {
   int64_t *vecs[SMAX]; int nvecs;
   int64_t strs[SMAX]; int nstrs;        /* dictionary codes of STRING literals          */
   uint8_t *bools[SMAX]; int nbools;
+  double fnums[SMAX]; int nfnums;       /* FLOAT operands                                */
+  int vfloat[SMAX];                     /* per vecs entry: holds double bits            */
   int64_t *owned[4 * SMAX]; int nowned;
   int mode; /* 0: filter.cu, 1: select.cu */
   int err;
@@ -463,6 +465,122 @@ static const int64_t *get_vec(rpn_state *s, int v) {
   if (s->nvecs == 0) { s->err = ALN_E_ARG; return NULL; }
   return s->vecs[--s->nvecs];
 }
+
+/* ---- double precision: CudaSet's float overloads (cm.cu:2178-2437, 2440-2773) ---------
+ * An operation with a double operand - a FLOAT (emit_float, operators.cu:114-118), an f64
+ * column (col_dec ALN_DEC_F64) or a vector computed from one - is evaluated in doubles:
+ * an int column converts by a plain cast of the stored integer (long_to_float_type,
+ * cm.cu:137-144), a NUMBER by digits / 10^precision; + - * / are IEEE double; comparisons
+ * are the EPSILON functors (cm.h:17; f_less .. f_not_equal_to, cm.cu:81-134) applied as
+ * (left = deeper operand, right = top). */
+static double dbits(int64_t b) { double x; memcpy(&x, &b, 8); return x; }
+static int64_t bitsd(double x) { int64_t b; memcpy(&b, &x, 8); return b; }
+static int is_f64_col(const rpn_state *s, int v) { return v >= 0 && s->dec[v] == ALN_DEC_F64; }
+
+static int f_compare(int code, double x, double y) {
+  const double e = 1.0E-8;
+  switch (code) {
+  case 1: return (y - x) > e;                                     /* f_less             */
+  case 2: return (x - y) > e;                                     /* f_greater          */
+  case 3: return (x - y) > e || (x - y) < -e;                     /* f_not_equal_to     */
+  case 4: return (x - y) < e && (x - y) > -e;                     /* f_equal_to         */
+  case 5: return (y - x) > e || ((x - y) < e && (x - y) > -e);    /* f_less_equal       */
+  case 6: return (x - y) > e || ((x - y) < e && (x - y) > -e);    /* f_greater_equal_to */
+  default: return 0;
+  }
+}
+
+/* whether either of the two top operands is a double, without popping */
+static int float_operands(const rpn_state *s) {
+  if (s->ntype < 2) return 0;
+  int vi = s->nvalue, ci = s->nvecs;
+  for (int k = 0; k < 2; ++k) {
+    int t = s->type[s->ntype - 1 - k];
+    if (t == T_FLOAT) return 1;
+    if (t == T_NAME || t == T_STRING) {
+      if (vi == 0) return 0;
+      int v = s->value[--vi];
+      if (t != T_NAME) continue;
+      if (v >= 0) { if (is_f64_col(s, v)) return 1; }
+      else if (ci > 0 && s->vfloat[--ci]) return 1;
+    }
+  }
+  return 0;
+}
+
+typedef struct { const int64_t *vec; int is_float, is_const; double c; } f_operand;
+
+static f_operand f_pop(rpn_state *s, int t) {
+  f_operand o = {NULL, 0, 0, 0.0};
+  if (t == T_FLOAT) {
+    o.is_const = 1;
+    o.c = POP(s->fnums, s->nfnums);
+  } else if (t == T_NUMBER) {
+    int64_t n = POP(s->nums, s->nnums);
+    uint32_t p = POP(s->prec, s->nprec);
+    double d = 1.0;
+    if (p > 18) { s->err = ALN_E_UNSUPPORTED; return o; }
+    for (uint32_t i = 0; i < p; ++i) d *= 10.0;
+    o.is_const = 1;
+    o.c = (double)n / d;
+  } else if (t == T_NAME) {
+    int v = POP(s->value, s->nvalue);
+    if (v >= 0) {
+      o.vec = s->cols[v];
+      o.is_float = is_f64_col(s, v);
+    } else {
+      if (s->nvecs == 0) { s->err = ALN_E_UNSUPPORTED; return o; }
+      o.is_float = s->vfloat[s->nvecs - 1];
+      o.vec = get_vec(s, v);
+      (void)POP(s->prec, s->nprec);
+    }
+  } else {
+    s->err = ALN_E_UNSUPPORTED;
+  }
+  return o;
+}
+
+static double f_at(const f_operand *o, uint64_t i) {
+  if (o->is_const) return o->c;
+  return o->is_float ? dbits(o->vec[i]) : (double)o->vec[i];
+}
+
+static double f_op(int op, double l, double r) {
+  switch (op) {
+  case ALN_RPN_ADD: return l + r;
+  case ALN_RPN_MINUS: return l - r;
+  case ALN_RPN_MUL: return l * r;
+  default: return l / r;
+  }
+}
+
+static void f_arith(rpn_state *s, int op) {
+  int s1 = POP(s->type, s->ntype), s2 = POP(s->type, s->ntype);
+  f_operand r = f_pop(s, s1), l = f_pop(s, s2);
+  if (s->err) return;
+  if (l.is_const && r.is_const) {
+    s->fnums[s->nfnums++] = f_op(op, l.c, r.c);
+    s->type[s->ntype++] = T_FLOAT;
+    return;
+  }
+  int64_t *out = new_vec(s);
+  for (uint64_t i = 0; i < s->n; ++i) out[i] = bitsd(f_op(op, f_at(&l, i), f_at(&r, i)));
+  s->prec[s->nprec++] = 0;
+  s->type[s->ntype++] = T_NAME;
+  s->value[s->nvalue++] = -1;
+  s->vfloat[s->nvecs] = 1;
+  s->vecs[s->nvecs++] = out;
+}
+
+static void f_cmp(rpn_state *s, int code) {
+  int s1 = POP(s->type, s->ntype), s2 = POP(s->type, s->ntype);
+  f_operand r = f_pop(s, s1), l = f_pop(s, s2);
+  if (s->err) return;
+  uint8_t *out = new_bool(s);
+  for (uint64_t i = 0; i < s->n; ++i) out[i] = (uint8_t)f_compare(code, f_at(&l, i), f_at(&r, i));
+  s->type[s->ntype++] = T_NAME;
+  s->bools[s->nbools++] = out;
+}
 static uint32_t get_decimals(rpn_state *s, int v) {
   if (v >= 0) return (uint32_t)s->dec[v];
   if (s->nprec == 0) { s->err = ALN_E_ARG; return 0; }
@@ -502,6 +620,7 @@ static void op_col_col(rpn_state *s, const int64_t *c1, const int64_t *c2, int o
 }
 
 static void rpn_arith(rpn_state *s, int op) {
+  if (float_operands(s)) { f_arith(s, op); return; }
   int s1 = POP(s->type, s->ntype), s2 = POP(s->type, s->ntype);
   if (s1 == T_NUMBER && s2 == T_NUMBER) {
     int64_t n1 = POP(s->nums, s->nnums), n2 = POP(s->nums, s->nnums);
@@ -516,6 +635,7 @@ static void rpn_arith(rpn_state *s, int op) {
     for (uint64_t i = 0; i < s->n; ++i) v[i] = res;
     s->type[s->ntype++] = T_NAME;
     s->value[s->nvalue++] = -1;
+    s->vfloat[s->nvecs] = 0;
     s->vecs[s->nvecs++] = v;
   } else if (s1 == T_NAME && s2 == T_NUMBER) {
     int v1 = POP(s->value, s->nvalue);
@@ -533,6 +653,7 @@ static void rpn_arith(rpn_state *s, int op) {
       if (s->mode == 0) op_col_const(s, t, n1, op, 1, p1, p2, out);
       else op_col_const(s, t, n1, op, 1, p2, p1, out);
     }
+    s->vfloat[s->nvecs] = 0;
     s->vecs[s->nvecs++] = out;
   } else if (s1 == T_NUMBER && s2 == T_NAME) {
     int64_t n1 = POP(s->nums, s->nnums);
@@ -546,6 +667,7 @@ static void rpn_arith(rpn_state *s, int op) {
     s->value[s->nvalue++] = -1;
     int64_t *out = new_vec(s);
     if (t) op_col_const(s, t, n1, op, 0, p2, p1, out);
+    s->vfloat[s->nvecs] = 0;
     s->vecs[s->nvecs++] = out;
   } else if (s1 == T_NAME && s2 == T_NAME) {
     int v1 = POP(s->value, s->nvalue), v2 = POP(s->value, s->nvalue);
@@ -557,6 +679,7 @@ static void rpn_arith(rpn_state *s, int op) {
     s->value[s->nvalue++] = -1;
     int64_t *out = new_vec(s);
     if (t && t1) op_col_col(s, t, t1, op, p2, p1, out);
+    s->vfloat[s->nvecs] = 0;
     s->vecs[s->nvecs++] = out;
   } else {
     s->err = ALN_E_UNSUPPORTED; /* arithmetic on strings: no result in the reference */
@@ -573,7 +696,9 @@ static void cmp_col_const(rpn_state *s, const int64_t *col, int64_t d, int code,
 }
 
 static void rpn_cmp(rpn_state *s, int code) {
+  if (code >= 1 && code <= 6 && float_operands(s)) { f_cmp(s, code); return; }
   int s1 = POP(s->type, s->ntype), s2 = POP(s->type, s->ntype);
+  if (s1 == T_FLOAT || s2 == T_FLOAT) { s->err = ALN_E_UNSUPPORTED; return; }
   uint8_t *out = new_bool(s);
   if (s1 == T_NUMBER && s2 == T_NUMBER) {
     int64_t n1 = POP(s->nums, s->nnums), n2 = POP(s->nums, s->nnums);
@@ -657,6 +782,10 @@ static int rpn_run(rpn_state *s, const aln_rpn *rpn, int nrpn, int mode) {
       s->type[s->ntype++] = T_NUMBER;
       s->prec[s->nprec++] = (uint32_t)o->arg;
       break;
+    case ALN_RPN_FLOAT:
+      s->fnums[s->nfnums++] = dbits(o->val);
+      s->type[s->ntype++] = T_FLOAT;
+      break;
     case ALN_RPN_STRING:
       s->strs[s->nstrs] = o->val;
       s->value[s->nvalue++] = -2 - s->nstrs;
@@ -682,7 +811,7 @@ static int rpn_run(rpn_state *s, const aln_rpn *rpn, int nrpn, int mode) {
       s->err = ALN_E_ARG;
     }
     if (s->ntype >= SMAX - 1 || s->nvalue >= SMAX - 1 || s->nnums >= SMAX - 1 ||
-        s->nstrs >= SMAX - 1 ||
+        s->nstrs >= SMAX - 1 || s->nfnums >= SMAX - 1 ||
         s->nprec >= SMAX - 1 || s->nvecs >= SMAX - 1 || s->nbools >= SMAX - 1)
       s->err = ALN_E_ARG;
   }
@@ -713,7 +842,10 @@ int orc_eval(const aln_rpn *rpn, int nrpn, const int64_t *const *cols, const int
   s->cols = cols; s->dec = col_dec; s->n = n;
   int err = rpn_run(s, rpn, nrpn, 1);
   if (!err) {
-    if (s->nnums > 0 && s->ntype > 0 && s->type[s->ntype - 1] == T_NUMBER) {
+    if (s->nfnums > 0 && s->ntype > 0 && s->type[s->ntype - 1] == T_FLOAT) {
+      for (uint64_t i = 0; i < n; ++i) out[i] = bitsd(s->fnums[s->nfnums - 1]);
+      *out_prec = ALN_DEC_F64;
+    } else if (s->nnums > 0 && s->ntype > 0 && s->type[s->ntype - 1] == T_NUMBER) {
       for (uint64_t i = 0; i < n; ++i) out[i] = s->nums[s->nnums - 1];
       *out_prec = (int32_t)s->prec[s->nprec - 1];
     } else if (s->nvalue > 0 && s->value[s->nvalue - 1] >= 0) {
@@ -722,7 +854,7 @@ int orc_eval(const aln_rpn *rpn, int nrpn, const int64_t *const *cols, const int
       *out_prec = col_dec[c];
     } else if (s->nvecs > 0) {
       memcpy(out, s->vecs[s->nvecs - 1], n * 8);
-      *out_prec = (int32_t)s->prec[s->nprec - 1];
+      *out_prec = s->vfloat[s->nvecs - 1] ? ALN_DEC_F64 : (int32_t)s->prec[s->nprec - 1];
     } else {
       err = ALN_E_ARG;
     }
@@ -766,6 +898,9 @@ int orc_zone_map_check(const aln_rpn *rpn, int nrpn, const int64_t *cmin, const 
   for (int i = 0; i < nrpn; ++i) {
     if (rpn[i].type == ALN_RPN_NAME && rpn[i].arg + 1 > maxc) maxc = rpn[i].arg + 1;
     if (rpn[i].type >= ALN_RPN_ADD && rpn[i].type <= ALN_RPN_DIV) { *out = 'R'; return 0; }
+    /* double operands: every segment is evaluated (no float zone maps are kept) */
+    if (rpn[i].type == ALN_RPN_FLOAT ||
+        (rpn[i].type == ALN_RPN_NAME && col_dec && col_dec[rpn[i].arg] == ALN_DEC_F64)) { *out = 'R'; return 0; }
   }
   int64_t mm[ALN_MAX_COLS][2];
   if (maxc > ALN_MAX_COLS) return ALN_E_ARG;

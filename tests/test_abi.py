@@ -4,6 +4,9 @@ oracle.  Compiled programs are run by a small Python interpreter of the instruct
 import os
 import re
 
+import math
+import struct
+
 import numpy as np
 import pytest
 
@@ -34,8 +37,24 @@ def test_no_oracle_in_product():
     assert b"liboracle" not in data and b"orc_" not in data
 
 
+def f2b(x):
+    return struct.unpack("<q", struct.pack("<d", x))[0]
+
+
+def b2f(b):
+    return struct.unpack("<d", struct.pack("<q", b))[0]
+
+
+def fcmp(code, x, y):
+    e = 1e-8
+    eq = (x - y) < e and (x - y) > -e
+    return {1: (y - x) > e, 2: (x - y) > e, 3: (x - y) > e or (x - y) < -e, 4: eq,
+            5: (y - x) > e or eq, 6: (x - y) > e or eq}[code]
+
+
 def run_prog(prog, cols):
-    """Reference interpreter of aln_insn programs (int64 wrap-around semantics)."""
+    """Reference interpreter of aln_insn programs (int64 wrap-around semantics; double
+    instructions on the IEEE bits held in the stack slots)."""
     M = (1 << 64) - 1
 
     def wrap(x):
@@ -51,6 +70,21 @@ def run_prog(prog, cols):
                 st.append(wrap(int(cols[a][i]) * k))
             elif op == L.I_CONST:
                 st.append(k)
+            elif op == L.I_I2F:
+                st[-1 - a] = f2b(float(st[-1 - a]))
+            elif op >= L.I_FADD:
+                r, l = b2f(st.pop()), b2f(st.pop())
+                if op == L.I_FADD:
+                    st.append(f2b(l + r))
+                elif op == L.I_FSUB:
+                    st.append(f2b(l - r))
+                elif op == L.I_FMUL:
+                    st.append(f2b(l * r))
+                elif op == L.I_FDIV:
+                    st.append(f2b(l / r if r != 0 else (math.copysign(math.inf, l) * math.copysign(1, r)
+                                                        if l != 0 and not math.isnan(l) else math.nan)))
+                else:
+                    st.append(int(fcmp(a, l, r)))
             else:
                 r, l = st.pop(), st.pop()
                 if op == L.I_ADD:
@@ -212,3 +246,82 @@ def test_operator_layer_exports_every_declared_symbol():
     for n in ("emit_filter", "emit_select", "emit_join", "emit_order", "emit_store", "emit_name",
               "emit_number", "emit_cmp", "emit_sum", "emit_count", "emit_sel_name"):
         assert n in names
+
+
+FLT = L.RPN_FLOAT
+
+
+def random_f64_rpn(r, ncols, fcols):
+    """A random predicate mixing f64 columns (slots in fcols), int columns, FLOATs and
+    decimal NUMBERs."""
+    def operand(d):
+        c = r.random()
+        if c < 0.4 or d > 1:
+            return [(NAME, int(r.integers(0, ncols)), 0)]
+        if c < 0.6:
+            return [(FLT, 0, f2b(float(r.choice([1.5, -2.25, 0.0, 3e-9, 100.125, 7.0]))))]
+        if c < 0.8:
+            return [(NUM, int(r.integers(0, 3)), int(r.integers(-200, 200)))]
+        return operand(d + 1) + operand(d + 1) + [(int(r.choice([ADD, MINUS, MUL, DIV])), 0, 0)]
+
+    def pred(d):
+        if d < 2 and r.random() < 0.4:
+            return pred(d + 1) + pred(d + 1) + [(int(r.choice([AND, OR])), 0, 0)]
+        a, b = operand(0), operand(0)
+        if not any(t == FLT or (t == NAME and x in fcols) for t, x, _ in a + b):
+            b = [(FLT, 0, f2b(0.5))]
+        return a + b + [(CMP, int(r.integers(1, 7)), 0)]
+
+    return pred(0)
+
+
+def test_compiled_f64_predicates_match_oracle():
+    """Programs with double operands, compiled by the C ABI and run by the interpreter above,
+    give the oracle's masks (EPSILON comparisons, plain casts of int columns)."""
+    r = np.random.default_rng(21)
+    f = r.choice([1.5, 1.5 + 5e-9, 1.5 - 2e-8, -2.25, 0.0, 7.0, 100.125], 64) + r.normal(0, 1, 64) * (r.random(64) < 0.5)
+    cols = [np.array([f2b(x) for x in f], np.int64), r.integers(-300, 300, 64),
+            np.array([f2b(x) for x in r.normal(0, 50, 64)], np.int64), r.integers(-300, 300, 64)]
+    dec = [L.DEC_F64, 0, L.DEC_F64, 2]
+    checked = 0
+    for _ in range(400):
+        ops = random_f64_rpn(r, 4, (0, 2))
+        try:
+            want = O.filter_mask(ops, cols, dec)
+        except L.AlnError:
+            with pytest.raises(L.AlnError):
+                L.compile_rpn(ops, dec)
+            continue
+        prog, _ = L.compile_rpn(ops, dec)
+        got = run_prog(prog, cols).astype(bool)
+        assert np.array_equal(got, want), ops
+        checked += 1
+    assert checked > 200
+
+
+def test_compiled_f64_expressions_match_oracle():
+    r = np.random.default_rng(22)
+    cols = [np.array([f2b(x) for x in r.normal(0, 100, 64)], np.int64), r.integers(-1000, 1000, 64)]
+    dec = [L.DEC_F64, 2]
+    for _ in range(300):
+        a = [(NAME, int(r.integers(0, 2)), 0)]
+        b = [(FLT, 0, f2b(float(r.normal(0, 10))))] if r.random() < 0.6 else [(NUM, int(r.integers(0, 3)), int(r.integers(-50, 50)))]
+        ops = (a + b if r.random() < 0.5 else b + a) + [(int(r.choice([ADD, MINUS, MUL, DIV])), 0, 0)]
+        if r.random() < 0.5:
+            ops = ops + [(NAME, 0, 0), (int(r.choice([ADD, MINUS, MUL])), 0, 0)]
+        want, wp = O.eval_expr(ops, cols, dec)
+        prog, p = L.compile_rpn(ops, dec, mode=1)
+        assert p == wp
+        got = run_prog(prog, cols)
+        if p == L.DEC_F64:  # NaN bit patterns may differ; compare as doubles
+            assert np.array_equal(got.view(np.float64), want.view(np.float64), equal_nan=True), ops
+        else:
+            assert np.array_equal(got, want), ops
+
+
+def test_f64_predicates_are_not_zone_mapped():
+    dec = [L.DEC_F64, 0]
+    assert L.zone_map_check([(NAME, 0, 0), (FLT, 0, f2b(2.0)), (CMP, 1, 0)], [0, 0], [1, 1], dec) == "R"
+    assert L.zone_map_check([(NAME, 1, 0), (FLT, 0, f2b(2.0)), (CMP, 1, 0)], [0, 0], [1, 1], dec) == "R"
+    assert O.zone_map_check([(NAME, 1, 0), (NUM, 0, 2), (CMP, 1, 0)], [0, 0], [1, 1], dec) == "A"
+    assert O.zone_map_check([(NAME, 0, 0), (FLT, 0, f2b(2.0)), (CMP, 1, 0)], [0, 0], [1, 1], dec) == "R"

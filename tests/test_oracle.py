@@ -486,3 +486,67 @@ def test_kahan_sum_exact_cases():
     v = np.concatenate([x, -x, [0.125]])
     r.shuffle(v)
     assert abs(O.kahan_sum(v) - 0.125) <= 2 * 2.0 ** -53 * float(np.abs(v).sum())
+
+
+# ---- f64 (type 1) operands: CudaSet's float overloads (cm.cu:2178-2437, 2440-2773) ---------
+def _bits(a):
+    return np.ascontiguousarray(a, np.float64).view(np.int64)
+
+
+FCODES = {1: lambda x, y: (y - x) > 1e-8, 2: lambda x, y: (x - y) > 1e-8,
+          3: lambda x, y: ((x - y) > 1e-8) | ((x - y) < -1e-8),
+          4: lambda x, y: ((x - y) < 1e-8) & ((x - y) > -1e-8),
+          5: lambda x, y: ((y - x) > 1e-8) | (((x - y) < 1e-8) & ((x - y) > -1e-8)),
+          6: lambda x, y: ((x - y) > 1e-8) | (((x - y) < 1e-8) & ((x - y) > -1e-8))}
+
+
+def test_f64_compare_closed_form():
+    """EPSILON comparisons (f_less .. f_not_equal_to, cm.cu:81-134) of an f64 column with a
+    FLOAT, with another f64 column, with an int column (plain cast, long_to_float_type) and
+    with a decimal NUMBER, written out with numpy; values sit inside, on and outside the
+    1e-8 band around the literal."""
+    r = np.random.default_rng(31)
+    lit = 2.5
+    f = lit + r.choice([0.0, 5e-9, -5e-9, 1e-8, -1e-8, 2e-8, -2e-8, 1.0, -1.0], 500)
+    g = f + r.choice([0.0, 3e-9, -3e-9, 3e-8], 500)
+    k = r.integers(-5, 6, 500)  # int column (decimal scale 2: 2.5 is stored as 250)
+    cols = [_bits(f), _bits(g), k]
+    dec = [-1, -1, 2]
+    FL = (12, 0, int(_bits([lit])[0]))
+    for code, fn in FCODES.items():
+        # column OP FLOAT, FLOAT OP column (the left operand is the deeper one)
+        assert np.array_equal(O.filter_mask([(1, 0, 0), FL, (4, code, 0)], cols, dec), fn(f, lit))
+        assert np.array_equal(O.filter_mask([FL, (1, 0, 0), (4, code, 0)], cols, dec), fn(lit, f))
+        assert np.array_equal(O.filter_mask([(1, 0, 0), (1, 1, 0), (4, code, 0)], cols, dec), fn(f, g))
+        # an int column converts by a plain cast of the stored integer
+        assert np.array_equal(O.filter_mask([(1, 2, 0), (1, 0, 0), (4, code, 0)], cols, dec),
+                              fn(k.astype(np.float64), f))
+        # a NUMBER by its decimal value: 25 at precision 1 is 2.5
+        assert np.array_equal(O.filter_mask([(1, 0, 0), (2, 1, 25), (4, code, 0)], cols, dec), fn(f, 2.5))
+
+
+def test_f64_arithmetic_closed_form():
+    """Double arithmetic in the stack's order (left = deeper operand), bit-equal to numpy's
+    IEEE operations, with FLOAT FLOAT folded."""
+    r = np.random.default_rng(32)
+    f = r.normal(0, 100, 300)
+    k = r.integers(-1000, 1000, 300)
+    cols = [_bits(f), k]
+    dec = [-1, 2]
+    FL = lambda x: (12, 0, int(_bits([x])[0]))  # noqa: E731
+    cases = [
+        ([(1, 0, 0), FL(1.5), (7, 0, 0)], f + 1.5),
+        ([FL(1.5), (1, 0, 0), (8, 0, 0)], 1.5 - f),
+        ([(1, 0, 0), (1, 1, 0), (9, 0, 0)], f * k.astype(np.float64)),
+        ([(1, 1, 0), (1, 0, 0), (10, 0, 0)], k.astype(np.float64) / f),
+        ([(1, 0, 0), (2, 2, 125), (8, 0, 0)], f - 1.25),
+        ([(1, 0, 0), FL(2.0), (9, 0, 0), (1, 1, 0), (7, 0, 0)], f * 2.0 + k.astype(np.float64)),
+        ([FL(1.5), FL(2.25), (9, 0, 0)], np.full(300, 1.5 * 2.25)),
+    ]
+    for ops, want in cases:
+        got, prec = O.eval_expr(ops, cols, dec)
+        assert prec == -1
+        assert np.array_equal(got, _bits(want)), ops
+    # an int-only expression keeps the int path and its scale
+    got, prec = O.eval_expr([(1, 1, 0), (2, 0, 3), (9, 0, 0)], cols, dec)
+    assert prec == 2 and np.array_equal(got, k * 3)
