@@ -3,13 +3,17 @@
 // The reference joins by sorting: the right (build) side is stably sorted by key
 // (operators.cu:1117-1157), each left (probe) segment is sorted carrying its row ids
 // (operators.cu:1184-1199), and moderngpu inner_join merges them into (left, right) pairs
-// (operators.cu:1239-1284).  Here the build side goes into an open-addressing table keyed
-// by MurmurHash64A (linear probing), and each probe row is looked up directly:
-//   * unique build keys (the primary-key case): a slot holds the build row;
+// (operators.cu:1239-1284).  Here each probe row is looked up directly in a structure over
+// the build side:
+//   * unique build keys whose range is at most 64 per build row (the primary-key case): a
+//     table with one 8-byte entry per 32 keys (bits present, rank base) and the build rows
+//     in key order;
+//   * other unique keys: an open-addressing table keyed by MurmurHash64A (linear probing), a
+//     slot holding the build row;
 //   * duplicate build keys: the build side is radix-sorted by key (stable, so equal keys
 //     keep ascending row order) and a slot holds a run of equal keys.
 // Pairs come out in probe-row order and, for one probe row, in ascending build-row order;
-// the count pass records each probe row's slot so the write pass does not probe again.
+// the count pass records each probe row's match so the write pass does not probe again.
 #include <string.h>
 
 #include "aln_internal.h"
@@ -163,6 +167,245 @@ __global__ void k_join_runs(const uint64_t *code, const uint32_t *head, const ui
   }
 }
 
+// ---- dense unique build keys: bit + rank table -----------------------------------------------
+// When the build keys are unique and their range is at most JD_DENSITY per build row, the
+// build side becomes a table over the key range with one 8-byte entry per 32 keys: the bits
+// of the keys present and the number of keys before the entry (its rank base), plus the
+// build rows in key order (rowid[rank]).  At 10M build keys over a 20M range the table is
+// 5 MB (L2 / MALL resident): a probe reads one entry, and a hit one row id.
+//   build: a byte per key (plain byte stores, no atomics: every XCD's L2 sees them merged at
+//          write-back), packed into the entries with a block scan of the counts, then the row
+//          ids scattered to their ranks;
+//   probe: pass 0 streams the probe keys and writes each row's build row (or none) and the
+//          tile's pair count; pass 1 streams those and writes the pairs of a tile through LDS,
+//          so the stores to the output are whole, contiguous lines.
+constexpr uint64_t JD_DENSITY = 64;  // key range <= 64 x build rows (and <= 2^31)
+constexpr int JD_BLOCK = 256;        // entries (32 keys each) per block of the packing pass
+
+__global__ void __launch_bounds__(256) k_jd_bytes(const int64_t *__restrict__ keys, uint64_t n, int64_t kmin,
+                                                  uint8_t *bytes) {
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256)
+    bytes[(uint64_t)keys[i] - (uint64_t)kmin] = 1;
+}
+
+// Entry w: the bits of keys 32w .. 32w+31 from the byte map; pass 0 writes the keys per
+// block, pass 1 the entries with their rank bases (block offset + keys of earlier entries).
+template <int PASS>
+__global__ void __launch_bounds__(JD_BLOCK) k_jd_pack(const uint8_t *__restrict__ bytes, uint64_t nent,
+                                                      uint32_t *__restrict__ bcount, const uint64_t *__restrict__ boff,
+                                                      uint2 *__restrict__ ent) {
+  __shared__ uint32_t wsum[JD_BLOCK / 64];
+  const uint64_t w = (uint64_t)blockIdx.x * JD_BLOCK + threadIdx.x;
+  uint32_t b = 0;
+  if (w < nent) {
+    const uint4 *q = reinterpret_cast<const uint4 *>(bytes + w * 32);
+    const uint4 x = q[0], y = q[1];
+    const uint32_t v[8] = {x.x, x.y, x.z, x.w, y.x, y.y, y.z, y.w};
+#pragma unroll
+    for (int k = 0; k < 8; ++k)  // byte j of word k is 0 or 1: gather bit 0 of each byte
+      b |= (((v[k] & 1u) | ((v[k] >> 7) & 2u) | ((v[k] >> 14) & 4u) | ((v[k] >> 21) & 8u))) << (4 * k);
+  }
+  const uint32_t n = (uint32_t)__popc(b);
+  const unsigned lane = __lane_id(), wid = threadIdx.x / 64;
+  uint32_t x = n;
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= (unsigned)o) x += y;
+  }
+  if (lane == 63) wsum[wid] = x;
+  __syncthreads();
+  if (PASS == 0) {
+    if (threadIdx.x == 0) {
+      uint32_t t = 0;
+      for (int k = 0; k < JD_BLOCK / 64; ++k) t += wsum[k];
+      bcount[blockIdx.x] = t;
+    }
+    return;
+  }
+  uint32_t before = 0;
+  for (int k = 0; k < (int)wid; ++k) before += wsum[k];
+  if (w < nent) ent[w] = make_uint2(b, (uint32_t)boff[blockIdx.x] + before + x - n);
+}
+
+// A key's build row (J_EMPTY: none) from its code.
+__device__ __forceinline__ uint32_t jd_find(const uint2 *__restrict__ ent, const uint32_t *__restrict__ rowid,
+                                            uint32_t c) {
+  const uint2 e = ent[c >> 5];
+  const uint32_t bit = 1u << (c & 31);
+  if (!(e.x & bit)) return J_EMPTY;
+  return rowid[e.y + (uint32_t)__popc(e.x & (bit - 1u))];
+}
+
+__global__ void __launch_bounds__(256) k_jd_rows(const int64_t *__restrict__ keys, uint64_t n, int64_t kmin,
+                                                 const uint2 *__restrict__ ent, uint32_t *rowid) {
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
+    const uint32_t c = (uint32_t)((uint64_t)keys[i] - (uint64_t)kmin);
+    const uint2 e = ent[c >> 5];
+    rowid[e.y + (uint32_t)__popc(e.x & ((1u << (c & 31)) - 1u))] = (uint32_t)i;
+  }
+}
+
+// Pass 0: each probe row's build row into match (coalesced), and the tile's pair count.  A
+// thread's loads go out stage by stage for all of its rows at once - the keys, then their
+// entries, then the row ids of the hits - so a thread waits three round trips, not one per
+// row and level (rows outside the key range read entry 0 and are masked afterwards).
+__global__ void __launch_bounds__(J_BLOCK) k_jd_match(const int64_t *__restrict__ pk, uint64_t np,
+                                                      const uint2 *__restrict__ ent,
+                                                      const uint32_t *__restrict__ rowid, int64_t kmin,
+                                                      uint64_t range, uint32_t *__restrict__ match,
+                                                      uint32_t *__restrict__ tcount) {
+  __shared__ uint32_t wsum[J_BLOCK / 64];
+  const uint64_t t0 = (uint64_t)blockIdx.x * J_TILE;
+  int64_t key[J_ITEMS];
+#pragma unroll
+  for (int j = 0; j < J_ITEMS; ++j) {  // striped: row t0 + j * J_BLOCK + thread (coalesced)
+    const uint64_t p = t0 + (uint64_t)j * J_BLOCK + threadIdx.x;
+    key[j] = p < np ? __builtin_nontemporal_load(pk + p) : kmin - 1;
+  }
+  uint32_t code[J_ITEMS];
+  uint2 e[J_ITEMS];
+#pragma unroll
+  for (int j = 0; j < J_ITEMS; ++j) {
+    const uint64_t c = (uint64_t)key[j] - (uint64_t)kmin;
+    code[j] = c < range ? (uint32_t)c : 0xFFFFFFFFu;
+    e[j] = ent[code[j] != 0xFFFFFFFFu ? code[j] >> 5 : 0u];
+  }
+  uint32_t m[J_ITEMS], n = 0;
+#pragma unroll
+  for (int j = 0; j < J_ITEMS; ++j) {
+    const uint32_t bit = 1u << (code[j] & 31);
+    const bool h = code[j] != 0xFFFFFFFFu && (e[j].x & bit);
+    m[j] = rowid[h ? e[j].y + (uint32_t)__popc(e[j].x & (bit - 1u)) : 0u];
+    m[j] = h ? m[j] : J_EMPTY;
+    n += h;
+  }
+#pragma unroll
+  for (int j = 0; j < J_ITEMS; ++j) {
+    const uint64_t p = t0 + (uint64_t)j * J_BLOCK + threadIdx.x;
+    if (p < np) __builtin_nontemporal_store(m[j], match + p);
+  }
+  for (int o = 32; o > 0; o >>= 1) n += __shfl_xor(n, o, 64);
+  if (__lane_id() == 0) wsum[threadIdx.x / 64] = n;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int k = 0; k < J_BLOCK / 64; ++k) t += wsum[k];
+    tcount[blockIdx.x] = t;
+  }
+}
+
+// Pass 1: a tile's pairs in probe-row order, compacted in LDS, then written as contiguous
+// runs of the two output arrays.
+__global__ void __launch_bounds__(J_BLOCK) k_jd_write(const uint32_t *__restrict__ match, uint64_t np,
+                                                      const uint64_t *__restrict__ toff, uint32_t *__restrict__ out_p,
+                                                      uint32_t *__restrict__ out_b) {
+  __shared__ uint32_t sp[J_TILE], sb[J_TILE];
+  __shared__ uint32_t wsum[J_BLOCK / 64];
+  const uint64_t t0 = (uint64_t)blockIdx.x * J_TILE;
+  const unsigned lane = __lane_id(), wid = threadIdx.x / 64;
+  uint32_t base = 0;  // pairs of the tile before this round
+#pragma unroll
+  for (int j = 0; j < J_ITEMS; ++j) {  // round j: rows t0 + j * J_BLOCK .. + J_BLOCK - 1, in order
+    const uint64_t p = t0 + (uint64_t)j * J_BLOCK + threadIdx.x;
+    const uint32_t m = p < np ? match[p] : J_EMPTY;
+    const bool h = m != J_EMPTY;
+    const uint64_t bal = __builtin_amdgcn_ballot_w64(h);
+    if (lane == 0) wsum[wid] = (uint32_t)__popcll(bal);
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+#pragma unroll
+    for (int k = 0; k < J_BLOCK / 64; ++k) {
+      const uint32_t c = wsum[k];
+      before += k < (int)wid ? c : 0u;
+      all += c;
+    }
+    if (h) {
+      const uint32_t q = base + before + (uint32_t)__popcll(bal & lanemask_lt());
+      sp[q] = (uint32_t)p;
+      sb[q] = m;
+    }
+    base += all;
+    __syncthreads();
+  }
+  const uint64_t o = toff[blockIdx.x];
+  for (uint32_t i = threadIdx.x; i < base; i += J_BLOCK) {
+    out_p[o + i] = sp[i];
+    out_b[o + i] = sb[i];
+  }
+}
+
+// The pairs from each probe row's build row (match) and the tiles' pair counts.
+static int join_write_pairs(aln_ctx *c, const uint32_t *match, uint64_t np, const uint32_t *tcount, uint64_t *toff,
+                            uint64_t ntiles, uint32_t *d_probe_idx, uint32_t *d_build_idx, uint64_t cap,
+                            uint64_t *h_npairs) {
+  uint64_t total = 0;
+  ALN_TRY(exclusive_scan_u32(c, tcount, ntiles, toff, &total));
+  *h_npairs = total;
+  if (total > cap)
+    return set_error(ALN_E_CAPACITY, "aln_hash_join: %llu pairs, capacity %llu", (unsigned long long)total,
+                     (unsigned long long)cap);
+  if (total == 0) return ALN_OK;
+  ALN_REQUIRE(d_probe_idx && d_build_idx, "aln_hash_join: no output buffers");
+  {
+    ProfScope p(c, "join_write");
+    hipLaunchKernelGGL(k_jd_write, dim3((unsigned)ntiles), dim3(J_BLOCK), 0, c->stream, match, np,
+                       (const uint64_t *)toff, d_probe_idx, d_build_idx);
+  }
+  ALN_TRY(check_launch("join_write"));
+  ALN_HIP(hipStreamSynchronize(c->stream));
+  return ALN_OK;
+}
+
+// The dense path; *taken = false (and nothing else done) when the build keys repeat.
+static int join_dense(aln_ctx *c, const int64_t *bk, uint64_t nb, const int64_t *pk, uint64_t np, int64_t kmin,
+                      uint64_t range, uint32_t *d_probe_idx, uint32_t *d_build_idx, uint64_t cap,
+                      uint64_t *h_npairs, bool *taken) {
+  *taken = false;
+  const uint64_t ntiles = (np + J_TILE - 1) / J_TILE;
+  uint32_t *match, *tcount;
+  uint64_t *toff;
+  const uint64_t nent = (range + 31) / 32, nblk = (nent + JD_BLOCK - 1) / JD_BLOCK;
+  uint8_t *bytes;
+  uint32_t *bcount, *rowid;
+  uint64_t *boff;
+  uint2 *ent;
+  ALN_TRY(ws(c, nent * 32, &bytes));  // whole entries: the packing pass reads 32 bytes each
+  ALN_TRY(ws(c, nblk, &bcount));
+  ALN_TRY(ws(c, nblk, &boff));
+  ALN_TRY(ws(c, nent, &ent));
+  ALN_TRY(fill_u32(c, reinterpret_cast<uint32_t *>(bytes), nent * 8, 0));
+  uint64_t distinct = 0;
+  {
+    ProfScope p(c, "join_build");
+    hipLaunchKernelGGL(k_jd_bytes, dim3(grid_for(nb, 256, 65536)), dim3(256), 0, c->stream, bk, nb, kmin, bytes);
+    hipLaunchKernelGGL(k_jd_pack<0>, dim3((unsigned)nblk), dim3(JD_BLOCK), 0, c->stream, (const uint8_t *)bytes,
+                       nent, bcount, (const uint64_t *)nullptr, (uint2 *)nullptr);
+  }
+  ALN_TRY(check_launch("join_dense_bytes"));
+  ALN_TRY(exclusive_scan_u32(c, bcount, nblk, boff, &distinct));
+  if (distinct != nb) return ALN_OK;  // repeated build keys
+  *taken = true;
+  ALN_TRY(ws(c, nb, &rowid));
+  {
+    ProfScope p(c, "join_build");
+    hipLaunchKernelGGL(k_jd_pack<1>, dim3((unsigned)nblk), dim3(JD_BLOCK), 0, c->stream, (const uint8_t *)bytes,
+                       nent, bcount, (const uint64_t *)boff, ent);
+    hipLaunchKernelGGL(k_jd_rows, dim3(grid_for(nb, 256, 65536)), dim3(256), 0, c->stream, bk, nb, kmin,
+                       (const uint2 *)ent, rowid);
+  }
+  ALN_TRY(check_launch("join_dense_rows"));
+  ALN_TRY(ws(c, np, &match));
+  ALN_TRY(ws(c, ntiles, &tcount));
+  ALN_TRY(ws(c, ntiles, &toff));
+  {
+    ProfScope p(c, "join_probe");
+    hipLaunchKernelGGL(k_jd_match, dim3((unsigned)ntiles), dim3(J_BLOCK), 0, c->stream, pk, np, (const uint2 *)ent,
+                       (const uint32_t *)rowid, kmin, range, match, tcount);
+  }
+  ALN_TRY(check_launch("join_dense_match"));
+  return join_write_pairs(c, match, np, tcount, toff, ntiles, d_probe_idx, d_build_idx, cap, h_npairs);
+}
+
 }  // namespace aln
 
 using namespace aln;
@@ -177,6 +420,17 @@ extern "C" int aln_hash_join(aln_ctx *c, const int64_t *d_build_keys, uint64_t n
   ws_reset(c);
   *h_npairs = 0;
   if (nbuild == 0 || nprobe == 0) return ALN_OK;
+  // dense unique build keys: the bitmap + rank index
+  int64_t bmin, bmax;
+  ALN_TRY(minmax_i64(c, d_build_keys, nbuild, &bmin, &bmax));
+  const uint64_t brange = (uint64_t)bmax - (uint64_t)bmin + 1;
+  if (brange != 0 && brange <= (1ull << 31) && brange <= JD_DENSITY * nbuild) {
+    bool taken = false;
+    ALN_TRY(join_dense(c, d_build_keys, nbuild, d_probe_keys, nprobe, bmin, brange, d_probe_idx, d_build_idx,
+                       cap, h_npairs, &taken));
+    if (taken) return ALN_OK;
+    ws_reset(c);
+  }
   uint64_t tcap = next_pow2(2 * nbuild < 1024 ? 1024 : 2 * nbuild);
   uint32_t *slot, *dup;
   int64_t *tk;
@@ -241,6 +495,8 @@ extern "C" int aln_hash_join(aln_ctx *c, const int64_t *d_build_keys, uint64_t n
                        d_probe_keys, nprobe, slot, tk, tcap - 1, rlen, match, tcount);
   }
   ALN_TRY(check_launch("join_probe_count"));
+  // unique keys: match holds the build row itself, written out through LDS
+  if (!rlen) return join_write_pairs(c, match, nprobe, tcount, toff, ntiles, d_probe_idx, d_build_idx, cap, h_npairs);
   uint64_t total = 0;
   ALN_TRY(exclusive_scan_u32(c, tcount, ntiles, toff, &total));
   *h_npairs = total;

@@ -205,33 +205,52 @@ def test_groupby_wide_keys(ctx):
 
 
 # ---- join --------------------------------------------------------------------------------
-@pytest.mark.parametrize("dups", [False, True])
-def test_hash_join(ctx, dups):
+@pytest.mark.parametrize("case", ["dense", "dups", "sparse", "wide", "dense_big"])
+def test_hash_join(ctx, case):
+    """dense: unique keys over a range of <= 64 per build row (bitmap + rank index); dups:
+    repeated keys (sorted runs in the hash table); sparse / wide: unique keys over a range
+    too wide for the bitmap (hash table), the second beyond 2^32; dense_big: 2M probe rows."""
     from alenka_amd import device as D
 
-    r = np.random.default_rng(9 + dups)
+    r = np.random.default_rng(9 + len(case))
     nb, npb = 20000, 50000
+    dups = case == "dups"
     if dups:
         build = r.integers(0, 5000, nb)
+    elif case == "sparse":
+        build = r.permutation(1 << 24)[:nb].astype(np.int64) * 7 - 300000
+    elif case == "wide":
+        build = r.integers(-(1 << 40), 1 << 40, nb)
+        build = np.unique(build)[:nb]
+        nb = build.size
     else:
-        build = r.permutation(1 << 20)[:nb].astype(np.int64) - 300000
-    probe = np.concatenate([build[r.integers(0, nb, npb // 2)], r.integers(-400000, 1 << 20, npb - npb // 2)])
+        if case == "dense_big":
+            nb, npb = 500000, 2000000
+        build = r.permutation(1 << 20 if nb < 100000 else 1 << 22)[:nb].astype(np.int64) - 300000
+    lo, hi = int(build.min()) - 1000, int(build.max()) + 1000
+    probe = np.concatenate([build[r.integers(0, nb, npb // 2)], r.integers(lo, hi, npb - npb // 2)])
     probe = r.permutation(probe)
     pi, bi = D.hash_join(ctx, dev(ctx, build), dev(ctx, probe))
     pi, bi = host(pi).astype(np.uint32), host(bi).astype(np.uint32)
     # expected: probe-row order, then ascending build row
-    exp = []
-    where = {}
-    for j, b in enumerate(build):
-        where.setdefault(int(b), []).append(j)
-    for i, p in enumerate(probe):
-        for j in where.get(int(p), []):
-            exp.append((i, j))
-    exp = np.array(exp, np.uint32).reshape(-1, 2)
-    assert np.array_equal(np.stack([pi, bi], 1), exp)
-    # same multiset as the reference's sort-merge join
-    ol, orr = O.join_sortmerge(probe, build)
-    assert sorted(zip(ol.tolist(), orr.tolist())) == sorted(zip(pi.tolist(), bi.tolist()))
+    if not dups:  # unique build keys: one pair per probe row whose key is a build key
+        pos = {int(b): j for j, b in enumerate(build)}
+        want_p = np.array([i for i, p in enumerate(probe) if int(p) in pos], np.uint32)
+        want_b = np.array([pos[int(probe[i])] for i in want_p], np.uint32)
+        assert np.array_equal(pi, want_p) and np.array_equal(bi, want_b)
+    else:
+        exp = []
+        where = {}
+        for j, b in enumerate(build):
+            where.setdefault(int(b), []).append(j)
+        for i, p in enumerate(probe):
+            for j in where.get(int(p), []):
+                exp.append((i, j))
+        exp = np.array(exp, np.uint32).reshape(-1, 2)
+        assert np.array_equal(np.stack([pi, bi], 1), exp)
+    if case != "dense_big":  # same multiset as the reference's sort-merge join
+        ol, orr = O.join_sortmerge(probe, build)
+        assert sorted(zip(ol.tolist(), orr.tolist())) == sorted(zip(pi.tolist(), bi.tolist()))
 
 
 @pytest.mark.parametrize("width", [3, 8, 25])
