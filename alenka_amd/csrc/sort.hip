@@ -6,10 +6,18 @@
 // (v - min, or max - v for DESC); when all codes fit in 64 bits they are concatenated and
 // sorted once, otherwise one stable radix sort per key runs from the last key to the first.
 //
-// Radix pass: 8-bit digits, 4096-item tiles (4 waves x 64 lanes x 16 items, striped so
-// that wave order then (item, lane) order is memory order).  Ranks inside a wave come from
-// ballot-based digit matching; a digit-major histogram scan gives each (digit, tile) its
-// global offset, which keeps every pass stable.
+// Radix sort: 8-bit digits, least significant first, each pass stable.  A first kernel
+// counts pass 0's digits per block of 16384 items and finds the bits in which the codes
+// differ; a pass whose digit every item shares is skipped.  Each pass scans its per-(digit,
+// block) counts in digit-major order, which gives every 4096-item tile the first output
+// position of each of its digits, and one kernel scatters: a block reorders its tile by
+// digit in LDS (ranks from digit matching through LDS lane masks, in memory order) and
+// stores every digit's run at consecutive addresses; each XCD takes a contiguous range of
+// tiles, so neighbouring runs meet in one L2.  The passes after the first count their own
+// digit from the keys the previous pass wrote.  A permutation sort stores only the key
+// bytes the later passes read (8 or 4 per key), and its last pass stores none.  The first
+// pass of an ORDER BY computes the codes from the key columns itself.  No kernel waits on
+// another workgroup.
 #include <string.h>
 
 #include <algorithm>
@@ -18,139 +26,6 @@
 #include "aln_internal.h"
 
 namespace aln {
-
-constexpr int RS_BLOCK = 256;
-constexpr int RS_WAVES = RS_BLOCK / 64;
-constexpr int RS_ITEMS = 16;
-constexpr int RS_TILE = RS_BLOCK * RS_ITEMS;
-constexpr int RS_WAVE_ITEMS = 64 * RS_ITEMS;
-
-__global__ void __launch_bounds__(RS_BLOCK) k_rs_hist(const uint64_t *__restrict__ keys, uint64_t n,
-                                                      uint32_t shift, uint32_t mask,
-                                                      uint32_t nblocks, uint32_t *__restrict__ hist) {
-  __shared__ uint32_t h[256];
-  h[threadIdx.x] = 0;
-  __syncthreads();
-  uint64_t base = (uint64_t)blockIdx.x * RS_TILE;
-  for (int j = 0; j < RS_ITEMS; ++j) {
-    uint64_t i = base + (uint64_t)j * RS_BLOCK + threadIdx.x;
-    if (i < n) atomicAdd(&h[(uint32_t)(keys[i] >> shift) & mask], 1u);
-  }
-  __syncthreads();
-  hist[(uint64_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
-}
-
-__global__ void __launch_bounds__(RS_BLOCK) k_rs_scatter(const uint64_t *__restrict__ kin,
-                                                         const uint32_t *__restrict__ vin, uint64_t n,
-                                                         uint32_t shift, uint32_t mask,
-                                                         uint32_t nblocks,
-                                                         const uint64_t *__restrict__ offs,
-                                                         uint64_t *__restrict__ kout,
-                                                         uint32_t *__restrict__ vout) {
-  __shared__ uint32_t wc[RS_WAVES][256];
-  __shared__ uint64_t goff[256];
-  const unsigned lane = __lane_id(), wid = threadIdx.x / 64;
-  for (int w = 0; w < RS_WAVES; ++w) wc[w][threadIdx.x] = 0;
-  goff[threadIdx.x] = offs[(uint64_t)threadIdx.x * nblocks + blockIdx.x];
-  __syncthreads();
-  const uint64_t base = (uint64_t)blockIdx.x * RS_TILE + (uint64_t)wid * RS_WAVE_ITEMS;
-  uint64_t key[RS_ITEMS];
-  uint32_t val[RS_ITEMS], rank[RS_ITEMS];
-  const uint64_t lt = lanemask_lt();
-#pragma unroll
-  for (int j = 0; j < RS_ITEMS; ++j) {
-    uint64_t i = base + (uint64_t)j * 64 + lane;
-    bool valid = i < n;
-    key[j] = valid ? kin[i] : 0;
-    val[j] = valid ? vin[i] : 0;
-    uint32_t d = (uint32_t)(key[j] >> shift) & mask;
-    uint64_t peers = __ballot(valid);
-#pragma unroll
-    for (int b = 0; b < 8; ++b) {
-      bool bit = (d >> b) & 1;
-      uint64_t m = __ballot(bit);
-      peers &= bit ? m : ~m;
-    }
-    uint32_t before = valid ? wc[wid][d] : 0;
-    uint64_t below = peers & lt;
-    rank[j] = before + (uint32_t)__popcll(below);
-    if (valid && below == 0) wc[wid][d] = before + (uint32_t)__popcll(peers);
-  }
-  __syncthreads();
-  {
-    uint32_t run = 0;
-    for (int w = 0; w < RS_WAVES; ++w) {
-      uint32_t t = wc[w][threadIdx.x];
-      wc[w][threadIdx.x] = run;
-      run += t;
-    }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int j = 0; j < RS_ITEMS; ++j) {
-    uint64_t i = base + (uint64_t)j * 64 + lane;
-    if (i < n) {
-      uint32_t d = (uint32_t)(key[j] >> shift) & mask;
-      uint64_t dest = goff[d] + wc[wid][d] + rank[j];
-      kout[dest] = key[j];
-      vout[dest] = val[j];
-    }
-  }
-}
-
-__global__ void k_copy_pairs(const uint64_t *ks, const uint32_t *vs, uint64_t n, uint64_t *kd,
-                             uint32_t *vd) {
-  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n;
-       i += (uint64_t)gridDim.x * 256) {
-    kd[i] = ks[i];
-    vd[i] = vs[i];
-  }
-}
-
-int radix_sort_pairs(aln_ctx *c, uint64_t *d_keys, uint32_t *d_vals, uint64_t n, uint32_t bits) {
-  if (n <= 1 || bits == 0) return ALN_OK;
-  ALN_REQUIRE(bits <= 64, "radix sort: %u key bits", bits);
-  uint64_t nb64 = (n + RS_TILE - 1) / RS_TILE;
-  ALN_REQUIRE(nb64 < (1u << 31) / 256, "radix sort: %llu items is too many",
-              (unsigned long long)n);
-  uint32_t nblocks = (uint32_t)nb64;
-  uint64_t *k2;
-  uint32_t *v2, *hist;
-  uint64_t *offs;
-  ALN_TRY(ws(c, n, &k2));
-  ALN_TRY(ws(c, n, &v2));
-  ALN_TRY(ws(c, (size_t)256 * nblocks, &hist));
-  ALN_TRY(ws(c, (size_t)256 * nblocks, &offs));
-  uint64_t *ka = d_keys, *kb = k2;
-  uint32_t *va = d_vals, *vb = v2;
-  uint32_t passes = (bits + 7) / 8;
-  for (uint32_t p = 0; p < passes; ++p) {
-    uint32_t shift = 8 * p;
-    uint32_t pb = bits - shift < 8 ? bits - shift : 8;
-    uint32_t mask = (1u << pb) - 1;
-    {
-      ProfScope ps(c, "radix_hist");
-      hipLaunchKernelGGL(k_rs_hist, dim3(nblocks), dim3(RS_BLOCK), 0, c->stream, ka, n, shift, mask,
-                         nblocks, hist);
-    }
-    ALN_TRY(check_launch("radix_hist"));
-    ALN_TRY(exclusive_scan_u32(c, hist, (uint64_t)256 * nblocks, offs, nullptr));
-    {
-      ProfScope ps(c, "radix_scatter");
-      hipLaunchKernelGGL(k_rs_scatter, dim3(nblocks), dim3(RS_BLOCK), 0, c->stream, ka, va, n,
-                         shift, mask, nblocks, offs, kb, vb);
-    }
-    ALN_TRY(check_launch("radix_scatter"));
-    std::swap(ka, kb);
-    std::swap(va, vb);
-  }
-  if (ka != d_keys) {
-    hipLaunchKernelGGL(k_copy_pairs, dim3(grid_for(n, 256, 65536)), dim3(256), 0, c->stream, ka, va,
-                       n, d_keys, d_vals);
-    ALN_TRY(check_launch("copy_pairs"));
-  }
-  return ALN_OK;
-}
 
 struct KeySet {
   const int64_t *k[ALN_MAX_KEYS];
@@ -177,43 +52,571 @@ __global__ void k_codes(KeySet ks, const uint32_t *__restrict__ perm, uint64_t n
   }
 }
 
-// Builds the codes of keys [k0, k1) (k0 most significant).  Returns total bits.
-static int key_codes(aln_ctx *c, const int64_t *const *keys, const int32_t *desc, int k0, int k1,
-                     uint64_t n, const uint32_t *perm, uint64_t *code, uint32_t *idx,
-                     uint32_t *bits_out) {
-  KeySet ks;
-  memset(&ks, 0, sizeof(ks));
-  uint32_t total = 0;
-  uint32_t bits[ALN_MAX_KEYS];
-  for (int k = k0; k < k1; ++k) {
-    int64_t mn, mx;
-    ALN_TRY(minmax_i64(c, keys[k], n, &mn, &mx));
-    bits[k - k0] = bits_for_range_u((uint64_t)mx - (uint64_t)mn);
-    ks.ref[k - k0] = (desc && desc[k]) ? mx : mn;
-    ks.desc[k - k0] = desc ? desc[k] : 0;
-    ks.k[k - k0] = keys[k];
-    total += bits[k - k0];
-  }
-  ks.n = k1 - k0;
-  uint32_t sh = total;
+__device__ __forceinline__ uint64_t ks_code(const KeySet &ks, uint64_t r) {
+  uint64_t c = 0;
   for (int k = 0; k < ks.n; ++k) {
-    sh -= bits[k];
-    ks.shift[k] = sh;
+    int64_t v = ks.k[k][r];
+    uint64_t f = ks.desc[k] ? (uint64_t)ks.ref[k] - (uint64_t)v : (uint64_t)v - (uint64_t)ks.ref[k];
+    if (ks.shift[k] < 64) c |= f << ks.shift[k];
   }
-  ALN_REQUIRE(total <= 64, "key codes: %u bits", total);
-  hipLaunchKernelGGL(k_codes, dim3(grid_for(n, 256, 65536)), dim3(256), 0, c->stream, ks, perm, n,
-                     code, idx);
-  ALN_TRY(check_launch("codes"));
-  *bits_out = total;
+  return c;
+}
+
+constexpr int RS_BITS = 8;
+constexpr int RS_BINS = 1 << RS_BITS;
+constexpr int RS_BLOCK = 512;
+constexpr int RS_WAVES = RS_BLOCK / 64;
+constexpr int RS_ITEMS = 8;
+constexpr int RS_TILE = RS_BLOCK * RS_ITEMS;  // items one scatter block reorders in LDS
+constexpr int RS_WAVE_ITEMS = 64 * RS_ITEMS;
+constexpr int RS_SUB = 4;
+constexpr int RS_SUPER = RS_TILE * RS_SUB;  // items one count block counts
+constexpr int RS_MAX_PASSES = 8;
+constexpr int RS_SCAN_BLOCK = 256;
+constexpr uint32_t RS_XCDS = 8;
+// From this size on the host reads which bits vary between the codes and skips the passes
+// whose digit every item shares; smaller sorts run every pass without a synchronisation.
+constexpr uint64_t RS_SYNC_MIN = 1ull << 20;
+static_assert(RS_BINS * 2 == RS_BLOCK, "threads 0..255 own one digit each");
+static_assert(RS_SUPER % (RS_BLOCK * 4) == 0, "count loop");
+
+template <int B>
+struct KeyWord;
+template <>
+struct KeyWord<4> {
+  typedef uint32_t T;
+};
+template <>
+struct KeyWord<8> {
+  typedef uint64_t T;
+};
+
+// Bytes a key of `bits` bits is stored in: 4 or 8 (scattered stores of narrower words
+// cost as much per item as 4-byte ones).
+static inline int key_word(uint32_t bits) { return bits <= 32 ? 4 : 8; }
+
+// The items of one pass.  KI (bytes per stored key) 4, 8: keys and values from arrays;
+// KI 0: codes computed from the key columns at row r = vals ? vals[i] : i, with value r.
+struct RsIn {
+  const void *keys;
+  const uint32_t *vals;
+  KeySet ks;
+};
+
+template <int KI>
+__device__ __forceinline__ uint64_t rs_key(const RsIn &in, uint64_t i) {
+  if constexpr (KI == 0) {
+    return ks_code(in.ks, in.vals ? in.vals[i] : i);
+  } else {
+    typedef typename KeyWord<KI>::T T;
+    return (uint64_t)__builtin_nontemporal_load(static_cast<const T *>(in.keys) + i);
+  }
+}
+template <int KI>
+__device__ __forceinline__ uint32_t rs_val(const RsIn &in, uint64_t i) {
+  if (KI == 0 && !in.vals) return (uint32_t)i;
+  return __builtin_nontemporal_load(in.vals + i);
+}
+
+// Codes (and, VALS, values) of the N items i0 + j * stride; items at n and beyond get
+// code and value 0.  From key columns, the rows are read first and then each column's N
+// values, so the loads of all N items are in flight together.
+template <int KI, int N, bool VALS>
+__device__ __forceinline__ void rs_load(const RsIn &in, uint64_t n, uint64_t i0, uint32_t stride,
+                                        uint64_t *key, uint32_t *val) {
+  if constexpr (KI == 0) {
+    uint64_t r[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      uint64_t i = i0 + (uint64_t)j * stride;
+      r[j] = i < n ? (in.vals ? (uint64_t)__builtin_nontemporal_load(in.vals + i) : i) : 0;
+      key[j] = 0;
+      if (VALS) val[j] = (uint32_t)r[j];
+    }
+    for (int k = 0; k < in.ks.n; ++k) {
+      const int64_t *col = in.ks.k[k];
+      const uint64_t ref = (uint64_t)in.ks.ref[k];
+      const uint32_t sh = in.ks.shift[k];
+      const bool desc = in.ks.desc[k];
+      int64_t v[N];
+#pragma unroll
+      for (int j = 0; j < N; ++j) v[j] = col[r[j]];
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        uint64_t f = desc ? ref - (uint64_t)v[j] : (uint64_t)v[j] - ref;
+        if (sh < 64) key[j] |= f << sh;
+      }
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      uint64_t i = i0 + (uint64_t)j * stride;
+      key[j] = i < n ? rs_key<KI>(in, i) : 0;
+      if (VALS) val[j] = i < n ? rs_val<KI>(in, i) : 0;
+    }
+  }
+}
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
+  const unsigned lane = __lane_id();
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    uint32_t y = __shfl_up(x, d, 64);
+    if (lane >= (unsigned)d) x += y;
+  }
+  return x;
+}
+
+// Digit counts of one pass, block s over items [s * RS_SUPER, +RS_SUPER): cnt[digit *
+// nsuper + s] (digit-major, for the scan) and, per tile of the block, the count of each
+// digit in the block's earlier tiles: sub[(s * RS_SUB + tile) * 256 + digit].  VARY: also
+// ORs code ^ (code of item 0) into *vary, the bits in which the codes differ.
+template <int KI, bool VARY>
+__global__ void __launch_bounds__(RS_BLOCK) k_rs_count(RsIn in, uint64_t n, uint32_t sh, uint32_t mask,
+                                                       uint32_t nsuper, uint32_t *__restrict__ cnt,
+                                                       uint16_t *__restrict__ sub,
+                                                       unsigned long long *__restrict__ vary) {
+  __shared__ uint32_t h[RS_SUB][RS_BINS];
+  __shared__ uint64_t wv[RS_WAVES];
+  const uint32_t t = threadIdx.x, s = blockIdx.x;
+  for (uint32_t i = t; i < RS_SUB * RS_BINS; i += RS_BLOCK) (&h[0][0])[i] = 0;
+  __syncthreads();
+  const uint64_t lo = (uint64_t)s * RS_SUPER;
+  uint64_t v = 0;
+  const uint64_t c0 = VARY ? rs_key<KI>(in, 0) : 0;
+  for (int r = 0; r < RS_SUPER / (RS_BLOCK * 4); ++r) {
+    uint64_t k[4];
+    const uint64_t i0 = lo + (uint64_t)r * 4 * RS_BLOCK + t;
+    rs_load<KI, 4, false>(in, n, i0, RS_BLOCK, k, nullptr);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      uint64_t i = i0 + (uint64_t)u * RS_BLOCK;
+      if (i < n) {
+        atomicAdd(&h[(uint32_t)((i - lo) / RS_TILE)][(uint32_t)(k[u] >> sh) & mask], 1u);
+        if (VARY) v |= k[u] ^ c0;
+      }
+    }
+  }
+  if (VARY) {
+    for (int d = 32; d > 0; d >>= 1) v |= __shfl_xor(v, d, 64);
+    if (__lane_id() == 0) wv[t / 64] = v;
+  }
+  __syncthreads();
+  if (t < RS_BINS) {
+    uint32_t run = 0;
+#pragma unroll
+    for (int j = 0; j < RS_SUB; ++j) {
+      sub[((uint64_t)s * RS_SUB + j) * RS_BINS + t] = (uint16_t)run;
+      run += h[j][t];
+    }
+    cnt[(uint64_t)t * nsuper + s] = run;
+  }
+  if (VARY && t == 0) {
+    uint64_t all = 0;
+    for (int w = 0; w < RS_WAVES; ++w) all |= wv[w];
+    if (all) atomicOr(vary, (unsigned long long)all);
+  }
+}
+
+// Exclusive scan of each digit's row of block counts, written transposed for the scatter
+// blocks: excl[s * 256 + d]; tot[d] = the row's sum (one block per digit).
+__global__ void __launch_bounds__(RS_SCAN_BLOCK) k_rs_rowscan(const uint32_t *__restrict__ cnt, uint32_t nsuper,
+                                                              uint32_t *__restrict__ excl,
+                                                              uint32_t *__restrict__ tot) {
+  __shared__ uint32_t wsum[RS_SCAN_BLOCK / 64];
+  const uint32_t d = blockIdx.x, t = threadIdx.x, lane = __lane_id(), wid = t / 64;
+  const uint32_t per = (nsuper + RS_SCAN_BLOCK - 1) / RS_SCAN_BLOCK;
+  const uint32_t *row = cnt + (uint64_t)d * nsuper;
+  const uint32_t b = t * per, e = b + per < nsuper ? b + per : nsuper;
+  uint32_t local = 0;
+  for (uint32_t i = b; i < e; ++i) local += row[i];
+  uint32_t incl = wave_incl_scan(local);
+  if (lane == 63) wsum[wid] = incl;
+  __syncthreads();
+  uint32_t before = incl - local;
+  for (uint32_t w = 0; w < wid; ++w) before += wsum[w];
+  for (uint32_t i = b; i < e; ++i) {
+    uint32_t x = row[i];
+    excl[(uint64_t)i * RS_BINS + d] = before;
+    before += x;
+  }
+  if (t == RS_SCAN_BLOCK - 1) tot[d] = before;
+}
+
+// base[d] = items of digits below d (one block).
+__global__ void __launch_bounds__(RS_BINS) k_rs_base(const uint32_t *__restrict__ tot,
+                                                     uint32_t *__restrict__ base) {
+  __shared__ uint32_t wsum[RS_BINS / 64];
+  const uint32_t d = threadIdx.x, lane = __lane_id(), wid = d / 64;
+  uint32_t x = tot[d];
+  uint32_t incl = wave_incl_scan(x);
+  if (lane == 63) wsum[wid] = incl;
+  __syncthreads();
+  uint32_t before = incl - x;
+  for (uint32_t w = 0; w < wid; ++w) before += wsum[w];
+  base[d] = before;
+}
+
+// One pass: digit = (stored key >> sh) & mask; keys are written as (stored key >> oshift)
+// in KO bytes (KO 0: not written).  Tile b of count block s = b / RS_SUB stores digit d
+// from position base[d] + excl[s * 256 + d] + sub[b * 256 + d].
+struct RsPass {
+  RsIn in;
+  uint64_t n;
+  const uint32_t *base, *excl;
+  const uint16_t *sub;
+  void *kout;
+  uint32_t *vout;
+  uint32_t sh, mask, oshift, nsuper;
+};
+
+// One tile of RS_TILE items per block (8 waves x 64 lanes x 8 items; wave, then (item,
+// lane) order is memory order): LDS lane masks match the digits of each wave row, per-wave
+// digit counts place the waves, the tile is reordered by digit in LDS and each digit's run
+// is stored at consecutive addresses.
+template <int KI, int KO>
+__global__ void __launch_bounds__(RS_BLOCK) k_rs_scatter(RsPass a) {
+  __shared__ uint64_t sk[RS_TILE];
+  __shared__ uint32_t sv[RS_TILE];
+  __shared__ uint16_t wc[RS_WAVES][RS_BINS];
+  __shared__ uint32_t gdel[RS_BINS];
+  __shared__ uint32_t wsum[RS_BINS / 64];
+  const unsigned t = threadIdx.x, lane = __lane_id(), wid = t / 64;
+  // Workgroups go to the 8 XCDs round-robin: give each XCD a contiguous range of tiles, so
+  // neighbouring tiles' runs of a digit (adjacent in the output) meet in the same L2.
+  const uint32_t q = gridDim.x / RS_XCDS, rm = gridDim.x % RS_XCDS;
+  const uint32_t x = blockIdx.x % RS_XCDS;
+  const uint32_t tile = x * q + (x < rm ? x : rm) + blockIdx.x / RS_XCDS;
+  const uint64_t n = a.n;
+  const uint32_t sh = a.sh, mask = a.mask;
+  // during ranking, sk holds one 64-bit lane mask per (wave, digit)
+  uint64_t *pm = sk;
+  static_assert(RS_WAVES * RS_BINS <= RS_TILE, "masks fit in the key buffer");
+  for (int w = t; w < RS_WAVES * RS_BINS; w += RS_BLOCK) {
+    (&wc[0][0])[w] = 0;
+    pm[w] = 0;
+  }
+  const uint64_t tbase = (uint64_t)tile * RS_TILE;
+  const uint64_t base = tbase + (uint64_t)wid * RS_WAVE_ITEMS;
+  // this tile's first output position of digit t, read before the items so that its
+  // latency overlaps theirs
+  uint32_t first = 0;
+  if (t < RS_BINS)
+    first = a.base[t] + a.excl[(uint64_t)(tile / RS_SUB) * RS_BINS + t] + a.sub[(uint64_t)tile * RS_BINS + t];
+  uint64_t key[RS_ITEMS];
+  uint32_t val[RS_ITEMS], rank[RS_ITEMS];
+  rs_load<KI, RS_ITEMS, true>(a.in, n, base + lane, 64, key, val);
+  __syncthreads();
+  const uint64_t lt = lanemask_lt();
+  // Lanes of one wave row with equal digits: each lane ORs its bit into the row's mask of
+  // its digit and reads the mask back (a wave's LDS operations complete in order); the
+  // lowest such lane then advances the digit's count and clears the mask.
+#pragma unroll
+  for (int j = 0; j < RS_ITEMS; ++j) {
+    bool valid = base + (uint64_t)j * 64 + lane < n;
+    uint32_t d = (uint32_t)(key[j] >> sh) & mask;
+    uint64_t *slot = pm + wid * RS_BINS + d;
+    if (valid) __hip_atomic_fetch_or(slot, 1ull << lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    uint64_t peers = valid ? __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) : 0;
+    uint32_t before = valid ? wc[wid][d] : 0;
+    uint64_t below = peers & lt;
+    rank[j] = before + (uint32_t)__popcll(below);
+    if (valid && below == 0) {
+      wc[wid][d] = (uint16_t)(before + (uint32_t)__popcll(peers));
+      __hip_atomic_store(slot, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+  }
+  __syncthreads();
+  uint32_t cnt = 0, incl = 0;
+  if (t < RS_BINS) {
+#pragma unroll
+    for (int w = 0; w < RS_WAVES; ++w) {
+      uint32_t x = wc[w][t];
+      wc[w][t] = (uint16_t)cnt;
+      cnt += x;
+    }
+    incl = wave_incl_scan(cnt);
+    if (lane == 63) wsum[wid] = incl;
+  }
+  __syncthreads();
+  if (t < RS_BINS) {
+    uint32_t tstart = incl - cnt;
+    for (unsigned w = 0; w < wid; ++w) tstart += wsum[w];
+#pragma unroll
+    for (int w = 0; w < RS_WAVES; ++w) wc[w][t] = (uint16_t)(wc[w][t] + tstart);
+    gdel[t] = first - tstart;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < RS_ITEMS; ++j) {
+    if (base + (uint64_t)j * 64 + lane < n) {
+      uint32_t d = (uint32_t)(key[j] >> sh) & mask;
+      uint32_t pos = wc[wid][d] + rank[j];
+      sk[pos] = key[j];
+      sv[pos] = val[j];
+    }
+  }
+  __syncthreads();
+  const uint32_t nvalid = n - tbase < (uint64_t)RS_TILE ? (uint32_t)(n - tbase) : (uint32_t)RS_TILE;
+#pragma unroll
+  for (int j = 0; j < RS_ITEMS; ++j) {
+    uint32_t i = (uint32_t)j * RS_BLOCK + t;
+    if (i < nvalid) {
+      uint64_t k = sk[i];
+      uint32_t dest = gdel[(uint32_t)(k >> sh) & mask] + i;
+      if constexpr (KO != 0) {
+        typedef typename KeyWord<KO>::T T;
+        static_cast<T *>(a.kout)[dest] = (T)(k >> a.oshift);
+      }
+      a.vout[dest] = sv[i];
+    }
+  }
+}
+
+__global__ void k_rs_vals(const uint32_t *__restrict__ vals, uint64_t n, uint32_t *__restrict__ vout) {
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256)
+    vout[i] = vals ? vals[i] : (uint32_t)i;
+}
+
+__global__ void k_copy_pairs(const uint64_t *ks, const uint32_t *vs, uint64_t n, uint64_t *kd,
+                             uint32_t *vd) {
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * 256) {
+    kd[i] = ks[i];
+    vd[i] = vs[i];
+  }
+}
+
+template <int KI, int KO>
+static void rs_scatter_launch(aln_ctx *c, uint32_t ntiles, const RsPass &a) {
+  hipLaunchKernelGGL((k_rs_scatter<KI, KO>), dim3(ntiles), dim3(RS_BLOCK), 0, c->stream, a);
+}
+
+static int rs_scatter(aln_ctx *c, int ki, int ko, uint32_t ntiles, const RsPass &a) {
+  switch (ki * 16 + ko) {
+#define RS_CASE(x, y)                      \
+  case x * 16 + y:                         \
+    rs_scatter_launch<x, y>(c, ntiles, a); \
+    break;
+    RS_CASE(0, 0) RS_CASE(0, 4) RS_CASE(0, 8)
+    RS_CASE(8, 0) RS_CASE(8, 4) RS_CASE(8, 8)
+    RS_CASE(4, 0) RS_CASE(4, 4)
+#undef RS_CASE
+    default:
+      ALN_REQUIRE(false, "radix sort: key widths %d -> %d", ki, ko);
+  }
+  return check_launch("radix_scatter");
+}
+
+template <int KI, bool VARY>
+static void rs_count_launch(aln_ctx *c, uint32_t nsuper, const RsIn &in, uint64_t n, uint32_t sh,
+                            uint32_t mask, uint32_t *cnt, uint16_t *sub, unsigned long long *vary) {
+  hipLaunchKernelGGL((k_rs_count<KI, VARY>), dim3(nsuper), dim3(RS_BLOCK), 0, c->stream, in, n, sh, mask,
+                     nsuper, cnt, sub, vary);
+}
+
+static int rs_count(aln_ctx *c, int ki, bool vary, uint32_t nsuper, const RsIn &in, uint64_t n,
+                    uint32_t sh, uint32_t mask, uint32_t *cnt, uint16_t *sub, unsigned long long *d_vary) {
+  switch (ki * 2 + (vary ? 1 : 0)) {
+    case 0: rs_count_launch<0, false>(c, nsuper, in, n, sh, mask, cnt, sub, d_vary); break;
+    case 1: rs_count_launch<0, true>(c, nsuper, in, n, sh, mask, cnt, sub, d_vary); break;
+    case 8: rs_count_launch<4, false>(c, nsuper, in, n, sh, mask, cnt, sub, d_vary); break;
+    case 16: rs_count_launch<8, false>(c, nsuper, in, n, sh, mask, cnt, sub, d_vary); break;
+    case 17: rs_count_launch<8, true>(c, nsuper, in, n, sh, mask, cnt, sub, d_vary); break;
+    default:
+      ALN_REQUIRE(false, "radix sort: key width %d", ki);
+  }
+  return check_launch("radix_count");
+}
+
+// Stable LSD sort of the items of `in` by the low `bits` bits of their codes.  from_keys:
+// the codes come from in.ks (first pass), else from the 8-byte array in.keys.  The values
+// land in vout (may be in.vals).  kout given: the codes land there, whole (may be in.keys);
+// otherwise each pass stores only the key bytes the later passes read.
+static int rs_sort(aln_ctx *c, RsIn in, bool from_keys, uint64_t n, uint32_t bits, uint64_t *kout,
+                   uint32_t *vout) {
+  if (n == 0) return ALN_OK;
+  ALN_REQUIRE(bits <= 64, "radix sort: %u key bits", bits);
+  ALN_REQUIRE(n < (1ull << 32), "radix sort: more than 2^32 items");
+  const bool keys_out = kout != nullptr;
+  const uint32_t passes = (bits + RS_BITS - 1) / RS_BITS;
+  const uint32_t nsuper = (uint32_t)((n + RS_SUPER - 1) / RS_SUPER);
+  const uint32_t ntiles = (uint32_t)((n + RS_TILE - 1) / RS_TILE);
+  const uint64_t ncnt = (uint64_t)RS_BINS * nsuper;
+  const int ki0 = from_keys ? 0 : 8;
+  auto pass_mask = [&](uint32_t p) {
+    uint32_t pb = bits - RS_BITS * p < RS_BITS ? bits - RS_BITS * p : RS_BITS;
+    return (1u << pb) - 1;
+  };
+  uint32_t *cnt = nullptr, *excl = nullptr, *tot = nullptr, *dbase = nullptr;
+  uint16_t *sub = nullptr;
+  unsigned long long *d_vary = nullptr;
+  if (passes > 0) {
+    ALN_TRY(ws(c, ncnt, &cnt));
+    ALN_TRY(ws(c, ncnt, &excl));
+    ALN_TRY(ws(c, (size_t)RS_BINS, &tot));
+    ALN_TRY(ws(c, (size_t)RS_BINS, &dbase));
+    ALN_TRY(ws(c, (size_t)nsuper * RS_SUB * RS_BINS, &sub));
+    ALN_TRY(ws(c, 1, &d_vary));
+  }
+  // pass 0's counts, and the bits in which the codes differ
+  const bool sync = n >= RS_SYNC_MIN;
+  if (passes > 0) {
+    if (sync) ALN_TRY(fill_u64(c, reinterpret_cast<uint64_t *>(d_vary), 1, 0));
+    ProfScope ps(c, "radix_count");
+    ALN_TRY(rs_count(c, ki0, sync, nsuper, in, n, 0, pass_mask(0), cnt, sub, d_vary));
+  }
+  uint64_t vary = ~0ull;
+  if (sync && passes > 0) ALN_TRY(copy_to_host(c, &vary, d_vary, 8));
+  uint32_t run[RS_MAX_PASSES], nrun = 0;
+  for (uint32_t p = 0; p < passes; ++p)
+    if ((vary >> (RS_BITS * p)) & pass_mask(p)) run[nrun++] = p;
+  if (nrun == 0) {  // every code equal: the input order is sorted
+    if (from_keys || in.vals != vout) {
+      hipLaunchKernelGGL(k_rs_vals, dim3(grid_for(n, 256, 65536)), dim3(256), 0, c->stream, in.vals, n, vout);
+      ALN_TRY(check_launch("radix_vals"));
+    }
+    ALN_REQUIRE(!keys_out || !from_keys, "radix sort: codes of the key columns are not kept");
+    if (keys_out && kout != in.keys)
+      ALN_HIP(hipMemcpyAsync(kout, in.keys, 8ull * n, hipMemcpyDeviceToDevice, c->stream));
+    return ALN_OK;
+  }
+  // Bytes each pass reads (ki) and writes (ko) per key: the bytes below the next pass's
+  // digit are dropped and the rest is stored in the smallest word that holds it.
+  int ki[RS_MAX_PASSES], ko[RS_MAX_PASSES];
+  uint32_t qin[RS_MAX_PASSES], qout[RS_MAX_PASSES];
+  for (uint32_t r = 0; r < nrun; ++r) {
+    qin[r] = r == 0 ? 0 : qout[r - 1];
+    ki[r] = r == 0 ? ki0 : ko[r - 1];
+    if (keys_out) {
+      qout[r] = 0;
+      ko[r] = 8;
+    } else if (r + 1 < nrun) {
+      qout[r] = run[r + 1];
+      ko[r] = key_word(bits - RS_BITS * qout[r]);
+    } else {
+      qout[r] = qin[r];
+      ko[r] = 0;
+    }
+  }
+  const bool alias1 = nrun == 1 && ((in.vals && in.vals == vout) || (keys_out && in.keys == kout));
+  void *kt[2] = {nullptr, nullptr};
+  uint32_t *vt[2] = {nullptr, nullptr};
+  if (nrun > 1 || alias1) {
+    ALN_TRY(ws(c, 8 * n, reinterpret_cast<uint8_t **>(&kt[0])));
+    ALN_TRY(ws(c, 8 * n, reinterpret_cast<uint8_t **>(&kt[1])));
+    ALN_TRY(ws(c, n, &vt[0]));
+    ALN_TRY(ws(c, n, &vt[1]));
+  }
+  const void *kprev = nullptr;
+  const uint32_t *vprev = nullptr;
+  for (uint32_t r = 0; r < nrun; ++r) {
+    const uint32_t p = run[r];
+    const bool direct = r + 1 == nrun && !alias1;
+    RsPass a;
+    a.in = in;
+    if (r > 0) {
+      a.in.keys = kprev;
+      a.in.vals = vprev;
+    }
+    a.n = n;
+    a.base = dbase;
+    a.excl = excl;
+    a.sub = sub;
+    a.kout = ko[r] == 0 ? nullptr : direct ? static_cast<void *>(kout) : kt[(nrun - 1 - r) & 1];
+    a.vout = direct ? vout : vt[(nrun - 1 - r) & 1];
+    a.sh = RS_BITS * (p - qin[r]);
+    a.mask = pass_mask(p);
+    a.oshift = RS_BITS * (qout[r] - qin[r]);
+    a.nsuper = nsuper;
+    if (r > 0 || p != 0) {
+      ProfScope ps(c, "radix_count");
+      ALN_TRY(rs_count(c, ki[r], false, nsuper, a.in, n, a.sh, a.mask, cnt, sub, nullptr));
+    }
+    {
+      ProfScope ps(c, "radix_scan");
+      hipLaunchKernelGGL(k_rs_rowscan, dim3(RS_BINS), dim3(RS_SCAN_BLOCK), 0, c->stream, cnt, nsuper, excl, tot);
+      hipLaunchKernelGGL(k_rs_base, dim3(1), dim3(RS_BINS), 0, c->stream, tot, dbase);
+    }
+    ALN_TRY(check_launch("radix_scan"));
+    {
+      ProfScope ps(c, "radix_scatter");
+      ALN_TRY(rs_scatter(c, ki[r], ko[r], ntiles, a));
+    }
+    kprev = a.kout;
+    vprev = a.vout;
+  }
+  if (alias1) {
+    if (keys_out) {
+      hipLaunchKernelGGL(k_copy_pairs, dim3(grid_for(n, 256, 65536)), dim3(256), 0, c->stream,
+                         static_cast<const uint64_t *>(kprev), vprev, n, kout, vout);
+      ALN_TRY(check_launch("radix_copy"));
+    } else {
+      ALN_HIP(hipMemcpyAsync(vout, vprev, 4ull * n, hipMemcpyDeviceToDevice, c->stream));
+    }
+  }
   return ALN_OK;
 }
 
-static int key_bits(aln_ctx *c, const int64_t *const *keys, int nkeys, uint64_t n,
-                    uint32_t *bits) {
+int radix_sort_pairs(aln_ctx *c, uint64_t *d_keys, uint32_t *d_vals, uint64_t n, uint32_t bits) {
+  if (n <= 1 || bits == 0) return ALN_OK;
+  RsIn in;
+  memset(&in, 0, sizeof(in));
+  in.keys = d_keys;
+  in.vals = d_vals;
+  return rs_sort(c, in, false, n, bits, d_keys, d_vals);
+}
+
+// Range of every key column: one min/max pass each.
+struct KeyRanges {
+  int64_t mn[ALN_MAX_KEYS], mx[ALN_MAX_KEYS];
+  uint32_t bits[ALN_MAX_KEYS], total;
+};
+
+static int key_ranges(aln_ctx *c, const int64_t *const *keys, int nkeys, uint64_t n, KeyRanges *r) {
+  r->total = 0;
   for (int k = 0; k < nkeys; ++k) {
-    int64_t mn, mx;
-    ALN_TRY(minmax_i64(c, keys[k], n, &mn, &mx));
-    bits[k] = bits_for_range_u((uint64_t)mx - (uint64_t)mn);
+    ALN_TRY(minmax_i64(c, keys[k], n, &r->mn[k], &r->mx[k]));
+    r->bits[k] = bits_for_range_u((uint64_t)r->mx[k] - (uint64_t)r->mn[k]);
+    r->total += r->bits[k];
+  }
+  return ALN_OK;
+}
+
+// The code layout of keys [k0, k1) (k0 most significant).  Returns the total bits.
+static uint32_t key_set(const KeyRanges &r, const int64_t *const *keys, const int32_t *desc, int k0,
+                        int k1, KeySet *ks) {
+  memset(ks, 0, sizeof(*ks));
+  uint32_t total = 0;
+  for (int k = k0; k < k1; ++k) {
+    ks->ref[k - k0] = (desc && desc[k]) ? r.mx[k] : r.mn[k];
+    ks->desc[k - k0] = desc ? desc[k] : 0;
+    ks->k[k - k0] = keys[k];
+    total += r.bits[k];
+  }
+  ks->n = k1 - k0;
+  uint32_t sh = total;
+  for (int k = 0; k < ks->n; ++k) {
+    sh -= r.bits[k0 + k];
+    ks->shift[k] = sh;
+  }
+  return total;
+}
+
+static int sort_with_ranges(aln_ctx *c, const KeyRanges &r, const int64_t *const *keys,
+                            const int32_t *desc, int nkeys, uint64_t n, uint32_t *d_perm) {
+  RsIn in;
+  memset(&in, 0, sizeof(in));
+  if (r.total <= 64) {  // one sort of the concatenated codes, computed by its first pass
+    key_set(r, keys, desc, 0, nkeys, &in.ks);
+    return rs_sort(c, in, true, n, r.total, nullptr, d_perm);
+  }
+  // LSD over keys, last key first, each sort stable (sorts.cu:17-70); each reads its key at
+  // the rows of the permutation so far
+  for (int k = nkeys - 1; k >= 0; --k) {
+    uint32_t b = key_set(r, keys, desc, k, k + 1, &in.ks);
+    in.vals = k == nkeys - 1 ? nullptr : d_perm;
+    ALN_TRY(rs_sort(c, in, true, n, b, nullptr, d_perm));
   }
   return ALN_OK;
 }
@@ -223,24 +626,17 @@ int sort_keys_impl(aln_ctx *c, const int64_t *const *keys, const int32_t *desc, 
   ALN_REQUIRE(nkeys >= 1 && nkeys <= ALN_MAX_KEYS, "sort: %d keys", nkeys);
   ALN_REQUIRE(n < (1ull << 32), "sort: more than 2^32 rows");
   if (n == 0) return ALN_OK;
-  uint32_t bits[ALN_MAX_KEYS], total = 0;
-  ALN_TRY(key_bits(c, keys, nkeys, n, bits));
-  for (int k = 0; k < nkeys; ++k) total += bits[k];
-  uint64_t *code;
-  ALN_TRY(ws(c, n, &code));
-  if (total <= 64) {
-    uint32_t b;
-    ALN_TRY(key_codes(c, keys, desc, 0, nkeys, n, nullptr, code, d_perm, &b));
-    return radix_sort_pairs(c, code, d_perm, n, b);
+  KeyRanges r;
+  if (nkeys == 1 && n >= RS_SYNC_MIN) {
+    // one key: codes over the whole int64 range, no min/max pass; the histogram finds the
+    // digits every key shares and those passes are skipped
+    r.mn[0] = INT64_MIN;
+    r.mx[0] = INT64_MAX;
+    r.bits[0] = r.total = 64;
+  } else {
+    ALN_TRY(key_ranges(c, keys, nkeys, n, &r));
   }
-  // LSD over keys, last key first, each pass stable (sorts.cu:17-70)
-  ALN_TRY(iota_u32(c, d_perm, n));
-  for (int k = nkeys - 1; k >= 0; --k) {
-    uint32_t b;
-    ALN_TRY(key_codes(c, keys, desc, k, k + 1, n, d_perm, code, nullptr, &b));
-    ALN_TRY(radix_sort_pairs(c, code, d_perm, n, b));
-  }
-  return ALN_OK;
+  return sort_with_ranges(c, r, keys, desc, nkeys, n, d_perm);
 }
 
 // ---- top-k: tile-wise bitonic sort of (code, index), keep the first k of each tile -----
@@ -353,14 +749,13 @@ int aln_topk(aln_ctx *c, const int64_t *const *d_keys, const int32_t *desc, int 
   *h_nout = 0;
   if (n == 0 || k == 0) return ALN_OK;
   ALN_REQUIRE(d_out_idx, "aln_topk: no output buffer");
-  uint32_t bits[ALN_MAX_KEYS], total = 0;
-  ALN_TRY(key_bits(c, d_keys, nkeys, n, bits));
-  for (int i = 0; i < nkeys; ++i) total += bits[i];
-  if (total > 64 || k > TK_TILE / 2) {
+  KeyRanges r;
+  ALN_TRY(key_ranges(c, d_keys, nkeys, n, &r));
+  if (r.total > 64 || k > TK_TILE / 2) {
     // wide keys or large k: full stable sort, then the prefix (emit_order + LIMIT)
     uint32_t *perm;
     ALN_TRY(ws(c, n, &perm));
-    ALN_TRY(sort_keys_impl(c, d_keys, desc, nkeys, n, perm));
+    ALN_TRY(sort_with_ranges(c, r, d_keys, desc, nkeys, n, perm));
     uint32_t kk = (uint32_t)(n < k ? n : k);
     ALN_HIP(hipMemcpyAsync(d_out_idx, perm, 4ull * kk, hipMemcpyDeviceToDevice, c->stream));
     ALN_HIP(hipStreamSynchronize(c->stream));
@@ -371,8 +766,11 @@ int aln_topk(aln_ctx *c, const int64_t *const *d_keys, const int32_t *desc, int 
   uint32_t *idx;
   ALN_TRY(ws(c, n, &code));
   ALN_TRY(ws(c, n, &idx));
-  uint32_t b;
-  ALN_TRY(key_codes(c, d_keys, desc, 0, nkeys, n, nullptr, code, idx, &b));
+  KeySet ks;
+  key_set(r, d_keys, desc, 0, nkeys, &ks);
+  hipLaunchKernelGGL(k_codes, dim3(grid_for(n, 256, 65536)), dim3(256), 0, c->stream, ks,
+                     (const uint32_t *)nullptr, n, code, idx);
+  ALN_TRY(check_launch("codes"));
   return topk_codes(c, code, idx, n, k, d_out_idx, h_nout);
 }
 

@@ -419,7 +419,10 @@ def batch_leg(ctx, D, fact, nfact, seed, nq, steps, sort_rows):
         perm = D.sort_keys(ctx, [keys])
         torch.cuda.synchronize()
         st.append(time.perf_counter() - t1)
-    ok = bool((keys[perm.long()].diff() >= 0).all().item())
+    # sorted and a permutation (compared, not subtracted: int64 differences can wrap)
+    sk = keys[perm.long()]
+    ok = bool((sk[1:] >= sk[:-1]).all().item()) and bool((torch.bincount(perm.long(), minlength=sort_rows) == 1).all().item())
+    del sk
     del keys, perm
     torch.cuda.empty_cache()
     sort_s = float(np.median(st))

@@ -325,6 +325,57 @@ def test_radix_sort_pairs(ctx):
     assert np.array_equal(host(vt), order.astype(np.int32))
 
 
+RADIX_CASES = [(1, 64, "u"), (2, 64, "u"), (4095, 64, "u"), (4096, 17, "u"), (4097, 8, "u"), (100003, 5, "u"),
+               ((1 << 20) + 3, 64, "u"), ((1 << 20) + 3, 40, "few"), (1 << 20, 64, "const"), ((1 << 20) + 5, 64, "hi"),
+               (300001, 64, "hi")]
+
+
+@pytest.mark.parametrize("n,bits,kind", RADIX_CASES)
+def test_radix_sort_pairs_shapes(ctx, n, bits, kind):
+    """Stable sort by the low `bits` bits: tile-boundary sizes, one-pass sorts (sorted through
+    a temporary), digits that every item shares (skipped passes at 2^20 items and up), all
+    keys equal, and bits above `bits` that must not take part."""
+    from alenka_amd import device as D
+
+    r = np.random.default_rng(n + bits)
+    if kind == "u":
+        k = r.integers(0, 1 << 63, n, dtype=np.int64).astype(np.uint64) * np.uint64(2) + r.integers(0, 2, n).astype(np.uint64)
+    elif kind == "few":  # 16 values, digits 0-1 and 3-4 shared by every key
+        k = r.integers(0, 16, n).astype(np.uint64) << np.uint64(20)
+    elif kind == "const":
+        k = np.full(n, 0x0123456789ABCDEF, np.uint64)
+    else:  # only the top digit varies
+        k = r.integers(0, 256, n).astype(np.uint64) << np.uint64(56)
+    kt = torch.as_tensor(k.view(np.int64)).to(ctx.device)
+    vt = torch.arange(n, dtype=torch.int32, device=ctx.device)
+    D.radix_sort_pairs(ctx, kt, vt, bits)
+    low = k & np.uint64((1 << bits) - 1 if bits < 64 else 0xFFFFFFFFFFFFFFFF)
+    order = np.argsort(low, kind="stable")
+    assert np.array_equal(host(vt), order.astype(np.int32))
+    assert np.array_equal(host(kt).view(np.uint64), k[order])
+
+
+@pytest.mark.parametrize("n,shape", [(60000, "wide_small_last"), (60000, "small_first_wide"),
+                                     ((1 << 20) + 7, "wide_small_last"), ((1 << 20) + 7, "three")])
+def test_sort_keys_lsd_shapes(ctx, n, shape):
+    """ORDER BY keys whose codes exceed 64 bits: one sort per key, last key first, each one
+    reading its key through the permutation so far; a key of at most 8 bits is a one-pass
+    sort whose permutation is both read and written."""
+    from alenka_amd import device as D
+
+    r = np.random.default_rng(n + len(shape))
+    full = r.integers(-(1 << 63), (1 << 63) - 1, n, dtype=np.int64)
+    small = r.integers(-100, 100, n)
+    if shape == "wide_small_last":
+        keys, desc = [full, small], [0, 1]
+    elif shape == "small_first_wide":
+        keys, desc = [small, full], [1, 0]
+    else:
+        keys, desc = [small, full, r.integers(0, 1 << 40, n)], [0, 1, 1]
+    perm = host(D.sort_keys(ctx, [dev(ctx, k) for k in keys], desc)).astype(np.uint32)
+    assert np.array_equal(perm, O.order(keys, desc))
+
+
 def test_hash_rows(ctx):
     from alenka_amd import device as D
 
