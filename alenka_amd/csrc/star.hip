@@ -626,6 +626,57 @@ __global__ void __launch_bounds__(ST_BLOCK, 8) k_star_dim(ScanDesc d, const SegP
   if (__lane_id() == 0) npass[region] = n;
 }
 
+// Sets the bits of the block's passing mid keys (code[j] != ~0) in the mid bitmap: keys of
+// neighbouring rows are usually close (a table stored in key order), so the block ORs them
+// into an LDS window of words over [its smallest word, its largest] first and then ORs the
+// window's non-zero words into the bitmap, consecutive words in one wave instruction;
+// keys beyond the window's PL_STAGE_WORDS words go to the bitmap directly.  The bits are
+// set without a returned value: a duplicate key shows as popcount(bitmap) < passing rows.
+// Reuses the tile's staging area (`win`), so every read of it must be done.
+__device__ __forceinline__ void mid_bits_window(uint32_t *win, const uint32_t *code, uint32_t *__restrict__ mbits) {
+  __shared__ uint32_t s_lo, s_hi;
+  if (threadIdx.x == 0) {
+    s_lo = 0xFFFFFFFFu;
+    s_hi = 0;
+  }
+  uint32_t lo = 0xFFFFFFFFu, hi = 0;
+#pragma unroll
+  for (int j = 0; j < ST_ITEMS; ++j)
+    if (code[j] != 0xFFFFFFFFu) {
+      lo = min(lo, code[j] >> 5);
+      hi = max(hi, code[j] >> 5);
+    }
+  for (int o = 32; o > 0; o >>= 1) {
+    lo = min(lo, (uint32_t)__shfl_xor((int)lo, o, 64));
+    hi = max(hi, (uint32_t)__shfl_xor((int)hi, o, 64));
+  }
+  __syncthreads();  // s_lo / s_hi set; the staging area is no longer read
+  if (__lane_id() == 0 && lo != 0xFFFFFFFFu) {
+    atomicMin(&s_lo, lo);
+    atomicMax(&s_hi, hi);
+  }
+  __syncthreads();
+  const uint32_t wlo = s_lo, whi = s_hi;
+  if (wlo > whi) return;  // no passing row in the block (uniform)
+  const uint32_t span = whi - wlo + 1 < (uint32_t)PL_STAGE_WORDS ? whi - wlo + 1 : (uint32_t)PL_STAGE_WORDS;
+  for (uint32_t w = threadIdx.x; w < span; w += ST_BLOCK) win[w] = 0;
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < ST_ITEMS; ++j)
+    if (code[j] != 0xFFFFFFFFu) {
+      const uint32_t wi = (code[j] >> 5) - wlo, bit = 1u << (code[j] & 31);
+      if (wi < span)
+        atomicOr(&win[wi], bit);
+      else
+        atomicOr(&mbits[code[j] >> 5], bit);
+    }
+  __syncthreads();
+  for (uint32_t w = threadIdx.x; w < span; w += ST_BLOCK) {
+    const uint32_t v = win[w];
+    if (v) atomicOr(&mbits[wlo + w], v);
+  }
+}
+
 // ---- M: mid scan, probe the dimension bitmap, compact passing rows ------------------------
 // Plan roles: 0 range, 1 foreign key (probes the dimension bitmap, staged), 2 own key
 // (staged), 3 and 4 the group columns (decoded later, for the groups that need them, from
@@ -705,17 +756,18 @@ __global__ void __launch_bounds__(ST_BLOCK, 8) k_star_mid(ScanDesc d, const SegP
   // entries); no barrier: each wave owns its region
   const uint64_t rbase = (uint64_t)region * PL_WAVE_ROWS;
   uint32_t nout = 0;
+  uint32_t mcode[ST_ITEMS];
 #pragma unroll
   for (int j = 0; j < ST_ITEMS; ++j) {
     const bool h = (passmask >> j) & 1;
     const uint64_t b = __ballot(h);
+    mcode[j] = 0xFFFFFFFFu;
     if (h) {
       const uint32_t i = j * ST_BLOCK + threadIdx.x;
       const int64_t key = fast ? (int64_t)((uint64_t)P.col[2].base + slot_code(ks, i))
                                : tile_get(ts, d.words, d.segs, kcol, seg, r0, i);
       const uint32_t code = (uint32_t)((uint64_t)key - (uint64_t)mkmin);  // key range < 2^32
-      if (mbits)  // no returned value: a duplicate key shows as popcount(mbits) < passing rows
-        atomicOr(&mbits[code >> 5], 1u << (code & 31));
+      mcode[j] = code;
       rent[rbase + nout + (uint32_t)__popcll(b & lanemask_lt())] = make_uint2(code, row0 + i);
     }
     nout += (uint32_t)__popcll(b);
@@ -724,6 +776,7 @@ __global__ void __launch_bounds__(ST_BLOCK, 8) k_star_mid(ScanDesc d, const SegP
     counts[region] = nout;
     probed[region] = nprobe;
   }
+  if (mbits) mid_bits_window(lds, mcode, mbits);
 }
 
 __device__ __forceinline__ uint64_t st_slot(uint32_t code, uint64_t cap) {
