@@ -255,15 +255,17 @@ def join_rows(ctx, idx, nrows, anti, cap=None):
     return out[: cnt.value]
 
 
-def groupby(ctx, keys, vals, aggs, order=L.ORDER_KEYS, seed=100, cap=None):
-    """Returns (out_keys list, out_vals list) trimmed to the group count."""
-    n = keys[0].numel()
+def groupby(ctx, keys, vals, aggs, order=L.ORDER_KEYS, seed=100, cap=None, n=None):
+    """Returns (out_keys list, out_vals list) trimmed to the group count.  No keys: one
+    group over n rows (n defaults to the first value column's length)."""
+    if n is None:
+        n = keys[0].numel() if keys else next(v.numel() for v in vals if v is not None)
     cap = max(n, 1) if cap is None else cap
     ok = [ctx.empty(cap) for _ in keys]
     ov = [ctx.empty(cap) for _ in vals]
-    kp = (C.c_void_p * len(keys))(*[k.data_ptr() for k in keys])
+    kp = (C.c_void_p * max(1, len(keys)))(*[k.data_ptr() for k in keys])
     vp = (C.c_void_p * max(1, len(vals)))(*[(v.data_ptr() if v is not None else 0) for v in vals])
-    okp = (C.c_void_p * len(ok))(*[k.data_ptr() for k in ok])
+    okp = (C.c_void_p * max(1, len(ok)))(*[k.data_ptr() for k in ok])
     ovp = (C.c_void_p * max(1, len(ov)))(*[v.data_ptr() for v in ov])
     ag = (C.c_int32 * max(1, len(aggs)))(*aggs)
     g = C.c_uint64()

@@ -902,7 +902,7 @@ static int do_select(const char *s, const char *f, int grp_cnt) {
   // one source segment: key order (order_inplace + GroupBy); several: the hash-ordered
   // merge of add()/count_avg() (merge.cu:92-496)
   const int order = (grp_cnt > 0 && v.cycles > 1) ? ALN_ORDER_HASH : ALN_ORDER_KEYS;
-  // keys: the GROUP BY columns (a constant key when there is none: one output row).  f64
+  // keys: the GROUP BY columns (none: one output row, a reduction).  f64
   // keys group by value: in key order they are sorted as doubles (order-preserving keys,
   // mapped back afterwards), in hash order the hash covers the doubles' 8 bytes as the
   // reference's merge does (merge.cu:131-133).
@@ -910,15 +910,7 @@ static int do_select(const char *s, const char *f, int grp_cnt) {
   std::vector<const int64_t *> keys;
   std::vector<int> keydec;
   std::vector<bool> keyf64;
-  if (grp_cnt == 0) {
-    DBufP z = dalloc(8 * (v.nrows ? v.nrows : 1));
-    if (!z) return fail(ALN_E_NOMEM, "device allocation failed");
-    if (v.nrows && hipMemset(z->p, 0, 8 * v.nrows) != hipSuccess) return fail(ALN_E_HIP, "memset failed");
-    keybufs.push_back(z);
-    keys.push_back((const int64_t *)z->p);
-    keydec.push_back(0);
-    keyf64.push_back(false);
-  } else {
+  if (grp_cnt > 0) {
     if (grp_cnt > ALN_MAX_KEYS) return fail(ALN_E_UNSUPPORTED, "GROUP BY on more than %d columns", ALN_MAX_KEYS);
     for (auto &g : groups) {
       DBufP d;

@@ -269,7 +269,8 @@ enum {
 };
 #define ALN_MAX_KEYS 4
 #define ALN_MAX_AGGS 8
-/* d_keys/d_vals/d_out_*: host arrays of device pointers.  COUNT ignores its value column. */
+/* d_keys/d_vals/d_out_*: host arrays of device pointers.  COUNT ignores its value column.
+ * nkeys 0 (no GROUP BY): one group over all n rows, a reduction (reduce, select.cu:549). */
 int aln_groupby(aln_ctx *ctx, const int64_t *const *d_keys, int nkeys,
                 const int64_t *const *d_vals, const int32_t *aggs, int nvals, uint64_t n,
                 int order, uint32_t seed, int64_t *const *d_out_keys,

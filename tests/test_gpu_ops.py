@@ -204,6 +204,83 @@ def test_groupby_wide_keys(ctx):
             assert np.array_equal(host(a), b)
 
 
+def _gb_check(ctx, keys, vals, aggs, order=None, bounds=None):
+    from alenka_amd import device as D
+
+    order = L.ORDER_KEYS if order is None else order
+    ok, ov = D.groupby(ctx, [dev(ctx, x) for x in keys], [dev(ctx, v) for v in vals], aggs, order=order, seed=100)
+    wk, wv = O.groupby(keys, vals, aggs, seg_bounds=bounds, seed=100)
+    for a, b in zip(ok + ov, wk + wv):
+        assert np.array_equal(host(a), b)
+    return len(wk[0])
+
+
+AGGS4 = [L.AGG_SUM, L.AGG_COUNT, L.AGG_MIN, L.AGG_MAX]
+
+
+@pytest.mark.parametrize("n", [1, 1000, 3_000_007])
+def test_groupby_no_keys(ctx, n):
+    """No GROUP BY: one group, a reduction over all rows (reduce, select.cu:549), against
+    numpy's sum / min / max / len; f64 SUM within 1e-12 of math.fsum, f64 MIN / MAX exact."""
+    import math
+
+    from alenka_amd import device as D
+
+    r = np.random.default_rng(n)
+    v = r.integers(-(1 << 40), 1 << 40, n)
+    ok, ov = D.groupby(ctx, [], [dev(ctx, v)] * 4, AGGS4)
+    assert ok == [] and all(x.numel() == 1 for x in ov)
+    assert [int(host(x)[0]) for x in ov] == [int(v.sum()), n, int(v.min()), int(v.max())]
+    f = r.standard_normal(n) * 1e6
+    fb = torch.as_tensor(f.view(np.int64)).to(ctx.device)
+    _, fv = D.groupby(ctx, [], [fb] * 3, [L.AGG_FSUM, L.AGG_FMIN, L.AGG_FMAX])
+    got = [host(x).view(np.float64)[0] for x in fv]
+    want = math.fsum(f)
+    assert abs(got[0] - want) <= 1e-12 * max(1.0, np.abs(f).sum())
+    assert got[1] == f.min() and got[2] == f.max()
+
+
+def test_groupby_constant_key(ctx):
+    """Keys that never vary: one group with that key."""
+    n = 500_001
+    r = np.random.default_rng(7)
+    assert _gb_check(ctx, [np.full(n, -42, np.int64)], [r.integers(-100, 100, n)] * 4, AGGS4) == 1
+
+
+@pytest.mark.parametrize("case", ["g10", "g1000_2keys", "dense_2e20", "zipf_sparse", "retry_3m", "clustered"])
+def test_groupby_paths(ctx, case):
+    """The group-by paths against the oracle: direct-indexed LDS slots (10 groups), hashed
+    LDS slots over direct-indexed globals (1000 groups of 2 keys), direct-indexed globals
+    (2^20 codes), the hash table with hot keys (Zipf over 40-bit keys), more than 2^20
+    groups (the first table overflows; the rows are aggregated again in a 2n table), and
+    sorted input (runs of equal keys in adjacent lanes); key order and hash order."""
+    r = np.random.default_rng(len(case))
+    if case == "g10":
+        n = 2_000_003
+        keys = [r.integers(0, 10, n) * 7 + 19920101]
+    elif case == "g1000_2keys":
+        n = 1_000_000
+        keys = [r.integers(0, 40, n), r.integers(-12, 13, n)]
+    elif case == "dense_2e20":
+        n = 1 << 20
+        keys = [r.integers(0, 1 << 20, n)]
+    elif case == "zipf_sparse":
+        n = 2_000_000
+        z = np.minimum(r.zipf(1.2, n), 1 << 20)
+        keys = [z * 1_000_003 + 5]
+    elif case == "retry_3m":
+        n = 3_000_000
+        keys = [r.integers(0, 1 << 40, n)]
+    else:
+        n = 1_500_000
+        keys = [np.sort(r.integers(0, 100_000, n))]
+    v = r.integers(-(1 << 30), 1 << 30, n)
+    g = _gb_check(ctx, keys, [v] * 4, AGGS4)
+    assert g > 1
+    bounds = [0, n // 3, 2 * n // 3, n]
+    _gb_check(ctx, keys, [v, v], [L.AGG_SUM, L.AGG_COUNT], order=L.ORDER_HASH, bounds=bounds)
+
+
 # ---- join --------------------------------------------------------------------------------
 @pytest.mark.parametrize("case", ["dense", "dups", "sparse", "wide", "dense_big"])
 def test_hash_join(ctx, case):
