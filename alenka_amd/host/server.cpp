@@ -3,9 +3,12 @@
 // table, or star pipelines (aln_star_args).  Each of `streams` worker slots owns an aln_ctx
 // on its own HIP stream and a device region; a slot that comes free takes the oldest waiting
 // request and, in batched mode, every other waiting query on the same table that fits the
-// batch and the memory budget, and answers them with one shared scan.  Requests that arrive
-// while slots are busy are admitted as slots and budget free up, so batches overlap on the
-// device.  In plain mode every query runs alone through the per-statement operators
+// batch and the memory budget, and answers them with one shared scan - or, for filter +
+// GROUP BY requests, runs the oldest alone through the operators when that has been
+// measured to take less time per request on the table (a single request always does).
+// Requests that arrive while slots are busy are admitted as slots and budget free up, so
+// batches overlap on the device.  In plain mode every query runs alone through the
+// per-statement operators
 // (aln_scan_filter -> aln_eval -> aln_groupby), the way the reference runs a script
 // statement by statement (emit_filter operators.cu:2306, emit_select 1801-2084).
 #include <string.h>
@@ -32,6 +35,11 @@ struct Table {
   uint32_t seg_rows = 0;
   uint64_t nrows = 0;
   std::vector<int64_t> cmin, cmax;  // per column, from the zone maps (or a device scan)
+  // milliseconds per filter + GROUP BY request on this table, run alone through the
+  // operators and inside a shared-scan batch (running means under the server lock; 0 = not
+  // measured yet), and the batch decisions taken since batching was last tried
+  double plain_ms = 0, batch_ms = 0;
+  uint32_t since_batch = 0;
 };
 
 struct Request {
@@ -92,7 +100,13 @@ double ms_between(Clock::time_point a, Clock::time_point b) {
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
-// Device bytes a request holds while it runs (what admission charges against the budget).
+// Device bytes of a filter + GROUP BY run through the operators: row ids, two evaluated
+// columns, group-by workspace and outputs.
+size_t plain_mem(const Table &t, uint64_t g) { return (size_t)(4 + 8 + 8 + 48) * t.nrows + 24 * g + (16u << 20); }
+
+// Device bytes a request holds while it runs (what admission charges against the budget):
+// plain, the operators' bytes; batched, dense group tables + compaction + outputs (a
+// batched-mode request run through the operators is charged their bytes when it starts).
 size_t request_mem(const aln_server *s, const Request &r) {
   if (r.kind == 1) {
     // bitmaps over both key ranges, per-block counts, group tables and candidates
@@ -104,10 +118,33 @@ size_t request_mem(const aln_server *s, const Request &r) {
   }
   const Table &t = s->tables[r.table];
   const uint64_t g = r.groups ? r.groups : 1;
-  if (s->cfg.plain)  // row ids, two evaluated columns, group-by sort/hash workspace and outputs
-    return (size_t)(4 + 8 + 8 + 48) * t.nrows + 24 * g + (16u << 20);
-  return align256(12 * g) + align256(36 * g) + 4096;  // dense tables + compaction + outputs
+  return s->cfg.plain ? plain_mem(t, g) : align256(12 * g) + align256(36 * g) + 4096;
 }
+
+// Batched mode, filter + GROUP BY: whether the requests taken together run through one
+// shared scan (true) or each alone through the operators (caller holds the lock).
+// A single request has nothing to share; a batch runs batched unless the measured
+// per-request time of batches on this table is not below that of plain runs, and then
+// every 8th such batch still runs batched so the estimate follows the load.
+constexpr uint32_t BATCH_RETRY = 8;
+
+bool use_batch(aln_server *s, const std::vector<Request *> &rs) {
+  Table &t = s->tables[rs[0]->table];
+  const uint64_t g = rs[0]->groups ? rs[0]->groups : 1;
+  if (s->mem_in_use - rs[0]->mem + plain_mem(t, g) > s->cfg.budget_bytes) return true;  // no room
+  if (rs.size() < 2) return false;
+  if (t.plain_ms <= 0 || t.batch_ms <= 0 || t.batch_ms < t.plain_ms) {
+    t.since_batch = 0;
+    return true;
+  }
+  if (++t.since_batch >= BATCH_RETRY) {
+    t.since_batch = 0;
+    return true;
+  }
+  return false;
+}
+
+void note_time(double *mean, double ms) { *mean = *mean <= 0 ? ms : 0.8 * *mean + 0.2 * ms; }
 
 void finish(aln_server *s, Request *r, int status, const char *err) {
   r->status = status;
@@ -318,6 +355,20 @@ void slot_main(aln_server *s, Slot *sl) {
     }
     std::vector<Request *> rs = take_batch(s);
     if (rs.empty()) continue;
+    const bool batched = rs[0]->kind == 0 && !s->cfg.plain && use_batch(s, rs);
+    if (rs[0]->kind == 0 && !s->cfg.plain && !batched) {
+      // run the oldest alone through the operators, charged their bytes; the others go back
+      // to the front of the queue, in order, for the next free slot
+      for (size_t i = rs.size(); i-- > 1;) {
+        s->mem_in_use -= rs[i]->mem;
+        s->pending.push_front(rs[i]);
+      }
+      rs.resize(1);
+      const size_t pm = plain_mem(s->tables[rs[0]->table], rs[0]->groups ? rs[0]->groups : 1);
+      s->mem_in_use += pm - rs[0]->mem;
+      rs[0]->mem = pm;
+      s->cv_work.notify_all();
+    }
     const uint64_t bid = s->next_batch++;
     ++s->running;
     s->st.max_concurrent = std::max<uint64_t>(s->st.max_concurrent, (uint64_t)s->running);
@@ -328,15 +379,18 @@ void slot_main(aln_server *s, Slot *sl) {
       r->batch_size = (int)rs.size();
     }
     lk.unlock();
-    int rc;
+    int rc = ALN_OK;
     if (rs[0]->kind == 1)
       rc = run_star_batch(sl, rs);
-    else if (s->cfg.plain)
-      rc = run_plain(s, sl, rs[0]);
-    else
+    else if (batched)
       rc = run_batched(s, sl, rs);
+    else
+      for (size_t i = 0; rc == ALN_OK && i < rs.size(); ++i) rc = run_plain(s, sl, rs[i]);
     const char *err = rc == ALN_OK ? nullptr : aln_last_error();
+    const double per = ms_between(t0, Clock::now()) / (double)rs.size();
     lk.lock();
+    if (rs[0]->kind == 0 && rc == ALN_OK) note_time(batched ? &s->tables[rs[0]->table].batch_ms
+                                                            : &s->tables[rs[0]->table].plain_ms, per);
     --s->running;
     size_t mem = 0;
     for (Request *r : rs) {
@@ -346,7 +400,7 @@ void slot_main(aln_server *s, Slot *sl) {
     s->mem_in_use -= mem;
     s->st.batches += 1;
     s->st.completed += rs.size();
-    if (rs.size() > 1) s->st.batched_queries += rs.size();
+    if (rs.size() > 1 && (batched || rs[0]->kind == 1)) s->st.batched_queries += rs.size();
     s->cv_done.notify_all();
     s->cv_work.notify_all();
   }
