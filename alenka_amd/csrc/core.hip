@@ -268,25 +268,42 @@ __global__ void __launch_bounds__(SCAN_BLOCK) k_scan_sums(uint64_t *sums, uint64
   if (threadIdx.x == 0) *total = carry;
 }
 
+// Each wave scans 16 rows of 64 consecutive items (a row per load, so loads and stores are
+// coalesced), carrying the row totals across the rows; then the waves' totals and the
+// tile's offset are added.
 template <typename T>
 __global__ void __launch_bounds__(SCAN_BLOCK) k_tile_scan(const T *__restrict__ in, uint64_t n,
                                                           const uint64_t *__restrict__ offs,
                                                           uint64_t *__restrict__ out) {
-  // blocked arrangement within the tile: thread t owns items t*ITEMS .. t*ITEMS+ITEMS-1
-  uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
+  __shared__ uint64_t wsum[SCAN_BLOCK / 64];
+  const unsigned lane = __lane_id(), wid = threadIdx.x / 64;
+  const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)wid * (64 * SCAN_ITEMS) + lane;
   uint64_t v[SCAN_ITEMS];
-  uint64_t s = 0;
+#pragma unroll
   for (int j = 0; j < SCAN_ITEMS; ++j) {
-    uint64_t i = base + j;
+    const uint64_t i = base + (uint64_t)j * 64;
     v[j] = i < n ? (uint64_t)in[i] : 0;
-    s += v[j];
   }
-  uint64_t tot;
-  uint64_t run = offs[blockIdx.x] + block_excl_scan<T>(s, &tot);
+  uint64_t carry = 0;
+#pragma unroll
   for (int j = 0; j < SCAN_ITEMS; ++j) {
-    uint64_t i = base + j;
-    if (i < n) out[i] = run;
-    run += v[j];
+    uint64_t x = v[j];
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint64_t y = __shfl_up(x, o, 64);
+      if (lane >= (unsigned)o) x += y;
+    }
+    const uint64_t row = __shfl(x, 63, 64);
+    v[j] = carry + x - v[j];  // exclusive within the wave's part of the tile
+    carry += row;
+  }
+  if (lane == 0) wsum[wid] = carry;
+  __syncthreads();
+  uint64_t off = offs[blockIdx.x];
+  for (unsigned w = 0; w < wid; ++w) off += wsum[w];
+#pragma unroll
+  for (int j = 0; j < SCAN_ITEMS; ++j) {
+    const uint64_t i = base + (uint64_t)j * 64;
+    if (i < n) out[i] = off + v[j];
   }
 }
 
