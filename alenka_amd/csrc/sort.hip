@@ -243,27 +243,14 @@ __global__ void __launch_bounds__(RS_SCAN_BLOCK) k_rs_rowscan(const uint32_t *__
   if (t == RS_SCAN_BLOCK - 1) tot[d] = before;
 }
 
-// base[d] = items of digits below d (one block).
-__global__ void __launch_bounds__(RS_BINS) k_rs_base(const uint32_t *__restrict__ tot,
-                                                     uint32_t *__restrict__ base) {
-  __shared__ uint32_t wsum[RS_BINS / 64];
-  const uint32_t d = threadIdx.x, lane = __lane_id(), wid = d / 64;
-  uint32_t x = tot[d];
-  uint32_t incl = wave_incl_scan(x);
-  if (lane == 63) wsum[wid] = incl;
-  __syncthreads();
-  uint32_t before = incl - x;
-  for (uint32_t w = 0; w < wid; ++w) before += wsum[w];
-  base[d] = before;
-}
-
 // One pass: digit = (stored key >> sh) & mask; keys are written as (stored key >> oshift)
 // in KO bytes (KO 0: not written).  Tile b of count block s = b / RS_SUB stores digit d
-// from position base[d] + excl[s * 256 + d] + sub[b * 256 + d].
+// from position base[d] + excl[s * 256 + d] + sub[b * 256 + d], base[d] = the items of
+// the digits below d (an exclusive scan of tot, made by every block).
 struct RsPass {
   RsIn in;
   uint64_t n;
-  const uint32_t *base, *excl;
+  const uint32_t *tot, *excl;
   const uint16_t *sub;
   void *kout;
   uint32_t *vout;
@@ -280,7 +267,7 @@ __global__ void __launch_bounds__(RS_BLOCK) k_rs_scatter(RsPass a) {
   __shared__ uint32_t sv[RS_TILE];
   __shared__ uint16_t wc[RS_WAVES][RS_BINS];
   __shared__ uint32_t gdel[RS_BINS];
-  __shared__ uint32_t wsum[RS_BINS / 64];
+  __shared__ uint32_t wsum[RS_BINS / 64], bsum[RS_BINS / 64];
   const unsigned t = threadIdx.x, lane = __lane_id(), wid = t / 64;
   // Workgroups go to the 8 XCDs round-robin: give each XCD a contiguous range of tiles, so
   // neighbouring tiles' runs of a digit (adjacent in the output) meet in the same L2.
@@ -300,9 +287,11 @@ __global__ void __launch_bounds__(RS_BLOCK) k_rs_scatter(RsPass a) {
   const uint64_t base = tbase + (uint64_t)wid * RS_WAVE_ITEMS;
   // this tile's first output position of digit t, read before the items so that its
   // latency overlaps theirs
-  uint32_t first = 0;
-  if (t < RS_BINS)
-    first = a.base[t] + a.excl[(uint64_t)(tile / RS_SUB) * RS_BINS + t] + a.sub[(uint64_t)tile * RS_BINS + t];
+  uint32_t first = 0, dtot = 0;
+  if (t < RS_BINS) {
+    first = a.excl[(uint64_t)(tile / RS_SUB) * RS_BINS + t] + a.sub[(uint64_t)tile * RS_BINS + t];
+    dtot = a.tot[t];
+  }
   uint64_t key[RS_ITEMS];
   uint32_t val[RS_ITEMS], rank[RS_ITEMS];
   rs_load<KI, RS_ITEMS, true>(a.in, n, base + lane, 64, key, val);
@@ -327,7 +316,7 @@ __global__ void __launch_bounds__(RS_BLOCK) k_rs_scatter(RsPass a) {
     }
   }
   __syncthreads();
-  uint32_t cnt = 0, incl = 0;
+  uint32_t cnt = 0, incl = 0, bincl = 0;
   if (t < RS_BINS) {
 #pragma unroll
     for (int w = 0; w < RS_WAVES; ++w) {
@@ -336,15 +325,22 @@ __global__ void __launch_bounds__(RS_BLOCK) k_rs_scatter(RsPass a) {
       cnt += x;
     }
     incl = wave_incl_scan(cnt);
-    if (lane == 63) wsum[wid] = incl;
+    bincl = wave_incl_scan(dtot);
+    if (lane == 63) {
+      wsum[wid] = incl;
+      bsum[wid] = bincl;
+    }
   }
   __syncthreads();
   if (t < RS_BINS) {
-    uint32_t tstart = incl - cnt;
-    for (unsigned w = 0; w < wid; ++w) tstart += wsum[w];
+    uint32_t tstart = incl - cnt, dbase = bincl - dtot;
+    for (unsigned w = 0; w < wid; ++w) {
+      tstart += wsum[w];
+      dbase += bsum[w];
+    }
 #pragma unroll
     for (int w = 0; w < RS_WAVES; ++w) wc[w][t] = (uint16_t)(wc[w][t] + tstart);
-    gdel[t] = first - tstart;
+    gdel[t] = dbase + first - tstart;
   }
   __syncthreads();
 #pragma unroll
@@ -448,14 +444,13 @@ static int rs_sort(aln_ctx *c, RsIn in, bool from_keys, uint64_t n, uint32_t bit
     uint32_t pb = bits - RS_BITS * p < RS_BITS ? bits - RS_BITS * p : RS_BITS;
     return (1u << pb) - 1;
   };
-  uint32_t *cnt = nullptr, *excl = nullptr, *tot = nullptr, *dbase = nullptr;
+  uint32_t *cnt = nullptr, *excl = nullptr, *tot = nullptr;
   uint16_t *sub = nullptr;
   unsigned long long *d_vary = nullptr;
   if (passes > 0) {
     ALN_TRY(ws(c, ncnt, &cnt));
     ALN_TRY(ws(c, ncnt, &excl));
     ALN_TRY(ws(c, (size_t)RS_BINS, &tot));
-    ALN_TRY(ws(c, (size_t)RS_BINS, &dbase));
     ALN_TRY(ws(c, (size_t)nsuper * RS_SUB * RS_BINS, &sub));
     ALN_TRY(ws(c, 1, &d_vary));
   }
@@ -520,7 +515,7 @@ static int rs_sort(aln_ctx *c, RsIn in, bool from_keys, uint64_t n, uint32_t bit
       a.in.vals = vprev;
     }
     a.n = n;
-    a.base = dbase;
+    a.tot = tot;
     a.excl = excl;
     a.sub = sub;
     a.kout = ko[r] == 0 ? nullptr : direct ? static_cast<void *>(kout) : kt[(nrun - 1 - r) & 1];
@@ -536,7 +531,6 @@ static int rs_sort(aln_ctx *c, RsIn in, bool from_keys, uint64_t n, uint32_t bit
     {
       ProfScope ps(c, "radix_scan");
       hipLaunchKernelGGL(k_rs_rowscan, dim3(RS_BINS), dim3(RS_SCAN_BLOCK), 0, c->stream, cnt, nsuper, excl, tot);
-      hipLaunchKernelGGL(k_rs_base, dim3(1), dim3(RS_BINS), 0, c->stream, tot, dbase);
     }
     ALN_TRY(check_launch("radix_scan"));
     {
