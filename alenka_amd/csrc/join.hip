@@ -169,18 +169,21 @@ __global__ void k_join_runs(const uint64_t *code, const uint32_t *head, const ui
 
 // ---- dense unique build keys: bit + rank table -----------------------------------------------
 // When the build keys are unique and their range is at most JD_DENSITY per build row, the
-// build side becomes a table over the key range with one 8-byte entry per 32 keys: the bits
-// of the keys present and the number of keys before the entry (its rank base), plus the
-// build rows in key order (rowid[rank]).  At 10M build keys over a 20M range the table is
-// 5 MB (L2 / MALL resident): a probe reads one entry, and a hit one row id.
+// build side becomes a table over the key range with one 32-byte group per 224 keys: seven
+// words with the bits of the keys present and the number of keys before the group (its
+// rank base), plus the build rows in key order (rowid[rank]).  At 10M build keys over a 20M
+// range the table is 2.9 MB, which stays in an XCD's 4 MB L2 (a probe reads one aligned
+// 32-byte group, inside one cache line); a hit then reads one row id.
 //   build: a byte per key (plain byte stores, no atomics: every XCD's L2 sees them merged at
-//          write-back), packed into the entries with a block scan of the counts, then the row
+//          write-back), packed into the groups with a block scan of the counts, then the row
 //          ids scattered to their ranks;
 //   probe: pass 0 streams the probe keys and writes each row's build row (or none) and the
 //          tile's pair count; pass 1 streams those and writes the pairs of a tile through LDS,
 //          so the stores to the output are whole, contiguous lines.
 constexpr uint64_t JD_DENSITY = 64;  // key range <= 64 x build rows (and <= 2^31)
-constexpr int JD_BLOCK = 256;        // entries (32 keys each) per block of the packing pass
+constexpr int JD_BLOCK = 256;        // groups per block of the packing pass
+constexpr uint32_t JD_GWORDS = 7;    // bit words per group (the eighth word is the rank base)
+constexpr uint32_t JD_GKEYS = 32 * JD_GWORDS;
 
 __global__ void __launch_bounds__(256) k_jd_bytes(const int64_t *__restrict__ keys, uint64_t n, int64_t kmin,
                                                   uint8_t *bytes) {
@@ -188,24 +191,34 @@ __global__ void __launch_bounds__(256) k_jd_bytes(const int64_t *__restrict__ ke
     bytes[(uint64_t)keys[i] - (uint64_t)kmin] = 1;
 }
 
-// Entry w: the bits of keys 32w .. 32w+31 from the byte map; pass 0 writes the keys per
-// block, pass 1 the entries with their rank bases (block offset + keys of earlier entries).
-template <int PASS>
-__global__ void __launch_bounds__(JD_BLOCK) k_jd_pack(const uint8_t *__restrict__ bytes, uint64_t nent,
-                                                      uint32_t *__restrict__ bcount, const uint64_t *__restrict__ boff,
-                                                      uint2 *__restrict__ ent) {
-  __shared__ uint32_t wsum[JD_BLOCK / 64];
-  const uint64_t w = (uint64_t)blockIdx.x * JD_BLOCK + threadIdx.x;
+// Bits of 32 keys from 32 bytes that are each 0 or 1 (bit 0 of each byte, in order).
+__device__ __forceinline__ uint32_t jd_byte_bits(uint4 x, uint4 y) {
+  const uint32_t v[8] = {x.x, x.y, x.z, x.w, y.x, y.y, y.z, y.w};
   uint32_t b = 0;
-  if (w < nent) {
-    const uint4 *q = reinterpret_cast<const uint4 *>(bytes + w * 32);
-    const uint4 x = q[0], y = q[1];
-    const uint32_t v[8] = {x.x, x.y, x.z, x.w, y.x, y.y, y.z, y.w};
 #pragma unroll
-    for (int k = 0; k < 8; ++k)  // byte j of word k is 0 or 1: gather bit 0 of each byte
-      b |= (((v[k] & 1u) | ((v[k] >> 7) & 2u) | ((v[k] >> 14) & 4u) | ((v[k] >> 21) & 8u))) << (4 * k);
+  for (int k = 0; k < 8; ++k)
+    b |= (((v[k] & 1u) | ((v[k] >> 7) & 2u) | ((v[k] >> 14) & 4u) | ((v[k] >> 21) & 8u))) << (4 * k);
+  return b;
+}
+
+// Group g: the bits of keys 224g .. 224g+223 from the byte map; pass 0 writes the keys per
+// block, pass 1 the groups with their rank bases (block offset + keys of earlier groups).
+template <int PASS>
+__global__ void __launch_bounds__(JD_BLOCK) k_jd_pack(const uint8_t *__restrict__ bytes, uint64_t ngrp,
+                                                      uint32_t *__restrict__ bcount, const uint64_t *__restrict__ boff,
+                                                      uint4 *__restrict__ grp) {
+  __shared__ uint32_t wsum[JD_BLOCK / 64];
+  const uint64_t g = (uint64_t)blockIdx.x * JD_BLOCK + threadIdx.x;
+  uint32_t w[JD_GWORDS] = {0, 0, 0, 0, 0, 0, 0};
+  uint32_t n = 0;
+  if (g < ngrp) {
+    const uint4 *q = reinterpret_cast<const uint4 *>(bytes + g * JD_GKEYS);
+#pragma unroll
+    for (uint32_t k = 0; k < JD_GWORDS; ++k) {
+      w[k] = jd_byte_bits(q[2 * k], q[2 * k + 1]);
+      n += (uint32_t)__popc(w[k]);
+    }
   }
-  const uint32_t n = (uint32_t)__popc(b);
   const unsigned lane = __lane_id(), wid = threadIdx.x / 64;
   uint32_t x = n;
   for (int o = 1; o < 64; o <<= 1) {
@@ -224,33 +237,44 @@ __global__ void __launch_bounds__(JD_BLOCK) k_jd_pack(const uint8_t *__restrict_
   }
   uint32_t before = 0;
   for (int k = 0; k < (int)wid; ++k) before += wsum[k];
-  if (w < nent) ent[w] = make_uint2(b, (uint32_t)boff[blockIdx.x] + before + x - n);
+  if (g < ngrp) {
+    grp[2 * g] = make_uint4(w[0], w[1], w[2], w[3]);
+    grp[2 * g + 1] = make_uint4(w[4], w[5], w[6], (uint32_t)boff[blockIdx.x] + before + x - n);
+  }
 }
 
-// A key's build row (J_EMPTY: none) from its code.
-__device__ __forceinline__ uint32_t jd_find(const uint2 *__restrict__ ent, const uint32_t *__restrict__ rowid,
-                                            uint32_t c) {
-  const uint2 e = ent[c >> 5];
-  const uint32_t bit = 1u << (c & 31);
-  if (!(e.x & bit)) return J_EMPTY;
-  return rowid[e.y + (uint32_t)__popc(e.x & (bit - 1u))];
+// Rank of key code c among the build keys (*hit: whether c is one of them), from its group.
+__device__ __forceinline__ uint32_t jd_rank(uint4 lo, uint4 hi, uint32_t c, bool *hit) {
+  const uint32_t wi = c >> 5, k = wi - (wi / JD_GWORDS) * JD_GWORDS, bit = 1u << (c & 31);
+  const uint32_t w[JD_GWORDS] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z};
+  uint32_t r = hi.w, word = 0;
+#pragma unroll
+  for (uint32_t j = 0; j < JD_GWORDS; ++j) {
+    if (j < k) r += (uint32_t)__popc(w[j]);
+    if (j == k) word = w[j];
+  }
+  *hit = (word & bit) != 0;
+  return r + (uint32_t)__popc(word & (bit - 1u));
 }
+
+__device__ __forceinline__ uint64_t jd_group(uint32_t c) { return (uint64_t)((c >> 5) / JD_GWORDS); }
 
 __global__ void __launch_bounds__(256) k_jd_rows(const int64_t *__restrict__ keys, uint64_t n, int64_t kmin,
-                                                 const uint2 *__restrict__ ent, uint32_t *rowid) {
+                                                 const uint4 *__restrict__ grp, uint32_t *rowid) {
   for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
     const uint32_t c = (uint32_t)((uint64_t)keys[i] - (uint64_t)kmin);
-    const uint2 e = ent[c >> 5];
-    rowid[e.y + (uint32_t)__popc(e.x & ((1u << (c & 31)) - 1u))] = (uint32_t)i;
+    const uint64_t g = jd_group(c);
+    bool hit;
+    rowid[jd_rank(grp[2 * g], grp[2 * g + 1], c, &hit)] = (uint32_t)i;
   }
 }
 
 // Pass 0: each probe row's build row into match (coalesced), and the tile's pair count.  A
 // thread's loads go out stage by stage for all of its rows at once - the keys, then their
-// entries, then the row ids of the hits - so a thread waits three round trips, not one per
-// row and level (rows outside the key range read entry 0 and are masked afterwards).
+// groups, then the row ids of the hits - so a thread waits three round trips, not one per
+// row and level (rows outside the key range read group 0 and are masked afterwards).
 __global__ void __launch_bounds__(J_BLOCK) k_jd_match(const int64_t *__restrict__ pk, uint64_t np,
-                                                      const uint2 *__restrict__ ent,
+                                                      const uint4 *__restrict__ grp,
                                                       const uint32_t *__restrict__ rowid, int64_t kmin,
                                                       uint64_t range, uint32_t *__restrict__ match,
                                                       uint32_t *__restrict__ tcount) {
@@ -263,19 +287,22 @@ __global__ void __launch_bounds__(J_BLOCK) k_jd_match(const int64_t *__restrict_
     key[j] = p < np ? __builtin_nontemporal_load(pk + p) : kmin - 1;
   }
   uint32_t code[J_ITEMS];
-  uint2 e[J_ITEMS];
+  uint4 lo[J_ITEMS], hi[J_ITEMS];
 #pragma unroll
   for (int j = 0; j < J_ITEMS; ++j) {
     const uint64_t c = (uint64_t)key[j] - (uint64_t)kmin;
     code[j] = c < range ? (uint32_t)c : 0xFFFFFFFFu;
-    e[j] = ent[code[j] != 0xFFFFFFFFu ? code[j] >> 5 : 0u];
+    const uint64_t g = code[j] != 0xFFFFFFFFu ? jd_group(code[j]) : 0u;
+    lo[j] = grp[2 * g];
+    hi[j] = grp[2 * g + 1];
   }
   uint32_t m[J_ITEMS], n = 0;
 #pragma unroll
   for (int j = 0; j < J_ITEMS; ++j) {
-    const uint32_t bit = 1u << (code[j] & 31);
-    const bool h = code[j] != 0xFFFFFFFFu && (e[j].x & bit);
-    m[j] = rowid[h ? e[j].y + (uint32_t)__popc(e[j].x & (bit - 1u)) : 0u];
+    bool h;
+    const uint32_t r = jd_rank(lo[j], hi[j], code[j] != 0xFFFFFFFFu ? code[j] : 0u, &h);
+    h = h && code[j] != 0xFFFFFFFFu;
+    m[j] = rowid[h ? r : 0u];
     m[j] = h ? m[j] : J_EMPTY;
     n += h;
   }
@@ -364,22 +391,22 @@ static int join_dense(aln_ctx *c, const int64_t *bk, uint64_t nb, const int64_t 
   const uint64_t ntiles = (np + J_TILE - 1) / J_TILE;
   uint32_t *match, *tcount;
   uint64_t *toff;
-  const uint64_t nent = (range + 31) / 32, nblk = (nent + JD_BLOCK - 1) / JD_BLOCK;
+  const uint64_t ngrp = (range + JD_GKEYS - 1) / JD_GKEYS, nblk = (ngrp + JD_BLOCK - 1) / JD_BLOCK;
   uint8_t *bytes;
   uint32_t *bcount, *rowid;
   uint64_t *boff;
-  uint2 *ent;
-  ALN_TRY(ws(c, nent * 32, &bytes));  // whole entries: the packing pass reads 32 bytes each
+  uint4 *grp;
+  ALN_TRY(ws(c, ngrp * JD_GKEYS, &bytes));  // whole groups: the packing pass reads 224 bytes each
   ALN_TRY(ws(c, nblk, &bcount));
   ALN_TRY(ws(c, nblk, &boff));
-  ALN_TRY(ws(c, nent, &ent));
-  ALN_TRY(fill_u32(c, reinterpret_cast<uint32_t *>(bytes), nent * 8, 0));
+  ALN_TRY(ws(c, 2 * ngrp, &grp));
+  ALN_TRY(fill_u32(c, reinterpret_cast<uint32_t *>(bytes), ngrp * (JD_GKEYS / 4), 0));
   uint64_t distinct = 0;
   {
     ProfScope p(c, "join_build");
     hipLaunchKernelGGL(k_jd_bytes, dim3(grid_for(nb, 256, 65536)), dim3(256), 0, c->stream, bk, nb, kmin, bytes);
     hipLaunchKernelGGL(k_jd_pack<0>, dim3((unsigned)nblk), dim3(JD_BLOCK), 0, c->stream, (const uint8_t *)bytes,
-                       nent, bcount, (const uint64_t *)nullptr, (uint2 *)nullptr);
+                       ngrp, bcount, (const uint64_t *)nullptr, (uint4 *)nullptr);
   }
   ALN_TRY(check_launch("join_dense_bytes"));
   ALN_TRY(exclusive_scan_u32(c, bcount, nblk, boff, &distinct));
@@ -389,9 +416,9 @@ static int join_dense(aln_ctx *c, const int64_t *bk, uint64_t nb, const int64_t 
   {
     ProfScope p(c, "join_build");
     hipLaunchKernelGGL(k_jd_pack<1>, dim3((unsigned)nblk), dim3(JD_BLOCK), 0, c->stream, (const uint8_t *)bytes,
-                       nent, bcount, (const uint64_t *)boff, ent);
+                       ngrp, bcount, (const uint64_t *)boff, grp);
     hipLaunchKernelGGL(k_jd_rows, dim3(grid_for(nb, 256, 65536)), dim3(256), 0, c->stream, bk, nb, kmin,
-                       (const uint2 *)ent, rowid);
+                       (const uint4 *)grp, rowid);
   }
   ALN_TRY(check_launch("join_dense_rows"));
   ALN_TRY(ws(c, np, &match));
@@ -399,7 +426,7 @@ static int join_dense(aln_ctx *c, const int64_t *bk, uint64_t nb, const int64_t 
   ALN_TRY(ws(c, ntiles, &toff));
   {
     ProfScope p(c, "join_probe");
-    hipLaunchKernelGGL(k_jd_match, dim3((unsigned)ntiles), dim3(J_BLOCK), 0, c->stream, pk, np, (const uint2 *)ent,
+    hipLaunchKernelGGL(k_jd_match, dim3((unsigned)ntiles), dim3(J_BLOCK), 0, c->stream, pk, np, (const uint4 *)grp,
                        (const uint32_t *)rowid, kmin, range, match, tcount);
   }
   ALN_TRY(check_launch("join_dense_match"));
