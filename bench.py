@@ -690,6 +690,43 @@ def widened_rows(ctx, D, seed, n=100_000_000, reps=5):
     return out
 
 
+def op_leg(ctx, D, op, seed, steps, warmup, sort_rows):
+    """One operator alone, for profiling it: `sort` = ORDER BY of sort_rows (int64 key, row
+    number) rows; `join` = 100M probe rows x 10M unique build keys over a 20M range (the
+    semi join's inner join).  Prints one JSON line of per-call times and returns."""
+    import numpy as np
+    import torch
+
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    if op == "sort":
+        keys, _ = D.gen_sort_input(ctx, seed, sort_rows, payload=False)
+        fn = lambda: D.sort_keys(ctx, [keys])  # noqa: E731
+    else:
+        probe = torch.randint(0, 20_000_000, (100_000_000,), generator=g, device="cuda", dtype=torch.int64)
+        build = torch.randperm(20_000_000, generator=g, device="cuda")[:10_000_000].to(torch.int64)
+        fn = lambda: D.hash_join(ctx, build, probe)  # noqa: E731
+    for _ in range(warmup):
+        out = fn()
+    ts = []
+    for _ in range(steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = fn()
+        torch.cuda.synchronize()
+        ts.append((time.perf_counter() - t0) * 1e3)
+    if op == "sort":
+        sk = keys[out.long()]
+        ok = bool((sk[1:] >= sk[:-1]).all().item())
+        rec = {"op": "sort", "rows": sort_rows, "checked_sorted": ok}
+    else:
+        pi, bi = out
+        ok = bool((build[bi.long()] == probe[pi.long()]).all().item())
+        rec = {"op": "join", "probe_rows": 100_000_000, "build_rows": 10_000_000, "pairs": int(pi.numel()),
+               "checked_keys_equal": ok}
+    rec.update({"steps": steps, "warmup": warmup, "ms": ts, "ms_median": float(np.median(ts))})
+    print(json.dumps(rec), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -716,6 +753,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-seconds", type=float, default=8.0)
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
+    ap.add_argument("--op", choices=("sort", "join"), default=None,
+                    help="time one operator alone (the ORDER BY or the 100M x 10M join) and exit")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (collectives and "
@@ -736,6 +775,9 @@ def main():
         del src, dst
         torch.cuda.empty_cache()
         log("copy calibration: 3 x clone of 1 GiB")
+    if args.op:
+        op_leg(ctx, D, args.op, args.seed, args.steps, args.warmup, args.sort_rows)
+        return
     model, threads = cpu_info()
     host = None if args.no_cpu_baseline else {}
     dim, mid, fact, code_of, nfact = build_tables(ctx, D, args.seed, args.scale, args.seg_rows, host=host)
