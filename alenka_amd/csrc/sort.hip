@@ -369,6 +369,263 @@ __global__ void __launch_bounds__(RS_BLOCK) k_rs_scatter(RsPass a) {
   }
 }
 
+// ---- bucket sort after two top-digit passes ------------------------------------------------
+// Once two stable passes on the two most significant varying digits have put the rows in
+// order of their codes' high part (code >> topshift, "buckets"), each block of k_rs_local
+// owns the buckets that start in its window of RL_W positions and sorts them, with their
+// continuation past the window, in LDS; only the permutation is stored.  A bucket that does
+// not end within RS_TILE positions of its window sets *overflow (the host then sorts by the
+// digit passes alone).
+//
+// In the block each code's high part is replaced by its bucket's ordinal, the rows of
+// [lo, hi) are packed to LDS positions 0 .. m-1, and three or four stable LDS passes rank
+// them by the next two varying digits and then the ordinal.  Rows that still tie on every
+// bit above the remaining digits stand in runs of consecutive positions (short for spread
+// keys); each row's place in its run is the number of rows of the run with a smaller code,
+// or an equal code and an earlier position.  A run longer than RL_RUN makes the block rank
+// all remaining digits by LDS passes instead, from the order it has (which keeps equal
+// codes in input order, so the result is the same).
+constexpr uint32_t RL_W = RS_TILE / 2;
+constexpr uint32_t RL_RUN = 32;
+
+struct RsLocal {
+  const uint64_t *keys;
+  const uint32_t *vals;
+  uint32_t *vout;
+  uint32_t *overflow;
+  uint64_t n;
+  uint32_t nblk, topshift, nlow;  // nlow >= 2 remaining varying digits:
+  uint32_t sh[RS_MAX_PASSES], mask[RS_MAX_PASSES];  // least significant first
+};
+
+// 6 waves per SIMD: three blocks per CU, as many as the LDS holds
+__global__ void __launch_bounds__(RS_BLOCK) __attribute__((amdgpu_waves_per_eu(6, 6))) k_rs_local(RsLocal a) {
+  __shared__ uint64_t sk[RS_TILE];
+  __shared__ uint32_t sv[RS_TILE];
+  __shared__ uint16_t wc[RS_WAVES][RS_BINS];
+  __shared__ uint32_t wsum[RS_WAVES];
+  __shared__ uint32_t s_lo, s_hi, s_long;
+  const unsigned t = threadIdx.x, lane = __lane_id(), wid = t / 64;
+  // each XCD takes a contiguous range of windows: neighbours share RL_W codes and one L2
+  const uint32_t q = a.nblk / RS_XCDS, rm = a.nblk % RS_XCDS, x = blockIdx.x % RS_XCDS;
+  const uint64_t n = a.n, w0 = (uint64_t)(x * q + min(x, rm) + blockIdx.x / RS_XCDS) * RL_W;
+  const uint64_t lt = lanemask_lt();
+  uint64_t key[RS_ITEMS];
+  uint32_t val[RS_ITEMS], rank[RS_ITEMS];
+  if (t == 0) {
+    s_lo = 0xFFFFFFFFu;
+    s_hi = 0xFFFFFFFFu;
+    s_long = 0;
+  }
+  // the codes and values of positions w0 + i, i = wave * 512 + row * 64 + lane (memory
+  // order); the values past the block's rows are mostly its neighbour's, in the same L2
+#pragma unroll
+  for (int j = 0; j < RS_ITEMS; ++j) {
+    const uint32_t i = wid * RS_WAVE_ITEMS + j * 64 + lane;
+    key[j] = w0 + i < n ? __builtin_nontemporal_load(a.keys + w0 + i) : ~0ull;
+    val[j] = w0 + i < n ? __builtin_nontemporal_load(a.vals + w0 + i) : 0u;
+    sk[i] = key[j] >> a.topshift;
+  }
+  const uint64_t tprev = w0 > 0 ? a.keys[w0 - 1] >> a.topshift : ~0ull;
+  __syncthreads();
+  // bucket starts: the first in the window (lo) and the first at or past its end (hi)
+  bool st[RS_ITEMS];
+#pragma unroll
+  for (int j = 0; j < RS_ITEMS; ++j) {
+    const uint32_t i = wid * RS_WAVE_ITEMS + j * 64 + lane;
+    st[j] = w0 + i < n && (w0 + i == 0 || (i == 0 ? tprev : sk[i - 1]) != sk[i]);
+    if (st[j]) atomicMin(i < RL_W ? &s_lo : &s_hi, i);
+  }
+  __syncthreads();
+  const uint32_t lo = s_lo;
+  if (lo == 0xFFFFFFFFu) return;  // no bucket starts here (uniform across the block)
+  uint32_t hi = s_hi;
+  if (hi == 0xFFFFFFFFu) {
+    if (w0 + RS_TILE < n) {  // the last bucket runs past the block's capacity
+      if (t == 0) atomicOr(a.overflow, 1u);
+      return;
+    }
+    hi = (uint32_t)(n - w0);
+  }
+  const uint32_t m = hi - lo;
+  // bucket ordinals (starts in (lo, i]) by a scan in memory order
+  uint32_t carry = 0;
+  uint32_t ord[RS_ITEMS];
+#pragma unroll
+  for (int j = 0; j < RS_ITEMS; ++j) {
+    const uint32_t i = wid * RS_WAVE_ITEMS + j * 64 + lane;
+    const uint32_t inc = wave_incl_scan((st[j] && i > lo && i < hi) ? 1u : 0u);
+    ord[j] = carry + inc;
+    carry += __shfl(inc, 63, 64);
+  }
+  if (lane == 0) wsum[wid] = carry;
+  __syncthreads();  // also: every read of the high parts in sk is done
+  uint32_t obase = 0, nb = 1;  // nb: buckets in [lo, hi)
+  for (unsigned w = 0; w < RS_WAVES; ++w) {
+    if (w < wid) obase += wsum[w];
+    nb += wsum[w];
+  }
+  const uint64_t lowmask = (1ull << a.topshift) - 1;
+#pragma unroll
+  for (int j = 0; j < RS_ITEMS; ++j) {
+    const uint32_t i = wid * RS_WAVE_ITEMS + j * 64 + lane;
+    if (i >= lo && i < hi) {
+      sk[i - lo] = ((uint64_t)(obase + ord[j]) << a.topshift) | (key[j] & lowmask);
+      sv[i - lo] = val[j];
+    }
+  }
+  // from here on a wave holds rows wid * S + j * 64 + lane, j < J: every wave has work
+  const uint32_t J = (m + RS_BLOCK - 1) / RS_BLOCK, S = 64 * J;
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < RS_ITEMS; ++j) {
+    const uint32_t i = wid * S + j * 64 + lane;
+    if (j < (int)J) {
+      key[j] = i < m ? sk[i] : 0;
+      val[j] = i < m ? sv[i] : 0;
+    }
+  }
+  // Stable LDS passes, each on the digit (key >> sh) & mask with ranks in position order:
+  // first the two digits above the cut and the ordinal (1 or 2 passes: nb <= RS_TILE), then,
+  // only after a long run, every remaining digit and the ordinal again.
+  const uint32_t nl = a.nlow, cut = a.sh[nl - 2];
+  const uint32_t nord = (nb > 1 ? 1u : 0u) + (nb > RS_BINS ? 1u : 0u), npre = 2 + nord;
+  uint32_t npass = npre;
+  for (uint32_t ps = 0; ps < npass; ++ps) {
+    uint32_t sh, mask;
+    if (ps < npre) {
+      sh = ps < 2 ? a.sh[nl - 2 + ps] : a.topshift + RS_BITS * (ps - 2);
+      mask = ps < 2 ? a.mask[nl - 2 + ps] : 0xFFu;
+    } else {
+      const uint32_t r = ps - npre;
+      sh = r < nl ? a.sh[r] : a.topshift + RS_BITS * (r - nl);
+      mask = r < nl ? a.mask[r] : 0xFFu;
+    }
+    uint64_t *pm = sk;  // lane masks of [wave][digit]
+    __syncthreads();    // every read of sk and sv before this pass is done
+    for (int w = t; w < RS_WAVES * RS_BINS; w += RS_BLOCK) {
+      (&wc[0][0])[w] = 0;
+      pm[w] = 0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < RS_ITEMS; ++j) {
+      if (j >= (int)J) break;
+      const uint32_t i = wid * S + j * 64 + lane;
+      const bool valid = i < m;
+      const uint32_t d = (uint32_t)(key[j] >> sh) & mask;
+      uint64_t *slot = pm + wid * RS_BINS + d;
+      if (valid) __hip_atomic_fetch_or(slot, 1ull << lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      const uint64_t peers = valid ? __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) : 0;
+      const uint32_t before = valid ? wc[wid][d] : 0;
+      const uint64_t below = peers & lt;
+      rank[j] = before + (uint32_t)__popcll(below);
+      if (valid && below == 0) {
+        wc[wid][d] = (uint16_t)(before + (uint32_t)__popcll(peers));
+        __hip_atomic_store(slot, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      }
+    }
+    __syncthreads();
+    uint32_t cnt = 0, incl = 0;
+    if (t < RS_BINS) {
+#pragma unroll
+      for (int w = 0; w < RS_WAVES; ++w) {
+        const uint32_t c = wc[w][t];
+        wc[w][t] = (uint16_t)cnt;
+        cnt += c;
+      }
+      incl = wave_incl_scan(cnt);
+      if (lane == 63) wsum[wid] = incl;
+    }
+    __syncthreads();
+    if (t < RS_BINS) {
+      uint32_t tstart = incl - cnt;
+      for (unsigned w = 0; w < wid; ++w) tstart += wsum[w];
+#pragma unroll
+      for (int w = 0; w < RS_WAVES; ++w) wc[w][t] = (uint16_t)(wc[w][t] + tstart);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < RS_ITEMS; ++j) {
+      if (j >= (int)J) break;
+      const uint32_t i = wid * S + j * 64 + lane;
+      if (i < m) {
+        const uint32_t pos = wc[wid][(uint32_t)(key[j] >> sh) & mask] + rank[j];
+        sk[pos] = key[j];
+        sv[pos] = val[j];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < RS_ITEMS; ++j) {
+      if (j >= (int)J) break;
+      const uint32_t i = wid * S + j * 64 + lane;
+      key[j] = i < m ? sk[i] : 0;
+      val[j] = i < m ? sv[i] : 0;
+    }
+    if (ps + 1 != npre || nl == 2) continue;
+    // rows that tie on key >> cut stand in runs; a run longer than RL_RUN holds two rows
+    // RL_RUN apart
+    bool lng = false;
+#pragma unroll
+    for (int j = 0; j < RS_ITEMS; ++j) {
+      if (j >= (int)J) break;
+      const uint32_t i = wid * S + j * 64 + lane;
+      lng |= i + RL_RUN < m && (sk[i + RL_RUN] >> cut) == (key[j] >> cut);
+    }
+    if (lng) s_long = 1;
+    __syncthreads();
+    if (s_long) npass += nl + nord;
+  }
+  if (nl == 2 || s_long) {  // the rows are in order
+#pragma unroll
+    for (int j = 0; j < RS_ITEMS; ++j) {
+      if (j >= (int)J) break;
+      const uint32_t i = wid * S + j * 64 + lane;
+      if (i < m) a.vout[w0 + lo + i] = val[j];
+    }
+    return;
+  }
+  // each row's place in its run: the rows of the run with a smaller code, or an equal code
+  // and an earlier position.  Runs of 2 from the neighbours' codes; longer runs (rare for
+  // spread keys) by a scan of the run.
+  uint32_t slow = 0;
+#pragma unroll
+  for (int j = 0; j < RS_ITEMS; ++j) {
+    if (j >= (int)J) break;
+    const uint32_t i = wid * S + j * 64 + lane;
+    if (i >= m) continue;
+    const uint64_t p = key[j] >> cut;
+    const uint64_t km1 = i >= 1 ? sk[i - 1] : 0, kp1 = i + 1 < m ? sk[i + 1] : 0;
+    const uint64_t km2 = i >= 2 ? sk[i - 2] : 0, kp2 = i + 2 < m ? sk[i + 2] : 0;
+    const bool em1 = i >= 1 && (km1 >> cut) == p, ep1 = i + 1 < m && (kp1 >> cut) == p;
+    const bool em2 = em1 && i >= 2 && (km2 >> cut) == p, ep2 = ep1 && i + 2 < m && (kp2 >> cut) == p;
+    uint32_t pos = i;
+    if (em1 && ep1 || em2 || ep2) {
+      slow |= 1u << j;
+      continue;
+    }
+    if (em1) pos = km1 <= key[j] ? i : i - 1;  // run [i-1, i]
+    if (ep1) pos = kp1 < key[j] ? i + 1 : i;   // run [i, i+1]
+    a.vout[w0 + lo + pos] = val[j];
+  }
+#pragma unroll 1
+  for (uint32_t j = 0; j < J; ++j) {
+    if (!((slow >> j) & 1)) continue;
+    const uint32_t i = wid * S + j * 64 + lane;
+    const uint64_t kk = sk[i], p = kk >> cut;
+    uint32_t rs = i, re = i + 1;
+    while (rs > 0 && (sk[rs - 1] >> cut) == p) --rs;
+    while (re < m && (sk[re] >> cut) == p) ++re;
+    uint32_t pos = rs;
+    for (uint32_t k = rs; k < re; ++k) {
+      const uint64_t c = sk[k];
+      pos += c < kk || (c == kk && k < i);
+    }
+    a.vout[w0 + lo + pos] = sv[i];
+  }
+}
+
 __global__ void k_rs_vals(const uint32_t *__restrict__ vals, uint64_t n, uint32_t *__restrict__ vout) {
   for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256)
     vout[i] = vals ? vals[i] : (uint32_t)i;
@@ -425,6 +682,109 @@ static int rs_count(aln_ctx *c, int ki, bool vary, uint32_t nsuper, const RsIn &
   return check_launch("radix_count");
 }
 
+// The bucket path of rs_sort (see k_rs_local): *done = false when a bucket overflowed.
+// kt and vt: two code arrays of 8n bytes and two row arrays of n words.
+static int rs_hybrid(aln_ctx *c, const RsIn &in, bool from_keys, uint64_t n, uint32_t bits, const uint32_t *run,
+                     uint32_t nrun, int counted, uint32_t *cnt, uint32_t *excl, uint32_t *tot,
+                     uint16_t *sub, uint64_t *const kt[2], uint32_t *const vt[2], uint32_t *vout,
+                     bool *done) {
+  *done = false;
+  const uint32_t nsuper = (uint32_t)((n + RS_SUPER - 1) / RS_SUPER);
+  const uint32_t ntiles = (uint32_t)((n + RS_TILE - 1) / RS_TILE);
+  auto pass_mask = [&](uint32_t p) {
+    uint32_t pb = bits - RS_BITS * p < RS_BITS ? bits - RS_BITS * p : RS_BITS;
+    return (1u << pb) - 1;
+  };
+  uint32_t *ovf;
+  ALN_TRY(ws(c, 1, &ovf));
+  ALN_TRY(fill_u32(c, ovf, 1, 0));
+  // two passes: the second most significant varying digit, then the most significant
+  for (uint32_t r = 0; r < 2; ++r) {
+    const uint32_t p = run[nrun - 2 + r];
+    RsPass a;
+    a.in = in;
+    if (r > 0) {
+      a.in.keys = kt[0];
+      a.in.vals = vt[0];
+    }
+    a.n = n;
+    a.tot = tot;
+    a.excl = excl;
+    a.sub = sub;
+    a.kout = kt[r];
+    a.vout = vt[r];
+    a.sh = RS_BITS * p;
+    a.mask = pass_mask(p);
+    a.oshift = 0;
+    a.nsuper = nsuper;
+    const int ki = r == 0 ? (from_keys ? 0 : 8) : 8;
+    if (r > 0 || (int)p != counted) {
+      ProfScope ps(c, "radix_count");
+      ALN_TRY(rs_count(c, ki, false, nsuper, a.in, n, a.sh, a.mask, cnt, sub, nullptr));
+    }
+    {
+      ProfScope ps(c, "radix_scan");
+      hipLaunchKernelGGL(k_rs_rowscan, dim3(RS_BINS), dim3(RS_SCAN_BLOCK), 0, c->stream, cnt, nsuper, excl, tot);
+    }
+    ALN_TRY(check_launch("radix_scan"));
+    {
+      ProfScope ps(c, "radix_scatter");
+      ALN_TRY(rs_scatter(c, ki, 8, ntiles, a));
+    }
+  }
+  // the permutation goes to vout unless vout is also the input's row list, which the digit
+  // passes would read again after an overflow: then to vt[0] (read by the second pass, now
+  // free) and copied over on success
+  const bool alias = in.vals && in.vals == vout;
+  RsLocal l;
+  memset(&l, 0, sizeof(l));
+  l.keys = kt[1];
+  l.vals = vt[1];
+  l.vout = alias ? vt[0] : vout;
+  l.overflow = ovf;
+  l.n = n;
+  l.nblk = (uint32_t)((n + RL_W - 1) / RL_W);
+  l.topshift = RS_BITS * run[nrun - 2];
+  l.nlow = nrun - 2;
+  for (uint32_t r = 0; r + 2 < nrun; ++r) {
+    l.sh[r] = RS_BITS * run[r];
+    l.mask[r] = pass_mask(run[r]);
+  }
+  {
+    ProfScope ps(c, "radix_local");
+    hipLaunchKernelGGL(k_rs_local, dim3(l.nblk), dim3(RS_BLOCK), 0, c->stream, l);
+  }
+  ALN_TRY(check_launch("radix_local"));
+  uint32_t o = 0;
+  ALN_TRY(copy_to_host(c, &o, ovf, 4));
+  *done = o == 0;
+  if (*done && alias) ALN_HIP(hipMemcpyAsync(vout, vt[0], 4ull * n, hipMemcpyDeviceToDevice, c->stream));
+  return ALN_OK;
+}
+
+// The bits in which the codes of the first RS_SAMPLE positions differ (a subset of those in
+// which all codes differ), computed on the host.
+constexpr uint64_t RS_SAMPLE = 4096;
+static int sample_vary(aln_ctx *c, const RsIn &in, bool from_keys, uint64_t n, uint64_t *vary) {
+  const uint64_t m = std::min<uint64_t>(n, RS_SAMPLE);
+  std::vector<uint64_t> code(m, 0);
+  if (!from_keys) {
+    ALN_TRY(copy_to_host(c, code.data(), in.keys, 8 * m));
+  } else {  // the first rows' keys: a sample of the rows whatever order in.vals gives them
+    std::vector<int64_t> v(m);
+    for (int k = 0; k < in.ks.n; ++k) {
+      ALN_TRY(copy_to_host(c, v.data(), in.ks.k[k], 8 * m));
+      for (uint64_t r = 0; r < m; ++r) {
+        const uint64_t f = in.ks.desc[k] ? (uint64_t)in.ks.ref[k] - (uint64_t)v[r] : (uint64_t)v[r] - (uint64_t)in.ks.ref[k];
+        if (in.ks.shift[k] < 64) code[r] |= f << in.ks.shift[k];
+      }
+    }
+  }
+  *vary = 0;
+  for (uint64_t r = 0; r < m; ++r) *vary |= code[r] ^ code[0];
+  return ALN_OK;
+}
+
 // Stable LSD sort of the items of `in` by the low `bits` bits of their codes.  from_keys:
 // the codes come from in.ks (first pass), else from the 8-byte array in.keys.  The values
 // land in vout (may be in.vals).  kout given: the codes land there, whole (may be in.keys);
@@ -454,12 +814,29 @@ static int rs_sort(aln_ctx *c, RsIn in, bool from_keys, uint64_t n, uint32_t bit
     ALN_TRY(ws(c, (size_t)nsuper * RS_SUB * RS_BINS, &sub));
     ALN_TRY(ws(c, 1, &d_vary));
   }
-  // pass 0's counts, and the bits in which the codes differ
+  // the bits in which the codes differ, and the counts of one digit of the input: the first
+  // the bucket path sorts by when it may run (the second most significant), else digit 0
   const bool sync = n >= RS_SYNC_MIN;
+  const bool bucket = !keys_out && sync && passes >= 4;
+  int counted = -1;  // the digit whose counts of the input are in cnt and sub
   if (passes > 0) {
+    counted = 0;
+    if (bucket) {  // the second most significant digit in which a sample of the codes differ
+      uint64_t sv = 0;
+      ALN_TRY(sample_vary(c, in, from_keys, n, &sv));
+      int top = -1;
+      for (int p = (int)passes - 1; p >= 0; --p)
+        if ((sv >> (RS_BITS * p)) & pass_mask(p)) {
+          if (top >= 0) {
+            counted = p;
+            break;
+          }
+          top = p;
+        }
+    }
     if (sync) ALN_TRY(fill_u64(c, reinterpret_cast<uint64_t *>(d_vary), 1, 0));
     ProfScope ps(c, "radix_count");
-    ALN_TRY(rs_count(c, ki0, sync, nsuper, in, n, 0, pass_mask(0), cnt, sub, d_vary));
+    ALN_TRY(rs_count(c, ki0, sync, nsuper, in, n, RS_BITS * counted, pass_mask(counted), cnt, sub, d_vary));
   }
   uint64_t vary = ~0ull;
   if (sync && passes > 0) ALN_TRY(copy_to_host(c, &vary, d_vary, 8));
@@ -475,6 +852,26 @@ static int rs_sort(aln_ctx *c, RsIn in, bool from_keys, uint64_t n, uint32_t bit
     if (keys_out && kout != in.keys)
       ALN_HIP(hipMemcpyAsync(kout, in.keys, 8ull * n, hipMemcpyDeviceToDevice, c->stream));
     return ALN_OK;
+  }
+  const bool alias1 = nrun == 1 && ((in.vals && in.vals == vout) || (keys_out && in.keys == kout));
+  void *kt[2] = {nullptr, nullptr};
+  uint32_t *vt[2] = {nullptr, nullptr};
+  if (nrun > 1 || alias1) {
+    ALN_TRY(ws(c, 8 * n, reinterpret_cast<uint8_t **>(&kt[0])));
+    ALN_TRY(ws(c, 8 * n, reinterpret_cast<uint8_t **>(&kt[1])));
+    ALN_TRY(ws(c, n, &vt[0]));
+    ALN_TRY(ws(c, n, &vt[1]));
+  }
+  // A permutation sort of many rows with at least 4 varying digits: two passes on the two
+  // most significant ones, then every bucket sorted on the rest in LDS (k_rs_local); if a
+  // bucket is too large for that, the digit passes below run instead, from the start, and
+  // count pass 0 again (the bucket path's passes used the count buffers).
+  if (bucket && nrun >= 4) {
+    bool done = false;
+    uint64_t *const kt64[2] = {static_cast<uint64_t *>(kt[0]), static_cast<uint64_t *>(kt[1])};
+    ALN_TRY(rs_hybrid(c, in, from_keys, n, bits, run, nrun, counted, cnt, excl, tot, sub, kt64, vt, vout, &done));
+    if (done) return ALN_OK;
+    counted = -1;
   }
   // Bytes each pass reads (ki) and writes (ko) per key: the bytes below the next pass's
   // digit are dropped and the rest is stored in the smallest word that holds it.
@@ -493,15 +890,6 @@ static int rs_sort(aln_ctx *c, RsIn in, bool from_keys, uint64_t n, uint32_t bit
       qout[r] = qin[r];
       ko[r] = 0;
     }
-  }
-  const bool alias1 = nrun == 1 && ((in.vals && in.vals == vout) || (keys_out && in.keys == kout));
-  void *kt[2] = {nullptr, nullptr};
-  uint32_t *vt[2] = {nullptr, nullptr};
-  if (nrun > 1 || alias1) {
-    ALN_TRY(ws(c, 8 * n, reinterpret_cast<uint8_t **>(&kt[0])));
-    ALN_TRY(ws(c, 8 * n, reinterpret_cast<uint8_t **>(&kt[1])));
-    ALN_TRY(ws(c, n, &vt[0]));
-    ALN_TRY(ws(c, n, &vt[1]));
   }
   const void *kprev = nullptr;
   const uint32_t *vprev = nullptr;
@@ -524,7 +912,7 @@ static int rs_sort(aln_ctx *c, RsIn in, bool from_keys, uint64_t n, uint32_t bit
     a.mask = pass_mask(p);
     a.oshift = RS_BITS * (qout[r] - qin[r]);
     a.nsuper = nsuper;
-    if (r > 0 || p != 0) {
+    if (r > 0 || (int)p != counted) {
       ProfScope ps(c, "radix_count");
       ALN_TRY(rs_count(c, ki[r], false, nsuper, a.in, n, a.sh, a.mask, cnt, sub, nullptr));
     }

@@ -453,6 +453,43 @@ def test_sort_keys_lsd_shapes(ctx, n, shape):
     assert np.array_equal(perm, O.order(keys, desc))
 
 
+@pytest.mark.parametrize("case", ["uniform_desc", "u32_keys", "u28_keys", "tiny_buckets", "overflow", "dups", "two_keys"])
+def test_sort_keys_bucket_path(ctx, case):
+    """ORDER BY of 2^20 rows and more with at least four varying digits: two passes on the
+    top two digits, then each bucket sorted in LDS; buckets too large for a block fall back
+    to the digit passes.  Cases: uniform 64-bit keys DESC; 32- and 28-bit keys (many small
+    buckets per block); >256 buckets in some blocks (two ordinal passes); buckets of 4096
+    rows (fallback); 300 distinct values (fallback); two keys packed into one code."""
+    from alenka_amd import device as D
+
+    r = np.random.default_rng(len(case) * 7)
+    desc = [0]
+    if case == "uniform_desc":
+        n = 3_000_001
+        keys, desc = [r.integers(-(1 << 63), (1 << 63) - 1, n, dtype=np.int64)], [1]
+    elif case == "u32_keys":
+        n = 2_000_003
+        keys = [r.integers(0, 1 << 32, n)]
+    elif case == "u28_keys":
+        n = 1_500_000
+        keys = [r.integers(0, 1 << 28, n) - (1 << 27)]
+    elif case == "tiny_buckets":
+        n = 1_100_000
+        top = np.concatenate([r.integers(0, 60000, 100_000), 60000 + r.integers(0, 5536, n - 100_000)])
+        keys = [(top << 24) | r.integers(0, 1 << 24, n)]
+    elif case == "overflow":
+        n = 4_000_000
+        keys = [(r.integers(0, 4, n) << 40) | r.integers(0, 1 << 32, n)]
+    elif case == "dups":
+        n = 2_000_000
+        keys = [r.integers(-(1 << 62), 1 << 62, 300)[r.integers(0, 300, n)]]
+    else:
+        n = 1_200_000
+        keys, desc = [r.integers(0, 1 << 20, n), r.integers(-(1 << 30), 1 << 30, n)], [1, 0]
+    perm = host(D.sort_keys(ctx, [dev(ctx, k) for k in keys], desc)).astype(np.uint32)
+    assert np.array_equal(perm, O.order(keys, desc))
+
+
 def test_hash_rows(ctx):
     from alenka_amd import device as D
 
