@@ -432,9 +432,21 @@ __global__ void __launch_bounds__(256) k_minmax_part(const int64_t *__restrict__
                                                      int64_t *__restrict__ pmax) {
   __shared__ int64_t smn[4], smx[4];
   int64_t lo = INT64_MAX, hi = INT64_MIN;
-  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n;
-       i += (uint64_t)gridDim.x * 256) {
-    int64_t v = d[i];
+  const uint64_t stride = (uint64_t)gridDim.x * 256;
+  uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  // eight independent coalesced loads in flight per thread
+  for (; i + 7 * stride < n; i += 8 * stride) {
+    int64_t v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = __builtin_nontemporal_load(d + i + k * stride);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      lo = v[k] < lo ? v[k] : lo;
+      hi = v[k] > hi ? v[k] : hi;
+    }
+  }
+  for (; i < n; i += stride) {
+    const int64_t v = d[i];
     lo = v < lo ? v : lo;
     hi = v > hi ? v : hi;
   }
@@ -461,7 +473,7 @@ __global__ void __launch_bounds__(256) k_minmax_part(const int64_t *__restrict__
 }
 
 int minmax_i64(aln_ctx *c, const int64_t *d, uint64_t n, int64_t *h_min, int64_t *h_max) {
-  const unsigned nb = grid_for(n, 256 * 16, 1024);
+  const unsigned nb = grid_for(n, 256 * 16, 2048);  // 8 blocks per CU; 32 KB of partials
   int64_t *part;
   ALN_TRY(ws(c, 2 * (size_t)nb, &part));
   if (n) {

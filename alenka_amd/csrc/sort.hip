@@ -1009,9 +1009,24 @@ int sort_keys_impl(aln_ctx *c, const int64_t *const *keys, const int32_t *desc, 
   ALN_REQUIRE(n < (1ull << 32), "sort: more than 2^32 rows");
   if (n == 0) return ALN_OK;
   KeyRanges r;
+  bool whole = false;
   if (nkeys == 1 && n >= RS_SYNC_MIN) {
-    // one key: codes over the whole int64 range, no min/max pass; the histogram finds the
-    // digits every key shares and those passes are skipped
+    // One key: codes over the whole int64 range, no min/max pass; the count pass finds the
+    // digits every key shares and those passes are skipped.  Unless values of a narrow range
+    // differ in many high bits (they cross zero or another power of two): checked on the
+    // first rows, where such keys get the min/max pass and codes of just their range.
+    std::vector<int64_t> v(std::min<uint64_t>(n, RS_SAMPLE));
+    ALN_TRY(copy_to_host(c, v.data(), keys[0], 8 * v.size()));
+    uint64_t vary = 0;
+    int64_t lo = v[0], hi = v[0];
+    for (int64_t x : v) {
+      vary |= (uint64_t)x ^ (uint64_t)v[0];
+      lo = std::min(lo, x);
+      hi = std::max(hi, x);
+    }
+    whole = bits_for_range_u(vary) <= bits_for_range_u((uint64_t)hi - (uint64_t)lo) + RS_BITS;
+  }
+  if (whole) {
     r.mn[0] = INT64_MIN;
     r.mx[0] = INT64_MAX;
     r.bits[0] = r.total = 64;

@@ -409,23 +409,16 @@ def batch_leg(ctx, D, fact, nfact, seed, nq, steps, sort_rows):
     plain_ms = np.array(plain_lat) * 1e3
     batched_qps = nq / (lat_ms.mean() / 1e3)
     plain_qps = nq / (plain_ms.sum() / 1e3)
-    # ORDER BY of a sort_rows-row result: int64 uniform key, int32 payload (row number)
-    keys, _ = D.gen_sort_input(ctx, seed, sort_rows, payload=False)
-    D.sort_keys(ctx, [keys])
-    torch.cuda.synchronize()
-    st = []
-    for _ in range(3):
-        t1 = time.perf_counter()
-        perm = D.sort_keys(ctx, [keys])
-        torch.cuda.synchronize()
-        st.append(time.perf_counter() - t1)
-    # sorted and a permutation (compared, not subtracted: int64 differences can wrap)
-    sk = keys[perm.long()]
-    ok = bool((sk[1:] >= sk[:-1]).all().item()) and bool((torch.bincount(perm.long(), minlength=sort_rows) == 1).all().item())
-    del sk
-    del keys, perm
-    torch.cuda.empty_cache()
-    sort_s = float(np.median(st))
+    # ORDER BY of a sort_rows-row result: uniform int64 keys, then signed 32-bit values
+    order_by = {}
+    for kind in ("int64", "int32"):
+        keys = sort_input(ctx, D, kind, seed, sort_rows)
+        st, ok = time_sort(ctx, D, keys, 1, 3)
+        del keys
+        torch.cuda.empty_cache()
+        sort_s = float(np.median(st)) / 1e3
+        order_by[kind] = {"rows": sort_rows, "ms": sort_s * 1e3, "rows_per_s": sort_rows / sort_s,
+                          "algorithmic_gbs": 24 * sort_rows / sort_s / 1e9, "sorted_stable": ok}
     return zc, {
         "queries": nq,
         "selectivity_range": [min(q["sel"] for q in qs), max(q["sel"] for q in qs)],
@@ -440,8 +433,7 @@ def batch_leg(ctx, D, fact, nfact, seed, nq, steps, sort_rows):
                                "p99": float(np.percentile(plain_ms, 99))},
         "speedup_qps": batched_qps / plain_qps,
         "results_match_plain": mismatches == 0,
-        "order_by": {"rows": sort_rows, "ms": sort_s * 1e3, "rows_per_s": sort_rows / sort_s,
-                     "algorithmic_gbs": 24 * sort_rows / sort_s / 1e9, "sorted": ok},
+        "order_by": order_by,
     }
 
 
@@ -690,21 +682,61 @@ def widened_rows(ctx, D, seed, n=100_000_000, reps=5):
     return out
 
 
-def op_leg(ctx, D, op, seed, steps, warmup, sort_rows):
-    """One operator alone, for profiling it: `sort` = ORDER BY of sort_rows (int64 key, row
-    number) rows; `join` = 100M probe rows x 10M unique build keys over a 20M range (the
-    semi join's inner join).  Prints one JSON line of per-call times and returns."""
+def sort_input(ctx, D, kind, seed, rows):
+    """ORDER BY keys: "int64" uniform over the whole range; "int32" uniform signed 32-bit
+    values (in the int64 column type), which cross zero."""
+    import torch
+
+    if kind == "int64":
+        return D.gen_sort_input(ctx, seed, rows, payload=False)[0]
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    return torch.randint(-(1 << 31), 1 << 31, (rows,), generator=g, device="cuda", dtype=torch.int64)
+
+
+def time_sort(ctx, D, keys, warmup, steps):
+    """Per-call ms of sort_keys over `keys`, and whether the last permutation is a stable
+    sort: keys in order, equal keys in row order, every row once (compared, not subtracted:
+    int64 differences can wrap)."""
+    import torch
+
+    for _ in range(warmup):
+        D.sort_keys(ctx, [keys])
+    ts = []
+    for _ in range(steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        perm = D.sort_keys(ctx, [keys])
+        torch.cuda.synchronize()
+        ts.append((time.perf_counter() - t0) * 1e3)
+    p = perm.long()
+    sk = keys[p]
+    ok = bool((sk[1:] >= sk[:-1]).all().item())
+    ok = ok and bool(((sk[1:] != sk[:-1]) | (p[1:] > p[:-1])).all().item())
+    ok = ok and bool((torch.bincount(p, minlength=keys.numel()) == 1).all().item())
+    return ts, ok
+
+
+def op_leg(ctx, D, op, seed, steps, warmup, sort_rows, sort_kinds=("int64", "int32")):
+    """One operator alone, for profiling it: `sort` = ORDER BY of sort_rows rows (int64
+    keys, then signed 32-bit keys; the permutation is the result); `join` = 100M probe rows
+    x 10M unique build keys over a 20M range (the semi join's inner join).  Prints one JSON
+    line of per-call times and returns."""
     import numpy as np
     import torch
 
     g = torch.Generator(device="cuda").manual_seed(seed)
     if op == "sort":
-        keys, _ = D.gen_sort_input(ctx, seed, sort_rows, payload=False)
-        fn = lambda: D.sort_keys(ctx, [keys])  # noqa: E731
-    else:
-        probe = torch.randint(0, 20_000_000, (100_000_000,), generator=g, device="cuda", dtype=torch.int64)
-        build = torch.randperm(20_000_000, generator=g, device="cuda")[:10_000_000].to(torch.int64)
-        fn = lambda: D.hash_join(ctx, build, probe)  # noqa: E731
+        rec = {"op": "sort", "rows": sort_rows, "steps": steps, "warmup": warmup}
+        for kind in sort_kinds:
+            keys = sort_input(ctx, D, kind, seed, sort_rows)
+            ts, ok = time_sort(ctx, D, keys, warmup, steps)
+            rec[kind] = {"ms": ts, "ms_median": float(np.median(ts)), "checked_sorted_stable": ok}
+            del keys
+        print(json.dumps(rec), flush=True)
+        return
+    probe = torch.randint(0, 20_000_000, (100_000_000,), generator=g, device="cuda", dtype=torch.int64)
+    build = torch.randperm(20_000_000, generator=g, device="cuda")[:10_000_000].to(torch.int64)
+    fn = lambda: D.hash_join(ctx, build, probe)  # noqa: E731
     for _ in range(warmup):
         out = fn()
     ts = []
@@ -714,15 +746,10 @@ def op_leg(ctx, D, op, seed, steps, warmup, sort_rows):
         out = fn()
         torch.cuda.synchronize()
         ts.append((time.perf_counter() - t0) * 1e3)
-    if op == "sort":
-        sk = keys[out.long()]
-        ok = bool((sk[1:] >= sk[:-1]).all().item())
-        rec = {"op": "sort", "rows": sort_rows, "checked_sorted": ok}
-    else:
-        pi, bi = out
-        ok = bool((build[bi.long()] == probe[pi.long()]).all().item())
-        rec = {"op": "join", "probe_rows": 100_000_000, "build_rows": 10_000_000, "pairs": int(pi.numel()),
-               "checked_keys_equal": ok}
+    pi, bi = out
+    ok = bool((build[bi.long()] == probe[pi.long()]).all().item())
+    rec = {"op": "join", "probe_rows": 100_000_000, "build_rows": 10_000_000, "pairs": int(pi.numel()),
+           "checked_keys_equal": ok}
     rec.update({"steps": steps, "warmup": warmup, "ms": ts, "ms_median": float(np.median(ts))})
     print(json.dumps(rec), flush=True)
 
@@ -745,6 +772,7 @@ def main():
     ap.add_argument("--batch-queries", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--sort-rows", type=int, default=100_000_000)
+    ap.add_argument("--sort-kinds", default="int64,int32", help="key kinds of --op sort: int64, int32")
     ap.add_argument("--no-serve", action="store_true", help="skip the served-mix leg")
     ap.add_argument("--no-widened", action="store_true", help="skip the operators around the hot path")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -776,7 +804,7 @@ def main():
         torch.cuda.empty_cache()
         log("copy calibration: 3 x clone of 1 GiB")
     if args.op:
-        op_leg(ctx, D, args.op, args.seed, args.steps, args.warmup, args.sort_rows)
+        op_leg(ctx, D, args.op, args.seed, args.steps, args.warmup, args.sort_rows, args.sort_kinds.split(","))
         return
     model, threads = cpu_info()
     host = None if args.no_cpu_baseline else {}
