@@ -172,14 +172,14 @@ __global__ void k_join_runs(const uint64_t *code, const uint32_t *head, const ui
 // build side becomes a table over the key range with one 32-byte group per 224 keys: seven
 // words with the bits of the keys present and the number of keys before the group (its
 // rank base), plus the build rows in key order (rowid[rank]).  At 10M build keys over a 20M
-// range the table is 2.9 MB, which stays in an XCD's 4 MB L2 (a probe reads one aligned
-// 32-byte group, inside one cache line); a hit then reads one row id.
+// range the table is 2.9 MB, which can stay in an XCD's 4 MB L2 (a probe reads one aligned
+// 32-byte group, inside one cache line) as long as no other random reads share the pass.
 //   build: a byte per key (plain byte stores, no atomics: every XCD's L2 sees them merged at
 //          write-back), packed into the groups with a block scan of the counts, then the row
 //          ids scattered to their ranks;
-//   probe: pass 0 streams the probe keys and writes each row's build row (or none) and the
-//          tile's pair count; pass 1 streams those and writes the pairs of a tile through LDS,
-//          so the stores to the output are whole, contiguous lines.
+//   probe: pass 0 streams the probe keys and writes each row's rank (or none) and the tile's
+//          pair count; pass 1 streams those, gathers the hits' row ids and writes the pairs
+//          of a tile through LDS, so the stores to the output are whole, contiguous lines.
 constexpr uint64_t JD_DENSITY = 64;  // key range <= 64 x build rows (and <= 2^31)
 constexpr int JD_BLOCK = 256;        // groups per block of the packing pass
 constexpr uint32_t JD_GWORDS = 7;    // bit words per group (the eighth word is the rank base)
@@ -269,13 +269,13 @@ __global__ void __launch_bounds__(256) k_jd_rows(const int64_t *__restrict__ key
   }
 }
 
-// Pass 0: each probe row's build row into match (coalesced), and the tile's pair count.  A
-// thread's loads go out stage by stage for all of its rows at once - the keys, then their
-// groups, then the row ids of the hits - so a thread waits three round trips, not one per
-// row and level (rows outside the key range read group 0 and are masked afterwards).
+// Pass 0: each probe row's build-key rank into match (coalesced), and the tile's pair
+// count.  A thread's loads go out stage by stage for all of its rows at once - the keys,
+// then their groups - so a thread waits two round trips, not one per row and level (rows
+// outside the key range read group 0 and are masked afterwards).  The only random reads
+// are the groups, so the table keeps its place in L2 (the row ids are gathered by pass 1).
 __global__ void __launch_bounds__(J_BLOCK) k_jd_match(const int64_t *__restrict__ pk, uint64_t np,
-                                                      const uint4 *__restrict__ grp,
-                                                      const uint32_t *__restrict__ rowid, int64_t kmin,
+                                                      const uint4 *__restrict__ grp, int64_t kmin,
                                                       uint64_t range, uint32_t *__restrict__ match,
                                                       uint32_t *__restrict__ tcount) {
   __shared__ uint32_t wsum[J_BLOCK / 64];
@@ -302,8 +302,7 @@ __global__ void __launch_bounds__(J_BLOCK) k_jd_match(const int64_t *__restrict_
     bool h;
     const uint32_t r = jd_rank(lo[j], hi[j], code[j] != 0xFFFFFFFFu ? code[j] : 0u, &h);
     h = h && code[j] != 0xFFFFFFFFu;
-    m[j] = rowid[h ? r : 0u];
-    m[j] = h ? m[j] : J_EMPTY;
+    m[j] = h ? r : J_EMPTY;
     n += h;
   }
 #pragma unroll
@@ -322,19 +321,32 @@ __global__ void __launch_bounds__(J_BLOCK) k_jd_match(const int64_t *__restrict_
 }
 
 // Pass 1: a tile's pairs in probe-row order, compacted in LDS, then written as contiguous
-// runs of the two output arrays.
+// runs of the two output arrays.  rowid given: match holds ranks, and the build row of a
+// hit is rowid[rank] (all of a thread's gathers in flight at once).
 __global__ void __launch_bounds__(J_BLOCK) k_jd_write(const uint32_t *__restrict__ match, uint64_t np,
+                                                      const uint32_t *__restrict__ rowid,
                                                       const uint64_t *__restrict__ toff, uint32_t *__restrict__ out_p,
                                                       uint32_t *__restrict__ out_b) {
   __shared__ uint32_t sp[J_TILE], sb[J_TILE];
   __shared__ uint32_t wsum[J_BLOCK / 64];
   const uint64_t t0 = (uint64_t)blockIdx.x * J_TILE;
   const unsigned lane = __lane_id(), wid = threadIdx.x / 64;
+  uint32_t mv[J_ITEMS];
+#pragma unroll
+  for (int j = 0; j < J_ITEMS; ++j) {
+    const uint64_t p = t0 + (uint64_t)j * J_BLOCK + threadIdx.x;
+    mv[j] = p < np ? __builtin_nontemporal_load(match + p) : J_EMPTY;
+  }
+  if (rowid) {
+#pragma unroll
+    for (int j = 0; j < J_ITEMS; ++j)
+      if (mv[j] != J_EMPTY) mv[j] = rowid[mv[j]];
+  }
   uint32_t base = 0;  // pairs of the tile before this round
 #pragma unroll
   for (int j = 0; j < J_ITEMS; ++j) {  // round j: rows t0 + j * J_BLOCK .. + J_BLOCK - 1, in order
     const uint64_t p = t0 + (uint64_t)j * J_BLOCK + threadIdx.x;
-    const uint32_t m = p < np ? match[p] : J_EMPTY;
+    const uint32_t m = mv[j];
     const bool h = m != J_EMPTY;
     const uint64_t bal = __builtin_amdgcn_ballot_w64(h);
     if (lane == 0) wsum[wid] = (uint32_t)__popcll(bal);
@@ -362,9 +374,10 @@ __global__ void __launch_bounds__(J_BLOCK) k_jd_write(const uint32_t *__restrict
 }
 
 // The pairs from each probe row's build row (match) and the tiles' pair counts.
-static int join_write_pairs(aln_ctx *c, const uint32_t *match, uint64_t np, const uint32_t *tcount, uint64_t *toff,
-                            uint64_t ntiles, uint32_t *d_probe_idx, uint32_t *d_build_idx, uint64_t cap,
-                            uint64_t *h_npairs) {
+// rowid: as for k_jd_write.
+static int join_write_pairs(aln_ctx *c, const uint32_t *match, const uint32_t *rowid, uint64_t np,
+                            const uint32_t *tcount, uint64_t *toff, uint64_t ntiles, uint32_t *d_probe_idx,
+                            uint32_t *d_build_idx, uint64_t cap, uint64_t *h_npairs) {
   uint64_t total = 0;
   ALN_TRY(exclusive_scan_u32(c, tcount, ntiles, toff, &total));
   *h_npairs = total;
@@ -375,7 +388,7 @@ static int join_write_pairs(aln_ctx *c, const uint32_t *match, uint64_t np, cons
   ALN_REQUIRE(d_probe_idx && d_build_idx, "aln_hash_join: no output buffers");
   {
     ProfScope p(c, "join_write");
-    hipLaunchKernelGGL(k_jd_write, dim3((unsigned)ntiles), dim3(J_BLOCK), 0, c->stream, match, np,
+    hipLaunchKernelGGL(k_jd_write, dim3((unsigned)ntiles), dim3(J_BLOCK), 0, c->stream, match, np, rowid,
                        (const uint64_t *)toff, d_probe_idx, d_build_idx);
   }
   ALN_TRY(check_launch("join_write"));
@@ -396,7 +409,7 @@ static int join_dense(aln_ctx *c, const int64_t *bk, uint64_t nb, const int64_t 
   uint32_t *bcount, *rowid;
   uint64_t *boff;
   uint4 *grp;
-  ALN_TRY(ws(c, ngrp * JD_GKEYS, &bytes));  // whole groups: the packing pass reads 224 bytes each
+  ALN_TRY(ws(c, ngrp * JD_GKEYS, &bytes));  // whole groups: the packing pass reads JD_GKEYS bytes each
   ALN_TRY(ws(c, nblk, &bcount));
   ALN_TRY(ws(c, nblk, &boff));
   ALN_TRY(ws(c, 2 * ngrp, &grp));
@@ -427,10 +440,10 @@ static int join_dense(aln_ctx *c, const int64_t *bk, uint64_t nb, const int64_t 
   {
     ProfScope p(c, "join_probe");
     hipLaunchKernelGGL(k_jd_match, dim3((unsigned)ntiles), dim3(J_BLOCK), 0, c->stream, pk, np, (const uint4 *)grp,
-                       (const uint32_t *)rowid, kmin, range, match, tcount);
+                       kmin, range, match, tcount);
   }
   ALN_TRY(check_launch("join_dense_match"));
-  return join_write_pairs(c, match, np, tcount, toff, ntiles, d_probe_idx, d_build_idx, cap, h_npairs);
+  return join_write_pairs(c, match, rowid, np, tcount, toff, ntiles, d_probe_idx, d_build_idx, cap, h_npairs);
 }
 
 }  // namespace aln
@@ -523,7 +536,9 @@ extern "C" int aln_hash_join(aln_ctx *c, const int64_t *d_build_keys, uint64_t n
   }
   ALN_TRY(check_launch("join_probe_count"));
   // unique keys: match holds the build row itself, written out through LDS
-  if (!rlen) return join_write_pairs(c, match, nprobe, tcount, toff, ntiles, d_probe_idx, d_build_idx, cap, h_npairs);
+  if (!rlen)
+    return join_write_pairs(c, match, nullptr, nprobe, tcount, toff, ntiles, d_probe_idx, d_build_idx, cap,
+                            h_npairs);
   uint64_t total = 0;
   ALN_TRY(exclusive_scan_u32(c, tcount, ntiles, toff, &total));
   *h_npairs = total;
