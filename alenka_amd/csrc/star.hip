@@ -383,6 +383,18 @@ __device__ __forceinline__ uint4 gload4(const uint4 *p) {
   return make_uint4(q.x, q.y, q.z, q.w);
 }
 
+// The same load, nontemporal: for staging, whose words are read once (streamed past the
+// caches, the mid scan runs ~3 % faster than with ordinary loads).
+__device__ __forceinline__ uint4 gload4_nt(const uint4 *p) {
+  typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+#if __HIP_DEVICE_COMPILE__
+  const v4u q = __builtin_nontemporal_load((const __attribute__((address_space(1))) v4u *)(const v4u *)p);
+#else
+  const v4u q = *reinterpret_cast<const v4u *>(p);  // host pass: never executed
+#endif
+  return make_uint4(q.x, q.y, q.z, q.w);
+}
+
 // Reads single rows of a segment's column from global memory without branches: the three
 // word loads always issue (a word a row does not need is read from the row's first word,
 // and a constant column reads `any`, some readable global address), so the loads of several
@@ -460,7 +472,7 @@ struct StageRegs {
       for (int t = 0; t < PL_CVEC; ++t) {
         const uint32_t v = t * ST_BLOCK + threadIdx.x;
         r[k][t] = make_uint4(0, 0, 0, 0);
-        if (v < nb[k]) r[k][t] = gload4(src + v);
+        if (v < nb[k]) r[k][t] = gload4_nt(src + v);
       }
     }
   }
@@ -961,47 +973,135 @@ __device__ __forceinline__ uint64_t bm_rank(const uint32_t *__restrict__ bits,
   return r;
 }
 
-// One wave per mid tile: the tile's 8 wave regions (see k_star_mid) are taken as one list,
-// each lane holding up to SC_Q entries with all their reads in flight together.  Entry
-// {code, row} makes row the mid row of group rank(code); the group's accumulators are
-// zeroed here too (every group index below the passing-key count is written exactly once).
-constexpr int SC_Q = 8;
-__global__ void __launch_bounds__(64) k_bm_scatter(const uint2 *__restrict__ rent, const uint32_t *__restrict__ counts,
-                                                  const uint32_t *__restrict__ bits,
-                                                  const uint64_t *__restrict__ rank,
-                                                  const uint64_t *__restrict__ rel, uint32_t *__restrict__ grow,
-                                                  unsigned long long *__restrict__ sum,
-                                                  unsigned int *__restrict__ cnt) {
-  constexpr int R = ST_BLOCK / 64;  // regions per tile
-  const uint64_t tile = blockIdx.x;
+// Code (bit position) of set bit number g of the mid bitmap, by one wave: a 64-way search of
+// the rank directory (rank[b] = set bits before 512-bit block b, non-decreasing) for the
+// last block with rank[b] <= g, then the block's 16 words.  Every lane returns the code.
+__device__ uint32_t bm_select_wave(const uint32_t *__restrict__ bits, const uint64_t *__restrict__ rank,
+                                   uint64_t nblk, uint32_t g) {
+  const unsigned lane = __lane_id();
+  uint64_t lo = 0, hi = nblk;  // rank[lo] <= g throughout (rank[0] = 0)
+  while (hi - lo > 1) {
+    const uint64_t step = (hi - lo + 63) / 64, i = lo + lane * step;
+    const uint64_t b = __builtin_amdgcn_ballot_w64(i < hi && rank[i] <= g);  // a prefix of the lanes
+    lo += (uint64_t)(63 - __builtin_clzll(b)) * step;
+    hi = lo + step < hi ? lo + step : hi;
+  }
+  const uint32_t r = g - (uint32_t)rank[lo];  // the bit's number inside block lo
+  const uint32_t w = lane < (unsigned)BM_BLOCK_WORDS ? bits[lo * BM_BLOCK_WORDS + lane] : 0u, pc = __popc(w);
+  uint32_t x = pc;  // inclusive scan of the words' counts
+  for (int o = 1; o < BM_BLOCK_WORDS; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= (unsigned)o) x += y;
+  }
+  const uint64_t hb = __builtin_amdgcn_ballot_w64(lane < (unsigned)BM_BLOCK_WORDS && x > r);
+  const int wl = __builtin_ctzll(hb);  // the word holding it
+  uint32_t v = __shfl(w, wl, 64);
+  for (uint32_t k = r - __shfl(x - pc, wl, 64); k; --k) v &= v - 1;  // drop the set bits below it
+  return (uint32_t)(lo * 512 + (uint64_t)wl * 32 + (uint32_t)__builtin_ctz(v));
+}
+
+// The passing mid rows as k_star_mid leaves them, with the mid bitmap's rank directory.
+struct MidRows {
+  const uint2 *rent;       // PL_WAVE_ROWS entries {code, row} per mid wave region
+  const uint32_t *counts;  // entries of each region
+  uint64_t ntiles;         // mid tiles (ST_BLOCK / 64 regions each)
+  const uint32_t *bits;    // mid bitmap, its rank9 directory and its block count
+  const uint64_t *rank, *rel;
+  uint64_t nblk;
+  uint32_t *grow;          // out: the mid row of group g
+};
+
+// Candidate mode: the top-k's candidate groups cj[0 .. n-1], n = min(*count, cap), and their
+// codes ccode[i] (k_topk_codes).  Up to TR_MAX of them go into an LDS hash in each block;
+// more, or full != 0: every group's row instead.
+constexpr uint32_t TR_BITS = 12, TR_HASH = 1u << TR_BITS, TR_MAX = TR_HASH / 2;
+
+// ccode[i] = code of candidate group cj[i], one wave per candidate (nothing when n > TR_MAX).
+__global__ void __launch_bounds__(256) k_topk_codes(MidRows M, const uint32_t *__restrict__ count, uint32_t cap,
+                                                   const uint32_t *__restrict__ cj, uint32_t *__restrict__ ccode) {
+  const uint32_t n = *count < cap ? *count : cap, i = blockIdx.x * 4 + threadIdx.x / 64;
+  if (n > TR_MAX || i >= n) return;  // (wave-uniform)
+  const uint32_t code = bm_select_wave(M.bits, M.rank, M.nblk, cj[i]);
+  if (__lane_id() == 0) ccode[i] = code;
+}
+
+// grow[g] for the groups that need it, from the passing mid rows: a block takes ST_BLOCK / 64
+// mid tiles, one per wave, whose regions it reads as one list, TR_Q entries per lane with all
+// their reads in flight (the first batch issued before the block builds its hash).
+// Candidate mode: a row whose code is in the hash is its candidate's row.  Full mode: every
+// row's group is the rank of its code.
+constexpr int TR_Q = 8;
+__global__ void __launch_bounds__(ST_BLOCK) k_topk_rows(MidRows M, const uint32_t *__restrict__ count, uint32_t cap,
+                                                       const uint32_t *__restrict__ cj,
+                                                       const uint32_t *__restrict__ ccode, int full) {
+  __shared__ uint32_t hk[TR_HASH], hg[TR_HASH];
+  const uint32_t n = full ? 0u : (*count < cap ? *count : cap);
+  const bool all = full || n > TR_MAX;
+  constexpr int R = ST_BLOCK / 64;
+  const unsigned lane = __lane_id(), wid = threadIdx.x / 64;
+  const uint64_t tile = (uint64_t)blockIdx.x * R + wid;
+  const bool mine = tile < M.ntiles;
   uint32_t off[R + 1];
   off[0] = 0;
 #pragma unroll
-  for (int r = 0; r < R; ++r) off[r + 1] = off[r] + counts[tile * R + r];
+  for (int r = 0; r < R; ++r) off[r + 1] = off[r] + (mine ? M.counts[tile * R + r] : 0u);
   const uint32_t total = off[R];
-  for (uint32_t base = 0; base < total; base += 64 * SC_Q) {
-    uint2 e[SC_Q];
+  uint2 e[TR_Q];
+  auto load = [&](uint32_t base) __attribute__((always_inline)) {
 #pragma unroll
-    for (int q = 0; q < SC_Q; ++q) {
-      const uint32_t idx = base + q * 64 + threadIdx.x;
+    for (int q = 0; q < TR_Q; ++q) {
+      const uint32_t idx = base + q * 64 + lane;
       uint32_t r = 0;
 #pragma unroll
       for (int k = 1; k < R; ++k) r += idx >= off[k];
-      const uint2 v = rent[(tile * R + r) * PL_WAVE_ROWS + (idx < total ? idx - off[r] : 0u)];  // in bounds
-      e[q] = idx < total ? v : make_uint2(0xFFFFFFFFu, 0u);
+      e[q] = make_uint2(0xFFFFFFFFu, 0u);
+      if (idx < total) e[q] = M.rent[(tile * R + r) * PL_WAVE_ROWS + (idx - off[r])];
     }
-    // every lane computes a rank for every q (an empty entry uses code 0) so that the
-    // directory reads of all q are in flight together; only real entries store
-    uint32_t g[SC_Q];
-#pragma unroll
-    for (int q = 0; q < SC_Q; ++q) g[q] = bm_rank9(bits, rank, rel, e[q].x != 0xFFFFFFFFu ? e[q].x : 0u);
-#pragma unroll
-    for (int q = 0; q < SC_Q; ++q)
-      if (e[q].x != 0xFFFFFFFFu) {
-        grow[g[q]] = e[q].y;
-        sum[g[q]] = 0;
-        cnt[g[q]] = 0;
+  };
+  load(0);
+  if (!all) {
+    for (uint32_t q = threadIdx.x; q < TR_HASH; q += ST_BLOCK) hk[q] = 0xFFFFFFFFu;
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < n; i += ST_BLOCK) {  // n <= TR_MAX: a free slot exists
+      const uint32_t g = cj[i], code = ccode[i];
+      uint32_t q = (code * 0x9E3779B1u) >> (32 - TR_BITS);
+      while (true) {
+        const uint32_t prev = atomicCAS(&hk[q], 0xFFFFFFFFu, code);
+        if (prev == 0xFFFFFFFFu) {
+          hg[q] = g;
+          break;
+        }
+        if (prev == code) break;  // (distinct groups have distinct codes)
+        q = (q + 1) & (TR_HASH - 1);
       }
+    }
+    __syncthreads();
+  }
+  for (uint32_t base = 0; base < total; base += 64 * TR_Q) {
+    if (base > 0) load(base);
+    if (all) {
+      uint32_t g[TR_Q];
+#pragma unroll
+      for (int q = 0; q < TR_Q; ++q) g[q] = bm_rank9(M.bits, M.rank, M.rel, e[q].x != 0xFFFFFFFFu ? e[q].x : 0u);
+#pragma unroll
+      for (int q = 0; q < TR_Q; ++q)
+        if (e[q].x != 0xFFFFFFFFu) M.grow[g[q]] = e[q].y;
+    } else {
+#pragma unroll
+      for (int q = 0; q < TR_Q; ++q) {
+        if (e[q].x == 0xFFFFFFFFu) continue;
+        uint32_t h = (e[q].x * 0x9E3779B1u) >> (32 - TR_BITS);
+        while (true) {  // the table holds at most half its slots: an empty one ends the probe
+          const uint32_t k = hk[h];
+          if (k == e[q].x) {
+            M.grow[hg[h]] = e[q].y;
+            break;
+          }
+          if (k == 0xFFFFFFFFu) break;
+          h = (h + 1) & (TR_HASH - 1);
+        }
+      }
+    }
   }
 }
 
@@ -1739,6 +1839,8 @@ __global__ void __launch_bounds__(TB_BLOCK) k_topk_bmax_held(const uint32_t *__r
   }
 }
 
+// attr == 0: the candidates' group attributes are not readable yet (their mid rows are
+// found afterwards, k_topk_rows) and clo is left to k_topk_clo.
 __global__ void __launch_bounds__(TB_BLOCK) k_topk_collect_held(const uint32_t *__restrict__ held, uint64_t nw,
                                                                 const unsigned long long *__restrict__ sum,
                                                                 GroupAttr ga, int64_t g1min, int64_t kmin,
@@ -1746,7 +1848,7 @@ __global__ void __launch_bounds__(TB_BLOCK) k_topk_collect_held(const uint32_t *
                                                                 uint32_t *__restrict__ count,
                                                                 uint64_t *__restrict__ chi,
                                                                 uint64_t *__restrict__ clo,
-                                                                uint32_t *__restrict__ cj) {
+                                                                uint32_t *__restrict__ cj, int attr) {
   const uint64_t t = *thr;
   for (uint64_t w = (uint64_t)blockIdx.x * TB_BLOCK + threadIdx.x; w < nw; w += (uint64_t)gridDim.x * TB_BLOCK) {
     uint32_t b = held[w];
@@ -1758,11 +1860,23 @@ __global__ void __launch_bounds__(TB_BLOCK) k_topk_collect_held(const uint32_t *
         const uint32_t pos = atomicAdd(count, 1u);
         if (pos < cap) {
           chi[pos] = ~o;
-          clo[pos] = ((uint64_t)(ga_g1(ga, g) - g1min) << 32) | (uint32_t)((uint64_t)ga_key(ga, g) - (uint64_t)kmin);
+          if (attr)
+            clo[pos] = ((uint64_t)(ga_g1(ga, g) - g1min) << 32) | (uint32_t)((uint64_t)ga_key(ga, g) - (uint64_t)kmin);
           cj[pos] = (uint32_t)g;
         }
       }
     }
+  }
+}
+
+// The second half of the candidates' keys (g1, then key) once their mid rows are known.
+__global__ void __launch_bounds__(TB_BLOCK) k_topk_clo(const uint32_t *__restrict__ count, uint32_t cap,
+                                                       const uint32_t *__restrict__ cj, GroupAttr ga, int64_t g1min,
+                                                       int64_t kmin, uint64_t *__restrict__ clo) {
+  const uint32_t n = *count < cap ? *count : cap;
+  for (uint32_t i = blockIdx.x * TB_BLOCK + threadIdx.x; i < n; i += gridDim.x * TB_BLOCK) {
+    const uint32_t g = cj[i];
+    clo[i] = ((uint64_t)(ga_g1(ga, g) - g1min) << 32) | (uint32_t)((uint64_t)ga_key(ga, g) - (uint64_t)kmin);
   }
 }
 
@@ -1790,12 +1904,28 @@ __global__ void k_topk_emit(const uint32_t *__restrict__ j, uint32_t k,
 // candidates are sorted and cut to k, repeatedly, until one tile remains.
 // With ctr (nctr device counters followed by room for the result block), the threshold
 // path reads the counters and its result with one copy into h_ctr and sets *ctr_read.
+// Every passing mid row's group into rows->grow (k_topk_rows, full mode).
+static int mid_rows_all(aln_ctx *c, const MidRows &rows) {
+  if (rows.ntiles == 0) return ALN_OK;
+  hipLaunchKernelGGL(k_topk_rows, dim3((unsigned)((rows.ntiles + ST_BLOCK / 64 - 1) / (ST_BLOCK / 64))),
+                     dim3(ST_BLOCK), 0, c->stream, rows, (const uint32_t *)nullptr, 0u, (const uint32_t *)nullptr,
+                     (const uint32_t *)nullptr, 1);
+  return check_launch("topk_rows");
+}
+
+// mrows != nullptr: ga reads the groups' mid rows from mrows->grow, which holds nothing yet;
+// the rows of the groups that need them are found here (only the candidates', when the
+// threshold path can take them).
 static int star_topk(aln_ctx *c, uint64_t m, const unsigned long long *sum, const unsigned int *cnt,
                      const GroupAttr &ga, uint32_t k, int64_t g1min, int64_t g1max, int64_t kmin,
                      aln_star_row *h_top, uint32_t *h_ntop, uint64_t *ctr = nullptr, uint32_t nctr = 0,
-                     uint64_t *h_ctr = nullptr, bool *ctr_read = nullptr, const uint32_t *held = nullptr) {
-  if (k <= (uint32_t)TS_K && g1max >= g1min && (uint64_t)g1max - (uint64_t)g1min < (1ull << 32) &&
-      m < (1ull << 32)) {
+                     uint64_t *h_ctr = nullptr, bool *ctr_read = nullptr, const uint32_t *held = nullptr,
+                     const MidRows *mrows = nullptr) {
+  const bool thr_path = k <= (uint32_t)TS_K && g1max >= g1min && (uint64_t)g1max - (uint64_t)g1min < (1ull << 32) &&
+                        m < (1ull << 32);
+  const bool lazy = mrows && held && thr_path;  // (with held, the path below has one result block)
+  if (mrows && !lazy) ALN_TRY(mid_rows_all(c, *mrows));
+  if (thr_path) {
     // threshold path: block maxima -> k-th largest -> the groups at or above it (over the
     // bit set of groups that hold rows when there is one, else over every group slot)
     {
@@ -1833,10 +1963,21 @@ static int star_topk(aln_ctx *c, uint64_t m, const unsigned long long *sum, cons
           hipLaunchKernelGGL(k_topk_kth, dim3(1), dim3(TB_BLOCK), 0, c->stream, bmax, (uint32_t)nb, k, thr);
           if (held)
             hipLaunchKernelGGL(k_topk_collect_held, dim3((unsigned)nb), dim3(TB_BLOCK), 0, c->stream, held, hw, sum, ga,
-                               g1min, kmin, thr, CAP, count, chi, clo, cj);
+                               g1min, kmin, thr, CAP, count, chi, clo, cj, lazy ? 0 : 1);
           else
             hipLaunchKernelGGL(k_topk_collect, dim3((unsigned)nb), dim3(TB_BLOCK), 0, c->stream, m, sum, cnt, ga,
                                g1min, kmin, thr, CAP, count, chi, clo, cj);
+          if (lazy) {  // the candidates' codes and mid rows, then their (g1, key)
+            uint32_t *ccode;
+            ALN_TRY(ws(c, TR_MAX, &ccode));
+            hipLaunchKernelGGL(k_topk_codes, dim3(TR_MAX / 4), dim3(256), 0, c->stream, *mrows,
+                               (const uint32_t *)count, CAP, (const uint32_t *)cj, ccode);
+            hipLaunchKernelGGL(k_topk_rows, dim3((unsigned)((mrows->ntiles + ST_BLOCK / 64 - 1) / (ST_BLOCK / 64))),
+                               dim3(ST_BLOCK), 0, c->stream, *mrows, (const uint32_t *)count, CAP,
+                               (const uint32_t *)cj, (const uint32_t *)ccode, 0);
+            hipLaunchKernelGGL(k_topk_clo, dim3(CAP / TB_BLOCK / 4), dim3(TB_BLOCK), 0, c->stream,
+                               (const uint32_t *)count, CAP, (const uint32_t *)cj, ga, g1min, kmin, clo);
+          }
           hipLaunchKernelGGL(k_topk_sel, dim3(1), dim3(TS_BLOCK), 0, c->stream, (uint64_t)CAP, count, sum, cnt, ga,
                              g1min, kmin, chi, clo, cj, k, oh, ol, oj);
           hipLaunchKernelGGL(k_topk_emit, dim3(1), dim3(64), 0, c->stream, oj, k, sum, cnt, ga, rows, nv);
@@ -1857,7 +1998,9 @@ static int star_topk(aln_ctx *c, uint64_t m, const unsigned long long *sum, cons
           *h_ntop = hres[1];
           return ALN_OK;
         }
-        // more groups tie at the threshold than the buffer holds: the passes below
+        // more groups tie at the threshold than the buffer holds: the passes below, which
+        // read every group's attributes
+        if (lazy) ALN_TRY(mid_rows_all(c, *mrows));
       }
     }
     // passes of at most ~16 candidates per thread until one block holds the result
@@ -2111,14 +2254,15 @@ extern "C" int aln_star_agg(aln_ctx *c, const aln_star_args *a, aln_star_row *h_
   const GroupAttr ga{nullptr, nullptr, nullptr, grow, mplan, a->mid.seg_rows};
   uint64_t *table = nullptr;
   uint64_t tcap = 0;
+  const MidRows rows{rent, mcnt, mblocks, mbits, rank, rel, mblk, grow};
   if (use_bm) {
-    // the group attributes in rank (= key) order
+    // groups are ranks (= key order): zeroed accumulators now; the mid row of a group is
+    // looked up later, for the groups the top-k or the group list needs
     if (m) {
       ProfScope p(c, "star_build");
-      hipLaunchKernelGGL(k_bm_scatter, dim3((unsigned)mblocks), dim3(64), 0, c->stream, rent, mcnt, mbits, rank, rel,
-                         grow, sum, cnt);
+      ALN_TRY(fill_u64(c, reinterpret_cast<uint64_t *>(sum), m, 0));
+      ALN_TRY(fill_u32(c, cnt, m, 0));
     }
-    ALN_TRY(check_launch("bm_scatter"));
   } else {
     uint64_t *moff;
     ALN_TRY(ws(c, mregions, &moff));
@@ -2168,7 +2312,7 @@ extern "C" int aln_star_agg(aln_ctx *c, const aln_star_args *a, aln_star_row *h_
   bool ctr_read = false;
   if (a->limit && m)
     ALN_TRY(star_topk(c, m, sum, cnt, ga, a->limit, g1min, g1max, mkmin, h_top, h_ntop, ctr, 8, hc, &ctr_read,
-                      held));
+                      held, use_bm ? &rows : nullptr));
   tm.mark(4);
   if (!ctr_read) ALN_TRY(copy_to_host(c, hc, ctr, sizeof(hc)));
   if (hdup[1]) return set_error(ALN_E_UNSUPPORTED, "aln_star_agg: duplicate mid keys");
@@ -2177,6 +2321,7 @@ extern "C" int aln_star_agg(aln_ctx *c, const aln_star_args *a, aln_star_row *h_
 
   // every group, in ascending key order
   if ((a->flags & 1) && m) {
+    if (use_bm) ALN_TRY(mid_rows_all(c, rows));
     uint32_t *flag, *idx;
     uint64_t *pos, *code;
     ALN_TRY(ws(c, m, &flag));

@@ -543,9 +543,10 @@ def test_star_q3(ctx, scale, seg_rows, join):
     assert np.array_equal(host(g), groups)
 
 
-def star_expected(t, dr, mr, fr, limit=10):
+def star_expected(t, dr, mr, fr, limit=10, coef=(0, 1, 100, -1)):
     """Oracle answer of a star query with range lists (col, lo, hi) on the q3 tables: the same
-    sort-merge joins and sort-based group-by as the q3 reference run, SUM(price*(100-disc))."""
+    sort-merge joins and sort-based group-by as the q3 reference run, SUM((a0 + a1 price) *
+    (b0 + b1 disc)) with coef = (a0, a1, b0, b1) (q3: price * (100 - disc))."""
     cols = dict(d=[t["c_key"], t["c_seg"]], m=[t["o_key"], t["o_cust"], t["o_date"], t["o_prio"]],
                 f=[t["l_okey"], t["l_price"], t["l_disc"], t["l_ship"]])
 
@@ -560,7 +561,8 @@ def star_expected(t, dr, mr, fr, limit=10):
     olc1 = o_ids[pl]
     ql, qr = O.join_sortmerge(t["l_okey"][l_ids], t["o_key"][olc1])
     lrow, orow = l_ids[ql], olc1[qr]
-    rev = t["l_price"][lrow] * (100 - t["l_disc"][lrow])
+    a0, a1, b0, b1 = coef
+    rev = (a0 + a1 * t["l_price"][lrow]) * (b0 + b1 * t["l_disc"][lrow])
     gk, gv = O.groupby([t["o_key"][orow], t["o_date"][orow], t["o_prio"][orow]], [rev, rev],
                        [L.AGG_SUM, L.AGG_COUNT])
     perm = O.order([gv[0], gk[1]], [1, 0])
@@ -609,6 +611,38 @@ def test_star_scan_shapes(ctx, case, join):
     assert rows == top
     assert (st.dim_pass, st.mid_pass, st.fact_pass) == (dpass, mpass, fpass)
     assert np.array_equal(host(g), groups)
+
+
+@pytest.mark.parametrize("join", ["bitmap", "hash"])
+@pytest.mark.parametrize("case", ["dense", "filtered"])
+def test_star_topk_ties(ctx, case, join):
+    """SUM(1) per joined row: a group's sum is its row count (1 to 7), so many groups tie at
+    the top-k threshold and the order falls to (o_date, o_key).  Every row passing gives
+    thousands of candidate groups at the threshold (the top-k then looks up every group's mid
+    row); the q3 filters give fewer (each candidate's row found by its code)."""
+    from alenka_amd import device as D
+
+    t = Q3.tables(91, 3000, 30000)
+    seg_rows = 4096
+    dim, mid, fact = encode_q3(ctx, t, seg_rows)
+    lo, hi = -(1 << 63), (1 << 63) - 1
+    building = int(t["code_of"][Q3.SEGMENTS.index("BUILDING")])
+    if case == "dense":
+        dr, mr, fr = [(1, lo, hi)], [(2, lo, hi)], [(3, lo, hi)]
+    else:
+        dr, mr, fr = [(1, building, building)], [(2, lo, Q3.DATE - 1)], [(3, Q3.DATE + 1, hi)]
+    top, groups, dpass, mpass, fpass = star_expected(t, dr, mr, fr, coef=(1, 0, 1, 0))
+    kth = np.sort(groups[:, 3])[::-1][9]  # the 10th largest sum: groups at or above it are candidates
+    assert int((groups[:, 3] >= kth).sum()) > (2048 if case == "dense" else 10)
+    spec = dict(d_key=0, m_key=0, m_fk=1, m_g1=2, m_g2=3, f_fk=0, f_x=1, f_y=2, a0=1, a1=0, b0=1, b1=0)
+    for want in (False, True):  # the top-k alone, and with the group list
+        rows, g, st = D.star_agg(ctx, dict(cols=dim, ranges=dr), dict(cols=mid, ranges=mr),
+                                 dict(cols=fact, ranges=fr), spec, limit=10, want_groups=want,
+                                 force_hash=join == "hash")
+        assert rows == top
+        assert (st.dim_pass, st.mid_pass, st.fact_pass) == (dpass, mpass, fpass)
+        if want:
+            assert np.array_equal(host(g), groups)
 
 
 @pytest.mark.parametrize("scale,seg_rows", [(0.02, 4096), (0.1, 65536)])
