@@ -557,7 +557,8 @@ def serve_leg(ctx, D, dim, mid, fact, zc, code_of, seed, rate, duration, budget_
     operators) on Poisson traces of the four templates.  Protocol: each form's saturation
     throughput from a burst; the latency target is the plain form's p99 at 80 % of its
     saturation; each form's max sustained q/s is the highest offered rate (a sweep around
-    its saturation) whose p99 stays within that target.  Also both forms at `rate`."""
+    its saturation, then two bisection steps between the last rate within the target and
+    the first beyond it) whose p99 stays within that target.  Also both forms at `rate`."""
     import torch
 
     scales = [1, 10, 100]
@@ -583,11 +584,28 @@ def serve_leg(ctx, D, dim, mid, fact, zc, code_of, seed, rate, duration, budget_
     log(f"serve SLO p99 {slo:.1f} ms")
     for f in forms:
         best = None
+        ok_lo, bad_hi = 0.0, None  # offered fractions of saturation within / beyond the target
         for k, frac in enumerate((0.5, 0.7, 0.85, 1.0, 1.2)):
             r = pt(f, frac * sat[f], span(frac * sat[f]), seed + 30 + k)
             log(f"serve {f} at {r['offered_qps']:.1f} q/s: {r['qps']:.1f} q/s, p99 {r['p99_ms']:.1f} ms")
-            if r["p99_ms"] <= slo and r["failed"] == 0 and (best is None or r["qps"] > best["qps"]):
+            ok = r["p99_ms"] <= slo and r["failed"] == 0
+            if ok and (best is None or r["qps"] > best["qps"]):
                 best = r
+            if ok and bad_hi is None:
+                ok_lo = frac
+            elif not ok and bad_hi is None:
+                bad_hi = frac
+        # two bisection steps between the last rate within the target and the first beyond it
+        for k in range(2 if bad_hi is not None else 0):
+            frac = (ok_lo + bad_hi) / 2
+            r = pt(f, frac * sat[f], span(frac * sat[f]), seed + 40 + k)
+            log(f"serve {f} at {r['offered_qps']:.1f} q/s: {r['qps']:.1f} q/s, p99 {r['p99_ms']:.1f} ms")
+            if r["p99_ms"] <= slo and r["failed"] == 0:
+                ok_lo = frac
+                if best is None or r["qps"] > best["qps"]:
+                    best = r
+            else:
+                bad_hi = frac
         out[f + "_at_slo"] = best
     if out["served_at_slo"] and out["plain_at_slo"]:
         out["speedup_qps_at_equal_p99"] = out["served_at_slo"]["qps"] / out["plain_at_slo"]["qps"]
