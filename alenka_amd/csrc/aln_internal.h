@@ -252,5 +252,10 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
   unsigned lane = __lane_id();
   return lane == 0 ? 0ull : (~0ull >> (64 - lane));
 }
+// Set bits of a wave mask in the lanes below this one: two mbcnt instructions (the same as
+// popcount(b & lanemask_lt()) without building the mask).
+__device__ __forceinline__ uint32_t below_count(uint64_t b) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(b >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)b, 0u));
+}
 
 }  // namespace aln

@@ -261,7 +261,7 @@ __global__ void __launch_bounds__(SC_BLOCK) k_scan(const ColTable *__restrict__ 
         before += w < (int)wid ? x : 0;
         all += x;
       }
-      if (pass) out[run + before + __popcll(bal & lanemask_lt())] = (uint32_t)(gbase + local);
+      if (pass) out[run + before + below_count(bal)] = (uint32_t)(gbase + local);
       run += all;
       __syncthreads();
     }

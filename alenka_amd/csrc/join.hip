@@ -359,7 +359,7 @@ __global__ void __launch_bounds__(J_BLOCK) k_jd_write(const uint32_t *__restrict
       all += c;
     }
     if (h) {
-      const uint32_t q = base + before + (uint32_t)__popcll(bal & lanemask_lt());
+      const uint32_t q = base + before + below_count(bal);
       sp[q] = (uint32_t)p;
       sb[q] = m;
     }

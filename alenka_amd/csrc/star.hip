@@ -780,7 +780,7 @@ __global__ void __launch_bounds__(ST_BLOCK, 8) k_star_mid(ScanDesc d, const SegP
                                : tile_get(ts, d.words, d.segs, kcol, seg, r0, i);
       const uint32_t code = (uint32_t)((uint64_t)key - (uint64_t)mkmin);  // key range < 2^32
       mcode[j] = code;
-      rent[rbase + nout + (uint32_t)__popcll(b & lanemask_lt())] = make_uint2(code, row0 + i);
+      rent[rbase + nout + below_count(b)] = make_uint2(code, row0 + i);
     }
     nout += (uint32_t)__popcll(b);
   }
@@ -1182,8 +1182,9 @@ __device__ __forceinline__ void fact_probe_hits(const SegPlan &P, const void *an
   for (int j = 0; j < ST_ITEMS; ++j) {
     const bool h = (code[j] != 0xFFFFFFFFu) & ((word[j] >> (code[j] & 31)) & 1);
     const uint64_t b = __builtin_amdgcn_ballot_w64(h);
+    if (b == 0) continue;  // (wave-uniform) no joined row in this row of the wave: most of them
     if (A.hitbuf) {  // slot of this row among the wave's joined rows; the first slots are deferred
-      const uint32_t pos = nmatch + (uint32_t)__popcll(b & lanemask_lt());
+      const uint32_t pos = nmatch + below_count(b);
       if (h && pos < FACT_HIT_CAP) {
         A.hitbuf[(uint64_t)cidx * FACT_HIT_CAP + pos] = make_uint2(code[j], r0 + j * ST_BLOCK + threadIdx.x);
       } else {
@@ -2680,7 +2681,7 @@ __global__ void __launch_bounds__(ST_BLOCK) k_sb_fact(SBScan t, const SBQuery *_
           const bool h = (hits >> u) & 1;
           const uint64_t b = __ballot(h);
           if (h) {
-            const uint32_t pos = woff[u][wave] + (uint32_t)__popcll(b & lanemask_lt());
+            const uint32_t pos = woff[u][wave] + below_count(b);
             hcode[pos] = code[u];
             hrow[pos] = (uint16_t)((ch * SB_CHUNK + u) * ST_BLOCK + threadIdx.x);
           }
