@@ -172,28 +172,35 @@ __device__ __forceinline__ uint32_t fast_code(const uint32_t *w, uint32_t bit0, 
 // j * 512 + t sits 512 * bits bits (16 * bits words) after row (j - 1) * 512 + t, so a
 // thread's rows share one funnel-shift amount and step through LDS by a constant.  Every
 // read stays inside the slots, which span a whole tile.  Returns the wave's probe count.
-template <bool RANGE>
-__device__ __forceinline__ uint32_t fast_probe(const FastTile &f, uint32_t nt, uint32_t off, uint32_t lim,
-                                               uint32_t (&code)[ST_ITEMS]) {
+template <bool RANGE, bool FULL>
+__device__ __forceinline__ uint32_t fast_probe_t(const FastTile &f, uint32_t nt, uint32_t off, uint32_t lim,
+                                                 uint32_t (&code)[ST_ITEMS]) {
   const uint32_t kp0 = f.kbit0 + threadIdx.x * f.kbits, rp0 = f.rbit0 + threadIdx.x * f.rbits;
   const uint32_t *kw = f.kw + (kp0 >> 5), *rw = f.rw + (rp0 >> 5);
   const uint32_t ksh = kp0 & 31, rsh = rp0 & 31, kstep = 16 * f.kbits, rstep = 16 * f.rbits;
   uint32_t n = 0;
 #pragma unroll
   for (int j = 0; j < ST_ITEMS; ++j) {
-    const uint32_t i = j * ST_BLOCK + threadIdx.x;
-    bool p = i < nt;
+    // the probing rows as a lane mask (FULL: a whole tile, every row exists), turned back
+    // into the lane's condition with inverse_ballot so that it stays a mask
+    uint64_t pm = FULL ? ~0ull : __builtin_amdgcn_ballot_w64(j * ST_BLOCK + threadIdx.x < nt);
     if (RANGE) {
       const uint32_t *q = rw + j * rstep;
       const uint32_t rc = __builtin_amdgcn_alignbit(q[1], q[0], rsh) & f.rmask;
-      p = p & (rc - f.clo <= f.cspan);
+      pm &= __builtin_amdgcn_ballot_w64(rc - f.clo <= f.cspan);
     }
     const uint32_t *q = kw + j * kstep;
     const uint32_t c = (__builtin_amdgcn_alignbit(q[1], q[0], ksh) & f.kmask) + off;
-    n += (uint32_t)__popcll(__builtin_amdgcn_ballot_w64(p));
-    code[j] = (p & (c < lim)) ? c : 0xFFFFFFFFu;
+    n += (uint32_t)__popcll(pm);
+    code[j] = (__builtin_amdgcn_inverse_ballot_w64(pm) & (c < lim)) ? c : 0xFFFFFFFFu;
   }
   return n;
+}
+template <bool RANGE>
+__device__ __forceinline__ uint32_t fast_probe(const FastTile &f, uint32_t nt, uint32_t off, uint32_t lim,
+                                               uint32_t (&code)[ST_ITEMS]) {
+  return nt == (uint32_t)ST_TILE ? fast_probe_t<RANGE, true>(f, nt, off, lim, code)
+                                 : fast_probe_t<RANGE, false>(f, nt, off, lim, code);
 }
 
 __device__ __forceinline__ bool tile_pass(const ScanDesc &d, const TileStage &ts, char mode,
@@ -471,8 +478,7 @@ struct StageRegs {
 #pragma unroll
       for (int t = 0; t < PL_CVEC; ++t) {
         const uint32_t v = t * ST_BLOCK + threadIdx.x;
-        r[k][t] = make_uint4(0, 0, 0, 0);
-        if (v < nb[k]) r[k][t] = gload4_nt(src + v);
+        if (v < nb[k]) r[k][t] = gload4_nt(src + v);  // (store() skips the rest)
       }
     }
   }
@@ -1180,9 +1186,11 @@ __device__ __forceinline__ void fact_probe_hits(const SegPlan &P, const void *an
   uint32_t hits = 0, nmatch = 0;
 #pragma unroll
   for (int j = 0; j < ST_ITEMS; ++j) {
-    const bool h = (code[j] != 0xFFFFFFFFu) & ((word[j] >> (code[j] & 31)) & 1);
-    const uint64_t b = __builtin_amdgcn_ballot_w64(h);
+    // the joined rows as a lane mask, from the two compares (kept a mask, not a lane value)
+    const uint64_t b = __builtin_amdgcn_ballot_w64(code[j] != 0xFFFFFFFFu) &
+                       __builtin_amdgcn_ballot_w64(((word[j] >> (code[j] & 31)) & 1) != 0);
     if (b == 0) continue;  // (wave-uniform) no joined row in this row of the wave: most of them
+    const bool h = __builtin_amdgcn_inverse_ballot_w64(b);
     if (A.hitbuf) {  // slot of this row among the wave's joined rows; the first slots are deferred
       const uint32_t pos = nmatch + below_count(b);
       if (h && pos < FACT_HIT_CAP) {
