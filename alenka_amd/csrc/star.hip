@@ -651,25 +651,39 @@ __global__ void __launch_bounds__(ST_BLOCK, 8) k_star_dim(ScanDesc d, const SegP
 // keys beyond the window's PL_STAGE_WORDS words go to the bitmap directly.  The bits are
 // set without a returned value: a duplicate key shows as popcount(bitmap) < passing rows.
 // Reuses the tile's staging area (`win`), so every read of it must be done.
+// Wave minimum / maximum through DPP row shifts and row broadcasts (lane 63 ends with the
+// whole wave's; lanes a step has no source for take the neutral value), read from lane 63.
+template <bool MAX>
+__device__ __forceinline__ uint32_t wave_minmax_u32(uint32_t v) {
+  constexpr int NEUTRAL = MAX ? 0 : -1;
+  auto step = [&](uint32_t x) __attribute__((always_inline)) { v = MAX ? max(v, x) : min(v, x); };
+  step((uint32_t)__builtin_amdgcn_update_dpp(NEUTRAL, (int)v, 0x111, 0xF, 0xF, false));  // row_shr:1
+  step((uint32_t)__builtin_amdgcn_update_dpp(NEUTRAL, (int)v, 0x112, 0xF, 0xF, false));  // row_shr:2
+  step((uint32_t)__builtin_amdgcn_update_dpp(NEUTRAL, (int)v, 0x114, 0xF, 0xF, false));  // row_shr:4
+  step((uint32_t)__builtin_amdgcn_update_dpp(NEUTRAL, (int)v, 0x118, 0xF, 0xF, false));  // row_shr:8
+  step((uint32_t)__builtin_amdgcn_update_dpp(NEUTRAL, (int)v, 0x142, 0xA, 0xF, false));  // row_bcast:15
+  step((uint32_t)__builtin_amdgcn_update_dpp(NEUTRAL, (int)v, 0x143, 0xC, 0xF, false));  // row_bcast:31
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
+
 __device__ __forceinline__ void mid_bits_window(uint32_t *win, const uint32_t *code, uint32_t *__restrict__ mbits) {
   __shared__ uint32_t s_lo, s_hi;
   if (threadIdx.x == 0) {
     s_lo = 0xFFFFFFFFu;
     s_hi = 0;
   }
-  uint32_t lo = 0xFFFFFFFFu, hi = 0;
+  // the lowest code, and the highest plus one (an absent code, ~0, is the largest value
+  // and wraps to 0 plus one), then their words
+  uint32_t lo = 0xFFFFFFFFu, hi1 = 0;
 #pragma unroll
-  for (int j = 0; j < ST_ITEMS; ++j)
-    if (code[j] != 0xFFFFFFFFu) {
-      lo = min(lo, code[j] >> 5);
-      hi = max(hi, code[j] >> 5);
-    }
-  for (int o = 32; o > 0; o >>= 1) {
-    lo = min(lo, (uint32_t)__shfl_xor((int)lo, o, 64));
-    hi = max(hi, (uint32_t)__shfl_xor((int)hi, o, 64));
+  for (int j = 0; j < ST_ITEMS; ++j) {
+    lo = min(lo, code[j]);
+    hi1 = max(hi1, code[j] + 1u);
   }
+  const uint32_t clo = wave_minmax_u32<false>(lo), hi = (wave_minmax_u32<true>(hi1) - 1u) >> 5;
+  lo = clo >> 5;
   __syncthreads();  // s_lo / s_hi set; the staging area is no longer read
-  if (__lane_id() == 0 && lo != 0xFFFFFFFFu) {
+  if (__lane_id() == 0 && clo != 0xFFFFFFFFu) {  // (a code is never ~0)
     atomicMin(&s_lo, lo);
     atomicMax(&s_hi, hi);
   }
@@ -757,16 +771,15 @@ __global__ void __launch_bounds__(ST_BLOCK, 8) k_star_mid(ScanDesc d, const SegP
     }
     nprobe = wave_sum(nprobe);
   }
-  uint32_t dw[ST_ITEMS];
+  // the probes in 32 bits: word index (~0: no probe; < 2^29 for any dimension range) and bit
+  uint32_t pw[ST_ITEMS], dw[ST_ITEMS];
+#pragma unroll
+  for (int j = 0; j < ST_ITEMS; ++j) pw[j] = dc[j] != ~0ull ? (uint32_t)(dc[j] >> 5) : 0xFFFFFFFFu;
 #pragma unroll
   for (int j = 0; j < ST_ITEMS; ++j) {  // only probing lanes load (fewer addresses per load)
     dw[j] = 0;
-    if (dc[j] != ~0ull) dw[j] = gload(bitmap + (dc[j] >> 5));
+    if (pw[j] != 0xFFFFFFFFu) dw[j] = gload(bitmap + pw[j]);
   }
-  uint32_t passmask = 0;
-#pragma unroll
-  for (int j = 0; j < ST_ITEMS; ++j)
-    passmask |= (uint32_t)(dc[j] != ~0ull && ((dw[j] >> (dc[j] & 31)) & 1)) << j;
   // the passing rows' keys: staged (fast) or decoded (generic)
   const PlanSlot ks = plan_slot(lds, P.col[2], r0);
   const uint32_t row0 = seg * d.seg_rows + r0;
@@ -777,10 +790,11 @@ __global__ void __launch_bounds__(ST_BLOCK, 8) k_star_mid(ScanDesc d, const SegP
   uint32_t mcode[ST_ITEMS];
 #pragma unroll
   for (int j = 0; j < ST_ITEMS; ++j) {
-    const bool h = (passmask >> j) & 1;
-    const uint64_t b = __ballot(h);
+    // the passing rows as a lane mask, from the two compares (kept a mask, not a lane value)
+    const uint64_t b = __builtin_amdgcn_ballot_w64(pw[j] != 0xFFFFFFFFu) &
+                       __builtin_amdgcn_ballot_w64(((dw[j] >> ((uint32_t)dc[j] & 31)) & 1) != 0);
     mcode[j] = 0xFFFFFFFFu;
-    if (h) {
+    if (__builtin_amdgcn_inverse_ballot_w64(b)) {
       const uint32_t i = j * ST_BLOCK + threadIdx.x;
       const int64_t key = fast ? (int64_t)((uint64_t)P.col[2].base + slot_code(ks, i))
                                : tile_get(ts, d.words, d.segs, kcol, seg, r0, i);
