@@ -780,12 +780,15 @@ __global__ void __launch_bounds__(ST_BLOCK, 8) k_star_mid(ScanDesc d, const SegP
     dw[j] = 0;
     if (pw[j] != 0xFFFFFFFFu) dw[j] = gload(bitmap + pw[j]);
   }
-  // the passing rows' keys: staged (fast) or decoded (generic)
+  // the passing rows' keys: staged (fast; code = kbase + the slot's code, its bit position
+  // a per-lane start plus a per-item step) or decoded (generic)
   const PlanSlot ks = plan_slot(lds, P.col[2], r0);
+  const uint32_t kbase = (uint32_t)((uint64_t)P.col[2].base - (uint64_t)mkmin);
+  const uint32_t kp0 = ks.bit0 + threadIdx.x * ks.bits, kstep = ST_BLOCK * ks.bits;
   const uint32_t row0 = seg * d.seg_rows + r0;
   // passing rows of the wave in item-major order (consecutive lanes write consecutive
   // entries); no barrier: each wave owns its region
-  const uint64_t rbase = (uint64_t)region * PL_WAVE_ROWS;
+  uint2 *const reg = rent + (uint64_t)region * PL_WAVE_ROWS;
   uint32_t nout = 0;
   uint32_t mcode[ST_ITEMS];
 #pragma unroll
@@ -796,11 +799,15 @@ __global__ void __launch_bounds__(ST_BLOCK, 8) k_star_mid(ScanDesc d, const SegP
     mcode[j] = 0xFFFFFFFFu;
     if (__builtin_amdgcn_inverse_ballot_w64(b)) {
       const uint32_t i = j * ST_BLOCK + threadIdx.x;
-      const int64_t key = fast ? (int64_t)((uint64_t)P.col[2].base + slot_code(ks, i))
-                               : tile_get(ts, d.words, d.segs, kcol, seg, r0, i);
-      const uint32_t code = (uint32_t)((uint64_t)key - (uint64_t)mkmin);  // key range < 2^32
+      uint32_t code;  // key - mkmin (key range < 2^32)
+      if (fast) {
+        const uint32_t pos = kp0 + j * kstep, wi = pos >> 5;
+        code = kbase + (__builtin_amdgcn_alignbit(ks.w[wi + 1], ks.w[wi], pos & 31) & ks.mask);
+      } else {
+        code = (uint32_t)((uint64_t)tile_get(ts, d.words, d.segs, kcol, seg, r0, i) - (uint64_t)mkmin);
+      }
       mcode[j] = code;
-      rent[rbase + nout + below_count(b)] = make_uint2(code, row0 + i);
+      reg[nout + below_count(b)] = make_uint2(code, row0 + i);
     }
     nout += (uint32_t)__popcll(b);
   }
