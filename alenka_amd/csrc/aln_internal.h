@@ -65,6 +65,15 @@ struct aln_ctx {
   std::vector<Pending> pending;
   std::vector<hipEvent_t> event_pool;
   std::map<std::string, std::pair<double, uint64_t>> prof;
+  // the star pipeline's key ranges from the zone maps of the last call's key columns
+  // (segment descriptors and counts): reused by the next call on the same columns, which
+  // checks them on the device and runs again with fresh ranges if they changed
+  struct StarRanges {
+    bool valid = false;
+    const void *segs[3];
+    uint32_t nsegs[3];
+    int64_t mn[3], mx[3];
+  } star_rng;
 };
 
 namespace aln {

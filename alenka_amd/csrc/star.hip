@@ -721,7 +721,8 @@ __global__ void __launch_bounds__(ST_BLOCK, 8) k_star_mid(ScanDesc d, const SegP
                                                        int64_t dkmin, uint64_t drange,
                                                        uint32_t *__restrict__ counts,
                                                        uint32_t *__restrict__ probed, uint2 *__restrict__ rent,
-                                                       uint32_t *__restrict__ mbits, int64_t mkmin) {
+                                                       uint32_t *__restrict__ mbits, int64_t mkmin,
+                                                       uint64_t mrange) {
   static_assert(sizeof(TileStage) <= PL_STAGE_WORDS * sizeof(uint32_t), "generic staging fits the LDS area");
   __shared__ __align__(16) uint32_t lds[PL_STAGE_WORDS];
   __shared__ int want[ALN_MAX_RANGES + 1];
@@ -797,19 +798,25 @@ __global__ void __launch_bounds__(ST_BLOCK, 8) k_star_mid(ScanDesc d, const SegP
     const uint64_t b = __builtin_amdgcn_ballot_w64(pw[j] != 0xFFFFFFFFu) &
                        __builtin_amdgcn_ballot_w64(((dw[j] >> ((uint32_t)dc[j] & 31)) & 1) != 0);
     mcode[j] = 0xFFFFFFFFu;
+    uint32_t code = 0;  // key - mkmin (key range < 2^32)
     if (__builtin_amdgcn_inverse_ballot_w64(b)) {
-      const uint32_t i = j * ST_BLOCK + threadIdx.x;
-      uint32_t code;  // key - mkmin (key range < 2^32)
       if (fast) {
         const uint32_t pos = kp0 + j * kstep, wi = pos >> 5;
         code = kbase + (__builtin_amdgcn_alignbit(ks.w[wi + 1], ks.w[wi], pos & 31) & ks.mask);
       } else {
-        code = (uint32_t)((uint64_t)tile_get(ts, d.words, d.segs, kcol, seg, r0, i) - (uint64_t)mkmin);
+        code = (uint32_t)((uint64_t)tile_get(ts, d.words, d.segs, kcol, seg, r0, j * ST_BLOCK + threadIdx.x) -
+                          (uint64_t)mkmin);
       }
-      mcode[j] = code;
-      reg[nout + below_count(b)] = make_uint2(code, row0 + i);
     }
-    nout += (uint32_t)__popcll(b);
+    // a key outside [mkmin, mkmin + mrange) can only come from ranges the caller reused
+    // after the columns changed (the run is then repeated): it is left out, so that no
+    // bitmap or group index ever leaves its array
+    const uint64_t b2 = __builtin_amdgcn_ballot_w64(__builtin_amdgcn_inverse_ballot_w64(b) && code < mrange);
+    if (__builtin_amdgcn_inverse_ballot_w64(b2)) {
+      mcode[j] = code;
+      reg[nout + below_count(b2)] = make_uint2(code, row0 + j * ST_BLOCK + threadIdx.x);
+    }
+    nout += (uint32_t)__popcll(b2);
   }
   if (lane == 0) {
     counts[region] = nout;
@@ -1509,6 +1516,16 @@ __global__ void k_cols_range(ColsRange cr, int64_t *out) {
   }
 }
 
+// Sets *stale when the ranges the kernel found (out, 2 per column) differ from the given ones.
+struct RangePair {
+  int64_t v[16];
+};
+__global__ void k_range_check(const int64_t *__restrict__ out, RangePair want, uint32_t n,
+                              unsigned long long *__restrict__ stale) {
+  const uint32_t i = threadIdx.x;
+  if (i < 2 * n && out[i] != want.v[i]) atomicOr(stale, 1ull);
+}
+
 // Zone-map ranges of up to 8 columns with one launch and one copy (nullptr entries: 0, 0).
 static int cols_range(aln_ctx *c, const aln_col *const *cols, int n, int64_t *mn, int64_t *mx) {
   ALN_REQUIRE(n <= 8, "cols_range: %d columns", n);
@@ -2152,27 +2169,65 @@ extern "C" int aln_star_agg(aln_ctx *c, const aln_star_args *a, aln_star_row *h_
   ALN_REQUIRE(a->f_x >= 0 && a->f_x < a->fact.ncols && a->f_y >= 0 && a->f_y < a->fact.ncols,
               "aln_star_agg: aggregate columns");
 
-  // key ranges from the zone maps
+  // key ranges from the zone maps: those of the last call when its key columns are these
+  // (checked on the device below; the run is repeated with fresh ones if they changed),
+  // else read back
   int64_t dkmin, dkmax, mkmin, mkmax, g1min, g1max;
-  {
-    const aln_col *rc[3] = {&a->dim.cols[a->d_key], &a->mid.cols[a->m_key],
-                            a->m_g1 >= 0 ? &a->mid.cols[a->m_g1] : nullptr};
-    int64_t mn[3], mx[3];
+  const aln_col *rc[3] = {&a->dim.cols[a->d_key], &a->mid.cols[a->m_key],
+                          a->m_g1 >= 0 ? &a->mid.cols[a->m_g1] : nullptr};
+  aln_ctx::StarRanges &R = c->star_rng;
+  bool spec = R.valid;
+  for (int i = 0; i < 3; ++i)
+    spec = spec && R.segs[i] == (rc[i] ? (const void *)rc[i]->segs : nullptr) &&
+           R.nsegs[i] == (rc[i] ? rc[i]->nsegs : 0u);
+  int64_t mn[3], mx[3];
+  int64_t *dfresh = nullptr;  // the fresh ranges, 2 per column, on the device (spec)
+  RangePair want;
+  uint32_t nused = 0;
+  if (spec) {
+    ColsRange cr;
+    for (int i = 0; i < 3; ++i) {
+      mn[i] = R.mn[i];
+      mx[i] = R.mx[i];
+      if (!rc[i]) continue;
+      cr.segs[nused] = rc[i]->segs;
+      cr.nsegs[nused] = rc[i]->nsegs;
+      want.v[2 * nused] = mn[i];
+      want.v[2 * nused + 1] = mx[i];
+      ++nused;
+    }
+    ALN_TRY(ws(c, 2 * nused, &dfresh));
+    hipLaunchKernelGGL(k_cols_range, dim3(nused), dim3(256), 0, c->stream, cr, dfresh);
+    ALN_TRY(check_launch("cols_range"));
+  } else {
+    R.valid = false;
     ALN_TRY(cols_range(c, rc, 3, mn, mx));
-    dkmin = mn[0], dkmax = mx[0], mkmin = mn[1], mkmax = mx[1], g1min = mn[2], g1max = mx[2];
   }
-  if (a->dim.cols[0].nsegs == 0 || a->mid.cols[0].nsegs == 0 || dkmax < dkmin || mkmax < mkmin) {
-    if (h_stats) *h_stats = st;
-    return ALN_OK;
-  }
-  uint64_t drange = (uint64_t)dkmax - (uint64_t)dkmin + 1;
-  uint64_t mrange = (uint64_t)mkmax - (uint64_t)mkmin + 1;
-  if (drange == 0 || drange > (1ull << 34))
-    return set_error(ALN_E_UNSUPPORTED, "aln_star_agg: dimension key range %llu too wide for a bitmap",
-                     (unsigned long long)drange);
-  if (mrange == 0 || mrange >= 0xFFFFFFFFull)
+  dkmin = mn[0], dkmax = mx[0], mkmin = mn[1], mkmax = mx[1], g1min = mn[2], g1max = mx[2];
+  const uint64_t drange = (uint64_t)dkmax - (uint64_t)dkmin + 1;
+  const uint64_t mrange = (uint64_t)mkmax - (uint64_t)mkmin + 1;
+  if (a->dim.cols[0].nsegs == 0 || a->mid.cols[0].nsegs == 0 || dkmax < dkmin || mkmax < mkmin ||
+      drange == 0 || drange > (1ull << 34) || mrange == 0 || mrange >= 0xFFFFFFFFull) {
+    // (reused ranges are never of this kind: they are kept only when these checks pass)
+    if (a->dim.cols[0].nsegs == 0 || a->mid.cols[0].nsegs == 0 || dkmax < dkmin || mkmax < mkmin) {
+      if (h_stats) *h_stats = st;
+      return ALN_OK;
+    }
+    if (drange == 0 || drange > (1ull << 34))
+      return set_error(ALN_E_UNSUPPORTED, "aln_star_agg: dimension key range %llu too wide for a bitmap",
+                       (unsigned long long)drange);
     return set_error(ALN_E_UNSUPPORTED, "aln_star_agg: mid key range %llu wider than 32 bits",
                      (unsigned long long)mrange);
+  }
+  if (!spec) {
+    R.valid = true;
+    for (int i = 0; i < 3; ++i) {
+      R.segs[i] = rc[i] ? (const void *)rc[i]->segs : nullptr;
+      R.nsegs[i] = rc[i] ? rc[i]->nsegs : 0u;
+      R.mn[i] = mn[i];
+      R.mx[i] = mx[i];
+    }
+  }
   // join structure for mid -> fact: bitmap + rank over dense key ranges (up to a 256 MB
   // bitmap), MurmurHash open addressing otherwise or when flags bit 1 asks for it
   const bool use_bm = !(a->flags & 2) && mrange <= (1ull << 31);
@@ -2185,12 +2240,14 @@ extern "C" int aln_star_agg(aln_ctx *c, const aln_star_args *a, aln_star_row *h_
   uint32_t *bitmap, *dup;
   // device counters, read back with one copy each time: 0 dim pass, 1 mid pass, 2 mid probe,
   // 3 fact pass, 4 fact probe, 5 distinct passing mid keys, 6 duplicate flags (two u32; the
-  // hash join's build sets the second), 7 distinct passing dimension keys
+  // hash join's build sets the second), 7 distinct passing dimension keys, 8 reused key
+  // ranges found stale
+  constexpr uint32_t NCTR = 9;
   uint64_t *ctr;
   uint32_t *dcnt, *mcnt, *mprobe, *fmatch, *fprobe;
   ALN_TRY(ws(c, bwords, &bitmap));
   // (room after the 8 counters for the top-k result block: 2 u32 + limit rows)
-  ALN_TRY(ws(c, 8 + (8 + (uint64_t)a->limit * sizeof(aln_star_row) + 7) / 8, &ctr));
+  ALN_TRY(ws(c, NCTR + (8 + (uint64_t)a->limit * sizeof(aln_star_row) + 7) / 8, &ctr));
   dup = reinterpret_cast<uint32_t *>(ctr + 6);
   const uint64_t dregions = dblocks * (ST_BLOCK / 64);
   ALN_TRY(ws(c, dregions, &dcnt));
@@ -2213,12 +2270,17 @@ extern "C" int aln_star_agg(aln_ctx *c, const aln_star_args *a, aln_star_row *h_
     const PlanJob jm{{-1, a->m_fk, a->m_key, a->m_g1, a->m_g2}, MID_STAGE, dkmin, dfast, (uint32_t)PL_OWN, ms, mplan};
     const PlanJob jf{{-1, a->f_fk, a->f_x, a->f_y, -1}, FACT_STAGE, mkmin, 1, (uint32_t)FACT_OWN, fs, fplan};
     const FillJob fj{{bitmap, reinterpret_cast<uint32_t *>(ctr), mbits, nullptr},
-                     {bwords, 16, use_bm ? mblk * BM_BLOCK_WORDS : 0, 0}};
+                     {bwords, 2 * NCTR, use_bm ? mblk * BM_BLOCK_WORDS : 0, 0}};
     const uint64_t most = std::max<uint64_t>(bwords, use_bm ? mblk * BM_BLOCK_WORDS : 0);
     const uint32_t pb = (ds + ms + fs + 127) / 128, fb = (uint32_t)std::min<uint64_t>((most + 1023) / 1024, 16384);
     hipLaunchKernelGGL(k_star_setup, dim3(pb + (fb ? fb : 1)), dim3(128), 0, c->stream, dd, md, fd, jd, jm, jf, pb,
                        fj);
     ALN_TRY(check_launch("star_setup"));
+  }
+  if (spec) {  // the reused ranges against the fresh ones (counter 8, zeroed by the setup)
+    hipLaunchKernelGGL(k_range_check, dim3(1), dim3(64), 0, c->stream, (const int64_t *)dfresh, want, nused,
+                       reinterpret_cast<unsigned long long *>(ctr + 8));
+    ALN_TRY(check_launch("range_check"));
   }
 
   tm.mark(0);
@@ -2237,7 +2299,7 @@ extern "C" int aln_star_agg(aln_ctx *c, const aln_star_args *a, aln_star_row *h_
   {
     ProfScope p(c, "star_mid");
     hipLaunchKernelGGL(k_star_mid, dim3((unsigned)mblocks), dim3(ST_BLOCK), 0, c->stream, md, mplan, a->m_key,
-                       a->m_fk, bitmap, dkmin, drange, mcnt, mprobe, rent, mbits, mkmin);
+                       a->m_fk, bitmap, dkmin, drange, mcnt, mprobe, rent, mbits, mkmin, mrange);
   }
   ALN_TRY(check_launch("star_mid"));
   // bitmap join: the rank directory now (its total = distinct passing keys), before the
@@ -2264,8 +2326,12 @@ extern "C" int aln_star_agg(aln_ctx *c, const aln_star_args *a, aln_star_row *h_
     uint64_t *out[5] = {ctr + 0, ctr + 1, ctr + 2, ctr + 7, ctr + 5};
     ALN_TRY(sum_u32_multi(c, use_bm ? 5 : 4, in, n, out, 1u << 3));
   }
-  uint64_t hc[8];
+  uint64_t hc[NCTR];
   ALN_TRY(copy_to_host(c, hc, ctr, sizeof(hc)));
+  if (hc[8]) {  // the columns changed since their ranges were taken: run again, reading them
+    R.valid = false;
+    return aln_star_agg(c, a, h_top, h_ntop, d_groups, groups_cap, h_stats);
+  }
   const uint32_t *hdup = reinterpret_cast<const uint32_t *>(hc + 6);
   distinct = hc[5];
   if (hc[7] != hc[0]) return set_error(ALN_E_UNSUPPORTED, "aln_star_agg: duplicate dimension keys");
@@ -2341,7 +2407,7 @@ extern "C" int aln_star_agg(aln_ctx *c, const aln_star_args *a, aln_star_row *h_
   // top-k
   bool ctr_read = false;
   if (a->limit && m)
-    ALN_TRY(star_topk(c, m, sum, cnt, ga, a->limit, g1min, g1max, mkmin, h_top, h_ntop, ctr, 8, hc, &ctr_read,
+    ALN_TRY(star_topk(c, m, sum, cnt, ga, a->limit, g1min, g1max, mkmin, h_top, h_ntop, ctr, NCTR, hc, &ctr_read,
                       held, use_bm ? &rows : nullptr));
   tm.mark(4);
   if (!ctr_read) ALN_TRY(copy_to_host(c, hc, ctr, sizeof(hc)));

@@ -354,7 +354,11 @@ typedef struct {
  * mid key range is at most 2^31 (order-preserving: a group's index is the rank of its key),
  * else into a MurmurHash64A open-addressing table.  Duplicate keys on either build side
  * return ALN_E_UNSUPPORTED (the host layer then runs the unfused operators).
- * Groups (if flags & 1) are written to d_groups (cap rows) in ascending key order. */
+ * Groups (if flags & 1) are written to d_groups (cap rows) in ascending key order.
+ * A call whose key columns (segment descriptor arrays and counts) are those of the
+ * context's previous call reuses their zone-map key ranges instead of reading them back;
+ * the device checks them against the zone maps and, if the columns changed in place, the
+ * call runs again with fresh ranges (same answer, one more pass). */
 int aln_star_agg(aln_ctx *ctx, const aln_star_args *args, aln_star_row *h_top, uint32_t *h_ntop,
                  aln_star_row *d_groups, uint64_t groups_cap, aln_star_stats *h_stats);
 

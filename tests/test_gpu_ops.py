@@ -645,6 +645,46 @@ def test_star_topk_ties(ctx, case, join):
             assert np.array_equal(host(g), groups)
 
 
+@pytest.mark.parametrize("join", ["bitmap", "hash"])
+def test_star_ranges_reused_then_changed(ctx, join):
+    """A call on the same key columns as the last one reuses their key ranges and checks them
+    on the device.  Here the dimension key and the orders' foreign key are rewritten in place
+    (same buffers and segment count, keys moved up by 100000, new zone maps): the reused
+    ranges are stale (with them no key would fall in the dimension bitmap), so the query must
+    run again with fresh ranges and give the oracle's answer for the new contents."""
+    import ctypes
+
+    import torch
+    from alenka_amd import device as D
+
+    t = Q3.tables(1234, 3000, 30000)
+    seg_rows = 4096
+    t2 = dict(t)
+    t2["c_key"] = t["c_key"] + 100000
+    t2["o_cust"] = t["o_cust"] + 100000
+    enc = lambda a, d=0, fb=0: D.Column.encode(ctx, dev(ctx, a), seg_rows, decimals=d, force_bits=fb)
+    dim = [enc(t["c_key"], fb=20), enc(t["c_seg"])]
+    mid = [enc(t["o_key"]), enc(t["o_cust"], fb=20), enc(t["o_date"]), enc(t["o_prio"])]
+    fact = [enc(t["l_okey"]), enc(t["l_price"], 2), enc(t["l_disc"], 2), enc(t["l_ship"])]
+    D_, M_, F_, spec = q3_args(t, dim, mid, fact)
+    force = join == "hash"
+    for tab in (t, t2):
+        if tab is t2:  # the new contents into the same buffers
+            for old, new in ((dim[0], enc(t2["c_key"], fb=20)), (mid[1], enc(t2["o_cust"], fb=20))):
+                assert old.words.numel() == new.words.numel() and old.nsegs == new.nsegs
+                old.words.copy_(new.words)
+                old.dsegs.copy_(new.dsegs)
+                ctypes.memmove(old.hsegs, new.hsegs, ctypes.sizeof(old.hsegs))
+            torch.cuda.synchronize()
+        top, groups, stats = Q3.run(tab)
+        for _ in range(2):  # the first call after the change, and one reusing its ranges
+            rows, g, st = D.star_agg(ctx, D_, M_, F_, spec, limit=10, want_groups=True, force_hash=force)
+            assert rows == top
+            assert (st.dim_pass, st.mid_pass, st.fact_pass) == (stats["dim_pass"], stats["mid_pass"],
+                                                                 stats["fact_pass"])
+            assert np.array_equal(host(g), groups)
+
+
 @pytest.mark.parametrize("scale,seg_rows", [(0.02, 4096), (0.1, 65536)])
 def test_star_q3_batch(ctx, scale, seg_rows):
     """q3 with different substitution parameters (segment, date) answered by one pass of each
