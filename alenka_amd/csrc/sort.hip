@@ -296,7 +296,6 @@ __global__ void __launch_bounds__(RS_BLOCK) k_rs_scatter(RsPass a) {
   uint32_t val[RS_ITEMS], rank[RS_ITEMS];
   rs_load<KI, RS_ITEMS, true>(a.in, n, base + lane, 64, key, val);
   __syncthreads();
-  const uint64_t lt = lanemask_lt();
   // Lanes of one wave row with equal digits: each lane ORs its bit into the row's mask of
   // its digit and reads the mask back (a wave's LDS operations complete in order); the
   // lowest such lane then advances the digit's count and clears the mask.
@@ -308,8 +307,8 @@ __global__ void __launch_bounds__(RS_BLOCK) k_rs_scatter(RsPass a) {
     if (valid) __hip_atomic_fetch_or(slot, 1ull << lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     uint64_t peers = valid ? __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) : 0;
     uint32_t before = valid ? wc[wid][d] : 0;
-    uint64_t below = peers & lt;
-    rank[j] = before + (uint32_t)__popcll(below);
+    const uint32_t below = below_count(peers);  // peers in lower lanes
+    rank[j] = before + below;
     if (valid && below == 0) {
       wc[wid][d] = (uint16_t)(before + (uint32_t)__popcll(peers));
       __hip_atomic_store(slot, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -409,7 +408,6 @@ __global__ void __launch_bounds__(RS_BLOCK) __attribute__((amdgpu_waves_per_eu(6
   // each XCD takes a contiguous range of windows: neighbours share RL_W codes and one L2
   const uint32_t q = a.nblk / RS_XCDS, rm = a.nblk % RS_XCDS, x = blockIdx.x % RS_XCDS;
   const uint64_t n = a.n, w0 = (uint64_t)(x * q + min(x, rm) + blockIdx.x / RS_XCDS) * RL_W;
-  const uint64_t lt = lanemask_lt();
   uint64_t key[RS_ITEMS];
   uint32_t val[RS_ITEMS], rank[RS_ITEMS];
   if (t == 0) {
@@ -518,8 +516,8 @@ __global__ void __launch_bounds__(RS_BLOCK) __attribute__((amdgpu_waves_per_eu(6
       if (valid) __hip_atomic_fetch_or(slot, 1ull << lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
       const uint64_t peers = valid ? __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) : 0;
       const uint32_t before = valid ? wc[wid][d] : 0;
-      const uint64_t below = peers & lt;
-      rank[j] = before + (uint32_t)__popcll(below);
+      const uint32_t below = below_count(peers);  // peers in lower lanes
+      rank[j] = before + below;
       if (valid && below == 0) {
         wc[wid][d] = (uint16_t)(before + (uint32_t)__popcll(peers));
         __hip_atomic_store(slot, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -540,7 +538,8 @@ __global__ void __launch_bounds__(RS_BLOCK) __attribute__((amdgpu_waves_per_eu(6
     __syncthreads();
     if (t < RS_BINS) {
       uint32_t tstart = incl - cnt;
-      for (unsigned w = 0; w < wid; ++w) tstart += wsum[w];
+#pragma unroll
+      for (int w = 0; w < RS_WAVES; ++w) tstart += w < (int)wid ? wsum[w] : 0u;
 #pragma unroll
       for (int w = 0; w < RS_WAVES; ++w) wc[w][t] = (uint16_t)(wc[w][t] + tstart);
     }
