@@ -666,7 +666,8 @@ __device__ __forceinline__ uint32_t wave_minmax_u32(uint32_t v) {
   return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
 }
 
-__device__ __forceinline__ void mid_bits_window(uint32_t *win, const uint32_t *code, uint32_t *__restrict__ mbits) {
+__device__ __forceinline__ void mid_bits_window(uint32_t *win, const uint32_t *code, uint32_t *__restrict__ mbits,
+                                                uint2 *__restrict__ trange) {
   __shared__ uint32_t s_lo, s_hi;
   if (threadIdx.x == 0) {
     s_lo = 0xFFFFFFFFu;
@@ -689,6 +690,7 @@ __device__ __forceinline__ void mid_bits_window(uint32_t *win, const uint32_t *c
   }
   __syncthreads();
   const uint32_t wlo = s_lo, whi = s_hi;
+  if (threadIdx.x == 0) trange[blockIdx.x] = make_uint2(wlo, whi);  // (empty: wlo > whi)
   if (wlo > whi) return;  // no passing row in the block (uniform)
   const uint32_t span = whi - wlo + 1 < (uint32_t)PL_STAGE_WORDS ? whi - wlo + 1 : (uint32_t)PL_STAGE_WORDS;
   for (uint32_t w = threadIdx.x; w < span; w += ST_BLOCK) win[w] = 0;
@@ -722,7 +724,7 @@ __global__ void __launch_bounds__(ST_BLOCK, 8) k_star_mid(ScanDesc d, const SegP
                                                        uint32_t *__restrict__ counts,
                                                        uint32_t *__restrict__ probed, uint2 *__restrict__ rent,
                                                        uint32_t *__restrict__ mbits, int64_t mkmin,
-                                                       uint64_t mrange) {
+                                                       uint64_t mrange, uint2 *__restrict__ trange) {
   static_assert(sizeof(TileStage) <= PL_STAGE_WORDS * sizeof(uint32_t), "generic staging fits the LDS area");
   __shared__ __align__(16) uint32_t lds[PL_STAGE_WORDS];
   __shared__ int want[ALN_MAX_RANGES + 1];
@@ -738,6 +740,7 @@ __global__ void __launch_bounds__(ST_BLOCK, 8) k_star_mid(ScanDesc d, const SegP
       counts[region] = 0;
       probed[region] = 0;
     }
+    if (mbits && threadIdx.x == 0) trange[blockIdx.x] = make_uint2(0xFFFFFFFFu, 0u);  // no passing row
     return;
   }
   const uint32_t nt = nrows - r0 < (uint32_t)ST_TILE ? nrows - r0 : (uint32_t)ST_TILE;
@@ -822,7 +825,7 @@ __global__ void __launch_bounds__(ST_BLOCK, 8) k_star_mid(ScanDesc d, const SegP
     counts[region] = nout;
     probed[region] = nprobe;
   }
-  if (mbits) mid_bits_window(lds, mcode, mbits);
+  if (mbits) mid_bits_window(lds, mcode, mbits, trange);
 }
 
 __device__ __forceinline__ uint64_t st_slot(uint32_t code, uint64_t cap) {
@@ -1042,6 +1045,7 @@ struct MidRows {
   const uint32_t *bits;    // mid bitmap, its rank9 directory and its block count
   const uint64_t *rank, *rel;
   uint64_t nblk;
+  const uint2 *trange;     // per mid tile: the bitmap words its passing keys span (or null)
   uint32_t *grow;          // out: the mid row of group g
 };
 
@@ -1068,13 +1072,16 @@ constexpr int TR_Q = 8;
 __global__ void __launch_bounds__(ST_BLOCK) k_topk_rows(MidRows M, const uint32_t *__restrict__ count, uint32_t cap,
                                                        const uint32_t *__restrict__ cj,
                                                        const uint32_t *__restrict__ ccode, int full) {
-  __shared__ uint32_t hk[TR_HASH], hg[TR_HASH];
+  __shared__ uint32_t hk[TR_HASH], hg[TR_HASH], cc[TR_MAX];
   const uint32_t n = full ? 0u : (*count < cap ? *count : cap);
   const bool all = full || n > TR_MAX;
   constexpr int R = ST_BLOCK / 64;
   const unsigned lane = __lane_id(), wid = threadIdx.x / 64;
   const uint64_t tile = (uint64_t)blockIdx.x * R + wid;
-  const bool mine = tile < M.ntiles;
+  bool mine = tile < M.ntiles;
+  // candidate mode with tile ranges: a tile whose passing keys' words hold no candidate's
+  // key is skipped (a table stored in key order leaves a few tiles to read)
+  const uint2 tr = mine && !all && M.trange ? M.trange[tile] : make_uint2(0u, 0xFFFFFFFFu);
   uint32_t off[R + 1];
   off[0] = 0;
 #pragma unroll
@@ -1092,12 +1099,13 @@ __global__ void __launch_bounds__(ST_BLOCK) k_topk_rows(MidRows M, const uint32_
       if (idx < total) e[q] = M.rent[(tile * R + r) * PL_WAVE_ROWS + (idx - off[r])];
     }
   };
-  load(0);
+  if (all) load(0);  // (candidate mode: after the tile test)
   if (!all) {
     for (uint32_t q = threadIdx.x; q < TR_HASH; q += ST_BLOCK) hk[q] = 0xFFFFFFFFu;
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < n; i += ST_BLOCK) {  // n <= TR_MAX: a free slot exists
       const uint32_t g = cj[i], code = ccode[i];
+      cc[i] = code;
       uint32_t q = (code * 0x9E3779B1u) >> (32 - TR_BITS);
       while (true) {
         const uint32_t prev = atomicCAS(&hk[q], 0xFFFFFFFFu, code);
@@ -1110,6 +1118,12 @@ __global__ void __launch_bounds__(ST_BLOCK) k_topk_rows(MidRows M, const uint32_
       }
     }
     __syncthreads();
+    bool any = false;
+    if (mine && tr.x <= tr.y)
+      for (uint32_t i = lane; i < n; i += 64) any |= (cc[i] >> 5) - tr.x <= tr.y - tr.x;
+    mine = mine && __builtin_amdgcn_ballot_w64(any) != 0;
+    if (!mine) return;  // (wave-uniform; no barrier follows)
+    load(0);
   }
   for (uint32_t base = 0; base < total; base += 64 * TR_Q) {
     if (base > 0) load(base);
@@ -2294,12 +2308,13 @@ extern "C" int aln_star_agg(aln_ctx *c, const aln_star_args *a, aln_star_row *h_
 
   // mid scan: passing rows of wave region b land in entries [b * PL_WAVE_ROWS, ...) of the
   // region arrays
-  uint2 *rent;
+  uint2 *rent, *trange = nullptr;
   ALN_TRY(ws(c, mregions * PL_WAVE_ROWS, &rent));
+  if (use_bm) ALN_TRY(ws(c, mblocks, &trange));
   {
     ProfScope p(c, "star_mid");
     hipLaunchKernelGGL(k_star_mid, dim3((unsigned)mblocks), dim3(ST_BLOCK), 0, c->stream, md, mplan, a->m_key,
-                       a->m_fk, bitmap, dkmin, drange, mcnt, mprobe, rent, mbits, mkmin, mrange);
+                       a->m_fk, bitmap, dkmin, drange, mcnt, mprobe, rent, mbits, mkmin, mrange, trange);
   }
   ALN_TRY(check_launch("star_mid"));
   // bitmap join: the rank directory now (its total = distinct passing keys), before the
@@ -2350,7 +2365,7 @@ extern "C" int aln_star_agg(aln_ctx *c, const aln_star_args *a, aln_star_row *h_
   const GroupAttr ga{nullptr, nullptr, nullptr, grow, mplan, a->mid.seg_rows};
   uint64_t *table = nullptr;
   uint64_t tcap = 0;
-  const MidRows rows{rent, mcnt, mblocks, mbits, rank, rel, mblk, grow};
+  const MidRows rows{rent, mcnt, mblocks, mbits, rank, rel, mblk, trange, grow};
   if (use_bm) {
     // groups are ranks (= key order): zeroed accumulators now; the mid row of a group is
     // looked up later, for the groups the top-k or the group list needs

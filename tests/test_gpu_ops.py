@@ -614,6 +614,28 @@ def test_star_scan_shapes(ctx, case, join):
 
 
 @pytest.mark.parametrize("join", ["bitmap", "hash"])
+def test_star_q3_orders_shuffled(ctx, join):
+    """The orders table's rows in random order (the answer does not depend on it): every
+    mid tile's passing keys then span most of the key range, so the top-k reads the rows of
+    its candidate groups from every tile instead of the few a key-ordered table leaves."""
+    from alenka_amd import device as D
+
+    t = Q3.tables(4321, 3000, 30000)
+    perm = np.random.default_rng(5).permutation(len(t["o_key"]))
+    t2 = dict(t)
+    for k in ("o_key", "o_cust", "o_date", "o_prio"):
+        t2[k] = t[k][perm]
+    top, groups, stats = Q3.run(t)
+    assert Q3.run(t2)[0] == top
+    dim, mid, fact = encode_q3(ctx, t2, 4096)
+    D_, M_, F_, spec = q3_args(t2, dim, mid, fact)
+    rows, g, st = D.star_agg(ctx, D_, M_, F_, spec, limit=10, want_groups=True, force_hash=join == "hash")
+    assert rows == top
+    assert (st.dim_pass, st.mid_pass, st.fact_pass) == (stats["dim_pass"], stats["mid_pass"], stats["fact_pass"])
+    assert np.array_equal(host(g), groups)
+
+
+@pytest.mark.parametrize("join", ["bitmap", "hash"])
 @pytest.mark.parametrize("case", ["dense", "filtered"])
 def test_star_topk_ties(ctx, case, join):
     """SUM(1) per joined row: a group's sum is its row count (1 to 7), so many groups tie at
