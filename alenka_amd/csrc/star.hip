@@ -1153,6 +1153,19 @@ __global__ void __launch_bounds__(ST_BLOCK) k_topk_rows(MidRows M, const uint32_
   }
 }
 
+// SUM and COUNT of groups 0 .. *d_m - 1 and their bits in held zeroed (the arrays are
+// allocated for more groups).
+__global__ void __launch_bounds__(256) k_zero_groups(unsigned long long *__restrict__ sum, unsigned int *__restrict__ cnt,
+                                                     uint32_t *__restrict__ held,
+                                                     const unsigned long long *__restrict__ d_m) {
+  const uint64_t m = *d_m, stride = (uint64_t)gridDim.x * 256;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < m; i += stride) {
+    sum[i] = 0;
+    cnt[i] = 0;
+    if (i < (m + 31) / 32) held[i] = 0;
+  }
+}
+
 // Plan roles: 0 range, 1 foreign key (probes the mid bitmap), 2 and 3 the aggregate's x
 // and y, read from global memory for the few rows that join.  Counts are per wave
 // (index b * 8 + w), so the block needs one barrier, after staging.
@@ -1874,10 +1887,13 @@ __global__ void __launch_bounds__(TB_BLOCK) k_topk_collect(uint64_t n, const uns
 constexpr uint32_t TL_BLOCKS = 1024;
 // The same two passes over the bit set of the groups that hold rows (one bit per group,
 // ~5 % set): a thread per 32 groups reads one word and the sums of its set bits only.
+// d_m (or null): the number of groups on the device, which bounds nw (allocated for more)
 __global__ void __launch_bounds__(TB_BLOCK) k_topk_bmax_held(const uint32_t *__restrict__ held, uint64_t nw,
                                                              const unsigned long long *__restrict__ sum,
-                                                             uint64_t *__restrict__ bmax) {
+                                                             uint64_t *__restrict__ bmax,
+                                                             const unsigned long long *__restrict__ d_m) {
   __shared__ uint64_t wm[TB_BLOCK / 64];
+  if (d_m) nw = (*d_m + 31) / 32 < nw ? (*d_m + 31) / 32 : nw;
   uint64_t m = 0;
   for (uint64_t w = (uint64_t)blockIdx.x * TB_BLOCK + threadIdx.x; w < nw; w += (uint64_t)gridDim.x * TB_BLOCK) {
     uint32_t b = held[w];
@@ -1909,8 +1925,10 @@ __global__ void __launch_bounds__(TB_BLOCK) k_topk_collect_held(const uint32_t *
                                                                 uint32_t *__restrict__ count,
                                                                 uint64_t *__restrict__ chi,
                                                                 uint64_t *__restrict__ clo,
-                                                                uint32_t *__restrict__ cj, int attr) {
+                                                                uint32_t *__restrict__ cj, int attr,
+                                                                const unsigned long long *__restrict__ d_m) {
   const uint64_t t = *thr;
+  if (d_m) nw = (*d_m + 31) / 32 < nw ? (*d_m + 31) / 32 : nw;
   for (uint64_t w = (uint64_t)blockIdx.x * TB_BLOCK + threadIdx.x; w < nw; w += (uint64_t)gridDim.x * TB_BLOCK) {
     uint32_t b = held[w];
     while (b) {
@@ -1974,6 +1992,8 @@ static int mid_rows_all(aln_ctx *c, const MidRows &rows) {
   return check_launch("topk_rows");
 }
 
+// d_m (with held): m only bounds the groups, whose number is *d_m on the device (the
+// threshold path must be taken then).
 // mrows != nullptr: ga reads the groups' mid rows from mrows->grow, which holds nothing yet;
 // the rows of the groups that need them are found here (only the candidates', when the
 // threshold path can take them).
@@ -1981,7 +2001,7 @@ static int star_topk(aln_ctx *c, uint64_t m, const unsigned long long *sum, cons
                      const GroupAttr &ga, uint32_t k, int64_t g1min, int64_t g1max, int64_t kmin,
                      aln_star_row *h_top, uint32_t *h_ntop, uint64_t *ctr = nullptr, uint32_t nctr = 0,
                      uint64_t *h_ctr = nullptr, bool *ctr_read = nullptr, const uint32_t *held = nullptr,
-                     const MidRows *mrows = nullptr) {
+                     const MidRows *mrows = nullptr, const unsigned long long *d_m = nullptr) {
   const bool thr_path = k <= (uint32_t)TS_K && g1max >= g1min && (uint64_t)g1max - (uint64_t)g1min < (1ull << 32) &&
                         m < (1ull << 32);
   const bool lazy = mrows && held && thr_path;  // (with held, the path below has one result block)
@@ -2018,13 +2038,13 @@ static int star_topk(aln_ctx *c, uint64_t m, const unsigned long long *sum, cons
           ProfScope p(c, "star_topk");
           if (held)
             hipLaunchKernelGGL(k_topk_bmax_held, dim3((unsigned)nb), dim3(TB_BLOCK), 0, c->stream, held, hw, sum,
-                               bmax);
+                               bmax, d_m);
           else
             hipLaunchKernelGGL(k_topk_bmax, dim3((unsigned)nb), dim3(TB_BLOCK), 0, c->stream, m, sum, cnt, bmax);
           hipLaunchKernelGGL(k_topk_kth, dim3(1), dim3(TB_BLOCK), 0, c->stream, bmax, (uint32_t)nb, k, thr);
           if (held)
             hipLaunchKernelGGL(k_topk_collect_held, dim3((unsigned)nb), dim3(TB_BLOCK), 0, c->stream, held, hw, sum, ga,
-                               g1min, kmin, thr, CAP, count, chi, clo, cj, lazy ? 0 : 1);
+                               g1min, kmin, thr, CAP, count, chi, clo, cj, lazy ? 0 : 1, d_m);
           else
             hipLaunchKernelGGL(k_topk_collect, dim3((unsigned)nb), dim3(TB_BLOCK), 0, c->stream, m, sum, cnt, ga,
                                g1min, kmin, thr, CAP, count, chi, clo, cj);
@@ -2060,7 +2080,12 @@ static int star_topk(aln_ctx *c, uint64_t m, const unsigned long long *sum, cons
           return ALN_OK;
         }
         // more groups tie at the threshold than the buffer holds: the passes below, which
-        // read every group's attributes
+        // read every group's attributes (and, with d_m, need the exact group count: the
+        // counters came back with the result)
+        if (d_m) {
+          ALN_REQUIRE(ctr, "star_topk: a device group count needs the counters");
+          m = h_ctr[5];
+        }
         if (lazy) ALN_TRY(mid_rows_all(c, *mrows));
       }
     }
@@ -2318,10 +2343,9 @@ extern "C" int aln_star_agg(aln_ctx *c, const aln_star_args *a, aln_star_row *h_
   }
   ALN_TRY(check_launch("star_mid"));
   // bitmap join: the rank directory now (its total = distinct passing keys), before the
-  // one synchronisation that reads the counters
+  // counters are summed
   uint64_t *rank = nullptr, *rel = nullptr;
   uint32_t *bcnt = nullptr;
-  uint64_t distinct = 0;
   if (use_bm) {
     ALN_TRY(ws(c, mblk, &bcnt));
     ALN_TRY(ws(c, mblk, &rank));
@@ -2341,21 +2365,37 @@ extern "C" int aln_star_agg(aln_ctx *c, const aln_star_args *a, aln_star_row *h_
     uint64_t *out[5] = {ctr + 0, ctr + 1, ctr + 2, ctr + 7, ctr + 5};
     ALN_TRY(sum_u32_multi(c, use_bm ? 5 : 4, in, n, out, 1u << 3));
   }
+  // No synchronisation here on the common path (bitmap join, threshold top-k, no group
+  // list): the group arrays are allocated for an upper bound of the group count (distinct
+  // passing mid keys <= mid rows, <= key range), zeroed up to the device's count of distinct
+  // passing keys (counter 5), and the counters are checked after the final readback.
+  const uint64_t m_ub = std::min<uint64_t>((uint64_t)a->mid.cols[0].nsegs * a->mid.seg_rows, mrange);
+  const bool nosync = use_bm && !(a->flags & 1) && a->limit > 0 && a->limit <= (uint32_t)TS_K &&
+                      g1max >= g1min && (uint64_t)g1max - (uint64_t)g1min < (1ull << 32) &&
+                      m_ub <= (1ull << 28);  // (16 B of group arrays per possible group)
   uint64_t hc[NCTR];
-  ALN_TRY(copy_to_host(c, hc, ctr, sizeof(hc)));
-  if (hc[8]) {  // the columns changed since their ranges were taken: run again, reading them
-    R.valid = false;
-    return aln_star_agg(c, a, h_top, h_ntop, d_groups, groups_cap, h_stats);
+  auto check_counters = [&]() -> int {  // after a readback of the counters into hc
+    const uint32_t *hdup = reinterpret_cast<const uint32_t *>(hc + 6);
+    if (hc[7] != hc[0]) return set_error(ALN_E_UNSUPPORTED, "aln_star_agg: duplicate dimension keys");
+    if (hdup[1] || (use_bm && hc[5] != hc[1]))
+      return set_error(ALN_E_UNSUPPORTED, "aln_star_agg: duplicate mid keys");
+    st.dim_pass = hc[0];
+    st.mid_pass = hc[1];
+    st.mid_probe = hc[2];
+    return ALN_OK;
+  };
+  uint64_t m;
+  if (nosync) {
+    m = m_ub;
+  } else {
+    ALN_TRY(copy_to_host(c, hc, ctr, sizeof(hc)));
+    if (hc[8]) {  // the columns changed since their ranges were taken: run again, reading them
+      R.valid = false;
+      return aln_star_agg(c, a, h_top, h_ntop, d_groups, groups_cap, h_stats);
+    }
+    ALN_TRY(check_counters());
+    m = hc[1];
   }
-  const uint32_t *hdup = reinterpret_cast<const uint32_t *>(hc + 6);
-  distinct = hc[5];
-  if (hc[7] != hc[0]) return set_error(ALN_E_UNSUPPORTED, "aln_star_agg: duplicate dimension keys");
-  if (hdup[1] || (use_bm && distinct != hc[1]))
-    return set_error(ALN_E_UNSUPPORTED, "aln_star_agg: duplicate mid keys");
-  st.dim_pass = hc[0];
-  const uint64_t m = hc[1];
-  st.mid_pass = m;
-  st.mid_probe = hc[2];
 
   unsigned long long *sum;
   unsigned int *cnt, *grow;
@@ -2369,10 +2409,18 @@ extern "C" int aln_star_agg(aln_ctx *c, const aln_star_args *a, aln_star_row *h_
   if (use_bm) {
     // groups are ranks (= key order): zeroed accumulators now; the mid row of a group is
     // looked up later, for the groups the top-k or the group list needs
+    ALN_TRY(ws(c, (m + 31) / 32, &held));  // bit set of the groups that hold rows
     if (m) {
       ProfScope p(c, "star_build");
-      ALN_TRY(fill_u64(c, reinterpret_cast<uint64_t *>(sum), m, 0));
-      ALN_TRY(fill_u32(c, cnt, m, 0));
+      if (nosync) {
+        hipLaunchKernelGGL(k_zero_groups, dim3(grid_for(m, 256, 8192)), dim3(256), 0, c->stream, sum, cnt, held,
+                           reinterpret_cast<const unsigned long long *>(ctr + 5));
+        ALN_TRY(check_launch("zero_groups"));
+      } else {
+        ALN_TRY(fill_u64(c, reinterpret_cast<uint64_t *>(sum), m, 0));
+        ALN_TRY(fill_u32(c, cnt, m, 0));
+        ALN_TRY(fill_u32(c, held, (m + 31) / 32, 0));
+      }
     }
   } else {
     uint64_t *moff;
@@ -2395,8 +2443,6 @@ extern "C" int aln_star_agg(aln_ctx *c, const aln_star_args *a, aln_star_row *h_
       if (use_bm) {
         uint2 *hitbuf;
         ALN_TRY(ws(c, fwaves * FACT_HIT_CAP, &hitbuf));
-        ALN_TRY(ws(c, (m + 31) / 32, &held));
-        ALN_TRY(fill_u32(c, held, (m + 31) / 32, 0));
         const FactArgs fa{a->f_fk, a->a0, a->a1, a->b0, a->b1, mbits, rank, rel, mkmin, mrange, sum, cnt,
                           fmatch, fprobe, hitbuf, held};
         hipLaunchKernelGGL(k_star_fact_bm, dim3((unsigned)fblocks), dim3(ST_BLOCK), 0, c->stream, fd, fplan, fa);
@@ -2423,10 +2469,18 @@ extern "C" int aln_star_agg(aln_ctx *c, const aln_star_args *a, aln_star_row *h_
   bool ctr_read = false;
   if (a->limit && m)
     ALN_TRY(star_topk(c, m, sum, cnt, ga, a->limit, g1min, g1max, mkmin, h_top, h_ntop, ctr, NCTR, hc, &ctr_read,
-                      held, use_bm ? &rows : nullptr));
+                      held, use_bm ? &rows : nullptr,
+                      nosync ? reinterpret_cast<const unsigned long long *>(ctr + 5) : nullptr));
   tm.mark(4);
   if (!ctr_read) ALN_TRY(copy_to_host(c, hc, ctr, sizeof(hc)));
-  if (hdup[1]) return set_error(ALN_E_UNSUPPORTED, "aln_star_agg: duplicate mid keys");
+  if (nosync) {
+    if (hc[8]) {  // the columns changed since their ranges were taken: run again, reading them
+      R.valid = false;
+      return aln_star_agg(c, a, h_top, h_ntop, d_groups, groups_cap, h_stats);
+    }
+    ALN_TRY(check_counters());
+  }
+  if (reinterpret_cast<const uint32_t *>(hc + 6)[1]) return set_error(ALN_E_UNSUPPORTED, "aln_star_agg: duplicate mid keys");
   st.fact_pass = hc[3];
   st.fact_probe = hc[4];
 
