@@ -9,7 +9,11 @@
 //   * groups are dense over [min(key)/div, max(key)/div] (the zone maps give the range), so
 //     a group's slot is its index and output is in key order, as one-segment GROUP BY;
 //   * the first `hot` groups of every query (where skewed keys concentrate) are aggregated
-//     in LDS by each block and flushed once at the end; the others use global atomics.
+//     in LDS and flushed once at the end; the others use global atomics.  The queries are
+//     split into groups whose LDS tables fit one block, and each block evaluates one query
+//     group over its share of the tiles (each group re-reads the table: the scan is cheap
+//     next to the aggregation, and larger tables keep more adds out of global atomics);
+//   * when every query sums the same expression, it is computed once per row.
 // A prepared batch owns its tables; aln_batch_run replays the launch sequence (table
 // fills + scan) from a captured HIP graph when asked to; aln_batch_autotune times the
 // grid / hot-set shapes on the real data and returns the fastest.
@@ -24,10 +28,24 @@
 namespace aln {
 
 constexpr int BQ_MAX = ALN_BATCH_MAX;
-constexpr int BQ_HOT_MAX = 32;
+// LDS aggregation: the first `hot` groups of each query (at most BQ_HOT_MAX) have a slot
+// (8-byte sum, 4-byte count) in the block's LDS; the queries are split into groups of `qg`
+// whose tables fit BQ_HOT_BYTES together, and each block evaluates one query group.
+constexpr int BQ_HOT_MAX = 4096;
+constexpr int BQ_HOT_BYTES = 48 * 1024;
+constexpr int BQ_HOT_DEFAULT = 256;
+constexpr int BQ_BLOCK = 512;
+constexpr int BQ_ITEMS = STG_TILE / BQ_BLOCK;  // rows per thread of a 4096-row tile
 
 constexpr int BQ_SLOTS = 4;  // distinct columns a batch's queries read (else the general loop)
 constexpr int BQ_CHUNK = 4;  // rows per thread decoded together
+
+// A group's accumulators: its sum and count share one 16-byte slot, so a row's two
+// atomics touch one cache line of the group table.
+struct __align__(16) BSlot {
+  unsigned long long sum;
+  unsigned int cnt, pad;
+};
 
 struct BQuery {
   int32_t pred_col, key_col, x_col, y_col;
@@ -39,16 +57,17 @@ struct BQuery {
   uint64_t magic;    // ceil(2^64 / div) for div >= 2: (key - base) / div of a 32-bit difference
   uint64_t ngroups;  // dense group range
   int64_t a0, a1, b0, b1;
-  unsigned long long *sum;  // [ngroups]
-  unsigned int *cnt;        // [ngroups]
+  BSlot *acc;  // [ngroups]
 };
 
 struct BArgs {
   const uint32_t *words[ALN_MAX_COLS];
   const aln_seg *segs[ALN_MAX_COLS];
-  const BQuery *q;  // device [nq]
+  const BQuery *q;  // device [nq] (the kernel reads it through a restrict parameter)
   int nq;
   int hot;
+  int qg, ngq;  // queries per query group, query groups (block b evaluates group b % ngq)
+  int shared_v; // every query sums the same (a0 + a1*x)(b0 + b1*y): computed once per row
   int stage_cols[2];
   int nstage;
   // columns every query shares (-1: per query)
@@ -74,119 +93,137 @@ __device__ __forceinline__ int64_t pick(int s, int64_t v0, int64_t v1, int64_t v
   return s == 0 ? v0 : s == 1 ? v1 : s == 2 ? v2 : v3;
 }
 
-// Adds (v, 1) of one (query, row) to its group: groups below `hot` go to the block's LDS
-// copy; group 0 - where Zipf ranks concentrate - is first summed over the wave when many
-// lanes hit it, so one lane issues the atomic for all of them.
-__device__ __forceinline__ void bq_add(const BQuery &q, int qi, int hot, uint64_t g, int64_t v, bool on,
-                                       unsigned long long (*hsum)[BQ_HOT_MAX], unsigned int (*hcnt)[BQ_HOT_MAX]) {
+__device__ __forceinline__ int64_t bq_value(const BQuery &q, int64_t x, int64_t y) {
+  return wmul((int64_t)((uint64_t)q.a0 + (uint64_t)wmul(q.a1, x)),
+              (int64_t)((uint64_t)q.b0 + (uint64_t)wmul(q.b1, y)));
+}
+
+// Adds (v, 1) of one (query, row) to its group: groups below `hot` go to the query's LDS
+// table (hsum/hcnt); group 0 - where Zipf ranks concentrate - is first summed over the
+// wave when many lanes hit it, so one lane issues the atomic for all of them.
+__device__ __forceinline__ void bq_add(const BQuery &q, int hot, uint64_t g, int64_t v, bool on,
+                                       unsigned long long *hsum, unsigned int *hcnt) {
   const uint64_t z = __ballot(on && g == 0);
   if (__popcll(z) >= 8) {
     int64_t s = on && g == 0 ? v : 0;
     for (int o = 32; o > 0; o >>= 1) s = (int64_t)((uint64_t)s + (uint64_t)__shfl_xor(s, o, 64));
     if (__lane_id() == (unsigned)(__ffsll((unsigned long long)z) - 1)) {
       if (hot > 0) {
-        atomicAdd(&hsum[qi][0], (unsigned long long)s);
-        atomicAdd(&hcnt[qi][0], (unsigned int)__popcll(z));
+        atomicAdd(&hsum[0], (unsigned long long)s);
+        atomicAdd(&hcnt[0], (unsigned int)__popcll(z));
       } else if (q.ngroups) {
-        atomicAdd(&q.sum[0], (unsigned long long)s);
-        atomicAdd(&q.cnt[0], (unsigned int)__popcll(z));
+        atomicAdd(&q.acc[0].sum, (unsigned long long)s);
+        atomicAdd(&q.acc[0].cnt, (unsigned int)__popcll(z));
       }
     }
     on = on && g != 0;
   }
   if (!on) return;
   if (g < (uint64_t)hot) {
-    atomicAdd(&hsum[qi][g], (unsigned long long)v);
-    atomicAdd(&hcnt[qi][g], 1u);
+    atomicAdd(&hsum[g], (unsigned long long)v);
+    atomicAdd(&hcnt[g], 1u);
   } else if (g < q.ngroups) {
-    atomicAdd(&q.sum[g], (unsigned long long)v);
-    atomicAdd(&q.cnt[g], 1u);
+    atomicAdd(&q.acc[g].sum, (unsigned long long)v);
+    atomicAdd(&q.acc[g].cnt, 1u);
   }
 }
 
-__global__ void __launch_bounds__(STG_BLOCK) k_batch(BArgs a) {
+// Persistent grid: block b evaluates query group b % ngq over the tiles b / ngq, + the
+// group's block count, ...; its LDS tables (dynamic: qg * hot sums, then qg * hot counts)
+// are added to the global ones at the end.
+__global__ void __launch_bounds__(BQ_BLOCK, 4) k_batch(BArgs a, const BQuery *__restrict__ qv) {
   __shared__ TileStage ts;
   __shared__ int want[2];
-  __shared__ unsigned long long hsum[BQ_MAX][BQ_HOT_MAX];
-  __shared__ unsigned int hcnt[BQ_MAX][BQ_HOT_MAX];
-  for (int i = threadIdx.x; i < BQ_MAX * BQ_HOT_MAX; i += STG_BLOCK) {
-    hsum[i / BQ_HOT_MAX][i % BQ_HOT_MAX] = 0;
-    hcnt[i / BQ_HOT_MAX][i % BQ_HOT_MAX] = 0;
+  extern __shared__ __align__(16) unsigned long long bq_lds[];
+  const uint32_t ngq = (uint32_t)a.ngq;
+  const int q0 = (int)(blockIdx.x % ngq) * a.qg;
+  const int q1 = q0 + a.qg < a.nq ? q0 + a.qg : a.nq;
+  const int hot = a.hot;
+  unsigned long long *hsum = bq_lds;
+  unsigned int *hcnt = reinterpret_cast<unsigned int *>(bq_lds + (size_t)a.qg * hot);
+  for (int i = threadIdx.x; i < (q1 - q0) * hot; i += BQ_BLOCK) {
+    hsum[i] = 0;
+    hcnt[i] = 0;
   }
   if (threadIdx.x < 2) want[threadIdx.x] = a.stage_cols[threadIdx.x];
   __syncthreads();
-  for (uint64_t tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
+  const uint64_t step = gridDim.x / ngq;  // blocks of this query group (gridDim.x is a multiple of ngq)
+  for (uint64_t tile = blockIdx.x / ngq; tile < a.ntiles; tile += step) {
     const uint32_t seg = (uint32_t)(tile / a.tps), t = (uint32_t)(tile % a.tps);
     const uint32_t nrows = a.segs[0][seg].nrows, r0 = t * STG_TILE;
     if (r0 >= nrows) continue;  // uniform across the block
     const uint32_t nt = nrows - r0 < (uint32_t)STG_TILE ? nrows - r0 : (uint32_t)STG_TILE;
     __syncthreads();  // the previous tile's LDS reads are done
-    stage_tile(ts, a.words, a.segs, want, a.nstage, seg, r0, nt);
+    stage_tile<BQ_BLOCK>(ts, a.words, a.segs, want, a.nstage, seg, r0, nt);
     __syncthreads();
     if (a.nslots > 0) {
       // every distinct column decoded once per row (HBM loads of BQ_CHUNK rows in flight),
-      // then each query's parameters are read once per BQ_CHUNK rows
-      for (int ch = 0; ch < STG_ITEMS / BQ_CHUNK; ++ch) {
-        int64_t v0[BQ_CHUNK], v1[BQ_CHUNK], v2[BQ_CHUNK], v3[BQ_CHUNK];
+      // the shared value once per row, then each query's parameters read once per chunk
+      for (int ch = 0; ch < BQ_ITEMS / BQ_CHUNK; ++ch) {
+        int64_t v0[BQ_CHUNK], v1[BQ_CHUNK], v2[BQ_CHUNK], v3[BQ_CHUNK], vs[BQ_CHUNK];
         bool ok[BQ_CHUNK];
 #pragma unroll
         for (int u = 0; u < BQ_CHUNK; ++u) {
-          const uint32_t i = (ch * BQ_CHUNK + u) * STG_BLOCK + threadIdx.x;
+          const uint32_t i = (ch * BQ_CHUNK + u) * BQ_BLOCK + threadIdx.x;
           ok[u] = i < nt;
           v0[u] = ok[u] ? tile_get(ts, a.words, a.segs, a.slot_col[0], seg, r0, i) : 0;
           v1[u] = ok[u] && a.nslots > 1 ? tile_get(ts, a.words, a.segs, a.slot_col[1], seg, r0, i) : 0;
           v2[u] = ok[u] && a.nslots > 2 ? tile_get(ts, a.words, a.segs, a.slot_col[2], seg, r0, i) : 0;
           v3[u] = ok[u] && a.nslots > 3 ? tile_get(ts, a.words, a.segs, a.slot_col[3], seg, r0, i) : 0;
+          vs[u] = 0;
         }
-        for (int qi = 0; qi < a.nq; ++qi) {
-          const BQuery &q = a.q[qi];
+        if (a.shared_v) {
+          const BQuery &q = qv[0];
+#pragma unroll
+          for (int u = 0; u < BQ_CHUNK; ++u)
+            vs[u] = bq_value(q, pick(q.xs, v0[u], v1[u], v2[u], v3[u]), pick(q.ys, v0[u], v1[u], v2[u], v3[u]));
+        }
+        for (int qi = q0; qi < q1; ++qi) {
+          const BQuery &q = qv[qi];
+          unsigned long long *qs = hsum + (size_t)(qi - q0) * hot;
+          unsigned int *qc = hcnt + (size_t)(qi - q0) * hot;
 #pragma unroll
           for (int u = 0; u < BQ_CHUNK; ++u) {
             const int64_t p = pick(q.ps, v0[u], v1[u], v2[u], v3[u]);
             const bool on = ok[u] && p >= q.lo && p <= q.hi;
-            const int64_t key = pick(q.ks, v0[u], v1[u], v2[u], v3[u]);
-            const int64_t x = pick(q.xs, v0[u], v1[u], v2[u], v3[u]);
-            const int64_t y = pick(q.ys, v0[u], v1[u], v2[u], v3[u]);
-            const uint64_t g = group_of(q, key);
-            const int64_t v = wmul((int64_t)((uint64_t)q.a0 + (uint64_t)wmul(q.a1, x)),
-                                   (int64_t)((uint64_t)q.b0 + (uint64_t)wmul(q.b1, y)));
-            bq_add(q, qi, a.hot, g, v, on, hsum, hcnt);
+            const uint64_t g = group_of(q, pick(q.ks, v0[u], v1[u], v2[u], v3[u]));
+            const int64_t v = a.shared_v ? vs[u]
+                                         : bq_value(q, pick(q.xs, v0[u], v1[u], v2[u], v3[u]),
+                                                    pick(q.ys, v0[u], v1[u], v2[u], v3[u]));
+            bq_add(q, hot, g, v, on, qs, qc);
           }
         }
       }
       continue;
     }
-    for (int j = 0; j < STG_ITEMS; ++j) {
-      const uint32_t i = j * STG_BLOCK + threadIdx.x;
+    for (int j = 0; j < BQ_ITEMS; ++j) {
+      const uint32_t i = j * BQ_BLOCK + threadIdx.x;
       const bool row = i < nt;
-      for (int qi = 0; qi < a.nq; ++qi) {
-        const BQuery &q = a.q[qi];
+      for (int qi = q0; qi < q1; ++qi) {
+        const BQuery &q = qv[qi];
         bool on = false;
         uint64_t g = 0;
         int64_t v = 0;
         if (row) {
           const int64_t p = tile_get(ts, a.words, a.segs, q.pred_col, seg, r0, i);
           if (p >= q.lo && p <= q.hi) {
-            const int64_t key = tile_get(ts, a.words, a.segs, q.key_col, seg, r0, i);
-            const int64_t x = tile_get(ts, a.words, a.segs, q.x_col, seg, r0, i);
-            const int64_t y = tile_get(ts, a.words, a.segs, q.y_col, seg, r0, i);
-            g = group_of(q, key);
-            v = wmul((int64_t)((uint64_t)q.a0 + (uint64_t)wmul(q.a1, x)),
-                     (int64_t)((uint64_t)q.b0 + (uint64_t)wmul(q.b1, y)));
+            g = group_of(q, tile_get(ts, a.words, a.segs, q.key_col, seg, r0, i));
+            v = bq_value(q, tile_get(ts, a.words, a.segs, q.x_col, seg, r0, i),
+                         tile_get(ts, a.words, a.segs, q.y_col, seg, r0, i));
             on = true;
           }
         }
-        bq_add(q, qi, a.hot, g, v, on, hsum, hcnt);
+        bq_add(q, hot, g, v, on, hsum + (size_t)(qi - q0) * hot, hcnt + (size_t)(qi - q0) * hot);
       }
     }
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < a.nq * a.hot; i += STG_BLOCK) {
-    const int qi = i / a.hot, g = i % a.hot;
-    const unsigned int c = hcnt[qi][g];
-    if (c && (uint64_t)g < a.q[qi].ngroups) {
-      atomicAdd(&a.q[qi].sum[g], hsum[qi][g]);
-      atomicAdd(&a.q[qi].cnt[g], c);
+  for (int i = threadIdx.x; i < (q1 - q0) * hot; i += BQ_BLOCK) {
+    const int qi = q0 + i / hot, g = i % hot;
+    const unsigned int c = hcnt[i];
+    if (c && (uint64_t)g < qv[qi].ngroups) {
+      atomicAdd(&qv[qi].acc[g].sum, hsum[i]);
+      atomicAdd(&qv[qi].acc[g].cnt, c);
     }
   }
 }
@@ -195,25 +232,24 @@ __global__ void k_batch_clear(const BQuery *q, int nq) {
   for (int qi = 0; qi < nq; ++qi) {
     const uint64_t n = q[qi].ngroups;
     for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
-      q[qi].sum[i] = 0;
-      q[qi].cnt[i] = 0;
+      q[qi].acc[i].sum = 0;
+      q[qi].acc[i].cnt = 0;
     }
   }
 }
 
-__global__ void k_batch_flags(const unsigned int *cnt, uint64_t n, uint32_t *flag) {
+__global__ void k_batch_flags(const BSlot *acc, uint64_t n, uint32_t *flag) {
   for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256)
-    flag[i] = cnt[i] != 0;
+    flag[i] = acc[i].cnt != 0;
 }
 __global__ void k_batch_emit(const uint32_t *flag, const uint64_t *pos, uint64_t n, int64_t gbase,
-                             const unsigned long long *sum, const unsigned int *cnt, int64_t *okey,
-                             int64_t *osum, int64_t *ocnt) {
+                             const BSlot *acc, int64_t *okey, int64_t *osum, int64_t *ocnt) {
   for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256)
     if (flag[i]) {
       const uint64_t o = pos[i];
       okey[o] = gbase + (int64_t)i;
-      osum[o] = (int64_t)sum[i];
-      ocnt[o] = cnt[i];
+      osum[o] = (int64_t)acc[i].sum;
+      ocnt[o] = acc[i].cnt;
     }
 }
 
@@ -266,7 +302,8 @@ int batch_alloc(aln_batch *b, size_t bytes, void **out) {
 int batch_enqueue(aln_batch *b, hipStream_t st) {
   hipLaunchKernelGGL(k_batch_clear, dim3(grid_for(b->max_groups, 256, 8192)), dim3(256), 0, st, b->dq,
                      b->args.nq);
-  if (b->args.ntiles) hipLaunchKernelGGL(k_batch, dim3(b->blocks), dim3(STG_BLOCK), 0, st, b->args);
+  const size_t lds = (size_t)b->args.qg * b->args.hot * (sizeof(unsigned long long) + sizeof(unsigned int));
+  if (b->args.ntiles) hipLaunchKernelGGL(k_batch, dim3(b->blocks), dim3(BQ_BLOCK), lds, st, b->args, b->dq);
   return check_launch("batch_scan");
 }
 
@@ -275,9 +312,24 @@ int set_tuning(aln_batch *b, const aln_batch_tuning *tune) {
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->ctx->device) != hipSuccess || cus <= 0)
     cus = 256;
   const int bpc = tune && tune->blocks_per_cu > 0 ? tune->blocks_per_cu : 2;
-  b->blocks = (uint32_t)(bpc * cus);
-  if (b->blocks > b->args.ntiles) b->blocks = (uint32_t)(b->args.ntiles ? b->args.ntiles : 1);
-  b->args.hot = tune && tune->hot_groups >= 0 ? std::min(tune->hot_groups, BQ_HOT_MAX) : BQ_HOT_MAX;
+  const int hot = tune && tune->hot_groups >= 0 ? std::min(tune->hot_groups, BQ_HOT_MAX) : BQ_HOT_DEFAULT;
+  // query groups whose LDS tables fit BQ_HOT_BYTES; every group gets the same number of
+  // blocks (the grid is a multiple of the group count), at most one per tile
+  const int nq = b->args.nq;
+  const int qmax = hot > 0 ? std::max(1, std::min(nq, BQ_HOT_BYTES / (hot * 12))) : nq;
+  const int ngq = (nq + qmax - 1) / qmax;
+  const int qg = (nq + ngq - 1) / ngq;  // the queries spread evenly over the groups
+  uint64_t per = (uint64_t)bpc * (uint64_t)cus / (uint64_t)ngq;
+  if (per > b->args.ntiles) per = b->args.ntiles;
+  if (per < 1) per = 1;
+  b->args.hot = hot;
+  b->args.qg = qg;
+  b->args.ngq = ngq;
+  b->blocks = (uint32_t)(per * (uint64_t)ngq);
+  const size_t lds = (size_t)qg * hot * (sizeof(unsigned long long) + sizeof(unsigned int));
+  if (lds)
+    ALN_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(k_batch), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds));
   b->use_graph = tune && tune->use_graph;
   if (b->exec) {
     (void)hipGraphExecDestroy(b->exec);
@@ -309,7 +361,7 @@ size_t batch_pool_bytes(const uint64_t *ngroups, int nq) {
   size_t t = al(sizeof(BQuery) * (size_t)nq);
   for (int i = 0; i < nq; ++i) {
     const size_t g = ngroups[i] ? ngroups[i] : 1;
-    t += al(8 * g) + al(4 * g);
+    t += al(sizeof(BSlot) * g);
   }
   return t;
 }
@@ -408,12 +460,9 @@ int batch_create(aln_ctx *c, const aln_col *cols, int ncols, uint32_t seg_rows, 
     for (int col : {q.pred_col, q.key_col})
       if (std::find(stage.begin(), stage.end(), col) == stage.end()) stage.push_back(col);
     void *p;
-    int r1 = batch_alloc(b, 8 * (bq.ngroups ? bq.ngroups : 1), &p);
+    const int r1 = batch_alloc(b, sizeof(BSlot) * (bq.ngroups ? bq.ngroups : 1), &p);
     if (r1) return bail(r1);
-    bq.sum = (unsigned long long *)p;
-    r1 = batch_alloc(b, 4 * (bq.ngroups ? bq.ngroups : 1), &p);
-    if (r1) return bail(r1);
-    bq.cnt = (unsigned int *)p;
+    bq.acc = (BSlot *)p;
   }
   // distinct columns: the staged ones first, then the rest in first-use order
   std::vector<int> distinct(stage.begin(), stage.end());
@@ -445,6 +494,11 @@ int batch_create(aln_ctx *c, const aln_col *cols, int ncols, uint32_t seg_rows, 
   }
   b->args.q = b->dq;
   b->args.nq = nq;
+  b->args.shared_v = b->args.nslots > 0;
+  for (int i = 1; i < nq; ++i)
+    if (qs[i].x_col != qs[0].x_col || qs[i].y_col != qs[0].y_col || qs[i].a0 != qs[0].a0 || qs[i].a1 != qs[0].a1 ||
+        qs[i].b0 != qs[0].b0 || qs[i].b1 != qs[0].b1)
+      b->args.shared_v = 0;
   b->args.nstage = (int)std::min<size_t>(stage.size(), 2);
   for (int i = 0; i < b->args.nstage; ++i) b->args.stage_cols[i] = stage[i];
   b->args.seg_rows = seg_rows;
@@ -520,7 +574,7 @@ static int batch_results(aln_batch *b, int q, int64_t *keys, int64_t *sums, int6
   uint64_t *pos;
   ALN_TRY(ws(c, n, &flag));
   ALN_TRY(ws(c, n, &pos));
-  hipLaunchKernelGGL(k_batch_flags, dim3(grid_for(n, 256, 65536)), dim3(256), 0, c->stream, bq.cnt, n, flag);
+  hipLaunchKernelGGL(k_batch_flags, dim3(grid_for(n, 256, 65536)), dim3(256), 0, c->stream, bq.acc, n, flag);
   ALN_TRY(check_launch("batch_flags"));
   uint64_t g = 0;
   ALN_TRY(exclusive_scan_u32(c, flag, n, pos, &g));
@@ -544,7 +598,7 @@ static int batch_results(aln_batch *b, int q, int64_t *keys, int64_t *sums, int6
     ALN_TRY(ws(c, g, &dc));
   }
   hipLaunchKernelGGL(k_batch_emit, dim3(grid_for(n, 256, 65536)), dim3(256), 0, c->stream, flag, pos, n, bq.gbase,
-                     bq.sum, bq.cnt, dk, ds, dc);
+                     bq.acc, dk, ds, dc);
   ALN_TRY(check_launch("batch_emit"));
   if (host) {
     ALN_TRY(copy_to_host(c, keys, dk, 8 * g));
@@ -586,8 +640,8 @@ int aln_batch_autotune(aln_batch *b, int reps, aln_batch_tuning *best, double *h
   ALN_REQUIRE(b && best, "aln_batch_autotune: bad arguments");
   aln_ctx *c = b->ctx;
   if (reps < 1) reps = 1;
-  const int bpcs[] = {1, 2, 4};
-  const int hots[] = {0, 8, 32};
+  const int bpcs[] = {2, 4};
+  const int hots[] = {64, 128, 256, 512, 1024};
   hipEvent_t e0, e1;
   ALN_HIP(hipEventCreate(&e0));
   ALN_HIP(hipEventCreate(&e1));

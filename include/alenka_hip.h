@@ -391,8 +391,10 @@ typedef struct {
   int64_t key_div;                         /* >= 1 */
   int64_t a0, a1, b0, b1;
 } aln_batch_query;
+/* Tuning: the queries are split into groups whose LDS tables (hot_groups slots of 12 bytes
+ * per query) fit 48 KB; each block evaluates one query group over its share of the tiles. */
 typedef struct {
-  int32_t hot_groups;    /* groups [0, hot) of each query aggregated in LDS (0..32; <0: 32) */
+  int32_t hot_groups;    /* groups [0, hot) of each query aggregated in LDS (0..4096; <0: 256) */
   int32_t blocks_per_cu; /* persistent grid size per compute unit (<=0: 2)                */
   int32_t use_graph;     /* replay the launch sequence from a captured HIP graph            */
 } aln_batch_tuning;

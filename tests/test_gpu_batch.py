@@ -62,7 +62,8 @@ def check_all(b, h, qs):
         assert np.array_equal(host(c), wc), f"query {i} counts"
 
 
-@pytest.mark.parametrize("seg_rows,hot,bpc", [(65536, -1, 0), (100000, 0, 1), (1 << 20, 8, 4)])
+@pytest.mark.parametrize("seg_rows,hot,bpc", [(65536, -1, 0), (100000, 0, 1), (1 << 20, 8, 4),
+                                                 (65536, 4096, 2), (100000, 1024, 1)])
 def test_batch_matches_oracle(ctx, seg_rows, hot, bpc):
     from alenka_amd import device as D
 
@@ -144,13 +145,23 @@ def test_batch_zipf_keys(ctx):
     cols = [D.Column.encode(ctx, dev(ctx, c), 65536) for c in h[:4]] + [D.Column.encode(ctx, z, 65536)]
     qs = [dict(pred_col=2, key_col=4, x_col=2, y_col=3, lo=90000 + i * 100000, hi=90000 + i * 100000 + 3000000,
                key_div=div, **REV) for i, div in enumerate([1, 10, 1000, 20000, 100000])]
-    for hot in (0, 32):
+    # hot 1024 and 4096: LDS tables for 4 and 1 queries per block (query groups of 4 + 1
+    # and 5 x 1), every query summing the same expression (computed once per row)
+    for hot in (0, 32, 1024, 4096):
         b = D.Batch(ctx, cols, qs, hot_groups=hot)
         try:
             b.run()
             check_all(b, h, qs)
         finally:
             b.close()
+    # four shared columns, expressions that differ per query (evaluated per query)
+    qv = [dict(q, a0=i, a1=1 + i, b0=100 - i, b1=-1 - i) for i, q in enumerate(qs)]
+    b = D.Batch(ctx, cols, qv, hot_groups=1024)
+    try:
+        b.run()
+        check_all(b, h, qv)
+    finally:
+        b.close()
 
 
 def test_order_by_sort_input(ctx):
