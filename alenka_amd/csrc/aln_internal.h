@@ -127,9 +127,12 @@ int radix_sort_pairs(aln_ctx *c, uint64_t *d_keys, uint32_t *d_vals, uint64_t n,
                      uint32_t bits);
 // Batched queries (batch.hip): a batch whose device tables are carved from a caller-owned
 // region instead of hipMalloc (pool NULL: hipMalloc), and the bytes such a region needs.
+// hsegs (per column, host copies of the segment headers) and hmin/hmax (per column value
+// ranges) may be given to spare the device reads; NULL: read from the columns.
 int batch_create(aln_ctx *c, const aln_col *cols, int ncols, uint32_t seg_rows,
                  const aln_batch_query *qs, int nq, const aln_batch_tuning *tune, void *pool,
-                 size_t pool_bytes, aln_batch **out);
+                 size_t pool_bytes, aln_batch **out, const aln_seg *const *hsegs,
+                 const int64_t *hmin, const int64_t *hmax);
 size_t batch_pool_bytes(const uint64_t *ngroups, int nq);
 // Groups of query q into three host vectors (keys, sums, counts), resized to fit.
 int batch_fetch(aln_batch *b, int q, std::vector<int64_t> *keys_sums_counts);

@@ -20,6 +20,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "aln_internal.h"
@@ -327,9 +328,15 @@ int set_tuning(aln_batch *b, const aln_batch_tuning *tune) {
   b->args.ngq = ngq;
   b->blocks = (uint32_t)(per * (uint64_t)ngq);
   const size_t lds = (size_t)qg * hot * (sizeof(unsigned long long) + sizeof(unsigned int));
-  if (lds)
+  // the kernel's dynamic LDS limit, raised once per process to the largest size asked for
+  static std::mutex mu;
+  static size_t lds_set = 0;
+  std::lock_guard<std::mutex> g(mu);
+  if (lds > lds_set) {
     ALN_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(k_batch), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)lds));
+                                (int)BQ_HOT_BYTES));
+    lds_set = BQ_HOT_BYTES;
+  }
   b->use_graph = tune && tune->use_graph;
   if (b->exec) {
     (void)hipGraphExecDestroy(b->exec);
@@ -367,7 +374,8 @@ size_t batch_pool_bytes(const uint64_t *ngroups, int nq) {
 }
 
 int batch_create(aln_ctx *c, const aln_col *cols, int ncols, uint32_t seg_rows, const aln_batch_query *qs,
-                 int nq, const aln_batch_tuning *tune, void *pool, size_t pool_bytes, aln_batch **out) {
+                 int nq, const aln_batch_tuning *tune, void *pool, size_t pool_bytes, aln_batch **out,
+                 const aln_seg *const *hsegs, const int64_t *hmin, const int64_t *hmax) {
   ALN_REQUIRE(c && cols && qs && out && seg_rows > 0, "aln_batch_create: bad arguments");
   ALN_REQUIRE(ncols >= 1 && ncols <= ALN_MAX_COLS, "aln_batch_create: %d columns", ncols);
   ALN_REQUIRE(nq >= 1 && nq <= BQ_MAX, "aln_batch_create: %d queries (1..%d)", nq, BQ_MAX);
@@ -385,10 +393,17 @@ int batch_create(aln_ctx *c, const aln_col *cols, int ncols, uint32_t seg_rows, 
   };
   // zone maps give each key column's range
   std::vector<std::vector<aln_seg>> hs(ncols);
+  if (hsegs)
+    for (int col = 0; col < ncols; ++col) hs[col].assign(hsegs[col], hsegs[col] + nsegs);
   auto range_of = [&](int col, int64_t *mn, int64_t *mx) -> int {
     if (hs[col].empty() && nsegs) {
       hs[col].resize(nsegs);
       ALN_TRY(copy_to_host(c, hs[col].data(), cols[col].segs, sizeof(aln_seg) * nsegs));
+    }
+    if (hmin && hmax) {  // the caller's ranges (a server computes them once per table)
+      *mn = hmin[col];
+      *mx = hmax[col];
+      return ALN_OK;
     }
     *mn = INT64_MAX;
     *mx = INT64_MIN;
@@ -515,7 +530,7 @@ extern "C" {
 
 int aln_batch_create(aln_ctx *c, const aln_col *cols, int ncols, uint32_t seg_rows,
                      const aln_batch_query *qs, int nq, const aln_batch_tuning *tune, aln_batch **out) {
-  return batch_create(c, cols, ncols, seg_rows, qs, nq, tune, nullptr, 0, out);
+  return batch_create(c, cols, ncols, seg_rows, qs, nq, tune, nullptr, 0, out, nullptr, nullptr, nullptr);
 }
 
 int aln_batch_set_tuning(aln_batch *b, const aln_batch_tuning *tune) {

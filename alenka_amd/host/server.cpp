@@ -17,6 +17,7 @@
 #include <chrono>
 #include <condition_variable>
 #include <deque>
+#include <map>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -35,10 +36,12 @@ struct Table {
   uint32_t seg_rows = 0;
   uint64_t nrows = 0;
   std::vector<int64_t> cmin, cmax;  // per column, from the zone maps (or a device scan)
+  std::vector<std::vector<aln_seg>> hsegs;  // per column, host copies of the segment headers
+  std::vector<const aln_seg *> hsp;         // their data pointers
   // milliseconds per filter + GROUP BY request on this table, run alone through the
   // operators and inside a shared-scan batch (running means under the server lock; 0 = not
   // measured yet), and the batch decisions taken since batching was last tried
-  double plain_ms = 0, batch_ms = 0;
+  double plain_ms = 0, batch_ms = 0, batch1_ms = 0;  // batch1: a shared-scan run of one request
   uint32_t since_batch = 0;
 };
 
@@ -90,6 +93,14 @@ struct aln_server {
   int running = 0;
   bool stop = false;
   aln_server_totals st{};
+  // star requests by fact table (first column's words, segments): milliseconds per request
+  // run alone and inside shared-scan batches (running means under the lock; 0 = not measured
+  // yet), and the batching decisions taken since batching was last tried
+  struct StarTimes {
+    double one_ms = 0, batch_ms = 0;
+    uint32_t since_batch = 0;
+  };
+  std::map<std::pair<const void *, uint32_t>, StarTimes> star_times;
 };
 
 namespace {
@@ -123,17 +134,45 @@ size_t request_mem(const aln_server *s, const Request &r) {
 
 // Batched mode, filter + GROUP BY: whether the requests taken together run through one
 // shared scan (true) or each alone through the operators (caller holds the lock).
-// A single request has nothing to share; a batch runs batched unless the measured
-// per-request time of batches on this table is not below that of plain runs, and then
-// every 8th such batch still runs batched so the estimate follows the load.
+// The requests run batched unless the measured per-request time of such runs on this
+// table (batches, or single requests through the batch kernel, measured apart) is not
+// below that of plain runs, and then every 8th time they still run batched so the
+// estimate follows the load.  (A single request gains no shared scan, but the batch
+// kernel aggregates without the operators' row ids and evaluated columns.)
 constexpr uint32_t BATCH_RETRY = 8;
 
 bool use_batch(aln_server *s, const std::vector<Request *> &rs) {
   Table &t = s->tables[rs[0]->table];
   const uint64_t g = rs[0]->groups ? rs[0]->groups : 1;
   if (s->mem_in_use - rs[0]->mem + plain_mem(t, g) > s->cfg.budget_bytes) return true;  // no room
+  const double bm = rs.size() < 2 ? t.batch1_ms : t.batch_ms;
+  if (bm > 0 && t.plain_ms <= 0) return false;  // measure the plain form too
+  if (bm <= 0 || bm < t.plain_ms) {
+    t.since_batch = 0;
+    return true;
+  }
+  if (++t.since_batch >= BATCH_RETRY) {
+    t.since_batch = 0;
+    return true;
+  }
+  return false;
+}
+
+// Star requests taken together: one shared-scan batch (aln_star_agg_batch) or the oldest
+// alone (caller holds the lock) - the same measured choice as for filter + GROUP BY, single
+// runs measured first (the single pipeline avoids the batch's per-query bitmaps and tables,
+// and was faster per request on every table size measured).
+aln_server::StarTimes &star_times_of(aln_server *s, const Request &r) {
+  if (r.scols[2].empty()) return s->star_times[{nullptr, 0u}];
+  const aln_col &c = r.scols[2][0];
+  return s->star_times[{(const void *)c.words, c.nsegs}];
+}
+
+bool use_star_batch(aln_server *s, const std::vector<Request *> &rs) {
   if (rs.size() < 2) return false;
-  if (t.plain_ms <= 0 || t.batch_ms <= 0 || t.batch_ms < t.plain_ms) {
+  aln_server::StarTimes &t = star_times_of(s, *rs[0]);
+  if (t.one_ms <= 0) return false;  // single runs are measured first
+  if (t.batch_ms <= 0 || t.batch_ms < t.one_ms) {
     t.since_batch = 0;
     return true;
   }
@@ -239,7 +278,7 @@ int run_batched(aln_server *s, Slot *sl, std::vector<Request *> &rs) {
   ALN_TRY(ensure_region(sl, need));
   aln_batch *b = nullptr;
   ALN_TRY(batch_create(sl->ctx, t.cols.data(), (int)t.cols.size(), t.seg_rows, qs.data(), (int)qs.size(),
-                       &s->cfg.tune, sl->region, sl->region_size, &b));
+                       &s->cfg.tune, sl->region, sl->region_size, &b, t.hsp.data(), t.cmin.data(), t.cmax.data()));
   int rc = aln_batch_run(b);
   for (size_t i = 0; rc == ALN_OK && i < rs.size(); ++i) rc = batch_fetch(b, (int)i, rs[i]->res);
   aln_batch_destroy(b);
@@ -355,7 +394,16 @@ void slot_main(aln_server *s, Slot *sl) {
     }
     std::vector<Request *> rs = take_batch(s);
     if (rs.empty()) continue;
-    const bool batched = rs[0]->kind == 0 && !s->cfg.plain && use_batch(s, rs);
+    const bool batched = !s->cfg.plain && (rs[0]->kind == 0 ? use_batch(s, rs) : use_star_batch(s, rs));
+    if (rs[0]->kind == 1 && rs.size() > 1 && !batched) {
+      // the oldest alone; the others back to the front of the queue, in order
+      for (size_t i = rs.size(); i-- > 1;) {
+        s->mem_in_use -= rs[i]->mem;
+        s->pending.push_front(rs[i]);
+      }
+      rs.resize(1);
+      s->cv_work.notify_all();
+    }
     if (rs[0]->kind == 0 && !s->cfg.plain && !batched) {
       // run the oldest alone through the operators, charged their bytes; the others go back
       // to the front of the queue, in order, for the next free slot
@@ -389,8 +437,13 @@ void slot_main(aln_server *s, Slot *sl) {
     const char *err = rc == ALN_OK ? nullptr : aln_last_error();
     const double per = ms_between(t0, Clock::now()) / (double)rs.size();
     lk.lock();
-    if (rs[0]->kind == 0 && rc == ALN_OK) note_time(batched ? &s->tables[rs[0]->table].batch_ms
-                                                            : &s->tables[rs[0]->table].plain_ms, per);
+    if (rs[0]->kind == 0 && rc == ALN_OK) {
+      Table &tb = s->tables[rs[0]->table];
+      note_time(!batched ? &tb.plain_ms : rs.size() < 2 ? &tb.batch1_ms : &tb.batch_ms, per);
+    } else if (rc == ALN_OK && !s->cfg.plain) {
+      aln_server::StarTimes &tt = star_times_of(s, *rs[0]);
+      note_time(batched ? &tt.batch_ms : &tt.one_ms, per);
+    }
     --s->running;
     size_t mem = 0;
     for (Request *r : rs) {
@@ -495,8 +548,10 @@ int aln_server_add_table(aln_server *s, const aln_col *cols, int ncols, uint32_t
   aln_ctx *c = s->admin;
   t.cmin.resize(ncols);
   t.cmax.resize(ncols);
+  t.hsegs.resize(ncols);
   for (int i = 0; i < ncols; ++i) {
-    std::vector<aln_seg> hs(nsegs);
+    std::vector<aln_seg> &hs = t.hsegs[i];
+    hs.resize(nsegs);
     if (nsegs) ALN_TRY(copy_to_host(c, hs.data(), cols[i].segs, sizeof(aln_seg) * nsegs));
     int64_t mn = INT64_MAX, mx = INT64_MIN;
     uint64_t rows = 0;
@@ -514,6 +569,7 @@ int aln_server_add_table(aln_server *s, const aln_col *cols, int ncols, uint32_t
     if (i == 0) t.nrows = rows;
   }
   ALN_REQUIRE(t.nrows < (1ull << 32), "aln_server_add_table: %llu rows", (unsigned long long)t.nrows);
+  for (auto &hs : t.hsegs) t.hsp.push_back(hs.data());
   s->tables.push_back(std::move(t));
   *h_table = (int32_t)s->tables.size() - 1;
   return ALN_OK;
