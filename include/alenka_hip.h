@@ -433,7 +433,7 @@ typedef struct {
   int32_t plain;     /* 1: no co-batching, operators one query at a time     */
   int32_t reserved;
   uint64_t budget_bytes; /* device bytes admitted requests may hold together  */
-  aln_batch_tuning tune;
+  aln_batch_tuning tune; /* batch kernel shape (blocks_per_cu <= 0: 1 here)   */
 } aln_server_config;
 typedef struct {
   int32_t status; /* ALN_OK or the request's error code */
