@@ -797,9 +797,9 @@ __global__ void __launch_bounds__(ST_BLOCK, 8) k_star_mid(ScanDesc d, const SegP
   uint32_t mcode[ST_ITEMS];
 #pragma unroll
   for (int j = 0; j < ST_ITEMS; ++j) {
-    // the passing rows as a lane mask, from the two compares (kept a mask, not a lane value)
-    const uint64_t b = __builtin_amdgcn_ballot_w64(pw[j] != 0xFFFFFFFFu) &
-                       __builtin_amdgcn_ballot_w64(((dw[j] >> ((uint32_t)dc[j] & 31)) & 1) != 0);
+    // the passing rows as a lane mask (kept a mask, not a lane value): the bit test alone,
+    // as a lane that does not probe holds a zero word
+    const uint64_t b = __builtin_amdgcn_ballot_w64(((dw[j] >> ((uint32_t)dc[j] & 31)) & 1) != 0);
     mcode[j] = 0xFFFFFFFFu;
     uint32_t code = 0;  // key - mkmin (key range < 2^32)
     if (__builtin_amdgcn_inverse_ballot_w64(b)) {
@@ -1241,9 +1241,9 @@ __device__ __forceinline__ void fact_probe_hits(const SegPlan &P, const void *an
   uint32_t hits = 0, nmatch = 0;
 #pragma unroll
   for (int j = 0; j < ST_ITEMS; ++j) {
-    // the joined rows as a lane mask, from the two compares (kept a mask, not a lane value)
-    const uint64_t b = __builtin_amdgcn_ballot_w64(code[j] != 0xFFFFFFFFu) &
-                       __builtin_amdgcn_ballot_w64(((word[j] >> (code[j] & 31)) & 1) != 0);
+    // the joined rows as a lane mask (kept a mask, not a lane value): the bit test alone,
+    // as a lane that does not probe holds a zero word
+    const uint64_t b = __builtin_amdgcn_ballot_w64(((word[j] >> (code[j] & 31)) & 1) != 0);
     if (b == 0) continue;  // (wave-uniform) no joined row in this row of the wave: most of them
     const bool h = __builtin_amdgcn_inverse_ballot_w64(b);
     if (A.hitbuf) {  // slot of this row among the wave's joined rows; the first slots are deferred
